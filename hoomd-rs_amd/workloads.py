@@ -1,0 +1,63 @@
+"""Seeded synthetic initial states for the sweep workloads.
+
+All generators return (box_edges, positions) with positions as an (N, D)
+float64 array centred on the origin, inside [-L/2, L/2).
+"""
+import math
+
+import numpy as np
+
+
+def square_lattice_disks(n_side=128, packing_fraction=0.70, diameter=1.0):
+    """Hard disks on an n_side x n_side square lattice scaled to the packing fraction."""
+    n = n_side * n_side
+    area = n * math.pi * (diameter / 2) ** 2 / packing_fraction
+    L = math.sqrt(area)
+    a = L / n_side
+    i, j = np.meshgrid(np.arange(n_side), np.arange(n_side), indexing="ij")
+    pos = np.stack([(i.ravel() + 0.5) * a - L / 2, (j.ravel() + 0.5) * a - L / 2], axis=1)
+    return np.array([L, L]), pos
+
+
+def fcc_spheres(n_cells=(63, 63, 63), packing_fraction=0.50, diameter=1.0):
+    """Spheres on an FCC lattice of n_cells conventional cells (4 sites each)."""
+    nx, ny, nz = n_cells
+    n = 4 * nx * ny * nz
+    volume = n * math.pi * diameter**3 / 6 / packing_fraction
+    return fcc_at_volume(n_cells, volume)
+
+
+def fcc_number_density(n_cells, number_density):
+    nx, ny, nz = n_cells
+    return fcc_at_volume(n_cells, 4 * nx * ny * nz / number_density)
+
+
+def fcc_at_volume(n_cells, volume):
+    nx, ny, nz = n_cells
+    a = (volume / (nx * ny * nz)) ** (1.0 / 3.0)
+    L = np.array([nx * a, ny * a, nz * a])
+    basis = np.array([[0.0, 0.0, 0.0], [0.5, 0.5, 0.0], [0.5, 0.0, 0.5], [0.0, 0.5, 0.5]]) + 0.25
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    cells = np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1).astype(np.float64)
+    pos = (cells[:, None, :] + basis[None, :, :]).reshape(-1, 3) * a - L / 2
+    return L, pos
+
+
+def random_dilute(n, L, dim, min_distance, seed):
+    """Random non-overlapping points by sequential rejection (small systems only)."""
+    rng = np.random.default_rng(seed)
+    L = np.asarray(L, np.float64)
+    pts = []
+    tries = 0
+    while len(pts) < n:
+        tries += 1
+        if tries > 1000 * n:
+            raise RuntimeError("could not place points")
+        p = rng.uniform(-L / 2, L / 2, size=dim)
+        if pts:
+            d = np.asarray(pts) - p
+            d -= L * np.round(d / L)
+            if np.min((d * d).sum(1)) < min_distance**2:
+                continue
+        pts.append(p)
+    return np.asarray(pts)

@@ -1,0 +1,281 @@
+"""ctypes view of the oracle (oracle/liboracle.so), the CPU restatement of the
+reference sweep.  Test infrastructure only."""
+import ctypes as C
+import pathlib
+import subprocess
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent.parent
+LIB_PATH = ROOT / "oracle" / "liboracle.so"
+
+_lib = None
+
+D_P = C.POINTER(C.c_double)
+U64_P = C.POINTER(C.c_uint64)
+I64_P = C.POINTER(C.c_int64)
+U32_P = C.POINTER(C.c_uint32)
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+class TraceRecord(C.Structure):
+    _fields_ = [("substep", C.c_uint32), ("color", C.c_uint32), ("space", C.c_uint64),
+                ("body", C.c_int64), ("status", C.c_int32), ("pad", C.c_int32)]
+
+
+TRACE_DTYPE = np.dtype([("substep", "<u4"), ("color", "<u4"), ("space", "<u8"),
+                        ("body", "<i8"), ("status", "<i4"), ("pad", "<i4")])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            subprocess.run(["make", "-C", str(ROOT / "oracle")], check=True,
+                           stdout=subprocess.DEVNULL)
+        L = C.CDLL(str(LIB_PATH))
+        L.oracle_sfc64_stream.argtypes = [C.c_uint64] * 4 + [C.c_int, U64_P, C.c_uint64]
+        L.oracle_counter_stream.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, U64_P, C.c_uint64]
+        L.oracle_sample.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, C.c_uint32,
+                                    D_P, C.c_uint64]
+        L.oracle_zig_tables.argtypes = [D_P, D_P]
+        L.oracle_translate.argtypes = [C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32,
+                                       D_P, C.c_double, D_P, U32_P]
+        L.oracle_wrap.argtypes = [C.c_int, D_P, D_P, D_P, C.c_uint64]
+        L.oracle_ghosts.argtypes = [C.c_int, D_P, C.c_double, D_P, D_P]
+        L.oracle_ghosts.restype = C.c_int
+        L.oracle_compute_dimensions.argtypes = [C.c_int, D_P, C.c_double, C.POINTER(C.c_int), D_P, I64_P]
+        L.oracle_compute_dimensions.restype = C.c_int
+        L.oracle_multi_to_index.argtypes = [C.c_int, I64_P, I64_P]
+        L.oracle_multi_to_index.restype = C.c_uint64
+        L.oracle_by_color.argtypes = [C.c_int, I64_P, U64_P]
+        L.oracle_point_to_space_fixed.argtypes = [C.c_int, C.c_double, D_P, C.c_int, D_P, I64_P, C.c_uint64]
+        L.oracle_checkerboard_origin.argtypes = [C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double,
+                                                 D_P, D_P, D_P, I64_P]
+        L.oracle_stencil.argtypes = [C.c_int, C.c_uint32, I64_P]
+        L.oracle_stencil.restype = C.c_uint64
+        L.oracle_cell_index.argtypes = [C.c_int, C.c_double, D_P, I64_P]
+        L.oracle_system_create.argtypes = [C.c_int, D_P, C.c_double, C.c_double, C.c_int, D_P,
+                                           C.c_uint64, C.c_uint32, D_P, C.c_uint64]
+        L.oracle_system_create.restype = C.c_void_p
+        L.oracle_system_destroy.argtypes = [C.c_void_p]
+        L.oracle_set_threads.argtypes = [C.c_int]
+        L.oracle_max_threads.restype = C.c_int
+        L.oracle_parallel_apply.argtypes = [C.c_void_p, C.c_double, C.c_double, U64_P, U64_P]
+        L.oracle_parallel_apply.restype = C.c_int
+        L.oracle_sweep_apply.argtypes = [C.c_void_p, C.c_double, C.c_double, U64_P, U64_P]
+        L.oracle_increment_step.argtypes = [C.c_void_p]
+        L.oracle_get_counter.argtypes = [C.c_void_p, U64_P, U32_P]
+        L.oracle_set_counter.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32]
+        L.oracle_n_ghosts.argtypes = [C.c_void_p]
+        L.oracle_n_ghosts.restype = C.c_uint64
+        L.oracle_get_positions.argtypes = [C.c_void_p, D_P, D_P]
+        L.oracle_total_energy.argtypes = [C.c_void_p]
+        L.oracle_total_energy.restype = C.c_double
+        L.oracle_delta_energy_one.argtypes = [C.c_void_p, C.c_uint64, D_P]
+        L.oracle_delta_energy_one.restype = C.c_double
+        L.oracle_trace_enable.argtypes = [C.c_void_p, C.c_int]
+        L.oracle_trace_size.argtypes = [C.c_void_p]
+        L.oracle_trace_size.restype = C.c_uint64
+        L.oracle_trace_read.argtypes = [C.c_void_p, C.c_void_p]
+        L.oracle_time_applies.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, D_P, U64_P]
+        L.oracle_time_applies.restype = C.c_uint64
+        _lib = L
+    return _lib
+
+
+def sfc64_stream(a, b, c, ctr, n, warm=True):
+    out = np.zeros(n, np.uint64)
+    lib().oracle_sfc64_stream(a, b, c, ctr, 1 if warm else 0, _p(out, U64_P), n)
+    return out
+
+
+def counter_stream(step, substep, seed, index, n):
+    out = np.zeros(n, np.uint64)
+    lib().oracle_counter_stream(step, substep, seed, index, _p(out, U64_P), n)
+    return out
+
+
+def sample(step, substep, seed, index, kind, n, param=0):
+    out = np.zeros(n, np.float64)
+    kinds = {"f64": 0, "open01": 1, "normal": 2, "index": 3}
+    lib().oracle_sample(step, substep, seed, index, kinds[kind], param, _p(out, D_P), n)
+    return out
+
+
+def zig_tables():
+    x = np.zeros(257)
+    f = np.zeros(257)
+    lib().oracle_zig_tables(_p(x, D_P), _p(f, D_P))
+    return x, f
+
+
+def wrap(L, pts):
+    L = np.ascontiguousarray(L, np.float64)
+    pts = np.ascontiguousarray(pts, np.float64)
+    out = np.zeros_like(pts)
+    dim = len(L)
+    lib().oracle_wrap(dim, _p(L, D_P), _p(pts, D_P), _p(out, D_P), pts.size // dim)
+    return out
+
+
+def ghosts(L, rng, pt):
+    L = np.ascontiguousarray(L, np.float64)
+    pt = np.ascontiguousarray(pt, np.float64)
+    dim = len(L)
+    out = np.zeros((8, dim))
+    n = lib().oracle_ghosts(dim, _p(L, D_P), rng, _p(pt, D_P), _p(out, D_P))
+    return out[:n]
+
+
+def compute_dimensions(edges, rng, periodic):
+    edges = np.ascontiguousarray(edges, np.float64)
+    dim = len(edges)
+    w = np.zeros(dim)
+    s = np.zeros(dim, np.int64)
+    per = (C.c_int * 1)(1 if periodic else 0)
+    rc = lib().oracle_compute_dimensions(dim, _p(edges, D_P), rng, per, _p(w, D_P), _p(s, I64_P))
+    if rc != 0:
+        return None
+    return w, s
+
+
+def multi_to_index(m, shape):
+    m = np.ascontiguousarray(m, np.int64)
+    shape = np.ascontiguousarray(shape, np.int64)
+    return int(lib().oracle_multi_to_index(len(m), _p(m, I64_P), _p(shape, I64_P)))
+
+
+def by_color(shape):
+    shape = np.ascontiguousarray(shape, np.int64)
+    dim = len(shape)
+    total = int(np.prod(shape // 2))
+    out = np.zeros((2**dim, total), np.uint64)
+    lib().oracle_by_color(dim, _p(shape, I64_P), _p(out, U64_P))
+    return out
+
+
+def point_to_space_fixed(rng, edges, periodic, pts):
+    edges = np.ascontiguousarray(edges, np.float64)
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, len(edges))
+    out = np.zeros(len(pts), np.int64)
+    lib().oracle_point_to_space_fixed(len(edges), rng, _p(edges, D_P), 1 if periodic else 0,
+                                      _p(pts, D_P), _p(out, I64_P), len(pts))
+    return out
+
+
+def checkerboard(dim, step, substep, seed, rng, edges):
+    edges = np.ascontiguousarray(edges, np.float64)
+    o = np.zeros(dim)
+    w = np.zeros(dim)
+    s = np.zeros(dim, np.int64)
+    lib().oracle_checkerboard_origin(dim, step, substep, seed, rng, _p(edges, D_P), _p(o, D_P),
+                                     _p(w, D_P), _p(s, I64_P))
+    return o, w, s
+
+
+def stencil(dim, radius):
+    out = np.zeros((2 * radius + 1) ** dim * dim, np.int64)
+    n = lib().oracle_stencil(dim, radius, _p(out, I64_P))
+    return out[: n * dim].reshape(n, dim)
+
+
+def cell_index(w, p):
+    p = np.ascontiguousarray(p, np.float64)
+    out = np.zeros(len(p), np.int64)
+    lib().oracle_cell_index(len(p), w, _p(p, D_P), _p(out, I64_P))
+    return out
+
+
+class OracleSystem:
+    """The restated Microstate + PairwiseCutoff + ParallelSweep/Sweep."""
+
+    MODELS = {"hard_sphere": 0, "lj": 1, "lj_shifted": 2}
+
+    def __init__(self, L, positions, *, model="hard_sphere", params=(1.0,), boundary_range=None,
+                 body_range=None, step=0, seed=0):
+        L = np.ascontiguousarray(L, np.float64)
+        self.dim = len(L)
+        pos = np.ascontiguousarray(positions, np.float64).reshape(-1, self.dim)
+        self.n = len(pos)
+        prm = np.zeros(4)
+        prm[: len(params)] = params
+        rng = params[0] if model == "hard_sphere" else params[2]
+        boundary_range = rng if boundary_range is None else boundary_range
+        body_range = rng if body_range is None else body_range
+        self._keep = (L, pos, prm)
+        self.h = lib().oracle_system_create(self.dim, _p(L, D_P), boundary_range, body_range,
+                                            self.MODELS[model], _p(prm, D_P), step, seed,
+                                            _p(pos, D_P), self.n)
+        if not self.h:
+            raise ValueError("invalid oracle system configuration")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_system_destroy(self.h)
+            self.h = None
+
+    def parallel_apply(self, kt, max_move):
+        a = np.zeros(1, np.uint64)
+        r = np.zeros(1, np.uint64)
+        rc = lib().oracle_parallel_apply(self.h, kt, max_move, _p(a, U64_P), _p(r, U64_P))
+        if rc != 0:
+            raise RuntimeError("checkerboard does not cover the system")
+        return int(a[0]), int(r[0])
+
+    def sweep_apply(self, kt, max_move):
+        a = np.zeros(1, np.uint64)
+        r = np.zeros(1, np.uint64)
+        lib().oracle_sweep_apply(self.h, kt, max_move, _p(a, U64_P), _p(r, U64_P))
+        return int(a[0]), int(r[0])
+
+    def increment_step(self):
+        lib().oracle_increment_step(self.h)
+
+    def counter(self):
+        s = np.zeros(1, np.uint64)
+        ss = np.zeros(1, np.uint32)
+        lib().oracle_get_counter(self.h, _p(s, U64_P), _p(ss, U32_P))
+        return int(s[0]), int(ss[0])
+
+    def set_counter(self, step, substep):
+        lib().oracle_set_counter(self.h, step, substep)
+
+    def positions(self):
+        b = np.zeros((self.n, self.dim))
+        s = np.zeros((self.n, self.dim))
+        lib().oracle_get_positions(self.h, _p(b, D_P), _p(s, D_P))
+        return b, s
+
+    def n_ghosts(self):
+        return int(lib().oracle_n_ghosts(self.h))
+
+    def total_energy(self):
+        return lib().oracle_total_energy(self.h)
+
+    def delta_energy_one(self, body, trial):
+        t = np.ascontiguousarray(trial, np.float64)
+        return lib().oracle_delta_energy_one(self.h, body, _p(t, D_P))
+
+    def trace(self, on=True):
+        lib().oracle_trace_enable(self.h, 1 if on else 0)
+
+    def read_trace(self):
+        n = lib().oracle_trace_size(self.h)
+        out = np.zeros(n, TRACE_DTYPE)
+        lib().oracle_trace_read(self.h, out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def time_applies(self, parallel, kt, max_move, seconds):
+        el = np.zeros(1)
+        na = np.zeros(1, np.uint64)
+        moves = lib().oracle_time_applies(self.h, 1 if parallel else 0, kt, max_move, seconds,
+                                          _p(el, D_P), _p(na, U64_P))
+        return int(moves), float(el[0]), int(na[0])
+
+
+def set_threads(n):
+    lib().oracle_set_threads(n)
