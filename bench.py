@@ -1,0 +1,149 @@
+"""Throughput of the MI355X local MC sweep on the headline workload.
+
+Workload (one "step" = one ParallelSweep::apply + Microstate::increment_step,
+the body of HardSphereSim::advance, benchmarks/src/mc/hard_sphere.rs:90-111):
+N = 1,000,188 unit-diameter hard spheres on an FCC lattice (4 x 63^3 sites)
+scaled to packing fraction 0.50 in a periodic cubic box, translation moves with
+maximum distance 0.08, seed 42.  Inputs are resident in HBM before the timed
+region.  The headline value is the mixed FP32/FP64 mode; the plain FP64 mode is
+timed the same way and must make identical decisions (checked here: identical
+counts and bit-identical final positions).
+
+Prints one JSON line (rank 0).
+"""
+import argparse
+import json
+import math
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "tests"))
+
+METRIC = "trial moves/s, hard spheres N=1M φ=0.5; mixed vs plain FP64, identical accepts"
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=800, help="timed applies (800 = 200 sweeps x 4)")
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample length")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison run")
+    ap.add_argument("--seed", type=int, default=42)
+    return ap.parse_args()
+
+
+def pass_bytes(precision, n, n_spaces, nonempty, accepted_per_pass, dim=3):
+    """Algorithmic HBM bytes of one pass launch (one colour of one cycle):
+    the colour list and the chosen bodies' positions for the active spaces,
+    the cell start offsets and every site position of the 3^D neighbourhoods
+    (together they cover the whole system once), and the writes of accepted
+    moves."""
+    colours = 2**dim
+    active = nonempty / colours
+    site = 16 if precision == "mixed" else 32
+    write = 80 if precision == "mixed" else 64
+    return 4 * active + 4 * (n_spaces + 1) + site * n + 32 * active + write * accepted_per_pass
+
+
+def run_gpu(hmc, L, body, site, precision, steps, warmup, seed, kT, d):
+    sim = hmc.GpuParallelSweep(L, body, sites=site, precision=precision)
+    # warm-up applies: steps 0 .. warmup-1
+    sim.run(0, seed, kT, d, warmup)
+    sim.profile(True)
+    t0 = time.perf_counter()
+    acc, rej = sim.run(warmup, seed, kT, d, steps)  # returns after the stream is synchronised
+    t1 = time.perf_counter()
+    pass_ms, pass_n, build_ms, build_n = sim.read_profile()
+    stats = sim.stats()
+    final = sim.positions()
+    sim.close()
+    return dict(elapsed=t1 - t0, accepted=acc, rejected=rej, pass_ms=pass_ms, pass_n=pass_n,
+                build_ms=build_ms, build_n=build_n, stats=stats, final=final)
+
+
+def cpu_baseline(L, pos, seconds, seed, kT, d):
+    import _oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    O.set_threads(threads)
+    sys_ = O.OracleSystem(L, pos, seed=seed)
+    sys_.time_applies(True, kT, d, 0.5)  # warm caches
+    moves, elapsed, applies = sys_.time_applies(True, kT, d, seconds)
+    return dict(value=moves / elapsed, unit="trial moves/s", cores=threads, kind="port",
+                sample=f"{applies} ParallelSweep applies of the same N=1,000,188 system ({elapsed:.1f} s), "
+                       f"C++ restatement of hoomd-rs 1.4.0 with OpenMP over same-colour spaces")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import _pkg
+    import _oracle as O
+    hmc = _pkg.pkg()
+    wl = _pkg.workloads()
+    kT, d = 1.0, 0.08
+    L, pos = wl.fcc_spheres((63, 63, 63), packing_fraction=0.50)
+    n = len(pos)
+    # the microstate's wrapped bodies and sites (Microstate::add_body)
+    ref = O.OracleSystem(L, pos, seed=args.seed)
+    body, site = ref.positions()
+    del ref
+
+    mixed = run_gpu(hmc, L, body, site, "mixed", args.steps, args.warmup, args.seed, kT, d)
+    trials = mixed["accepted"] + mixed["rejected"]
+    value = trials / mixed["elapsed"]
+    out = {
+        "metric": METRIC, "value": value, "unit": "trial moves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * mixed["elapsed"] / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": "hard spheres 3D, FCC 4*63^3, packing fraction 0.50, translate d=0.08, "
+                               "ParallelSweep apply per step", "n_bodies": n, "box_edge": float(L[0]),
+                   "precision": "mixed FP32 pair test with FP64 band re-test", "seed": args.seed},
+        "sweeps_per_s": value / n,
+        "acceptance": mixed["accepted"] / max(trials, 1),
+    }
+    # roofline of the dominant kernel (the per-colour trial pass)
+    n_spaces = 100**3
+    cycles = mixed["stats"]["cycles"] / max(mixed["stats"]["applies"], 1)
+    passes = args.steps * 8 * cycles
+    nonempty = trials / (args.steps * cycles)
+    acc_per_pass = mixed["accepted"] / passes
+    if mixed["pass_n"]:
+        avg_ms = mixed["pass_ms"] / mixed["pass_n"]
+        b = pass_bytes("mixed", n, n_spaces, nonempty, acc_per_pass)
+        achieved = b / (avg_ms * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                           "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                           "kernel": "k_pass<3,MIXED>", "avg_launch_ms": avg_ms,
+                           "algorithmic_bytes_per_launch": b, "launches": mixed["pass_n"],
+                           "build_ms_per_apply": mixed["build_ms"] / max(mixed["build_n"], 1)}
+    out["band_retests"] = mixed["stats"]["band_retests"]  # FP64 re-tests over warm-up + timed applies
+    out["exp_slow_path"] = mixed["stats"]["exp_slow_path"]
+    if not args.no_fp64:
+        f64 = run_gpu(hmc, L, body, site, "fp64", args.steps, args.warmup, args.seed, kT, d)
+        same = (f64["accepted"], f64["rejected"]) == (mixed["accepted"], mixed["rejected"])
+        import numpy as np
+        same = same and np.array_equal(f64["final"][0], mixed["final"][0]) and np.array_equal(
+            f64["final"][1], mixed["final"][1])
+        v64 = (f64["accepted"] + f64["rejected"]) / f64["elapsed"]
+        out["fp64"] = {"value": v64, "ms_per_step": 1e3 * f64["elapsed"] / args.steps,
+                       "avg_pass_ms": f64["pass_ms"] / max(f64["pass_n"], 1),
+                       "identical_decisions_and_positions": bool(same)}
+        out["mixed_over_fp64"] = value / v64
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline(L, pos, args.cpu_seconds, args.seed, kT, d)
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = value / cb["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,208 @@
+"""hoomd-rs_amd: MI355X-native hoomd-mc local trial-move sweep.
+
+The product is libhmc.so (HIP kernels for gfx950 behind the C ABI declared in
+include/hmc.h).  This module is a thin ctypes view of that ABI for Python
+callers (tests, benchmark).  It has no fallback: if libhmc.so is missing or
+no GPU is present the calls raise.
+"""
+import ctypes as C
+import pathlib
+
+import numpy as np
+
+PKG = pathlib.Path(__file__).resolve().parent
+LIB_PATH = PKG / "libhmc.so"
+
+HMC_OK = 0
+HMC_ERR_INVALID = -1
+HMC_ERR_UNSUPPORTED = -2
+HMC_ERR_DEVICE = -3
+HMC_ERR_COVER = -4
+HMC_ERR_STATE = -5
+
+HMC_HARD_SPHERE = 0
+HMC_FP64 = 0
+HMC_MIXED = 1
+
+# every entry point declared in include/hmc.h
+EXPORTS = ["hmc_create", "hmc_set_state", "hmc_get_state", "hmc_apply_translate", "hmc_run_translate",
+           "hmc_trace_enable", "hmc_trace_read", "hmc_get_stats", "hmc_profile_enable", "hmc_profile_read",
+           "hmc_debug_math", "hmc_debug_normals", "hmc_last_error", "hmc_destroy"]
+
+
+class HmcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"hmc error {code}: {msg}")
+        self.code = code
+
+
+class Config(C.Structure):
+    _fields_ = [("dim", C.c_uint32), ("n_bodies", C.c_uint64), ("box_edges", C.c_double * 3),
+                ("boundary_range", C.c_double), ("body_interaction_range", C.c_double),
+                ("spatial_cell_width", C.c_double),
+                ("model", C.c_int32), ("params", C.c_double * 4), ("precision", C.c_int32),
+                ("device", C.c_int32)]
+
+
+class Count(C.Structure):
+    _fields_ = [("accepted", C.c_uint64), ("rejected", C.c_uint64)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("applies", C.c_uint64), ("cycles", C.c_uint64), ("pair_tests", C.c_uint64),
+                ("band_retests", C.c_uint64), ("exp_slow_path", C.c_uint64), ("guard_cycles", C.c_uint64)]
+
+
+TRACE_DTYPE = np.dtype([("substep", "<u4"), ("color", "<u4"), ("space", "<u8"),
+                        ("body", "<i8"), ("status", "<i4"), ("pad", "<i4")])
+
+_lib = None
+D_P = C.POINTER(C.c_double)
+
+
+def lib():
+    """Load libhmc.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise FileNotFoundError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = C.CDLL(str(LIB_PATH))
+        vp = C.c_void_p
+        L.hmc_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
+        L.hmc_set_state.argtypes = [vp, D_P, D_P]
+        L.hmc_get_state.argtypes = [vp, D_P, D_P]
+        L.hmc_apply_translate.argtypes = [vp, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
+                                          C.POINTER(Count), C.POINTER(C.c_uint32)]
+        L.hmc_run_translate.argtypes = [vp, C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_uint32,
+                                        C.POINTER(Count)]
+        L.hmc_trace_enable.argtypes = [vp, C.c_int, C.c_uint64]
+        L.hmc_trace_read.argtypes = [vp, vp, C.c_uint64, C.POINTER(C.c_uint64)]
+        L.hmc_get_stats.argtypes = [vp, C.POINTER(Stats)]
+        L.hmc_profile_enable.argtypes = [vp, C.c_int]
+        L.hmc_profile_read.argtypes = [vp, D_P, C.POINTER(C.c_uint64), D_P, C.POINTER(C.c_uint64)]
+        L.hmc_debug_math.argtypes = [C.c_int, C.c_int, D_P, D_P, C.c_uint64]
+        L.hmc_debug_normals.argtypes = [C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, D_P,
+                                        C.c_uint64]
+        L.hmc_last_error.argtypes = [vp]
+        L.hmc_last_error.restype = C.c_char_p
+        L.hmc_destroy.argtypes = [vp]
+        L.hmc_destroy.restype = None
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(D_P)
+
+
+class GpuParallelSweep:
+    """A system resident on the GPU, advanced by ParallelSweep::apply with
+    Translate moves (hoomd-mc/src/parallel_sweep.rs:390-421)."""
+
+    def __init__(self, box_edges, positions, *, sites=None, diameter=1.0, boundary_range=None,
+                 body_interaction_range=None, spatial_cell_width=None, precision="fp64", device=0):
+        """positions / sites: the microstate's body and site positions (already
+        wrapped into the box, as Microstate::add_body stores them); sites=None
+        derives them as wrap(body + 0)."""
+        pos = np.ascontiguousarray(positions, np.float64)
+        self.dim = pos.shape[1]
+        self.n = pos.shape[0]
+        cfg = Config()
+        cfg.dim = self.dim
+        cfg.n_bodies = self.n
+        for a in range(self.dim):
+            cfg.box_edges[a] = float(box_edges[a])
+        cfg.boundary_range = diameter if boundary_range is None else boundary_range
+        cfg.body_interaction_range = diameter if body_interaction_range is None else body_interaction_range
+        # VecCell::with_search_radius(maximum interaction range), as the
+        # reference's benchmarks build the microstate
+        cfg.spatial_cell_width = diameter if spatial_cell_width is None else spatial_cell_width
+        cfg.model = HMC_HARD_SPHERE
+        cfg.params[0] = diameter
+        cfg.precision = {"fp64": HMC_FP64, "mixed": HMC_MIXED}[precision]
+        cfg.device = device
+        h = C.c_void_p()
+        rc = lib().hmc_create(C.byref(cfg), C.byref(h))
+        if rc != HMC_OK:
+            raise HmcError(rc, "hmc_create failed")
+        self.h = h
+        self.set_positions(pos, sites)
+
+    def _check(self, rc):
+        if rc != HMC_OK:
+            raise HmcError(rc, lib().hmc_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hmc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def apply(self, step, substep, seed, kT, max_distance):
+        cnt = Count()
+        used = C.c_uint32()
+        self._check(lib().hmc_apply_translate(self.h, step, substep, seed, kT, max_distance, C.byref(cnt),
+                                              C.byref(used)))
+        return int(cnt.accepted), int(cnt.rejected), int(used.value)
+
+    def run(self, step0, seed, kT, max_distance, n_steps):
+        cnt = Count()
+        self._check(lib().hmc_run_translate(self.h, step0, seed, kT, max_distance, n_steps, C.byref(cnt)))
+        return int(cnt.accepted), int(cnt.rejected)
+
+    def positions(self):
+        b = np.zeros((self.n, self.dim))
+        s = np.zeros((self.n, self.dim))
+        self._check(lib().hmc_get_state(self.h, _p(b), _p(s)))
+        return b, s
+
+    def set_positions(self, body, site=None):
+        body = np.ascontiguousarray(body, np.float64)
+        site_p = None if site is None else _p(np.ascontiguousarray(site, np.float64))
+        self._check(lib().hmc_set_state(self.h, _p(body), site_p))
+
+    def trace(self, on=True, capacity=1 << 22):
+        self._check(lib().hmc_trace_enable(self.h, 1 if on else 0, capacity))
+
+    def read_trace(self, capacity=1 << 22):
+        out = np.zeros(capacity, TRACE_DTYPE)
+        n = C.c_uint64()
+        self._check(lib().hmc_trace_read(self.h, out.ctypes.data_as(C.c_void_p), capacity, C.byref(n)))
+        return out[: n.value]
+
+    def stats(self):
+        s = Stats()
+        self._check(lib().hmc_get_stats(self.h, C.byref(s)))
+        return {k: int(getattr(s, k)) for k, _ in Stats._fields_}
+
+    def profile(self, on=True):
+        self._check(lib().hmc_profile_enable(self.h, 1 if on else 0))
+
+    def read_profile(self):
+        pm = np.zeros(1)
+        bm = np.zeros(1)
+        pn = C.c_uint64()
+        bn = C.c_uint64()
+        self._check(lib().hmc_profile_read(self.h, _p(pm), C.byref(pn), _p(bm), C.byref(bn)))
+        return float(pm[0]), int(pn.value), float(bm[0]), int(bn.value)
+
+
+def debug_math(kind, x, device=0):
+    kinds = {"cbrt": 0, "log": 1, "exp": 2, "sqrt": 3, "div": 4}
+    x = np.ascontiguousarray(x, np.float64)
+    n = x.size // 2 if kind == "div" else x.size
+    out = np.zeros(n)
+    rc = lib().hmc_debug_math(device, kinds[kind], _p(x), _p(out), n)
+    if rc != HMC_OK:
+        raise HmcError(rc, "hmc_debug_math failed")
+    return out
+
+
+def debug_normals(step, substep, seed, index, n, device=0):
+    out = np.zeros(n)
+    rc = lib().hmc_debug_normals(device, step, substep, seed, index, _p(out), n)
+    if rc != HMC_OK:
+        raise HmcError(rc, "hmc_debug_normals failed")
+    return out

@@ -1,0 +1,149 @@
+/* hmc.h -- C ABI of libhmc.so, the MI355X (gfx950) implementation of
+ * hoomd-mc's checkerboard-parallel local trial-move sweep.
+ *
+ * The library replaces, for built-in shapes and interactions only, the body of
+ *   impl Trial for ParallelSweep<Translate<Cartesian<D>>, HypercuboidCheckerboard<D>, Point, Point>
+ *   (hoomd-mc/src/parallel_sweep.rs:317-421)
+ * acting on a Microstate with a Periodic<Hypercuboid<D>> boundary
+ * (hoomd-microstate/src/boundary/periodic/cuboid.rs) and a
+ * PairwiseCutoff<HardSphere> Hamiltonian
+ * (hoomd-interaction/src/pairwise/hard_shape.rs:127-174).
+ * Everything else keeps using the reference's CPU path.
+ *
+ * Conventions: every function returns an int status (HMC_OK == 0, negative on
+ * error) and never throws; positions are row-major (n_bodies x dim) doubles;
+ * a context is single-threaded (one caller at a time, like &mut self).
+ */
+#ifndef HMC_H
+#define HMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hmc_ctx hmc_ctx;
+
+/* hoomd_mc::Count (hoomd-mc/src/lib.rs:205-233). */
+typedef struct {
+  uint64_t accepted;
+  uint64_t rejected;
+} hmc_count;
+
+enum {
+  HMC_OK = 0,
+  HMC_ERR_INVALID = -1,     /* bad argument or configuration */
+  HMC_ERR_UNSUPPORTED = -2, /* combination not offloaded: keep the CPU path */
+  HMC_ERR_DEVICE = -3,      /* HIP runtime error */
+  HMC_ERR_COVER = -4,       /* a body is outside the checkerboard (the reference panics) */
+  HMC_ERR_STATE = -5        /* no state set, or trace buffer overflow */
+};
+
+/* Pair model (the Hamiltonian H of PairwiseCutoff<H>). */
+enum {
+  HMC_HARD_SPHERE = 0 /* params[0] = diameter */
+};
+
+/* Arithmetic of the overlap test. Both modes make identical decisions. */
+enum {
+  HMC_FP64 = 0,  /* every pair in FP64, exactly as the reference */
+  HMC_MIXED = 1  /* FP32 pair test; pairs inside its rigorous error band are
+                    re-tested in FP64 */
+};
+
+typedef struct {
+  uint32_t dim;                  /* 2 or 3 (Cartesian<D>) */
+  uint64_t n_bodies;             /* single-site point bodies */
+  double box_edges[3];           /* Hypercuboid edge lengths (first dim used) */
+  double boundary_range;         /* Periodic::new(maximum_interaction_range, ..)
+                                    (boundary/periodic.rs:96-110): ghost criterion */
+  double body_interaction_range; /* ParallelSweep::new(body_interaction_range, ..)
+                                    (parallel_sweep.rs:138-146): checkerboard width */
+  double spatial_cell_width;     /* cell width of the microstate's VecCell
+                                    (hoomd-spatial/src/vec_cell.rs:286-299); decides
+                                    which images are visited at exact contact */
+  int32_t model;                 /* HMC_HARD_SPHERE */
+  double params[4];
+  int32_t precision;             /* HMC_FP64 or HMC_MIXED */
+  int32_t device;                /* HIP device ordinal */
+} hmc_config;
+
+/* One trial of a trace: replaces the per-space BodyTrial status of
+ * parallel_sweep.rs:26-52 (status 0 Rejected, 1 Accepted). */
+typedef struct {
+  uint32_t substep;
+  uint32_t color;
+  uint64_t space;
+  int64_t body;
+  int32_t status;
+  int32_t pad;
+} hmc_trace_record;
+
+typedef struct {
+  uint64_t applies;         /* apply calls since creation */
+  uint64_t cycles;          /* colour cycles (= substeps consumed) */
+  uint64_t pair_tests;      /* not maintained (0) */
+  uint64_t band_retests;    /* MIXED: pairs re-tested in FP64 */
+  uint64_t exp_slow_path;   /* exp comparisons decided by the correctly rounded exp */
+  uint64_t guard_cycles;    /* cycles run by the single-workgroup overflow guard */
+} hmc_stats;
+
+/* Allocate device state for one system.  Replaces building a ParallelSweep and
+ * a device mirror of the Microstate.  HMC_ERR_UNSUPPORTED for anything the
+ * GPU path does not cover (the caller keeps the CPU sweep). */
+int hmc_create(const hmc_config* cfg, hmc_ctx** out);
+
+/* Upload body positions (the microstate's wrapped body properties, body index
+ * order).  site_pos may be NULL: sites are then wrap(body + 0) as
+ * Microstate::add_body / update_body_properties set them
+ * (hoomd-microstate/src/microstate.rs:777-835, 985-1034). */
+int hmc_set_state(hmc_ctx* ctx, const double* body_pos, const double* site_pos);
+
+/* Download body and site positions in body index order (site_pos may be NULL). */
+int hmc_get_state(hmc_ctx* ctx, double* body_pos, double* site_pos);
+
+/* One ParallelSweep::apply with Translate moves (parallel_sweep.rs:390-421).
+ * (step, substep, seed) = microstate.counter(); kT = macrostate.temperature();
+ * max_distance = Translate::maximum_distance.  On return *substeps_used holds
+ * the number of colour cycles: the caller increments the microstate substep by
+ * that much, as the reference's apply does. */
+int hmc_apply_translate(hmc_ctx* ctx, uint64_t step, uint32_t substep, uint32_t seed, double kT,
+                        double max_distance, hmc_count* out, uint32_t* substeps_used);
+
+/* n_steps iterations of { ParallelSweep::apply; Microstate::increment_step }
+ * (the body of HardSphereSim::advance, benchmarks/src/mc/hard_sphere.rs:90-111)
+ * starting at (step0, substep 0), with no host synchronisation between
+ * applies.  *out receives the summed count. */
+int hmc_run_translate(hmc_ctx* ctx, uint64_t step0, uint32_t seed, double kT, double max_distance,
+                      uint32_t n_steps, hmc_count* out);
+
+/* Record every non-empty trial of the following applies (testing/replay). */
+int hmc_trace_enable(hmc_ctx* ctx, int on, uint64_t capacity);
+int hmc_trace_read(hmc_ctx* ctx, hmc_trace_record* buf, uint64_t cap, uint64_t* n);
+
+int hmc_get_stats(hmc_ctx* ctx, hmc_stats* out);
+
+/* Average duration (ms) of the most recent timed launches of the pass kernel,
+ * measured with HIP events on the library's stream; 0 when not timed. */
+int hmc_profile_enable(hmc_ctx* ctx, int on);
+int hmc_profile_read(hmc_ctx* ctx, double* pass_ms_total, uint64_t* pass_launches, double* build_ms_total,
+                     uint64_t* builds);
+
+/* Evaluate the library's device math on n inputs (kind 0 cbrt, 1 correctly
+ * rounded log, 2 correctly rounded exp, 3 sqrt, 4 division in[2i]/in[2i+1]).
+ * Test hook for bit-exactness checks of the device arithmetic. */
+int hmc_debug_math(int device, int kind, const double* in, double* out, uint64_t n);
+
+/* Draw n standard normals (ziggurat) from stream (step, substep, seed, index). */
+int hmc_debug_normals(int device, uint64_t step, uint32_t substep, uint32_t seed, uint64_t index, double* out,
+                      uint64_t n);
+
+const char* hmc_last_error(const hmc_ctx* ctx);
+void hmc_destroy(hmc_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HMC_H */

@@ -333,7 +333,7 @@ struct System {
       }
       bt.pos = np;
       double dh = delta_energy_one(bi, np);
-      if (dh != INF && (dh <= 0.0 || rand_f64(rng) < std::exp(-dh / kt)))
+      if (dh != INF && (dh <= 0.0 || lib_lt_exp(rand_f64(rng), -dh / kt)))
         bt.status = Accepted;
       else
         bt.status = Rejected;
@@ -383,7 +383,7 @@ struct System {
     for (uint64_t bi = 0; bi < bodies.size(); ++bi) {
       Vec<D> np = boundary.wrap(propose_translate(rng, bodies[bi], max_move));
       double dh = delta_energy_one(bi, np);
-      if (dh != INF && (dh <= 0.0 || rand_f64(rng) < std::exp(-dh / kt))) {
+      if (dh != INF && (dh <= 0.0 || lib_lt_exp(rand_f64(rng), -dh / kt))) {
         update_body_properties(bi, np);
         ++acc;
       } else {

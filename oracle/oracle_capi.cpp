@@ -65,6 +65,23 @@ System<D>* make(const double* L, double boundary_range, double body_range, int m
 
 extern "C" {
 
+// ---- libm mode -----------------------------------------------------------------
+void oracle_set_libm(int cr) { libm().mode = cr; }
+void oracle_libm_counters(uint64_t* out) {
+  out[0] = libm().log_calls.load();
+  out[1] = libm().log_divergent.load();
+  out[2] = libm().exp_calls.load();
+  out[3] = libm().exp_divergent.load();
+}
+void oracle_libm_reset() {
+  libm().log_calls = 0;
+  libm().log_divergent = 0;
+  libm().exp_calls = 0;
+  libm().exp_divergent = 0;
+}
+double oracle_cr_log(double x) { return cr_log(x); }
+double oracle_cr_exp(double y) { return cr_exp(y); }
+
 // ---- RNG and adapter probes -------------------------------------------------
 void oracle_sfc64_stream(uint64_t a, uint64_t b, uint64_t c, uint64_t ctr, int warm, uint64_t* out, uint64_t n) {
   Sfc64 r;
