@@ -39,16 +39,19 @@ def parse():
 
 
 def pass_bytes(precision, n, n_spaces, nonempty, accepted_per_pass, dim=3):
-    """Algorithmic HBM bytes of one pass launch (one colour of one cycle):
-    the colour list and the chosen bodies' positions for the active spaces,
-    the cell start offsets and every site position of the 3^D neighbourhoods
-    (together they cover the whole system once), and the writes of accepted
-    moves."""
+    """Algorithmic HBM bytes of one pass launch (one colour of one cycle).
+
+    Reads: the cell start offsets of every space and the neighbour copy of
+    every site (the 3^D neighbourhoods of one colour's spaces cover the whole
+    system once; 16 B float4 in mixed mode, 32 B double4 in FP64), and for each
+    trial the chosen body's index (4 B) and FP64 position (32 B).  Writes: per
+    accepted move the body position and site (2 x 32 B) and the neighbour copy.
+    """
     colours = 2**dim
     active = nonempty / colours
     site = 16 if precision == "mixed" else 32
-    write = 80 if precision == "mixed" else 64
-    return 4 * active + 4 * (n_spaces + 1) + site * n + 32 * active + write * accepted_per_pass
+    write = 64 + site
+    return 4 * (n_spaces + 1) + site * n + 36 * active + write * accepted_per_pass
 
 
 def run_gpu(hmc, L, body, site, precision, steps, warmup, seed, kT, d):

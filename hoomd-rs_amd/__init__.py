@@ -41,7 +41,9 @@ class Config(C.Structure):
                 ("boundary_range", C.c_double), ("body_interaction_range", C.c_double),
                 ("spatial_cell_width", C.c_double),
                 ("model", C.c_int32), ("params", C.c_double * 4), ("precision", C.c_int32),
-                ("device", C.c_int32)]
+                ("device", C.c_int32), ("flags", C.c_uint32)]
+
+FLAG_GLOBAL_PASS = 1
 
 
 class Count(C.Structure):
@@ -50,7 +52,8 @@ class Count(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("applies", C.c_uint64), ("cycles", C.c_uint64), ("pair_tests", C.c_uint64),
-                ("band_retests", C.c_uint64), ("exp_slow_path", C.c_uint64), ("guard_cycles", C.c_uint64)]
+                ("band_retests", C.c_uint64), ("exp_slow_path", C.c_uint64), ("guard_cycles", C.c_uint64),
+                ("tile_overflows", C.c_uint64), ("tiled", C.c_uint32), ("tile_shape", C.c_uint32 * 3)]
 
 
 TRACE_DTYPE = np.dtype([("substep", "<u4"), ("color", "<u4"), ("space", "<u8"),
@@ -100,10 +103,12 @@ class GpuParallelSweep:
     Translate moves (hoomd-mc/src/parallel_sweep.rs:390-421)."""
 
     def __init__(self, box_edges, positions, *, sites=None, diameter=1.0, boundary_range=None,
-                 body_interaction_range=None, spatial_cell_width=None, precision="fp64", device=0):
+                 body_interaction_range=None, spatial_cell_width=None, precision="fp64", device=0,
+                 lds_tiles=True):
         """positions / sites: the microstate's body and site positions (already
         wrapped into the box, as Microstate::add_body stores them); sites=None
-        derives them as wrap(body + 0)."""
+        derives them as wrap(body + 0).  lds_tiles=False runs every trial from
+        global memory (same decisions)."""
         pos = np.ascontiguousarray(positions, np.float64)
         self.dim = pos.shape[1]
         self.n = pos.shape[0]
@@ -121,6 +126,7 @@ class GpuParallelSweep:
         cfg.params[0] = diameter
         cfg.precision = {"fp64": HMC_FP64, "mixed": HMC_MIXED}[precision]
         cfg.device = device
+        cfg.flags = 0 if lds_tiles else FLAG_GLOBAL_PASS
         h = C.c_void_p()
         rc = lib().hmc_create(C.byref(cfg), C.byref(h))
         if rc != HMC_OK:
@@ -175,7 +181,9 @@ class GpuParallelSweep:
     def stats(self):
         s = Stats()
         self._check(lib().hmc_get_stats(self.h, C.byref(s)))
-        return {k: int(getattr(s, k)) for k, _ in Stats._fields_}
+        out = {k: getattr(s, k) for k, _ in Stats._fields_}
+        out["tile_shape"] = tuple(int(v) for v in s.tile_shape)
+        return {k: v if k == "tile_shape" else int(v) for k, v in out.items()}
 
     def profile(self, on=True):
         self._check(lib().hmc_profile_enable(self.h, 1 if on else 0))

@@ -108,6 +108,7 @@ HMC_HD double hmc_log_cr(double x) {
   dd s = dd_mul(f, f);
   dd t = f;
   dd sum = f;
+#pragma unroll 1
   for (int k = 1; k <= 24; ++k) {
     t = dd_mul(t, s);
     dd term = dd_div(t, dd{(double)(2 * k + 1), 0.0});
@@ -123,6 +124,7 @@ HMC_HD double hmc_exp_cr(double y) {
   double k = round(y * 1.4426950408889634);
   dd r = dd_add(dd{y, 0.0}, dd_mul_d(dd_ln2(), -k));
   dd p = {1.0, 0.0};
+#pragma unroll 1
   for (int n = 27; n >= 1; --n) {
     p = dd_div(dd_mul(p, r), dd{(double)n, 0.0});
     p = dd_add(p, dd{1.0, 0.0});

@@ -62,18 +62,20 @@ __device__ __forceinline__ uint32_t choose_index(Sfc64& r, uint32_t len) {
 
 // StandardNormal by the symmetric 256-layer ziggurat with the Marsaglia tail.
 // The wedge test uses an exact exp comparison; the tail uses a correctly
-// rounded log.  *ambiguous counts slow-path exp decisions.
-__device__ __forceinline__ double standard_normal(Sfc64& r, unsigned* ambiguous) {
+// rounded log.  zx / zf: the layer tables (kZigX / kZigF or a copy of them in
+// LDS).  *ambiguous counts slow-path exp decisions.
+__device__ __forceinline__ double standard_normal(Sfc64& r, const double* zx, const double* zf,
+                                                  unsigned* ambiguous) {
   for (;;) {
     uint64_t bits = r.next();
     int i = (int)(bits & 0xff);
     double u = __longlong_as_double((long long)((bits >> 12) | 0x4000000000000000ull)) - 3.0;
-    double xi = kZigX[i];
+    double xi = zx[i];
     double x = u * xi;
-    double xn = kZigX[i + 1];
+    double xn = zx[i + 1];
     if (fabs(x) < xn) return x;
     if (i == 0) {
-      const double R = kZigX[1];
+      const double R = zx[1];
       double tx = 1.0, ty = 0.0;
       while (-2.0 * ty < tx * tx) {
         double x_ = open01(r);
@@ -83,7 +85,7 @@ __device__ __forceinline__ double standard_normal(Sfc64& r, unsigned* ambiguous)
       }
       return (u < 0.0) ? tx - R : R - tx;
     }
-    double f0 = kZigF[i], f1 = kZigF[i + 1];
+    double f0 = zf[i], f1 = zf[i + 1];
     double lhs = f1 + (f0 - f1) * uniform01(r);
     if (hmc_lt_exp(lhs, -x * x / 2.0, ambiguous)) return x;
   }

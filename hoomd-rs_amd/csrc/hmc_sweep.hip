@@ -3,31 +3,33 @@
 // hard-sphere pair energy.  Exposed through the C ABI in include/hmc.h.
 //
 // Per apply (ParallelSweep::apply, hoomd-mc/src/parallel_sweep.rs:390-421):
-//   setup    draw the checkerboard origin from Counter(step, substep, seed)
-//            .index(u64::MAX) (parallel_sweep.rs:182, hypercuboid.rs:315-341)
-//   build    counting sort of the bodies into checkerboard spaces: bin
-//            (atomic histogram) -> exclusive scan -> scatter -> place.  place
-//            orders each space by body index (the order ParallelSweep's
-//            per-space lists have, parallel_sweep.rs:194-200) and moves the
-//            body/site data into the new space-sorted slots, so every later
-//            neighbour gather reads contiguous memory.
-//   passes   for each colour cycle and colour: one lane per non-empty space
-//            of that colour runs the whole trial (choose, propose, wrap, space
-//            check, overlap scan of the 3^D neighbour spaces, accept, in-place
-//            write).  Same-colour spaces never touch each other's neighbour
-//            spaces, so in-place writes need no synchronisation.
-//   guard    the number of colour cycles is ceil(N / non-empty spaces); the
-//            host launches its best guess of pass kernels and one
-//            single-workgroup guard kernel runs any cycles beyond it.
+//   setup   draw the checkerboard origin from Counter(step, substep, seed)
+//           .index(u64::MAX) (parallel_sweep.rs:182, hypercuboid.rs:315-341)
+//   build   counting sort of the bodies into checkerboard spaces: bin (atomic
+//           histogram, body order) -> exclusive scan (which also counts the
+//           non-empty spaces and so the number of colour cycles) -> scatter ->
+//           place.  place orders each space by body index (the order of
+//           ParallelSweep's per-space lists, parallel_sweep.rs:194-200) and
+//           writes the cell arrays the passes read: the body index and the
+//           site position (FP32 in mixed mode, FP64 otherwise) of every slot.
+//   passes  for each colour cycle and colour: one lane per space of that
+//           colour runs the whole trial (choose, propose, wrap, space check,
+//           overlap scan of the 3^D neighbour spaces, accept, in-place write);
+//           lanes of empty spaces idle.  Same-colour spaces never touch each
+//           other's neighbour spaces, so in-place writes need no barriers.
+//   guard   the host launches its best guess of the cycle count; one
+//           single-workgroup kernel runs any cycles beyond it.
 //
-// Data layout in HBM: slot-ordered (space-sorted) structure of arrays,
-// double-buffered between applies: body index (u32), body position (double4),
-// site position + ghost flags (double4), site position in FP32 + flags
-// (float4, mixed mode only).
+// Data in HBM:
+//   bodies (body-index order, authoritative, FP64): body position double4,
+//       site position double4 with the ghost-image flags in .w
+//   cells (slot order = space order, rebuilt each apply): body index u32 and
+//       the neighbour copy of the site (float4 mixed / double4 FP64)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -42,6 +44,7 @@ namespace hmc {
 constexpr int kBlock = 256;
 constexpr int kScanBlock = 1024;  // elements per scan block (256 threads x 4)
 constexpr int kGuardThreads = 1024;
+constexpr int kAccSlots = 1024;
 
 // Constant description of one system, passed by value to the kernels.
 struct Geometry {
@@ -57,7 +60,8 @@ struct Geometry {
   double width[3];
   int32_t shape[3];
   uint32_t stride[3];
-  int32_t reach[3];  // neighbour spaces scanned on each side (1, or 2 when width ~ sigma)
+  uint32_t half_shape[3];  // spaces of one colour along each axis
+  int32_t reach[3];        // neighbour spaces scanned on each side (1, or 2 when width ~ sigma)
   double sigma2;
   double edge2;     // sigma^2 (1 - 2^-30): at or above it the spatial-structure rule is applied
   double vc_width;  // VecCell cell width of the reference microstate
@@ -75,23 +79,27 @@ struct ApplyState {
   double origin[3];
   uint32_t cycles;
   uint32_t nonempty;
-  uint32_t colour_count[8];
   uint32_t error;
   uint32_t pad;
-  unsigned long long accepted_total;
+  unsigned long long accepted_total;  // filled on the host from the slot counters
   unsigned long long trials_total;
   unsigned long long cycles_total;
   unsigned long long band_retests;
   unsigned long long exp_slow;
   unsigned long long guard_cycles;
+  unsigned long long tile_overflows;
   unsigned long long trace_n;
 };
 
-struct Slots {
-  uint32_t* body;
+struct Bodies {
   double4* bpos;
   double4* spos;
-  float4* spos32;
+};
+
+struct Cells {
+  uint32_t* body;
+  float4* pos32;
+  double4* pos64;
 };
 
 struct Work {
@@ -100,9 +108,9 @@ struct Work {
   uint32_t* elem_space;
   uint32_t* elem_rank;
   uint32_t* tmp_body;
-  uint32_t* tmp_src;
-  uint32_t* colour_list;
   uint32_t* block_sums;
+  uint32_t* block_ne;
+  unsigned long long* acc_slots;  // accepted moves, one counter per pass block slot
   hmc_trace_record* trace;
   uint64_t trace_cap;
   int tracing;
@@ -151,27 +159,30 @@ __device__ __forceinline__ uint32_t space_of(const Geometry& g, const double* or
   return index;
 }
 
+// The t-th space of colour c, in the order of
+// HypercuboidCheckerboard::construct_space_indices_by_color (hypercuboid.rs:231-255):
+// multi-index 2h + offset(c), h running over the half grid, last axis fastest.
 template <int D>
-__device__ __forceinline__ uint32_t colour_of(const Geometry& g, uint32_t s) {
-  uint32_t c = 0;
-  for (int a = 0; a < D; ++a) {
-    uint32_t m = (s / g.stride[a]) % (uint32_t)g.shape[a];
-    c = (c << 1) | (m & 1u);
+__device__ __forceinline__ uint32_t colour_space(const Geometry& g, uint32_t c, uint32_t t, int32_t* m) {
+  uint32_t s = 0;
+  for (int a = D - 1; a >= 0; --a) {
+    const uint32_t h = t % g.half_shape[a];
+    t /= g.half_shape[a];
+    const uint32_t off = (c >> (D - 1 - a)) & 1u;
+    m[a] = (int32_t)(2 * h + off);
+    s += (2 * h + off) * g.stride[a];
   }
-  return c;
+  return s;
 }
 
-__device__ __forceinline__ void pack_site(double4& out, const double* x, uint32_t flags) {
-  out.x = x[0];
-  out.y = x[1];
-  out.z = x[2];
-  out.w = __longlong_as_double((long long)flags);
+__device__ __forceinline__ double4 pack_site(const double* x, uint32_t flags) {
+  return make_double4(x[0], x[1], x[2], __longlong_as_double((long long)flags));
 }
-__device__ __forceinline__ void pack_site32(float4& out, const double* x, uint32_t flags) {
-  out.x = (float)x[0];
-  out.y = (float)x[1];
-  out.z = (float)x[2];
-  out.w = __int_as_float((int)flags);
+__device__ __forceinline__ float4 pack_site32(const double* x, uint32_t flags) {
+  return make_float4((float)x[0], (float)x[1], (float)x[2], __int_as_float((int)flags));
+}
+__device__ __forceinline__ float4 site32_of(double4 s) {
+  return make_float4((float)s.x, (float)s.y, (float)s.z, __int_as_float((int)__double_as_longlong(s.w)));
 }
 
 // Wave-aggregated count of a per-lane 0/1 flag.
@@ -205,7 +216,6 @@ __global__ void k_setup(Geometry g, ApplyState* st, uint64_t step, uint32_t subs
     double u = uniform01(rng);
     st->origin[a] = -g.L[a] / 2.0 - u * g.width[a];
   }
-  for (int c = 0; c < 8; ++c) st->colour_count[c] = 0;
   st->cycles = 0;
   st->nonempty = 0;
 }
@@ -215,22 +225,23 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
   if (i < n) p[i] = v;
 }
 
+// Space of every body (in body order) and its arrival rank in that space.
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_bin(Geometry g, ApplyState* st, Slots cur, Work w) {
-  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= g.n_bodies) return;
-  double4 p4 = cur.bpos[e];
+__global__ void __launch_bounds__(kBlock) k_bin(Geometry g, ApplyState* st, Bodies B, Work w) {
+  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= g.n_bodies) return;
+  double4 p4 = B.bpos[b];
   double x[3] = {p4.x, p4.y, p4.z};
   uint32_t s = space_of<D>(g, st->origin, x);
   if (s == UINT32_MAX) {
     atomicOr(&st->error, 1u);
     s = 0;
   }
-  w.elem_space[e] = s;
-  w.elem_rank[e] = atomicAdd(&w.cnt[s], 1u);
+  w.elem_space[b] = s;
+  w.elem_rank[b] = atomicAdd(&w.cnt[s], 1u);
 }
 
-// Exclusive scan of cnt[0..n) into start[0..n], start[n] = total.
+// Exclusive scan within a workgroup through LDS.
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds, uint32_t* total) {
   const int tid = threadIdx.x;
   const int nt = blockDim.x;
@@ -248,15 +259,17 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* l
   return incl - v;
 }
 
+// Scan of cnt in blocks of 1024; also counts the non-empty spaces per block.
 __global__ void __launch_bounds__(256) k_scan_blocks(const uint32_t* cnt, uint32_t* start, uint32_t* block_sums,
-                                                    uint32_t n) {
+                                                    uint32_t* block_ne, uint32_t n) {
   __shared__ uint32_t lds[256];
   uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * 4;
   uint32_t v[4];
-  uint32_t sum = 0;
+  uint32_t sum = 0, ne = 0;
   for (int k = 0; k < 4; ++k) {
     v[k] = (base + k < n) ? cnt[base + k] : 0;
     sum += v[k];
+    ne += v[k] ? 1u : 0u;
   }
   uint32_t total;
   uint32_t off = block_exclusive_scan(sum, lds, &total);
@@ -264,20 +277,28 @@ __global__ void __launch_bounds__(256) k_scan_blocks(const uint32_t* cnt, uint32
     if (base + k < n) start[base + k] = off;
     off += v[k];
   }
-  if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+  uint32_t ne_total;
+  block_exclusive_scan(ne, lds, &ne_total);
+  if (threadIdx.x == 0) {
+    block_sums[blockIdx.x] = total;
+    block_ne[blockIdx.x] = ne_total;
+  }
 }
 
-// Single workgroup: exclusive scan of the block sums (up to 1024 * 16).
-__global__ void __launch_bounds__(1024) k_scan_top(uint32_t* block_sums, uint32_t nblocks, uint32_t* start,
-                                                  uint32_t n) {
+// Single workgroup: exclusive scan of the block sums (up to 1024 * 16 blocks),
+// start[n] = N, and the colour-cycle count ceil(N / non-empty spaces).
+__global__ void __launch_bounds__(1024) k_scan_top(Geometry g, ApplyState* st, uint32_t* block_sums,
+                                                  const uint32_t* block_ne, uint32_t nblocks, uint32_t* start) {
   __shared__ uint32_t lds[1024];
   const int per = 16;
   uint32_t base = threadIdx.x * per;
   uint32_t v[per];
-  uint32_t sum = 0;
+  uint32_t sum = 0, ne = 0;
   for (int k = 0; k < per; ++k) {
-    v[k] = (base + k < nblocks) ? block_sums[base + k] : 0;
+    const bool in = base + k < nblocks;
+    v[k] = in ? block_sums[base + k] : 0;
     sum += v[k];
+    ne += in ? block_ne[base + k] : 0;
   }
   uint32_t total;
   uint32_t off = block_exclusive_scan(sum, lds, &total);
@@ -285,7 +306,18 @@ __global__ void __launch_bounds__(1024) k_scan_top(uint32_t* block_sums, uint32_
     if (base + k < nblocks) block_sums[base + k] = off;
     off += v[k];
   }
-  if (threadIdx.x == 0) start[n] = total;
+  uint32_t ne_total;
+  block_exclusive_scan(ne, lds, &ne_total);
+  if (threadIdx.x == 0) {
+    start[g.n_spaces] = total;
+    st->nonempty = ne_total;
+    // ParallelSweep::apply loops colour cycles until count.total() >= N
+    // (parallel_sweep.rs:402); each cycle counts one trial per non-empty space.
+    const uint32_t cycles = ne_total ? (g.n_bodies + ne_total - 1) / ne_total : 0;
+    st->cycles = cycles;
+    st->cycles_total += cycles;
+    st->trials_total += (unsigned long long)cycles * ne_total;
+  }
 }
 
 __global__ void k_scan_add(uint32_t* start, const uint32_t* block_sums, uint32_t n) {
@@ -293,53 +325,36 @@ __global__ void k_scan_add(uint32_t* start, const uint32_t* block_sums, uint32_t
   if (i < n) start[i] += block_sums[i / kScanBlock];
 }
 
-__global__ void k_scatter(Geometry g, Slots cur, Work w) {
-  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= g.n_bodies) return;
-  uint32_t dst = w.start[w.elem_space[e]] + w.elem_rank[e];
-  w.tmp_body[dst] = cur.body[e];
-  w.tmp_src[dst] = e;
+__global__ void k_scatter(Geometry g, Work w) {
+  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= g.n_bodies) return;
+  w.tmp_body[w.start[w.elem_space[b]] + w.elem_rank[b]] = b;
 }
 
-// Order each space by body index and move the data into the new slots.
-template <int D, int PREC>
-__global__ void __launch_bounds__(kBlock) k_place(Geometry g, ApplyState* st, Slots cur, Slots nxt, Work w) {
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+// Order each space by body index and write the cell arrays.
+template <int PREC>
+__global__ void __launch_bounds__(kBlock) k_place(Geometry g, Bodies B, Cells C, Work w) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= g.n_bodies) return;
-  uint32_t e = w.tmp_src[p];
-  uint32_t s = w.elem_space[e];
-  uint32_t b = w.tmp_body[p];
-  uint32_t s0 = w.start[s], s1 = w.start[s + 1];
+  const uint32_t b = w.tmp_body[p];
+  const uint32_t s = w.elem_space[b];
+  const uint32_t s0 = w.start[s], s1 = w.start[s + 1];
   uint32_t rank = 0;
   for (uint32_t q = s0; q < s1; ++q) rank += w.tmp_body[q] < b ? 1u : 0u;
-  uint32_t dst = s0 + rank;
-  nxt.body[dst] = b;
-  nxt.bpos[dst] = cur.bpos[e];
-  nxt.spos[dst] = cur.spos[e];
-  if (PREC == HMC_MIXED) nxt.spos32[dst] = cur.spos32[e];
-  if (rank == 0) {
-    uint32_t c = colour_of<D>(g, s);
-    uint32_t idx = atomicAdd(&st->colour_count[c], 1u);
-    w.colour_list[c * g.per_colour + idx] = s;
-  }
+  const uint32_t dst = s0 + rank;
+  C.body[dst] = b;
+  const double4 sp = B.spos[b];
+  if (PREC == HMC_MIXED)
+    C.pos32[dst] = site32_of(sp);
+  else
+    C.pos64[dst] = sp;
 }
 
-__global__ void k_finish_build(Geometry g, ApplyState* st) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  uint32_t ne = 0;
-  for (int c = 0; c < (1 << g.dim); ++c) ne += st->colour_count[c];
-  st->nonempty = ne;
-  uint32_t cycles = ne ? (g.n_bodies + ne - 1) / ne : 0;
-  st->cycles = cycles;
-  st->cycles_total += cycles;
-  st->trials_total += (unsigned long long)cycles * ne;
-}
-
-// Ball::sample (hoomd-vector/src/distribution.rs:39-70) added to the body
-// position (Translate::propose, hoomd-mc/src/translate/cartesian.rs:39-52).
+// Ball::sample (hoomd-vector/src/distribution.rs:39-70): the displacement of
+// Translate::propose (hoomd-mc/src/translate/cartesian.rs:39-52).
 template <int D>
-__device__ __forceinline__ void propose_translate(Sfc64& rng, double r, double* x, unsigned* exp_slow) {
-  double v[3];
+__device__ __forceinline__ void ball_sample(Sfc64& rng, double r, const double* zx, const double* zf, double* v,
+                                            unsigned* exp_slow) {
   if (D == 2) {
     for (;;) {
       double c0 = uniform01(rng);
@@ -350,9 +365,15 @@ __device__ __forceinline__ void propose_translate(Sfc64& rng, double r, double* 
       if (n2 < r * r) break;
     }
   } else {
-    double p0 = standard_normal(rng, exp_slow);
-    double p1 = standard_normal(rng, exp_slow);
-    double p2 = standard_normal(rng, exp_slow);
+    // three draws through one copy of the sampler (shift register, no array)
+    double p0 = 0.0, p1 = 0.0, p2 = 0.0;
+#pragma unroll 1
+    for (int i = 0; i < 3; ++i) {
+      double z = standard_normal(rng, zx, zf, exp_slow);
+      p0 = p1;
+      p1 = p2;
+      p2 = z;
+    }
     double norm = sqrt(fma(p2, p2, fma(p1, p1, fma(p0, p0, 0.0))));
     double q = r / norm;
     double sc = q * hmc_cbrt(uniform01(rng));
@@ -360,7 +381,6 @@ __device__ __forceinline__ void propose_translate(Sfc64& rng, double r, double* 
     v[1] = p1 * sc;
     v[2] = p2 * sc;
   }
-  for (int a = 0; a < D; ++a) x[a] += v[a];
 }
 
 // Does site j (FP64) overlap the trial site t?  Image per axis: the direct one
@@ -441,36 +461,81 @@ __device__ __forceinline__ int overlap32(const Geometry& g, const float* t, floa
   return 2;
 }
 
+// FP64 re-test of a pair inside the FP32 error band (rare; kept out of line
+// so the unrolled neighbour scan stays small).
+template <int D>
+__device__ __noinline__ bool retest64(const Geometry& g, const Bodies& B, const Cells& C, uint32_t q, double t0,
+                                      double t1, double t2) {
+  const double t[3] = {t0, t1, t2};
+  return overlap64<D>(g, t, B.spos[C.body[q]]);
+}
+
+// Overlap of the trial site with the site in slot q, in the context's
+// precision (the FP32 filter re-tests its error band in FP64).
+template <int D, int PREC>
+__device__ __forceinline__ bool pair_hits(const Geometry& g, const Bodies& B, const Cells& C, uint32_t q,
+                                          const double* t, const float* t32, unsigned& retests) {
+  if (PREC == HMC_MIXED) {
+    int r = overlap32<D>(g, t32, C.pos32[q]);
+    if (r == 2) {
+      ++retests;
+      r = retest64<D>(g, B, C, q, t[0], t[1], t[2]) ? 1 : 0;
+    }
+    return r != 0;
+  }
+  return overlap64<D>(g, t, C.pos64[q]);
+}
+
 struct TrialOut {
   unsigned accepted;
   unsigned retests;
   unsigned exp_slow;
 };
 
-// One trial in space s (parallel_sweep.rs:234-277).
-template <int D, int PREC>
-__device__ __forceinline__ void trial(const Geometry& g, const ApplyState* st, const Slots& S, const Work& w,
-                                      uint32_t s, uint32_t colour, uint32_t substep, TrialOut& o) {
-  const uint32_t s0 = w.start[s];
-  const uint32_t n = w.start[s + 1] - s0;
+template <int PREC>
+struct CellCopy;
+template <>
+struct CellCopy<HMC_MIXED> {
+  using type = float4;
+  __device__ static type load(const Cells& C, uint32_t q) { return C.pos32[q]; }
+};
+template <>
+struct CellCopy<HMC_FP64> {
+  using type = double4;
+  __device__ static type load(const Cells& C, uint32_t q) { return C.pos64[q]; }
+};
 
-  // neighbour spaces (periodic) within reach on each axis, without repeats
-  // when the axis has fewer than 2 * reach + 1 spaces
+// One trial in the non-empty space s (multi-index m) with slots [s0, s0 + n)
+// (parallel_sweep.rs:234-277).
+//
+// FAST (every axis has >= 3 spaces and the scan reaches one space out): the
+// 3^D neighbour ranges and the first body of the space are loaded before the
+// random draws and the proposal arithmetic, so their latency is hidden; the
+// neighbours are then tested in rounds -- round k tests the k-th slot of
+// every neighbour space -- whose loads are all issued before any is tested.
+// Otherwise a general loop visits the de-duplicated neighbour spaces (reach 1
+// or 2) one by one.
+template <int D, int PREC, bool FAST>
+__device__ __forceinline__ void trial(const Geometry& g, const ApplyState* st, const Bodies& B, const Cells& C,
+                                      const Work& w, const double* zx, const double* zf, uint32_t s,
+                                      const int32_t* m, uint32_t s0, uint32_t n, uint32_t colour, uint32_t substep,
+                                      TrialOut& o) {
+  constexpr int NC = D == 3 ? 27 : 9;
+  using VT = typename CellCopy<PREC>::type;
   uint32_t nb[3][5];
   int nn[3] = {1, 1, 1};
   for (int a = 0; a < D; ++a) {
     const int32_t sh = g.shape[a];
-    const int32_t m = (int32_t)((s / g.stride[a]) % (uint32_t)sh);
     const int32_t r = g.reach[a];
-    if (r == 1 && sh >= 3) {
-      nb[a][0] = (uint32_t)(m == 0 ? sh - 1 : m - 1) * g.stride[a];
-      nb[a][1] = (uint32_t)m * g.stride[a];
-      nb[a][2] = (uint32_t)(m == sh - 1 ? 0 : m + 1) * g.stride[a];
+    if (FAST || (r == 1 && sh >= 3)) {
+      nb[a][0] = (uint32_t)(m[a] == 0 ? sh - 1 : m[a] - 1) * g.stride[a];
+      nb[a][1] = (uint32_t)m[a] * g.stride[a];
+      nb[a][2] = (uint32_t)(m[a] == sh - 1 ? 0 : m[a] + 1) * g.stride[a];
       nn[a] = 3;
     } else {
       int k = 0;
-      for (int o = -r; o <= r; ++o) {
-        int32_t mm = ((m + o) % sh + sh) % sh;
+      for (int off = -r; off <= r; ++off) {
+        int32_t mm = ((m[a] + off) % sh + sh) % sh;
         bool seen = false;
         for (int j = 0; j < k; ++j) seen |= nb[a][j] == (uint32_t)mm * g.stride[a];
         if (!seen) nb[a][k++] = (uint32_t)mm * g.stride[a];
@@ -478,46 +543,94 @@ __device__ __forceinline__ void trial(const Geometry& g, const ApplyState* st, c
       nn[a] = k;
     }
   }
+  uint32_t lo[FAST ? NC : 1], cnt[FAST ? NC : 1];
+  if (FAST) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int i0 = D == 3 ? c / 9 : c / 3;
+      const int i1 = D == 3 ? (c / 3) % 3 : c % 3;
+      const int i2 = D == 3 ? c % 3 : 0;
+      const uint32_t ns = nb[0][i0] + nb[1][i1] + (D == 3 ? nb[2][i2] : 0u);
+      lo[c] = w.start[ns];
+      cnt[c] = w.start[ns + 1];
+    }
+  }
+  // the first body of the space (chosen whenever the space holds one body)
+  const uint32_t body0 = C.body[s0];
+  const double4 bp0 = B.bpos[body0];
 
   Sfc64 rng = counter_rng(st->step, substep, st->seed, s);
-  const uint32_t slot = s0 + choose_index(rng, n);
-  const double4 bp = S.bpos[slot];
-  double x[3] = {bp.x, bp.y, bp.z};
+  const uint32_t pick = choose_index(rng, n);
+  double v[3] = {0.0, 0.0, 0.0};
   unsigned exp_slow = 0;
-  propose_translate<D>(rng, st->max_move, x, &exp_slow);
+  ball_sample<D>(rng, st->max_move, zx, zf, v, &exp_slow);
   o.exp_slow = exp_slow;
+  const uint32_t slot = s0 + pick;
+  uint32_t body = body0;
+  double4 bp = bp0;
+  if (pick != 0) {
+    body = C.body[slot];
+    bp = B.bpos[body];
+  }
+  double x[3] = {bp.x, bp.y, bp.z};
+  for (int a = 0; a < D; ++a) x[a] += v[a];
   for (int a = 0; a < D; ++a) x[a] = wrap1(x[a], g.L[a]);  // generate_trial_moves: wrap(propose(..))
 
   bool accept = false;
   if (space_of<D>(g, st->origin, x) == s) {
     // trial site: wrap(body.transform(site)) (pairwise_cutoff.rs:436)
-    double t[3];
+    double t[3] = {0.0, 0.0, 0.0};
     for (int a = 0; a < D; ++a) t[a] = wrap1(x[a] + 0.0, g.L[a]);
-    float t32[3];
+    float t32[3] = {0.0f, 0.0f, 0.0f};
     for (int a = 0; a < D; ++a) t32[a] = (float)t[a];
     bool hit = false;
     unsigned retests = 0;
-    for (int i0 = 0; i0 < nn[0] && !hit; ++i0) {
-      for (int i1 = 0; i1 < nn[1] && !hit; ++i1) {
-        for (int i2 = 0; i2 < (D == 3 ? nn[2] : 1) && !hit; ++i2) {
-          uint32_t ns = nb[0][i0] + nb[1][i1] + (D == 3 ? nb[2][i2] : 0u);
-          uint32_t q0 = w.start[ns], q1 = w.start[ns + 1];
-          for (uint32_t q = q0; q < q1; ++q) {
-            if (q == slot) continue;
+    if (FAST) {
+      uint32_t rounds = 0;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        cnt[c] -= lo[c];
+        rounds = max(rounds, cnt[c]);
+      }
+      // loads in flight per batch: all neighbours for the FP32 copy, a third
+      // of them for the FP64 copy (register budget)
+      constexpr int CH = 9;
+      for (uint32_t k = 0; k < rounds && !hit; ++k) {
+#pragma unroll
+        for (int c0 = 0; c0 < NC; c0 += CH) {
+          VT vq[CH];
+#pragma unroll
+          for (int j = 0; j < CH; ++j) {
+            const int c = c0 + j;
+            const uint32_t q = k < cnt[c] ? lo[c] + k : slot;  // a valid slot either way
+            vq[j] = CellCopy<PREC>::load(C, q);
+          }
+#pragma unroll
+          for (int j = 0; j < CH; ++j) {
+            const int c = c0 + j;
+            const uint32_t q = lo[c] + k;
+            const bool valid = k < cnt[c] && q != slot;
             if (PREC == HMC_MIXED) {
-              int r = overlap32<D>(g, t32, S.spos32[q]);
-              if (r == 2) {
+              const int r = overlap32<D>(g, t32, *(const float4*)&vq[j]);
+              hit |= valid && r == 1;
+              if (valid && r == 2) {
                 ++retests;
-                r = overlap64<D>(g, t, S.spos[q]) ? 1 : 0;
+                hit |= retest64<D>(g, B, C, q, t[0], t[1], t[2]);
               }
-              if (r) {
-                hit = true;
-                break;
-              }
-            } else if (overlap64<D>(g, t, S.spos[q])) {
-              hit = true;
-              break;
+            } else {
+              hit |= valid && overlap64<D>(g, t, *(const double4*)&vq[j]);
             }
+          }
+        }
+      }
+    } else {
+      for (int i0 = 0; i0 < nn[0] && !hit; ++i0) {
+        for (int i1 = 0; i1 < nn[1] && !hit; ++i1) {
+          for (int i2 = 0; i2 < (D == 3 ? nn[2] : 1) && !hit; ++i2) {
+            const uint32_t ns = nb[0][i0] + nb[1][i1] + (D == 3 ? nb[2][i2] : 0u);
+            const uint32_t q0 = w.start[ns], q1 = w.start[ns + 1];
+            for (uint32_t q = q0; q < q1 && !hit; ++q)
+              if (q != slot) hit = pair_hits<D, PREC>(g, B, C, q, t, t32, retests);
           }
         }
       }
@@ -531,16 +644,14 @@ __device__ __forceinline__ void trial(const Geometry& g, const ApplyState* st, c
       double nbp[3] = {0.0, 0.0, 0.0}, nsp[3] = {0.0, 0.0, 0.0};
       for (int a = 0; a < D; ++a) nbp[a] = wrap1(x[a], g.L[a]);
       for (int a = 0; a < D; ++a) nsp[a] = wrap1(nbp[a] + 0.0, g.L[a]);
-      uint32_t fl = ghost_flags<D>(g, nsp);
-      S.bpos[slot] = make_double4(nbp[0], nbp[1], nbp[2], 0.0);
-      double4 sp;
-      pack_site(sp, nsp, fl);
-      S.spos[slot] = sp;
-      if (PREC == HMC_MIXED) {
-        float4 s32;
-        pack_site32(s32, nsp, fl);
-        S.spos32[slot] = s32;
-      }
+      const uint32_t fl = ghost_flags<D>(g, nsp);
+      B.bpos[body] = make_double4(nbp[0], nbp[1], nbp[2], 0.0);
+      const double4 sp = pack_site(nsp, fl);
+      B.spos[body] = sp;
+      if (PREC == HMC_MIXED)
+        C.pos32[slot] = pack_site32(nsp, fl);
+      else
+        C.pos64[slot] = sp;
     }
   }
   o.accepted = accept ? 1u : 0u;
@@ -551,7 +662,7 @@ __device__ __forceinline__ void trial(const Geometry& g, const ApplyState* st, c
       r.substep = substep;
       r.color = colour;
       r.space = s;
-      r.body = (int64_t)S.body[slot];
+      r.body = (int64_t)body;
       r.status = accept ? 1 : 0;
       r.pad = 0;
       w.trace[idx] = r;
@@ -559,44 +670,377 @@ __device__ __forceinline__ void trial(const Geometry& g, const ApplyState* st, c
   }
 }
 
-// One colour of one cycle: one lane per non-empty space of the colour.
-template <int D, int PREC>
-__global__ void __launch_bounds__(kBlock) k_pass(Geometry g, ApplyState* st, Slots S, Work w, uint32_t colour,
-                                                 uint32_t cycle, uint32_t xcd_chunk) {
-  if (cycle >= st->cycles) return;
+// Copy of the ziggurat tables in LDS (the normal sampler's table reads are
+// data dependent; LDS keeps them off the global-memory latency path).
+__device__ __forceinline__ void stage_zig(double* zx, double* zf) {
+  for (int i = threadIdx.x; i < 257; i += blockDim.x) {
+    zx[i] = kZigX[i];
+    zf[i] = kZigF[i];
+  }
+}
+
+// One colour of one cycle: one lane per space of the colour.
+template <int D, int PREC, bool FAST>
+__global__ void __launch_bounds__(kBlock) k_pass(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
+                                                 uint32_t colour, uint32_t cycle, uint32_t xcd_chunk) {
+  if (cycle >= st->cycles) return;  // uniform over the grid
+  __shared__ unsigned blk_acc;
+  __shared__ double zx[257], zf[257];
+  if (threadIdx.x == 0) blk_acc = 0;
+  if (D == 3) stage_zig(zx, zf);
+  __syncthreads();
   // XCD-aware block order: consecutive logical blocks (neighbouring spaces)
   // land on the same XCD and share its L2.
-  uint32_t b = blockIdx.x;
-  uint32_t lb = (b & 7u) * xcd_chunk + (b >> 3);
-  uint32_t t = lb * kBlock + threadIdx.x;
-  uint32_t count = st->colour_count[colour];
-  if (t >= count) return;
-  uint32_t s = w.colour_list[colour * g.per_colour + t];
+  const uint32_t b = blockIdx.x;
+  const uint32_t lb = (b & 7u) * xcd_chunk + (b >> 3);
+  const uint32_t t = lb * kBlock + threadIdx.x;
   TrialOut o{0, 0, 0};
-  trial<D, PREC>(g, st, S, w, s, colour, st->substep0 + cycle, o);
-  wave_count(&st->accepted_total, o.accepted != 0);
+  if (t < g.per_colour) {
+    int32_t m[3] = {0, 0, 0};
+    const uint32_t s = colour_space<D>(g, colour, t, m);
+    const uint32_t s0 = w.start[s];
+    const uint32_t n = w.start[s + 1] - s0;
+    if (n) trial<D, PREC, FAST>(g, st, B, C, w, zx, zf, s, m, s0, n, colour, st->substep0 + cycle, o);
+  }
+  const unsigned long long mk = __ballot(o.accepted != 0);
+  if (__lane_id() == 0 && mk) atomicAdd(&blk_acc, (unsigned)__popcll(mk));
   lane_add(&st->band_retests, o.retests);
   lane_add(&st->exp_slow, o.exp_slow);
+  __syncthreads();
+  if (threadIdx.x == 0 && blk_acc) atomicAdd(&w.acc_slots[b & (kAccSlots - 1)], (unsigned long long)blk_acc);
+}
+
+// ------------------------------------------------------- LDS-staged tiles --
+// A workgroup runs the trials of a T0 x T1 (x T2) block of one colour's
+// spaces.  Their 3^D neighbourhoods form a box of (2T+1)^D spaces.  A trial
+// reads only the neighbours of its own space, which are never spaces of the
+// running colour (every neighbour differs by one in some axis and the space
+// counts are even), so nothing a trial reads changes during the pass and a
+// snapshot of the box taken at its start is exactly what every trial must
+// see.  (In the last, partial tile along an axis the box wraps onto spaces
+// of other tiles; those cells are staged but no active trial reads them.)
+// The box is staged into LDS with batched, coalesced loads (cell ranges,
+// then the neighbour copies of the sites); the trials then read their
+// neighbours from LDS.
+struct TileCfg {
+  uint32_t T[3];     // colour spaces per tile along each axis (1 for unused axes)
+  uint32_t nt[3];    // tiles along each axis
+  uint32_t E[3];     // staged box edge 2T + 1 (1 for unused axes)
+  uint32_t trials;   // T0 T1 T2
+  uint32_t ncell;    // E0 E1 E2
+  uint32_t maxb;     // body capacity of the LDS copy
+  uint32_t ntiles;
+  uint32_t xcd_chunk;
+  uint32_t lds_bytes;
+};
+
+constexpr int kTileThreads = 256;        // upper bound of the workgroup size
+constexpr int kCellsPerThread = 16;      // staged cells per thread (ncell <= 16 x threads)
+constexpr int kStageRanks = 3;           // per-cell bodies copied in unrolled batches
+constexpr size_t kTileLds = 72 * 1024;   // dynamic LDS per workgroup
+
+// Global slot of the LDS copy `idx`, which belongs to staged cell k (box
+// origin mlo).  Only the rare FP64 band re-test needs it.
+template <int D>
+__device__ __noinline__ uint32_t staged_slot(const Geometry& g, const Work& w, const TileCfg& tc,
+                                             const uint32_t* cinfo, const int32_t* mlo, uint32_t k,
+                                             uint32_t idx) {
+  uint32_t ns = 0, r = k;
+  for (int a = D - 1; a >= 0; --a) {
+    const int32_t sh = g.shape[a];
+    int32_t mm = mlo[a] + (int32_t)(r % tc.E[a]);
+    r /= tc.E[a];
+    mm = mm < 0 ? mm + sh : (mm >= sh ? mm - sh : mm);
+    ns += (uint32_t)mm * g.stride[a];
+  }
+  return w.start[ns] + (idx - (cinfo[k] >> 16));
+}
+
+// One trial of the tile: space s with slots [s0, s0 + n), centre cell cc of
+// the staged box (parallel_sweep.rs:234-277, as trial() above).
+template <int D, int PREC>
+__device__ __forceinline__ void trial_lds(const Geometry& g, const ApplyState* st, const Bodies& B,
+                                          const Cells& C, const Work& w, const TileCfg& tc,
+                                          const uint32_t* cinfo, const int32_t* mlo,
+                                          const typename CellCopy<PREC>::type* lb, const double* zx,
+                                          const double* zf, uint32_t s, const uint32_t* cc, uint32_t s0,
+                                          uint32_t n, uint32_t body0, double4 bp0, uint32_t colour,
+                                          uint32_t substep, TrialOut& o) {
+  using VT = typename CellCopy<PREC>::type;
+  constexpr int NR = D == 3 ? 9 : 3;  // rows of three cells along the last axis
+  uint32_t rlo[NR], rhi[NR], rcell[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int i0 = D == 3 ? r / 3 : r;
+    const int i1 = D == 3 ? r % 3 : 0;
+    uint32_t kb;
+    if (D == 3)
+      kb = ((cc[0] + i0 - 1) * tc.E[1] + (cc[1] + i1 - 1)) * tc.E[2] + cc[2] - 1;
+    else
+      kb = (cc[0] + i0 - 1) * tc.E[1] + cc[1] - 1;
+    const uint32_t a = cinfo[kb], z = cinfo[kb + 2];
+    rcell[r] = kb;
+    rlo[r] = a >> 16;
+    rhi[r] = (z >> 16) + (z & 0xffffu);
+  }
+  const uint32_t kcen = D == 3 ? (cc[0] * tc.E[1] + cc[1]) * tc.E[2] + cc[2] : cc[0] * tc.E[1] + cc[1];
+
+  Sfc64 rng = counter_rng(st->step, substep, st->seed, s);
+  const uint32_t pick = choose_index(rng, n);
+  double v[3] = {0.0, 0.0, 0.0};
+  unsigned exp_slow = 0;
+  ball_sample<D>(rng, st->max_move, zx, zf, v, &exp_slow);
+  o.exp_slow = exp_slow;
+  const uint32_t slot = s0 + pick;
+  const uint32_t self = (cinfo[kcen] >> 16) + pick;
+  uint32_t body = body0;
+  double4 bp = bp0;
+  if (pick != 0) {
+    body = C.body[slot];
+    bp = B.bpos[body];
+  }
+  double x[3] = {bp.x, bp.y, bp.z};
+  for (int a = 0; a < D; ++a) x[a] += v[a];
+  for (int a = 0; a < D; ++a) x[a] = wrap1(x[a], g.L[a]);  // generate_trial_moves: wrap(propose(..))
+
+  bool accept = false;
+  if (space_of<D>(g, st->origin, x) == s) {
+    double t[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < D; ++a) t[a] = wrap1(x[a] + 0.0, g.L[a]);  // pairwise_cutoff.rs:436
+    float t32[3] = {0.0f, 0.0f, 0.0f};
+    for (int a = 0; a < D; ++a) t32[a] = (float)t[a];
+    bool hit = false;
+    unsigned retests = 0;
+    uint32_t rounds = 0;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) rounds = max(rounds, rhi[r] - rlo[r]);
+    for (uint32_t k = 0; k < rounds && !hit; ++k) {
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const uint32_t idx = rlo[r] + k;
+        const bool valid = idx < rhi[r] && idx != self;
+        const VT vq = lb[valid ? idx : self];
+        if (PREC == HMC_MIXED) {
+          const int res = overlap32<D>(g, t32, *(const float4*)&vq);
+          hit |= valid && res == 1;
+          if (valid && res == 2) {
+            // global slot of the LDS entry: one of the row's three cells
+            const uint32_t c1 = cinfo[rcell[r] + 1] >> 16, c2 = cinfo[rcell[r] + 2] >> 16;
+            const uint32_t cell = idx < c1 ? rcell[r] : (idx < c2 ? rcell[r] + 1 : rcell[r] + 2);
+            const uint32_t q = staged_slot<D>(g, w, tc, cinfo, mlo, cell, idx);
+            ++retests;
+            hit |= retest64<D>(g, B, C, q, t[0], t[1], t[2]);
+          }
+        } else {
+          hit |= valid && overlap64<D>(g, t, *(const double4*)&vq);
+        }
+      }
+    }
+    o.retests = retests;
+    accept = !hit;  // hard spheres: dH is 0 or +inf (parallel_sweep.rs:267-273)
+    if (accept) {
+      // update_body_properties (microstate.rs:985-1034)
+      double nbp[3] = {0.0, 0.0, 0.0}, nsp[3] = {0.0, 0.0, 0.0};
+      for (int a = 0; a < D; ++a) nbp[a] = wrap1(x[a], g.L[a]);
+      for (int a = 0; a < D; ++a) nsp[a] = wrap1(nbp[a] + 0.0, g.L[a]);
+      const uint32_t fl = ghost_flags<D>(g, nsp);
+      B.bpos[body] = make_double4(nbp[0], nbp[1], nbp[2], 0.0);
+      const double4 sp = pack_site(nsp, fl);
+      B.spos[body] = sp;
+      if (PREC == HMC_MIXED)
+        C.pos32[slot] = pack_site32(nsp, fl);
+      else
+        C.pos64[slot] = sp;
+    }
+  }
+  o.accepted = accept ? 1u : 0u;
+  if (w.tracing) {
+    unsigned long long idx = atomicAdd((unsigned long long*)&st->trace_n, 1ull);
+    if (idx < w.trace_cap) {
+      hmc_trace_record r;
+      r.substep = substep;
+      r.color = colour;
+      r.space = s;
+      r.body = (int64_t)body;
+      r.status = accept ? 1 : 0;
+      r.pad = 0;
+      w.trace[idx] = r;
+    }
+  }
+}
+
+template <int D, int PREC>
+__global__ void __launch_bounds__(kTileThreads) k_pass_tile(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
+                                                            uint32_t colour, uint32_t cycle, TileCfg tc) {
+  using VT = typename CellCopy<PREC>::type;
+  if (cycle >= st->cycles) return;  // uniform over the grid
+  const uint32_t b = blockIdx.x;
+  const uint32_t tile = (b & 7u) * tc.xcd_chunk + (b >> 3);  // XCD-aware tile order
+  if (tile >= tc.ntiles) return;                              // uniform over the block
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint32_t* cinfo = (uint32_t*)smem;  // per staged cell: LDS offset << 16 | count
+  VT* lb = (VT*)(smem + ((tc.ncell * 4u + 15u) & ~15u));
+  __shared__ double zx[257], zf[257];
+  __shared__ uint32_t scan_tmp[kTileThreads];
+  __shared__ unsigned blk_acc;
+  const uint32_t tid = threadIdx.x, NT = blockDim.x;
+
+  uint32_t tcd[3] = {0, 0, 0};
+  {
+    uint32_t r = tile;
+    for (int a = D - 1; a >= 0; --a) {
+      tcd[a] = r % tc.nt[a];
+      r /= tc.nt[a];
+    }
+  }
+  // this thread's trial space
+  bool active = tid < tc.trials;
+  uint32_t cc[3] = {0, 0, 0};
+  uint32_t s = 0;
+  {
+    uint32_t r = tid;
+    for (int a = D - 1; a >= 0; --a) {
+      const uint32_t l = r % tc.T[a];
+      r /= tc.T[a];
+      const uint32_t h = tcd[a] * tc.T[a] + l;
+      if (h >= g.half_shape[a]) active = false;
+      const uint32_t off = (colour >> (D - 1 - a)) & 1u;
+      s += (2 * h + off) * g.stride[a];
+      cc[a] = 2 * l + 1;  // centre cell in box coordinates
+    }
+  }
+  uint32_t s0 = 0, n = 0, body0 = 0;
+  double4 bp0 = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (active) {
+    s0 = w.start[s];
+    n = w.start[s + 1] - s0;
+    if (n) {
+      body0 = C.body[s0];
+      bp0 = B.bpos[body0];
+    }
+  }
+  if (D == 3) stage_zig(zx, zf);
+  if (tid == 0) blk_acc = 0;
+
+  // 1. ranges of the staged cells (box origin: 2 T tcd + offset - 1 per axis)
+  int32_t mlo[3] = {0, 0, 0};
+  for (int a = 0; a < D; ++a) mlo[a] = (int32_t)(2 * tcd[a] * tc.T[a] + ((colour >> (D - 1 - a)) & 1u)) - 1;
+  uint32_t src[kCellsPerThread];  // first global slot of each of this thread's cells
+  {
+    uint32_t a1[kCellsPerThread];
+#pragma unroll
+    for (int i = 0; i < kCellsPerThread; ++i) {
+      const uint32_t k = tid + i * NT;
+      uint32_t ns = 0;
+      uint32_t r = k < tc.ncell ? k : 0;
+      for (int a = D - 1; a >= 0; --a) {
+        const int32_t sh = g.shape[a];
+        int32_t mm = mlo[a] + (int32_t)(r % tc.E[a]);
+        r /= tc.E[a];
+        mm = mm < 0 ? mm + sh : (mm >= sh ? mm - sh : mm);
+        ns += (uint32_t)mm * g.stride[a];
+      }
+      src[i] = w.start[ns];
+      a1[i] = w.start[ns + 1];
+    }
+#pragma unroll
+    for (int i = 0; i < kCellsPerThread; ++i) {
+      const uint32_t k = tid + i * NT;
+      if (k < tc.ncell) cinfo[k] = a1[i] - src[i];
+    }
+  }
+  __syncthreads();
+  // 2. LDS offsets: exclusive scan of the counts over contiguous chunks
+  uint32_t total = 0;
+  {
+    const uint32_t per = (tc.ncell + NT - 1) / NT;
+    const uint32_t k0 = min(tid * per, tc.ncell), k1 = min(k0 + per, tc.ncell);
+    uint32_t sum = 0;
+    for (uint32_t k = k0; k < k1; ++k) sum += cinfo[k];
+    uint32_t off = block_exclusive_scan(sum, scan_tmp, &total);
+    for (uint32_t k = k0; k < k1; ++k) {
+      const uint32_t c = cinfo[k];
+      cinfo[k] = (min(off, 0xffffu) << 16) | c;
+      off += c;
+    }
+  }
+  __syncthreads();
+  const bool overflow = total > tc.maxb;
+  if (!overflow) {
+    // 3. copy the neighbour copies of the sites, rank by rank in unrolled batches
+    uint32_t info[kCellsPerThread];
+    uint32_t most = 0;
+#pragma unroll
+    for (int i = 0; i < kCellsPerThread; ++i) {
+      const uint32_t k = tid + i * NT;
+      info[i] = k < tc.ncell ? cinfo[k] : 0u;
+      most = max(most, info[i] & 0xffffu);
+    }
+#pragma unroll
+    for (int r = 0; r < kStageRanks; ++r) {
+      VT tmp[kCellsPerThread];
+#pragma unroll
+      for (int i = 0; i < kCellsPerThread; ++i) {
+        const bool ok = (uint32_t)r < (info[i] & 0xffffu);
+        tmp[i] = CellCopy<PREC>::load(C, ok ? src[i] + r : 0u);
+      }
+#pragma unroll
+      for (int i = 0; i < kCellsPerThread; ++i)
+        if ((uint32_t)r < (info[i] & 0xffffu)) lb[(info[i] >> 16) + r] = tmp[i];
+    }
+    if (most > (uint32_t)kStageRanks) {
+#pragma unroll
+      for (int i = 0; i < kCellsPerThread; ++i)
+        for (uint32_t r = kStageRanks; r < (info[i] & 0xffffu); ++r)
+          lb[(info[i] >> 16) + r] = CellCopy<PREC>::load(C, src[i] + r);
+    }
+  }
+  __syncthreads();
+  // 4. the trials
+  TrialOut o{0, 0, 0};
+  if (active && n) {
+    if (!overflow)
+      trial_lds<D, PREC>(g, st, B, C, w, tc, cinfo, mlo, lb, zx, zf, s, cc, s0, n, body0, bp0, colour,
+                         st->substep0 + cycle, o);
+    else {
+      int32_t m[3] = {0, 0, 0};
+      for (int a = 0; a < D; ++a) m[a] = (int32_t)((s / g.stride[a]) % (uint32_t)g.shape[a]);
+      trial<D, PREC, false>(g, st, B, C, w, zx, zf, s, m, s0, n, colour, st->substep0 + cycle, o);
+    }
+  }
+  const unsigned long long mk = __ballot(o.accepted != 0);
+  if (__lane_id() == 0 && mk) atomicAdd(&blk_acc, (unsigned)__popcll(mk));
+  lane_add(&st->band_retests, o.retests);
+  lane_add(&st->exp_slow, o.exp_slow);
+  if (tid == 0 && overflow) atomicAdd(&st->tile_overflows, 1ull);
+  __syncthreads();
+  if (tid == 0 && blk_acc) atomicAdd(&w.acc_slots[b & (kAccSlots - 1)], (unsigned long long)blk_acc);
 }
 
 // Runs the cycles beyond the ones the host launched, in one workgroup.
 template <int D, int PREC>
-__global__ void __launch_bounds__(kGuardThreads) k_guard(Geometry g, ApplyState* st, Slots S, Work w,
+__global__ void __launch_bounds__(kGuardThreads) k_guard(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
                                                          uint32_t first_cycle) {
   const uint32_t cycles = st->cycles;
+  if (first_cycle >= cycles) return;
+  __shared__ double zx[257], zf[257];
+  if (D == 3) stage_zig(zx, zf);
+  __syncthreads();
   const int ncol = 1 << D;
+  const uint32_t rounds = (g.per_colour + kGuardThreads - 1) / kGuardThreads;
   for (uint32_t k = first_cycle; k < cycles; ++k) {
     for (int c = 0; c < ncol; ++c) {
-      const uint32_t count = st->colour_count[c];
-      const uint32_t rounds = (count + kGuardThreads - 1) / kGuardThreads;
       for (uint32_t r = 0; r < rounds; ++r) {
         const uint32_t t = r * kGuardThreads + threadIdx.x;
         TrialOut o{0, 0, 0};
-        if (t < count) {
-          uint32_t s = w.colour_list[c * g.per_colour + t];
-          trial<D, PREC>(g, st, S, w, s, (uint32_t)c, st->substep0 + k, o);
+        if (t < g.per_colour) {
+          int32_t m[3] = {0, 0, 0};
+          const uint32_t s = colour_space<D>(g, (uint32_t)c, t, m);
+          const uint32_t s0 = w.start[s];
+          const uint32_t n = w.start[s + 1] - s0;
+          if (n) trial<D, PREC, false>(g, st, B, C, w, zx, zf, s, m, s0, n, (uint32_t)c, st->substep0 + k, o);
         }
-        wave_count(&st->accepted_total, o.accepted != 0);
+        wave_count(&w.acc_slots[0], o.accepted != 0);
         lane_add(&st->band_retests, o.retests);
         lane_add(&st->exp_slow, o.exp_slow);
       }
@@ -606,30 +1050,23 @@ __global__ void __launch_bounds__(kGuardThreads) k_guard(Geometry g, ApplyState*
   }
 }
 
-// Initial site positions and flags from body positions.
-template <int D, int PREC>
-__global__ void k_init_sites(Geometry g, Slots S, int have_sites) {
+// Site positions (given, or wrap(body + 0)) and their ghost flags.
+template <int D>
+__global__ void k_init_sites(Geometry g, Bodies B, int have_sites) {
   uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= g.n_bodies) return;
-  double4 b = S.bpos[e];
+  double4 b = B.bpos[e];
   double x[3] = {b.x, b.y, b.z};
   double s[3] = {0.0, 0.0, 0.0};
   if (have_sites) {
-    double4 sp = S.spos[e];
+    double4 sp = B.spos[e];
     s[0] = sp.x;
     s[1] = sp.y;
     s[2] = sp.z;
   } else {
     for (int a = 0; a < D; ++a) s[a] = wrap1(x[a] + 0.0, g.L[a]);
   }
-  uint32_t fl = ghost_flags<D>(g, s);
-  double4 sp;
-  pack_site(sp, s, fl);
-  S.spos[e] = sp;
-  float4 s32;
-  pack_site32(s32, s, fl);
-  S.spos32[e] = s32;
-  S.body[e] = e;
+  B.spos[e] = pack_site(s, ghost_flags<D>(g, s));
 }
 
 // ------------------------------------------------------------- debug hooks --
@@ -652,7 +1089,7 @@ __global__ void k_debug_normals(uint64_t step, uint32_t substep, uint32_t seed, 
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   Sfc64 r = counter_rng(step, substep, seed, index);
   unsigned slow = 0;
-  for (uint64_t i = 0; i < n; ++i) out[i] = standard_normal(r, &slow);
+  for (uint64_t i = 0; i < n; ++i) out[i] = standard_normal(r, kZigX, kZigF, &slow);
 }
 
 }  // namespace hmc
@@ -665,11 +1102,15 @@ struct hmc_ctx {
   Geometry g;
   int device;
   hipStream_t stream = nullptr;
-  Slots a{}, b{};  // a: current state, b: next
+  Bodies bodies{};
+  Cells cells{};
   Work w{};
   ApplyState* st = nullptr;
   bool have_state = false;
   uint32_t cycles_guess = 1;
+  bool fast = false;  // every axis: >= 3 spaces and reach 1
+  bool tiled = false;  // LDS-staged pass (requires fast)
+  TileCfg tc{};
   uint32_t n_scan_blocks = 0;
   uint64_t applies = 0;
   std::string err;
@@ -720,43 +1161,49 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->profiling) {
     e0 = take_event(c);
-    hipEventRecord(e0, s);
+    (void)hipEventRecord(e0, s);
   }
   k_setup<<<1, 64, 0, s>>>(g, c->st, step, substep, seed, kT, max_move, from_args, advance);
   k_fill_u32<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.cnt, g.n_spaces, 0u);
-  k_bin<D><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->a, c->w);
-  k_scan_blocks<<<c->n_scan_blocks, 256, 0, s>>>(c->w.cnt, c->w.start, c->w.block_sums, g.n_spaces);
-  k_scan_top<<<1, 1024, 0, s>>>(c->w.block_sums, c->n_scan_blocks, c->w.start, g.n_spaces);
+  if (N) k_bin<D><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
+  k_scan_blocks<<<c->n_scan_blocks, 256, 0, s>>>(c->w.cnt, c->w.start, c->w.block_sums, c->w.block_ne, g.n_spaces);
+  k_scan_top<<<1, 1024, 0, s>>>(g, c->st, c->w.block_sums, c->w.block_ne, c->n_scan_blocks, c->w.start);
   k_scan_add<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.start, c->w.block_sums, g.n_spaces);
-  k_scatter<<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->a, c->w);
-  k_place<D, PREC><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->a, c->b, c->w);
-  k_finish_build<<<1, 64, 0, s>>>(g, c->st);
+  if (N) {
+    k_scatter<<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->w);
+    k_place<PREC><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->bodies, c->cells, c->w);
+  }
   if (c->profiling) {
     e1 = take_event(c);
-    hipEventRecord(e1, s);
+    (void)hipEventRecord(e1, s);
     c->build_events.push_back({e0, e1});
   }
-  // passes on the new slots
   const uint32_t blocks = grid_for(g.per_colour, kBlock);
   const uint32_t chunk = (blocks + 7) / 8;
   const uint32_t grid = chunk * 8;
+  const uint32_t tile_threads = ((c->tc.trials + 63) / 64) * 64;
   for (uint32_t k = 0; k < c->cycles_guess; ++k) {
     for (int col = 0; col < (1 << D); ++col) {
       hipEvent_t p0 = nullptr, p1 = nullptr;
       if (c->profiling) {
         p0 = take_event(c);
-        hipEventRecord(p0, s);
+        (void)hipEventRecord(p0, s);
       }
-      k_pass<D, PREC><<<grid, kBlock, 0, s>>>(g, c->st, c->b, c->w, (uint32_t)col, k, chunk);
+      if (c->tiled)
+        k_pass_tile<D, PREC><<<c->tc.xcd_chunk * 8, tile_threads, c->tc.lds_bytes, s>>>(
+            g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k, c->tc);
+      else if (c->fast)
+        k_pass<D, PREC, true><<<grid, kBlock, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k, chunk);
+      else
+        k_pass<D, PREC, false><<<grid, kBlock, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k, chunk);
       if (c->profiling) {
         p1 = take_event(c);
-        hipEventRecord(p1, s);
+        (void)hipEventRecord(p1, s);
         c->pass_events.push_back({p0, p1});
       }
     }
   }
-  k_guard<D, PREC><<<1, kGuardThreads, 0, s>>>(g, c->st, c->b, c->w, c->cycles_guess);
-  std::swap(c->a, c->b);
+  k_guard<D, PREC><<<1, kGuardThreads, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, c->cycles_guess);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(c, HMC_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
   c->applies += 1;
@@ -772,28 +1219,31 @@ int dispatch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32
   return launch_apply<3, HMC_MIXED>(c, from_args, advance, step, substep, seed, kT, max_move);
 }
 
+void collect_events(hmc_ctx* c, std::vector<std::pair<hipEvent_t, hipEvent_t>>& evs, double& ms_total,
+                    uint64_t& n) {
+  for (auto& pr : evs) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+      ms_total += ms;
+      n += 1;
+    }
+    c->ev_pool.push_back(pr.first);
+    c->ev_pool.push_back(pr.second);
+  }
+  evs.clear();
+}
+
 int read_state(hmc_ctx* c, ApplyState* out) {
+  std::vector<unsigned long long> slots(kAccSlots);
   HIPCHK(c, hipMemcpyAsync(out, c->st, sizeof(ApplyState), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(slots.data(), c->w.acc_slots, kAccSlots * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  out->accepted_total = 0;
+  for (unsigned long long v : slots) out->accepted_total += v;
   if (c->profiling) {
-    for (auto& pr : c->pass_events) {
-      float ms = 0.f;
-      hipEventElapsedTime(&ms, pr.first, pr.second);
-      c->pass_ms += ms;
-      c->pass_n += 1;
-      c->ev_pool.push_back(pr.first);
-      c->ev_pool.push_back(pr.second);
-    }
-    for (auto& pr : c->build_events) {
-      float ms = 0.f;
-      hipEventElapsedTime(&ms, pr.first, pr.second);
-      c->build_ms += ms;
-      c->build_n += 1;
-      c->ev_pool.push_back(pr.first);
-      c->ev_pool.push_back(pr.second);
-    }
-    c->pass_events.clear();
-    c->build_events.clear();
+    collect_events(c, c->pass_events, c->pass_ms, c->pass_n);
+    collect_events(c, c->build_events, c->build_ms, c->build_n);
   }
   if (out->error) return fail(c, HMC_ERR_COVER, "a body lies outside the checkerboard");
   return HMC_OK;
@@ -802,6 +1252,62 @@ int read_state(hmc_ctx* c, ApplyState* out) {
 }  // namespace
 
 extern "C" {
+
+// Tile shape of the LDS-staged pass for a system with `occ` bodies per space
+// on a device with `cus` CUs.  Candidates: at most kTileThreads trials, at
+// most kCellsPerThread staged cells per thread, at most kTileLds bytes of
+// dynamic LDS with a body capacity of 1.2x the box's expected count plus 64
+// (a tile whose box holds more runs its trials from global memory).  A
+// workgroup's time is set by a fixed number of dependent memory phases, so
+// the shape that needs the fewest rounds of resident workgroups wins; among
+// those, the least launched lanes and staging.
+static bool choose_tiles(const Geometry& g, int D, double occ, size_t vt, int cus, TileCfg* out) {
+  const uint32_t lim[3] = {std::min<uint32_t>(g.half_shape[0], 16), D > 1 ? std::min<uint32_t>(g.half_shape[1], 16) : 1,
+                           D > 2 ? std::min<uint32_t>(g.half_shape[2], 32) : 1};
+  const size_t static_lds = 2 * 257 * sizeof(double) + kTileThreads * sizeof(uint32_t) + 16;
+  const size_t cu_lds = 160 * 1024;
+  const uint32_t cu_waves = 8;  // two waves per SIMD at the kernel's register use
+  double best_rounds = 0.0, best_work = 0.0;
+  bool found = false;
+  for (uint32_t t0 = 1; t0 <= lim[0]; ++t0)
+    for (uint32_t t1 = 1; t1 <= lim[1]; ++t1)
+      for (uint32_t t2 = 1; t2 <= lim[2]; ++t2) {
+        const uint32_t T[3] = {t0, t1, t2};
+        const uint32_t trials = T[0] * T[1] * T[2];
+        if (trials > (uint32_t)kTileThreads) continue;
+        const uint32_t threads = ((trials + 63) / 64) * 64;
+        uint32_t ncell = 1, ntiles = 1;
+        for (int a = 0; a < D; ++a) {
+          ncell *= 2 * T[a] + 1;
+          ntiles *= (g.half_shape[a] + T[a] - 1) / T[a];
+        }
+        if (ncell > threads * kCellsPerThread || ncell > 0xffffu) continue;
+        const uint32_t maxb = (uint32_t)std::ceil(1.2 * occ * ncell) + 64;
+        const size_t lds = (((size_t)ncell * 4 + 15) & ~(size_t)15) + (size_t)maxb * vt;
+        if (lds > kTileLds || maxb > 0xffffu) continue;
+        const uint32_t per_cu = std::max<uint32_t>(
+            1, std::min<uint32_t>((uint32_t)(cu_lds / (lds + static_lds)), cu_waves / (threads / 64)));
+        const double rounds = std::ceil((double)ntiles / ((double)per_cu * cus));
+        const double work = (double)ntiles * threads * (1.0 + 0.02 * (double)ncell / trials);
+        if (!found || rounds < best_rounds || (rounds == best_rounds && work < best_work)) {
+          found = true;
+          best_rounds = rounds;
+          best_work = work;
+          for (int a = 0; a < 3; ++a) {
+            out->T[a] = T[a];
+            out->nt[a] = a < D ? (g.half_shape[a] + T[a] - 1) / T[a] : 1;
+            out->E[a] = a < D ? 2 * T[a] + 1 : 1;
+          }
+          out->trials = trials;
+          out->ncell = ncell;
+          out->maxb = maxb;
+          out->ntiles = ntiles;
+          out->xcd_chunk = (ntiles + 7) / 8;
+          out->lds_bytes = (uint32_t)lds;
+        }
+      }
+  return found;
+}
 
 int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
   if (!cfg || !out) return HMC_ERR_INVALID;
@@ -850,6 +1356,7 @@ int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
     }
     if (inside % 2 != 0) inside -= 1;
     g.shape[a] = (int32_t)inside;
+    g.half_shape[a] = (uint32_t)(inside / 2);
     g.width[a] = L / (double)inside;
     // a pair closer than sigma lies in neighbouring spaces unless the width is
     // within rounding of sigma; then the scan reaches two spaces out
@@ -868,6 +1375,7 @@ int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
   for (int a = D; a < 3; ++a) {
     g.reach[a] = 0;
     g.shape[a] = 1;
+    g.half_shape[a] = 1;
     g.stride[a] = 0;
     g.L[a] = 1.0;
     g.Lf[a] = 1.0f;
@@ -875,50 +1383,75 @@ int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
   }
   g.n_spaces = (uint32_t)spaces;
   g.per_colour = g.n_spaces >> D;
+  c->fast = true;
+  for (int a = 0; a < D; ++a) c->fast = c->fast && g.reach[a] == 1 && g.shape[a] >= 3;
   g.sigma2 = sigma * sigma;
   g.edge2 = g.sigma2 * (1.0 - 0x1p-30);
   g.vc_width = cfg->spatial_cell_width;
   g.vc_k = std::ceil(sigma / cfg->spatial_cell_width);
   // Mixed mode band.  Per axis |d32 - d64| <= E with
   // E = 2^-24 (3.5 Lmax + 2 sigma) + 2^-53 (1.5 Lmax + 2 sigma), from the
-  // roundings of t, x, L, x + T and t - x; the band on d^2 near
-  // sigma^2 is 2 E sqrt(D) d + D E^2 + 4 (2^-24 + 2^-53) d^2 at d = 1.01 sigma.
+  // roundings of t, x, L, x + T and t - x; the band on d^2 near sigma^2 is
+  // 2 E sqrt(D) d + D E^2 + 4 (2^-24 + 2^-53) d^2 at d = 1.01 sigma.
   double lmax = 0.0;
   for (int a = 0; a < D; ++a) lmax = std::max(lmax, g.L[a]);
   double E = (0x1p-24 * (3.5 * lmax + 2.0 * sigma) + 0x1p-53 * (1.5 * lmax + 2.0 * sigma)) * 1.01;
   double dref = 1.01 * sigma;
-  double B = (2.0 * E * std::sqrt((double)D) * dref + D * E * E + 4.0 * (0x1p-24 + 0x1p-53) * dref * dref) * 1.1;
-  if (cfg->precision == HMC_MIXED && !(B < 0.01 * g.sigma2)) {
+  double Bd = (2.0 * E * std::sqrt((double)D) * dref + D * E * E + 4.0 * (0x1p-24 + 0x1p-53) * dref * dref) * 1.1;
+  if (cfg->precision == HMC_MIXED && !(Bd < 0.01 * g.sigma2)) {
     delete c;
     return HMC_ERR_UNSUPPORTED;
   }
   // the float thresholds are rounded outward
-  g.band_lo = (float)(g.edge2 - B);
-  if ((double)g.band_lo > g.edge2 - B) g.band_lo = std::nextafter(g.band_lo, 0.0f);
-  g.band_hi = (float)(g.sigma2 + B);
-  if ((double)g.band_hi < g.sigma2 + B) g.band_hi = std::nextafter(g.band_hi, 1e30f);
+  g.band_lo = (float)(g.edge2 - Bd);
+  if ((double)g.band_lo > g.edge2 - Bd) g.band_lo = std::nextafter(g.band_lo, 0.0f);
+  g.band_hi = (float)(g.sigma2 + Bd);
+  if ((double)g.band_hi < g.sigma2 + Bd) g.band_hi = std::nextafter(g.band_hi, 1e30f);
 
   if (hipSetDevice(c->device) != hipSuccess) {
     delete c;
     return HMC_ERR_DEVICE;
   }
+  if (c->fast && !(cfg->flags & HMC_FLAG_GLOBAL_PASS)) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0) {
+      delete c;
+      return HMC_ERR_DEVICE;
+    }
+    const size_t vt = cfg->precision == HMC_MIXED ? sizeof(float4) : sizeof(double4);
+    c->tiled = choose_tiles(g, D, (double)cfg->n_bodies / (double)spaces, vt, cus, &c->tc);
+    if (c->tiled) {
+      const void* fn = nullptr;
+      if (D == 2)
+        fn = cfg->precision == HMC_MIXED ? (const void*)k_pass_tile<2, HMC_MIXED> : (const void*)k_pass_tile<2, HMC_FP64>;
+      else
+        fn = cfg->precision == HMC_MIXED ? (const void*)k_pass_tile<3, HMC_MIXED> : (const void*)k_pass_tile<3, HMC_FP64>;
+      if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->tc.lds_bytes) != hipSuccess) {
+        delete c;
+        return HMC_ERR_DEVICE;
+      }
+    }
+  }
   const uint32_t N = g.n_bodies;
   c->n_scan_blocks = grid_for(g.n_spaces, kScanBlock);
   bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
-  for (Slots* sl : {&c->a, &c->b}) {
-    ok = ok && dalloc(&sl->body, N) == hipSuccess && dalloc(&sl->bpos, N) == hipSuccess &&
-         dalloc(&sl->spos, N) == hipSuccess && dalloc(&sl->spos32, N) == hipSuccess;
-  }
+  ok = ok && dalloc(&c->bodies.bpos, N) == hipSuccess && dalloc(&c->bodies.spos, N) == hipSuccess &&
+       dalloc(&c->cells.body, N) == hipSuccess;
+  if (cfg->precision == HMC_MIXED)
+    ok = ok && dalloc(&c->cells.pos32, N) == hipSuccess;
+  else
+    ok = ok && dalloc(&c->cells.pos64, N) == hipSuccess;
   ok = ok && dalloc(&c->w.cnt, g.n_spaces) == hipSuccess && dalloc(&c->w.start, g.n_spaces + 1) == hipSuccess &&
        dalloc(&c->w.elem_space, N) == hipSuccess && dalloc(&c->w.elem_rank, N) == hipSuccess &&
-       dalloc(&c->w.tmp_body, N) == hipSuccess && dalloc(&c->w.tmp_src, N) == hipSuccess &&
-       dalloc(&c->w.colour_list, g.n_spaces) == hipSuccess &&
-       dalloc(&c->w.block_sums, c->n_scan_blocks) == hipSuccess && dalloc(&c->st, 1) == hipSuccess;
+       dalloc(&c->w.tmp_body, N) == hipSuccess && dalloc(&c->w.block_sums, c->n_scan_blocks) == hipSuccess &&
+       dalloc(&c->w.block_ne, c->n_scan_blocks) == hipSuccess && dalloc(&c->st, 1) == hipSuccess &&
+       dalloc(&c->w.acc_slots, kAccSlots) == hipSuccess;
   if (!ok) {
     hmc_destroy(c);
     return HMC_ERR_DEVICE;
   }
-  if (hipMemset(c->st, 0, sizeof(ApplyState)) != hipSuccess) {
+  if (hipMemset(c->st, 0, sizeof(ApplyState)) != hipSuccess ||
+      hipMemset(c->w.acc_slots, 0, kAccSlots * sizeof(unsigned long long)) != hipSuccess) {
     hmc_destroy(c);
     return HMC_ERR_DEVICE;
   }
@@ -944,40 +1477,39 @@ int hmc_set_state(hmc_ctx* c, const double* body_pos, const double* site_pos) {
     }
   }
   HIPCHK(c, hipSetDevice(c->device));
-  HIPCHK(c, hipMemcpyAsync(c->a.bpos, hb.data(), N * sizeof(double4), hipMemcpyHostToDevice, c->stream));
-  if (site_pos)
-    HIPCHK(c, hipMemcpyAsync(c->a.spos, hs.data(), N * sizeof(double4), hipMemcpyHostToDevice, c->stream));
   if (N) {
+    HIPCHK(c, hipMemcpyAsync(c->bodies.bpos, hb.data(), N * sizeof(double4), hipMemcpyHostToDevice, c->stream));
+    if (site_pos)
+      HIPCHK(c, hipMemcpyAsync(c->bodies.spos, hs.data(), N * sizeof(double4), hipMemcpyHostToDevice, c->stream));
     if (D == 2)
-      k_init_sites<2, 0><<<grid_for(N, 256), 256, 0, c->stream>>>(c->g, c->a, site_pos ? 1 : 0);
+      k_init_sites<2><<<grid_for(N, 256), 256, 0, c->stream>>>(c->g, c->bodies, site_pos ? 1 : 0);
     else
-      k_init_sites<3, 0><<<grid_for(N, 256), 256, 0, c->stream>>>(c->g, c->a, site_pos ? 1 : 0);
+      k_init_sites<3><<<grid_for(N, 256), 256, 0, c->stream>>>(c->g, c->bodies, site_pos ? 1 : 0);
+    HIPCHK(c, hipGetLastError());
   }
-  HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->have_state = true;
   return HMC_OK;
 }
 
 int hmc_get_state(hmc_ctx* c, double* body_pos, double* site_pos) {
-  if (!c || !c->have_state) return c ? fail(c, HMC_ERR_STATE, "no state") : HMC_ERR_INVALID;
+  if (!c) return HMC_ERR_INVALID;
+  if (!c->have_state) return fail(c, HMC_ERR_STATE, "no state");
   const uint32_t N = c->g.n_bodies;
   const int D = c->g.dim;
-  std::vector<uint32_t> body(N);
   std::vector<double4> hb(N), hs(N);
   HIPCHK(c, hipSetDevice(c->device));
-  HIPCHK(c, hipMemcpyAsync(body.data(), c->a.body, N * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(hb.data(), c->a.bpos, N * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(hs.data(), c->a.spos, N * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+  if (N) {
+    HIPCHK(c, hipMemcpyAsync(hb.data(), c->bodies.bpos, N * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(hs.data(), c->bodies.spos, N * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (uint32_t i = 0; i < N; ++i) {
-    uint32_t b = body[i];
-    if (b >= N) return fail(c, HMC_ERR_STATE, "corrupt body index");
     double v[3] = {hb[i].x, hb[i].y, hb[i].z};
     double u[3] = {hs[i].x, hs[i].y, hs[i].z};
     for (int a = 0; a < D; ++a) {
-      if (body_pos) body_pos[(size_t)b * D + a] = v[a];
-      if (site_pos) site_pos[(size_t)b * D + a] = u[a];
+      if (body_pos) body_pos[(size_t)i * D + a] = v[a];
+      if (site_pos) site_pos[(size_t)i * D + a] = u[a];
     }
   }
   return HMC_OK;
@@ -1037,6 +1569,7 @@ int hmc_run_translate(hmc_ctx* c, uint64_t step0, uint32_t seed, double kT, doub
 int hmc_trace_enable(hmc_ctx* c, int on, uint64_t capacity) {
   if (!c) return HMC_ERR_INVALID;
   HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   if (c->w.trace) {
     HIPCHK(c, hipFree(c->w.trace));
     c->w.trace = nullptr;
@@ -1079,6 +1612,9 @@ int hmc_get_stats(hmc_ctx* c, hmc_stats* out) {
   out->band_retests = s.band_retests;
   out->exp_slow_path = s.exp_slow;
   out->guard_cycles = s.guard_cycles;
+  out->tile_overflows = s.tile_overflows;
+  out->tiled = c->tiled ? 1u : 0u;
+  for (int a = 0; a < 3; ++a) out->tile_shape[a] = c->tiled ? c->tc.T[a] : 0u;
   return HMC_OK;
 }
 
@@ -1109,7 +1645,7 @@ int hmc_debug_math(int device, int kind, const double* in, double* out, uint64_t
   uint64_t nin = kind == 4 ? 2 * n : n;
   double *din = nullptr, *dout = nullptr;
   if (dalloc(&din, nin) != hipSuccess || dalloc(&dout, n) != hipSuccess) {
-    hipFree(din);
+    (void)hipFree(din);
     return HMC_ERR_DEVICE;
   }
   int rc = HMC_OK;
@@ -1119,8 +1655,8 @@ int hmc_debug_math(int device, int kind, const double* in, double* out, uint64_t
     if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = HMC_ERR_DEVICE;
   }
   if (!rc && hipMemcpy(out, dout, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) rc = HMC_ERR_DEVICE;
-  hipFree(din);
-  hipFree(dout);
+  (void)hipFree(din);
+  (void)hipFree(dout);
   return rc;
 }
 
@@ -1134,7 +1670,7 @@ int hmc_debug_normals(int device, uint64_t step, uint32_t substep, uint32_t seed
   k_debug_normals<<<1, 64>>>(step, substep, seed, index, d, n);
   if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = HMC_ERR_DEVICE;
   if (!rc && hipMemcpy(out, d, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) rc = HMC_ERR_DEVICE;
-  hipFree(d);
+  (void)hipFree(d);
   return rc;
 }
 
@@ -1142,28 +1678,17 @@ const char* hmc_last_error(const hmc_ctx* c) { return c ? c->err.c_str() : "null
 
 void hmc_destroy(hmc_ctx* c) {
   if (!c) return;
-  hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
-  for (Slots* sl : {&c->a, &c->b}) {
-    hipFree(sl->body);
-    hipFree(sl->bpos);
-    hipFree(sl->spos);
-    hipFree(sl->spos32);
-  }
-  hipFree(c->w.cnt);
-  hipFree(c->w.start);
-  hipFree(c->w.elem_space);
-  hipFree(c->w.elem_rank);
-  hipFree(c->w.tmp_body);
-  hipFree(c->w.tmp_src);
-  hipFree(c->w.colour_list);
-  hipFree(c->w.block_sums);
-  hipFree(c->w.trace);
-  hipFree(c->st);
-  for (auto& pr : c->pass_events) c->ev_pool.push_back(pr.first), c->ev_pool.push_back(pr.second);
-  for (auto& pr : c->build_events) c->ev_pool.push_back(pr.first), c->ev_pool.push_back(pr.second);
-  for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
-  if (c->stream) hipStreamDestroy(c->stream);
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : {(void*)c->bodies.bpos, (void*)c->bodies.spos, (void*)c->cells.body, (void*)c->cells.pos32,
+                  (void*)c->cells.pos64, (void*)c->w.cnt, (void*)c->w.start, (void*)c->w.elem_space,
+                  (void*)c->w.elem_rank, (void*)c->w.tmp_body, (void*)c->w.block_sums, (void*)c->w.block_ne,
+                  (void*)c->w.acc_slots, (void*)c->w.trace, (void*)c->st})
+    if (p) (void)hipFree(p);
+  collect_events(c, c->pass_events, c->pass_ms, c->pass_n);
+  collect_events(c, c->build_events, c->build_ms, c->build_n);
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 

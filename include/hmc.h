@@ -67,7 +67,15 @@ typedef struct {
   double params[4];
   int32_t precision;             /* HMC_FP64 or HMC_MIXED */
   int32_t device;                /* HIP device ordinal */
+  uint32_t flags;                /* HMC_FLAG_* (0: defaults) */
 } hmc_config;
+
+/* hmc_config.flags */
+enum {
+  HMC_FLAG_GLOBAL_PASS = 1 /* run every trial from global memory instead of
+                              staging each workgroup's neighbourhood box in LDS
+                              (same decisions; for comparison and testing) */
+};
 
 /* One trial of a trace: replaces the per-space BodyTrial status of
  * parallel_sweep.rs:26-52 (status 0 Rejected, 1 Accepted). */
@@ -87,6 +95,10 @@ typedef struct {
   uint64_t band_retests;    /* MIXED: pairs re-tested in FP64 */
   uint64_t exp_slow_path;   /* exp comparisons decided by the correctly rounded exp */
   uint64_t guard_cycles;    /* cycles run by the single-workgroup overflow guard */
+  uint64_t tile_overflows;  /* LDS tiles whose box exceeded the staging capacity
+                               (their trials ran from global memory) */
+  uint32_t tiled;           /* 1 if the pass stages neighbourhoods in LDS */
+  uint32_t tile_shape[3];   /* colour spaces per workgroup along each axis */
 } hmc_stats;
 
 /* Allocate device state for one system.  Replaces building a ParallelSweep and
