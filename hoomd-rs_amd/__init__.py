@@ -21,6 +21,8 @@ HMC_ERR_COVER = -4
 HMC_ERR_STATE = -5
 
 HMC_HARD_SPHERE = 0
+HMC_LJ = 1
+HMC_LJ_SHIFTED = 2
 HMC_FP64 = 0
 HMC_MIXED = 1
 
@@ -53,7 +55,8 @@ class Count(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("applies", C.c_uint64), ("cycles", C.c_uint64), ("pair_tests", C.c_uint64),
                 ("band_retests", C.c_uint64), ("exp_slow_path", C.c_uint64), ("guard_cycles", C.c_uint64),
-                ("tile_overflows", C.c_uint64), ("tiled", C.c_uint32), ("tile_shape", C.c_uint32 * 3)]
+                ("tile_overflows", C.c_uint64), ("tiled", C.c_uint32), ("tile_shape", C.c_uint32 * 3),
+                ("ambiguous", C.c_uint64)]
 
 
 TRACE_DTYPE = np.dtype([("substep", "<u4"), ("color", "<u4"), ("space", "<u8"),
@@ -104,11 +107,16 @@ class GpuParallelSweep:
 
     def __init__(self, box_edges, positions, *, sites=None, diameter=1.0, boundary_range=None,
                  body_interaction_range=None, spatial_cell_width=None, precision="fp64", device=0,
-                 lds_tiles=True):
+                 lds_tiles=True, model="hard_sphere", params=None):
         """positions / sites: the microstate's body and site positions (already
         wrapped into the box, as Microstate::add_body stores them); sites=None
         derives them as wrap(body + 0).  lds_tiles=False runs every trial from
-        global memory (same decisions)."""
+        global memory (same decisions).
+
+        model: "hard_sphere" (params = (diameter,), default (diameter,)),
+        "lj" (params = (epsilon, sigma, r_cut)) or "lj_shifted"
+        (params = (epsilon, sigma, r_cut, r_shift)); the ranges default to the
+        model's interaction range (diameter or r_cut)."""
         pos = np.ascontiguousarray(positions, np.float64)
         self.dim = pos.shape[1]
         self.n = pos.shape[0]
@@ -117,13 +125,18 @@ class GpuParallelSweep:
         cfg.n_bodies = self.n
         for a in range(self.dim):
             cfg.box_edges[a] = float(box_edges[a])
-        cfg.boundary_range = diameter if boundary_range is None else boundary_range
-        cfg.body_interaction_range = diameter if body_interaction_range is None else body_interaction_range
+        model_id = {"hard_sphere": HMC_HARD_SPHERE, "lj": HMC_LJ, "lj_shifted": HMC_LJ_SHIFTED}[model]
+        if params is None:
+            params = (diameter,) if model == "hard_sphere" else (1.0, 1.0, 2.5, 2.5)
+        rng = params[0] if model == "hard_sphere" else params[2]
+        cfg.boundary_range = rng if boundary_range is None else boundary_range
+        cfg.body_interaction_range = rng if body_interaction_range is None else body_interaction_range
         # VecCell::with_search_radius(maximum interaction range), as the
         # reference's benchmarks build the microstate
-        cfg.spatial_cell_width = diameter if spatial_cell_width is None else spatial_cell_width
-        cfg.model = HMC_HARD_SPHERE
-        cfg.params[0] = diameter
+        cfg.spatial_cell_width = rng if spatial_cell_width is None else spatial_cell_width
+        cfg.model = model_id
+        for i, v in enumerate(params):
+            cfg.params[i] = float(v)
         cfg.precision = {"fp64": HMC_FP64, "mixed": HMC_MIXED}[precision]
         cfg.device = device
         cfg.flags = 0 if lds_tiles else FLAG_GLOBAL_PASS

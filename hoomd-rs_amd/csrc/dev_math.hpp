@@ -20,6 +20,9 @@
 #include <cstdint>
 
 #define HMC_HD __host__ __device__ __forceinline__
+// Rare slow paths are kept out of line: they would otherwise triple the size
+// of the sweep kernels, whose instruction fetch is shared by two CUs.
+#define HMC_HD_COLD __host__ __device__ __noinline__
 
 namespace hmc {
 
@@ -96,7 +99,7 @@ HMC_HD dd dd_div(dd x, dd y) {
 HMC_HD dd dd_ln2() { return {0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56}; }
 
 // Correctly rounded log(x) for finite x > 0.
-HMC_HD double hmc_log_cr(double x) {
+HMC_HD_COLD double hmc_log_cr(double x) {
   int e;
   double m = frexp(x, &e);
   if (m < 0.70710678118654752440) {
@@ -120,7 +123,7 @@ HMC_HD double hmc_log_cr(double x) {
 }
 
 // Correctly rounded exp(y) for -708 <= y <= 709.
-HMC_HD double hmc_exp_cr(double y) {
+HMC_HD_COLD double hmc_exp_cr(double y) {
   double k = round(y * 1.4426950408889634);
   dd r = dd_add(dd{y, 0.0}, dd_mul_d(dd_ln2(), -k));
   dd p = {1.0, 0.0};

@@ -33,7 +33,7 @@ struct Sfc64 {
 // Counter(step, substep, seed).index(index).make_rng()
 __device__ __forceinline__ Sfc64 counter_rng(uint64_t step, uint32_t substep, uint32_t seed, uint64_t index) {
   Sfc64 r{step, ((uint64_t)substep << 32) | (uint64_t)seed, index, 0};
-#pragma unroll
+#pragma unroll 1
   for (int i = 0; i < 12; ++i) r.next();
   return r;
 }

@@ -67,6 +67,12 @@ struct Geometry {
   double vc_width;  // VecCell cell width of the reference microstate
   double vc_k;      // VecCell stencil radius ceil(sigma / vc_width)
   float band_lo, band_hi;  // mixed mode: d2 < band_lo overlaps, d2 >= band_hi does not
+  int model;               // HMC_HARD_SPHERE, HMC_LJ, HMC_LJ_SHIFTED
+  double lj_eps4;          // 4 epsilon (lennard_jones.rs:110)
+  double lj_sigma;
+  double lj_rc2;           // r_cut^2 (isotropic.rs:151)
+  double lj_rc2_edge;      // r_cut^2 (1 - 2^-30): at or above it the spatial-structure rule is applied
+  double lj_shift;         // E(r_shift) for the shifted potential, else 0
 };
 
 // Device-resident per-apply state.
@@ -88,6 +94,7 @@ struct ApplyState {
   unsigned long long exp_slow;
   unsigned long long guard_cycles;
   unsigned long long tile_overflows;
+  unsigned long long ambiguous;  // decisions the error bound could not settle
   unsigned long long trace_n;
 };
 
@@ -126,13 +133,14 @@ __device__ __forceinline__ double wrap1(double x, double L) {
 }
 
 // Ghost image flags of a site (cuboid.rs:108-161): 2 bits per axis,
-// 1 = image at x + L, 2 = image at x - L; all zero when the site lies outside
-// [-L/2, L/2) on any axis (the reference then places no ghosts).
+// 1 = image at x + L, 2 = image at x - L; kNoImages alone when the site lies
+// outside [-L/2, L/2) on any axis (the reference then places no ghosts).
+constexpr uint32_t kNoImages = 0x100u;
 template <int D>
 __device__ __forceinline__ uint32_t ghost_flags(const Geometry& g, const double* x) {
   uint32_t f = 0;
   for (int a = 0; a < D; ++a) {
-    if (x[a] < -g.half[a] || x[a] >= g.half[a]) return 0;
+    if (x[a] < -g.half[a] || x[a] >= g.half[a]) return kNoImages;
     if (x[a] > g.ghost_hi[a])
       f |= 2u << (2 * a);
     else if (x[a] < g.ghost_lo[a])
@@ -393,7 +401,7 @@ __device__ __forceinline__ void ball_sample(Sfc64& rng, double r, const double* 
 // axis (hoomd-spatial/src/vec_cell.rs:356-358, 710-733).  Away from contact
 // the rule cannot change the answer, so it is evaluated only there.
 template <int D>
-__device__ __noinline__ bool overlap64_edge(const Geometry& g, const double* t, double4 sj) {
+__device__ __noinline__ double edge_r2(const Geometry& g, const double* t, double4 sj) {
   uint32_t fl = (uint32_t)__double_as_longlong(sj.w);
   double xj[3] = {sj.x, sj.y, sj.z};
   double d2 = 0.0;
@@ -414,10 +422,15 @@ __device__ __noinline__ bool overlap64_edge(const Geometry& g, const double* t, 
         any = true;
       }
     }
-    if (!any) return false;
+    if (!any) return __longlong_as_double(0x7ff0000000000000ll);  // no visited image
     d2 = d2 + best * best;
   }
-  return d2 < g.sigma2;
+  return d2;
+}
+
+template <int D>
+__device__ __forceinline__ bool overlap64_edge(const Geometry& g, const double* t, double4 sj) {
+  return edge_r2<D>(g, t, sj) < g.sigma2;
 }
 
 template <int D>
@@ -578,9 +591,10 @@ __device__ __forceinline__ void trial(const Geometry& g, const ApplyState* st, c
 
   bool accept = false;
   if (space_of<D>(g, st->origin, x) == s) {
-    // trial site: wrap(body.transform(site)) (pairwise_cutoff.rs:436)
+    // trial site: wrap(body.transform(site)) = wrap(x + 0) (pairwise_cutoff.rs:436);
+    // x + 0.0 differs from x only for -0.0, which wraps to the same value
     double t[3] = {0.0, 0.0, 0.0};
-    for (int a = 0; a < D; ++a) t[a] = wrap1(x[a] + 0.0, g.L[a]);
+    for (int a = 0; a < D; ++a) t[a] = wrap1(x[a], g.L[a]);
     float t32[3] = {0.0f, 0.0f, 0.0f};
     for (int a = 0; a < D; ++a) t32[a] = (float)t[a];
     bool hit = false;
@@ -640,12 +654,11 @@ __device__ __forceinline__ void trial(const Geometry& g, const ApplyState* st, c
     accept = !hit;
     if (accept) {
       // update_body_properties (microstate.rs:985-1034): body = wrap(props),
-      // site = wrap(body + 0)
-      double nbp[3] = {0.0, 0.0, 0.0}, nsp[3] = {0.0, 0.0, 0.0};
-      for (int a = 0; a < D; ++a) nbp[a] = wrap1(x[a], g.L[a]);
-      for (int a = 0; a < D; ++a) nsp[a] = wrap1(nbp[a] + 0.0, g.L[a]);
+      // which is t; site = wrap(body + 0)
+      double nsp[3] = {0.0, 0.0, 0.0};
+      for (int a = 0; a < D; ++a) nsp[a] = wrap1(t[a] + 0.0, g.L[a]);
       const uint32_t fl = ghost_flags<D>(g, nsp);
-      B.bpos[body] = make_double4(nbp[0], nbp[1], nbp[2], 0.0);
+      B.bpos[body] = make_double4(t[0], t[1], t[2], 0.0);
       const double4 sp = pack_site(nsp, fl);
       B.spos[body] = sp;
       if (PREC == HMC_MIXED)
@@ -719,9 +732,23 @@ __global__ void __launch_bounds__(kBlock) k_pass(Geometry g, ApplyState* st, Bod
 // snapshot of the box taken at its start is exactly what every trial must
 // see.  (In the last, partial tile along an axis the box wraps onto spaces
 // of other tiles; those cells are staged but no active trial reads them.)
-// The box is staged into LDS with batched, coalesced loads (cell ranges,
-// then the neighbour copies of the sites); the trials then read their
-// neighbours from LDS.
+//
+// The pass is a chain of dependent memory phases, so the kernel is laid out
+// to keep that chain short: (A) the cell ranges of the box, the trial's own
+// range and the ziggurat tables are requested together; (B) the counts are
+// scanned into LDS offsets while the body indices of the trial's space are in
+// flight and the random proposal is drawn; (C) the neighbour copies of the
+// sites and the chosen body's position are requested together, one LDS entry
+// per lane; (D) the trials test their neighbours from LDS.
+//
+// Mixed precision stages each site relative to the box centre c, as the
+// image nearest c (the box spans less than L - 2 w on every axis, so that
+// image is the only one within reach of any trial in the box and the
+// difference of two staged positions is the minimum-image displacement).
+// The FP32 test is then a plain difference; pairs inside its error band are
+// re-tested with the FP64 sites and the reference's image rules.  A site the
+// reference gives no images (outside [-L/2, L/2) after rounding) is staged
+// as NaN, which always lands in the band.
 struct TileCfg {
   uint32_t T[3];     // colour spaces per tile along each axis (1 for unused axes)
   uint32_t nt[3];    // tiles along each axis
@@ -732,158 +759,75 @@ struct TileCfg {
   uint32_t ntiles;
   uint32_t xcd_chunk;
   uint32_t lds_bytes;
+  float band_lo, band_hi;  // FP32 band of the box-relative test
+  uint32_t step[3];        // the workgroup size written in the box's mixed radix (E0, E1, E2)
 };
 
 constexpr int kTileThreads = 256;        // upper bound of the workgroup size
 constexpr int kCellsPerThread = 16;      // staged cells per thread (ncell <= 16 x threads)
-constexpr int kStageRanks = 3;           // per-cell bodies copied in unrolled batches
+constexpr int kStageBatch = 8;           // staged sites per lane per batch of loads
+constexpr int kStagePerLane = 16;        // staged sites per lane at most (maxb <= 16 x threads)
 constexpr size_t kTileLds = 72 * 1024;   // dynamic LDS per workgroup
 
-// Global slot of the LDS copy `idx`, which belongs to staged cell k (box
-// origin mlo).  Only the rare FP64 band re-test needs it.
-template <int D>
-__device__ __noinline__ uint32_t staged_slot(const Geometry& g, const Work& w, const TileCfg& tc,
-                                             const uint32_t* cinfo, const int32_t* mlo, uint32_t k,
-                                             uint32_t idx) {
-  uint32_t ns = 0, r = k;
-  for (int a = D - 1; a >= 0; --a) {
-    const int32_t sh = g.shape[a];
-    int32_t mm = mlo[a] + (int32_t)(r % tc.E[a]);
-    r /= tc.E[a];
-    mm = mm < 0 ? mm + sh : (mm >= sh ? mm - sh : mm);
-    ns += (uint32_t)mm * g.stride[a];
+// Inclusive scan over the 64 lanes of a wavefront.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = (int)__lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
   }
-  return w.start[ns] + (idx - (cinfo[k] >> 16));
+  return v;
 }
 
-// One trial of the tile: space s with slots [s0, s0 + n), centre cell cc of
-// the staged box (parallel_sweep.rs:234-277, as trial() above).
+// Image of an FP32 position nearest the box centre c, relative to c.
+template <int D>
+__device__ __forceinline__ void box_relative(const Geometry& g, const float* c32, const float* x, float* r) {
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    float u = x[a] - c32[a];
+    const float h = 0.5f * g.Lf[a];
+    if (u >= h)
+      u -= g.Lf[a];
+    else if (u < -h)
+      u += g.Lf[a];
+    r[a] = u;
+  }
+}
+
+// The trial of space s from global memory, for a tile whose box holds more
+// bodies than its LDS copy (rare; kept out of line so that the tile kernel's
+// code stays small).
 template <int D, int PREC>
-__device__ __forceinline__ void trial_lds(const Geometry& g, const ApplyState* st, const Bodies& B,
-                                          const Cells& C, const Work& w, const TileCfg& tc,
-                                          const uint32_t* cinfo, const int32_t* mlo,
-                                          const typename CellCopy<PREC>::type* lb, const double* zx,
-                                          const double* zf, uint32_t s, const uint32_t* cc, uint32_t s0,
-                                          uint32_t n, uint32_t body0, double4 bp0, uint32_t colour,
-                                          uint32_t substep, TrialOut& o) {
-  using VT = typename CellCopy<PREC>::type;
-  constexpr int NR = D == 3 ? 9 : 3;  // rows of three cells along the last axis
-  uint32_t rlo[NR], rhi[NR], rcell[NR];
-#pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    const int i0 = D == 3 ? r / 3 : r;
-    const int i1 = D == 3 ? r % 3 : 0;
-    uint32_t kb;
-    if (D == 3)
-      kb = ((cc[0] + i0 - 1) * tc.E[1] + (cc[1] + i1 - 1)) * tc.E[2] + cc[2] - 1;
-    else
-      kb = (cc[0] + i0 - 1) * tc.E[1] + cc[1] - 1;
-    const uint32_t a = cinfo[kb], z = cinfo[kb + 2];
-    rcell[r] = kb;
-    rlo[r] = a >> 16;
-    rhi[r] = (z >> 16) + (z & 0xffffu);
-  }
-  const uint32_t kcen = D == 3 ? (cc[0] * tc.E[1] + cc[1]) * tc.E[2] + cc[2] : cc[0] * tc.E[1] + cc[1];
-
-  Sfc64 rng = counter_rng(st->step, substep, st->seed, s);
-  const uint32_t pick = choose_index(rng, n);
-  double v[3] = {0.0, 0.0, 0.0};
-  unsigned exp_slow = 0;
-  ball_sample<D>(rng, st->max_move, zx, zf, v, &exp_slow);
-  o.exp_slow = exp_slow;
-  const uint32_t slot = s0 + pick;
-  const uint32_t self = (cinfo[kcen] >> 16) + pick;
-  uint32_t body = body0;
-  double4 bp = bp0;
-  if (pick != 0) {
-    body = C.body[slot];
-    bp = B.bpos[body];
-  }
-  double x[3] = {bp.x, bp.y, bp.z};
-  for (int a = 0; a < D; ++a) x[a] += v[a];
-  for (int a = 0; a < D; ++a) x[a] = wrap1(x[a], g.L[a]);  // generate_trial_moves: wrap(propose(..))
-
-  bool accept = false;
-  if (space_of<D>(g, st->origin, x) == s) {
-    double t[3] = {0.0, 0.0, 0.0};
-    for (int a = 0; a < D; ++a) t[a] = wrap1(x[a] + 0.0, g.L[a]);  // pairwise_cutoff.rs:436
-    float t32[3] = {0.0f, 0.0f, 0.0f};
-    for (int a = 0; a < D; ++a) t32[a] = (float)t[a];
-    bool hit = false;
-    unsigned retests = 0;
-    uint32_t rounds = 0;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) rounds = max(rounds, rhi[r] - rlo[r]);
-    for (uint32_t k = 0; k < rounds && !hit; ++k) {
-#pragma unroll
-      for (int r = 0; r < NR; ++r) {
-        const uint32_t idx = rlo[r] + k;
-        const bool valid = idx < rhi[r] && idx != self;
-        const VT vq = lb[valid ? idx : self];
-        if (PREC == HMC_MIXED) {
-          const int res = overlap32<D>(g, t32, *(const float4*)&vq);
-          hit |= valid && res == 1;
-          if (valid && res == 2) {
-            // global slot of the LDS entry: one of the row's three cells
-            const uint32_t c1 = cinfo[rcell[r] + 1] >> 16, c2 = cinfo[rcell[r] + 2] >> 16;
-            const uint32_t cell = idx < c1 ? rcell[r] : (idx < c2 ? rcell[r] + 1 : rcell[r] + 2);
-            const uint32_t q = staged_slot<D>(g, w, tc, cinfo, mlo, cell, idx);
-            ++retests;
-            hit |= retest64<D>(g, B, C, q, t[0], t[1], t[2]);
-          }
-        } else {
-          hit |= valid && overlap64<D>(g, t, *(const double4*)&vq);
-        }
-      }
-    }
-    o.retests = retests;
-    accept = !hit;  // hard spheres: dH is 0 or +inf (parallel_sweep.rs:267-273)
-    if (accept) {
-      // update_body_properties (microstate.rs:985-1034)
-      double nbp[3] = {0.0, 0.0, 0.0}, nsp[3] = {0.0, 0.0, 0.0};
-      for (int a = 0; a < D; ++a) nbp[a] = wrap1(x[a], g.L[a]);
-      for (int a = 0; a < D; ++a) nsp[a] = wrap1(nbp[a] + 0.0, g.L[a]);
-      const uint32_t fl = ghost_flags<D>(g, nsp);
-      B.bpos[body] = make_double4(nbp[0], nbp[1], nbp[2], 0.0);
-      const double4 sp = pack_site(nsp, fl);
-      B.spos[body] = sp;
-      if (PREC == HMC_MIXED)
-        C.pos32[slot] = pack_site32(nsp, fl);
-      else
-        C.pos64[slot] = sp;
-    }
-  }
-  o.accepted = accept ? 1u : 0u;
-  if (w.tracing) {
-    unsigned long long idx = atomicAdd((unsigned long long*)&st->trace_n, 1ull);
-    if (idx < w.trace_cap) {
-      hmc_trace_record r;
-      r.substep = substep;
-      r.color = colour;
-      r.space = s;
-      r.body = (int64_t)body;
-      r.status = accept ? 1 : 0;
-      r.pad = 0;
-      w.trace[idx] = r;
-    }
-  }
+__device__ __noinline__ TrialOut trial_from_global(const Geometry& g, const ApplyState* st, const Bodies& B,
+                                                   const Cells& C, const Work& w, const double* zx, const double* zf,
+                                                   uint32_t s, uint32_t s0, uint32_t n, uint32_t colour,
+                                                   uint32_t substep) {
+  int32_t m[3] = {0, 0, 0};
+  for (int a = 0; a < D; ++a) m[a] = (int32_t)((s / g.stride[a]) % (uint32_t)g.shape[a]);
+  TrialOut o{0, 0, 0};
+  trial<D, PREC, false>(g, st, B, C, w, zx, zf, s, m, s0, n, colour, substep, o);
+  return o;
 }
 
 template <int D, int PREC>
 __global__ void __launch_bounds__(kTileThreads) k_pass_tile(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
                                                             uint32_t colour, uint32_t cycle, TileCfg tc) {
   using VT = typename CellCopy<PREC>::type;
+  constexpr uint32_t kWords = sizeof(VT) / 4;  // 32-bit words per LDS entry
   if (cycle >= st->cycles) return;  // uniform over the grid
   const uint32_t b = blockIdx.x;
   const uint32_t tile = (b & 7u) * tc.xcd_chunk + (b >> 3);  // XCD-aware tile order
   if (tile >= tc.ntiles) return;                              // uniform over the block
   extern __shared__ __align__(16) unsigned char smem[];
-  uint32_t* cinfo = (uint32_t*)smem;  // per staged cell: LDS offset << 16 | count
+  uint32_t* cinfo = (uint32_t*)smem;  // per staged cell: count, then LDS offset << 16 | count
   VT* lb = (VT*)(smem + ((tc.ncell * 4u + 15u) & ~15u));
+  uint32_t* lw = (uint32_t*)lb;  // first word of each entry: source slot while staging
   __shared__ double zx[257], zf[257];
-  __shared__ uint32_t scan_tmp[kTileThreads];
+  __shared__ uint32_t wave_tot[kTileThreads / 64];
   __shared__ unsigned blk_acc;
   const uint32_t tid = threadIdx.x, NT = blockDim.x;
+  const uint32_t substep = st->substep0 + cycle;
 
   uint32_t tcd[3] = {0, 0, 0};
   {
@@ -893,7 +837,9 @@ __global__ void __launch_bounds__(kTileThreads) k_pass_tile(Geometry g, ApplySta
       r /= tc.nt[a];
     }
   }
-  // this thread's trial space
+  int32_t mlo[3] = {0, 0, 0};  // box origin: 2 T tcd + offset - 1 per axis
+  for (int a = 0; a < D; ++a) mlo[a] = (int32_t)(2 * tcd[a] * tc.T[a] + ((colour >> (D - 1 - a)) & 1u)) - 1;
+  // this thread's trial space and its centre cell in the box
   bool active = tid < tc.trials;
   uint32_t cc[3] = {0, 0, 0};
   uint32_t s = 0;
@@ -904,108 +850,271 @@ __global__ void __launch_bounds__(kTileThreads) k_pass_tile(Geometry g, ApplySta
       r /= tc.T[a];
       const uint32_t h = tcd[a] * tc.T[a] + l;
       if (h >= g.half_shape[a]) active = false;
-      const uint32_t off = (colour >> (D - 1 - a)) & 1u;
-      s += (2 * h + off) * g.stride[a];
-      cc[a] = 2 * l + 1;  // centre cell in box coordinates
+      s += (2 * h + ((colour >> (D - 1 - a)) & 1u)) * g.stride[a];
+      cc[a] = 2 * l + 1;
     }
   }
-  uint32_t s0 = 0, n = 0, body0 = 0;
-  double4 bp0 = make_double4(0.0, 0.0, 0.0, 0.0);
+
+  // (A) cell ranges of the box, the own range, the ziggurat tables
+  uint32_t s0 = 0, s1 = 0;
   if (active) {
     s0 = w.start[s];
-    n = w.start[s + 1] - s0;
-    if (n) {
-      body0 = C.body[s0];
-      bp0 = B.bpos[body0];
+    s1 = w.start[s + 1];
+  }
+  // cells tid + i NT: box digits e, advanced by the workgroup size with carries
+  uint32_t src[kCellsPerThread], a1[kCellsPerThread];
+  uint32_t e[3] = {0, 0, 0};
+  {
+    uint32_t r = tid;
+    for (int a = D - 1; a > 0; --a) {
+      e[a] = r % tc.E[a];
+      r /= tc.E[a];
+    }
+    e[0] = r;  // the leading digit is not reduced: e[0] >= E[0] past the box
+  }
+#pragma unroll
+  for (int i = 0; i < kCellsPerThread; ++i) {
+    uint32_t ns = 0;
+    for (int a = 0; a < D; ++a) {
+      const int32_t sh = g.shape[a];
+      int32_t mm = mlo[a] + (int32_t)e[a];
+      mm = mm < 0 ? mm + sh : (mm >= sh ? mm - sh : mm);
+      ns += (uint32_t)mm * g.stride[a];
+    }
+    // cells past the box (only the first digit can run past it) load cell 0's range
+    if (e[0] >= tc.E[0]) ns = 0;
+    src[i] = w.start[ns];
+    a1[i] = w.start[ns + 1];
+    uint32_t carry = 0;
+    for (int a = D - 1; a >= 0; --a) {
+      e[a] += tc.step[a] + carry;
+      carry = 0;
+      if (a > 0 && e[a] >= tc.E[a]) {
+        e[a] -= tc.E[a];
+        carry = 1;
+      }
     }
   }
   if (D == 3) stage_zig(zx, zf);
   if (tid == 0) blk_acc = 0;
-
-  // 1. ranges of the staged cells (box origin: 2 T tcd + offset - 1 per axis)
-  int32_t mlo[3] = {0, 0, 0};
-  for (int a = 0; a < D; ++a) mlo[a] = (int32_t)(2 * tcd[a] * tc.T[a] + ((colour >> (D - 1 - a)) & 1u)) - 1;
-  uint32_t src[kCellsPerThread];  // first global slot of each of this thread's cells
-  {
-    uint32_t a1[kCellsPerThread];
 #pragma unroll
-    for (int i = 0; i < kCellsPerThread; ++i) {
-      const uint32_t k = tid + i * NT;
-      uint32_t ns = 0;
-      uint32_t r = k < tc.ncell ? k : 0;
-      for (int a = D - 1; a >= 0; --a) {
-        const int32_t sh = g.shape[a];
-        int32_t mm = mlo[a] + (int32_t)(r % tc.E[a]);
-        r /= tc.E[a];
-        mm = mm < 0 ? mm + sh : (mm >= sh ? mm - sh : mm);
-        ns += (uint32_t)mm * g.stride[a];
-      }
-      src[i] = w.start[ns];
-      a1[i] = w.start[ns + 1];
-    }
-#pragma unroll
-    for (int i = 0; i < kCellsPerThread; ++i) {
-      const uint32_t k = tid + i * NT;
-      if (k < tc.ncell) cinfo[k] = a1[i] - src[i];
-    }
+  for (int i = 0; i < kCellsPerThread; ++i) {
+    const uint32_t k = tid + i * NT;
+    if (k < tc.ncell) cinfo[k] = a1[i] - src[i];
+  }
+  const uint32_t n = s1 - s0;
+  const bool go = active && n != 0;
+  // body indices of the own space (the chosen one is among them unless pick >= 3)
+  uint32_t ob[3] = {0, 0, 0};
+  if (go) {
+    ob[0] = C.body[s0];
+    if (n > 1) ob[1] = C.body[s0 + 1];
+    if (n > 2) ob[2] = C.body[s0 + 2];
   }
   __syncthreads();
-  // 2. LDS offsets: exclusive scan of the counts over contiguous chunks
+
+  // (B) LDS offsets: exclusive scan of the counts over contiguous chunks
   uint32_t total = 0;
   {
     const uint32_t per = (tc.ncell + NT - 1) / NT;
     const uint32_t k0 = min(tid * per, tc.ncell), k1 = min(k0 + per, tc.ncell);
     uint32_t sum = 0;
     for (uint32_t k = k0; k < k1; ++k) sum += cinfo[k];
-    uint32_t off = block_exclusive_scan(sum, scan_tmp, &total);
+    const uint32_t incl = wave_incl_scan(sum);
+    const uint32_t wid = tid >> 6;
+    if (__lane_id() == 63) wave_tot[wid] = incl;
+    __syncthreads();
+    uint32_t off = incl - sum;
+    for (uint32_t wv = 0; wv < (NT >> 6); ++wv) {
+      const uint32_t t = wave_tot[wv];
+      if (wv < wid) off += t;
+      total += t;
+    }
     for (uint32_t k = k0; k < k1; ++k) {
       const uint32_t c = cinfo[k];
       cinfo[k] = (min(off, 0xffffu) << 16) | c;
       off += c;
     }
   }
-  __syncthreads();
-  const bool overflow = total > tc.maxb;
+  const bool overflow = total > tc.maxb;  // uniform over the block
+  // the proposal (parallel_sweep.rs:240-251): choose, then Translate
+  Sfc64 rng = counter_rng(st->step, substep, st->seed, s);
+  uint32_t pick = 0;
+  double v[3] = {0.0, 0.0, 0.0};
+  unsigned exp_slow = 0;
+  if (go && !overflow) {
+    pick = choose_index(rng, n);
+    ball_sample<D>(rng, st->max_move, zx, zf, v, &exp_slow);
+  }
+  __syncthreads();  // packed offsets visible
+
+  // (C) staging: source slot of every LDS entry, then one load per entry
   if (!overflow) {
-    // 3. copy the neighbour copies of the sites, rank by rank in unrolled batches
-    uint32_t info[kCellsPerThread];
-    uint32_t most = 0;
 #pragma unroll
     for (int i = 0; i < kCellsPerThread; ++i) {
       const uint32_t k = tid + i * NT;
-      info[i] = k < tc.ncell ? cinfo[k] : 0u;
-      most = max(most, info[i] & 0xffffu);
+      if (k < tc.ncell) {
+        const uint32_t ci = cinfo[k];
+        const uint32_t o = ci >> 16, c = ci & 0xffffu;
+        for (uint32_t r = 0; r < c; ++r) lw[(o + r) * kWords] = src[i] + r;
+      }
     }
+  }
+  uint32_t body = ob[pick < 2 ? pick : 2];
+  if (go && !overflow && pick > 2) body = C.body[s0 + pick];
+  double4 bp = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (go && !overflow) bp = B.bpos[body];
+  __syncthreads();
+  double c64[3] = {0.0, 0.0, 0.0};
+  float c32[3] = {0.0f, 0.0f, 0.0f};
+  if (PREC == HMC_MIXED) {
+    for (int a = 0; a < D; ++a) {
+      double c = st->origin[a] + ((double)mlo[a] + 0.5 * (double)tc.E[a]) * g.width[a];
+      c -= g.L[a] * floor((c + g.half[a]) / g.L[a]);
+      c64[a] = c;
+      c32[a] = (float)c;
+    }
+  }
+  if (!overflow) {
+    for (uint32_t j0 = 0; j0 * NT < total; j0 += kStageBatch) {
+      VT tmp[kStageBatch];
+      uint32_t q[kStageBatch];
 #pragma unroll
-    for (int r = 0; r < kStageRanks; ++r) {
-      VT tmp[kCellsPerThread];
-#pragma unroll
-      for (int i = 0; i < kCellsPerThread; ++i) {
-        const bool ok = (uint32_t)r < (info[i] & 0xffffu);
-        tmp[i] = CellCopy<PREC>::load(C, ok ? src[i] + r : 0u);
+      for (int j = 0; j < kStageBatch; ++j) {
+        const uint32_t idx = tid + (j0 + j) * NT;
+        q[j] = idx < total ? lw[idx * kWords] : 0u;
       }
 #pragma unroll
-      for (int i = 0; i < kCellsPerThread; ++i)
-        if ((uint32_t)r < (info[i] & 0xffffu)) lb[(info[i] >> 16) + r] = tmp[i];
-    }
-    if (most > (uint32_t)kStageRanks) {
+      for (int j = 0; j < kStageBatch; ++j) tmp[j] = CellCopy<PREC>::load(C, q[j]);
 #pragma unroll
-      for (int i = 0; i < kCellsPerThread; ++i)
-        for (uint32_t r = kStageRanks; r < (info[i] & 0xffffu); ++r)
-          lb[(info[i] >> 16) + r] = CellCopy<PREC>::load(C, src[i] + r);
+      for (int j = 0; j < kStageBatch; ++j) {
+        const uint32_t idx = tid + (j0 + j) * NT;
+        if (idx < total) {
+          if (PREC == HMC_MIXED) {
+            const float4 p = *(const float4*)&tmp[j];
+            const float x[3] = {p.x, p.y, p.z};
+            float r[3] = {0.0f, 0.0f, 0.0f};
+            box_relative<D>(g, c32, x, r);
+            if ((uint32_t)__float_as_int(p.w) & kNoImages) r[0] = __int_as_float(0x7fc00000);
+            *(float4*)&lb[idx] = make_float4(r[0], r[1], r[2], __int_as_float((int)q[j]));
+          } else {
+            lb[idx] = tmp[j];
+          }
+        }
+      }
     }
   }
   __syncthreads();
-  // 4. the trials
-  TrialOut o{0, 0, 0};
-  if (active && n) {
-    if (!overflow)
-      trial_lds<D, PREC>(g, st, B, C, w, tc, cinfo, mlo, lb, zx, zf, s, cc, s0, n, body0, bp0, colour,
-                         st->substep0 + cycle, o);
-    else {
-      int32_t m[3] = {0, 0, 0};
-      for (int a = 0; a < D; ++a) m[a] = (int32_t)((s / g.stride[a]) % (uint32_t)g.shape[a]);
-      trial<D, PREC, false>(g, st, B, C, w, zx, zf, s, m, s0, n, colour, st->substep0 + cycle, o);
+
+  // (D) the trials
+  TrialOut o{0, 0, exp_slow};
+  if (go && overflow) {
+    o = trial_from_global<D, PREC>(g, st, B, C, w, zx, zf, s, s0, n, colour, substep);
+  } else if (go) {
+    constexpr int NR = D == 3 ? 9 : 3;  // rows of three cells along the last axis
+    uint32_t rlo[NR], rhi[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int i0 = D == 3 ? r / 3 : r;
+      const int i1 = D == 3 ? r % 3 : 0;
+      uint32_t kb;
+      if (D == 3)
+        kb = ((cc[0] + i0 - 1) * tc.E[1] + (cc[1] + i1 - 1)) * tc.E[2] + cc[2] - 1;
+      else
+        kb = (cc[0] + i0 - 1) * tc.E[1] + cc[1] - 1;
+      const uint32_t a = cinfo[kb], z = cinfo[kb + 2];
+      rlo[r] = a >> 16;
+      rhi[r] = (z >> 16) + (z & 0xffffu);
+    }
+    const uint32_t kcen = D == 3 ? (cc[0] * tc.E[1] + cc[1]) * tc.E[2] + cc[2] : cc[0] * tc.E[1] + cc[1];
+    const uint32_t self = (cinfo[kcen] >> 16) + pick;
+    const uint32_t slot = s0 + pick;
+    double x[3] = {bp.x, bp.y, bp.z};
+    for (int a = 0; a < D; ++a) x[a] += v[a];
+    for (int a = 0; a < D; ++a) x[a] = wrap1(x[a], g.L[a]);  // generate_trial_moves: wrap(propose(..))
+
+    bool accept = false;
+    if (space_of<D>(g, st->origin, x) == s) {
+      // the trial site wrap(x + 0) (pairwise_cutoff.rs:436); x + 0.0 differs
+      // from x only for -0.0, which wraps to the same value
+      double t[3] = {0.0, 0.0, 0.0};
+      for (int a = 0; a < D; ++a) t[a] = wrap1(x[a], g.L[a]);
+      bool hit = false;
+      unsigned retests = 0;
+      uint32_t rounds = 0;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) rounds = max(rounds, rhi[r] - rlo[r]);
+      if (PREC == HMC_MIXED) {
+        float t32[3] = {0.0f, 0.0f, 0.0f}, tr[3] = {0.0f, 0.0f, 0.0f};
+        for (int a = 0; a < D; ++a) t32[a] = (float)t[a];
+        box_relative<D>(g, c32, t32, tr);
+        const float lo = tc.band_lo, hi = tc.band_hi;
+        for (uint32_t k = 0; k < rounds && !hit; ++k) {
+          uint32_t amb = 0;  // rows whose pair lies inside the error band
+#pragma unroll
+          for (int r = 0; r < NR; ++r) {
+            const uint32_t idx = rlo[r] + k;
+            const bool valid = idx < rhi[r] && idx != self;
+            const float4 p = *(const float4*)&lb[valid ? idx : self];
+            const float dx = p.x - tr[0], dy = p.y - tr[1];
+            float d2 = dx * dx + dy * dy;
+            if (D == 3) {
+              const float dz = p.z - tr[2];
+              d2 = d2 + dz * dz;
+            }
+            hit |= valid && d2 < lo;
+            if (valid && !(d2 < lo) && !(d2 >= hi)) amb |= 1u << r;
+          }
+          // FP64 re-test of the band pairs (rare): one call site
+          while (amb && !hit) {
+            const int r = __ffs(amb) - 1;
+            amb &= amb - 1;
+            const float4 p = *(const float4*)&lb[rlo[r] + k];
+            ++retests;
+            hit |= retest64<D>(g, B, C, (uint32_t)__float_as_int(p.w), t[0], t[1], t[2]);
+          }
+        }
+      } else {
+        for (uint32_t k = 0; k < rounds && !hit; ++k) {
+#pragma unroll
+          for (int r = 0; r < NR; ++r) {
+            const uint32_t idx = rlo[r] + k;
+            const bool valid = idx < rhi[r] && idx != self;
+            const double4 p = *(const double4*)&lb[valid ? idx : self];
+            hit |= valid && overlap64<D>(g, t, p);
+          }
+        }
+      }
+      o.retests = retests;
+      accept = !hit;  // hard spheres: dH is 0 or +inf (parallel_sweep.rs:267-273)
+      if (accept) {
+        // update_body_properties (microstate.rs:985-1034): body = wrap(x),
+        // which is t; site = wrap(body + 0)
+        double nsp[3] = {0.0, 0.0, 0.0};
+        for (int a = 0; a < D; ++a) nsp[a] = wrap1(t[a] + 0.0, g.L[a]);
+        const uint32_t fl = ghost_flags<D>(g, nsp);
+        B.bpos[body] = make_double4(t[0], t[1], t[2], 0.0);
+        const double4 sp = pack_site(nsp, fl);
+        B.spos[body] = sp;
+        if (PREC == HMC_MIXED)
+          C.pos32[slot] = pack_site32(nsp, fl);
+        else
+          C.pos64[slot] = sp;
+      }
+    }
+    o.accepted = accept ? 1u : 0u;
+    if (w.tracing) {
+      unsigned long long ti = atomicAdd((unsigned long long*)&st->trace_n, 1ull);
+      if (ti < w.trace_cap) {
+        hmc_trace_record rec;
+        rec.substep = substep;
+        rec.color = colour;
+        rec.space = s;
+        rec.body = (int64_t)body;
+        rec.status = accept ? 1 : 0;
+        rec.pad = 0;
+        w.trace[ti] = rec;
+      }
     }
   }
   const unsigned long long mk = __ballot(o.accepted != 0);
@@ -1042,6 +1151,286 @@ __global__ void __launch_bounds__(kGuardThreads) k_guard(Geometry g, ApplyState*
         }
         wave_count(&w.acc_slots[0], o.accepted != 0);
         lane_add(&st->band_retests, o.retests);
+        lane_add(&st->exp_slow, o.exp_slow);
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) st->guard_cycles += 1;
+  }
+}
+
+// ------------------------------------------------------------ Lennard-Jones --
+// Isotropic<LennardJones<12, 6>> and Isotropic<Shifted<LennardJones>>
+// (isotropic.rs:147-156, lennard_jones.rs:107-111, shifted.rs:92-94):
+// r = sqrt(r^2), s = sigma / r, E = (4 eps) (s^12 - s^6) - E(r_shift), with
+// s^12 = s^4 s^8 and s^6 = s^2 s^4 as powi's square chain evaluates them
+// (lj_shift is 0 for the unshifted potential, which subtracts exactly).
+__device__ __forceinline__ double lj_energy(const Geometry& g, double r2) {
+  const double r = sqrt(r2);
+  const double sr = g.lj_sigma / r;
+  const double s2 = sr * sr, s4 = s2 * s2, s8 = s4 * s4;
+  const double s12 = s4 * s8, s6 = s2 * s4;
+  return g.lj_eps4 * (s12 - s6) - g.lj_shift;
+}
+
+// Squared distance from q to site sj over the images the reference's
+// neighbour query visits: the per-axis nearest image (only one image can lie
+// within r_cut <= L/2), and within 2^-30 of the cutoff the VecCell rule.
+template <int D>
+__device__ __forceinline__ double lj_r2(const Geometry& g, const double* q, double4 sj) {
+  const uint32_t fl = (uint32_t)__double_as_longlong(sj.w);
+  const double xj[3] = {sj.x, sj.y, sj.z};
+  double d2 = 0.0;
+  for (int a = 0; a < D; ++a) {
+    double d = q[a] - xj[a];
+    const uint32_t f = (fl >> (2 * a)) & 3u;
+    if (f) {
+      const double dg = q[a] - (xj[a] + (f == 1u ? g.L[a] : -g.L[a]));
+      if (fabs(dg) < fabs(d)) d = dg;
+    }
+    d2 = d2 + d * d;
+  }
+  if (d2 >= g.lj_rc2_edge && d2 < g.lj_rc2) d2 = edge_r2<D>(g, q, sj);
+  return d2;
+}
+
+struct WaveSums {
+  double ef, af, ei, ai;  // final / initial energy and sum of |terms|
+  uint32_t nf, ni;        // number of non-zero terms
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+// Metropolis decision of parallel_sweep.rs:267-273 for dH known to within
+// +-delta of the reference's value (same terms, other summation order):
+// accept iff dH <= 0 or u < exp(-dH / kT), u drawn only when dH > 0.
+// Returns 1 accept, 0 reject; *ambiguous is set when the interval
+// [dH - delta, dH + delta] admits both answers (then dH itself decides).
+__device__ __forceinline__ int lj_metropolis(Sfc64& rng, double dH, double delta, double kT, bool* ambiguous) {
+  const double hi = dH + delta, lo = dH - delta;
+  *ambiguous = false;
+  if (hi <= 0.0) return 1;
+  const double u = uniform01(rng);
+  // exp(q) for q within the interval, with the roundings of -dH / kT and of
+  // exp itself covered by a relative margin of 2^-45 (1 + |q|)
+  const double qa = -hi / kT;
+  const double ea = exp(qa);
+  if (u < ea * (1.0 - 0x1p-45 * (1.0 + fabs(qa)))) return 1;
+  if (lo > 0.0) {
+    const double qb = -lo / kT;
+    const double eb = exp(qb);
+    if (u >= eb * (1.0 + 0x1p-45 * (1.0 + fabs(qb)))) return 0;
+  }
+  *ambiguous = true;
+  if (dH <= 0.0) return 1;
+  return u < hmc_exp_cr(-dH / kT) ? 1 : 0;
+}
+
+struct LjOut {
+  unsigned accepted;
+  unsigned ambiguous;
+  unsigned exp_slow;
+};
+
+// One LJ trial (parallel_sweep.rs:234-277) by a whole wavefront: the random
+// proposal is drawn identically in every lane, the lanes share out the
+// candidate sites of the 3^D (or 5^D) neighbour spaces, compute the final
+// and initial pair energies of each in FP64 exactly as the reference does,
+// and reduce.  The reference sums in its VecCell order; any two summation
+// orders of n terms differ by at most 2 gamma_{n-1} sum|terms|, so the
+// decision is exact whenever that bound cannot change it (counted otherwise).
+template <int D>
+__device__ void trial_lj(const Geometry& g, const ApplyState* st, const Bodies& B, const Cells& C, const Work& w,
+                         const double* zx, const double* zf, uint32_t s, const int32_t* m, uint32_t s0, uint32_t n,
+                         uint32_t colour, uint32_t substep, LjOut& o) {
+  const uint32_t lane = __lane_id();
+  Sfc64 rng = counter_rng(st->step, substep, st->seed, s);
+  const uint32_t pick = choose_index(rng, n);
+  double v[3] = {0.0, 0.0, 0.0};
+  unsigned exp_slow = 0;
+  ball_sample<D>(rng, st->max_move, zx, zf, v, &exp_slow);
+  o.exp_slow = lane == 0 ? exp_slow : 0u;
+  const uint32_t slot = s0 + pick;
+  const uint32_t body = C.body[slot];
+  const double4 bp = B.bpos[body];
+  const double4 sp = B.spos[body];
+  double x[3] = {bp.x, bp.y, bp.z};
+  for (int a = 0; a < D; ++a) x[a] += v[a];
+  for (int a = 0; a < D; ++a) x[a] = wrap1(x[a], g.L[a]);  // generate_trial_moves: wrap(propose(..))
+  bool accept = false;
+  if (space_of<D>(g, st->origin, x) == s) {
+    double t[3] = {0.0, 0.0, 0.0};  // wrap(x + 0) == wrap(x)
+    for (int a = 0; a < D; ++a) t[a] = wrap1(x[a], g.L[a]);
+    const double xi[3] = {sp.x, sp.y, sp.z};
+    // neighbour spaces (de-duplicated per axis), as segments of slots
+    uint32_t nb[3][5];
+    uint32_t nn[3] = {1, 1, 1};
+    for (int a = 0; a < D; ++a) {
+      const int32_t sh = g.shape[a], r = g.reach[a];
+      uint32_t k = 0;
+      for (int off = -r; off <= r; ++off) {
+        const int32_t mm = ((m[a] + off) % sh + sh) % sh;
+        bool seen = false;
+        for (uint32_t j = 0; j < k; ++j) seen |= nb[a][j] == (uint32_t)mm * g.stride[a];
+        if (!seen) nb[a][k++] = (uint32_t)mm * g.stride[a];
+      }
+      nn[a] = k;
+    }
+    const uint32_t NS = nn[0] * nn[1] * nn[2];  // at most 125
+    uint32_t seg_lo[2] = {0, 0}, seg_pre[2] = {0, 0};
+    uint32_t base = 0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t j = lane + 64u * h;
+      uint32_t cnt = 0, lo = 0;
+      if (j < NS) {
+        const uint32_t i2 = j % nn[2], i1 = (j / nn[2]) % nn[1], i0 = j / (nn[2] * nn[1]);
+        const uint32_t ns = nb[0][i0] + (D > 1 ? nb[1][i1] : 0u) + (D > 2 ? nb[2][i2] : 0u);
+        lo = w.start[ns];
+        cnt = w.start[ns + 1] - lo;
+      }
+      const uint32_t incl = wave_incl_scan(cnt);
+      seg_lo[h] = lo;
+      seg_pre[h] = base + incl - cnt;
+      base += __shfl(incl, 63, 64);
+    }
+    const uint32_t total = base;
+    WaveSums ws{0.0, 0.0, 0.0, 0.0, 0u, 0u};
+    // every lane runs the same number of rounds: the shuffles below read
+    // segment data held by other lanes, which must all be active
+    for (uint32_t c0 = 0; c0 < total; c0 += 64) {
+      const uint32_t c = c0 + lane;
+      // segment of candidate c: the last j with pre[j] <= c, by a fixed-depth
+      // binary search over pre[0..NS) (pre[0] = 0, NS <= 128)
+      // (entry j lives in register j / 64 of lane j % 64; both registers are
+      // shuffled and the reader selects)
+      uint32_t jlo = 0;
+#pragma unroll
+      for (uint32_t stepw = 64; stepw >= 1; stepw >>= 1) {
+        const uint32_t mid = jlo + stepw;
+        const uint32_t pa = __shfl(seg_pre[0], (int)(mid & 63u), 64);
+        const uint32_t pb = __shfl(seg_pre[1], (int)(mid & 63u), 64);
+        if (mid < NS && (mid < 64 ? pa : pb) <= c) jlo = mid;
+      }
+      const uint32_t pa = __shfl(seg_pre[0], (int)(jlo & 63u), 64), pb = __shfl(seg_pre[1], (int)(jlo & 63u), 64);
+      const uint32_t la = __shfl(seg_lo[0], (int)(jlo & 63u), 64), lb2 = __shfl(seg_lo[1], (int)(jlo & 63u), 64);
+      const uint32_t plo = jlo < 64 ? pa : pb;
+      const uint32_t llo = jlo < 64 ? la : lb2;
+      const uint32_t q = llo + (c - plo);
+      // skip lanes past the end and the body's own site (filter body_tag != site_j.body_tag)
+      if (c >= total || q == slot) continue;
+      const double4 sj = C.pos64[q];
+      const double r2f = lj_r2<D>(g, t, sj);
+      if (r2f < g.lj_rc2) {
+        const double e = lj_energy(g, r2f);
+        ws.ef += e;
+        ws.af += fabs(e);
+        ws.nf += 1;
+      }
+      const double r2i = lj_r2<D>(g, xi, sj);
+      if (r2i < g.lj_rc2) {
+        const double e = lj_energy(g, r2i);
+        ws.ei += e;
+        ws.ai += fabs(e);
+        ws.ni += 1;
+      }
+    }
+    const double ef = wave_sum(ws.ef), af = wave_sum(ws.af);
+    const double ei = wave_sum(ws.ei), ai = wave_sum(ws.ai);
+    const uint32_t nf = wave_sum(ws.nf), ni = wave_sum(ws.ni);
+    // energy_final - energy_initial (pairwise_cutoff.rs:919-941)
+    const double dH = ef - ei;
+    const double nmax = (double)max(max(nf, ni), 1u);
+    const double gam = nmax * 0x1p-53 / (1.0 - nmax * 0x1p-53);
+    const double delta = (2.0 * gam * (af + ai) + 0x1p-52 * fabs(dH)) * (1.0 + 0x1p-20) + 0x1p-1070;
+    bool amb = false;
+    accept = lj_metropolis(rng, dH, delta, st->kT, &amb) != 0;
+    o.ambiguous = (amb && lane == 0) ? 1u : 0u;
+    if (accept && lane == 0) {
+      // update_body_properties (microstate.rs:985-1034)
+      double nsp[3] = {0.0, 0.0, 0.0};
+      for (int a = 0; a < D; ++a) nsp[a] = wrap1(t[a] + 0.0, g.L[a]);
+      const uint32_t fl = ghost_flags<D>(g, nsp);
+      B.bpos[body] = make_double4(t[0], t[1], t[2], 0.0);
+      const double4 nspk = pack_site(nsp, fl);
+      B.spos[body] = nspk;
+      C.pos64[slot] = nspk;
+    }
+  }
+  o.accepted = (accept && lane == 0) ? 1u : 0u;
+  if (w.tracing && lane == 0) {
+    unsigned long long ti = atomicAdd((unsigned long long*)&st->trace_n, 1ull);
+    if (ti < w.trace_cap) {
+      hmc_trace_record rec;
+      rec.substep = substep;
+      rec.color = colour;
+      rec.space = s;
+      rec.body = (int64_t)body;
+      rec.status = accept ? 1 : 0;
+      rec.pad = 0;
+      w.trace[ti] = rec;
+    }
+  }
+}
+
+// One colour of one cycle for LJ: one wavefront per space of the colour.
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_pass_lj(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
+                                                    uint32_t colour, uint32_t cycle, uint32_t xcd_chunk) {
+  if (cycle >= st->cycles) return;  // uniform over the grid
+  __shared__ double zx[257], zf[257];
+  __shared__ unsigned blk_acc;
+  if (D == 3) stage_zig(zx, zf);
+  if (threadIdx.x == 0) blk_acc = 0;
+  __syncthreads();
+  const uint32_t b = blockIdx.x;
+  const uint32_t lb = (b & 7u) * xcd_chunk + (b >> 3);
+  const uint32_t t = lb * (kBlock / 64) + (threadIdx.x >> 6);  // one space per wavefront
+  LjOut o{0, 0, 0};
+  if (t < g.per_colour) {
+    int32_t m[3] = {0, 0, 0};
+    const uint32_t s = colour_space<D>(g, colour, t, m);
+    const uint32_t s0 = w.start[s];
+    const uint32_t n = w.start[s + 1] - s0;
+    if (n) trial_lj<D>(g, st, B, C, w, zx, zf, s, m, s0, n, colour, st->substep0 + cycle, o);
+  }
+  if (o.accepted) atomicAdd(&blk_acc, 1u);
+  lane_add(&st->ambiguous, o.ambiguous);
+  lane_add(&st->exp_slow, o.exp_slow);
+  __syncthreads();
+  if (threadIdx.x == 0 && blk_acc) atomicAdd(&w.acc_slots[b & (kAccSlots - 1)], (unsigned long long)blk_acc);
+}
+
+// LJ cycles beyond the ones the host launched, in one workgroup.
+template <int D>
+__global__ void __launch_bounds__(kGuardThreads) k_guard_lj(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
+                                                            uint32_t first_cycle) {
+  const uint32_t cycles = st->cycles;
+  if (first_cycle >= cycles) return;
+  __shared__ double zx[257], zf[257];
+  if (D == 3) stage_zig(zx, zf);
+  __syncthreads();
+  const uint32_t waves = kGuardThreads / 64, wid = threadIdx.x >> 6;
+  for (uint32_t k = first_cycle; k < cycles; ++k) {
+    for (uint32_t c = 0; c < (1u << D); ++c) {
+      for (uint32_t t = wid; t < g.per_colour; t += waves) {
+        LjOut o{0, 0, 0};
+        int32_t m[3] = {0, 0, 0};
+        const uint32_t s = colour_space<D>(g, c, t, m);
+        const uint32_t s0 = w.start[s];
+        const uint32_t n = w.start[s + 1] - s0;
+        if (n) trial_lj<D>(g, st, B, C, w, zx, zf, s, m, s0, n, c, st->substep0 + k, o);
+        if (o.accepted) atomicAdd(&w.acc_slots[0], 1ull);
+        lane_add(&st->ambiguous, o.ambiguous);
         lane_add(&st->exp_slow, o.exp_slow);
       }
       __syncthreads();
@@ -1182,14 +1571,22 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
   const uint32_t chunk = (blocks + 7) / 8;
   const uint32_t grid = chunk * 8;
   const uint32_t tile_threads = ((c->tc.trials + 63) / 64) * 64;
-  for (uint32_t k = 0; k < c->cycles_guess; ++k) {
+  const bool lj = g.model != HMC_HARD_SPHERE;
+  // LJ needs many cycles per apply and its guard is slow: launch one spare
+  // cycle (its passes exit at once when not needed)
+  const uint32_t launched = c->cycles_guess + (lj ? 1u : 0u);
+  const uint32_t lj_blocks = grid_for(g.per_colour, kBlock / 64);
+  const uint32_t lj_chunk = (lj_blocks + 7) / 8;
+  for (uint32_t k = 0; k < launched; ++k) {
     for (int col = 0; col < (1 << D); ++col) {
       hipEvent_t p0 = nullptr, p1 = nullptr;
       if (c->profiling) {
         p0 = take_event(c);
         (void)hipEventRecord(p0, s);
       }
-      if (c->tiled)
+      if (lj)
+        k_pass_lj<D><<<lj_chunk * 8, kBlock, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k, lj_chunk);
+      else if (c->tiled)
         k_pass_tile<D, PREC><<<c->tc.xcd_chunk * 8, tile_threads, c->tc.lds_bytes, s>>>(
             g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k, c->tc);
       else if (c->fast)
@@ -1203,7 +1600,10 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
       }
     }
   }
-  k_guard<D, PREC><<<1, kGuardThreads, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, c->cycles_guess);
+  if (lj)
+    k_guard_lj<D><<<1, kGuardThreads, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, launched);
+  else
+    k_guard<D, PREC><<<1, kGuardThreads, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, launched);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(c, HMC_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
   c->applies += 1;
@@ -1253,6 +1653,24 @@ int read_state(hmc_ctx* c, ApplyState* out) {
 
 extern "C" {
 
+// FP32 thresholds around contact for a per-axis error bound E on the FP32
+// displacement: d2 < lo overlaps (and lies below the edge region), d2 >= hi
+// does not.  The bound on d^2 near sigma^2 is 2 E sqrt(D) d + D E^2 +
+// 4 (2^-24 + 2^-53) d^2 at d = 1.01 sigma (E must be far below sigma), with
+// 10% margin; the float thresholds are rounded outward.
+static bool contact_band(double E, double sigma, double sigma2, double edge2, int D, float* lo, float* hi) {
+  const double dref = 1.01 * sigma;
+  const double Bd = (2.0 * E * std::sqrt((double)D) * dref + D * E * E + 4.0 * (0x1p-24 + 0x1p-53) * dref * dref) * 1.1;
+  if (!(Bd < 0.01 * sigma2)) return false;
+  float l = (float)(edge2 - Bd);
+  if ((double)l > edge2 - Bd) l = std::nextafter(l, 0.0f);
+  float h = (float)(sigma2 + Bd);
+  if ((double)h < sigma2 + Bd) h = std::nextafter(h, 1e30f);
+  *lo = l;
+  *hi = h;
+  return true;
+}
+
 // Tile shape of the LDS-staged pass for a system with `occ` bodies per space
 // on a device with `cus` CUs.  Candidates: at most kTileThreads trials, at
 // most kCellsPerThread staged cells per thread, at most kTileLds bytes of
@@ -1262,9 +1680,15 @@ extern "C" {
 // the shape that needs the fewest rounds of resident workgroups wins; among
 // those, the least launched lanes and staging.
 static bool choose_tiles(const Geometry& g, int D, double occ, size_t vt, int cus, TileCfg* out) {
-  const uint32_t lim[3] = {std::min<uint32_t>(g.half_shape[0], 16), D > 1 ? std::min<uint32_t>(g.half_shape[1], 16) : 1,
-                           D > 2 ? std::min<uint32_t>(g.half_shape[2], 32) : 1};
-  const size_t static_lds = 2 * 257 * sizeof(double) + kTileThreads * sizeof(uint32_t) + 16;
+  // the box (2T + 1 spaces) must span at most shape - 2 spaces per axis, so
+  // that box-relative positions are minimum-image positions
+  uint32_t lim[3] = {1, 1, 1};
+  const uint32_t cap[3] = {16, 16, 32};
+  for (int a = 0; a < D; ++a) {
+    if (g.shape[a] < 5) return false;
+    lim[a] = std::min<uint32_t>(std::min<uint32_t>(g.half_shape[a], cap[a]), (uint32_t)(g.shape[a] - 3) / 2);
+  }
+  const size_t static_lds = 2 * 257 * sizeof(double) + 64;
   const size_t cu_lds = 160 * 1024;
   const uint32_t cu_waves = 8;  // two waves per SIMD at the kernel's register use
   double best_rounds = 0.0, best_work = 0.0;
@@ -1284,7 +1708,7 @@ static bool choose_tiles(const Geometry& g, int D, double occ, size_t vt, int cu
         if (ncell > threads * kCellsPerThread || ncell > 0xffffu) continue;
         const uint32_t maxb = (uint32_t)std::ceil(1.2 * occ * ncell) + 64;
         const size_t lds = (((size_t)ncell * 4 + 15) & ~(size_t)15) + (size_t)maxb * vt;
-        if (lds > kTileLds || maxb > 0xffffu) continue;
+        if (lds > kTileLds || maxb > 0xffffu || maxb > threads * kStagePerLane) continue;
         const uint32_t per_cu = std::max<uint32_t>(
             1, std::min<uint32_t>((uint32_t)(cu_lds / (lds + static_lds)), cu_waves / (threads / 64)));
         const double rounds = std::ceil((double)ntiles / ((double)per_cu * cus));
@@ -1304,6 +1728,13 @@ static bool choose_tiles(const Geometry& g, int D, double occ, size_t vt, int cu
           out->ntiles = ntiles;
           out->xcd_chunk = (ntiles + 7) / 8;
           out->lds_bytes = (uint32_t)lds;
+          uint32_t r = threads;
+          for (int a = D - 1; a > 0; --a) {
+            out->step[a] = r % out->E[a];
+            r /= out->E[a];
+          }
+          out->step[0] = r;  // the leading digit is not reduced
+          for (int a = D; a < 3; ++a) out->step[a] = 0;
         }
       }
   return found;
@@ -1313,14 +1744,21 @@ int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
   if (!cfg || !out) return HMC_ERR_INVALID;
   *out = nullptr;
   if (cfg->dim != 2 && cfg->dim != 3) return HMC_ERR_UNSUPPORTED;
-  if (cfg->model != HMC_HARD_SPHERE) return HMC_ERR_UNSUPPORTED;
+  const bool lj = cfg->model == HMC_LJ || cfg->model == HMC_LJ_SHIFTED;
+  if (cfg->model != HMC_HARD_SPHERE && !lj) return HMC_ERR_UNSUPPORTED;
   if (cfg->precision != HMC_FP64 && cfg->precision != HMC_MIXED) return HMC_ERR_INVALID;
+  if (lj && cfg->precision != HMC_FP64) return HMC_ERR_UNSUPPORTED;  // LJ: FP64 pair energies only
   if (cfg->n_bodies >= 0xffffffffull) return HMC_ERR_UNSUPPORTED;
   const int D = (int)cfg->dim;
-  const double sigma = cfg->params[0];
+  // interaction range: the diameter (hard_shape.rs:170-174) or r_cut (isotropic.rs)
+  const double sigma = lj ? cfg->params[2] : cfg->params[0];
   if (!(sigma > 0.0) || !(cfg->body_interaction_range > 0.0) || !(cfg->boundary_range >= 0.0) ||
       !(cfg->spatial_cell_width > 0.0))
     return HMC_ERR_INVALID;
+  if (lj && (!(cfg->params[0] >= 0.0) || !(cfg->params[1] > 0.0) || !std::isfinite(cfg->params[0]) ||
+             !std::isfinite(cfg->params[1])))
+    return HMC_ERR_INVALID;
+  if (cfg->model == HMC_LJ_SHIFTED && !(cfg->params[3] > 0.0)) return HMC_ERR_INVALID;
   double lmin = cfg->box_edges[0];
   for (int a = 0; a < D; ++a) {
     if (!(cfg->box_edges[a] > 0.0)) return HMC_ERR_INVALID;
@@ -1387,32 +1825,39 @@ int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
   for (int a = 0; a < D; ++a) c->fast = c->fast && g.reach[a] == 1 && g.shape[a] >= 3;
   g.sigma2 = sigma * sigma;
   g.edge2 = g.sigma2 * (1.0 - 0x1p-30);
+  g.model = cfg->model;
+  if (lj) {
+    // lennard_jones.rs:107-111, shifted.rs:92-94, isotropic.rs:147-156
+    const double eps = cfg->params[0], lsig = cfg->params[1], rc = cfg->params[2];
+    g.lj_eps4 = 4.0 * eps;
+    g.lj_sigma = lsig;
+    g.lj_rc2 = rc * rc;
+    g.lj_rc2_edge = g.lj_rc2 * (1.0 - 0x1p-30);
+    g.lj_shift = 0.0;
+    if (cfg->model == HMC_LJ_SHIFTED) {
+      const double sr = lsig / cfg->params[3];
+      const double s2 = sr * sr, s4 = s2 * s2, s8 = s4 * s4;
+      g.lj_shift = g.lj_eps4 * (s4 * s8 - s2 * s4);
+    }
+  }
   g.vc_width = cfg->spatial_cell_width;
   g.vc_k = std::ceil(sigma / cfg->spatial_cell_width);
-  // Mixed mode band.  Per axis |d32 - d64| <= E with
+  // Mixed mode band of the global pass.  Per axis |d32 - d64| <= E with
   // E = 2^-24 (3.5 Lmax + 2 sigma) + 2^-53 (1.5 Lmax + 2 sigma), from the
-  // roundings of t, x, L, x + T and t - x; the band on d^2 near sigma^2 is
-  // 2 E sqrt(D) d + D E^2 + 4 (2^-24 + 2^-53) d^2 at d = 1.01 sigma.
+  // roundings of t, x, L, x + T and t - x.
   double lmax = 0.0;
   for (int a = 0; a < D; ++a) lmax = std::max(lmax, g.L[a]);
-  double E = (0x1p-24 * (3.5 * lmax + 2.0 * sigma) + 0x1p-53 * (1.5 * lmax + 2.0 * sigma)) * 1.01;
-  double dref = 1.01 * sigma;
-  double Bd = (2.0 * E * std::sqrt((double)D) * dref + D * E * E + 4.0 * (0x1p-24 + 0x1p-53) * dref * dref) * 1.1;
-  if (cfg->precision == HMC_MIXED && !(Bd < 0.01 * g.sigma2)) {
+  const double E = (0x1p-24 * (3.5 * lmax + 2.0 * sigma) + 0x1p-53 * (1.5 * lmax + 2.0 * sigma)) * 1.01;
+  if (!contact_band(E, sigma, g.sigma2, g.edge2, D, &g.band_lo, &g.band_hi) && cfg->precision == HMC_MIXED) {
     delete c;
     return HMC_ERR_UNSUPPORTED;
   }
-  // the float thresholds are rounded outward
-  g.band_lo = (float)(g.edge2 - Bd);
-  if ((double)g.band_lo > g.edge2 - Bd) g.band_lo = std::nextafter(g.band_lo, 0.0f);
-  g.band_hi = (float)(g.sigma2 + Bd);
-  if ((double)g.band_hi < g.sigma2 + Bd) g.band_hi = std::nextafter(g.band_hi, 1e30f);
 
   if (hipSetDevice(c->device) != hipSuccess) {
     delete c;
     return HMC_ERR_DEVICE;
   }
-  if (c->fast && !(cfg->flags & HMC_FLAG_GLOBAL_PASS)) {
+  if (c->fast && !lj && !(cfg->flags & HMC_FLAG_GLOBAL_PASS)) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0) {
       delete c;
@@ -1420,6 +1865,17 @@ int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
     }
     const size_t vt = cfg->precision == HMC_MIXED ? sizeof(float4) : sizeof(double4);
     c->tiled = choose_tiles(g, D, (double)cfg->n_bodies / (double)spaces, vt, cus, &c->tc);
+    if (c->tiled && cfg->precision == HMC_MIXED) {
+      // Band of the box-relative test.  Per axis: the FP32 roundings of x and
+      // t (2^-25 L each), of x - c and t - c (2^-24 L each), of L in the image
+      // shift (2^-24 L, twice), of the shifted values (2^-24 R each, R the
+      // largest box-relative coordinate) and of their difference
+      // (2^-24 |d|); the reference's own roundings add 2^-53 (1.5 L + |d|).
+      double R = 0.0;
+      for (int a = 0; a < D; ++a) R = std::max(R, 0.5 * c->tc.E[a] * g.width[a] + g.width[a]);
+      const double Et = (0x1p-24 * (5.1 * lmax + 2.0 * R + 2.0 * sigma) + 0x1p-53 * (1.5 * lmax + 2.0 * sigma)) * 1.01;
+      c->tiled = contact_band(Et, sigma, g.sigma2, g.edge2, D, &c->tc.band_lo, &c->tc.band_hi);
+    }
     if (c->tiled) {
       const void* fn = nullptr;
       if (D == 2)
@@ -1613,6 +2069,7 @@ int hmc_get_stats(hmc_ctx* c, hmc_stats* out) {
   out->exp_slow_path = s.exp_slow;
   out->guard_cycles = s.guard_cycles;
   out->tile_overflows = s.tile_overflows;
+  out->ambiguous = s.ambiguous;
   out->tiled = c->tiled ? 1u : 0u;
   for (int a = 0; a < 3; ++a) out->tile_shape[a] = c->tiled ? c->tc.T[a] : 0u;
   return HMC_OK;

@@ -42,7 +42,11 @@ enum {
 
 /* Pair model (the Hamiltonian H of PairwiseCutoff<H>). */
 enum {
-  HMC_HARD_SPHERE = 0 /* params[0] = diameter */
+  HMC_HARD_SPHERE = 0, /* HardSphere (pairwise/hard_shape.rs:127-174): params[0] = diameter */
+  HMC_LJ = 1,          /* Isotropic<LennardJones<12,6>> (isotropic.rs, univariate/lennard_jones.rs):
+                          params = {epsilon, sigma, r_cut, -}; HMC_FP64 only */
+  HMC_LJ_SHIFTED = 2   /* Isotropic<Shifted<LennardJones<12,6>>> (univariate/shifted.rs):
+                          params = {epsilon, sigma, r_cut, r_shift}; HMC_FP64 only */
 };
 
 /* Arithmetic of the overlap test. Both modes make identical decisions. */
@@ -99,6 +103,10 @@ typedef struct {
                                (their trials ran from global memory) */
   uint32_t tiled;           /* 1 if the pass stages neighbourhoods in LDS */
   uint32_t tile_shape[3];   /* colour spaces per workgroup along each axis */
+  uint64_t ambiguous;       /* LJ: Metropolis decisions that the bound on the
+                               summation-order difference could not settle
+                               (decided from the GPU's own sum; 0 means every
+                               decision provably equals the reference's) */
 } hmc_stats;
 
 /* Allocate device state for one system.  Replaces building a ParallelSweep and

@@ -1,0 +1,107 @@
+"""GPU parity for Lennard-Jones Metropolis sweeps (PairwiseCutoff<Isotropic<
+[Shifted<]LennardJones>>>) through the C ABI, against the oracle.
+
+Bar: per-apply counts, sorted per-trial traces and final FP64 positions bit
+for bit, and no ambiguous decision.  The GPU sums the pair energies in
+another order than the reference's VecCell order; every decision is taken
+only when the rigorous bound on that difference cannot change it, and the
+library counts the ones where it could (expected: none).
+"""
+import ctypes as C
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import _oracle as O
+import _pkg
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def hmc():
+    p = _pkg.pkg()
+    p.lib()
+    O.lib().oracle_set_libm.argtypes = [C.c_int]
+    O.lib().oracle_set_libm(1)
+    yield p
+    O.lib().oracle_set_libm(0)
+
+
+@pytest.fixture(scope="module")
+def wl():
+    return _pkg.workloads()
+
+
+def _replay_lj(hmc, L, pos, model, params, kT, d, n_applies, seed=42, trace=True):
+    ref = O.OracleSystem(L, pos, model=model, params=params, seed=seed)
+    b, s = ref.positions()
+    gpu = hmc.GpuParallelSweep(L, b, sites=s, model=model, params=params)
+    cap = 64 * len(pos) + 1024
+    if trace:
+        gpu.trace(True, capacity=cap)
+        ref.trace(True)
+    for k in range(n_applies):
+        step, sub = ref.counter()
+        ga, gr, used = gpu.apply(step, sub, seed, kT, d)
+        oa, orj = ref.parallel_apply(kT, d)
+        _, sub2 = ref.counter()
+        assert (ga, gr, used) == (oa, orj, sub2 - sub), (k, ga, gr, used, oa, orj, sub2 - sub)
+        if trace:
+            gt = gpu.read_trace(capacity=cap)
+            ot = ref.read_trace()
+            gt = gt[gt["status"] != 2]
+            ot = ot[ot["status"] != 2]
+            key = ["substep", "color", "space", "body", "status"]
+            gt = gt[np.lexsort((gt["space"], gt["color"], gt["substep"]))][key]
+            ot = ot[np.lexsort((ot["space"], ot["color"], ot["substep"]))][key]
+            assert np.array_equal(gt, ot), k
+        ref.increment_step()
+    gb, gs = gpu.positions()
+    ob, os_ = ref.positions()
+    assert np.array_equal(gb.view(np.uint64), ob.view(np.uint64))
+    assert np.array_equal(gs.view(np.uint64), os_.view(np.uint64))
+    st = gpu.stats()
+    gpu.close()
+    assert st["ambiguous"] == 0, st
+    return st, ref
+
+
+def test_lj_shifted_fcc_3d(hmc, wl):
+    # 864 LJ particles, FCC at number density 0.8, kT 1.2, r_cut = r_shift = 2.5
+    L, pos = wl.fcc_number_density((6, 6, 6), 0.8)
+    t = time.time()
+    st, _ = _replay_lj(hmc, L, pos, "lj_shifted", (1.0, 1.0, 2.5, 2.5), 1.2, 0.1, 10)
+    log(f"LJ shifted N=864: 10 applies bit-identical ({time.time() - t:.1f}s) {st}")
+
+
+def test_lj_plain_square_2d(hmc, wl):
+    # 2D LJ without shift on a 24 x 24 square lattice at density 0.7
+    L, pos = wl.square_lattice_disks(n_side=24, packing_fraction=0.7 * np.pi / 4)
+    st, _ = _replay_lj(hmc, L, pos, "lj", (1.0, 1.0, 2.5), 1.0, 0.15, 10, seed=3)
+    log(f"LJ 2D N=576: 10 applies bit-identical {st}")
+
+
+def test_lj_space_width_equal_to_cutoff(hmc, wl):
+    # box 10 with r_cut 2.5: four spaces of width exactly r_cut per axis, so the
+    # neighbour scan reaches two spaces out (and wraps onto itself), and the
+    # FCC second shell sits exactly at the cutoff
+    L, pos = wl.fcc_at_volume((4, 4, 4), 1000.0)
+    st, _ = _replay_lj(hmc, L, pos, "lj_shifted", (1.0, 1.0, 2.5, 2.5), 1.5, 0.2, 8, seed=9)
+    log(f"LJ reach-2 N=256: 8 applies bit-identical {st}")
+
+
+def test_lj_config_c4_full_size(hmc, wl):
+    # N = 524,288 LJ particles: FCC 64 x 64 x 32 cells at number density 0.8
+    # (box 109.44 x 109.44 x 54.72), kT 1.2, shifted at r_cut 2.5, d = 0.1
+    L, pos = wl.fcc_number_density((64, 64, 32), 0.8)
+    t = time.time()
+    st, ref = _replay_lj(hmc, L, pos, "lj_shifted", (1.0, 1.0, 2.5, 2.5), 1.2, 0.1, 2, trace=False)
+    log(f"C4 N={len(pos)}: 2 applies bit-identical ({time.time() - t:.1f}s) {st}")
