@@ -35,7 +35,26 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison run")
     ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--profile-period", type=int, default=8,
+                    help="time the kernels of every k-th timed apply with HIP events")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of known size (HBM counter calibration)")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """A plain device-to-device copy of 1 GiB, for scaling the FETCH_SIZE and
+    WRITE_SIZE counters of a profiled run to bytes."""
+    import torch
+    n = 1 << 30
+    x = torch.empty(n, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    x.fill_(1)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": 3}), flush=True)
 
 
 def pass_bytes(precision, n, n_spaces, nonempty, accepted_per_pass, dim=3):
@@ -54,11 +73,11 @@ def pass_bytes(precision, n, n_spaces, nonempty, accepted_per_pass, dim=3):
     return 4 * (n_spaces + 1) + site * n + 36 * active + write * accepted_per_pass
 
 
-def run_gpu(hmc, L, body, site, precision, steps, warmup, seed, kT, d):
+def run_gpu(hmc, L, body, site, precision, steps, warmup, seed, kT, d, period):
     sim = hmc.GpuParallelSweep(L, body, sites=site, precision=precision)
     # warm-up applies: steps 0 .. warmup-1
     sim.run(0, seed, kT, d, warmup)
-    sim.profile(True)
+    sim.profile(True, period)
     t0 = time.perf_counter()
     acc, rej = sim.run(warmup, seed, kT, d, steps)  # returns after the stream is synchronised
     t1 = time.perf_counter()
@@ -68,6 +87,18 @@ def run_gpu(hmc, L, body, site, precision, steps, warmup, seed, kT, d):
     sim.close()
     return dict(elapsed=t1 - t0, accepted=acc, rejected=rej, pass_ms=pass_ms, pass_n=pass_n,
                 build_ms=build_ms, build_n=build_n, stats=stats, final=final)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of the pass kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain copy), or None."""
+    f = ROOT / "profiles" / "pass_tile_c2_hbm_traffic.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:") and kernel.startswith("k_pass_tile"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def cpu_baseline(L, pos, seconds, seed, kT, d):
@@ -84,6 +115,9 @@ def cpu_baseline(L, pos, seconds, seed, kT, d):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
@@ -100,7 +134,7 @@ def main():
     body, site = ref.positions()
     del ref
 
-    mixed = run_gpu(hmc, L, body, site, "mixed", args.steps, args.warmup, args.seed, kT, d)
+    mixed = run_gpu(hmc, L, body, site, "mixed", args.steps, args.warmup, args.seed, kT, d, args.profile_period)
     trials = mixed["accepted"] + mixed["rejected"]
     value = trials / mixed["elapsed"]
     out = {
@@ -123,15 +157,16 @@ def main():
         avg_ms = mixed["pass_ms"] / mixed["pass_n"]
         b = pass_bytes("mixed", n, n_spaces, nonempty, acc_per_pass)
         achieved = b / (avg_ms * 1e-3) / 1e9
+        kernel = "k_pass_tile<3,MIXED>" if mixed["stats"]["tiled"] else "k_pass<3,MIXED>"
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                           "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                           "kernel": "k_pass<3,MIXED>", "avg_launch_ms": avg_ms,
+                           "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(kernel),
+                           "kernel": kernel, "avg_launch_ms": avg_ms,
                            "algorithmic_bytes_per_launch": b, "launches": mixed["pass_n"],
                            "build_ms_per_apply": mixed["build_ms"] / max(mixed["build_n"], 1)}
     out["band_retests"] = mixed["stats"]["band_retests"]  # FP64 re-tests over warm-up + timed applies
     out["exp_slow_path"] = mixed["stats"]["exp_slow_path"]
     if not args.no_fp64:
-        f64 = run_gpu(hmc, L, body, site, "fp64", args.steps, args.warmup, args.seed, kT, d)
+        f64 = run_gpu(hmc, L, body, site, "fp64", args.steps, args.warmup, args.seed, kT, d, args.profile_period)
         same = (f64["accepted"], f64["rejected"]) == (mixed["accepted"], mixed["rejected"])
         import numpy as np
         same = same and np.array_equal(f64["final"][0], mixed["final"][0]) and np.array_equal(

@@ -198,8 +198,9 @@ class GpuParallelSweep:
         out["tile_shape"] = tuple(int(v) for v in s.tile_shape)
         return {k: v if k == "tile_shape" else int(v) for k, v in out.items()}
 
-    def profile(self, on=True):
-        self._check(lib().hmc_profile_enable(self.h, 1 if on else 0))
+    def profile(self, on=True, period=1):
+        """Time every period-th apply's launches with HIP events (on=False: off)."""
+        self._check(lib().hmc_profile_enable(self.h, int(period) if on else 0))
 
     def read_profile(self):
         pm = np.zeros(1)

@@ -1504,6 +1504,7 @@ struct hmc_ctx {
   uint64_t applies = 0;
   std::string err;
   bool profiling = false;
+  uint32_t profile_period = 1;  // time the launches of every k-th apply
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pass_events, build_events;
   double pass_ms = 0.0, build_ms = 0.0;
@@ -1547,8 +1548,9 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
   const Geometry& g = c->g;
   hipStream_t s = c->stream;
   const uint32_t N = g.n_bodies;
+  const bool prof = c->profiling && c->applies % c->profile_period == 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (c->profiling) {
+  if (prof) {
     e0 = take_event(c);
     (void)hipEventRecord(e0, s);
   }
@@ -1562,7 +1564,7 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
     k_scatter<<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->w);
     k_place<PREC><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->bodies, c->cells, c->w);
   }
-  if (c->profiling) {
+  if (prof) {
     e1 = take_event(c);
     (void)hipEventRecord(e1, s);
     c->build_events.push_back({e0, e1});
@@ -1580,7 +1582,7 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
   for (uint32_t k = 0; k < launched; ++k) {
     for (int col = 0; col < (1 << D); ++col) {
       hipEvent_t p0 = nullptr, p1 = nullptr;
-      if (c->profiling) {
+      if (prof) {
         p0 = take_event(c);
         (void)hipEventRecord(p0, s);
       }
@@ -1593,7 +1595,7 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
         k_pass<D, PREC, true><<<grid, kBlock, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k, chunk);
       else
         k_pass<D, PREC, false><<<grid, kBlock, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k, chunk);
-      if (c->profiling) {
+      if (prof) {
         p1 = take_event(c);
         (void)hipEventRecord(p1, s);
         c->pass_events.push_back({p0, p1});
@@ -2077,7 +2079,8 @@ int hmc_get_stats(hmc_ctx* c, hmc_stats* out) {
 
 int hmc_profile_enable(hmc_ctx* c, int on) {
   if (!c) return HMC_ERR_INVALID;
-  c->profiling = on != 0;
+  c->profiling = on > 0;
+  c->profile_period = on > 0 ? (uint32_t)on : 1u;
   c->pass_ms = c->build_ms = 0.0;
   c->pass_n = c->build_n = 0;
   return HMC_OK;

@@ -144,9 +144,11 @@ int hmc_trace_read(hmc_ctx* ctx, hmc_trace_record* buf, uint64_t cap, uint64_t* 
 
 int hmc_get_stats(hmc_ctx* ctx, hmc_stats* out);
 
-/* Average duration (ms) of the most recent timed launches of the pass kernel,
- * measured with HIP events on the library's stream; 0 when not timed. */
-int hmc_profile_enable(hmc_ctx* ctx, int on);
+/* Time the build and pass launches with HIP events on the library's stream:
+ * period <= 0 turns timing off; period k >= 1 times every k-th apply (a small
+ * k adds event records between launches).  hmc_profile_read returns the
+ * summed durations (ms) and launch counts since the last enable. */
+int hmc_profile_enable(hmc_ctx* ctx, int period);
 int hmc_profile_read(hmc_ctx* ctx, double* pass_ms_total, uint64_t* pass_launches, double* build_ms_total,
                      uint64_t* builds);
 

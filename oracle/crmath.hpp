@@ -79,9 +79,32 @@ inline double cr_exp(double y) {
   return std::ldexp(p.hi + p.lo, (int)k);
 }
 
+// sin(x) and cos(x) for |x| <= 2 (the rotation sampler calls them on
+// [0, pi/2]): Taylor series in Horner form, 20 terms, in double-double.
+inline double cr_sin(double x) {
+  DD x2 = dd_mul({x, 0.0}, {x, 0.0});
+  DD p = {1.0, 0.0};
+  for (int n = 20; n >= 1; --n) {
+    DD t = dd_div(dd_mul(p, x2), {(double)(2 * n) * (double)(2 * n + 1), 0.0});
+    p = dd_add({1.0, 0.0}, {-t.hi, -t.lo});
+  }
+  DD r = dd_mul(p, {x, 0.0});
+  return r.hi + r.lo;
+}
+inline double cr_cos(double x) {
+  DD x2 = dd_mul({x, 0.0}, {x, 0.0});
+  DD p = {1.0, 0.0};
+  for (int n = 20; n >= 1; --n) {
+    DD t = dd_div(dd_mul(p, x2), {(double)(2 * n - 1) * (double)(2 * n), 0.0});
+    p = dd_add({1.0, 0.0}, {-t.hi, -t.lo});
+  }
+  return p.hi + p.lo;
+}
+
 struct LibmState {
   int mode = 0;  // 0 glibc, 1 correctly rounded
   std::atomic<uint64_t> log_calls{0}, log_divergent{0}, exp_calls{0}, exp_divergent{0};
+  std::atomic<uint64_t> trig_calls{0}, trig_divergent{0};
 };
 inline LibmState& libm() {
   static LibmState s;
@@ -93,6 +116,25 @@ inline double lib_log(double x) {
   double g = std::log(x), c = cr_log(x);
   s.log_calls.fetch_add(1, std::memory_order_relaxed);
   if (g != c) s.log_divergent.fetch_add(1, std::memory_order_relaxed);
+  return s.mode ? c : g;
+}
+
+// sin / cos as the reference calls them (glibc), or correctly rounded; the
+// correctly rounded versions cover |x| <= 2 only.
+inline double lib_sin(double x) {
+  LibmState& s = libm();
+  double g = std::sin(x);
+  double c = std::fabs(x) <= 2.0 ? cr_sin(x) : g;
+  s.trig_calls.fetch_add(1, std::memory_order_relaxed);
+  if (g != c) s.trig_divergent.fetch_add(1, std::memory_order_relaxed);
+  return s.mode ? c : g;
+}
+inline double lib_cos(double x) {
+  LibmState& s = libm();
+  double g = std::cos(x);
+  double c = std::fabs(x) <= 2.0 ? cr_cos(x) : g;
+  s.trig_calls.fetch_add(1, std::memory_order_relaxed);
+  if (g != c) s.trig_divergent.fetch_add(1, std::memory_order_relaxed);
   return s.mode ? c : g;
 }
 

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <memory>
 
+#include "convex.hpp"
 #include "geometry.hpp"
 #include "mc.hpp"
 #include "rng.hpp"
@@ -78,9 +79,18 @@ void oracle_libm_reset() {
   libm().log_divergent = 0;
   libm().exp_calls = 0;
   libm().exp_divergent = 0;
+  libm().trig_calls = 0;
+  libm().trig_divergent = 0;
+}
+// sin/cos calls, and those where glibc and the correctly rounded value differ
+void oracle_trig_counters(uint64_t* out) {
+  out[0] = libm().trig_calls.load();
+  out[1] = libm().trig_divergent.load();
 }
 double oracle_cr_log(double x) { return cr_log(x); }
 double oracle_cr_exp(double y) { return cr_exp(y); }
+double oracle_cr_sin(double x) { return cr_sin(x); }
+double oracle_cr_cos(double x) { return cr_cos(x); }
 
 // ---- RNG and adapter probes -------------------------------------------------
 void oracle_sfc64_stream(uint64_t a, uint64_t b, uint64_t c, uint64_t ctr, int warm, uint64_t* out, uint64_t n) {
@@ -370,6 +380,136 @@ uint64_t oracle_time_applies(void* h, int parallel, double kt, double max_move, 
     }
   }
   *applies = n;
+  return total;
+}
+
+
+// ---- hard convex polyhedra (oracle/convex.hpp) -------------------------------
+static std::vector<V3> read_verts(const double* v, uint32_t nv) {
+  std::vector<V3> out(nv);
+  for (uint32_t i = 0; i < nv; ++i) out[i] = {v[3 * i], v[3 * i + 1], v[3 * i + 2]};
+  return out;
+}
+static Quat read_quat(const double* q) { return Quat{q[0], {q[1], q[2], q[3]}}; }
+
+// collide3d on a polytope with itself at separation vij and relative
+// orientation qij = (s, x, y, z)
+int oracle_convex_collide(const double* verts, uint32_t nv, const double* vij, const double* qij) {
+  Polytope p;
+  p.set(read_verts(verts, nv));
+  return collide3d(p, p, V3{vij[0], vij[1], vij[2]}, read_quat(qij)) ? 1 : 0;
+}
+int oracle_convex_intersects_global(const double* verts, uint32_t nv, const double* ri, const double* qi,
+                                    const double* rj, const double* qj) {
+  Polytope p;
+  p.set(read_verts(verts, nv));
+  return intersects_at_global(p, V3{ri[0], ri[1], ri[2]}, read_quat(qi), V3{rj[0], rj[1], rj[2]}, read_quat(qj))
+             ? 1
+             : 0;
+}
+// quaternion probes: op 0 product a*b, 1 rotate b[1..3] by a, 2 rotation-matrix rotate
+void oracle_quat_op(int op, const double* a, const double* b, double* out) {
+  Quat qa = read_quat(a);
+  if (op == 0) {
+    Quat r = qmul(qa, read_quat(b));
+    out[0] = r.s;
+    out[1] = r.v[0];
+    out[2] = r.v[1];
+    out[3] = r.v[2];
+  } else {
+    V3 x{b[1], b[2], b[3]};
+    V3 r = op == 1 ? qrotate(qa, x) : mrotate(rotmat(qa), x);
+    out[0] = 0.0;
+    out[1] = r[0];
+    out[2] = r[1];
+    out[3] = r[2];
+  }
+}
+void oracle_versor_displacement(uint64_t step, uint32_t substep, uint32_t seed, uint64_t index, double sigma,
+                                double* out, uint64_t n) {
+  Sfc64 rng = counter_rng(step, substep, seed, index);
+  for (uint64_t i = 0; i < n; ++i) {
+    Quat q = versor_displacement(rng, sigma);
+    out[4 * i] = q.s;
+    out[4 * i + 1] = q.v[0];
+    out[4 * i + 2] = q.v[1];
+    out[4 * i + 3] = q.v[2];
+  }
+}
+
+void* oracle_convex_create(const double* L, double boundary_range, double body_range, const double* verts,
+                           uint32_t nv, uint64_t step, uint32_t seed, const double* pos, const double* quat,
+                           uint64_t n) {
+  auto* s = new ConvexSystem({L[0], L[1], L[2]}, boundary_range, read_verts(verts, nv));
+  s->body_range = body_range;
+  s->step = step;
+  s->seed = seed;
+  if (!s->boundary.valid() || s->range() > boundary_range) {
+    delete s;
+    return nullptr;
+  }
+  for (uint64_t b = 0; b < n; ++b) s->add_body(V3{pos[3 * b], pos[3 * b + 1], pos[3 * b + 2]}, read_quat(quat + 4 * b));
+  return s;
+}
+void oracle_convex_destroy(void* h) { delete (ConvexSystem*)h; }
+int oracle_convex_apply(void* h, int rotate, double max_move, uint64_t* acc, uint64_t* rej) {
+  return ((ConvexSystem*)h)->parallel_apply(rotate != 0, max_move, *acc, *rej) ? 0 : -1;
+}
+void oracle_convex_increment_step(void* h) { ((ConvexSystem*)h)->increment_step(); }
+void oracle_convex_get_counter(void* h, uint64_t* step, uint32_t* substep) {
+  *step = ((ConvexSystem*)h)->step;
+  *substep = ((ConvexSystem*)h)->substep;
+}
+// bodies and sites in body order: positions n x 3, orientations n x 4
+void oracle_convex_state(void* h, double* bpos, double* bq, double* spos, double* sq) {
+  auto* S = (ConvexSystem*)h;
+  for (size_t b = 0; b < S->bpos.size(); ++b) {
+    const CSite& st = S->sites.items[S->sites.indices[b]];
+    for (int i = 0; i < 3; ++i) {
+      bpos[3 * b + i] = S->bpos[b][i];
+      spos[3 * b + i] = st.pos[i];
+    }
+    bq[4 * b] = S->bq[b].s;
+    sq[4 * b] = st.q.s;
+    for (int i = 0; i < 3; ++i) {
+      bq[4 * b + 1 + i] = S->bq[b].v[i];
+      sq[4 * b + 1 + i] = st.q.v[i];
+    }
+  }
+}
+int oracle_convex_any_overlap(void* h) { return ((ConvexSystem*)h)->any_overlap() ? 1 : 0; }
+void oracle_convex_trace_enable(void* h, int on) {
+  ((ConvexSystem*)h)->tracing = on != 0;
+  ((ConvexSystem*)h)->trace.clear();
+}
+uint64_t oracle_convex_trace_size(void* h) { return ((ConvexSystem*)h)->trace.size(); }
+void oracle_convex_trace_read(void* h, void* out) {
+  auto* S = (ConvexSystem*)h;
+  std::memcpy(out, S->trace.data(), S->trace.size() * sizeof(TraceRecord));
+  S->trace.clear();
+}
+// Timed loop of {translate apply; rotate apply; increment_step}, as the
+// reference's convex-shape benchmarks advance (benchmarks/src/mc/octahedron.rs:115-127).
+uint64_t oracle_convex_time(void* h, double d, double a, double seconds, double* elapsed, uint64_t* steps) {
+  using clk = std::chrono::steady_clock;
+  auto* S = (ConvexSystem*)h;
+  auto t0 = clk::now();
+  uint64_t total = 0, n = 0;
+  for (;;) {
+    uint64_t x = 0, y = 0;
+    S->parallel_apply(false, d, x, y);
+    total += x + y;
+    S->parallel_apply(true, a, x, y);
+    total += x + y;
+    S->increment_step();
+    ++n;
+    double dt = std::chrono::duration<double>(clk::now() - t0).count();
+    if (dt >= seconds) {
+      *elapsed = dt;
+      break;
+    }
+  }
+  *steps = n;
   return total;
 }
 

@@ -279,3 +279,141 @@ class OracleSystem:
 
 def set_threads(n):
     lib().oracle_set_threads(n)
+
+
+# ---------------------------------------------------------- convex polyhedra --
+CUBE = np.array([[x, y, z] for z in (-0.5, 0.5) for y in (-0.5, 0.5) for x in (-0.5, 0.5)], np.float64)
+CUBE = CUBE[[0, 1, 3, 2, 4, 5, 7, 6]]  # the reference fixture's vertex order
+
+
+def _convex_lib():
+    L = lib()
+    if not getattr(L, "_convex_ready", False):
+        L.oracle_convex_collide.argtypes = [D_P, C.c_uint32, D_P, D_P]
+        L.oracle_convex_collide.restype = C.c_int
+        L.oracle_convex_intersects_global.argtypes = [D_P, C.c_uint32, D_P, D_P, D_P, D_P]
+        L.oracle_convex_intersects_global.restype = C.c_int
+        L.oracle_quat_op.argtypes = [C.c_int, D_P, D_P, D_P]
+        L.oracle_versor_displacement.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, C.c_double,
+                                                 D_P, C.c_uint64]
+        L.oracle_convex_create.argtypes = [D_P, C.c_double, C.c_double, D_P, C.c_uint32, C.c_uint64, C.c_uint32,
+                                           D_P, D_P, C.c_uint64]
+        L.oracle_convex_create.restype = C.c_void_p
+        L.oracle_convex_destroy.argtypes = [C.c_void_p]
+        L.oracle_convex_apply.argtypes = [C.c_void_p, C.c_int, C.c_double, U64_P, U64_P]
+        L.oracle_convex_apply.restype = C.c_int
+        L.oracle_convex_increment_step.argtypes = [C.c_void_p]
+        L.oracle_convex_get_counter.argtypes = [C.c_void_p, U64_P, U32_P]
+        L.oracle_convex_state.argtypes = [C.c_void_p, D_P, D_P, D_P, D_P]
+        L.oracle_convex_any_overlap.argtypes = [C.c_void_p]
+        L.oracle_convex_any_overlap.restype = C.c_int
+        L.oracle_convex_trace_enable.argtypes = [C.c_void_p, C.c_int]
+        L.oracle_convex_trace_size.argtypes = [C.c_void_p]
+        L.oracle_convex_trace_size.restype = C.c_uint64
+        L.oracle_convex_trace_read.argtypes = [C.c_void_p, C.c_void_p]
+        L.oracle_convex_time.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, D_P, U64_P]
+        L.oracle_convex_time.restype = C.c_uint64
+        L.oracle_trig_counters.argtypes = [U64_P]
+        L._convex_ready = True
+    return L
+
+
+def _d(a, n=None):
+    a = np.ascontiguousarray(a, np.float64)
+    return a
+
+
+def convex_collide(verts, vij, qij):
+    v = _d(verts)
+    return bool(_convex_lib().oracle_convex_collide(_p(v, D_P), len(v), _p(_d(vij), D_P), _p(_d(qij), D_P)))
+
+
+def convex_intersects_global(verts, ri, qi, rj, qj):
+    v = _d(verts)
+    args = [_p(_d(x), D_P) for x in (ri, qi, rj, qj)]
+    return bool(_convex_lib().oracle_convex_intersects_global(_p(v, D_P), len(v), *args))
+
+
+def quat_op(op, a, b):
+    out = np.zeros(4)
+    _convex_lib().oracle_quat_op(op, _p(_d(a), D_P), _p(_d(b), D_P), _p(out, D_P))
+    return out
+
+
+def versor_displacement(step, substep, seed, index, sigma, n):
+    out = np.zeros((n, 4))
+    _convex_lib().oracle_versor_displacement(step, substep, seed, index, sigma, _p(out, D_P), n)
+    return out
+
+
+def trig_counters():
+    out = np.zeros(2, np.uint64)
+    _convex_lib().oracle_trig_counters(_p(out, U64_P))
+    return int(out[0]), int(out[1])
+
+
+class ConvexOracle:
+    """The restated Microstate<OrientedPoint> + HardShape<Convex> + ParallelSweep."""
+
+    def __init__(self, L, positions, orientations, verts, *, boundary_range=None, body_range=None, step=0, seed=0):
+        self.L = _d(L)
+        pos = _d(positions).reshape(-1, 3)
+        q = _d(orientations).reshape(-1, 4)
+        v = _d(verts).reshape(-1, 3)
+        r = float(np.sqrt(np.max(np.sum(v * v, axis=1)))) * 2.0
+        self.range = r
+        self.n = len(pos)
+        self._keep = (pos, q, v)
+        self.h = _convex_lib().oracle_convex_create(_p(self.L, D_P), r if boundary_range is None else boundary_range,
+                                                    r if body_range is None else body_range, _p(v, D_P), len(v),
+                                                    step, seed, _p(pos, D_P), _p(q, D_P), self.n)
+        if not self.h:
+            raise ValueError("invalid convex oracle configuration")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _convex_lib().oracle_convex_destroy(self.h)
+            self.h = None
+
+    def apply(self, rotate, max_move):
+        a = np.zeros(1, np.uint64)
+        r = np.zeros(1, np.uint64)
+        rc = _convex_lib().oracle_convex_apply(self.h, 1 if rotate else 0, max_move, _p(a, U64_P), _p(r, U64_P))
+        if rc:
+            raise RuntimeError("a body lies outside the checkerboard")
+        return int(a[0]), int(r[0])
+
+    def increment_step(self):
+        _convex_lib().oracle_convex_increment_step(self.h)
+
+    def counter(self):
+        s = np.zeros(1, np.uint64)
+        ss = np.zeros(1, np.uint32)
+        _convex_lib().oracle_convex_get_counter(self.h, _p(s, U64_P), _p(ss, U32_P))
+        return int(s[0]), int(ss[0])
+
+    def state(self):
+        bp = np.zeros((self.n, 3))
+        bq = np.zeros((self.n, 4))
+        sp = np.zeros((self.n, 3))
+        sq = np.zeros((self.n, 4))
+        _convex_lib().oracle_convex_state(self.h, _p(bp, D_P), _p(bq, D_P), _p(sp, D_P), _p(sq, D_P))
+        return bp, bq, sp, sq
+
+    def any_overlap(self):
+        return bool(_convex_lib().oracle_convex_any_overlap(self.h))
+
+    def trace(self, on=True):
+        _convex_lib().oracle_convex_trace_enable(self.h, 1 if on else 0)
+
+    def read_trace(self):
+        n = _convex_lib().oracle_convex_trace_size(self.h)
+        out = np.zeros(n, TRACE_DTYPE)
+        _convex_lib().oracle_convex_trace_read(self.h, out.ctypes.data)
+        return out
+
+    def time_steps(self, d, a, seconds):
+        el = np.zeros(1)
+        st = np.zeros(1, np.uint64)
+        moves = _convex_lib().oracle_convex_time(self.h, d, a, seconds, _p(el, D_P), _p(st, U64_P))
+        return int(moves), float(el[0]), int(st[0])

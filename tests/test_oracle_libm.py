@@ -1,4 +1,5 @@
-"""The oracle's correctly rounded log/exp against 45-digit decimal vectors, and
+"""The oracle's correctly rounded log/exp (45-digit decimal vectors) and sin/cos
+(60-digit Taylor vectors), and
 the size of the glibc boundary those functions stand in for."""
 import ctypes as C
 import json
@@ -6,6 +7,7 @@ import math
 import pathlib
 
 import numpy as np
+import pytest
 
 import _oracle as O
 
@@ -18,6 +20,10 @@ def _lib():
     L.oracle_cr_log.restype = C.c_double
     L.oracle_cr_exp.argtypes = [C.c_double]
     L.oracle_cr_exp.restype = C.c_double
+    L.oracle_cr_sin.argtypes = [C.c_double]
+    L.oracle_cr_sin.restype = C.c_double
+    L.oracle_cr_cos.argtypes = [C.c_double]
+    L.oracle_cr_cos.restype = C.c_double
     L.oracle_set_libm.argtypes = [C.c_int]
     L.oracle_libm_counters.argtypes = [O.U64_P]
     return L
@@ -37,6 +43,17 @@ def test_cr_exp_matches_decimal():
     for yh, wh in d["exp"]:
         y, want = float.fromhex(yh), float.fromhex(wh)
         assert L.oracle_cr_exp(y) == want, (yh, wh)
+
+
+@pytest.mark.parametrize("fn", ["sin", "cos"])
+def test_cr_sin_cos_match_taylor_at_60_digits(fn):
+    # the rotation sampler's sin/cos of half the angle, on [0, pi/2]
+    L = _lib()
+    f = getattr(L, f"oracle_cr_{fn}")
+    d = json.loads(GOLDEN.read_text())
+    for xh, wh in d[fn]:
+        x, want = float.fromhex(xh), float.fromhex(wh)
+        assert f(x) == want, (fn, xh, wh)
 
 
 def test_glibc_log_is_not_always_correctly_rounded():
