@@ -23,13 +23,17 @@ HMC_ERR_STATE = -5
 HMC_HARD_SPHERE = 0
 HMC_LJ = 1
 HMC_LJ_SHIFTED = 2
+HMC_CONVEX_POLYHEDRON = 3
 HMC_FP64 = 0
 HMC_MIXED = 1
 
 # every entry point declared in include/hmc.h
-EXPORTS = ["hmc_create", "hmc_set_state", "hmc_get_state", "hmc_apply_translate", "hmc_run_translate",
+EXPORTS = ["hmc_create", "hmc_set_state", "hmc_get_state", "hmc_apply_translate", "hmc_apply_rotate",
+           "hmc_set_orientations", "hmc_get_orientations", "hmc_run_translate", "hmc_total_energy",
            "hmc_trace_enable", "hmc_trace_read", "hmc_get_stats", "hmc_profile_enable", "hmc_profile_read",
-           "hmc_debug_math", "hmc_debug_normals", "hmc_last_error", "hmc_destroy"]
+           "hmc_debug_math", "hmc_debug_normals", "hmc_debug_convex", "hmc_last_error", "hmc_destroy",
+           "hmc_batch_create", "hmc_batch_set_state", "hmc_batch_get_state", "hmc_batch_run_translate",
+           "hmc_batch_last_error", "hmc_batch_destroy"]
 
 
 class HmcError(RuntimeError):
@@ -43,7 +47,8 @@ class Config(C.Structure):
                 ("boundary_range", C.c_double), ("body_interaction_range", C.c_double),
                 ("spatial_cell_width", C.c_double),
                 ("model", C.c_int32), ("params", C.c_double * 4), ("precision", C.c_int32),
-                ("device", C.c_int32), ("flags", C.c_uint32)]
+                ("device", C.c_int32), ("flags", C.c_uint32),
+                ("vertices", C.POINTER(C.c_double)), ("n_vertices", C.c_uint32)]
 
 FLAG_GLOBAL_PASS = 1
 
@@ -81,6 +86,20 @@ def lib():
                                           C.POINTER(Count), C.POINTER(C.c_uint32)]
         L.hmc_run_translate.argtypes = [vp, C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_uint32,
                                         C.POINTER(Count)]
+        L.hmc_apply_rotate.argtypes = [vp, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
+                                       C.POINTER(Count), C.POINTER(C.c_uint32)]
+        L.hmc_set_orientations.argtypes = [vp, D_P, D_P]
+        L.hmc_total_energy.argtypes = [vp, D_P, D_P]
+        L.hmc_batch_create.argtypes = [C.c_uint32, C.POINTER(Config), C.POINTER(vp)]
+        L.hmc_batch_set_state.argtypes = [vp, C.c_uint32, D_P, D_P]
+        L.hmc_batch_get_state.argtypes = [vp, C.c_uint32, D_P, D_P]
+        L.hmc_batch_run_translate.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), C.c_double, D_P,
+                                              C.c_uint32, C.c_int, C.POINTER(Count)]
+        L.hmc_batch_last_error.argtypes = [vp]
+        L.hmc_batch_last_error.restype = C.c_char_p
+        L.hmc_batch_destroy.argtypes = [vp]
+        L.hmc_get_orientations.argtypes = [vp, D_P, D_P]
+        L.hmc_debug_convex.argtypes = [C.c_int, C.c_int, D_P, C.c_uint32, D_P, D_P, C.c_uint64]
         L.hmc_trace_enable.argtypes = [vp, C.c_int, C.c_uint64]
         L.hmc_trace_read.argtypes = [vp, vp, C.c_uint64, C.POINTER(C.c_uint64)]
         L.hmc_get_stats.argtypes = [vp, C.POINTER(Stats)]
@@ -107,7 +126,8 @@ class GpuParallelSweep:
 
     def __init__(self, box_edges, positions, *, sites=None, diameter=1.0, boundary_range=None,
                  body_interaction_range=None, spatial_cell_width=None, precision="fp64", device=0,
-                 lds_tiles=True, model="hard_sphere", params=None):
+                 lds_tiles=True, model="hard_sphere", params=None, vertices=None, orientations=None,
+                 site_orientations=None):
         """positions / sites: the microstate's body and site positions (already
         wrapped into the box, as Microstate::add_body stores them); sites=None
         derives them as wrap(body + 0).  lds_tiles=False runs every trial from
@@ -115,8 +135,10 @@ class GpuParallelSweep:
 
         model: "hard_sphere" (params = (diameter,), default (diameter,)),
         "lj" (params = (epsilon, sigma, r_cut)) or "lj_shifted"
-        (params = (epsilon, sigma, r_cut, r_shift)); the ranges default to the
-        model's interaction range (diameter or r_cut)."""
+        (params = (epsilon, sigma, r_cut, r_shift)) or "convex" (hard convex
+        polyhedra: vertices (k x 3), orientations (n x 4, s x y z; identity by
+        default)); the ranges default to the model's interaction range
+        (diameter, r_cut or twice the circumsphere radius)."""
         pos = np.ascontiguousarray(positions, np.float64)
         self.dim = pos.shape[1]
         self.n = pos.shape[0]
@@ -125,10 +147,18 @@ class GpuParallelSweep:
         cfg.n_bodies = self.n
         for a in range(self.dim):
             cfg.box_edges[a] = float(box_edges[a])
-        model_id = {"hard_sphere": HMC_HARD_SPHERE, "lj": HMC_LJ, "lj_shifted": HMC_LJ_SHIFTED}[model]
+        model_id = {"hard_sphere": HMC_HARD_SPHERE, "lj": HMC_LJ, "lj_shifted": HMC_LJ_SHIFTED,
+                    "convex": HMC_CONVEX_POLYHEDRON}[model]
         if params is None:
             params = (diameter,) if model == "hard_sphere" else (1.0, 1.0, 2.5, 2.5)
-        rng = params[0] if model == "hard_sphere" else params[2]
+        if model == "convex":
+            self._verts = np.ascontiguousarray(vertices, np.float64).reshape(-1, 3)
+            cfg.vertices = _p(self._verts)
+            cfg.n_vertices = len(self._verts)
+            rng = 2.0 * float(np.sqrt(np.max(np.sum(self._verts**2, axis=1))))
+            params = ()
+        else:
+            rng = params[0] if model == "hard_sphere" else params[2]
         cfg.boundary_range = rng if boundary_range is None else boundary_range
         cfg.body_interaction_range = rng if body_interaction_range is None else body_interaction_range
         # VecCell::with_search_radius(maximum interaction range), as the
@@ -146,6 +176,8 @@ class GpuParallelSweep:
             raise HmcError(rc, "hmc_create failed")
         self.h = h
         self.set_positions(pos, sites)
+        if orientations is not None:
+            self.set_orientations(orientations, site_orientations)
 
     def _check(self, rc):
         if rc != HMC_OK:
@@ -165,6 +197,32 @@ class GpuParallelSweep:
         self._check(lib().hmc_apply_translate(self.h, step, substep, seed, kT, max_distance, C.byref(cnt),
                                               C.byref(used)))
         return int(cnt.accepted), int(cnt.rejected), int(used.value)
+
+    def apply_rotate(self, step, substep, seed, kT, max_rotation):
+        cnt = Count()
+        used = C.c_uint32()
+        self._check(lib().hmc_apply_rotate(self.h, step, substep, seed, kT, max_rotation, C.byref(cnt),
+                                           C.byref(used)))
+        return int(cnt.accepted), int(cnt.rejected), int(used.value)
+
+    def total_energy(self):
+        """(energy, bound): PairwiseCutoff::total_energy of the current state and
+        the bound on its difference from any other summation order."""
+        e = C.c_double()
+        b = C.c_double()
+        self._check(lib().hmc_total_energy(self.h, C.byref(e), C.byref(b)))
+        return e.value, b.value
+
+    def orientations(self):
+        b = np.zeros((self.n, 4))
+        s = np.zeros((self.n, 4))
+        self._check(lib().hmc_get_orientations(self.h, _p(b), _p(s)))
+        return b, s
+
+    def set_orientations(self, body, site=None):
+        body = np.ascontiguousarray(body, np.float64)
+        site_p = None if site is None else _p(np.ascontiguousarray(site, np.float64))
+        self._check(lib().hmc_set_orientations(self.h, _p(body), site_p))
 
     def run(self, step0, seed, kT, max_distance, n_steps):
         cnt = Count()
@@ -212,7 +270,7 @@ class GpuParallelSweep:
 
 
 def debug_math(kind, x, device=0):
-    kinds = {"cbrt": 0, "log": 1, "exp": 2, "sqrt": 3, "div": 4}
+    kinds = {"cbrt": 0, "log": 1, "exp": 2, "sqrt": 3, "div": 4, "sin": 5, "cos": 6}
     x = np.ascontiguousarray(x, np.float64)
     n = x.size // 2 if kind == "div" else x.size
     out = np.zeros(n)
@@ -228,3 +286,81 @@ def debug_normals(step, substep, seed, index, n, device=0):
     if rc != HMC_OK:
         raise HmcError(rc, "hmc_debug_normals failed")
     return out
+
+
+def debug_convex(kind, vertices, rows, device=0):
+    """Device convex routines on rows of inputs: "collide" (v_ij[3], q_ij[4]),
+    "global" (r_i[3], q_i[4], r_j[3], q_j[4]) or "rotate" (step, substep,
+    seed, index, sigma)."""
+    k = {"collide": 0, "global": 1, "rotate": 2}[kind]
+    v = np.ascontiguousarray(vertices, np.float64).reshape(-1, 3)
+    x = np.ascontiguousarray(rows, np.float64)
+    n = x.shape[0]
+    out = np.zeros((n, 4) if k == 2 else n)
+    rc = lib().hmc_debug_convex(device, k, _p(v), len(v), _p(x), _p(out), n)
+    if rc != HMC_OK:
+        raise HmcError(rc, "hmc_debug_convex failed")
+    return out
+
+
+class GpuBatch:
+    """Independent hard-sphere/disk replicas advanced together (one set of
+    launches per step, replayed from a captured hipGraph)."""
+
+    def __init__(self, systems, *, diameter=1.0, precision="fp64", device=0):
+        """systems: list of (box_edges, body_positions, site_positions or None)."""
+        self.R = len(systems)
+        cfgs = (Config * self.R)()
+        self.dims = []
+        for r, (L, pos, _site) in enumerate(systems):
+            pos = np.asarray(pos, np.float64)
+            c = cfgs[r]
+            c.dim = pos.shape[1]
+            c.n_bodies = pos.shape[0]
+            for a in range(c.dim):
+                c.box_edges[a] = float(L[a])
+            c.boundary_range = c.body_interaction_range = c.spatial_cell_width = diameter
+            c.model = HMC_HARD_SPHERE
+            c.params[0] = diameter
+            c.precision = {"fp64": HMC_FP64, "mixed": HMC_MIXED}[precision]
+            c.device = device
+            self.dims.append((pos.shape[0], pos.shape[1]))
+        h = C.c_void_p()
+        rc = lib().hmc_batch_create(self.R, cfgs, C.byref(h))
+        if rc != HMC_OK:
+            raise HmcError(rc, "hmc_batch_create failed")
+        self.h = h
+        for r, (_L, pos, site) in enumerate(systems):
+            self.set_positions(r, pos, site)
+
+    def _check(self, rc):
+        if rc != HMC_OK:
+            raise HmcError(rc, lib().hmc_batch_last_error(self.h).decode())
+
+    def set_positions(self, r, body, site=None):
+        body = np.ascontiguousarray(body, np.float64)
+        site_p = None if site is None else _p(np.ascontiguousarray(site, np.float64))
+        self._check(lib().hmc_batch_set_state(self.h, r, _p(body), site_p))
+
+    def positions(self, r):
+        n, d = self.dims[r]
+        b = np.zeros((n, d))
+        s = np.zeros((n, d))
+        self._check(lib().hmc_batch_get_state(self.h, r, _p(b), _p(s)))
+        return b, s
+
+    def run(self, step0, seeds, kT, max_distance, n_steps, use_graph=True):
+        st = (C.c_uint64 * self.R)(*[int(x) for x in np.broadcast_to(step0, (self.R,))])
+        sd = (C.c_uint32 * self.R)(*[int(x) for x in np.broadcast_to(seeds, (self.R,))])
+        md = np.ascontiguousarray(np.broadcast_to(np.asarray(max_distance, np.float64), (self.R,)))
+        out = (Count * self.R)()
+        self._check(lib().hmc_batch_run_translate(self.h, st, sd, kT, _p(md), n_steps, 1 if use_graph else 0, out))
+        return [(int(c.accepted), int(c.rejected)) for c in out]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hmc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -7,6 +7,7 @@
 //    sampler produces (checked by sampling in the test suite).
 //  * hmc_log_cr / hmc_exp_cr: correctly rounded natural log and exp evaluated
 //    in double-double arithmetic (about 100 bits) and rounded once.
+//  * hmc_sin_cr / hmc_cos_cr: correctly rounded sin and cos on |x| <= 2.
 //  * hmc_lt_exp: decides lhs < exp(y) exactly (as if exp were correctly
 //    rounded): the fast library exp decides unless lhs lies within 2^-44
 //    (relative) of it, in which case the correctly rounded exp decides.
@@ -133,6 +134,31 @@ HMC_HD_COLD double hmc_exp_cr(double y) {
     p = dd_add(p, dd{1.0, 0.0});
   }
   return ldexp(p.hi + p.lo, (int)k);
+}
+
+// Correctly rounded sin(x) and cos(x) for |x| <= 2 (the rotation sampler
+// evaluates them at half the rotation angle, in [0, pi/2]): Taylor series in
+// Horner form, 20 terms, in double-double, rounded once.
+HMC_HD_COLD double hmc_sin_cr(double x) {
+  const dd x2 = dd_mul(dd{x, 0.0}, dd{x, 0.0});
+  dd p = {1.0, 0.0};
+#pragma unroll 1
+  for (int n = 20; n >= 1; --n) {
+    const dd t = dd_div(dd_mul(p, x2), dd{(double)(2 * n) * (double)(2 * n + 1), 0.0});
+    p = dd_add(dd{1.0, 0.0}, dd{-t.hi, -t.lo});
+  }
+  const dd r = dd_mul(p, dd{x, 0.0});
+  return r.hi + r.lo;
+}
+HMC_HD_COLD double hmc_cos_cr(double x) {
+  const dd x2 = dd_mul(dd{x, 0.0}, dd{x, 0.0});
+  dd p = {1.0, 0.0};
+#pragma unroll 1
+  for (int n = 20; n >= 1; --n) {
+    const dd t = dd_div(dd_mul(p, x2), dd{(double)(2 * n - 1) * (double)(2 * n), 0.0});
+    p = dd_add(dd{1.0, 0.0}, dd{-t.hi, -t.lo});
+  }
+  return p.hi + p.lo;
 }
 
 // lhs < exp(y), decided as if exp were correctly rounded.  *ambiguous counts

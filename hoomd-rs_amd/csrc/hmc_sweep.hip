@@ -32,11 +32,13 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
 #include "../../include/hmc.h"
 #include "dev_math.hpp"
+#include "dev_convex.hpp"
 #include "dev_rng.hpp"
 
 namespace hmc {
@@ -45,6 +47,7 @@ constexpr int kBlock = 256;
 constexpr int kScanBlock = 1024;  // elements per scan block (256 threads x 4)
 constexpr int kGuardThreads = 1024;
 constexpr int kAccSlots = 1024;
+constexpr int kMaxVertices = 32;  // ConvexPolyhedron = ConvexPolytope<3, 32>
 
 // Constant description of one system, passed by value to the kernels.
 struct Geometry {
@@ -73,6 +76,10 @@ struct Geometry {
   double lj_rc2;           // r_cut^2 (isotropic.rs:151)
   double lj_rc2_edge;      // r_cut^2 (1 - 2^-30): at or above it the spatial-structure rule is applied
   double lj_shift;         // E(r_shift) for the shifted potential, else 0
+  const double* verts;     // convex polyhedron: vertices (x, y, z triples), device memory
+  uint32_t nverts;
+  double cv_rcut2;         // (R + R)^2, R the circumsphere radius (convex.rs:163-166)
+  double cv_edge2;         // cv_rcut2 (1 - 2^-30): at or above it the spatial-structure rule is applied
 };
 
 // Device-resident per-apply state.
@@ -86,7 +93,7 @@ struct ApplyState {
   uint32_t cycles;
   uint32_t nonempty;
   uint32_t error;
-  uint32_t pad;
+  uint32_t move;  // 0 Translate, 1 Rotate<Versor>
   unsigned long long accepted_total;  // filled on the host from the slot counters
   unsigned long long trials_total;
   unsigned long long cycles_total;
@@ -101,12 +108,15 @@ struct ApplyState {
 struct Bodies {
   double4* bpos;
   double4* spos;
+  double4* bquat;  // oriented bodies: (s, x, y, z); null otherwise
+  double4* squat;
 };
 
 struct Cells {
   uint32_t* body;
   float4* pos32;
   double4* pos64;
+  double4* quat64;  // site orientation copy of oriented bodies
 };
 
 struct Work {
@@ -206,8 +216,9 @@ __device__ __forceinline__ void lane_add(unsigned long long* dst, unsigned v) {
 
 // ----------------------------------------------------------------- kernels --
 __global__ void k_setup(Geometry g, ApplyState* st, uint64_t step, uint32_t substep, uint32_t seed, double kT,
-                        double max_move, int from_args, int advance) {
+                        double max_move, int from_args, int advance, uint32_t move) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  st->move = move;
   if (from_args) {
     st->step = step;
     st->substep0 = substep;
@@ -296,7 +307,8 @@ __global__ void __launch_bounds__(256) k_scan_blocks(const uint32_t* cnt, uint32
 // Single workgroup: exclusive scan of the block sums (up to 1024 * 16 blocks),
 // start[n] = N, and the colour-cycle count ceil(N / non-empty spaces).
 __global__ void __launch_bounds__(1024) k_scan_top(Geometry g, ApplyState* st, uint32_t* block_sums,
-                                                  const uint32_t* block_ne, uint32_t nblocks, uint32_t* start) {
+                                                  const uint32_t* block_ne, uint32_t nblocks, uint32_t* start,
+                                                  int count) {
   __shared__ uint32_t lds[1024];
   const int per = 16;
   uint32_t base = threadIdx.x * per;
@@ -319,6 +331,7 @@ __global__ void __launch_bounds__(1024) k_scan_top(Geometry g, ApplyState* st, u
   if (threadIdx.x == 0) {
     start[g.n_spaces] = total;
     st->nonempty = ne_total;
+    if (!count) return;  // a rebuild for an energy evaluation: no trials follow
     // ParallelSweep::apply loops colour cycles until count.total() >= N
     // (parallel_sweep.rs:402); each cycle counts one trial per non-empty space.
     const uint32_t cycles = ne_total ? (g.n_bodies + ne_total - 1) / ne_total : 0;
@@ -351,6 +364,7 @@ __global__ void __launch_bounds__(kBlock) k_place(Geometry g, Bodies B, Cells C,
   for (uint32_t q = s0; q < s1; ++q) rank += w.tmp_body[q] < b ? 1u : 0u;
   const uint32_t dst = s0 + rank;
   C.body[dst] = b;
+  if (C.quat64) C.quat64[dst] = B.squat[b];
   const double4 sp = B.spos[b];
   if (PREC == HMC_MIXED)
     C.pos32[dst] = site32_of(sp);
@@ -1439,6 +1453,549 @@ __global__ void __launch_bounds__(kGuardThreads) k_guard_lj(Geometry g, ApplySta
   }
 }
 
+// ------------------------------------------------------ convex polyhedra --
+// Image of site sj nearest q per axis (the one the reference's neighbour
+// query can find within 2 R <= L/2), its squared distance
+// distance_squared(q, image) (cartesian.rs:339-342) and, within 2^-30 of the
+// circumsphere cutoff, the VecCell visitation rule as for hard spheres.
+__device__ __noinline__ double convex_edge_image(const Geometry& g, const double* q, double4 sj, v3* rj) {
+  const uint32_t fl = (uint32_t)__double_as_longlong(sj.w);
+  const double xj[3] = {sj.x, sj.y, sj.z};
+  double img[3] = {0.0, 0.0, 0.0};
+  for (int a = 0; a < 3; ++a) {
+    const double ct = floor(q[a] / g.vc_width);
+    bool any = false;
+    double best = 0.0, bx = 0.0;
+    if (fabs(floor(xj[a] / g.vc_width) - ct) <= g.vc_k) {
+      best = q[a] - xj[a];
+      bx = xj[a];
+      any = true;
+    }
+    const uint32_t f = (fl >> (2 * a)) & 3u;
+    if (f) {
+      const double gx = xj[a] + (f == 1u ? g.L[a] : -g.L[a]);
+      if (fabs(floor(gx / g.vc_width) - ct) <= g.vc_k) {
+        const double dg = q[a] - gx;
+        if (!any || fabs(dg) < fabs(best)) {
+          best = dg;
+          bx = gx;
+        }
+        any = true;
+      }
+    }
+    if (!any) return __longlong_as_double(0x7ff0000000000000ll);
+    img[a] = bx;
+  }
+  *rj = {img[0], img[1], img[2]};
+  double d2 = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const double d = q[a] - img[a];
+    d2 = d2 + d * d;
+  }
+  return d2;
+}
+
+__device__ __forceinline__ double convex_image(const Geometry& g, const double* q, double4 sj, v3* rj) {
+  const uint32_t fl = (uint32_t)__double_as_longlong(sj.w);
+  const double xj[3] = {sj.x, sj.y, sj.z};
+  double img[3] = {xj[0], xj[1], xj[2]};
+  for (int a = 0; a < 3; ++a) {
+    const uint32_t f = (fl >> (2 * a)) & 3u;
+    if (f) {
+      const double gx = xj[a] + (f == 1u ? g.L[a] : -g.L[a]);
+      if (fabs(q[a] - gx) < fabs(q[a] - xj[a])) img[a] = gx;
+    }
+  }
+  double d2 = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const double d = q[a] - img[a];
+    d2 = d2 + d * d;
+  }
+  *rj = {img[0], img[1], img[2]};
+  if (d2 >= g.cv_edge2 && d2 < g.cv_rcut2) d2 = convex_edge_image(g, q, sj, rj);
+  return d2;
+}
+
+// One trial on a hard convex polyhedron (parallel_sweep.rs:234-277) with
+// Translate (move 0) or Rotate<Versor> (move 1; rotate/versor.rs:111-122:
+// orientation = delta * old, position wrapped again).  One lane per trial.
+__device__ void trial_convex(const Geometry& g, const ApplyState* st, const Bodies& B, const Cells& C,
+                             const Work& w, const double* zx, const double* zf, const double* vx, uint32_t s,
+                             const int32_t* m, uint32_t s0, uint32_t n, uint32_t colour, uint32_t substep,
+                             TrialOut& o) {
+  Sfc64 rng = counter_rng(st->step, substep, st->seed, s);
+  const uint32_t pick = choose_index(rng, n);
+  const uint32_t slot = s0 + pick;
+  const uint32_t body = C.body[slot];
+  const double4 bp = B.bpos[body];
+  const double4 bq = B.bquat[body];
+  double x[3] = {bp.x, bp.y, bp.z};
+  quat q = {bq.x, {bq.y, bq.z, bq.w}};
+  unsigned exp_slow = 0;
+  if (st->move == 0) {
+    double v[3] = {0.0, 0.0, 0.0};
+    ball_sample<3>(rng, st->max_move, zx, zf, v, &exp_slow);
+    for (int a = 0; a < 3; ++a) x[a] += v[a];
+  } else {
+    q = qmul(versor_displacement(rng, st->max_move, zx, zf, &exp_slow), q);
+  }
+  o.exp_slow = exp_slow;
+  for (int a = 0; a < 3; ++a) x[a] = wrap1(x[a], g.L[a]);  // wrap(propose(..))
+  bool accept = false;
+  if (space_of<3>(g, st->origin, x) == s) {
+    // the trial site: wrap(x + rotate(q, 0)) with orientation q * identity
+    // (property/oriented_point.rs:107-113)
+    const v3 r0 = qrotate(q, v3{0.0, 0.0, 0.0});
+    const double t[3] = {wrap1(x[0] + r0.x, g.L[0]), wrap1(x[1] + r0.y, g.L[1]), wrap1(x[2] + r0.z, g.L[2])};
+    const quat qt = qmul(q, quat{1.0, {0.0, 0.0, 0.0}});
+    const v3 tv = {t[0], t[1], t[2]};
+    uint32_t nb[3][5];
+    int nn[3] = {1, 1, 1};
+    for (int a = 0; a < 3; ++a) {
+      const int32_t sh = g.shape[a], r = g.reach[a];
+      int k = 0;
+      for (int off = -r; off <= r; ++off) {
+        const int32_t mm = ((m[a] + off) % sh + sh) % sh;
+        bool seen = false;
+        for (int j = 0; j < k; ++j) seen |= nb[a][j] == (uint32_t)mm * g.stride[a];
+        if (!seen) nb[a][k++] = (uint32_t)mm * g.stride[a];
+      }
+      nn[a] = k;
+    }
+    bool hit = false;
+    for (int i0 = 0; i0 < nn[0] && !hit; ++i0)
+      for (int i1 = 0; i1 < nn[1] && !hit; ++i1)
+        for (int i2 = 0; i2 < nn[2] && !hit; ++i2) {
+          const uint32_t ns = nb[0][i0] + nb[1][i1] + nb[2][i2];
+          const uint32_t q0 = w.start[ns], q1 = w.start[ns + 1];
+          for (uint32_t qs = q0; qs < q1 && !hit; ++qs) {
+            if (qs == slot) continue;
+            v3 rj;
+            const double d2 = convex_image(g, t, C.pos64[qs], &rj);
+            if (d2 < g.cv_rcut2) {
+              const double4 qj4 = C.quat64[qs];
+              hit = convex_overlap(vx, g.nverts, g.cv_rcut2, d2, tv, qt, rj, quat{qj4.x, {qj4.y, qj4.z, qj4.w}});
+            }
+          }
+        }
+    accept = !hit;  // hard shapes: dH is 0 or +inf (parallel_sweep.rs:267-273)
+    if (accept) {
+      // update_body_properties (microstate.rs:985-1034): body = wrap(props),
+      // site = wrap(body.transform(site))
+      double nb3[3], ns3[3];
+      for (int a = 0; a < 3; ++a) nb3[a] = wrap1(x[a], g.L[a]);
+      const double r0a[3] = {r0.x, r0.y, r0.z};
+      for (int a = 0; a < 3; ++a) ns3[a] = wrap1(nb3[a] + r0a[a], g.L[a]);
+      const uint32_t fl = ghost_flags<3>(g, ns3);
+      B.bpos[body] = make_double4(nb3[0], nb3[1], nb3[2], 0.0);
+      B.bquat[body] = make_double4(q.s, q.v.x, q.v.y, q.v.z);
+      const double4 sp = pack_site(ns3, fl);
+      const double4 sq = make_double4(qt.s, qt.v.x, qt.v.y, qt.v.z);
+      B.spos[body] = sp;
+      B.squat[body] = sq;
+      C.pos64[slot] = sp;
+      C.quat64[slot] = sq;
+    }
+  }
+  o.accepted = accept ? 1u : 0u;
+  if (w.tracing) {
+    unsigned long long ti = atomicAdd((unsigned long long*)&st->trace_n, 1ull);
+    if (ti < w.trace_cap) {
+      hmc_trace_record rec;
+      rec.substep = substep;
+      rec.color = colour;
+      rec.space = s;
+      rec.body = (int64_t)body;
+      rec.status = accept ? 1 : 0;
+      rec.pad = 0;
+      w.trace[ti] = rec;
+    }
+  }
+}
+
+__device__ __forceinline__ void stage_verts(const Geometry& g, double* vx) {
+  for (uint32_t i = threadIdx.x; i < 3 * g.nverts; i += blockDim.x) vx[i] = g.verts[i];
+}
+
+// One colour of one cycle for convex polyhedra: one lane per space.
+__global__ void __launch_bounds__(kBlock) k_pass_convex(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
+                                                        uint32_t colour, uint32_t cycle, uint32_t xcd_chunk) {
+  if (cycle >= st->cycles) return;  // uniform over the grid
+  __shared__ unsigned blk_acc;
+  __shared__ double zx[257], zf[257];
+  __shared__ double vx[3 * kMaxVertices];
+  if (threadIdx.x == 0) blk_acc = 0;
+  stage_zig(zx, zf);
+  stage_verts(g, vx);
+  __syncthreads();
+  const uint32_t b = blockIdx.x;
+  const uint32_t lb = (b & 7u) * xcd_chunk + (b >> 3);
+  const uint32_t t = lb * kBlock + threadIdx.x;
+  TrialOut o{0, 0, 0};
+  if (t < g.per_colour) {
+    int32_t m[3] = {0, 0, 0};
+    const uint32_t s = colour_space<3>(g, colour, t, m);
+    const uint32_t s0 = w.start[s];
+    const uint32_t n = w.start[s + 1] - s0;
+    if (n) trial_convex(g, st, B, C, w, zx, zf, vx, s, m, s0, n, colour, st->substep0 + cycle, o);
+  }
+  const unsigned long long mk = __ballot(o.accepted != 0);
+  if (__lane_id() == 0 && mk) atomicAdd(&blk_acc, (unsigned)__popcll(mk));
+  lane_add(&st->exp_slow, o.exp_slow);
+  __syncthreads();
+  if (threadIdx.x == 0 && blk_acc) atomicAdd(&w.acc_slots[b & (kAccSlots - 1)], (unsigned long long)blk_acc);
+}
+
+// Convex cycles beyond the ones the host launched, in one workgroup.
+__global__ void __launch_bounds__(kGuardThreads) k_guard_convex(Geometry g, ApplyState* st, Bodies B, Cells C,
+                                                                Work w, uint32_t first_cycle) {
+  const uint32_t cycles = st->cycles;
+  if (first_cycle >= cycles) return;
+  __shared__ double zx[257], zf[257];
+  __shared__ double vx[3 * kMaxVertices];
+  stage_zig(zx, zf);
+  stage_verts(g, vx);
+  __syncthreads();
+  const uint32_t rounds = (g.per_colour + kGuardThreads - 1) / kGuardThreads;
+  for (uint32_t k = first_cycle; k < cycles; ++k) {
+    for (uint32_t c = 0; c < 8; ++c) {
+      for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t t = r * kGuardThreads + threadIdx.x;
+        TrialOut o{0, 0, 0};
+        if (t < g.per_colour) {
+          int32_t m[3] = {0, 0, 0};
+          const uint32_t s = colour_space<3>(g, c, t, m);
+          const uint32_t s0 = w.start[s];
+          const uint32_t n = w.start[s + 1] - s0;
+          if (n) trial_convex(g, st, B, C, w, zx, zf, vx, s, m, s0, n, c, st->substep0 + k, o);
+        }
+        wave_count(&w.acc_slots[0], o.accepted != 0);
+        lane_add(&st->exp_slow, o.exp_slow);
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) st->guard_cycles += 1;
+  }
+}
+
+// Site orientations: given, or body * identity (oriented_point.rs:107-113).
+__global__ void k_init_site_quats(Geometry g, Bodies B, int have_sites) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= g.n_bodies || have_sites) return;
+  const double4 b = B.bquat[e];
+  const quat qs = qmul(quat{b.x, {b.y, b.z, b.w}}, quat{1.0, {0.0, 0.0, 0.0}});
+  B.squat[e] = make_double4(qs.s, qs.v.x, qs.v.y, qs.v.z);
+}
+
+// Test hook: collide3d (kind 0, in = v_ij, q_ij: 7 values), the global pair
+// test (kind 1, in = r_i, q_i, r_j, q_j: 14 values) or the rotation sampler
+// (kind 2, in = step, substep, seed, index, sigma: out 4 values).
+__global__ void k_debug_convex(int kind, const double* vx, uint32_t nv, double rcut2, const double* in,
+                               double* out, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (kind == 0) {
+    const double* p = in + 7 * i;
+    out[i] = collide3d(vx, nv, v3{p[0], p[1], p[2]}, quat{p[3], {p[4], p[5], p[6]}}) ? 1.0 : 0.0;
+  } else if (kind == 1) {
+    const double* p = in + 14 * i;
+    const v3 ri = {p[0], p[1], p[2]}, rj = {p[7], p[8], p[9]};
+    double d2 = 0.0;
+    const double dd[3] = {p[0] - p[7], p[1] - p[8], p[2] - p[9]};
+    for (int a = 0; a < 3; ++a) d2 = d2 + dd[a] * dd[a];
+    out[i] = convex_overlap(vx, nv, rcut2, d2, ri, quat{p[3], {p[4], p[5], p[6]}}, rj,
+                            quat{p[10], {p[11], p[12], p[13]}})
+                 ? 1.0
+                 : 0.0;
+  } else {
+    const double* p = in + 5 * i;
+    Sfc64 rng = counter_rng((uint64_t)p[0], (uint32_t)p[1], (uint32_t)p[2], (uint64_t)p[3]);
+    unsigned slow = 0;
+    const quat q = versor_displacement(rng, p[4], kZigX, kZigF, &slow);
+    out[4 * i] = q.s;
+    out[4 * i + 1] = q.v.x;
+    out[4 * i + 2] = q.v.y;
+    out[4 * i + 3] = q.v.z;
+  }
+}
+
+
+// --------------------------------------------------------- replica batches --
+// Many independent systems (each with its own box, checkerboard, counter and
+// seed) advanced by the same launches: bodies, spaces and one colour's spaces
+// of all replicas are concatenated, and every thread finds its replica from
+// the running offsets.  Each replica's trials are exactly those of a context
+// of its own (same spaces, same streams), so the replay bar carries over.
+struct BatchRep {
+  Geometry g;
+  uint32_t body_off, space_off, colour_off, pad;
+};
+
+// the last r with off[r] <= x (off[0] = 0, R entries)
+__device__ __forceinline__ uint32_t find_replica(const uint32_t* off, uint32_t R, uint32_t x) {
+  uint32_t lo = 0, hi = R;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= x)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__global__ void k_bsetup(const BatchRep* reps, ApplyState* sts, uint32_t R, const uint64_t* step,
+                         const uint32_t* seed, double kT, const double* max_move, int from_args, int advance) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  const Geometry& g = reps[r].g;
+  ApplyState* st = sts + r;
+  st->move = 0;
+  if (from_args) {
+    st->step = step[r];
+    st->substep0 = 0;
+    st->seed = seed[r];
+  } else if (advance) {
+    st->step += 1;  // Microstate::increment_step (microstate.rs:412-415)
+    st->substep0 = 0;
+  }
+  st->kT = kT;
+  st->max_move = max_move[r];
+  Sfc64 rng = counter_rng(st->step, st->substep0, st->seed, UINT64_MAX);
+  for (int a = 0; a < g.dim; ++a) {
+    const double u = uniform01(rng);
+    st->origin[a] = -g.L[a] / 2.0 - u * g.width[a];
+  }
+  st->cycles = 0;
+  st->nonempty = 0;
+}
+
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_bbin(const BatchRep* reps, ApplyState* sts, const uint32_t* boff,
+                                                 uint32_t R, uint32_t n_total, Bodies B, Work w) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n_total) return;
+  const uint32_t r = find_replica(boff, R, b);
+  const double4 p4 = B.bpos[b];
+  const double x[3] = {p4.x, p4.y, p4.z};
+  uint32_t s = space_of<D>(reps[r].g, sts[r].origin, x);
+  if (s == UINT32_MAX) {
+    atomicOr(&sts[r].error, 1u);
+    s = 0;
+  }
+  const uint32_t gs = reps[r].space_off + s;
+  w.elem_space[b] = gs;
+  w.elem_rank[b] = atomicAdd(&w.cnt[gs], 1u);
+}
+
+// Non-empty spaces and colour cycles of each replica (one workgroup per replica).
+__global__ void __launch_bounds__(256) k_bnonempty(const BatchRep* reps, ApplyState* sts, const uint32_t* cnt) {
+  __shared__ uint32_t red[256];
+  const uint32_t r = blockIdx.x;
+  const Geometry& g = reps[r].g;
+  const uint32_t s0 = reps[r].space_off;
+  uint32_t ne = 0;
+  for (uint32_t s = threadIdx.x; s < g.n_spaces; s += 256) ne += cnt[s0 + s] ? 1u : 0u;
+  red[threadIdx.x] = ne;
+  __syncthreads();
+  for (int h = 128; h >= 1; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    ApplyState* st = sts + r;
+    const uint32_t ne_total = red[0];
+    st->nonempty = ne_total;
+    const uint32_t cycles = ne_total ? (g.n_bodies + ne_total - 1) / ne_total : 0;
+    st->cycles = cycles;
+    st->cycles_total += cycles;
+    st->trials_total += (unsigned long long)cycles * ne_total;
+  }
+}
+
+// One colour of one cycle of every replica: one lane per space.
+template <int D, int PREC, bool FAST>
+__global__ void __launch_bounds__(kBlock) k_bpass(const BatchRep* reps, ApplyState* sts, const uint32_t* coff,
+                                                  uint32_t R, uint32_t per_colour_total, Bodies B, Cells C, Work w,
+                                                  uint32_t colour, uint32_t cycle, uint32_t xcd_chunk) {
+  __shared__ double zx[257], zf[257];
+  if (D == 3) stage_zig(zx, zf);
+  __syncthreads();
+  const uint32_t b = blockIdx.x;
+  const uint32_t lb = (b & 7u) * xcd_chunk + (b >> 3);
+  const uint32_t t = lb * kBlock + threadIdx.x;
+  if (t >= per_colour_total) return;
+  const uint32_t r = find_replica(coff, R, t);
+  ApplyState* st = sts + r;
+  if (cycle >= st->cycles) return;
+  const BatchRep& rep = reps[r];
+  const Geometry g = rep.g;
+  int32_t m[3] = {0, 0, 0};
+  const uint32_t s = colour_space<D>(g, colour, t - rep.colour_off, m);
+  Work wr = w;
+  wr.start = w.start + rep.space_off;
+  const uint32_t s0 = wr.start[s];
+  const uint32_t n = wr.start[s + 1] - s0;
+  if (!n) return;
+  TrialOut o{0, 0, 0};
+  trial<D, PREC, FAST>(g, st, B, C, wr, zx, zf, s, m, s0, n, colour, st->substep0 + cycle, o);
+  if (o.accepted) atomicAdd(&w.acc_slots[r], 1ull);
+  lane_add(&st->band_retests, o.retests);
+  lane_add(&st->exp_slow, o.exp_slow);
+}
+
+// Cycles of one replica beyond the ones launched (one workgroup per replica).
+template <int D, int PREC>
+__global__ void __launch_bounds__(kGuardThreads) k_bguard(const BatchRep* reps, ApplyState* sts, Bodies B, Cells C,
+                                                          Work w, uint32_t first_cycle) {
+  const uint32_t r = blockIdx.x;
+  ApplyState* st = sts + r;
+  const uint32_t cycles = st->cycles;
+  if (first_cycle >= cycles) return;
+  __shared__ double zx[257], zf[257];
+  if (D == 3) stage_zig(zx, zf);
+  __syncthreads();
+  const Geometry g = reps[r].g;
+  Work wr = w;
+  wr.start = w.start + reps[r].space_off;
+  const uint32_t rounds = (g.per_colour + kGuardThreads - 1) / kGuardThreads;
+  for (uint32_t k = first_cycle; k < cycles; ++k) {
+    for (int c = 0; c < (1 << D); ++c) {
+      for (uint32_t q = 0; q < rounds; ++q) {
+        const uint32_t t = q * kGuardThreads + threadIdx.x;
+        TrialOut o{0, 0, 0};
+        if (t < g.per_colour) {
+          int32_t m[3] = {0, 0, 0};
+          const uint32_t s = colour_space<D>(g, (uint32_t)c, t, m);
+          const uint32_t s0 = wr.start[s];
+          const uint32_t n = wr.start[s + 1] - s0;
+          if (n) trial<D, PREC, false>(g, st, B, C, wr, zx, zf, s, m, s0, n, (uint32_t)c, st->substep0 + k, o);
+        }
+        wave_count(&w.acc_slots[r], o.accepted != 0);
+        lane_add(&st->band_retests, o.retests);
+        lane_add(&st->exp_slow, o.exp_slow);
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) st->guard_cycles += 1;
+  }
+}
+
+// ------------------------------------------------------------- total energy --
+// PairwiseCutoff::total_energy (pairwise_cutoff.rs:720-745): the sum over
+// sites i and neighbours j with site_tag_i < site_tag_j (body index = site
+// tag for single-site bodies) of the pair energy, infinite on any hard
+// overlap.  Per block: the partial sum, the sum of |terms|, the number of
+// non-zero terms and an overlap flag, reduced in a fixed order.
+constexpr int kEnergyBlock = 256;
+
+template <int D>
+__global__ void __launch_bounds__(kEnergyBlock) k_energy(Geometry g, const ApplyState* st, Bodies B, Work w,
+                                                         double* part) {
+  __shared__ double vx[3 * kMaxVertices];
+  __shared__ double re[kEnergyBlock], ra[kEnergyBlock], rn[kEnergyBlock];
+  __shared__ int any_inf;
+  if (threadIdx.x == 0) any_inf = 0;
+  if (g.model == HMC_CONVEX_POLYHEDRON) stage_verts(g, vx);
+  __syncthreads();
+  const uint32_t i = blockIdx.x * kEnergyBlock + threadIdx.x;
+  double e = 0.0, a = 0.0, cnt = 0.0;
+  bool inf = false;
+  if (i < g.n_bodies) {
+    const double4 si = B.spos[i];
+    const double t[3] = {si.x, si.y, si.z};
+    const double4 bp = B.bpos[i];
+    const double x[3] = {bp.x, bp.y, bp.z};
+    uint32_t s = space_of<D>(g, st->origin, x);
+    if (s == UINT32_MAX) s = 0;  // outside the checkerboard: reported as an error by the host
+    int32_t m[3] = {0, 0, 0};
+    for (int ax = 0; ax < D; ++ax) m[ax] = (int32_t)((s / g.stride[ax]) % (uint32_t)g.shape[ax]);
+    uint32_t nb[3][5];
+    int nn[3] = {1, 1, 1};
+    for (int ax = 0; ax < D; ++ax) {
+      const int32_t sh = g.shape[ax], r = g.reach[ax];
+      int k = 0;
+      for (int off = -r; off <= r; ++off) {
+        const int32_t mm = ((m[ax] + off) % sh + sh) % sh;
+        bool seen = false;
+        for (int j = 0; j < k; ++j) seen |= nb[ax][j] == (uint32_t)mm * g.stride[ax];
+        if (!seen) nb[ax][k++] = (uint32_t)mm * g.stride[ax];
+      }
+      nn[ax] = k;
+    }
+    quat qi = {1.0, {0.0, 0.0, 0.0}};
+    if (g.model == HMC_CONVEX_POLYHEDRON) {
+      const double4 q4 = B.squat[i];
+      qi = {q4.x, {q4.y, q4.z, q4.w}};
+    }
+    for (int i0 = 0; i0 < nn[0] && !inf; ++i0)
+      for (int i1 = 0; i1 < nn[1] && !inf; ++i1)
+        for (int i2 = 0; i2 < (D == 3 ? nn[2] : 1) && !inf; ++i2) {
+          const uint32_t ns = nb[0][i0] + nb[1][i1] + (D == 3 ? nb[2][i2] : 0u);
+          const uint32_t q0 = w.start[ns], q1 = w.start[ns + 1];
+          for (uint32_t q = q0; q < q1 && !inf; ++q) {
+            const uint32_t j = w.tmp_body[q];
+            if (j <= i) continue;
+            const double4 sj = B.spos[j];
+            if (g.model == HMC_HARD_SPHERE) {
+              inf = overlap64<D>(g, t, sj);
+            } else if (g.model == HMC_CONVEX_POLYHEDRON) {
+              v3 rj;
+              const double d2 = convex_image(g, t, sj, &rj);
+              if (d2 < g.cv_rcut2) {
+                const double4 q4 = B.squat[j];
+                inf = convex_overlap(vx, g.nverts, g.cv_rcut2, d2, v3{t[0], t[1], t[2]}, qi, rj,
+                                     quat{q4.x, {q4.y, q4.z, q4.w}});
+              }
+            } else {
+              const double r2 = lj_r2<D>(g, t, sj);
+              if (r2 < g.lj_rc2) {
+                const double one = lj_energy(g, r2);
+                e += one;
+                a += fabs(one);
+                cnt += 1.0;
+              }
+            }
+          }
+        }
+  }
+  if (inf) any_inf = 1;
+  re[threadIdx.x] = e;
+  ra[threadIdx.x] = a;
+  rn[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int h = kEnergyBlock / 2; h >= 1; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      re[threadIdx.x] += re[threadIdx.x + h];
+      ra[threadIdx.x] += ra[threadIdx.x + h];
+      rn[threadIdx.x] += rn[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[4 * blockIdx.x] = re[0];
+    part[4 * blockIdx.x + 1] = ra[0];
+    part[4 * blockIdx.x + 2] = rn[0];
+    part[4 * blockIdx.x + 3] = any_inf ? 1.0 : 0.0;
+  }
+}
+
+// Sum of the block partials in block order (one thread; the partial count is small).
+__global__ void k_energy_final(const double* part, uint32_t nblocks, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double e = 0.0, a = 0.0, n = 0.0, inf = 0.0;
+  for (uint32_t b = 0; b < nblocks; ++b) {
+    e += part[4 * b];
+    a += part[4 * b + 1];
+    n += part[4 * b + 2];
+    inf = fmax(inf, part[4 * b + 3]);
+  }
+  out[0] = e;
+  out[1] = a;
+  out[2] = n;
+  out[3] = inf;
+}
+
 // Site positions (given, or wrap(body + 0)) and their ghost flags.
 template <int D>
 __global__ void k_init_sites(Geometry g, Bodies B, int have_sites) {
@@ -1468,6 +2025,8 @@ __global__ void k_debug_math(int kind, const double* in, double* out, uint64_t n
     case 1: r = hmc_log_cr(in[i]); break;
     case 2: r = hmc_exp_cr(in[i]); break;
     case 3: r = sqrt(in[i]); break;
+    case 5: r = hmc_sin_cr(in[i]); break;
+    case 6: r = hmc_cos_cr(in[i]); break;
     default: r = in[2 * i] / in[2 * i + 1]; break;
   }
   out[i] = r;
@@ -1500,6 +2059,7 @@ struct hmc_ctx {
   bool fast = false;  // every axis: >= 3 spaces and reach 1
   bool tiled = false;  // LDS-staged pass (requires fast)
   TileCfg tc{};
+  double* verts = nullptr;  // convex polyhedron vertices (device)
   uint32_t n_scan_blocks = 0;
   uint64_t applies = 0;
   std::string err;
@@ -1544,7 +2104,7 @@ hipEvent_t take_event(hmc_ctx* c) {
 
 template <int D, int PREC>
 int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t substep, uint32_t seed, double kT,
-                 double max_move) {
+                 double max_move, uint32_t move) {
   const Geometry& g = c->g;
   hipStream_t s = c->stream;
   const uint32_t N = g.n_bodies;
@@ -1554,11 +2114,11 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
     e0 = take_event(c);
     (void)hipEventRecord(e0, s);
   }
-  k_setup<<<1, 64, 0, s>>>(g, c->st, step, substep, seed, kT, max_move, from_args, advance);
+  k_setup<<<1, 64, 0, s>>>(g, c->st, step, substep, seed, kT, max_move, from_args, advance, move);
   k_fill_u32<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.cnt, g.n_spaces, 0u);
   if (N) k_bin<D><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
   k_scan_blocks<<<c->n_scan_blocks, 256, 0, s>>>(c->w.cnt, c->w.start, c->w.block_sums, c->w.block_ne, g.n_spaces);
-  k_scan_top<<<1, 1024, 0, s>>>(g, c->st, c->w.block_sums, c->w.block_ne, c->n_scan_blocks, c->w.start);
+  k_scan_top<<<1, 1024, 0, s>>>(g, c->st, c->w.block_sums, c->w.block_ne, c->n_scan_blocks, c->w.start, 1);
   k_scan_add<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.start, c->w.block_sums, g.n_spaces);
   if (N) {
     k_scatter<<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->w);
@@ -1573,7 +2133,8 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
   const uint32_t chunk = (blocks + 7) / 8;
   const uint32_t grid = chunk * 8;
   const uint32_t tile_threads = ((c->tc.trials + 63) / 64) * 64;
-  const bool lj = g.model != HMC_HARD_SPHERE;
+  const bool lj = g.model == HMC_LJ || g.model == HMC_LJ_SHIFTED;
+  const bool convex = g.model == HMC_CONVEX_POLYHEDRON;
   // LJ needs many cycles per apply and its guard is slow: launch one spare
   // cycle (its passes exit at once when not needed)
   const uint32_t launched = c->cycles_guess + (lj ? 1u : 0u);
@@ -1588,6 +2149,8 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
       }
       if (lj)
         k_pass_lj<D><<<lj_chunk * 8, kBlock, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k, lj_chunk);
+      else if (convex)
+        k_pass_convex<<<grid, kBlock, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k, chunk);
       else if (c->tiled)
         k_pass_tile<D, PREC><<<c->tc.xcd_chunk * 8, tile_threads, c->tc.lds_bytes, s>>>(
             g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k, c->tc);
@@ -1604,6 +2167,8 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
   }
   if (lj)
     k_guard_lj<D><<<1, kGuardThreads, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, launched);
+  else if (convex)
+    k_guard_convex<<<1, kGuardThreads, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, launched);
   else
     k_guard<D, PREC><<<1, kGuardThreads, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, launched);
   hipError_t e = hipGetLastError();
@@ -1613,12 +2178,13 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
 }
 
 int dispatch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t substep, uint32_t seed,
-                   double kT, double max_move) {
+                   double kT, double max_move, uint32_t move) {
   int d = c->g.dim, p = c->cfg.precision;
-  if (d == 2 && p == HMC_FP64) return launch_apply<2, HMC_FP64>(c, from_args, advance, step, substep, seed, kT, max_move);
-  if (d == 2) return launch_apply<2, HMC_MIXED>(c, from_args, advance, step, substep, seed, kT, max_move);
-  if (p == HMC_FP64) return launch_apply<3, HMC_FP64>(c, from_args, advance, step, substep, seed, kT, max_move);
-  return launch_apply<3, HMC_MIXED>(c, from_args, advance, step, substep, seed, kT, max_move);
+  if (d == 2 && p == HMC_FP64)
+    return launch_apply<2, HMC_FP64>(c, from_args, advance, step, substep, seed, kT, max_move, move);
+  if (d == 2) return launch_apply<2, HMC_MIXED>(c, from_args, advance, step, substep, seed, kT, max_move, move);
+  if (p == HMC_FP64) return launch_apply<3, HMC_FP64>(c, from_args, advance, step, substep, seed, kT, max_move, move);
+  return launch_apply<3, HMC_MIXED>(c, from_args, advance, step, substep, seed, kT, max_move, move);
 }
 
 void collect_events(hmc_ctx* c, std::vector<std::pair<hipEvent_t, hipEvent_t>>& evs, double& ms_total,
@@ -1648,6 +2214,104 @@ int read_state(hmc_ctx* c, ApplyState* out) {
     collect_events(c, c->build_events, c->build_ms, c->build_n);
   }
   if (out->error) return fail(c, HMC_ERR_COVER, "a body lies outside the checkerboard");
+  return HMC_OK;
+}
+
+}  // namespace
+
+struct hmc_batch {
+  int device = 0;
+  uint32_t R = 0;
+  int dim = 3;
+  int precision = HMC_FP64;
+  bool fast = true;
+  double kT = 1.0;
+  std::vector<BatchRep> hreps;
+  std::vector<uint32_t> boff, coff;  // R + 1 running offsets of bodies and colour spaces
+  uint32_t n_total = 0, s_total = 0, pc_total = 0;
+  Geometry gtot{};  // totals for the replica-agnostic kernels
+  BatchRep* reps = nullptr;
+  ApplyState* sts = nullptr;  // R + 1 (the last one is scratch)
+  uint32_t *dboff = nullptr, *dcoff = nullptr;
+  uint64_t* dstep = nullptr;
+  uint32_t* dseed = nullptr;
+  double* dmove = nullptr;
+  Bodies bodies{};
+  Cells cells{};
+  Work w{};
+  hipStream_t stream = nullptr;
+  uint32_t n_scan_blocks = 0;
+  uint32_t cycles_guess = 1;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint32_t graph_cycles = 0;
+  std::string err;
+};
+
+namespace {
+
+int bfail(hmc_batch* b, int code, const std::string& msg) {
+  if (b) b->err = msg;
+  return code;
+}
+
+#define BHIPCHK(b, expr)                                                                     \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) return bfail((b), HMC_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// The launches of one step of every replica (ParallelSweep::apply with
+// Translate, then increment_step when advance is set).
+template <int D, int PREC>
+void launch_batch_step(hmc_batch* b, int from_args, int advance, uint32_t launched) {
+  hipStream_t s = b->stream;
+  k_bsetup<<<grid_for(b->R, 64), 64, 0, s>>>(b->reps, b->sts, b->R, b->dstep, b->dseed, b->kT, b->dmove, from_args,
+                                             advance);
+  k_fill_u32<<<grid_for(b->s_total, 256), 256, 0, s>>>(b->w.cnt, b->s_total, 0u);
+  k_bbin<D><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->reps, b->sts, b->dboff, b->R, b->n_total, b->bodies,
+                                                            b->w);
+  k_scan_blocks<<<b->n_scan_blocks, 256, 0, s>>>(b->w.cnt, b->w.start, b->w.block_sums, b->w.block_ne, b->s_total);
+  k_scan_top<<<1, 1024, 0, s>>>(b->gtot, b->sts + b->R, b->w.block_sums, b->w.block_ne, b->n_scan_blocks, b->w.start,
+                                0);
+  k_scan_add<<<grid_for(b->s_total, 256), 256, 0, s>>>(b->w.start, b->w.block_sums, b->s_total);
+  k_bnonempty<<<b->R, 256, 0, s>>>(b->reps, b->sts, b->w.cnt);
+  k_scatter<<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->gtot, b->w);
+  k_place<PREC><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->gtot, b->bodies, b->cells, b->w);
+  const uint32_t blocks = grid_for(b->pc_total, kBlock);
+  const uint32_t chunk = (blocks + 7) / 8;
+  for (uint32_t k = 0; k < launched; ++k)
+    for (int col = 0; col < (1 << D); ++col) {
+      if (b->fast)
+        k_bpass<D, PREC, true><<<chunk * 8, kBlock, 0, s>>>(b->reps, b->sts, b->dcoff, b->R, b->pc_total, b->bodies,
+                                                            b->cells, b->w, (uint32_t)col, k, chunk);
+      else
+        k_bpass<D, PREC, false><<<chunk * 8, kBlock, 0, s>>>(b->reps, b->sts, b->dcoff, b->R, b->pc_total,
+                                                             b->bodies, b->cells, b->w, (uint32_t)col, k, chunk);
+    }
+  k_bguard<D, PREC><<<b->R, kGuardThreads, 0, s>>>(b->reps, b->sts, b->bodies, b->cells, b->w, launched);
+}
+
+void batch_step(hmc_batch* b, int from_args, int advance, uint32_t launched) {
+  if (b->dim == 2 && b->precision == HMC_FP64)
+    launch_batch_step<2, HMC_FP64>(b, from_args, advance, launched);
+  else if (b->dim == 2)
+    launch_batch_step<2, HMC_MIXED>(b, from_args, advance, launched);
+  else if (b->precision == HMC_FP64)
+    launch_batch_step<3, HMC_FP64>(b, from_args, advance, launched);
+  else
+    launch_batch_step<3, HMC_MIXED>(b, from_args, advance, launched);
+}
+
+int batch_read(hmc_batch* b, std::vector<ApplyState>& st, std::vector<unsigned long long>& acc) {
+  st.resize(b->R);
+  acc.resize(b->R);
+  BHIPCHK(b, hipMemcpyAsync(st.data(), b->sts, b->R * sizeof(ApplyState), hipMemcpyDeviceToHost, b->stream));
+  BHIPCHK(b, hipMemcpyAsync(acc.data(), b->w.acc_slots, b->R * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                            b->stream));
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  for (uint32_t r = 0; r < b->R; ++r)
+    if (st[r].error) return bfail(b, HMC_ERR_COVER, "a body lies outside the checkerboard (replica " + std::to_string(r) + ")");
   return HMC_OK;
 }
 
@@ -1742,18 +2406,36 @@ static bool choose_tiles(const Geometry& g, int D, double occ, size_t vt, int cu
   return found;
 }
 
-int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
-  if (!cfg || !out) return HMC_ERR_INVALID;
-  *out = nullptr;
+// Geometry of one system: validation, checkerboard dimensions, model
+// constants and the FP32 contact band.  Shared by a context and by every
+// replica of a batch.
+static int make_geometry(const hmc_config* cfg, Geometry& g, bool* fast, double* sigma_out, double* lmax_out) {
   if (cfg->dim != 2 && cfg->dim != 3) return HMC_ERR_UNSUPPORTED;
   const bool lj = cfg->model == HMC_LJ || cfg->model == HMC_LJ_SHIFTED;
-  if (cfg->model != HMC_HARD_SPHERE && !lj) return HMC_ERR_UNSUPPORTED;
+  const bool convex = cfg->model == HMC_CONVEX_POLYHEDRON;
+  if (cfg->model != HMC_HARD_SPHERE && !lj && !convex) return HMC_ERR_UNSUPPORTED;
   if (cfg->precision != HMC_FP64 && cfg->precision != HMC_MIXED) return HMC_ERR_INVALID;
-  if (lj && cfg->precision != HMC_FP64) return HMC_ERR_UNSUPPORTED;  // LJ: FP64 pair energies only
+  if ((lj || convex) && cfg->precision != HMC_FP64) return HMC_ERR_UNSUPPORTED;  // FP64 only
+  if (convex && cfg->dim != 3) return HMC_ERR_UNSUPPORTED;
+  if (convex && (!cfg->vertices || cfg->n_vertices < 1 || cfg->n_vertices > (uint32_t)kMaxVertices))
+    return HMC_ERR_INVALID;
   if (cfg->n_bodies >= 0xffffffffull) return HMC_ERR_UNSUPPORTED;
   const int D = (int)cfg->dim;
-  // interaction range: the diameter (hard_shape.rs:170-174) or r_cut (isotropic.rs)
-  const double sigma = lj ? cfg->params[2] : cfg->params[0];
+  // circumsphere radius of a polytope (convex_polytope.rs:248-257): sqrt of
+  // fold(0, max) over the vertices' squared norms
+  double cv_radius = 0.0;
+  if (convex) {
+    double m2 = 0.0;
+    for (uint32_t i = 0; i < cfg->n_vertices; ++i) {
+      const double* v = cfg->vertices + 3 * i;
+      if (!std::isfinite(v[0]) || !std::isfinite(v[1]) || !std::isfinite(v[2])) return HMC_ERR_INVALID;
+      m2 = std::fmax(m2, std::fma(v[2], v[2], std::fma(v[1], v[1], std::fma(v[0], v[0], 0.0))));
+    }
+    cv_radius = std::sqrt(m2);
+  }
+  // interaction range: the diameter (hard_shape.rs:170-174), r_cut (isotropic.rs)
+  // or twice the circumsphere radius (hard_shape.rs:111-118)
+  const double sigma = lj ? cfg->params[2] : convex ? cv_radius * 2.0 : cfg->params[0];
   if (!(sigma > 0.0) || !(cfg->body_interaction_range > 0.0) || !(cfg->boundary_range >= 0.0) ||
       !(cfg->spatial_cell_width > 0.0))
     return HMC_ERR_INVALID;
@@ -1773,10 +2455,6 @@ int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
   // colour must not interact).
   if (sigma > cfg->boundary_range || sigma > cfg->body_interaction_range) return HMC_ERR_INVALID;
 
-  auto* c = new hmc_ctx;
-  c->cfg = *cfg;
-  c->device = cfg->device;
-  Geometry& g = c->g;
   std::memset(&g, 0, sizeof g);
   g.dim = D;
   g.n_bodies = (uint32_t)cfg->n_bodies;
@@ -1791,8 +2469,7 @@ int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
     // HypercuboidCheckerboard::compute_dimensions (hypercuboid.rs:188-228)
     int64_t inside = (int64_t)std::floor(L / cfg->body_interaction_range);
     if (inside < 2) {
-      delete c;
-      return HMC_ERR_INVALID;
+        return HMC_ERR_INVALID;
     }
     if (inside % 2 != 0) inside -= 1;
     g.shape[a] = (int32_t)inside;
@@ -1804,7 +2481,6 @@ int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
     spaces *= (uint64_t)inside;
   }
   if (spaces >= 0xffffffffull || spaces > (uint64_t)kScanBlock * 1024 * 16) {
-    delete c;
     return HMC_ERR_UNSUPPORTED;
   }
   uint32_t str = 1;
@@ -1823,11 +2499,16 @@ int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
   }
   g.n_spaces = (uint32_t)spaces;
   g.per_colour = g.n_spaces >> D;
-  c->fast = true;
-  for (int a = 0; a < D; ++a) c->fast = c->fast && g.reach[a] == 1 && g.shape[a] >= 3;
+  *fast = true;
+  for (int a = 0; a < D; ++a) *fast = *fast && g.reach[a] == 1 && g.shape[a] >= 3;
   g.sigma2 = sigma * sigma;
   g.edge2 = g.sigma2 * (1.0 - 0x1p-30);
   g.model = cfg->model;
+  if (convex) {
+    g.nverts = cfg->n_vertices;
+    g.cv_rcut2 = (cv_radius + cv_radius) * (cv_radius + cv_radius);
+    g.cv_edge2 = g.cv_rcut2 * (1.0 - 0x1p-30);
+  }
   if (lj) {
     // lennard_jones.rs:107-111, shifted.rs:92-94, isotropic.rs:147-156
     const double eps = cfg->params[0], lsig = cfg->params[1], rc = cfg->params[2];
@@ -1851,15 +2532,39 @@ int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
   for (int a = 0; a < D; ++a) lmax = std::max(lmax, g.L[a]);
   const double E = (0x1p-24 * (3.5 * lmax + 2.0 * sigma) + 0x1p-53 * (1.5 * lmax + 2.0 * sigma)) * 1.01;
   if (!contact_band(E, sigma, g.sigma2, g.edge2, D, &g.band_lo, &g.band_hi) && cfg->precision == HMC_MIXED) {
-    delete c;
     return HMC_ERR_UNSUPPORTED;
   }
 
+  *sigma_out = sigma;
+  *lmax_out = lmax;
+  return HMC_OK;
+}
+
+int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
+  if (!cfg || !out) return HMC_ERR_INVALID;
+  *out = nullptr;
+  Geometry g0;
+  bool fast0 = false;
+  double sigma = 0.0, lmax = 0.0;
+  {
+    const int rc = make_geometry(cfg, g0, &fast0, &sigma, &lmax);
+    if (rc) return rc;
+  }
+  const int D = (int)cfg->dim;
+  const bool lj = cfg->model == HMC_LJ || cfg->model == HMC_LJ_SHIFTED;
+  const bool convex = cfg->model == HMC_CONVEX_POLYHEDRON;
+  const uint64_t spaces = g0.n_spaces;
+  auto* c = new hmc_ctx;
+  c->cfg = *cfg;
+  c->device = cfg->device;
+  c->g = g0;
+  c->fast = fast0;
+  Geometry& g = c->g;
   if (hipSetDevice(c->device) != hipSuccess) {
     delete c;
     return HMC_ERR_DEVICE;
   }
-  if (c->fast && !lj && !(cfg->flags & HMC_FLAG_GLOBAL_PASS)) {
+  if (c->fast && !lj && !convex && !(cfg->flags & HMC_FLAG_GLOBAL_PASS)) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0) {
       delete c;
@@ -1904,6 +2609,17 @@ int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
        dalloc(&c->w.tmp_body, N) == hipSuccess && dalloc(&c->w.block_sums, c->n_scan_blocks) == hipSuccess &&
        dalloc(&c->w.block_ne, c->n_scan_blocks) == hipSuccess && dalloc(&c->st, 1) == hipSuccess &&
        dalloc(&c->w.acc_slots, kAccSlots) == hipSuccess;
+  if (ok && convex) {
+    // orientations start at the identity; vertices live on the device
+    ok = dalloc(&c->bodies.bquat, N) == hipSuccess && dalloc(&c->bodies.squat, N) == hipSuccess &&
+         dalloc(&c->cells.quat64, N) == hipSuccess && dalloc(&c->verts, 3 * (size_t)g.nverts) == hipSuccess;
+    std::vector<double4> ident(std::max<uint32_t>(N, 1), make_double4(1.0, 0.0, 0.0, 0.0));
+    ok = ok && hipMemcpy(c->bodies.bquat, ident.data(), N * sizeof(double4), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(c->bodies.squat, ident.data(), N * sizeof(double4), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(c->verts, cfg->vertices, 3 * (size_t)g.nverts * sizeof(double), hipMemcpyHostToDevice) ==
+             hipSuccess;
+    g.verts = c->verts;
+  }
   if (!ok) {
     hmc_destroy(c);
     return HMC_ERR_DEVICE;
@@ -1950,6 +2666,96 @@ int hmc_set_state(hmc_ctx* c, const double* body_pos, const double* site_pos) {
   return HMC_OK;
 }
 
+int hmc_set_orientations(hmc_ctx* c, const double* body_quat, const double* site_quat) {
+  if (!c || (!body_quat && c->g.n_bodies)) return HMC_ERR_INVALID;
+  if (c->g.model != HMC_CONVEX_POLYHEDRON) return fail(c, HMC_ERR_UNSUPPORTED, "bodies have no orientation");
+  const uint32_t N = c->g.n_bodies;
+  std::vector<double4> hb(N), hs(site_quat ? N : 0);
+  for (uint32_t i = 0; i < N; ++i) {
+    const double* q = body_quat + 4 * (size_t)i;
+    hb[i] = make_double4(q[0], q[1], q[2], q[3]);
+    if (site_quat) {
+      const double* u = site_quat + 4 * (size_t)i;
+      hs[i] = make_double4(u[0], u[1], u[2], u[3]);
+    }
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  if (N) {
+    HIPCHK(c, hipMemcpyAsync(c->bodies.bquat, hb.data(), N * sizeof(double4), hipMemcpyHostToDevice, c->stream));
+    if (site_quat)
+      HIPCHK(c, hipMemcpyAsync(c->bodies.squat, hs.data(), N * sizeof(double4), hipMemcpyHostToDevice, c->stream));
+    k_init_site_quats<<<grid_for(N, 256), 256, 0, c->stream>>>(c->g, c->bodies, site_quat ? 1 : 0);
+    HIPCHK(c, hipGetLastError());
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return HMC_OK;
+}
+
+int hmc_get_orientations(hmc_ctx* c, double* body_quat, double* site_quat) {
+  if (!c) return HMC_ERR_INVALID;
+  if (c->g.model != HMC_CONVEX_POLYHEDRON) return fail(c, HMC_ERR_UNSUPPORTED, "bodies have no orientation");
+  const uint32_t N = c->g.n_bodies;
+  std::vector<double4> hb(N), hs(N);
+  HIPCHK(c, hipSetDevice(c->device));
+  if (N) {
+    HIPCHK(c, hipMemcpyAsync(hb.data(), c->bodies.bquat, N * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(hs.data(), c->bodies.squat, N * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (uint32_t i = 0; i < N; ++i) {
+    const double4 b = hb[i], q = hs[i];
+    if (body_quat) {
+      body_quat[4 * (size_t)i] = b.x;
+      body_quat[4 * (size_t)i + 1] = b.y;
+      body_quat[4 * (size_t)i + 2] = b.z;
+      body_quat[4 * (size_t)i + 3] = b.w;
+    }
+    if (site_quat) {
+      site_quat[4 * (size_t)i] = q.x;
+      site_quat[4 * (size_t)i + 1] = q.y;
+      site_quat[4 * (size_t)i + 2] = q.z;
+      site_quat[4 * (size_t)i + 3] = q.w;
+    }
+  }
+  return HMC_OK;
+}
+
+int hmc_debug_convex(int device, int kind, const double* vertices, uint32_t n_vertices, const double* in,
+                     double* out, uint64_t n) {
+  if (!vertices || n_vertices < 1 || n_vertices > (uint32_t)kMaxVertices || (n && (!in || !out)) || kind < 0 ||
+      kind > 2)
+    return HMC_ERR_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return HMC_ERR_DEVICE;
+  const size_t w_in = kind == 0 ? 7 : kind == 1 ? 14 : 5, w_out = kind == 2 ? 4 : 1;
+  double m2 = 0.0;
+  for (uint32_t i = 0; i < n_vertices; ++i) {
+    const double* v = vertices + 3 * i;
+    m2 = std::fmax(m2, std::fma(v[2], v[2], std::fma(v[1], v[1], std::fma(v[0], v[0], 0.0))));
+  }
+  const double R = std::sqrt(m2);
+  double *dv = nullptr, *di = nullptr, *dout = nullptr;
+  int rc = HMC_OK;
+  if (hipMalloc((void**)&dv, 3 * n_vertices * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&di, std::max<uint64_t>(n, 1) * w_in * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&dout, std::max<uint64_t>(n, 1) * w_out * sizeof(double)) != hipSuccess) {
+    rc = HMC_ERR_DEVICE;
+  } else if (n) {
+    if (hipMemcpy(dv, vertices, 3 * n_vertices * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(di, in, n * w_in * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      rc = HMC_ERR_DEVICE;
+    if (!rc) {
+      k_debug_convex<<<grid_for(n, 64), 64>>>(kind, dv, n_vertices, (R + R) * (R + R), di, dout, n);
+      if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+          hipMemcpy(out, dout, n * w_out * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = HMC_ERR_DEVICE;
+    }
+  }
+  if (dv) (void)hipFree(dv);
+  if (di) (void)hipFree(di);
+  if (dout) (void)hipFree(dout);
+  return rc;
+}
+
 int hmc_get_state(hmc_ctx* c, double* body_pos, double* site_pos) {
   if (!c) return HMC_ERR_INVALID;
   if (!c->have_state) return fail(c, HMC_ERR_STATE, "no state");
@@ -1973,16 +2779,18 @@ int hmc_get_state(hmc_ctx* c, double* body_pos, double* site_pos) {
   return HMC_OK;
 }
 
-int hmc_apply_translate(hmc_ctx* c, uint64_t step, uint32_t substep, uint32_t seed, double kT,
-                        double max_distance, hmc_count* out, uint32_t* substeps_used) {
+static int apply_one(hmc_ctx* c, uint64_t step, uint32_t substep, uint32_t seed, double kT, double max_move,
+                     uint32_t move, hmc_count* out, uint32_t* substeps_used) {
   if (!c) return HMC_ERR_INVALID;
   if (!c->have_state) return fail(c, HMC_ERR_STATE, "hmc_set_state was not called");
-  if (!(max_distance > 0.0) || !(kT > 0.0)) return fail(c, HMC_ERR_INVALID, "max_distance and kT must be > 0");
+  if (!(max_move > 0.0) || !(kT > 0.0)) return fail(c, HMC_ERR_INVALID, "the move size and kT must be > 0");
+  if (move == 1 && c->g.model != HMC_CONVEX_POLYHEDRON)
+    return fail(c, HMC_ERR_UNSUPPORTED, "rotation moves need oriented bodies");
   HIPCHK(c, hipSetDevice(c->device));
   ApplyState before;
   int rc = read_state(c, &before);
   if (rc) return rc;
-  rc = dispatch_apply(c, 1, 0, step, substep, seed, kT, max_distance);
+  rc = dispatch_apply(c, 1, 0, step, substep, seed, kT, max_move, move);
   if (rc) return rc;
   ApplyState after;
   rc = read_state(c, &after);
@@ -1998,6 +2806,16 @@ int hmc_apply_translate(hmc_ctx* c, uint64_t step, uint32_t substep, uint32_t se
   return HMC_OK;
 }
 
+int hmc_apply_translate(hmc_ctx* c, uint64_t step, uint32_t substep, uint32_t seed, double kT,
+                        double max_distance, hmc_count* out, uint32_t* substeps_used) {
+  return apply_one(c, step, substep, seed, kT, max_distance, 0u, out, substeps_used);
+}
+
+int hmc_apply_rotate(hmc_ctx* c, uint64_t step, uint32_t substep, uint32_t seed, double kT, double max_rotation,
+                     hmc_count* out, uint32_t* substeps_used) {
+  return apply_one(c, step, substep, seed, kT, max_rotation, 1u, out, substeps_used);
+}
+
 int hmc_run_translate(hmc_ctx* c, uint64_t step0, uint32_t seed, double kT, double max_distance,
                       uint32_t n_steps, hmc_count* out) {
   if (!c) return HMC_ERR_INVALID;
@@ -2008,7 +2826,7 @@ int hmc_run_translate(hmc_ctx* c, uint64_t step0, uint32_t seed, double kT, doub
   int rc = read_state(c, &before);
   if (rc) return rc;
   for (uint32_t i = 0; i < n_steps; ++i) {
-    rc = dispatch_apply(c, i == 0 ? 1 : 0, i == 0 ? 0 : 1, step0, 0, seed, kT, max_distance);
+    rc = dispatch_apply(c, i == 0 ? 1 : 0, i == 0 ? 0 : 1, step0, 0, seed, kT, max_distance, 0u);
     if (rc) return rc;
   }
   ApplyState after;
@@ -2021,6 +2839,55 @@ int hmc_run_translate(hmc_ctx* c, uint64_t step0, uint32_t seed, double kT, doub
     out->accepted = acc;
     out->rejected = tri - acc;
   }
+  return HMC_OK;
+}
+
+int hmc_total_energy(hmc_ctx* c, double* energy, double* bound) {
+  if (!c || !energy) return HMC_ERR_INVALID;
+  if (!c->have_state) return fail(c, HMC_ERR_STATE, "hmc_set_state was not called");
+  const Geometry& g = c->g;
+  const uint32_t N = g.n_bodies;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (N == 0) {
+    *energy = 0.0;
+    if (bound) *bound = 0.0;
+    return HMC_OK;
+  }
+  hipStream_t s = c->stream;
+  const uint32_t nblk = grid_for(N, kEnergyBlock);
+  double* part = nullptr;
+  HIPCHK(c, hipMallocAsync((void**)&part, (4 * (size_t)nblk + 4) * sizeof(double), s));
+  // cells for the neighbour search: the checkerboard of the current counter
+  k_setup<<<1, 64, 0, s>>>(g, c->st, 0, 0, 0, 1.0, 1.0, 0, 0, 0u);
+  k_fill_u32<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.cnt, g.n_spaces, 0u);
+  if (g.dim == 2)
+    k_bin<2><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
+  else
+    k_bin<3><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
+  k_scan_blocks<<<c->n_scan_blocks, 256, 0, s>>>(c->w.cnt, c->w.start, c->w.block_sums, c->w.block_ne, g.n_spaces);
+  k_scan_top<<<1, 1024, 0, s>>>(g, c->st, c->w.block_sums, c->w.block_ne, c->n_scan_blocks, c->w.start, 0);
+  k_scan_add<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.start, c->w.block_sums, g.n_spaces);
+  k_scatter<<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->w);
+  if (g.dim == 2)
+    k_energy<2><<<nblk, kEnergyBlock, 0, s>>>(g, c->st, c->bodies, c->w, part);
+  else
+    k_energy<3><<<nblk, kEnergyBlock, 0, s>>>(g, c->st, c->bodies, c->w, part);
+  k_energy_final<<<1, 64, 0, s>>>(part, nblk, part + 4 * (size_t)nblk);
+  double out[4] = {0, 0, 0, 0};
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, part + 4 * (size_t)nblk, sizeof out, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFreeAsync(part, s);
+  if (e != hipSuccess) return fail(c, HMC_ERR_DEVICE, std::string("total energy: ") + hipGetErrorString(e));
+  // the next apply rebuilds the cells; the neighbour copies were not touched
+  ApplyState st;
+  int rc = read_state(c, &st);
+  if (rc) return rc;
+  // any two summation orders of n terms differ by at most 2 gamma_{n-1} sum|terms|
+  const double n = std::max(out[2], 1.0);
+  const double gam = n * 0x1p-53 / (1.0 - n * 0x1p-53);
+  *energy = out[3] != 0.0 ? std::numeric_limits<double>::infinity() : out[0];
+  if (bound) *bound = out[3] != 0.0 ? 0.0 : 2.0 * gam * out[1];
   return HMC_OK;
 }
 
@@ -2140,7 +3007,8 @@ void hmc_destroy(hmc_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (void* p : {(void*)c->bodies.bpos, (void*)c->bodies.spos, (void*)c->cells.body, (void*)c->cells.pos32,
+  for (void* p : {(void*)c->bodies.bpos, (void*)c->bodies.spos, (void*)c->bodies.bquat, (void*)c->bodies.squat,
+                  (void*)c->cells.body, (void*)c->cells.pos32, (void*)c->cells.quat64, (void*)c->verts,
                   (void*)c->cells.pos64, (void*)c->w.cnt, (void*)c->w.start, (void*)c->w.elem_space,
                   (void*)c->w.elem_rank, (void*)c->w.tmp_body, (void*)c->w.block_sums, (void*)c->w.block_ne,
                   (void*)c->w.acc_slots, (void*)c->w.trace, (void*)c->st})
@@ -2150,6 +3018,217 @@ void hmc_destroy(hmc_ctx* c) {
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+}
+
+
+// ------------------------------------------------------------------ batches --
+int hmc_batch_create(uint32_t n_replicas, const hmc_config* cfgs, hmc_batch** out) {
+  if (!out || !cfgs || n_replicas < 1 || n_replicas > (uint32_t)kAccSlots) return HMC_ERR_INVALID;
+  *out = nullptr;
+  auto* b = new hmc_batch;
+  b->R = n_replicas;
+  b->device = cfgs[0].device;
+  b->dim = (int)cfgs[0].dim;
+  b->precision = cfgs[0].precision;
+  b->hreps.resize(n_replicas);
+  b->boff.assign(n_replicas + 1, 0);
+  b->coff.assign(n_replicas + 1, 0);
+  uint64_t nb = 0, ns = 0, npc = 0;
+  for (uint32_t r = 0; r < n_replicas; ++r) {
+    const hmc_config* cfg = cfgs + r;
+    if (cfg->model != HMC_HARD_SPHERE || (int)cfg->dim != b->dim || cfg->precision != b->precision ||
+        cfg->device != b->device) {
+      delete b;
+      return HMC_ERR_UNSUPPORTED;
+    }
+    bool fast = false;
+    double sigma = 0.0, lmax = 0.0;
+    BatchRep& rep = b->hreps[r];
+    std::memset(&rep, 0, sizeof rep);
+    const int rc = make_geometry(cfg, rep.g, &fast, &sigma, &lmax);
+    if (rc) {
+      delete b;
+      return rc;
+    }
+    b->fast = b->fast && fast;
+    rep.body_off = (uint32_t)nb;
+    rep.space_off = (uint32_t)ns;
+    rep.colour_off = (uint32_t)npc;
+    b->boff[r] = (uint32_t)nb;
+    b->coff[r] = (uint32_t)npc;
+    nb += rep.g.n_bodies;
+    ns += rep.g.n_spaces;
+    npc += rep.g.per_colour;
+  }
+  if (nb >= 0x7fffffffull || ns >= 0x7fffffffull || ns > (uint64_t)kScanBlock * 1024 * 16) {
+    delete b;
+    return HMC_ERR_UNSUPPORTED;
+  }
+  b->boff[n_replicas] = (uint32_t)nb;
+  b->coff[n_replicas] = (uint32_t)npc;
+  b->n_total = (uint32_t)nb;
+  b->s_total = (uint32_t)ns;
+  b->pc_total = (uint32_t)npc;
+  std::memset(&b->gtot, 0, sizeof b->gtot);
+  b->gtot.dim = b->dim;
+  b->gtot.n_bodies = b->n_total;
+  b->gtot.n_spaces = b->s_total;
+  if (hipSetDevice(b->device) != hipSuccess) {
+    delete b;
+    return HMC_ERR_DEVICE;
+  }
+  const uint32_t N = b->n_total, R = b->R;
+  b->n_scan_blocks = grid_for(b->s_total, kScanBlock);
+  bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess;
+  ok = ok && dalloc(&b->reps, R) == hipSuccess && dalloc(&b->sts, R + 1) == hipSuccess &&
+       dalloc(&b->dboff, R + 1) == hipSuccess && dalloc(&b->dcoff, R + 1) == hipSuccess &&
+       dalloc(&b->dstep, R) == hipSuccess && dalloc(&b->dseed, R) == hipSuccess && dalloc(&b->dmove, R) == hipSuccess;
+  ok = ok && dalloc(&b->bodies.bpos, N) == hipSuccess && dalloc(&b->bodies.spos, N) == hipSuccess &&
+       dalloc(&b->cells.body, N) == hipSuccess;
+  if (b->precision == HMC_MIXED)
+    ok = ok && dalloc(&b->cells.pos32, N) == hipSuccess;
+  else
+    ok = ok && dalloc(&b->cells.pos64, N) == hipSuccess;
+  ok = ok && dalloc(&b->w.cnt, b->s_total) == hipSuccess && dalloc(&b->w.start, b->s_total + 1) == hipSuccess &&
+       dalloc(&b->w.elem_space, N) == hipSuccess && dalloc(&b->w.elem_rank, N) == hipSuccess &&
+       dalloc(&b->w.tmp_body, N) == hipSuccess && dalloc(&b->w.block_sums, b->n_scan_blocks) == hipSuccess &&
+       dalloc(&b->w.block_ne, b->n_scan_blocks) == hipSuccess && dalloc(&b->w.acc_slots, kAccSlots) == hipSuccess;
+  ok = ok && hipMemcpy(b->reps, b->hreps.data(), R * sizeof(BatchRep), hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(b->dboff, b->boff.data(), (R + 1) * sizeof(uint32_t), hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(b->dcoff, b->coff.data(), (R + 1) * sizeof(uint32_t), hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemset(b->sts, 0, (R + 1) * sizeof(ApplyState)) == hipSuccess &&
+       hipMemset(b->w.acc_slots, 0, kAccSlots * sizeof(unsigned long long)) == hipSuccess;
+  if (!ok) {
+    hmc_batch_destroy(b);
+    return HMC_ERR_DEVICE;
+  }
+  b->cycles_guess = 1;
+  for (uint32_t r = 0; r < R; ++r) {
+    const Geometry& g = b->hreps[r].g;
+    b->cycles_guess = std::max<uint32_t>(b->cycles_guess, (g.n_bodies + g.n_spaces - 1) / std::max<uint32_t>(g.n_spaces, 1));
+  }
+  *out = b;
+  return HMC_OK;
+}
+
+int hmc_batch_set_state(hmc_batch* b, uint32_t replica, const double* body_pos, const double* site_pos) {
+  if (!b || replica >= b->R) return HMC_ERR_INVALID;
+  const Geometry& g = b->hreps[replica].g;
+  const uint32_t N = g.n_bodies, off = b->boff[replica];
+  const int D = b->dim;
+  if (!body_pos && N) return HMC_ERR_INVALID;
+  std::vector<double4> hb(N), hs(site_pos ? N : 0);
+  for (uint32_t i = 0; i < N; ++i) {
+    double v[3] = {0, 0, 0}, u[3] = {0, 0, 0};
+    for (int a = 0; a < D; ++a) v[a] = body_pos[(size_t)i * D + a];
+    hb[i] = make_double4(v[0], v[1], v[2], 0.0);
+    if (site_pos) {
+      for (int a = 0; a < D; ++a) u[a] = site_pos[(size_t)i * D + a];
+      hs[i] = make_double4(u[0], u[1], u[2], 0.0);
+    }
+  }
+  BHIPCHK(b, hipSetDevice(b->device));
+  if (N) {
+    BHIPCHK(b, hipMemcpyAsync(b->bodies.bpos + off, hb.data(), N * sizeof(double4), hipMemcpyHostToDevice, b->stream));
+    if (site_pos)
+      BHIPCHK(b, hipMemcpyAsync(b->bodies.spos + off, hs.data(), N * sizeof(double4), hipMemcpyHostToDevice,
+                                b->stream));
+    Bodies sub{b->bodies.bpos + off, b->bodies.spos + off, nullptr, nullptr};
+    if (D == 2)
+      k_init_sites<2><<<grid_for(N, 256), 256, 0, b->stream>>>(g, sub, site_pos ? 1 : 0);
+    else
+      k_init_sites<3><<<grid_for(N, 256), 256, 0, b->stream>>>(g, sub, site_pos ? 1 : 0);
+    BHIPCHK(b, hipGetLastError());
+  }
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  return HMC_OK;
+}
+
+int hmc_batch_get_state(hmc_batch* b, uint32_t replica, double* body_pos, double* site_pos) {
+  if (!b || replica >= b->R) return HMC_ERR_INVALID;
+  const uint32_t N = b->hreps[replica].g.n_bodies, off = b->boff[replica];
+  const int D = b->dim;
+  std::vector<double4> hb(N), hs(N);
+  BHIPCHK(b, hipSetDevice(b->device));
+  if (N) {
+    BHIPCHK(b, hipMemcpyAsync(hb.data(), b->bodies.bpos + off, N * sizeof(double4), hipMemcpyDeviceToHost, b->stream));
+    BHIPCHK(b, hipMemcpyAsync(hs.data(), b->bodies.spos + off, N * sizeof(double4), hipMemcpyDeviceToHost, b->stream));
+  }
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  for (uint32_t i = 0; i < N; ++i) {
+    const double bv[3] = {hb[i].x, hb[i].y, hb[i].z}, sv[3] = {hs[i].x, hs[i].y, hs[i].z};
+    for (int a = 0; a < D; ++a) {
+      if (body_pos) body_pos[(size_t)i * D + a] = bv[a];
+      if (site_pos) site_pos[(size_t)i * D + a] = sv[a];
+    }
+  }
+  return HMC_OK;
+}
+
+int hmc_batch_run_translate(hmc_batch* b, const uint64_t* step0, const uint32_t* seeds, double kT,
+                            const double* max_distance, uint32_t n_steps, int use_graph, hmc_count* out) {
+  if (!b || !step0 || !seeds || !max_distance || !(kT > 0.0)) return HMC_ERR_INVALID;
+  for (uint32_t r = 0; r < b->R; ++r)
+    if (!(max_distance[r] > 0.0)) return bfail(b, HMC_ERR_INVALID, "max_distance must be > 0");
+  BHIPCHK(b, hipSetDevice(b->device));
+  b->kT = kT;
+  BHIPCHK(b, hipMemcpyAsync(b->dstep, step0, b->R * sizeof(uint64_t), hipMemcpyHostToDevice, b->stream));
+  BHIPCHK(b, hipMemcpyAsync(b->dseed, seeds, b->R * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
+  BHIPCHK(b, hipMemcpyAsync(b->dmove, max_distance, b->R * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  std::vector<ApplyState> st0, st1;
+  std::vector<unsigned long long> a0, a1;
+  int rc = batch_read(b, st0, a0);
+  if (rc) return rc;
+  const uint32_t launched = b->cycles_guess;
+  for (uint32_t i = 0; i < n_steps; ++i) {
+    if (i == 0 || !use_graph) {
+      batch_step(b, i == 0 ? 1 : 0, i == 0 ? 0 : 1, launched);
+      continue;
+    }
+    if (!b->exec || b->graph_cycles != launched) {
+      if (b->exec) (void)hipGraphExecDestroy(b->exec);
+      if (b->graph) (void)hipGraphDestroy(b->graph);
+      b->exec = nullptr;
+      b->graph = nullptr;
+      BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      batch_step(b, 0, 1, launched);
+      BHIPCHK(b, hipStreamEndCapture(b->stream, &b->graph));
+      BHIPCHK(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+      b->graph_cycles = launched;
+    }
+    BHIPCHK(b, hipGraphLaunch(b->exec, b->stream));
+  }
+  BHIPCHK(b, hipGetLastError());
+  rc = batch_read(b, st1, a1);
+  if (rc) return rc;
+  uint32_t most = 1;
+  for (uint32_t r = 0; r < b->R; ++r) {
+    most = std::max<uint32_t>(most, st1[r].cycles);
+    if (out) {
+      out[r].accepted = a1[r] - a0[r];
+      out[r].rejected = (st1[r].trials_total - st0[r].trials_total) - out[r].accepted;
+    }
+  }
+  if (n_steps) b->cycles_guess = most;
+  return HMC_OK;
+}
+
+const char* hmc_batch_last_error(const hmc_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+void hmc_batch_destroy(hmc_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (void* p : {(void*)b->reps, (void*)b->sts, (void*)b->dboff, (void*)b->dcoff, (void*)b->dstep, (void*)b->dseed,
+                  (void*)b->dmove, (void*)b->bodies.bpos, (void*)b->bodies.spos, (void*)b->cells.body,
+                  (void*)b->cells.pos32, (void*)b->cells.pos64, (void*)b->w.cnt, (void*)b->w.start,
+                  (void*)b->w.elem_space, (void*)b->w.elem_rank, (void*)b->w.tmp_body, (void*)b->w.block_sums,
+                  (void*)b->w.block_ne, (void*)b->w.acc_slots})
+    if (p) (void)hipFree(p);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
 }
 
 }  // extern "C"

@@ -61,3 +61,19 @@ def random_dilute(n, L, dim, min_distance, seed):
                 continue
         pts.append(p)
     return np.asarray(pts)
+
+
+def batched_replicas(n_replicas=256, seed=2024, k_range=(8, 25), phi_range=(0.30, 0.54)):
+    """Replicas of hard spheres on FCC lattices: N per replica log-uniform in
+    [4 k_min^3, 4 k_max^3] rounded to an FCC count 4 k^3, packing fractions
+    evenly spaced over phi_range.  Returns a list of (box_edges, positions)."""
+    rng = np.random.default_rng(seed)
+    lo, hi = 4 * k_range[0] ** 3, 4 * k_range[1] ** 3
+    out = []
+    for r in range(n_replicas):
+        n = math.exp(rng.uniform(math.log(lo), math.log(hi)))
+        k = int(round((n / 4.0) ** (1.0 / 3.0)))
+        k = min(max(k, k_range[0]), k_range[1])
+        phi = phi_range[0] + (phi_range[1] - phi_range[0]) * (r / max(n_replicas - 1, 1))
+        out.append(fcc_spheres((k, k, k), packing_fraction=phi))
+    return out

@@ -45,8 +45,11 @@ enum {
   HMC_HARD_SPHERE = 0, /* HardSphere (pairwise/hard_shape.rs:127-174): params[0] = diameter */
   HMC_LJ = 1,          /* Isotropic<LennardJones<12,6>> (isotropic.rs, univariate/lennard_jones.rs):
                           params = {epsilon, sigma, r_cut, -}; HMC_FP64 only */
-  HMC_LJ_SHIFTED = 2   /* Isotropic<Shifted<LennardJones<12,6>>> (univariate/shifted.rs):
+  HMC_LJ_SHIFTED = 2,  /* Isotropic<Shifted<LennardJones<12,6>>> (univariate/shifted.rs):
                           params = {epsilon, sigma, r_cut, r_shift}; HMC_FP64 only */
+  HMC_CONVEX_POLYHEDRON = 3 /* HardShape<Convex<ConvexPolyhedron>> (pairwise/hard_shape.rs:79-118,
+                               hoomd-geometry/src/convex.rs:146-170) on OrientedPoint<Cartesian<3>,
+                               Versor> bodies: hmc_config.vertices; dim 3, HMC_FP64 only */
 };
 
 /* Arithmetic of the overlap test. Both modes make identical decisions. */
@@ -72,6 +75,10 @@ typedef struct {
   int32_t precision;             /* HMC_FP64 or HMC_MIXED */
   int32_t device;                /* HIP device ordinal */
   uint32_t flags;                /* HMC_FLAG_* (0: defaults) */
+  const double* vertices;        /* HMC_CONVEX_POLYHEDRON: n_vertices x 3 vertex coordinates
+                                    (ConvexPolytope<3, 32>, shape/convex_polytope.rs); read at
+                                    hmc_create only */
+  uint32_t n_vertices;           /* 1..32 */
 } hmc_config;
 
 /* hmc_config.flags */
@@ -131,12 +138,32 @@ int hmc_get_state(hmc_ctx* ctx, double* body_pos, double* site_pos);
 int hmc_apply_translate(hmc_ctx* ctx, uint64_t step, uint32_t substep, uint32_t seed, double kT,
                         double max_distance, hmc_count* out, uint32_t* substeps_used);
 
+/* One ParallelSweep::apply with Rotate<Versor> moves (rotate/versor.rs:111-122;
+ * parallel_sweep.rs:390-421) on oriented bodies; max_rotation =
+ * Rotate::maximum_rotation.  Same conventions as hmc_apply_translate. */
+int hmc_apply_rotate(hmc_ctx* ctx, uint64_t step, uint32_t substep, uint32_t seed, double kT,
+                     double max_rotation, hmc_count* out, uint32_t* substeps_used);
+
+/* Orientations of oriented bodies, (s, x, y, z) per body in body index order
+ * (the microstate's stored Versors).  site_quat may be NULL: sites are then
+ * body * identity as OrientedPoint::transform sets them
+ * (property/oriented_point.rs:107-113).  Bodies start with the identity. */
+int hmc_set_orientations(hmc_ctx* ctx, const double* body_quat, const double* site_quat);
+int hmc_get_orientations(hmc_ctx* ctx, double* body_quat, double* site_quat);
+
 /* n_steps iterations of { ParallelSweep::apply; Microstate::increment_step }
  * (the body of HardSphereSim::advance, benchmarks/src/mc/hard_sphere.rs:90-111)
  * starting at (step0, substep 0), with no host synchronisation between
  * applies.  *out receives the summed count. */
 int hmc_run_translate(hmc_ctx* ctx, uint64_t step0, uint32_t seed, double kT, double max_distance,
                       uint32_t n_steps, hmc_count* out);
+
+/* PairwiseCutoff::total_energy (hoomd-interaction/src/pairwise_cutoff.rs:720-745)
+ * of the current state: +inf on any hard overlap.  The reference sums in its
+ * VecCell order, the device in a fixed order of its own; *bound (may be NULL)
+ * receives 2 gamma_{n-1} sum|terms|, which bounds the difference between any
+ * two summation orders of the same n terms. */
+int hmc_total_energy(hmc_ctx* ctx, double* energy, double* bound);
 
 /* Record every non-empty trial of the following applies (testing/replay). */
 int hmc_trace_enable(hmc_ctx* ctx, int on, uint64_t capacity);
@@ -153,9 +180,18 @@ int hmc_profile_read(hmc_ctx* ctx, double* pass_ms_total, uint64_t* pass_launche
                      uint64_t* builds);
 
 /* Evaluate the library's device math on n inputs (kind 0 cbrt, 1 correctly
- * rounded log, 2 correctly rounded exp, 3 sqrt, 4 division in[2i]/in[2i+1]).
+ * rounded log, 2 correctly rounded exp, 3 sqrt, 4 division in[2i]/in[2i+1],
+ * 5 correctly rounded sin, 6 correctly rounded cos).
  * Test hook for bit-exactness checks of the device arithmetic. */
 int hmc_debug_math(int device, int kind, const double* in, double* out, uint64_t n);
+
+/* Evaluate the device convex-polyhedron routines on n inputs: kind 0
+ * collide3d of the polytope with itself (in: v_ij[3], q_ij[4]; out 0/1),
+ * kind 1 the global pair test (in: r_i[3], q_i[4], r_j[3], q_j[4]; out 0/1),
+ * kind 2 the Rotate<Versor> displacement (in: step, substep, seed, index,
+ * sigma; out: q[4]).  Test hook. */
+int hmc_debug_convex(int device, int kind, const double* vertices, uint32_t n_vertices, const double* in,
+                     double* out, uint64_t n);
 
 /* Draw n standard normals (ziggurat) from stream (step, substep, seed, index). */
 int hmc_debug_normals(int device, uint64_t step, uint32_t substep, uint32_t seed, uint64_t index, double* out,
@@ -163,6 +199,25 @@ int hmc_debug_normals(int device, uint64_t step, uint32_t substep, uint32_t seed
 
 const char* hmc_last_error(const hmc_ctx* ctx);
 void hmc_destroy(hmc_ctx* ctx);
+
+/* ---- replica batches ------------------------------------------------------
+ * Many independent hard-sphere/disk systems (own box, checkerboard, counter,
+ * seed and maximum distance each; same dim and precision) advanced together:
+ * every step of every replica is one set of launches, captured once in a
+ * hipGraph and replayed.  Each replica's trials and results are exactly those
+ * of an hmc_ctx of its own.  No counterpart in the reference, which runs
+ * replicas as separate Simulations. */
+typedef struct hmc_batch hmc_batch;
+int hmc_batch_create(uint32_t n_replicas, const hmc_config* cfgs, hmc_batch** out);
+int hmc_batch_set_state(hmc_batch* batch, uint32_t replica, const double* body_pos, const double* site_pos);
+int hmc_batch_get_state(hmc_batch* batch, uint32_t replica, double* body_pos, double* site_pos);
+/* n_steps iterations of { apply; increment_step } for every replica r from
+ * (step0[r], substep 0) with seeds[r] and max_distance[r]; out[r] receives the
+ * summed counts.  use_graph = 0 launches every step directly (same results). */
+int hmc_batch_run_translate(hmc_batch* batch, const uint64_t* step0, const uint32_t* seeds, double kT,
+                            const double* max_distance, uint32_t n_steps, int use_graph, hmc_count* out);
+const char* hmc_batch_last_error(const hmc_batch* batch);
+void hmc_batch_destroy(hmc_batch* batch);
 
 #ifdef __cplusplus
 }

@@ -105,3 +105,37 @@ def test_lj_config_c4_full_size(hmc, wl):
     t = time.time()
     st, ref = _replay_lj(hmc, L, pos, "lj_shifted", (1.0, 1.0, 2.5, 2.5), 1.2, 0.1, 2, trace=False)
     log(f"C4 N={len(pos)}: 2 applies bit-identical ({time.time() - t:.1f}s) {st}")
+
+
+def test_total_energy_matches_oracle(hmc, wl):
+    # PairwiseCutoff::total_energy on the device: within the summation-order
+    # bound of the oracle's VecCell-order sum, before and after moves
+    L, pos = wl.fcc_number_density((6, 6, 6), 0.8)
+    prm = (1.0, 1.0, 2.5, 2.5)
+    ref = O.OracleSystem(L, pos, model="lj_shifted", params=prm, seed=4)
+    b, s = ref.positions()
+    gpu = hmc.GpuParallelSweep(L, b, sites=s, model="lj_shifted", params=prm)
+    for k in range(3):
+        e, bound = gpu.total_energy()
+        eo = ref.total_energy()
+        assert 0.0 < bound < 1e-6 * abs(eo) and abs(e - eo) <= bound, (k, e, eo, bound)
+        step, sub = ref.counter()
+        gpu.apply(step, sub, 4, 1.2, 0.1)
+        ref.parallel_apply(1.2, 0.1)
+        ref.increment_step()
+    gpu.close()
+
+
+def test_total_energy_hard_shapes(hmc, wl):
+    # 0 for a valid hard-sphere state, +inf once two spheres overlap
+    L, pos = wl.fcc_spheres((5, 5, 5), packing_fraction=0.5)
+    ref = O.OracleSystem(L, pos, seed=1)
+    b, s = ref.positions()
+    gpu = hmc.GpuParallelSweep(L, b, sites=s)
+    assert gpu.total_energy()[0] == 0.0 == ref.total_energy()
+    bad = b.copy()
+    bad[7] = bad[3] + np.array([0.3, 0.0, 0.0])
+    ref2 = O.OracleSystem(L, bad, seed=1)
+    gpu.set_positions(ref2.positions()[0], ref2.positions()[1])
+    assert gpu.total_energy()[0] == np.inf == ref2.total_energy()
+    gpu.close()
