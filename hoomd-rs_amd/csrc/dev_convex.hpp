@@ -94,7 +94,10 @@ __device__ __forceinline__ bool tiny3(const v3& a, double tol) {
 }
 
 // collide3d for a polytope with itself (xenocollide.rs:157-292, 378-420).
-__device__ __noinline__ bool collide3d(const double* vx, uint32_t nv, v3 vij, quat qij) {
+__device__ __noinline__ bool collide3d(const double* vx, uint32_t nv, double vx0, double vy0, double vz0, double qs,
+                                       double qx, double qy, double qz) {
+  const v3 vij = {vx0, vy0, vz0};
+  const quat qij = {qs, {qx, qy, qz}};
   const double TOL = 2e-12;
   const int MAX_ITER = 1024;
   const mat3 R = rotmat(qij);
@@ -167,7 +170,9 @@ __device__ __forceinline__ bool convex_overlap(const double* vx, uint32_t nv, do
   if (d2 >= rcut2) return false;
   const v3 rab = sub3(rj, ri);
   const quat inv = {qi.s, neg3(qi.v)};
-  return collide3d(vx, nv, qrotate(inv, rab), qmul(inv, qj));
+  const v3 v = qrotate(inv, rab);
+  const quat o = qmul(inv, qj);
+  return collide3d(vx, nv, v.x, v.y, v.z, o.s, o.v.x, o.v.y, o.v.z);
 }
 
 // VersorDisplacement::sample (hoomd-mc/src/rotate/versor.rs:39-72).

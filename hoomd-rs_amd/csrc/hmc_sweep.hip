@@ -80,6 +80,10 @@ struct Geometry {
   uint32_t nverts;
   double cv_rcut2;         // (R + R)^2, R the circumsphere radius (convex.rs:163-166)
   double cv_edge2;         // cv_rcut2 (1 - 2^-30): at or above it the spatial-structure rule is applied
+  // A copy of this struct in device memory.  Out-of-line helpers receive this
+  // pointer instead of the address of a kernel argument, which would make
+  // every lane copy the whole struct to scratch memory at kernel entry.
+  const Geometry* self;
 };
 
 // Device-resident per-apply state.
@@ -415,9 +419,12 @@ __device__ __forceinline__ void ball_sample(Sfc64& rng, double r, const double* 
 // axis (hoomd-spatial/src/vec_cell.rs:356-358, 710-733).  Away from contact
 // the rule cannot change the answer, so it is evaluated only there.
 template <int D>
-__device__ __noinline__ double edge_r2(const Geometry& g, const double* t, double4 sj) {
-  uint32_t fl = (uint32_t)__double_as_longlong(sj.w);
-  double xj[3] = {sj.x, sj.y, sj.z};
+__device__ __noinline__ double edge_r2(const Geometry* gp, double t0, double t1, double t2, double sx, double sy,
+                                       double sz, double sw) {
+  const Geometry& g = *gp;
+  const double t[3] = {t0, t1, t2};
+  uint32_t fl = (uint32_t)__double_as_longlong(sw);
+  double xj[3] = {sx, sy, sz};
   double d2 = 0.0;
   for (int a = 0; a < D; ++a) {
     double ct = floor(t[a] / g.vc_width);
@@ -444,7 +451,7 @@ __device__ __noinline__ double edge_r2(const Geometry& g, const double* t, doubl
 
 template <int D>
 __device__ __forceinline__ bool overlap64_edge(const Geometry& g, const double* t, double4 sj) {
-  return edge_r2<D>(g, t, sj) < g.sigma2;
+  return edge_r2<D>(g.self, t[0], t[1], D == 3 ? t[2] : 0.0, sj.x, sj.y, sj.z, sj.w) < g.sigma2;
 }
 
 template <int D>
@@ -491,10 +498,10 @@ __device__ __forceinline__ int overlap32(const Geometry& g, const float* t, floa
 // FP64 re-test of a pair inside the FP32 error band (rare; kept out of line
 // so the unrolled neighbour scan stays small).
 template <int D>
-__device__ __noinline__ bool retest64(const Geometry& g, const Bodies& B, const Cells& C, uint32_t q, double t0,
-                                      double t1, double t2) {
+__device__ __noinline__ bool retest64(const Geometry* gp, const double4* spos, const uint32_t* cbody, uint32_t q,
+                                      double t0, double t1, double t2) {
   const double t[3] = {t0, t1, t2};
-  return overlap64<D>(g, t, B.spos[C.body[q]]);
+  return overlap64<D>(*gp, t, spos[cbody[q]]);
 }
 
 // Overlap of the trial site with the site in slot q, in the context's
@@ -506,7 +513,7 @@ __device__ __forceinline__ bool pair_hits(const Geometry& g, const Bodies& B, co
     int r = overlap32<D>(g, t32, C.pos32[q]);
     if (r == 2) {
       ++retests;
-      r = retest64<D>(g, B, C, q, t[0], t[1], t[2]) ? 1 : 0;
+      r = retest64<D>(g.self, B.spos, C.body, q, t[0], t[1], t[2]) ? 1 : 0;
     }
     return r != 0;
   }
@@ -643,7 +650,7 @@ __device__ __forceinline__ void trial(const Geometry& g, const ApplyState* st, c
               hit |= valid && r == 1;
               if (valid && r == 2) {
                 ++retests;
-                hit |= retest64<D>(g, B, C, q, t[0], t[1], t[2]);
+                hit |= retest64<D>(g.self, B.spos, C.body, q, t[0], t[1], t[2]);
               }
             } else {
               hit |= valid && overlap64<D>(g, t, *(const double4*)&vq[j]);
@@ -810,10 +817,9 @@ __device__ __forceinline__ void box_relative(const Geometry& g, const float* c32
 }
 
 // The trial of space s from global memory, for a tile whose box holds more
-// bodies than its LDS copy (rare; kept out of line so that the tile kernel's
-// code stays small).
+// bodies than its LDS copy (rare).
 template <int D, int PREC>
-__device__ __noinline__ TrialOut trial_from_global(const Geometry& g, const ApplyState* st, const Bodies& B,
+__device__ __forceinline__ TrialOut trial_from_global(const Geometry& g, const ApplyState* st, const Bodies& B,
                                                    const Cells& C, const Work& w, const double* zx, const double* zf,
                                                    uint32_t s, uint32_t s0, uint32_t n, uint32_t colour,
                                                    uint32_t substep) {
@@ -918,12 +924,13 @@ __global__ void __launch_bounds__(kTileThreads) k_pass_tile(Geometry g, ApplySta
   }
   const uint32_t n = s1 - s0;
   const bool go = active && n != 0;
-  // body indices of the own space (the chosen one is among them unless pick >= 3)
-  uint32_t ob[3] = {0, 0, 0};
+  // body indices of the own space (the chosen one is among them unless
+  // pick >= 3); three scalars, so that the loads stay in flight together
+  uint32_t ob0 = 0, ob1 = 0, ob2 = 0;
   if (go) {
-    ob[0] = C.body[s0];
-    if (n > 1) ob[1] = C.body[s0 + 1];
-    if (n > 2) ob[2] = C.body[s0 + 2];
+    ob0 = C.body[s0];
+    if (n > 1) ob1 = C.body[s0 + 1];
+    if (n > 2) ob2 = C.body[s0 + 2];
   }
   __syncthreads();
 
@@ -974,7 +981,7 @@ __global__ void __launch_bounds__(kTileThreads) k_pass_tile(Geometry g, ApplySta
       }
     }
   }
-  uint32_t body = ob[pick < 2 ? pick : 2];
+  uint32_t body = pick == 0 ? ob0 : (pick == 1 ? ob1 : ob2);
   if (go && !overflow && pick > 2) body = C.body[s0 + pick];
   double4 bp = make_double4(0.0, 0.0, 0.0, 0.0);
   if (go && !overflow) bp = B.bpos[body];
@@ -1085,7 +1092,7 @@ __global__ void __launch_bounds__(kTileThreads) k_pass_tile(Geometry g, ApplySta
             amb &= amb - 1;
             const float4 p = *(const float4*)&lb[rlo[r] + k];
             ++retests;
-            hit |= retest64<D>(g, B, C, (uint32_t)__float_as_int(p.w), t[0], t[1], t[2]);
+            hit |= retest64<D>(g.self, B.spos, C.body, (uint32_t)__float_as_int(p.w), t[0], t[1], t[2]);
           }
         }
       } else {
@@ -1204,7 +1211,7 @@ __device__ __forceinline__ double lj_r2(const Geometry& g, const double* q, doub
     }
     d2 = d2 + d * d;
   }
-  if (d2 >= g.lj_rc2_edge && d2 < g.lj_rc2) d2 = edge_r2<D>(g, q, sj);
+  if (d2 >= g.lj_rc2_edge && d2 < g.lj_rc2) d2 = edge_r2<D>(g.self, q[0], q[1], D == 3 ? q[2] : 0.0, sj.x, sj.y, sj.z, sj.w);
   return d2;
 }
 
@@ -1458,9 +1465,12 @@ __global__ void __launch_bounds__(kGuardThreads) k_guard_lj(Geometry g, ApplySta
 // query can find within 2 R <= L/2), its squared distance
 // distance_squared(q, image) (cartesian.rs:339-342) and, within 2^-30 of the
 // circumsphere cutoff, the VecCell visitation rule as for hard spheres.
-__device__ __noinline__ double convex_edge_image(const Geometry& g, const double* q, double4 sj, v3* rj) {
-  const uint32_t fl = (uint32_t)__double_as_longlong(sj.w);
-  const double xj[3] = {sj.x, sj.y, sj.z};
+__device__ __noinline__ double4 convex_edge_image(const Geometry* gp, double q0, double q1, double q2, double sx,
+                                                  double sy, double sz, double sw) {
+  const Geometry& g = *gp;
+  const double q[3] = {q0, q1, q2};
+  const uint32_t fl = (uint32_t)__double_as_longlong(sw);
+  const double xj[3] = {sx, sy, sz};
   double img[3] = {0.0, 0.0, 0.0};
   for (int a = 0; a < 3; ++a) {
     const double ct = floor(q[a] / g.vc_width);
@@ -1483,16 +1493,15 @@ __device__ __noinline__ double convex_edge_image(const Geometry& g, const double
         any = true;
       }
     }
-    if (!any) return __longlong_as_double(0x7ff0000000000000ll);
+    if (!any) return make_double4(__longlong_as_double(0x7ff0000000000000ll), 0.0, 0.0, 0.0);
     img[a] = bx;
   }
-  *rj = {img[0], img[1], img[2]};
   double d2 = 0.0;
   for (int a = 0; a < 3; ++a) {
     const double d = q[a] - img[a];
     d2 = d2 + d * d;
   }
-  return d2;
+  return make_double4(d2, img[0], img[1], img[2]);  // squared distance, image
 }
 
 __device__ __forceinline__ double convex_image(const Geometry& g, const double* q, double4 sj, v3* rj) {
@@ -1512,7 +1521,11 @@ __device__ __forceinline__ double convex_image(const Geometry& g, const double* 
     d2 = d2 + d * d;
   }
   *rj = {img[0], img[1], img[2]};
-  if (d2 >= g.cv_edge2 && d2 < g.cv_rcut2) d2 = convex_edge_image(g, q, sj, rj);
+  if (d2 >= g.cv_edge2 && d2 < g.cv_rcut2) {
+    const double4 e = convex_edge_image(g.self, q[0], q[1], q[2], sj.x, sj.y, sj.z, sj.w);
+    d2 = e.x;
+    *rj = {e.y, e.z, e.w};
+  }
   return d2;
 }
 
@@ -1696,7 +1709,7 @@ __global__ void k_debug_convex(int kind, const double* vx, uint32_t nv, double r
   if (i >= n) return;
   if (kind == 0) {
     const double* p = in + 7 * i;
-    out[i] = collide3d(vx, nv, v3{p[0], p[1], p[2]}, quat{p[3], {p[4], p[5], p[6]}}) ? 1.0 : 0.0;
+    out[i] = collide3d(vx, nv, p[0], p[1], p[2], p[3], p[4], p[5], p[6]) ? 1.0 : 0.0;
   } else if (kind == 1) {
     const double* p = in + 14 * i;
     const v3 ri = {p[0], p[1], p[2]}, rj = {p[7], p[8], p[9]};
@@ -1829,7 +1842,7 @@ __global__ void __launch_bounds__(kBlock) k_bpass(const BatchRep* reps, ApplySta
   ApplyState* st = sts + r;
   if (cycle >= st->cycles) return;
   const BatchRep& rep = reps[r];
-  const Geometry g = rep.g;
+  const Geometry& g = rep.g;
   int32_t m[3] = {0, 0, 0};
   const uint32_t s = colour_space<D>(g, colour, t - rep.colour_off, m);
   Work wr = w;
@@ -1855,7 +1868,7 @@ __global__ void __launch_bounds__(kGuardThreads) k_bguard(const BatchRep* reps, 
   __shared__ double zx[257], zf[257];
   if (D == 3) stage_zig(zx, zf);
   __syncthreads();
-  const Geometry g = reps[r].g;
+  const Geometry& g = reps[r].g;
   Work wr = w;
   wr.start = w.start + reps[r].space_off;
   const uint32_t rounds = (g.per_colour + kGuardThreads - 1) / kGuardThreads;
@@ -2060,6 +2073,7 @@ struct hmc_ctx {
   bool tiled = false;  // LDS-staged pass (requires fast)
   TileCfg tc{};
   double* verts = nullptr;  // convex polyhedron vertices (device)
+  Geometry* gdev = nullptr;  // device copy of g (g.self)
   uint32_t n_scan_blocks = 0;
   uint64_t applies = 0;
   std::string err;
@@ -2631,6 +2645,16 @@ int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
   }
   // first guess of the number of colour cycles per apply: every space occupied
   c->cycles_guess = (uint32_t)std::max<uint64_t>(1, (N + g.n_spaces - 1) / std::max<uint32_t>(g.n_spaces, 1));
+  // the device copy of the geometry (complete from here on)
+  if (dalloc(&c->gdev, 1) != hipSuccess) {
+    hmc_destroy(c);
+    return HMC_ERR_DEVICE;
+  }
+  g.self = c->gdev;
+  if (hipMemcpy(c->gdev, &g, sizeof(Geometry), hipMemcpyHostToDevice) != hipSuccess) {
+    hmc_destroy(c);
+    return HMC_ERR_DEVICE;
+  }
   *out = c;
   return HMC_OK;
 }
@@ -3009,6 +3033,7 @@ void hmc_destroy(hmc_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void* p : {(void*)c->bodies.bpos, (void*)c->bodies.spos, (void*)c->bodies.bquat, (void*)c->bodies.squat,
                   (void*)c->cells.body, (void*)c->cells.pos32, (void*)c->cells.quat64, (void*)c->verts,
+                  (void*)c->gdev,
                   (void*)c->cells.pos64, (void*)c->w.cnt, (void*)c->w.start, (void*)c->w.elem_space,
                   (void*)c->w.elem_rank, (void*)c->w.tmp_body, (void*)c->w.block_sums, (void*)c->w.block_ne,
                   (void*)c->w.acc_slots, (void*)c->w.trace, (void*)c->st})
@@ -3093,6 +3118,8 @@ int hmc_batch_create(uint32_t n_replicas, const hmc_config* cfgs, hmc_batch** ou
        dalloc(&b->w.elem_space, N) == hipSuccess && dalloc(&b->w.elem_rank, N) == hipSuccess &&
        dalloc(&b->w.tmp_body, N) == hipSuccess && dalloc(&b->w.block_sums, b->n_scan_blocks) == hipSuccess &&
        dalloc(&b->w.block_ne, b->n_scan_blocks) == hipSuccess && dalloc(&b->w.acc_slots, kAccSlots) == hipSuccess;
+  if (ok)
+    for (uint32_t r = 0; r < R; ++r) b->hreps[r].g.self = &b->reps[r].g;
   ok = ok && hipMemcpy(b->reps, b->hreps.data(), R * sizeof(BatchRep), hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(b->dboff, b->boff.data(), (R + 1) * sizeof(uint32_t), hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(b->dcoff, b->coff.data(), (R + 1) * sizeof(uint32_t), hipMemcpyHostToDevice) == hipSuccess &&
