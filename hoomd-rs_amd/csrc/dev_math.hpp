@@ -6,7 +6,8 @@
 //    uses; it matches that library bit for bit on the inputs the 3D ball
 //    sampler produces (checked by sampling in the test suite).
 //  * hmc_log_cr / hmc_exp_cr: correctly rounded natural log and exp evaluated
-//    in double-double arithmetic (about 100 bits) and rounded once.
+//    in double-double arithmetic (about 100 bits) and rounded once; log uses a
+//    64-point table reduction (dd_tables.inc).
 //  * hmc_sin_cr / hmc_cos_cr: correctly rounded sin and cos on |x| <= 2.
 //  * hmc_lt_exp: decides lhs < exp(y) exactly (as if exp were correctly
 //    rounded): the fast library exp decides unless lhs lies within 2^-44
@@ -99,27 +100,33 @@ HMC_HD dd dd_div(dd x, dd y) {
 // ln 2 = 0.693147180559945309417232121458176568... as hi + lo
 HMC_HD dd dd_ln2() { return {0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56}; }
 
-// Correctly rounded log(x) for finite x > 0.
-HMC_HD_COLD double hmc_log_cr(double x) {
+#include "dd_tables.inc"
+
+__device__ __forceinline__ dd dd_const(const double (*t)[2], int i) { return dd{t[i][0], t[i][1]}; }
+
+// Correctly rounded log(x) for finite x > 0: x = 2^e m with m in
+// [sqrt(1/2), sqrt(2)), m = c (1 + r) with c = j/64 the nearest table point
+// (c = 1 exactly around x = 1, so no cancellation there), and
+// log(m) = log(c) + 2 atanh(f), f = (m - c)/(m + c), |f| < 0.0056, whose
+// series is summed to f^15 in double-double (truncation below 2^-120).
+__device__ __noinline__ double hmc_log_cr(double x) {
   int e;
   double m = frexp(x, &e);
   if (m < 0.70710678118654752440) {
     m *= 2.0;
     e -= 1;
   }
-  double num = m - 1.0;  // exact (Sterbenz)
-  dd f = dd_div(dd{num, 0.0}, two_sum(m, 1.0));
-  dd s = dd_mul(f, f);
-  dd t = f;
-  dd sum = f;
+  const int j = (int)rint(m * 64.0);  // 45 .. 91
+  const double c = (double)j * 0x1p-6;
+  const double num = m - c;  // exact (Sterbenz)
+  const dd f = dd_div(dd{num, 0.0}, two_sum(m, c));
+  const dd s2 = dd_mul(f, f);
+  dd p = dd_const(kInvOdd, 7);
 #pragma unroll 1
-  for (int k = 1; k <= 24; ++k) {
-    t = dd_mul(t, s);
-    dd term = dd_div(t, dd{(double)(2 * k + 1), 0.0});
-    sum = dd_add(sum, term);
-  }
-  dd lm = dd{2.0 * sum.hi, 2.0 * sum.lo};
-  dd res = dd_add(dd_mul_d(dd_ln2(), (double)e), lm);
+  for (int k = 6; k >= 0; --k) p = dd_add(dd_mul(p, s2), dd_const(kInvOdd, k));
+  const dd at = dd_mul(p, f);
+  const dd lm = {2.0 * at.hi, 2.0 * at.lo};
+  const dd res = dd_add(dd_add(dd_mul_d(dd_ln2(), (double)e), dd_const(kLogC, j - 45)), lm);
   return res.hi + res.lo;
 }
 
@@ -138,24 +145,25 @@ HMC_HD_COLD double hmc_exp_cr(double y) {
 
 // Correctly rounded sin(x) and cos(x) for |x| <= 2 (the rotation sampler
 // evaluates them at half the rotation angle, in [0, pi/2]): Taylor series in
-// Horner form, 20 terms, in double-double, rounded once.
-HMC_HD_COLD double hmc_sin_cr(double x) {
+// Horner form, 20 terms, in double-double with double-double reciprocal
+// coefficients, rounded once.
+__device__ __noinline__ double hmc_sin_cr(double x) {
   const dd x2 = dd_mul(dd{x, 0.0}, dd{x, 0.0});
   dd p = {1.0, 0.0};
 #pragma unroll 1
   for (int n = 20; n >= 1; --n) {
-    const dd t = dd_div(dd_mul(p, x2), dd{(double)(2 * n) * (double)(2 * n + 1), 0.0});
+    const dd t = dd_mul(dd_mul(p, x2), dd_const(kInvSin, n));
     p = dd_add(dd{1.0, 0.0}, dd{-t.hi, -t.lo});
   }
   const dd r = dd_mul(p, dd{x, 0.0});
   return r.hi + r.lo;
 }
-HMC_HD_COLD double hmc_cos_cr(double x) {
+__device__ __noinline__ double hmc_cos_cr(double x) {
   const dd x2 = dd_mul(dd{x, 0.0}, dd{x, 0.0});
   dd p = {1.0, 0.0};
 #pragma unroll 1
   for (int n = 20; n >= 1; --n) {
-    const dd t = dd_div(dd_mul(p, x2), dd{(double)(2 * n - 1) * (double)(2 * n), 0.0});
+    const dd t = dd_mul(dd_mul(p, x2), dd_const(kInvCos, n));
     p = dd_add(dd{1.0, 0.0}, dd{-t.hi, -t.lo});
   }
   return p.hi + p.lo;
