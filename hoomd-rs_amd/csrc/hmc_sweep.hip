@@ -2078,7 +2078,13 @@ struct hmc_ctx {
   uint64_t applies = 0;
   std::string err;
   bool profiling = false;
+  bool capturing = false;       // recording a graph: no event timing
   uint32_t profile_period = 1;  // time the launches of every k-th apply
+  // one apply (with increment_step) captured as a graph, and what it depends on
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  double graph_kT = 0.0, graph_move_size = 0.0;
+  uint32_t graph_move = 0, graph_launched = 0;
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pass_events, build_events;
   double pass_ms = 0.0, build_ms = 0.0;
@@ -2122,7 +2128,7 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
   const Geometry& g = c->g;
   hipStream_t s = c->stream;
   const uint32_t N = g.n_bodies;
-  const bool prof = c->profiling && c->applies % c->profile_period == 0;
+  const bool prof = c->profiling && !c->capturing && c->applies % c->profile_period == 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (prof) {
     e0 = take_event(c);
@@ -2199,6 +2205,15 @@ int dispatch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32
   if (d == 2) return launch_apply<2, HMC_MIXED>(c, from_args, advance, step, substep, seed, kT, max_move, move);
   if (p == HMC_FP64) return launch_apply<3, HMC_FP64>(c, from_args, advance, step, substep, seed, kT, max_move, move);
   return launch_apply<3, HMC_MIXED>(c, from_args, advance, step, substep, seed, kT, max_move, move);
+}
+
+// Forget the captured step graph (its kernel arguments name buffers that are
+// about to change).
+void drop_graph(hmc_ctx* c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
 }
 
 void collect_events(hmc_ctx* c, std::vector<std::pair<hipEvent_t, hipEvent_t>>& evs, double& ms_total,
@@ -2849,9 +2864,38 @@ int hmc_run_translate(hmc_ctx* c, uint64_t step0, uint32_t seed, double kT, doub
   ApplyState before;
   int rc = read_state(c, &before);
   if (rc) return rc;
+  const bool lj = c->g.model == HMC_LJ || c->g.model == HMC_LJ_SHIFTED;
+  const bool graphs = !(c->cfg.flags & HMC_FLAG_NO_GRAPH);
   for (uint32_t i = 0; i < n_steps; ++i) {
-    rc = dispatch_apply(c, i == 0 ? 1 : 0, i == 0 ? 0 : 1, step0, 0, seed, kT, max_distance, 0u);
-    if (rc) return rc;
+    const bool timed = c->profiling && c->applies % c->profile_period == 0;
+    if (i == 0 || !graphs || timed) {
+      rc = dispatch_apply(c, i == 0 ? 1 : 0, i == 0 ? 0 : 1, step0, 0, seed, kT, max_distance, 0u);
+      if (rc) return rc;
+      continue;
+    }
+    // later steps: replay one captured { apply; increment_step }
+    const uint32_t launched = c->cycles_guess + (lj ? 1u : 0u);
+    if (!c->graph_exec || c->graph_kT != kT || c->graph_move_size != max_distance || c->graph_move != 0u ||
+        c->graph_launched != launched) {
+      drop_graph(c);
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      c->capturing = true;
+      rc = dispatch_apply(c, 0, 1, step0, 0, seed, kT, max_distance, 0u);
+      c->capturing = false;
+      c->applies -= 1;  // counted when replayed
+      hipGraph_t gr = nullptr;
+      const hipError_t ce = hipStreamEndCapture(c->stream, &gr);
+      if (rc) return rc;
+      HIPCHK(c, ce);
+      c->graph = gr;
+      HIPCHK(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+      c->graph_kT = kT;
+      c->graph_move_size = max_distance;
+      c->graph_move = 0u;
+      c->graph_launched = launched;
+    }
+    HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+    c->applies += 1;
   }
   ApplyState after;
   rc = read_state(c, &after);
@@ -2919,6 +2963,7 @@ int hmc_trace_enable(hmc_ctx* c, int on, uint64_t capacity) {
   if (!c) return HMC_ERR_INVALID;
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  drop_graph(c);
   if (c->w.trace) {
     HIPCHK(c, hipFree(c->w.trace));
     c->w.trace = nullptr;
@@ -3031,6 +3076,7 @@ void hmc_destroy(hmc_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
   for (void* p : {(void*)c->bodies.bpos, (void*)c->bodies.spos, (void*)c->bodies.bquat, (void*)c->bodies.squat,
                   (void*)c->cells.body, (void*)c->cells.pos32, (void*)c->cells.quat64, (void*)c->verts,
                   (void*)c->gdev,

@@ -83,9 +83,12 @@ typedef struct {
 
 /* hmc_config.flags */
 enum {
-  HMC_FLAG_GLOBAL_PASS = 1 /* run every trial from global memory instead of
-                              staging each workgroup's neighbourhood box in LDS
-                              (same decisions; for comparison and testing) */
+  HMC_FLAG_GLOBAL_PASS = 1, /* run every trial from global memory instead of
+                               staging each workgroup's neighbourhood box in LDS
+                               (same decisions; for comparison and testing) */
+  HMC_FLAG_NO_GRAPH = 2     /* hmc_run_translate launches every apply directly
+                               instead of replaying a captured hipGraph
+                               (same results) */
 };
 
 /* One trial of a trace: replaces the per-space BodyTrial status of
@@ -154,7 +157,9 @@ int hmc_get_orientations(hmc_ctx* ctx, double* body_quat, double* site_quat);
 /* n_steps iterations of { ParallelSweep::apply; Microstate::increment_step }
  * (the body of HardSphereSim::advance, benchmarks/src/mc/hard_sphere.rs:90-111)
  * starting at (step0, substep 0), with no host synchronisation between
- * applies.  *out receives the summed count. */
+ * applies; after the first apply the launches of one step are replayed from a
+ * captured hipGraph (HMC_FLAG_NO_GRAPH: launched directly).  *out receives
+ * the summed count. */
 int hmc_run_translate(hmc_ctx* ctx, uint64_t step0, uint32_t seed, double kT, double max_distance,
                       uint32_t n_steps, hmc_count* out);
 

@@ -205,10 +205,11 @@ def test_body_outside_checkerboard_is_an_error(hmc):
     assert e.value.code == hmc.HMC_ERR_COVER
 
 
-def test_run_equals_repeated_apply(hmc, wl):
+@pytest.mark.parametrize("graphs", [True, False], ids=["graph", "direct"])
+def test_run_equals_repeated_apply(hmc, wl, graphs):
     L, pos = wl.fcc_spheres((8, 8, 8), packing_fraction=0.5)
     a = hmc.GpuParallelSweep(L, pos, precision="mixed")
-    b = hmc.GpuParallelSweep(L, pos, precision="mixed")
+    b = hmc.GpuParallelSweep(L, pos, precision="mixed", graphs=graphs)
     tot = [0, 0]
     for step in range(6):
         x, y, _ = a.apply(100 + step, 0, 9, 1.0, 0.08)
@@ -217,6 +218,29 @@ def test_run_equals_repeated_apply(hmc, wl):
     assert list(b.run(100, 9, 1.0, 0.08, 6)) == tot
     assert np.array_equal(a.positions()[0], b.positions()[0])
     assert np.array_equal(a.positions()[1], b.positions()[1])
+
+
+def test_run_graph_replays_across_parameter_changes(hmc, wl):
+    # runs in pieces with the move size, kT and model changing between them
+    # (each change re-captures the step graph), host reads in between, against
+    # directly launched applies; LJ also launches a spare cycle per apply
+    for model, prec, params, pieces in (("hard_sphere", "mixed", None,
+                                         [(0.08, 1.0, 5), (0.08, 1.0, 3), (0.12, 1.0, 4)]),
+                                        ("lj_shifted", "fp64", (1.0, 1.0, 2.5, 2.5),
+                                         [(0.1, 1.2, 4), (0.1, 0.9, 4), (0.15, 0.9, 3)])):
+        if model == "hard_sphere":
+            L, pos = wl.fcc_spheres((7, 7, 7), packing_fraction=0.5)
+        else:
+            L, pos = wl.fcc_number_density((6, 6, 6), 0.8)
+        g = hmc.GpuParallelSweep(L, pos, precision=prec, model=model, params=params)
+        d = hmc.GpuParallelSweep(L, pos, precision=prec, model=model, params=params, graphs=False)
+        step = 10
+        for dist, kT, n in pieces:
+            assert g.run(step, 5, kT, dist, n) == d.run(step, 5, kT, dist, n), (model, step)
+            assert np.array_equal(g.positions()[1], d.positions()[1]), (model, step)
+            step += n
+        g.close()
+        d.close()
 
 
 # ---------------------------------------------------------- full size (C2) --
