@@ -22,7 +22,12 @@ struct Sfc64 {
   __device__ __forceinline__ uint64_t next() {
     uint64_t out = a + b + ctr;
     a = b ^ (b >> 11);
-    b = c + (c << 3);
+    // c + 8c as a shift and an add: folded into a multiply it would read an
+    // unused high half from whatever register holds it, and a register with a
+    // load in flight makes the wave wait for that load
+    uint64_t c8 = c << 3;
+    asm("" : "+v"(c8));
+    b = c + c8;
     c = ((c << 24) | (c >> 40)) + out;
     ctr += 1;
     return out;

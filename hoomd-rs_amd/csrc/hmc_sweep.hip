@@ -97,7 +97,9 @@ struct ApplyState {
   uint32_t cycles;
   uint32_t nonempty;
   uint32_t error;
-  uint32_t move;  // 0 Translate, 1 Rotate<Versor>
+  uint32_t move;      // 0 Translate, 1 Rotate<Versor>
+  uint32_t grid_bar;  // arrivals at the persistent apply's grid barriers
+  uint32_t pad;
   unsigned long long accepted_total;  // filled on the host from the slot counters
   unsigned long long trials_total;
   unsigned long long cycles_total;
@@ -135,6 +137,7 @@ struct Work {
   hmc_trace_record* trace;
   uint64_t trace_cap;
   int tracing;
+  uint32_t* gflags;  // persistent apply: one release flag per workgroup, kGridFlagStride words apart
 };
 
 // ----------------------------------------------------------- small helpers --
@@ -241,6 +244,7 @@ __global__ void k_setup(Geometry g, ApplyState* st, uint64_t step, uint32_t subs
   }
   st->cycles = 0;
   st->nonempty = 0;
+  st->grid_bar = 0;
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
@@ -754,13 +758,20 @@ __global__ void __launch_bounds__(kBlock) k_pass(Geometry g, ApplyState* st, Bod
 // see.  (In the last, partial tile along an axis the box wraps onto spaces
 // of other tiles; those cells are staged but no active trial reads them.)
 //
-// The pass is a chain of dependent memory phases, so the kernel is laid out
-// to keep that chain short: (A) the cell ranges of the box, the trial's own
-// range and the ziggurat tables are requested together; (B) the counts are
-// scanned into LDS offsets while the body indices of the trial's space are in
-// flight and the random proposal is drawn; (C) the neighbour copies of the
-// sites and the chosen body's position are requested together, one LDS entry
-// per lane; (D) the trials test their neighbours from LDS.
+// Spaces are numbered with the last axis fastest and the slots are grouped
+// by space in that order, so every row of the box along the last axis is one
+// contiguous run of slots, or two where the row wraps round the periodic
+// edge.  The wrap falls at the same cell of every row of a box, and a box
+// spans at most shape - 2 spaces, so there is at most one.  The pass is a
+// chain of dependent phases, kept to two barriers:
+//  (A) every row's E + 2 cell starts are loaded (the E + 1 boundaries of its
+//      cells and the start of the second run), one coalesced sweep over the
+//      box, while wave 0 loads the two runs of each row and scans the row
+//      lengths into LDS offsets;
+//  (B) a half-wave per row copies the row's runs into LDS with loads of
+//      consecutive slots; the proposal is drawn while the first of those
+//      loads are in flight; the loaded starts become LDS cell offsets;
+//  (C) the trials test their neighbours from LDS.
 //
 // Mixed precision stages each site relative to the box centre c, as the
 // image nearest c (the box spans less than L - 2 w on every axis, so that
@@ -776,19 +787,23 @@ struct TileCfg {
   uint32_t E[3];     // staged box edge 2T + 1 (1 for unused axes)
   uint32_t trials;   // T0 T1 T2
   uint32_t ncell;    // E0 E1 E2
+  uint32_t rows;     // rows of the box along the last axis (ncell / E[D-1])
   uint32_t maxb;     // body capacity of the LDS copy
   uint32_t ntiles;
   uint32_t xcd_chunk;
   uint32_t lds_bytes;
   float band_lo, band_hi;  // FP32 band of the box-relative test
-  uint32_t step[3];        // the workgroup size written in the box's mixed radix (E0, E1, E2)
+  uint32_t step[3];        // the workgroup size in the mixed radix (E0, .., E[D-2], E[D-1] + 2) of the row starts
 };
 
-constexpr int kTileThreads = 256;        // upper bound of the workgroup size
-constexpr int kCellsPerThread = 16;      // staged cells per thread (ncell <= 16 x threads)
-constexpr int kStageBatch = 8;           // staged sites per lane per batch of loads
-constexpr int kStagePerLane = 16;        // staged sites per lane at most (maxb <= 16 x threads)
-constexpr size_t kTileLds = 72 * 1024;   // dynamic LDS per workgroup
+constexpr int kTileThreads = 256;       // upper bound of the workgroup size
+constexpr int kValsPerThread = 16;      // row starts per thread: rows (E + 2) <= 16 x threads
+constexpr int kRowsPerLane = 4;         // rows per lane of wave 0: rows <= 256
+// rows per half-wave whose loads are in flight together (the kernel runs at
+// two waves per SIMD, so registers are not the limit)
+template <int PREC>
+constexpr int kRowBatch = PREC == HMC_MIXED ? 16 : 8;
+constexpr size_t kTileLds = 72 * 1024;  // dynamic LDS per workgroup
 
 // Inclusive scan over the 64 lanes of a wavefront.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
@@ -816,6 +831,22 @@ __device__ __forceinline__ void box_relative(const Geometry& g, const float* c32
   }
 }
 
+// The LDS entry of a staged site: box-relative FP32 position with the source
+// slot in w (mixed), or the FP64 copy itself.
+template <int D, int PREC>
+__device__ __forceinline__ typename CellCopy<PREC>::type stage_entry(const Geometry& g, const float* c32,
+                                                                   typename CellCopy<PREC>::type v, uint32_t slot) {
+  if constexpr (PREC == HMC_MIXED) {
+    const float x[3] = {v.x, v.y, v.z};
+    float r[3] = {0.0f, 0.0f, 0.0f};
+    box_relative<D>(g, c32, x, r);
+    if ((uint32_t)__float_as_int(v.w) & kNoImages) r[0] = __int_as_float(0x7fc00000);
+    return make_float4(r[0], r[1], r[2], __int_as_float((int)slot));
+  } else {
+    return v;
+  }
+}
+
 // The trial of space s from global memory, for a tile whose box holds more
 // bodies than its LDS copy (rare).
 template <int D, int PREC>
@@ -834,20 +865,31 @@ template <int D, int PREC>
 __global__ void __launch_bounds__(kTileThreads) k_pass_tile(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
                                                             uint32_t colour, uint32_t cycle, TileCfg tc) {
   using VT = typename CellCopy<PREC>::type;
-  constexpr uint32_t kWords = sizeof(VT) / 4;  // 32-bit words per LDS entry
+  constexpr int AL = D - 1;  // the row axis (stride 1)
   if (cycle >= st->cycles) return;  // uniform over the grid
   const uint32_t b = blockIdx.x;
   const uint32_t tile = (b & 7u) * tc.xcd_chunk + (b >> 3);  // XCD-aware tile order
   if (tile >= tc.ntiles) return;                              // uniform over the block
   extern __shared__ __align__(16) unsigned char smem[];
-  uint32_t* cinfo = (uint32_t*)smem;  // per staged cell: count, then LDS offset << 16 | count
-  VT* lb = (VT*)(smem + ((tc.ncell * 4u + 15u) & ~15u));
-  uint32_t* lw = (uint32_t*)lb;  // first word of each entry: source slot while staging
+  const uint32_t R = tc.rows, EL = tc.E[AL], CW = EL + 1;
+  // per row: LDS offset, first run's start slot and length, second run's
+  // start slot; ri[R].x = the number of staged sites
+  uint4* ri = (uint4*)smem;
+  uint32_t* coff = (uint32_t*)(smem + 16u * (R + 1));  // per row: LDS offsets of its E + 1 cell boundaries
+  VT* lb = (VT*)(smem + ((16u * (R + 1) + 4u * R * CW + 15u) & ~15u));
   __shared__ double zx[257], zf[257];
-  __shared__ uint32_t wave_tot[kTileThreads / 64];
   __shared__ unsigned blk_acc;
   const uint32_t tid = threadIdx.x, NT = blockDim.x;
+  const uint32_t wid = tid >> 6, lane = tid & 63u;
   const uint32_t substep = st->substep0 + cycle;
+  // the apply's parameters, read once here: the state is written by atomics,
+  // so these are vector loads, and read later they would wait behind the
+  // staging loads
+  const uint64_t step = st->step;
+  const uint32_t seed = st->seed;
+  const double max_move = st->max_move;
+  double origin[3] = {0.0, 0.0, 0.0};
+  for (int a = 0; a < D; ++a) origin[a] = st->origin[a];
 
   uint32_t tcd[3] = {0, 0, 0};
   {
@@ -874,54 +916,89 @@ __global__ void __launch_bounds__(kTileThreads) k_pass_tile(Geometry g, ApplySta
       cc[a] = 2 * l + 1;
     }
   }
+  // along a row, cells [0, p) are spaces m0 .. m0 + p - 1 and cells [p, E)
+  // wrap round to spaces 0 .. E - p - 1 (p = E: no wrap)
+  const int32_t SL = g.shape[AL], ml = mlo[AL];
+  const uint32_t p = ml < 0 ? 1u : (uint32_t)min((int32_t)EL, SL - ml);
+  const uint32_t m0 = (uint32_t)(ml < 0 ? ml + SL : ml);
 
-  // (A) cell ranges of the box, the own range, the ziggurat tables
+  // (A) the own range, the row starts, the runs of each row (wave 0), the
+  // ziggurat tables
   uint32_t s0 = 0, s1 = 0;
   if (active) {
     s0 = w.start[s];
     s1 = w.start[s + 1];
   }
-  // cells tid + i NT: box digits e, advanced by the workgroup size with carries
-  uint32_t src[kCellsPerThread], a1[kCellsPerThread];
-  uint32_t e[3] = {0, 0, 0};
-  {
-    uint32_t r = tid;
-    for (int a = D - 1; a > 0; --a) {
-      e[a] = r % tc.E[a];
-      r /= tc.E[a];
-    }
-    e[0] = r;  // the leading digit is not reduced: e[0] >= E[0] past the box
-  }
+  // wave 0: the two runs of each of its rows (all loads issued before any is
+  // used; rows past the box load row R - 1 again)
+  const uint32_t rpl = (R + 63) / 64;  // rows per lane of wave 0 (uniform)
+  uint32_t rq[kRowsPerLane][4];
+  if (wid == 0) {
 #pragma unroll
-  for (int i = 0; i < kCellsPerThread; ++i) {
-    uint32_t ns = 0;
-    for (int a = 0; a < D; ++a) {
-      const int32_t sh = g.shape[a];
-      int32_t mm = mlo[a] + (int32_t)e[a];
-      mm = mm < 0 ? mm + sh : (mm >= sh ? mm - sh : mm);
-      ns += (uint32_t)mm * g.stride[a];
+    for (int u = 0; u < kRowsPerLane; ++u) {
+      const uint32_t r = min(lane * rpl + (uint32_t)u, R - 1);
+      uint32_t sb = 0, q = r;
+      for (int a = AL - 1; a >= 0; --a) {
+        const uint32_t e = a > 0 ? q % tc.E[a] : q;
+        q = a > 0 ? q / tc.E[a] : 0u;
+        const int32_t sh = g.shape[a];
+        int32_t mm = mlo[a] + (int32_t)e;
+        mm = mm < 0 ? mm + sh : (mm >= sh ? mm - sh : mm);
+        sb += (uint32_t)mm * g.stride[a];
+      }
+      rq[u][0] = w.start[sb + m0];
+      rq[u][1] = w.start[sb + m0 + p];
+      rq[u][2] = w.start[sb];
+      rq[u][3] = w.start[sb + (EL - p)];
     }
-    // cells past the box (only the first digit can run past it) load cell 0's range
-    if (e[0] >= tc.E[0]) ns = 0;
-    src[i] = w.start[ns];
-    a1[i] = w.start[ns + 1];
-    uint32_t carry = 0;
-    for (int a = D - 1; a >= 0; --a) {
-      e[a] += tc.step[a] + carry;
-      carry = 0;
-      if (a > 0 && e[a] >= tc.E[a]) {
-        e[a] -= tc.E[a];
-        carry = 1;
+  }
+  // value j of row r is the start of space m0 + j (j <= p) or j - p - 1
+  // (j > p) of the row; thread tid takes values tid + i NT, walking the
+  // digits (row digits, j) by the workgroup size
+  const uint32_t nvals = (R * (EL + 2) + NT - 1) / NT;  // uniform
+  uint32_t raw[kValsPerThread], key[kValsPerThread];
+  {
+    uint32_t d[3] = {0, 0, 0};
+    uint32_t r = tid;
+    for (int a = AL; a > 0; --a) {
+      const uint32_t rad = a == AL ? EL + 2 : tc.E[a];
+      d[a] = r % rad;
+      r /= rad;
+    }
+    d[0] = r;  // the leading digit is not reduced: d[0] >= E[0] past the box
+#pragma unroll
+    for (int i = 0; i < kValsPerThread; ++i) {
+      raw[i] = 0u;
+      key[i] = 0xffffffffu;
+      if ((uint32_t)i < nvals) {
+        if (d[0] < tc.E[0]) {
+          uint32_t sb = 0, row = 0;
+          for (int a = 0; a < AL; ++a) {
+            const int32_t sh = g.shape[a];
+            int32_t mm = mlo[a] + (int32_t)d[a];
+            mm = mm < 0 ? mm + sh : (mm >= sh ? mm - sh : mm);
+            sb += (uint32_t)mm * g.stride[a];
+            row = row * tc.E[a] + d[a];
+          }
+          const uint32_t j = d[AL];
+          raw[i] = w.start[j <= p ? sb + m0 + j : sb + (j - p - 1)];
+          key[i] = (row << 8) | j;
+        }
+        uint32_t carry = 0;
+        for (int a = AL; a >= 0; --a) {
+          d[a] += tc.step[a] + carry;
+          carry = 0;
+          const uint32_t rad = a == AL ? EL + 2 : tc.E[a];
+          if (a > 0 && d[a] >= rad) {
+            d[a] -= rad;
+            carry = 1;
+          }
+        }
       }
     }
   }
   if (D == 3) stage_zig(zx, zf);
   if (tid == 0) blk_acc = 0;
-#pragma unroll
-  for (int i = 0; i < kCellsPerThread; ++i) {
-    const uint32_t k = tid + i * NT;
-    if (k < tc.ncell) cinfo[k] = a1[i] - src[i];
-  }
   const uint32_t n = s1 - s0;
   const bool go = active && n != 0;
   // body indices of the own space (the chosen one is among them unless
@@ -932,102 +1009,125 @@ __global__ void __launch_bounds__(kTileThreads) k_pass_tile(Geometry g, ApplySta
     if (n > 1) ob1 = C.body[s0 + 1];
     if (n > 2) ob2 = C.body[s0 + 2];
   }
+  if (wid == 0) {
+    uint32_t len[kRowsPerLane], sum = 0;
+#pragma unroll
+    for (int u = 0; u < kRowsPerLane; ++u) {
+      const uint32_t r = lane * rpl + (uint32_t)u;
+      len[u] = (uint32_t)u < rpl && r < R ? (rq[u][1] - rq[u][0]) + (rq[u][3] - rq[u][2]) : 0u;
+      sum += len[u];
+    }
+    const uint32_t incl = wave_incl_scan(sum);
+    uint32_t off = incl - sum;
+#pragma unroll
+    for (int u = 0; u < kRowsPerLane; ++u) {
+      const uint32_t r = lane * rpl + (uint32_t)u;
+      if ((uint32_t)u < rpl && r < R) ri[r] = make_uint4(off, rq[u][0], rq[u][1] - rq[u][0], rq[u][2]);
+      off += len[u];
+    }
+    if (lane == 63) ri[R] = make_uint4(incl, 0u, 0u, 0u);
+  }
+  // everything of (A) arrives before the barrier, so that no wait in (B) has
+  // to count these loads
+  uint32_t held = ob0 ^ ob1 ^ ob2 ^ seed ^ (uint32_t)step ^ (uint32_t)__double2hiint(max_move);
+#pragma unroll
+  for (int i = 0; i < kValsPerThread; ++i) held ^= raw[i];
+  asm volatile("" ::"v"(held));
   __syncthreads();
 
-  // (B) LDS offsets: exclusive scan of the counts over contiguous chunks
-  uint32_t total = 0;
-  {
-    const uint32_t per = (tc.ncell + NT - 1) / NT;
-    const uint32_t k0 = min(tid * per, tc.ncell), k1 = min(k0 + per, tc.ncell);
-    uint32_t sum = 0;
-    for (uint32_t k = k0; k < k1; ++k) sum += cinfo[k];
-    const uint32_t incl = wave_incl_scan(sum);
-    const uint32_t wid = tid >> 6;
-    if (__lane_id() == 63) wave_tot[wid] = incl;
-    __syncthreads();
-    uint32_t off = incl - sum;
-    for (uint32_t wv = 0; wv < (NT >> 6); ++wv) {
-      const uint32_t t = wave_tot[wv];
-      if (wv < wid) off += t;
-      total += t;
-    }
-    for (uint32_t k = k0; k < k1; ++k) {
-      const uint32_t c = cinfo[k];
-      cinfo[k] = (min(off, 0xffffu) << 16) | c;
-      off += c;
+  // (B) staging, with the proposal drawn and the cell offsets written while
+  // the first loads are in flight
+  const uint32_t total = ri[R].x;
+  const bool overflow = total > tc.maxb;  // uniform over the block
+  float c32[3] = {0.0f, 0.0f, 0.0f};
+  if (PREC == HMC_MIXED) {
+    for (int a = 0; a < D; ++a) {
+      double c = origin[a] + ((double)mlo[a] + 0.5 * (double)tc.E[a]) * g.width[a];
+      c -= g.L[a] * floor((c + g.half[a]) / g.L[a]);
+      c32[a] = (float)c;
     }
   }
-  const bool overflow = total > tc.maxb;  // uniform over the block
+  // a half-wave per row: entry e of row r comes from slot b0 + e (e < l0) or
+  // b1 + e - l0; the first 32 entries of RB rows are loaded together
+  constexpr int RB = kRowBatch<PREC>;
+  const uint32_t hw = tid >> 5, hl = tid & 31u, NH = NT >> 5;
+  VT tmp[RB];
+  uint32_t dst[RB], src[RB];
+  uint32_t longrows = 0;  // rows of the batch with more than 32 entries
+  auto issue = [&](uint32_t r0) {
+    uint4 q[RB];
+    uint32_t end[RB];
+#pragma unroll
+    for (int u = 0; u < RB; ++u) {
+      const uint32_t r = min(r0 + (uint32_t)u * NH, R - 1);
+      q[u] = ri[r];
+      end[u] = ri[r + 1].x;
+    }
+    longrows = 0;
+#pragma unroll
+    for (int u = 0; u < RB; ++u) {
+      const uint32_t len = r0 + (uint32_t)u * NH < R ? end[u] - q[u].x : 0u;
+      dst[u] = 0xffffffffu;
+      src[u] = 0u;
+      if (hl < len) {
+        src[u] = hl < q[u].z ? q[u].y + hl : q[u].w + (hl - q[u].z);
+        dst[u] = q[u].x + hl;
+      }
+      if (len > 32u) longrows |= 1u << u;
+      if (dst[u] != 0xffffffffu) tmp[u] = CellCopy<PREC>::load(C, src[u]);
+    }
+  };
+  auto finish = [&](uint32_t r0) {
+#pragma unroll
+    for (int u = 0; u < RB; ++u)
+      if (dst[u] != 0xffffffffu) lb[dst[u]] = stage_entry<D, PREC>(g, c32, tmp[u], src[u]);
+    // entries past the first 32 of a row (rare)
+    while (longrows) {
+      const int u = __ffs(longrows) - 1;
+      longrows &= longrows - 1;
+      const uint32_t r = r0 + (uint32_t)u * NH;
+      const uint4 q = ri[r];
+      const uint32_t len = ri[r + 1].x - q.x;
+      for (uint32_t e = 32u + hl; e < len; e += 32u) {
+        const uint32_t sl = e < q.z ? q.y + e : q.w + (e - q.z);
+        lb[q.x + e] = stage_entry<D, PREC>(g, c32, CellCopy<PREC>::load(C, sl), sl);
+      }
+    }
+  };
+  if (!overflow) issue(hw);
   // the proposal (parallel_sweep.rs:240-251): choose, then Translate
-  Sfc64 rng = counter_rng(st->step, substep, st->seed, s);
+  Sfc64 rng = counter_rng(step, substep, seed, s);
   uint32_t pick = 0;
   double v[3] = {0.0, 0.0, 0.0};
   unsigned exp_slow = 0;
   if (go && !overflow) {
     pick = choose_index(rng, n);
-    ball_sample<D>(rng, st->max_move, zx, zf, v, &exp_slow);
-  }
-  __syncthreads();  // packed offsets visible
-
-  // (C) staging: source slot of every LDS entry, then one load per entry
-  if (!overflow) {
-#pragma unroll
-    for (int i = 0; i < kCellsPerThread; ++i) {
-      const uint32_t k = tid + i * NT;
-      if (k < tc.ncell) {
-        const uint32_t ci = cinfo[k];
-        const uint32_t o = ci >> 16, c = ci & 0xffffu;
-        for (uint32_t r = 0; r < c; ++r) lw[(o + r) * kWords] = src[i] + r;
-      }
-    }
+    ball_sample<D>(rng, max_move, zx, zf, v, &exp_slow);
   }
   uint32_t body = pick == 0 ? ob0 : (pick == 1 ? ob1 : ob2);
-  if (go && !overflow && pick > 2) body = C.body[s0 + pick];
   double4 bp = make_double4(0.0, 0.0, 0.0, 0.0);
-  if (go && !overflow) bp = B.bpos[body];
-  __syncthreads();
-  double c64[3] = {0.0, 0.0, 0.0};
-  float c32[3] = {0.0f, 0.0f, 0.0f};
-  if (PREC == HMC_MIXED) {
-    for (int a = 0; a < D; ++a) {
-      double c = st->origin[a] + ((double)mlo[a] + 0.5 * (double)tc.E[a]) * g.width[a];
-      c -= g.L[a] * floor((c + g.half[a]) / g.L[a]);
-      c64[a] = c;
-      c32[a] = (float)c;
-    }
-  }
   if (!overflow) {
-    for (uint32_t j0 = 0; j0 * NT < total; j0 += kStageBatch) {
-      VT tmp[kStageBatch];
-      uint32_t q[kStageBatch];
+    // LDS offsets of the cell boundaries: start - run start + run offset
 #pragma unroll
-      for (int j = 0; j < kStageBatch; ++j) {
-        const uint32_t idx = tid + (j0 + j) * NT;
-        q[j] = idx < total ? lw[idx * kWords] : 0u;
+    for (int i = 0; i < kValsPerThread; ++i) {
+      if (key[i] != 0xffffffffu) {
+        const uint32_t r = key[i] >> 8, j = key[i] & 0xffu;
+        const uint4 q = ri[r];
+        coff[r * CW + (j <= p ? j : j - 1)] = j <= p ? q.x + (raw[i] - q.y) : q.x + q.z + (raw[i] - q.w);
       }
-#pragma unroll
-      for (int j = 0; j < kStageBatch; ++j) tmp[j] = CellCopy<PREC>::load(C, q[j]);
-#pragma unroll
-      for (int j = 0; j < kStageBatch; ++j) {
-        const uint32_t idx = tid + (j0 + j) * NT;
-        if (idx < total) {
-          if (PREC == HMC_MIXED) {
-            const float4 p = *(const float4*)&tmp[j];
-            const float x[3] = {p.x, p.y, p.z};
-            float r[3] = {0.0f, 0.0f, 0.0f};
-            box_relative<D>(g, c32, x, r);
-            if ((uint32_t)__float_as_int(p.w) & kNoImages) r[0] = __int_as_float(0x7fc00000);
-            *(float4*)&lb[idx] = make_float4(r[0], r[1], r[2], __int_as_float((int)q[j]));
-          } else {
-            lb[idx] = tmp[j];
-          }
-        }
-      }
+    }
+    finish(hw);
+    // the chosen body's position, needed after the barrier
+    if (go && pick > 2) body = C.body[s0 + pick];
+    if (go) bp = B.bpos[body];
+    for (uint32_t r0 = hw + NH * RB; r0 < R; r0 += NH * RB) {
+      issue(r0);
+      finish(r0);
     }
   }
   __syncthreads();
 
-  // (D) the trials
+  // (C) the trials
   TrialOut o{0, 0, exp_slow};
   if (go && overflow) {
     o = trial_from_global<D, PREC>(g, st, B, C, w, zx, zf, s, s0, n, colour, substep);
@@ -1038,17 +1138,12 @@ __global__ void __launch_bounds__(kTileThreads) k_pass_tile(Geometry g, ApplySta
     for (int r = 0; r < NR; ++r) {
       const int i0 = D == 3 ? r / 3 : r;
       const int i1 = D == 3 ? r % 3 : 0;
-      uint32_t kb;
-      if (D == 3)
-        kb = ((cc[0] + i0 - 1) * tc.E[1] + (cc[1] + i1 - 1)) * tc.E[2] + cc[2] - 1;
-      else
-        kb = (cc[0] + i0 - 1) * tc.E[1] + cc[1] - 1;
-      const uint32_t a = cinfo[kb], z = cinfo[kb + 2];
-      rlo[r] = a >> 16;
-      rhi[r] = (z >> 16) + (z & 0xffffu);
+      const uint32_t row = D == 3 ? (cc[0] + i0 - 1) * tc.E[1] + (cc[1] + i1 - 1) : cc[0] + i0 - 1;
+      rlo[r] = coff[row * CW + cc[AL] - 1];
+      rhi[r] = coff[row * CW + cc[AL] + 2];
     }
-    const uint32_t kcen = D == 3 ? (cc[0] * tc.E[1] + cc[1]) * tc.E[2] + cc[2] : cc[0] * tc.E[1] + cc[1];
-    const uint32_t self = (cinfo[kcen] >> 16) + pick;
+    const uint32_t crow = D == 3 ? cc[0] * tc.E[1] + cc[1] : cc[0];
+    const uint32_t self = coff[crow * CW + cc[AL]] + pick;
     const uint32_t slot = s0 + pick;
     double x[3] = {bp.x, bp.y, bp.z};
     for (int a = 0; a < D; ++a) x[a] += v[a];
@@ -1143,6 +1238,522 @@ __global__ void __launch_bounds__(kTileThreads) k_pass_tile(Geometry g, ApplySta
   lane_add(&st->band_retests, o.retests);
   lane_add(&st->exp_slow, o.exp_slow);
   if (tid == 0 && overflow) atomicAdd(&st->tile_overflows, 1ull);
+  __syncthreads();
+  if (tid == 0 && blk_acc) atomicAdd(&w.acc_slots[b & (kAccSlots - 1)], (unsigned long long)blk_acc);
+}
+
+// ------------------------------------------------ persistent apply (mixed) --
+// One cooperative launch runs every cycle and colour of an apply.  Workgroup
+// b owns a region of 2 T spaces per axis (the last region along an axis may
+// be shorter) and keeps the region plus a one-space halo -- every cell its
+// trials can read -- in LDS for the whole apply: staged once, its own cells
+// updated in place by its accepted moves, and after the pass of colour c the
+// halo cells of colour c (the only ones other workgroups have changed)
+// reloaded from global memory.  A grid barrier separates the colour passes.
+//
+// The XCDs have separate L2 caches.  Writes are plain (they combine in the
+// writer's L2); at each grid barrier every wave waits for its stores, and one
+// lane per workgroup writes its XCD's L2 back before arriving (an agent-scope
+// release fence) and invalidates its caches after the last arrival (an
+// agent-scope acquire fence), so the site copies and FP64 sites written in
+// one colour pass are what every workgroup reads in the next.
+//
+// An LDS entry holds the site relative to the box centre in FP32 (NaN for a
+// site without images, as in the per-colour kernel) with the body index in w.
+constexpr uint32_t kBarrierSpins = 1u << 22;  // about 0.2 s of 128-cycle sleeps
+constexpr uint32_t kErrBarrier = 2u;          // ApplyState::error: a grid barrier timed out
+constexpr uint32_t kGridFlagStride = 64;      // words between release flags (256 B: separate memory channels)
+
+// Every workgroup of the launch arrives and waits until all have arrived gen
+// times.  The launch is cooperative, so all workgroups are resident.  Each
+// workgroup counts its arrival once on a shared counter and then polls a
+// flag of its own; the last to arrive sets every flag to gen.  (Polling the
+// shared counter instead keeps hundreds of loads queued on one memory
+// channel, which stalls every access of the workgroups still working.)  The
+// wait is bounded: past the bound the state's error word is set and the
+// barrier returns, so every wave still ends.
+__device__ __forceinline__ void grid_sync(ApplyState* st, uint32_t* gflags, uint32_t gen, unsigned* last) {
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have completed
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const uint32_t before = __hip_atomic_fetch_add(&st->grid_bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *last = before + 1 == gen * gridDim.x ? 1u : 0u;
+  }
+  __syncthreads();
+  if (*last && threadIdx.x < 64)
+    for (uint32_t i = threadIdx.x; i < gridDim.x; i += 64)
+      __hip_atomic_store(&gflags[i * kGridFlagStride], gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0) {
+    uint32_t spins = 0;
+    while (__hip_atomic_load(&gflags[blockIdx.x * kGridFlagStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+           gen) {
+      __builtin_amdgcn_s_sleep(2);
+      if (++spins > kBarrierSpins) {
+        atomicOr(&st->error, kErrBarrier);
+        break;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+// FP64 re-test of a band pair (rare; out of line), the site given by body.
+template <int D>
+__device__ __noinline__ bool retest64_body(const Geometry* gp, const double4* spos, uint32_t body, double t0,
+                                           double t1, double t2) {
+  const double t[3] = {t0, t1, t2};
+  return overlap64<D>(*gp, t, spos[body]);
+}
+
+// The trial of space s (slots [s0, s0 + n)) from global memory, for a region
+// whose box holds more bodies than its LDS copy (rare).  Same proposal and
+// decision as every other path.
+template <int D>
+__device__ __noinline__ TrialOut trial_global_dev(const Geometry* gp, ApplyState* st, double4* bpos, double4* spos,
+                                                  const uint32_t* cbody, float4* pos32, const uint32_t* start,
+                                                  hmc_trace_record* trace, unsigned long long trace_cap,
+                                                  const double* zx, const double* zf, uint32_t s, uint32_t s0,
+                                                  uint32_t n, uint32_t colour, uint32_t substep) {
+  const Geometry& g = *gp;
+  TrialOut o{0, 0, 0};
+  int32_t m[3] = {0, 0, 0};
+  for (int a = 0; a < D; ++a) m[a] = (int32_t)((s / g.stride[a]) % (uint32_t)g.shape[a]);
+  Sfc64 rng = counter_rng(st->step, substep, st->seed, s);
+  const uint32_t pick = choose_index(rng, n);
+  double v[3] = {0.0, 0.0, 0.0};
+  unsigned exp_slow = 0;
+  ball_sample<D>(rng, st->max_move, zx, zf, v, &exp_slow);
+  o.exp_slow = exp_slow;
+  const uint32_t slot = s0 + pick;
+  const uint32_t body = cbody[slot];
+  const double4 bp = bpos[body];
+  double x[3] = {bp.x, bp.y, bp.z};
+  for (int a = 0; a < D; ++a) x[a] += v[a];
+  for (int a = 0; a < D; ++a) x[a] = wrap1(x[a], g.L[a]);
+  bool accept = false;
+  if (space_of<D>(g, st->origin, x) == s) {
+    double t[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < D; ++a) t[a] = wrap1(x[a], g.L[a]);
+    float t32[3] = {0.0f, 0.0f, 0.0f};
+    for (int a = 0; a < D; ++a) t32[a] = (float)t[a];
+    bool hit = false;
+    for (int i0 = -1; i0 <= 1 && !hit; ++i0)
+      for (int i1 = -1; i1 <= 1 && !hit; ++i1)
+        for (int i2 = (D == 3 ? -1 : 0); i2 <= (D == 3 ? 1 : 0) && !hit; ++i2) {
+          const int di[3] = {i0, i1, i2};
+          uint32_t ns = 0;
+          for (int a = 0; a < D; ++a) {
+            const int32_t sh = g.shape[a];
+            int32_t mm = m[a] + di[a];
+            mm = mm < 0 ? mm + sh : (mm >= sh ? mm - sh : mm);
+            ns += (uint32_t)mm * g.stride[a];
+          }
+          const uint32_t q0 = start[ns], q1 = start[ns + 1];
+          for (uint32_t q = q0; q < q1 && !hit; ++q) {
+            if (q == slot) continue;
+            int r = overlap32<D>(g, t32, pos32[q]);
+            if (r == 2) {
+              ++o.retests;
+              r = retest64_body<D>(gp, spos, cbody[q], t[0], t[1], t[2]) ? 1 : 0;
+            }
+            hit = r != 0;
+          }
+        }
+    accept = !hit;
+    if (accept) {
+      double nsp[3] = {0.0, 0.0, 0.0};
+      for (int a = 0; a < D; ++a) nsp[a] = wrap1(t[a] + 0.0, g.L[a]);
+      const uint32_t fl = ghost_flags<D>(g, nsp);
+      bpos[body] = make_double4(t[0], t[1], t[2], 0.0);
+      spos[body] = pack_site(nsp, fl);
+      pos32[slot] = pack_site32(nsp, fl);
+    }
+  }
+  o.accepted = accept ? 1u : 0u;
+  if (trace) {
+    unsigned long long ti = atomicAdd((unsigned long long*)&st->trace_n, 1ull);
+    if (ti < trace_cap) {
+      hmc_trace_record rec;
+      rec.substep = substep;
+      rec.color = colour;
+      rec.space = s;
+      rec.body = (int64_t)body;
+      rec.status = accept ? 1 : 0;
+      rec.pad = 0;
+      trace[ti] = rec;
+    }
+  }
+  return o;
+}
+
+template <int D>
+__global__ void __launch_bounds__(kTileThreads) k_apply_tile(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
+                                                             TileCfg tc) {
+  using VT = float4;
+  constexpr int AL = D - 1;  // the row axis (stride 1)
+  constexpr uint32_t NCOL = 1u << D;
+  const uint32_t b = blockIdx.x;
+  const uint32_t tile = (b & 7u) * tc.xcd_chunk + (b >> 3);  // XCD-aware region order
+  const bool live = tile < tc.ntiles;  // padding workgroups still take part in every grid barrier
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t R = tc.rows, EL = tc.E[AL], CW = EL + 1;
+  uint4* ri = (uint4*)smem;  // as in k_pass_tile
+  uint32_t* coff = (uint32_t*)(smem + 16u * (R + 1));
+  VT* lb = (VT*)(smem + ((16u * (R + 1) + 4u * R * CW + 15u) & ~15u));
+  __shared__ double zx[257], zf[257];
+  __shared__ unsigned blk_acc, bar_last;
+  const uint32_t tid = threadIdx.x, NT = blockDim.x;
+  const uint32_t wid = tid >> 6, lane = tid & 63u;
+  const uint32_t cycles = st->cycles, substep0 = st->substep0;
+  const uint64_t step = st->step;
+  const uint32_t seed = st->seed;
+  const double max_move = st->max_move;
+  double origin[3] = {0.0, 0.0, 0.0};
+  for (int a = 0; a < D; ++a) origin[a] = st->origin[a];
+
+  // region: spaces org .. org + rl - 1 per axis; box: org - 1 .. org + 2 T
+  uint32_t tcd[3] = {0, 0, 0};
+  {
+    uint32_t r = live ? tile : 0u;
+    for (int a = D - 1; a >= 0; --a) {
+      tcd[a] = r % tc.nt[a];
+      r /= tc.nt[a];
+    }
+  }
+  int32_t org[3] = {0, 0, 0}, mlo[3] = {0, 0, 0}, rl[3] = {1, 1, 1};
+  for (int a = 0; a < D; ++a) {
+    org[a] = (int32_t)(2 * tcd[a] * tc.T[a]);
+    mlo[a] = org[a] - 1;
+    rl[a] = min((int32_t)(2 * tc.T[a]), g.shape[a] - org[a]);
+  }
+  const int32_t SL = g.shape[AL], ml = mlo[AL];
+  const uint32_t p = ml < 0 ? 1u : (uint32_t)min((int32_t)EL, SL - ml);
+  const uint32_t m0 = (uint32_t)(ml < 0 ? ml + SL : ml);
+  float c32[3] = {0.0f, 0.0f, 0.0f};
+  for (int a = 0; a < D; ++a) {
+    double c = origin[a] + ((double)mlo[a] + 0.5 * (double)tc.E[a]) * g.width[a];
+    c -= g.L[a] * floor((c + g.half[a]) / g.L[a]);
+    c32[a] = (float)c;
+  }
+  // this thread's trial digits (one space of each colour)
+  uint32_t hh[3] = {0, 0, 0};
+  {
+    uint32_t r = tid;
+    for (int a = D - 1; a >= 0; --a) {
+      hh[a] = r % tc.T[a];
+      r /= tc.T[a];
+    }
+  }
+  const bool has_trial = live && tid < tc.trials;
+
+  // ---- staging (once): rows, cell offsets, sites with their body indices
+  const uint32_t rpl = (R + 63) / 64;
+  uint32_t rq[kRowsPerLane][4];
+  if (live && wid == 0) {
+#pragma unroll
+    for (int u = 0; u < kRowsPerLane; ++u) {
+      const uint32_t r = min(lane * rpl + (uint32_t)u, R - 1);
+      uint32_t sb = 0, q = r;
+      for (int a = AL - 1; a >= 0; --a) {
+        const uint32_t e = a > 0 ? q % tc.E[a] : q;
+        q = a > 0 ? q / tc.E[a] : 0u;
+        const int32_t sh = g.shape[a];
+        int32_t mm = mlo[a] + (int32_t)e;
+        mm = mm < 0 ? mm + sh : (mm >= sh ? mm - sh : mm);
+        sb += (uint32_t)mm * g.stride[a];
+      }
+      rq[u][0] = w.start[sb + m0];
+      rq[u][1] = w.start[sb + m0 + p];
+      rq[u][2] = w.start[sb];
+      rq[u][3] = w.start[sb + (EL - p)];
+    }
+  }
+  const uint32_t nvals = (R * (EL + 2) + NT - 1) / NT;
+  uint32_t raw[kValsPerThread], key[kValsPerThread];
+  {
+    uint32_t d[3] = {0, 0, 0};
+    uint32_t r = tid;
+    for (int a = AL; a > 0; --a) {
+      const uint32_t rad = a == AL ? EL + 2 : tc.E[a];
+      d[a] = r % rad;
+      r /= rad;
+    }
+    d[0] = r;
+#pragma unroll
+    for (int i = 0; i < kValsPerThread; ++i) {
+      raw[i] = 0u;
+      key[i] = 0xffffffffu;
+      if (live && (uint32_t)i < nvals) {
+        if (d[0] < tc.E[0]) {
+          uint32_t sb = 0, row = 0;
+          for (int a = 0; a < AL; ++a) {
+            const int32_t sh = g.shape[a];
+            int32_t mm = mlo[a] + (int32_t)d[a];
+            mm = mm < 0 ? mm + sh : (mm >= sh ? mm - sh : mm);
+            sb += (uint32_t)mm * g.stride[a];
+            row = row * tc.E[a] + d[a];
+          }
+          const uint32_t j = d[AL];
+          raw[i] = w.start[j <= p ? sb + m0 + j : sb + (j - p - 1)];
+          key[i] = (row << 8) | j;
+        }
+        uint32_t carry = 0;
+        for (int a = AL; a >= 0; --a) {
+          d[a] += tc.step[a] + carry;
+          carry = 0;
+          const uint32_t rad = a == AL ? EL + 2 : tc.E[a];
+          if (a > 0 && d[a] >= rad) {
+            d[a] -= rad;
+            carry = 1;
+          }
+        }
+      }
+    }
+  }
+  if (D == 3) stage_zig(zx, zf);
+  if (tid == 0) blk_acc = 0;
+  if (live && wid == 0) {
+    uint32_t len[kRowsPerLane], sum = 0;
+#pragma unroll
+    for (int u = 0; u < kRowsPerLane; ++u) {
+      const uint32_t r = lane * rpl + (uint32_t)u;
+      len[u] = (uint32_t)u < rpl && r < R ? (rq[u][1] - rq[u][0]) + (rq[u][3] - rq[u][2]) : 0u;
+      sum += len[u];
+    }
+    const uint32_t incl = wave_incl_scan(sum);
+    uint32_t off = incl - sum;
+#pragma unroll
+    for (int u = 0; u < kRowsPerLane; ++u) {
+      const uint32_t r = lane * rpl + (uint32_t)u;
+      if ((uint32_t)u < rpl && r < R) ri[r] = make_uint4(off, rq[u][0], rq[u][1] - rq[u][0], rq[u][2]);
+      off += len[u];
+    }
+    if (lane == 63) ri[R] = make_uint4(incl, 0u, 0u, 0u);
+  }
+  __syncthreads();
+  const uint32_t total = live ? ri[R].x : 0u;
+  const bool overflow = total > tc.maxb;  // uniform over the workgroup
+  if (live && !overflow) {
+    const uint32_t hw = tid >> 5, hl = tid & 31u, NH = NT >> 5;
+    for (uint32_t r0 = hw; r0 < R; r0 += NH * 8u) {
+      float4 tmp[8];
+      uint32_t dst[8], src[8], bod[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t r = r0 + (uint32_t)u * NH;
+        dst[u] = 0xffffffffu;
+        src[u] = 0u;
+        if (r < R) {
+          const uint4 q = ri[r];
+          const uint32_t len = ri[r + 1].x - q.x;
+          if (hl < len) {
+            src[u] = hl < q.z ? q.y + hl : q.w + (hl - q.z);
+            dst[u] = q.x + hl;
+          }
+        }
+        if (dst[u] != 0xffffffffu) {
+          tmp[u] = C.pos32[src[u]];
+          bod[u] = C.body[src[u]];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (dst[u] != 0xffffffffu) lb[dst[u]] = stage_entry<D, HMC_MIXED>(g, c32, tmp[u], bod[u]);
+      // entries past the first 32 of a row (rare)
+#pragma unroll 1
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t r = r0 + (uint32_t)u * NH;
+        if (r >= R) continue;
+        const uint4 q = ri[r];
+        const uint32_t len = ri[r + 1].x - q.x;
+        for (uint32_t e = 32u + hl; e < len; e += 32u) {
+          const uint32_t sl = e < q.z ? q.y + e : q.w + (e - q.z);
+          lb[q.x + e] = stage_entry<D, HMC_MIXED>(g, c32, C.pos32[sl], C.body[sl]);
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kValsPerThread; ++i) {
+      if (key[i] != 0xffffffffu) {
+        const uint32_t r = key[i] >> 8, j = key[i] & 0xffu;
+        const uint4 q = ri[r];
+        coff[r * CW + (j <= p ? j : j - 1)] = j <= p ? q.x + (raw[i] - q.y) : q.x + q.z + (raw[i] - q.w);
+      }
+    }
+  }
+  __syncthreads();
+
+  // slot of LDS entry e of row r
+  auto slot_of = [&](uint32_t r, uint32_t e) -> uint32_t {
+    const uint4 q = ri[r];
+    const uint32_t o = e - q.x;
+    return o < q.z ? q.y + o : q.w + (o - q.z);
+  };
+
+  unsigned retests = 0, exp_slow = 0;
+  uint32_t gen = 0;
+  for (uint32_t k = 0; k < cycles; ++k) {
+    const uint32_t substep = substep0 + k;
+    for (uint32_t colour = 0; colour < NCOL; ++colour) {
+      // ---- the trials of this colour
+      bool active = has_trial;
+      uint32_t s = 0, cb[3] = {0, 0, 0};
+      for (int a = 0; a < D; ++a) {
+        const uint32_t l = 2 * hh[a] + ((colour >> (D - 1 - a)) & 1u);
+        if ((int32_t)l >= rl[a]) active = false;
+        s += (uint32_t)(org[a] + (int32_t)l) * g.stride[a];
+        cb[a] = l + 1;
+      }
+      unsigned accepted = 0;
+      uint32_t body = 0;
+      if (active && overflow) {
+        const uint32_t s0 = w.start[s], n = w.start[s + 1] - s0;
+        if (n) {
+          const TrialOut o = trial_global_dev<D>(g.self, st, B.bpos, B.spos, C.body, C.pos32, w.start,
+                                                 w.tracing ? w.trace : nullptr, w.trace_cap, zx, zf, s, s0, n,
+                                                 colour, substep);
+          accepted = o.accepted;
+          retests += o.retests;
+          exp_slow += o.exp_slow;
+        }
+      } else if (active) {
+        const uint32_t crow = D == 3 ? cb[0] * tc.E[1] + cb[1] : cb[0];
+        const uint32_t e0 = coff[crow * CW + cb[AL]], n = coff[crow * CW + cb[AL] + 1] - e0;
+        if (n) {
+          // positions of the space's first bodies, in flight during the draw
+          const uint32_t pb0 = __float_as_int(lb[e0].w);
+          const uint32_t pb1 = n > 1 ? (uint32_t)__float_as_int(lb[e0 + 1].w) : pb0;
+          const double4 q0 = B.bpos[pb0];
+          const double4 q1 = B.bpos[pb1];
+          Sfc64 rng = counter_rng(step, substep, seed, s);
+          const uint32_t pick = choose_index(rng, n);
+          double v[3] = {0.0, 0.0, 0.0};
+          unsigned es = 0;
+          ball_sample<D>(rng, max_move, zx, zf, v, &es);
+          exp_slow += es;
+          const uint32_t self = e0 + pick;
+          body = pick == 0 ? pb0 : (pick == 1 ? pb1 : (uint32_t)__float_as_int(lb[self].w));
+          double4 bp = pick == 0 ? q0 : q1;
+          if (pick > 1) bp = B.bpos[body];
+          double x[3] = {bp.x, bp.y, bp.z};
+          for (int a = 0; a < D; ++a) x[a] += v[a];
+          for (int a = 0; a < D; ++a) x[a] = wrap1(x[a], g.L[a]);  // generate_trial_moves: wrap(propose(..))
+          if (space_of<D>(g, origin, x) == s) {
+            constexpr int NR = D == 3 ? 9 : 3;
+            uint32_t rlo[NR], rhi[NR];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+              const int i0 = D == 3 ? r / 3 : r;
+              const int i1 = D == 3 ? r % 3 : 0;
+              const uint32_t row = D == 3 ? (cb[0] + i0 - 1) * tc.E[1] + (cb[1] + i1 - 1) : cb[0] + i0 - 1;
+              rlo[r] = coff[row * CW + cb[AL] - 1];
+              rhi[r] = coff[row * CW + cb[AL] + 2];
+            }
+            double t[3] = {0.0, 0.0, 0.0};
+            for (int a = 0; a < D; ++a) t[a] = wrap1(x[a], g.L[a]);
+            bool hit = false;
+            uint32_t rounds = 0;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) rounds = max(rounds, rhi[r] - rlo[r]);
+            float t32[3] = {0.0f, 0.0f, 0.0f}, tr[3] = {0.0f, 0.0f, 0.0f};
+            for (int a = 0; a < D; ++a) t32[a] = (float)t[a];
+            box_relative<D>(g, c32, t32, tr);
+            const float lo = tc.band_lo, hi = tc.band_hi;
+            for (uint32_t kk = 0; kk < rounds && !hit; ++kk) {
+              uint32_t amb = 0;
+#pragma unroll
+              for (int r = 0; r < NR; ++r) {
+                const uint32_t idx = rlo[r] + kk;
+                const bool valid = idx < rhi[r] && idx != self;
+                const float4 q = lb[valid ? idx : self];
+                const float dx = q.x - tr[0], dy = q.y - tr[1];
+                float d2 = dx * dx + dy * dy;
+                if (D == 3) {
+                  const float dz = q.z - tr[2];
+                  d2 = d2 + dz * dz;
+                }
+                hit |= valid && d2 < lo;
+                if (valid && !(d2 < lo) && !(d2 >= hi)) amb |= 1u << r;
+              }
+              while (amb && !hit) {
+                const int r = __ffs(amb) - 1;
+                amb &= amb - 1;
+                const float4 q = lb[rlo[r] + kk];
+                ++retests;
+                hit |= retest64_body<D>(g.self, B.spos, (uint32_t)__float_as_int(q.w), t[0], t[1], t[2]);
+              }
+            }
+            if (!hit) {
+              // update_body_properties (microstate.rs:985-1034)
+              double nsp[3] = {0.0, 0.0, 0.0};
+              for (int a = 0; a < D; ++a) nsp[a] = wrap1(t[a] + 0.0, g.L[a]);
+              const uint32_t fl = ghost_flags<D>(g, nsp);
+              const float4 s32 = pack_site32(nsp, fl);
+              const uint32_t slot = slot_of(crow, self);
+              B.bpos[body] = make_double4(t[0], t[1], t[2], 0.0);
+              B.spos[body] = pack_site(nsp, fl);
+              C.pos32[slot] = s32;
+              lb[self] = stage_entry<D, HMC_MIXED>(g, c32, s32, body);
+              accepted = 1;
+            }
+          }
+          if (w.tracing) {
+            unsigned long long ti = atomicAdd((unsigned long long*)&st->trace_n, 1ull);
+            if (ti < w.trace_cap) {
+              hmc_trace_record rec;
+              rec.substep = substep;
+              rec.color = colour;
+              rec.space = s;
+              rec.body = (int64_t)body;
+              rec.status = (int32_t)accepted;
+              rec.pad = 0;
+              w.trace[ti] = rec;
+            }
+          }
+        }
+      }
+      const unsigned long long mk = __ballot(accepted != 0);
+      if (__lane_id() == 0 && mk) atomicAdd(&blk_acc, (unsigned)__popcll(mk));
+      if (k + 1 == cycles && colour + 1 == NCOL) break;  // no later pass reads the halo
+      grid_sync(st, w.gflags, ++gen, &bar_last);
+      // ---- reload the halo cells of this colour: box cells b with
+      // b = 1 - off (mod 2) per axis whose space lies outside the region
+      if (live && !overflow) {
+        uint32_t nc = 1, cnt[3] = {1, 1, 1};
+        for (int a = 0; a < D; ++a) {
+          cnt[a] = tc.T[a] + 1;
+          nc *= cnt[a];
+        }
+        for (uint32_t i = tid; i < nc; i += NT) {
+          uint32_t r = i, bc[3] = {0, 0, 0};
+          bool own = true;
+          for (int a = D - 1; a >= 0; --a) {
+            const uint32_t j = r % cnt[a];
+            r /= cnt[a];
+            bc[a] = 2 * j + (((colour >> (D - 1 - a)) & 1u) ? 0u : 1u);
+            const int32_t l = (int32_t)bc[a] - 1;
+            if (l < 0 || l >= rl[a]) own = false;
+          }
+          if (own) continue;
+          const uint32_t row = D == 3 ? bc[0] * tc.E[1] + bc[1] : bc[0];
+          const uint32_t ea = coff[row * CW + bc[AL]], eb = coff[row * CW + bc[AL] + 1];
+          for (uint32_t e = ea; e < eb; ++e) {
+            const float4 v = C.pos32[slot_of(row, e)];
+            lb[e] = stage_entry<D, HMC_MIXED>(g, c32, v, (uint32_t)__float_as_int(lb[e].w));
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  lane_add(&st->band_retests, retests);
+  lane_add(&st->exp_slow, exp_slow);
+  if (tid == 0 && live && overflow) atomicAdd(&st->tile_overflows, 1ull);
+  // this workgroup's flag back to 0 for the next apply (no later barrier
+  // writes it: the last one has been passed)
+  if (tid == 0 && gen) __hip_atomic_store(&w.gflags[b * kGridFlagStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
   if (tid == 0 && blk_acc) atomicAdd(&w.acc_slots[b & (kAccSlots - 1)], (unsigned long long)blk_acc);
 }
@@ -1781,6 +2392,7 @@ __global__ void k_bsetup(const BatchRep* reps, ApplyState* sts, uint32_t R, cons
   }
   st->cycles = 0;
   st->nonempty = 0;
+  st->grid_bar = 0;
 }
 
 template <int D>
@@ -2072,6 +2684,9 @@ struct hmc_ctx {
   bool fast = false;  // every axis: >= 3 spaces and reach 1
   bool tiled = false;  // LDS-staged pass (requires fast)
   TileCfg tc{};
+  bool persist = false;  // one cooperative launch per apply (k_apply_tile)
+  TileCfg pc{};
+  uint32_t pgrid = 0, pthreads = 0;
   double* verts = nullptr;  // convex polyhedron vertices (device)
   Geometry* gdev = nullptr;  // device copy of g (g.self)
   uint32_t n_scan_blocks = 0;
@@ -2148,6 +2763,27 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
     e1 = take_event(c);
     (void)hipEventRecord(e1, s);
     c->build_events.push_back({e0, e1});
+  }
+  if constexpr (PREC == HMC_MIXED) {
+    if (c->persist) {
+      hipEvent_t p0 = nullptr, p1 = nullptr;
+      if (prof) {
+        p0 = take_event(c);
+        (void)hipEventRecord(p0, s);
+      }
+      ApplyState* stp = c->st;
+      void* args[] = {(void*)&g, (void*)&stp, (void*)&c->bodies, (void*)&c->cells, (void*)&c->w, (void*)&c->pc};
+      const hipError_t le = hipLaunchCooperativeKernel((const void*)k_apply_tile<D>, dim3(c->pgrid),
+                                                       dim3(c->pthreads), args, c->pc.lds_bytes, s);
+      if (le != hipSuccess) return fail(c, HMC_ERR_DEVICE, std::string("cooperative launch: ") + hipGetErrorString(le));
+      if (prof) {
+        p1 = take_event(c);
+        (void)hipEventRecord(p1, s);
+        c->pass_events.push_back({p0, p1});
+      }
+      c->applies += 1;
+      return HMC_OK;
+    }
   }
   const uint32_t blocks = grid_for(g.per_colour, kBlock);
   const uint32_t chunk = (blocks + 7) / 8;
@@ -2242,6 +2878,7 @@ int read_state(hmc_ctx* c, ApplyState* out) {
     collect_events(c, c->pass_events, c->pass_ms, c->pass_n);
     collect_events(c, c->build_events, c->build_ms, c->build_n);
   }
+  if (out->error & kErrBarrier) return fail(c, HMC_ERR_DEVICE, "a grid barrier of the persistent apply timed out");
   if (out->error) return fail(c, HMC_ERR_COVER, "a body lies outside the checkerboard");
   return HMC_OK;
 }
@@ -2368,9 +3005,10 @@ static bool contact_band(double E, double sigma, double sigma2, double edge2, in
 
 // Tile shape of the LDS-staged pass for a system with `occ` bodies per space
 // on a device with `cus` CUs.  Candidates: at most kTileThreads trials, at
-// most kCellsPerThread staged cells per thread, at most kTileLds bytes of
-// dynamic LDS with a body capacity of 1.2x the box's expected count plus 64
-// (a tile whose box holds more runs its trials from global memory).  A
+// most 64 kRowsPerLane rows and kValsPerThread row starts per thread, at most
+// kTileLds bytes of dynamic LDS with a body capacity of 1.2x the box's
+// expected count plus 64 (a tile whose box holds more runs its trials from
+// global memory).  A
 // workgroup's time is set by a fixed number of dependent memory phases, so
 // the shape that needs the fewest rounds of resident workgroups wins; among
 // those, the least launched lanes and staging.
@@ -2400,10 +3038,12 @@ static bool choose_tiles(const Geometry& g, int D, double occ, size_t vt, int cu
           ncell *= 2 * T[a] + 1;
           ntiles *= (g.half_shape[a] + T[a] - 1) / T[a];
         }
-        if (ncell > threads * kCellsPerThread || ncell > 0xffffu) continue;
+        const uint32_t EL = 2 * T[D - 1] + 1, rows = ncell / EL;
+        if (rows > 64u * kRowsPerLane || rows * (EL + 2) > threads * kValsPerThread) continue;
         const uint32_t maxb = (uint32_t)std::ceil(1.2 * occ * ncell) + 64;
-        const size_t lds = (((size_t)ncell * 4 + 15) & ~(size_t)15) + (size_t)maxb * vt;
-        if (lds > kTileLds || maxb > 0xffffu || maxb > threads * kStagePerLane) continue;
+        const size_t lds = ((16 * ((size_t)rows + 1) + 4 * (size_t)rows * (EL + 1) + 15) & ~(size_t)15) +
+                           (size_t)maxb * vt;
+        if (lds > kTileLds) continue;
         const uint32_t per_cu = std::max<uint32_t>(
             1, std::min<uint32_t>((uint32_t)(cu_lds / (lds + static_lds)), cu_waves / (threads / 64)));
         const double rounds = std::ceil((double)ntiles / ((double)per_cu * cus));
@@ -2419,16 +3059,92 @@ static bool choose_tiles(const Geometry& g, int D, double occ, size_t vt, int cu
           }
           out->trials = trials;
           out->ncell = ncell;
+          out->rows = rows;
           out->maxb = maxb;
           out->ntiles = ntiles;
           out->xcd_chunk = (ntiles + 7) / 8;
           out->lds_bytes = (uint32_t)lds;
           uint32_t r = threads;
           for (int a = D - 1; a > 0; --a) {
-            out->step[a] = r % out->E[a];
-            r /= out->E[a];
+            const uint32_t rad = a == D - 1 ? EL + 2 : out->E[a];
+            out->step[a] = r % rad;
+            r /= rad;
           }
           out->step[0] = r;  // the leading digit is not reduced
+          for (int a = D; a < 3; ++a) out->step[a] = 0;
+        }
+      }
+  return found;
+}
+
+// Region shape of the persistent apply (k_apply_tile).  A region is 2 T
+// spaces per axis; its box (the region and a one-space halo, 2 T + 2 spaces)
+// must span at most shape - 2 spaces.  Every region must be resident at once,
+// so a shape qualifies only if all its workgroups fit on the device (LDS: the
+// box's offsets and a site capacity of 1.1x its expected count plus 64).
+// Among those the fewest lanes per workgroup win -- a colour pass lasts as
+// long as one workgroup's trials -- then the most trials on those lanes, then
+// the least staging.
+static bool choose_regions(const Geometry& g, int D, double occ, int cus, TileCfg* out) {
+  uint32_t lim[3] = {1, 1, 1};
+  const uint32_t cap[3] = {16, 16, 32};
+  for (int a = 0; a < D; ++a) {
+    if (g.shape[a] < 6) return false;
+    lim[a] = std::min<uint32_t>(std::min<uint32_t>(g.half_shape[a], cap[a]), (uint32_t)(g.shape[a] - 4) / 2);
+  }
+  const size_t static_lds = 2 * 257 * sizeof(double) + 64;
+  const size_t cu_lds = 160 * 1024;
+  const uint32_t cu_waves = 8;  // two waves per SIMD at the kernel's register use
+  uint32_t best_threads = 0, best_trials = 0;
+  double best_work = 0.0;
+  bool found = false;
+  for (uint32_t t0 = 1; t0 <= lim[0]; ++t0)
+    for (uint32_t t1 = 1; t1 <= lim[1]; ++t1)
+      for (uint32_t t2 = 1; t2 <= lim[2]; ++t2) {
+        const uint32_t T[3] = {t0, t1, t2};
+        const uint32_t trials = T[0] * T[1] * T[2];
+        if (trials > (uint32_t)kTileThreads) continue;
+        const uint32_t threads = ((trials + 63) / 64) * 64;
+        uint32_t ncell = 1, ntiles = 1;
+        for (int a = 0; a < D; ++a) {
+          ncell *= 2 * T[a] + 2;
+          ntiles *= (g.half_shape[a] + T[a] - 1) / T[a];
+        }
+        const uint32_t EL = 2 * T[D - 1] + 2, rows = ncell / EL;
+        if (rows > 64u * kRowsPerLane || rows * (EL + 2) > threads * kValsPerThread) continue;
+        const uint32_t maxb = (uint32_t)std::ceil(1.1 * occ * ncell) + 64;
+        const size_t lds = ((16 * ((size_t)rows + 1) + 4 * (size_t)rows * (EL + 1) + 15) & ~(size_t)15) +
+                           (size_t)maxb * sizeof(float4);
+        if (lds > kTileLds) continue;
+        const uint32_t per_cu = std::min<uint32_t>((uint32_t)(cu_lds / (lds + static_lds)), cu_waves / (threads / 64));
+        const uint32_t grid = ((ntiles + 7) / 8) * 8;
+        if (per_cu == 0 || grid > per_cu * (uint32_t)cus) continue;
+        const double work = (double)ntiles * ncell;
+        if (!found || threads < best_threads || (threads == best_threads && trials > best_trials) ||
+            (threads == best_threads && trials == best_trials && work < best_work)) {
+          found = true;
+          best_threads = threads;
+          best_trials = trials;
+          best_work = work;
+          for (int a = 0; a < 3; ++a) {
+            out->T[a] = a < D ? T[a] : 1;
+            out->nt[a] = a < D ? (g.half_shape[a] + T[a] - 1) / T[a] : 1;
+            out->E[a] = a < D ? 2 * T[a] + 2 : 1;
+          }
+          out->trials = trials;
+          out->ncell = ncell;
+          out->rows = rows;
+          out->maxb = maxb;
+          out->ntiles = ntiles;
+          out->xcd_chunk = (ntiles + 7) / 8;
+          out->lds_bytes = (uint32_t)lds;
+          uint32_t r = threads;
+          for (int a = D - 1; a > 0; --a) {
+            const uint32_t rad = a == D - 1 ? EL + 2 : out->E[a];
+            out->step[a] = r % rad;
+            r /= rad;
+          }
+          out->step[0] = r;
           for (int a = D; a < 3; ++a) out->step[a] = 0;
         }
       }
@@ -2621,6 +3337,34 @@ int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
       if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->tc.lds_bytes) != hipSuccess) {
         delete c;
         return HMC_ERR_DEVICE;
+      }
+    }
+    int coop = 0;
+    if (c->tiled && cfg->precision == HMC_MIXED && !(cfg->flags & HMC_FLAG_PER_COLOUR) &&
+        hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, c->device) == hipSuccess && coop) {
+      TileCfg pc{};
+      if (choose_regions(g, D, (double)cfg->n_bodies / (double)spaces, cus, &pc)) {
+        double R = 0.0;
+        for (int a = 0; a < D; ++a) R = std::max(R, 0.5 * pc.E[a] * g.width[a] + g.width[a]);
+        const double Et =
+            (0x1p-24 * (5.1 * lmax + 2.0 * R + 2.0 * sigma) + 0x1p-53 * (1.5 * lmax + 2.0 * sigma)) * 1.01;
+        const void* fn = D == 2 ? (const void*)k_apply_tile<2> : (const void*)k_apply_tile<3>;
+        const uint32_t threads = ((pc.trials + 63) / 64) * 64;
+        int per_cu = 0;
+        if (contact_band(Et, sigma, g.sigma2, g.edge2, D, &pc.band_lo, &pc.band_hi) &&
+            hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pc.lds_bytes) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, pc.lds_bytes) == hipSuccess &&
+            (uint64_t)per_cu * (uint64_t)cus >= (uint64_t)pc.xcd_chunk * 8u) {
+          c->persist = true;
+          c->pc = pc;
+          c->pgrid = pc.xcd_chunk * 8u;
+          c->pthreads = threads;
+          const size_t fbytes = (size_t)c->pgrid * kGridFlagStride * sizeof(uint32_t);
+          if (hipMalloc((void**)&c->w.gflags, fbytes) != hipSuccess || hipMemset(c->w.gflags, 0, fbytes) != hipSuccess) {
+            delete c;
+            return HMC_ERR_DEVICE;
+          }
+        }
       }
     }
   }
@@ -2865,7 +3609,8 @@ int hmc_run_translate(hmc_ctx* c, uint64_t step0, uint32_t seed, double kT, doub
   int rc = read_state(c, &before);
   if (rc) return rc;
   const bool lj = c->g.model == HMC_LJ || c->g.model == HMC_LJ_SHIFTED;
-  const bool graphs = !(c->cfg.flags & HMC_FLAG_NO_GRAPH);
+  // (a cooperative launch is not captured: the persistent apply launches directly)
+  const bool graphs = !(c->cfg.flags & HMC_FLAG_NO_GRAPH) && !c->persist;
   for (uint32_t i = 0; i < n_steps; ++i) {
     const bool timed = c->profiling && c->applies % c->profile_period == 0;
     if (i == 0 || !graphs || timed) {
@@ -3008,8 +3753,8 @@ int hmc_get_stats(hmc_ctx* c, hmc_stats* out) {
   out->guard_cycles = s.guard_cycles;
   out->tile_overflows = s.tile_overflows;
   out->ambiguous = s.ambiguous;
-  out->tiled = c->tiled ? 1u : 0u;
-  for (int a = 0; a < 3; ++a) out->tile_shape[a] = c->tiled ? c->tc.T[a] : 0u;
+  out->tiled = c->persist ? 2u : (c->tiled ? 1u : 0u);
+  for (int a = 0; a < 3; ++a) out->tile_shape[a] = c->persist ? c->pc.T[a] : (c->tiled ? c->tc.T[a] : 0u);
   return HMC_OK;
 }
 
@@ -3082,7 +3827,7 @@ void hmc_destroy(hmc_ctx* c) {
                   (void*)c->gdev,
                   (void*)c->cells.pos64, (void*)c->w.cnt, (void*)c->w.start, (void*)c->w.elem_space,
                   (void*)c->w.elem_rank, (void*)c->w.tmp_body, (void*)c->w.block_sums, (void*)c->w.block_ne,
-                  (void*)c->w.acc_slots, (void*)c->w.trace, (void*)c->st})
+                  (void*)c->w.acc_slots, (void*)c->w.trace, (void*)c->w.gflags, (void*)c->st})
     if (p) (void)hipFree(p);
   collect_events(c, c->pass_events, c->pass_ms, c->pass_n);
   collect_events(c, c->build_events, c->build_ms, c->build_n);

@@ -86,8 +86,11 @@ enum {
   HMC_FLAG_GLOBAL_PASS = 1, /* run every trial from global memory instead of
                                staging each workgroup's neighbourhood box in LDS
                                (same decisions; for comparison and testing) */
-  HMC_FLAG_NO_GRAPH = 2     /* hmc_run_translate launches every apply directly
+  HMC_FLAG_NO_GRAPH = 2,    /* hmc_run_translate launches every apply directly
                                instead of replaying a captured hipGraph
+                               (same results) */
+  HMC_FLAG_PER_COLOUR = 4   /* MIXED hard shapes: one launch per colour pass
+                               instead of one persistent launch per apply
                                (same results) */
 };
 
@@ -111,7 +114,10 @@ typedef struct {
   uint64_t guard_cycles;    /* cycles run by the single-workgroup overflow guard */
   uint64_t tile_overflows;  /* LDS tiles whose box exceeded the staging capacity
                                (their trials ran from global memory) */
-  uint32_t tiled;           /* 1 if the pass stages neighbourhoods in LDS */
+  uint32_t tiled;           /* 1: one launch per colour pass, staging each
+                               workgroup's neighbourhood box in LDS; 2: one
+                               persistent launch per apply, each workgroup
+                               keeping its region and halo in LDS; 0: no LDS */
   uint32_t tile_shape[3];   /* colour spaces per workgroup along each axis */
   uint64_t ambiguous;       /* LJ: Metropolis decisions that the bound on the
                                summation-order difference could not settle

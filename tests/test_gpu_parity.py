@@ -102,15 +102,25 @@ def _sorted_trace(t):
     return t[order][["substep", "color", "space", "body", "status"]]
 
 
-def _mirror(hmc, ref, L, precision, lds_tiles=True):
+# pass layouts: one persistent launch per apply (mixed hard shapes; FP64
+# falls back to the per-colour tiles), one LDS-tiled launch per colour, or
+# every trial from global memory
+MODES = {"persist": dict(), "percolour": dict(per_colour=True), "global": dict(lds_tiles=False)}
+
+
+def expected_tiled(mode, precision):
+    return {"persist": 2 if precision == "mixed" else 1, "percolour": 1, "global": 0}[mode]
+
+
+def _mirror(hmc, ref, L, precision, mode="persist"):
     """GPU copy of the oracle microstate (its wrapped bodies and sites)."""
     b, s = ref.positions()
-    return hmc.GpuParallelSweep(L, b, sites=s, precision=precision, lds_tiles=lds_tiles)
+    return hmc.GpuParallelSweep(L, b, sites=s, precision=precision, **MODES[mode])
 
 
-def _replay(hmc, L, pos, d, n_applies, precision, seed=42, trace=True, step0=0, lds_tiles=True):
+def _replay(hmc, L, pos, d, n_applies, precision, seed=42, trace=True, step0=0, mode="persist"):
     ref = O.OracleSystem(L, pos, seed=seed, step=step0)
-    gpu = _mirror(hmc, ref, L, precision, lds_tiles)
+    gpu = _mirror(hmc, ref, L, precision, mode)
     if trace:
         gpu.trace(True, capacity=4 * len(pos) * 4 + 1024)
         ref.trace(True)
@@ -135,37 +145,38 @@ def _replay(hmc, L, pos, d, n_applies, precision, seed=42, trace=True, step0=0, 
     return st
 
 
-@pytest.mark.parametrize("lds_tiles", [True, False], ids=["lds", "global"])
+@pytest.mark.parametrize("mode", list(MODES))
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_replay_hard_disks(hmc, wl, precision, lds_tiles):
+def test_replay_hard_disks(hmc, wl, precision, mode):
     L, pos = wl.square_lattice_disks(n_side=64, packing_fraction=0.70)
     t = time.time()
-    st = _replay(hmc, L, pos, 0.15, 20, precision, lds_tiles=lds_tiles)
-    assert st["tiled"] == int(lds_tiles) and st["tile_overflows"] == 0
+    st = _replay(hmc, L, pos, 0.15, 20, precision, mode=mode)
+    assert st["tiled"] == expected_tiled(mode, precision) and st["tile_overflows"] == 0
     log(f"hard disks N=4096 {precision}: 20 applies bit-identical ({time.time() - t:.1f}s) {st}")
 
 
-@pytest.mark.parametrize("lds_tiles", [True, False], ids=["lds", "global"])
+@pytest.mark.parametrize("mode", list(MODES))
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_replay_hard_spheres(hmc, wl, precision, lds_tiles):
+def test_replay_hard_spheres(hmc, wl, precision, mode):
     L, pos = wl.fcc_spheres((10, 10, 10), packing_fraction=0.50)
     t = time.time()
-    st = _replay(hmc, L, pos, 0.08, 12, precision, lds_tiles=lds_tiles)
-    assert st["tiled"] == int(lds_tiles) and st["tile_overflows"] == 0
+    st = _replay(hmc, L, pos, 0.08, 12, precision, mode=mode)
+    assert st["tiled"] == expected_tiled(mode, precision) and st["tile_overflows"] == 0
     log(f"hard spheres N=4000 {precision}: 12 applies bit-identical ({time.time() - t:.1f}s) {st} "
         f"libm divergences (log, exp) {libm_divergences()}")
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_replay_clustered_tile_overflow(hmc, wl, precision):
+@pytest.mark.parametrize("precision,mode", [("fp64", "percolour"), ("mixed", "percolour"), ("mixed", "persist")])
+def test_replay_clustered_tile_overflow(hmc, wl, precision, mode):
     # a dense FCC block in one corner of a mostly empty box: the LDS capacity
-    # is sized from the mean occupancy, so the tiles around the block overflow
-    # and run their trials from global memory
+    # is sized from the mean occupancy, so the tiles (regions) around the
+    # block overflow and run their trials from global memory, next to tiles
+    # that run from LDS
     L1, block = wl.fcc_spheres((6, 6, 6), packing_fraction=0.50)
     L = np.array([24.5, 24.5, 24.5])  # 24 spaces of width 1.02 per axis
     pos = block - (-L1 / 2) + (-L / 2 + 0.75)
-    st = _replay(hmc, L, pos, 0.08, 8, precision)
-    assert st["tiled"] == 1 and st["tile_overflows"] > 0, st
+    st = _replay(hmc, L, pos, 0.08, 8, precision, mode=mode)
+    assert st["tiled"] == expected_tiled(mode, precision) and st["tile_overflows"] > 0, st
     log(f"clustered N={len(pos)} {precision}: 8 applies bit-identical {st}")
 
 
