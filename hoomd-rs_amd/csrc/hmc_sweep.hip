@@ -1263,6 +1263,7 @@ __global__ void __launch_bounds__(kTileThreads) k_pass_tile(Geometry g, ApplySta
 constexpr uint32_t kBarrierSpins = 1u << 22;  // about 0.2 s of 128-cycle sleeps
 constexpr uint32_t kErrBarrier = 2u;          // ApplyState::error: a grid barrier timed out
 constexpr uint32_t kGridFlagStride = 64;      // words between release flags (256 B: separate memory channels)
+constexpr int kRefreshHeld = 4;               // halo entries per thread whose reloads overlap the next draw
 
 // Every workgroup of the launch arrives and waits until all have arrived gen
 // times.  The launch is cooperative, so all workgroups are resident.  Each
@@ -1594,159 +1595,200 @@ __global__ void __launch_bounds__(kTileThreads) k_apply_tile(Geometry g, ApplySt
 
   unsigned retests = 0, exp_slow = 0;
   uint32_t gen = 0;
-  for (uint32_t k = 0; k < cycles; ++k) {
-    const uint32_t substep = substep0 + k;
-    for (uint32_t colour = 0; colour < NCOL; ++colour) {
-      // ---- the trials of this colour
-      bool active = has_trial;
-      uint32_t s = 0, cb[3] = {0, 0, 0};
+  // The proposal of pass i (cycle i / 2^D, colour i mod 2^D) depends only on
+  // the counter, the space's body count and the chosen body's position, which
+  // no other colour changes; it is drawn while the previous colour's halo is
+  // being reloaded.
+  bool pgo = false;
+  uint32_t ps = 0, pcrow = 0, pe0 = 0, pn = 0, ppick = 0, pbody = 0;
+  uint32_t pcb[3] = {0, 0, 0};
+  double pv[3] = {0.0, 0.0, 0.0};
+  double4 pbp = make_double4(0.0, 0.0, 0.0, 0.0);
+  auto propose = [&](uint32_t i) {
+    const uint32_t colour = i & (NCOL - 1), substep = substep0 + (i >> D);
+    bool active = has_trial && !overflow;
+    uint32_t s = 0;
+    for (int a = 0; a < D; ++a) {
+      const uint32_t l = 2 * hh[a] + ((colour >> (D - 1 - a)) & 1u);
+      if ((int32_t)l >= rl[a]) active = false;
+      s += (uint32_t)(org[a] + (int32_t)l) * g.stride[a];
+      pcb[a] = l + 1;
+    }
+    ps = s;
+    pgo = false;
+    if (!active) return;
+    pcrow = D == 3 ? pcb[0] * tc.E[1] + pcb[1] : pcb[0];
+    pe0 = coff[pcrow * CW + pcb[AL]];
+    pn = coff[pcrow * CW + pcb[AL] + 1] - pe0;
+    if (!pn) return;
+    pgo = true;
+    // positions of the space's first bodies, in flight during the draw
+    const uint32_t pb0 = __float_as_int(lb[pe0].w);
+    const uint32_t pb1 = pn > 1 ? (uint32_t)__float_as_int(lb[pe0 + 1].w) : pb0;
+    const double4 q0 = B.bpos[pb0];
+    const double4 q1 = B.bpos[pb1];
+    Sfc64 rng = counter_rng(step, substep, seed, s);
+    ppick = choose_index(rng, pn);
+    unsigned es = 0;
+    ball_sample<D>(rng, max_move, zx, zf, pv, &es);
+    exp_slow += es;
+    pbody = ppick == 0 ? pb0 : (ppick == 1 ? pb1 : (uint32_t)__float_as_int(lb[pe0 + ppick].w));
+    pbp = ppick == 0 ? q0 : q1;
+    if (ppick > 1) pbp = B.bpos[pbody];
+  };
+
+  const uint32_t npass = cycles * NCOL;
+  if (npass) propose(0);
+  for (uint32_t i = 0; i < npass; ++i) {
+    const uint32_t colour = i & (NCOL - 1), substep = substep0 + (i >> D);
+    unsigned accepted = 0;
+    if (has_trial && overflow) {
+      bool active = true;
+      uint32_t s = 0;
       for (int a = 0; a < D; ++a) {
         const uint32_t l = 2 * hh[a] + ((colour >> (D - 1 - a)) & 1u);
         if ((int32_t)l >= rl[a]) active = false;
         s += (uint32_t)(org[a] + (int32_t)l) * g.stride[a];
-        cb[a] = l + 1;
       }
-      unsigned accepted = 0;
-      uint32_t body = 0;
-      if (active && overflow) {
-        const uint32_t s0 = w.start[s], n = w.start[s + 1] - s0;
-        if (n) {
-          const TrialOut o = trial_global_dev<D>(g.self, st, B.bpos, B.spos, C.body, C.pos32, w.start,
-                                                 w.tracing ? w.trace : nullptr, w.trace_cap, zx, zf, s, s0, n,
-                                                 colour, substep);
-          accepted = o.accepted;
-          retests += o.retests;
-          exp_slow += o.exp_slow;
+      const uint32_t s0 = w.start[s], n = w.start[s + 1] - s0;
+      if (active && n) {
+        const TrialOut o = trial_global_dev<D>(g.self, st, B.bpos, B.spos, C.body, C.pos32, w.start,
+                                               w.tracing ? w.trace : nullptr, w.trace_cap, zx, zf, s, s0, n,
+                                               colour, substep);
+        accepted = o.accepted;
+        retests += o.retests;
+        exp_slow += o.exp_slow;
+      }
+    } else if (pgo) {
+      // ---- the trial of this pass from LDS
+      const uint32_t s = ps, self = pe0 + ppick, body = pbody;
+      double x[3] = {pbp.x, pbp.y, pbp.z};
+      for (int a = 0; a < D; ++a) x[a] += pv[a];
+      for (int a = 0; a < D; ++a) x[a] = wrap1(x[a], g.L[a]);  // generate_trial_moves: wrap(propose(..))
+      if (space_of<D>(g, origin, x) == s) {
+        constexpr int NR = D == 3 ? 9 : 3;
+        uint32_t rlo[NR], rhi[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          const int i0 = D == 3 ? r / 3 : r;
+          const int i1 = D == 3 ? r % 3 : 0;
+          const uint32_t row = D == 3 ? (pcb[0] + i0 - 1) * tc.E[1] + (pcb[1] + i1 - 1) : pcb[0] + i0 - 1;
+          rlo[r] = coff[row * CW + pcb[AL] - 1];
+          rhi[r] = coff[row * CW + pcb[AL] + 2];
         }
-      } else if (active) {
-        const uint32_t crow = D == 3 ? cb[0] * tc.E[1] + cb[1] : cb[0];
-        const uint32_t e0 = coff[crow * CW + cb[AL]], n = coff[crow * CW + cb[AL] + 1] - e0;
-        if (n) {
-          // positions of the space's first bodies, in flight during the draw
-          const uint32_t pb0 = __float_as_int(lb[e0].w);
-          const uint32_t pb1 = n > 1 ? (uint32_t)__float_as_int(lb[e0 + 1].w) : pb0;
-          const double4 q0 = B.bpos[pb0];
-          const double4 q1 = B.bpos[pb1];
-          Sfc64 rng = counter_rng(step, substep, seed, s);
-          const uint32_t pick = choose_index(rng, n);
-          double v[3] = {0.0, 0.0, 0.0};
-          unsigned es = 0;
-          ball_sample<D>(rng, max_move, zx, zf, v, &es);
-          exp_slow += es;
-          const uint32_t self = e0 + pick;
-          body = pick == 0 ? pb0 : (pick == 1 ? pb1 : (uint32_t)__float_as_int(lb[self].w));
-          double4 bp = pick == 0 ? q0 : q1;
-          if (pick > 1) bp = B.bpos[body];
-          double x[3] = {bp.x, bp.y, bp.z};
-          for (int a = 0; a < D; ++a) x[a] += v[a];
-          for (int a = 0; a < D; ++a) x[a] = wrap1(x[a], g.L[a]);  // generate_trial_moves: wrap(propose(..))
-          if (space_of<D>(g, origin, x) == s) {
-            constexpr int NR = D == 3 ? 9 : 3;
-            uint32_t rlo[NR], rhi[NR];
+        double t[3] = {0.0, 0.0, 0.0};
+        for (int a = 0; a < D; ++a) t[a] = wrap1(x[a], g.L[a]);
+        bool hit = false;
+        uint32_t rounds = 0;
 #pragma unroll
-            for (int r = 0; r < NR; ++r) {
-              const int i0 = D == 3 ? r / 3 : r;
-              const int i1 = D == 3 ? r % 3 : 0;
-              const uint32_t row = D == 3 ? (cb[0] + i0 - 1) * tc.E[1] + (cb[1] + i1 - 1) : cb[0] + i0 - 1;
-              rlo[r] = coff[row * CW + cb[AL] - 1];
-              rhi[r] = coff[row * CW + cb[AL] + 2];
-            }
-            double t[3] = {0.0, 0.0, 0.0};
-            for (int a = 0; a < D; ++a) t[a] = wrap1(x[a], g.L[a]);
-            bool hit = false;
-            uint32_t rounds = 0;
+        for (int r = 0; r < NR; ++r) rounds = max(rounds, rhi[r] - rlo[r]);
+        float t32[3] = {0.0f, 0.0f, 0.0f}, tr[3] = {0.0f, 0.0f, 0.0f};
+        for (int a = 0; a < D; ++a) t32[a] = (float)t[a];
+        box_relative<D>(g, c32, t32, tr);
+        const float lo = tc.band_lo, hi = tc.band_hi;
+        for (uint32_t kk = 0; kk < rounds && !hit; ++kk) {
+          uint32_t amb = 0;
 #pragma unroll
-            for (int r = 0; r < NR; ++r) rounds = max(rounds, rhi[r] - rlo[r]);
-            float t32[3] = {0.0f, 0.0f, 0.0f}, tr[3] = {0.0f, 0.0f, 0.0f};
-            for (int a = 0; a < D; ++a) t32[a] = (float)t[a];
-            box_relative<D>(g, c32, t32, tr);
-            const float lo = tc.band_lo, hi = tc.band_hi;
-            for (uint32_t kk = 0; kk < rounds && !hit; ++kk) {
-              uint32_t amb = 0;
-#pragma unroll
-              for (int r = 0; r < NR; ++r) {
-                const uint32_t idx = rlo[r] + kk;
-                const bool valid = idx < rhi[r] && idx != self;
-                const float4 q = lb[valid ? idx : self];
-                const float dx = q.x - tr[0], dy = q.y - tr[1];
-                float d2 = dx * dx + dy * dy;
-                if (D == 3) {
-                  const float dz = q.z - tr[2];
-                  d2 = d2 + dz * dz;
-                }
-                hit |= valid && d2 < lo;
-                if (valid && !(d2 < lo) && !(d2 >= hi)) amb |= 1u << r;
-              }
-              while (amb && !hit) {
-                const int r = __ffs(amb) - 1;
-                amb &= amb - 1;
-                const float4 q = lb[rlo[r] + kk];
-                ++retests;
-                hit |= retest64_body<D>(g.self, B.spos, (uint32_t)__float_as_int(q.w), t[0], t[1], t[2]);
-              }
+          for (int r = 0; r < NR; ++r) {
+            const uint32_t idx = rlo[r] + kk;
+            const bool valid = idx < rhi[r] && idx != self;
+            const float4 q = lb[valid ? idx : self];
+            const float dx = q.x - tr[0], dy = q.y - tr[1];
+            float d2 = dx * dx + dy * dy;
+            if (D == 3) {
+              const float dz = q.z - tr[2];
+              d2 = d2 + dz * dz;
             }
-            if (!hit) {
-              // update_body_properties (microstate.rs:985-1034)
-              double nsp[3] = {0.0, 0.0, 0.0};
-              for (int a = 0; a < D; ++a) nsp[a] = wrap1(t[a] + 0.0, g.L[a]);
-              const uint32_t fl = ghost_flags<D>(g, nsp);
-              const float4 s32 = pack_site32(nsp, fl);
-              const uint32_t slot = slot_of(crow, self);
-              B.bpos[body] = make_double4(t[0], t[1], t[2], 0.0);
-              B.spos[body] = pack_site(nsp, fl);
-              C.pos32[slot] = s32;
-              lb[self] = stage_entry<D, HMC_MIXED>(g, c32, s32, body);
-              accepted = 1;
-            }
+            hit |= valid && d2 < lo;
+            if (valid && !(d2 < lo) && !(d2 >= hi)) amb |= 1u << r;
           }
-          if (w.tracing) {
-            unsigned long long ti = atomicAdd((unsigned long long*)&st->trace_n, 1ull);
-            if (ti < w.trace_cap) {
-              hmc_trace_record rec;
-              rec.substep = substep;
-              rec.color = colour;
-              rec.space = s;
-              rec.body = (int64_t)body;
-              rec.status = (int32_t)accepted;
-              rec.pad = 0;
-              w.trace[ti] = rec;
-            }
+          while (amb && !hit) {
+            const int r = __ffs(amb) - 1;
+            amb &= amb - 1;
+            const float4 q = lb[rlo[r] + kk];
+            ++retests;
+            hit |= retest64_body<D>(g.self, B.spos, (uint32_t)__float_as_int(q.w), t[0], t[1], t[2]);
           }
+        }
+        if (!hit) {
+          // update_body_properties (microstate.rs:985-1034)
+          double nsp[3] = {0.0, 0.0, 0.0};
+          for (int a = 0; a < D; ++a) nsp[a] = wrap1(t[a] + 0.0, g.L[a]);
+          const uint32_t fl = ghost_flags<D>(g, nsp);
+          const float4 s32 = pack_site32(nsp, fl);
+          const uint32_t slot = slot_of(pcrow, self);
+          B.bpos[body] = make_double4(t[0], t[1], t[2], 0.0);
+          B.spos[body] = pack_site(nsp, fl);
+          C.pos32[slot] = s32;
+          lb[self] = stage_entry<D, HMC_MIXED>(g, c32, s32, body);
+          accepted = 1;
         }
       }
-      const unsigned long long mk = __ballot(accepted != 0);
-      if (__lane_id() == 0 && mk) atomicAdd(&blk_acc, (unsigned)__popcll(mk));
-      if (k + 1 == cycles && colour + 1 == NCOL) break;  // no later pass reads the halo
-      grid_sync(st, w.gflags, ++gen, &bar_last);
-      // ---- reload the halo cells of this colour: box cells b with
-      // b = 1 - off (mod 2) per axis whose space lies outside the region
-      if (live && !overflow) {
-        uint32_t nc = 1, cnt[3] = {1, 1, 1};
-        for (int a = 0; a < D; ++a) {
-          cnt[a] = tc.T[a] + 1;
-          nc *= cnt[a];
+      if (w.tracing) {
+        unsigned long long ti = atomicAdd((unsigned long long*)&st->trace_n, 1ull);
+        if (ti < w.trace_cap) {
+          hmc_trace_record rec;
+          rec.substep = substep;
+          rec.color = colour;
+          rec.space = s;
+          rec.body = (int64_t)body;
+          rec.status = (int32_t)accepted;
+          rec.pad = 0;
+          w.trace[ti] = rec;
         }
-        for (uint32_t i = tid; i < nc; i += NT) {
-          uint32_t r = i, bc[3] = {0, 0, 0};
-          bool own = true;
-          for (int a = D - 1; a >= 0; --a) {
-            const uint32_t j = r % cnt[a];
-            r /= cnt[a];
-            bc[a] = 2 * j + (((colour >> (D - 1 - a)) & 1u) ? 0u : 1u);
-            const int32_t l = (int32_t)bc[a] - 1;
-            if (l < 0 || l >= rl[a]) own = false;
-          }
-          if (own) continue;
-          const uint32_t row = D == 3 ? bc[0] * tc.E[1] + bc[1] : bc[0];
-          const uint32_t ea = coff[row * CW + bc[AL]], eb = coff[row * CW + bc[AL] + 1];
-          for (uint32_t e = ea; e < eb; ++e) {
-            const float4 v = C.pos32[slot_of(row, e)];
+      }
+    }
+    const unsigned long long mk = __ballot(accepted != 0);
+    if (__lane_id() == 0 && mk) atomicAdd(&blk_acc, (unsigned)__popcll(mk));
+    if (i + 1 == npass) break;  // no later pass reads the halo
+    grid_sync(st, w.gflags, ++gen, &bar_last);
+    // ---- reload the halo cells of this colour (box cells b with b = 1 - off
+    // (mod 2) per axis whose space lies outside the region): the first
+    // kRefreshHeld entries' loads are issued before the next proposal is
+    // drawn and written after it
+    uint32_t hdst[kRefreshHeld];
+    float4 hval[kRefreshHeld];
+    uint32_t nheld = 0;
+    if (live && !overflow) {
+      uint32_t nc = 1, cnt[3] = {1, 1, 1};
+      for (int a = 0; a < D; ++a) {
+        cnt[a] = tc.T[a] + 1;
+        nc *= cnt[a];
+      }
+      for (uint32_t ci = tid; ci < nc; ci += NT) {
+        uint32_t r = ci, bc[3] = {0, 0, 0};
+        bool own = true;
+        for (int a = D - 1; a >= 0; --a) {
+          const uint32_t j = r % cnt[a];
+          r /= cnt[a];
+          bc[a] = 2 * j + (((colour >> (D - 1 - a)) & 1u) ? 0u : 1u);
+          const int32_t l = (int32_t)bc[a] - 1;
+          if (l < 0 || l >= rl[a]) own = false;
+        }
+        if (own) continue;
+        const uint32_t row = D == 3 ? bc[0] * tc.E[1] + bc[1] : bc[0];
+        const uint32_t ea = coff[row * CW + bc[AL]], eb = coff[row * CW + bc[AL] + 1];
+        for (uint32_t e = ea; e < eb; ++e) {
+          const float4 v = C.pos32[slot_of(row, e)];
+          if (nheld < (uint32_t)kRefreshHeld) {
+#pragma unroll
+            for (int h = 0; h < kRefreshHeld; ++h)
+              if ((uint32_t)h == nheld) {
+                hdst[h] = e;
+                hval[h] = v;
+              }
+            ++nheld;
+          } else {
             lb[e] = stage_entry<D, HMC_MIXED>(g, c32, v, (uint32_t)__float_as_int(lb[e].w));
           }
         }
       }
-      __syncthreads();
     }
+    propose(i + 1);
+#pragma unroll
+    for (int h = 0; h < kRefreshHeld; ++h)
+      if ((uint32_t)h < nheld) lb[hdst[h]] = stage_entry<D, HMC_MIXED>(g, c32, hval[h], (uint32_t)__float_as_int(lb[hdst[h]].w));
+    __syncthreads();
   }
   lane_add(&st->band_retests, retests);
   lane_add(&st->exp_slow, exp_slow);
@@ -3340,7 +3382,7 @@ int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
       }
     }
     int coop = 0;
-    if (c->tiled && cfg->precision == HMC_MIXED && !(cfg->flags & HMC_FLAG_PER_COLOUR) &&
+    if (c->tiled && cfg->precision == HMC_MIXED && (cfg->flags & HMC_FLAG_PERSISTENT) &&
         hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, c->device) == hipSuccess && coop) {
       TileCfg pc{};
       if (choose_regions(g, D, (double)cfg->n_bodies / (double)spaces, cus, &pc)) {
