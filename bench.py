@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison run")
     ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--pass-layout", choices=["percolour", "persistent"], default="persistent",
+                    help="mixed precision: one launch per colour pass, or one persistent launch per apply")
     ap.add_argument("--profile-period", type=int, default=8,
                     help="time the kernels of every k-th timed apply with HIP events")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -73,8 +75,8 @@ def pass_bytes(precision, n, n_spaces, nonempty, accepted_per_pass, dim=3):
     return 4 * (n_spaces + 1) + site * n + 36 * active + write * accepted_per_pass
 
 
-def run_gpu(hmc, L, body, site, precision, steps, warmup, seed, kT, d, period):
-    sim = hmc.GpuParallelSweep(L, body, sites=site, precision=precision)
+def run_gpu(hmc, L, body, site, precision, steps, warmup, seed, kT, d, period, persistent=False):
+    sim = hmc.GpuParallelSweep(L, body, sites=site, precision=precision, persistent=persistent)
     # warm-up applies: steps 0 .. warmup-1
     sim.run(0, seed, kT, d, warmup)
     sim.profile(True, period)
@@ -90,15 +92,28 @@ def run_gpu(hmc, L, body, site, precision, steps, warmup, seed, kT, d, period):
 
 
 def measured_traffic(kernel):
-    """HBM bytes per launch of the pass kernel from the committed rocprofv3
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain copy), or None."""
-    f = ROOT / "profiles" / "pass_tile_c2_hbm_traffic.md"
-    if not f.exists():
+    name = {"k_pass_tile": "pass_tile_c2_hbm_traffic.md", "k_apply_tile": "apply_tile_c2_hbm_traffic.md"}
+    f = ROOT / "profiles" / name.get(kernel.split("<")[0], "")
+    if not f.is_file():
         return None
     for line in f.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:") and kernel.startswith("k_pass_tile"):
+        if line.startswith("traffic_bytes_per_launch:"):
             return float(line.split(":", 1)[1])
     return None
+
+
+def apply_bytes(n, n_spaces, trials, accepted):
+    """Algorithmic HBM bytes of one persistent apply launch (every colour pass
+    of every cycle).
+
+    Reads: the cell start offsets of every space, every site's FP32 copy and
+    body index once (the region boxes are staged once; what the halo reloads
+    after each pass re-read is not counted), and for each trial the chosen
+    body's FP64 position (32 B).  Writes: per accepted move the body position
+    and site (2 x 32 B) and the FP32 copy (16 B)."""
+    return 4 * (n_spaces + 1) + 20 * n + 32 * trials + 80 * accepted
 
 
 def cpu_baseline(L, pos, seconds, seed, kT, d):
@@ -134,7 +149,8 @@ def main():
     body, site = ref.positions()
     del ref
 
-    mixed = run_gpu(hmc, L, body, site, "mixed", args.steps, args.warmup, args.seed, kT, d, args.profile_period)
+    mixed = run_gpu(hmc, L, body, site, "mixed", args.steps, args.warmup, args.seed, kT, d, args.profile_period,
+                    persistent=args.pass_layout == "persistent")
     trials = mixed["accepted"] + mixed["rejected"]
     value = trials / mixed["elapsed"]
     out = {
@@ -143,11 +159,13 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
         "config": {"workload": "hard spheres 3D, FCC 4*63^3, packing fraction 0.50, translate d=0.08, "
                                "ParallelSweep apply per step", "n_bodies": n, "box_edge": float(L[0]),
-                   "precision": "mixed FP32 pair test with FP64 band re-test", "seed": args.seed},
+                   "precision": "mixed FP32 pair test with FP64 band re-test", "seed": args.seed,
+                   "pass_layout": args.pass_layout},
         "sweeps_per_s": value / n,
         "acceptance": mixed["accepted"] / max(trials, 1),
     }
-    # roofline of the dominant kernel (the per-colour trial pass)
+    # roofline of the dominant kernel: the persistent apply (one launch per
+    # apply) or the per-colour trial pass
     n_spaces = 100**3
     cycles = mixed["stats"]["cycles"] / max(mixed["stats"]["applies"], 1)
     passes = args.steps * 8 * cycles
@@ -155,9 +173,13 @@ def main():
     acc_per_pass = mixed["accepted"] / passes
     if mixed["pass_n"]:
         avg_ms = mixed["pass_ms"] / mixed["pass_n"]
-        b = pass_bytes("mixed", n, n_spaces, nonempty, acc_per_pass)
+        if mixed["stats"]["tiled"] == 2:
+            b = apply_bytes(n, n_spaces, trials / args.steps, mixed["accepted"] / args.steps)
+            kernel = "k_apply_tile<3>"
+        else:
+            b = pass_bytes("mixed", n, n_spaces, nonempty, acc_per_pass)
+            kernel = "k_pass_tile<3,MIXED>" if mixed["stats"]["tiled"] else "k_pass<3,MIXED>"
         achieved = b / (avg_ms * 1e-3) / 1e9
-        kernel = "k_pass_tile<3,MIXED>" if mixed["stats"]["tiled"] else "k_pass<3,MIXED>"
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                            "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(kernel),
                            "kernel": kernel, "avg_launch_ms": avg_ms,

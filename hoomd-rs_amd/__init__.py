@@ -52,7 +52,7 @@ class Config(C.Structure):
 
 FLAG_GLOBAL_PASS = 1
 FLAG_NO_GRAPH = 2
-FLAG_PERSISTENT = 4
+FLAG_PER_COLOUR = 4
 
 
 class Count(C.Structure):
@@ -129,14 +129,16 @@ class GpuParallelSweep:
     def __init__(self, box_edges, positions, *, sites=None, diameter=1.0, boundary_range=None,
                  body_interaction_range=None, spatial_cell_width=None, precision="fp64", device=0,
                  lds_tiles=True, model="hard_sphere", params=None, vertices=None, orientations=None,
-                 site_orientations=None, graphs=True, persistent=False):
+                 site_orientations=None, graphs=True, persistent=True):
         """positions / sites: the microstate's body and site positions (already
         wrapped into the box, as Microstate::add_body stores them); sites=None
         derives them as wrap(body + 0).  lds_tiles=False runs every trial from
         global memory (same decisions).  graphs=False makes run() launch every
         apply directly instead of replaying a captured hipGraph (same results).
-        persistent=True runs mixed-precision hard shapes as one persistent
-        launch per apply instead of one launch per colour pass (same results).
+        persistent=False keeps mixed-precision hard shapes on one launch per
+        colour pass instead of one persistent launch per apply (same results;
+        the persistent launch needs cooperative launch support and a region
+        shape whose workgroups all fit on the device at once).
 
         model: "hard_sphere" (params = (diameter,), default (diameter,)),
         "lj" (params = (epsilon, sigma, r_cut)) or "lj_shifted"
@@ -175,7 +177,7 @@ class GpuParallelSweep:
         cfg.precision = {"fp64": HMC_FP64, "mixed": HMC_MIXED}[precision]
         cfg.device = device
         cfg.flags = ((0 if lds_tiles else FLAG_GLOBAL_PASS) | (0 if graphs else FLAG_NO_GRAPH) |
-                     (FLAG_PERSISTENT if persistent else 0))
+                     (0 if persistent else FLAG_PER_COLOUR))
         h = C.c_void_p()
         rc = lib().hmc_create(C.byref(cfg), C.byref(h))
         if rc != HMC_OK:

@@ -98,7 +98,7 @@ struct ApplyState {
   uint32_t nonempty;
   uint32_t error;
   uint32_t move;      // 0 Translate, 1 Rotate<Versor>
-  uint32_t grid_bar;  // arrivals at the persistent apply's grid barriers
+  uint32_t epoch;     // applies since creation (persistent apply: region flag generation)
   uint32_t pad;
   unsigned long long accepted_total;  // filled on the host from the slot counters
   unsigned long long trials_total;
@@ -137,7 +137,7 @@ struct Work {
   hmc_trace_record* trace;
   uint64_t trace_cap;
   int tracing;
-  uint32_t* gflags;  // persistent apply: one release flag per workgroup, kGridFlagStride words apart
+  unsigned long long* rflags;  // persistent apply: passes finished per region, kRegionFlagStride apart
 };
 
 // ----------------------------------------------------------- small helpers --
@@ -244,7 +244,7 @@ __global__ void k_setup(Geometry g, ApplyState* st, uint64_t step, uint32_t subs
   }
   st->cycles = 0;
   st->nonempty = 0;
-  st->grid_bar = 0;
+  st->epoch += 1;
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
@@ -1261,38 +1261,41 @@ __global__ void __launch_bounds__(kTileThreads) k_pass_tile(Geometry g, ApplySta
 // An LDS entry holds the site relative to the box centre in FP32 (NaN for a
 // site without images, as in the per-colour kernel) with the body index in w.
 constexpr uint32_t kBarrierSpins = 1u << 22;  // about 0.2 s of 128-cycle sleeps
-constexpr uint32_t kErrBarrier = 2u;          // ApplyState::error: a grid barrier timed out
-constexpr uint32_t kGridFlagStride = 64;      // words between release flags (256 B: separate memory channels)
+constexpr uint32_t kErrBarrier = 2u;          // ApplyState::error: a wait for neighbour regions timed out
+constexpr uint32_t kRegionFlagStride = 32;    // 64-bit words between region flags (256 B: separate channels)
 constexpr int kRefreshHeld = 4;               // halo entries per thread whose reloads overlap the next draw
 
-// Every workgroup of the launch arrives and waits until all have arrived gen
-// times.  The launch is cooperative, so all workgroups are resident.  Each
-// workgroup counts its arrival once on a shared counter and then polls a
-// flag of its own; the last to arrive sets every flag to gen.  (Polling the
-// shared counter instead keeps hundreds of loads queued on one memory
-// channel, which stalls every access of the workgroups still working.)  The
-// wait is bounded: past the bound the state's error word is set and the
-// barrier returns, so every wave still ends.
-__device__ __forceinline__ void grid_sync(ApplyState* st, uint32_t* gflags, uint32_t gen, unsigned* last) {
+// Neighbour synchronisation of the persistent apply.  region_publish records
+// that this region has finished the passes counted in val (epoch in the high
+// word); region_wait returns once every neighbour region (wave 0's lane
+// l < nnb watches region nbt) has too.  A region only ever reads its own
+// cells and its neighbours' cells, so once all of them have finished pass i
+// none can be a pass behind or ahead of it while it runs pass i + 1: nothing
+// it reads is being written.  The stores of the finished pass are written
+// back from the L2 (agent-scope release fence) before the flag is set, and
+// the caches are invalidated (agent-scope acquire fence) after the
+// neighbours' flags are seen.  The launch is cooperative, so every region is
+// resident.  The wait is bounded: past the bound the state's error word is
+// set and the wait returns, so every wave still ends.
+__device__ __forceinline__ void region_publish(unsigned long long* rflags, uint32_t tile, unsigned long long val) {
   __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have completed
   __syncthreads();
   if (threadIdx.x == 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    const uint32_t before = __hip_atomic_fetch_add(&st->grid_bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    *last = before + 1 == gen * gridDim.x ? 1u : 0u;
+    __hip_atomic_store(&rflags[tile * kRegionFlagStride], val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  __syncthreads();
-  if (*last && threadIdx.x < 64)
-    for (uint32_t i = threadIdx.x; i < gridDim.x; i += 64)
-      __hip_atomic_store(&gflags[i * kGridFlagStride], gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (threadIdx.x == 0) {
-    uint32_t spins = 0;
-    while (__hip_atomic_load(&gflags[blockIdx.x * kGridFlagStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
-           gen) {
-      __builtin_amdgcn_s_sleep(2);
-      if (++spins > kBarrierSpins) {
-        atomicOr(&st->error, kErrBarrier);
-        break;
+}
+__device__ __forceinline__ void region_wait(ApplyState* st, unsigned long long* rflags, uint32_t nbt, uint32_t nnb,
+                                            unsigned long long val) {
+  if (threadIdx.x < 64) {
+    if (threadIdx.x < nnb) {
+      uint32_t spins = 0;
+      while (__hip_atomic_load(&rflags[nbt * kRegionFlagStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < val) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > kBarrierSpins) {
+          atomicOr(&st->error, kErrBarrier);
+          break;
+        }
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -1404,10 +1407,12 @@ __global__ void __launch_bounds__(kTileThreads) k_apply_tile(Geometry g, ApplySt
   uint32_t* coff = (uint32_t*)(smem + 16u * (R + 1));
   VT* lb = (VT*)(smem + ((16u * (R + 1) + 4u * R * CW + 15u) & ~15u));
   __shared__ double zx[257], zf[257];
-  __shared__ unsigned blk_acc, bar_last;
+  __shared__ unsigned blk_acc;
   const uint32_t tid = threadIdx.x, NT = blockDim.x;
   const uint32_t wid = tid >> 6, lane = tid & 63u;
   const uint32_t cycles = st->cycles, substep0 = st->substep0;
+  if (!live) return;  // padding workgroups take part in nothing
+  const unsigned long long epoch = (unsigned long long)st->epoch << 32;
   const uint64_t step = st->step;
   const uint32_t seed = st->seed;
   const double max_move = st->max_move;
@@ -1448,6 +1453,21 @@ __global__ void __launch_bounds__(kTileThreads) k_apply_tile(Geometry g, ApplySt
     }
   }
   const bool has_trial = live && tid < tc.trials;
+  // wave 0's lane l < nnb watches neighbour region l (3^D - 1 of them, with
+  // the periodic wrap of the region grid; duplicates and the region itself
+  // where the grid has fewer than three regions on an axis are harmless)
+  const uint32_t nnb = D == 3 ? 26u : 8u;
+  uint32_t nbt = tile;
+  if (wid == 0 && lane < nnb) {
+    uint32_t q = lane < nnb / 2 ? lane : lane + 1;  // skip the centre
+    uint32_t nd[3] = {0, 0, 0};
+    for (int a = D - 1; a >= 0; --a) {
+      nd[a] = q % 3u;
+      q /= 3u;
+    }
+    nbt = 0;
+    for (int a = 0; a < D; ++a) nbt = nbt * tc.nt[a] + (tcd[a] + tc.nt[a] + nd[a] - 1u) % tc.nt[a];
+  }
 
   // ---- staging (once): rows, cell offsets, sites with their body indices
   const uint32_t rpl = (R + 63) / 64;
@@ -1594,11 +1614,10 @@ __global__ void __launch_bounds__(kTileThreads) k_apply_tile(Geometry g, ApplySt
   };
 
   unsigned retests = 0, exp_slow = 0;
-  uint32_t gen = 0;
   // The proposal of pass i (cycle i / 2^D, colour i mod 2^D) depends only on
   // the counter, the space's body count and the chosen body's position, which
-  // no other colour changes; it is drawn while the previous colour's halo is
-  // being reloaded.
+  // no other colour changes; it is drawn while the neighbour regions finish
+  // the previous pass.
   bool pgo = false;
   uint32_t ps = 0, pcrow = 0, pe0 = 0, pn = 0, ppick = 0, pbody = 0;
   uint32_t pcb[3] = {0, 0, 0};
@@ -1741,11 +1760,14 @@ __global__ void __launch_bounds__(kTileThreads) k_apply_tile(Geometry g, ApplySt
     const unsigned long long mk = __ballot(accepted != 0);
     if (__lane_id() == 0 && mk) atomicAdd(&blk_acc, (unsigned)__popcll(mk));
     if (i + 1 == npass) break;  // no later pass reads the halo
-    grid_sync(st, w.gflags, ++gen, &bar_last);
-    // ---- reload the halo cells of this colour (box cells b with b = 1 - off
-    // (mod 2) per axis whose space lies outside the region): the first
-    // kRefreshHeld entries' loads are issued before the next proposal is
-    // drawn and written after it
+    region_publish(w.rflags, tile, epoch | (i + 1));
+    // the next proposal reads only static cell data and this region's own
+    // bodies, none of which pass i changed: drawn while the neighbours finish
+    propose(i + 1);
+    region_wait(st, w.rflags, nbt, nnb, epoch | (i + 1));
+    // ---- reload the halo cells of this colour: box cells b with b = 1 - off
+    // (mod 2) per axis whose space lies outside the region (the first
+    // kRefreshHeld entries of a thread are loaded together)
     uint32_t hdst[kRefreshHeld];
     float4 hval[kRefreshHeld];
     uint32_t nheld = 0;
@@ -1784,7 +1806,6 @@ __global__ void __launch_bounds__(kTileThreads) k_apply_tile(Geometry g, ApplySt
         }
       }
     }
-    propose(i + 1);
 #pragma unroll
     for (int h = 0; h < kRefreshHeld; ++h)
       if ((uint32_t)h < nheld) lb[hdst[h]] = stage_entry<D, HMC_MIXED>(g, c32, hval[h], (uint32_t)__float_as_int(lb[hdst[h]].w));
@@ -1793,9 +1814,6 @@ __global__ void __launch_bounds__(kTileThreads) k_apply_tile(Geometry g, ApplySt
   lane_add(&st->band_retests, retests);
   lane_add(&st->exp_slow, exp_slow);
   if (tid == 0 && live && overflow) atomicAdd(&st->tile_overflows, 1ull);
-  // this workgroup's flag back to 0 for the next apply (no later barrier
-  // writes it: the last one has been passed)
-  if (tid == 0 && gen) __hip_atomic_store(&w.gflags[b * kGridFlagStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
   if (tid == 0 && blk_acc) atomicAdd(&w.acc_slots[b & (kAccSlots - 1)], (unsigned long long)blk_acc);
 }
@@ -2434,7 +2452,7 @@ __global__ void k_bsetup(const BatchRep* reps, ApplyState* sts, uint32_t R, cons
   }
   st->cycles = 0;
   st->nonempty = 0;
-  st->grid_bar = 0;
+  st->epoch += 1;
 }
 
 template <int D>
@@ -3382,7 +3400,7 @@ int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
       }
     }
     int coop = 0;
-    if (c->tiled && cfg->precision == HMC_MIXED && (cfg->flags & HMC_FLAG_PERSISTENT) &&
+    if (c->tiled && cfg->precision == HMC_MIXED && !(cfg->flags & HMC_FLAG_PER_COLOUR) &&
         hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, c->device) == hipSuccess && coop) {
       TileCfg pc{};
       if (choose_regions(g, D, (double)cfg->n_bodies / (double)spaces, cus, &pc)) {
@@ -3401,8 +3419,8 @@ int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
           c->pc = pc;
           c->pgrid = pc.xcd_chunk * 8u;
           c->pthreads = threads;
-          const size_t fbytes = (size_t)c->pgrid * kGridFlagStride * sizeof(uint32_t);
-          if (hipMalloc((void**)&c->w.gflags, fbytes) != hipSuccess || hipMemset(c->w.gflags, 0, fbytes) != hipSuccess) {
+          const size_t fbytes = (size_t)pc.ntiles * kRegionFlagStride * sizeof(unsigned long long);
+          if (hipMalloc((void**)&c->w.rflags, fbytes) != hipSuccess || hipMemset(c->w.rflags, 0, fbytes) != hipSuccess) {
             delete c;
             return HMC_ERR_DEVICE;
           }
@@ -3869,7 +3887,7 @@ void hmc_destroy(hmc_ctx* c) {
                   (void*)c->gdev,
                   (void*)c->cells.pos64, (void*)c->w.cnt, (void*)c->w.start, (void*)c->w.elem_space,
                   (void*)c->w.elem_rank, (void*)c->w.tmp_body, (void*)c->w.block_sums, (void*)c->w.block_ne,
-                  (void*)c->w.acc_slots, (void*)c->w.trace, (void*)c->w.gflags, (void*)c->st})
+                  (void*)c->w.acc_slots, (void*)c->w.trace, (void*)c->w.rflags, (void*)c->st})
     if (p) (void)hipFree(p);
   collect_events(c, c->pass_events, c->pass_ms, c->pass_n);
   collect_events(c, c->build_events, c->build_ms, c->build_n);

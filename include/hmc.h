@@ -89,10 +89,9 @@ enum {
   HMC_FLAG_NO_GRAPH = 2,    /* hmc_run_translate launches every apply directly
                                instead of replaying a captured hipGraph
                                (same results) */
-  HMC_FLAG_PERSISTENT = 4   /* MIXED hard shapes: one persistent launch per
-                               apply, each workgroup keeping its region and
-                               halo in LDS across the colour passes, instead of
-                               one launch per colour pass (same results) */
+  HMC_FLAG_PER_COLOUR = 4   /* MIXED hard shapes: one launch per colour pass
+                               instead of the default single persistent launch
+                               per apply (same results) */
 };
 
 /* One trial of a trace: replaces the per-space BodyTrial status of

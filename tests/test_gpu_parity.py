@@ -105,7 +105,7 @@ def _sorted_trace(t):
 # pass layouts: one persistent launch per apply (mixed hard shapes; FP64
 # falls back to the per-colour tiles), one LDS-tiled launch per colour, or
 # every trial from global memory
-MODES = {"persist": dict(persistent=True), "percolour": dict(), "global": dict(lds_tiles=False)}
+MODES = {"persist": dict(), "percolour": dict(persistent=False), "global": dict(lds_tiles=False)}
 
 
 def expected_tiled(mode, precision):
