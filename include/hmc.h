@@ -70,7 +70,7 @@ typedef struct {
   double spatial_cell_width;     /* cell width of the microstate's VecCell
                                     (hoomd-spatial/src/vec_cell.rs:286-299); decides
                                     which images are visited at exact contact */
-  int32_t model;                 /* HMC_HARD_SPHERE */
+  int32_t model;                 /* HMC_HARD_SPHERE, HMC_LJ, HMC_LJ_SHIFTED or HMC_CONVEX_POLYHEDRON */
   double params[4];
   int32_t precision;             /* HMC_FP64 or HMC_MIXED */
   int32_t device;                /* HIP device ordinal */
@@ -163,9 +163,10 @@ int hmc_get_orientations(hmc_ctx* ctx, double* body_quat, double* site_quat);
 /* n_steps iterations of { ParallelSweep::apply; Microstate::increment_step }
  * (the body of HardSphereSim::advance, benchmarks/src/mc/hard_sphere.rs:90-111)
  * starting at (step0, substep 0), with no host synchronisation between
- * applies; after the first apply the launches of one step are replayed from a
- * captured hipGraph (HMC_FLAG_NO_GRAPH: launched directly).  *out receives
- * the summed count. */
+ * applies.  With one launch per colour pass, the launches of one step after
+ * the first are replayed from a captured hipGraph (HMC_FLAG_NO_GRAPH:
+ * launched directly); the persistent apply is always launched directly.
+ * *out receives the summed count. */
 int hmc_run_translate(hmc_ctx* ctx, uint64_t step0, uint32_t seed, double kT, double max_distance,
                       uint32_t n_steps, hmc_count* out);
 
