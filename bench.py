@@ -19,6 +19,8 @@ import pathlib
 import sys
 import time
 
+import numpy as np
+
 ROOT = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "tests"))
 
@@ -35,6 +37,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison run")
     ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--no-other", action="store_true", help="skip the short runs of the other configurations")
     ap.add_argument("--pass-layout", choices=["percolour", "persistent"], default="persistent",
                     help="mixed precision: one launch per colour pass, or one persistent launch per apply")
     ap.add_argument("--profile-period", type=int, default=8,
@@ -116,6 +119,61 @@ def apply_bytes(n, n_spaces, trials, accepted):
     return 4 * (n_spaces + 1) + 20 * n + 32 * trials + 80 * accepted
 
 
+def other_workloads(hmc, wl):
+    """Short timed runs of the other configurations (trial moves/s; inputs
+    resident on the device; a few warm-up steps first)."""
+    res = {}
+
+    def timed(fn, warm, steps):
+        fn(0, warm)
+        t0 = time.perf_counter()
+        trials = fn(warm, steps)
+        return trials / (time.perf_counter() - t0), trials
+
+    # C1: hard disks, 128 x 128 square lattice at packing fraction 0.70, d = 0.15
+    L, pos = wl.square_lattice_disks()
+    sim = hmc.GpuParallelSweep(L, wl.wrap(L, pos), precision="mixed")
+    v, t = timed(lambda s0, k: sum(sim.run(s0, 42, 1.0, 0.15, k)), 20, 400)
+    res["C1_hard_disks_16384"] = {"value": v, "unit": "trial moves/s", "steps": 400, "precision": "mixed",
+                                  "pass_layout": {2: "persistent", 1: "percolour", 0: "global"}[sim.stats()["tiled"]]}
+    sim.close()
+    # C4: shifted LJ, FCC 64 x 64 x 32 at number density 0.8, kT 1.2, r_cut 2.5, d = 0.1
+    L, pos = wl.fcc_number_density((64, 64, 32), 0.8)
+    sim = hmc.GpuParallelSweep(L, wl.wrap(L, pos), model="lj_shifted", params=(1.0, 1.0, 2.5, 2.5))
+    v, t = timed(lambda s0, k: sum(sim.run(s0, 42, 1.2, 0.1, k)), 1, 4)
+    res["C4_shifted_lj_524288"] = {"value": v, "unit": "trial moves/s", "steps": 4, "precision": "fp64"}
+    sim.close()
+    # C3: unit cubes, simple cubic 64^3 at packing fraction 0.55; a step is a
+    # translate apply (d = 0.05) and a rotate apply (0.1), as the reference's
+    # polyhedron benchmark; counts are read back after every apply
+    L, pos = wl.simple_cubic()
+    sim = hmc.GpuParallelSweep(L, wl.wrap(L, pos), model="convex", vertices=wl.UNIT_CUBE,
+                               orientations=np.tile([1.0, 0.0, 0.0, 0.0], (len(pos), 1)))
+    state = {"sub": 0}
+
+    def c3(s0, k):
+        tot = 0
+        for step in range(s0, s0 + k):
+            a, r, used = sim.apply(step, 0, 42, 1.0, 0.05)
+            a2, r2, _ = sim.apply_rotate(step, used, 42, 1.0, 0.1)
+            tot += a + r + a2 + r2
+        return tot
+    v, t = timed(c3, 1, 4)
+    res["C3_cubes_262144"] = {"value": v, "unit": "trial moves/s", "steps": 4, "precision": "fp64",
+                              "moves": "translate 0.05 + rotate 0.1 per step"}
+    sim.close()
+    # C5: 256 replicas of hard spheres (FCC, N 2,048 .. 62,500, packing
+    # fractions 0.30 .. 0.54), seeds 42 + r, d = 0.08, one graph per step
+    systems = wl.batched_replicas()
+    batch = hmc.GpuBatch([(L, wl.wrap(L, p), None) for L, p in systems], precision="mixed")
+    seeds = [42 + r for r in range(len(systems))]
+    v, t = timed(lambda s0, k: sum(a + r for a, r in batch.run(s0, seeds, 1.0, 0.08, k)), 2, 40)
+    res["C5_replicas_256"] = {"value": v, "unit": "trial moves/s", "steps": 40, "precision": "mixed",
+                              "bodies": int(sum(len(p) for _, p in systems))}
+    batch.close()
+    return res
+
+
 def cpu_baseline(L, pos, seconds, seed, kT, d):
     import _oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
@@ -138,16 +196,14 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
     import _pkg
-    import _oracle as O
     hmc = _pkg.pkg()
     wl = _pkg.workloads()
     kT, d = 1.0, 0.08
     L, pos = wl.fcc_spheres((63, 63, 63), packing_fraction=0.50)
     n = len(pos)
-    # the microstate's wrapped bodies and sites (Microstate::add_body)
-    ref = O.OracleSystem(L, pos, seed=args.seed)
-    body, site = ref.positions()
-    del ref
+    # the microstate's stored bodies (Microstate::add_body wraps them); the
+    # library derives the sites as wrap(body + 0)
+    body, site = wl.wrap(L, pos), None
 
     mixed = run_gpu(hmc, L, body, site, "mixed", args.steps, args.warmup, args.seed, kT, d, args.profile_period,
                     persistent=args.pass_layout == "persistent")
@@ -198,6 +254,8 @@ def main():
                        "avg_pass_ms": f64["pass_ms"] / max(f64["pass_n"], 1),
                        "identical_decisions_and_positions": bool(same)}
         out["mixed_over_fp64"] = value / v64
+    if not args.no_other:
+        out["other_workloads"] = other_workloads(hmc, wl)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(L, pos, args.cpu_seconds, args.seed, kT, d)
         out["cpu_baseline"] = cb

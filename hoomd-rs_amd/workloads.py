@@ -77,3 +77,34 @@ def batched_replicas(n_replicas=256, seed=2024, k_range=(8, 25), phi_range=(0.30
         phi = phi_range[0] + (phi_range[1] - phi_range[0]) * (r / max(n_replicas - 1, 1))
         out.append(fcc_spheres((k, k, k), packing_fraction=phi))
     return out
+
+
+def wrap(box_edges, positions):
+    """Periodic<Hypercuboid>::wrap (hoomd-microstate/src/boundary/periodic/
+    cuboid.rs:90-103), the form in which Microstate::add_body stores body
+    positions: per axis lambda = x / L, lambda - round(lambda) with halves
+    rounded away from zero (Rust's f64::round), +1/2 mapped to -1/2, times L.
+    Not idempotent in floating point."""
+    L = np.asarray(box_edges, np.float64)[: np.shape(positions)[1]]
+    lam = np.asarray(positions, np.float64) / L
+    t = np.trunc(lam)
+    r = np.where(np.abs(lam - t) >= 0.5, t + np.sign(lam), t)  # keeps round(-0.0) = -0.0
+    lam = lam - r
+    lam = np.where(lam == 0.5, -0.5, lam)
+    return lam * L
+
+
+# ConvexPolytope vertices of the unit cube, in the order of the reference's
+# test fixtures
+UNIT_CUBE = np.array([[-0.5, -0.5, -0.5], [0.5, -0.5, -0.5], [0.5, 0.5, -0.5], [-0.5, 0.5, -0.5],
+                      [-0.5, -0.5, 0.5], [0.5, -0.5, 0.5], [0.5, 0.5, 0.5], [-0.5, 0.5, 0.5]])
+
+
+def simple_cubic(n_side=64, packing_fraction=0.55, particle_volume=1.0):
+    """n_side^3 bodies of the given volume on a simple cubic lattice at the
+    packing fraction, centred in a cubic box.  Returns (box_edges, positions)."""
+    L = (n_side**3 * particle_volume / packing_fraction) ** (1.0 / 3.0)
+    a = L / n_side
+    i, j, k = np.meshgrid(np.arange(n_side), np.arange(n_side), np.arange(n_side), indexing="ij")
+    pos = (np.stack([i.ravel(), j.ravel(), k.ravel()], 1) + 0.5) * a - L / 2
+    return np.array([L, L, L]), pos
