@@ -98,7 +98,7 @@ struct ApplyState {
   uint32_t nonempty;
   uint32_t error;
   uint32_t move;      // 0 Translate, 1 Rotate<Versor>
-  uint32_t epoch;     // applies since creation (persistent apply: region flag generation)
+  uint32_t epoch;     // applies since creation
   uint32_t pad;
   unsigned long long accepted_total;  // filled on the host from the slot counters
   unsigned long long trials_total;
@@ -137,7 +137,7 @@ struct Work {
   hmc_trace_record* trace;
   uint64_t trace_cap;
   int tracing;
-  unsigned long long* rflags;  // persistent apply: passes finished per region, kRegionFlagStride apart
+  uint32_t* mbox;  // persistent apply: per region two pass counters (by parity), kMailboxStride words apart
 };
 
 // ----------------------------------------------------------- small helpers --
@@ -1262,61 +1262,113 @@ __global__ void __launch_bounds__(kTileThreads) k_pass_tile(Geometry g, ApplySta
 // site without images, as in the per-colour kernel) with the body index in w.
 constexpr uint32_t kBarrierSpins = 1u << 22;  // about 0.2 s of 128-cycle sleeps
 constexpr uint32_t kErrBarrier = 2u;          // ApplyState::error: a wait for neighbour regions timed out
-constexpr uint32_t kRegionFlagStride = 32;    // 64-bit words between region flags (256 B: separate channels)
-constexpr int kRefreshHeld = 4;               // halo entries per thread whose reloads overlap the next draw
+constexpr uint32_t kMailboxStride = 64;       // words between region mailboxes (256 B: separate channels)
+constexpr int kRefreshHeld = 2;               // halo entries per thread loaded together in a refresh
+constexpr int kStageRows = 4;                 // rows per half-wave staged together by the persistent apply
 
-// Neighbour synchronisation of the persistent apply.  region_publish records
-// that this region has finished the passes counted in val (epoch in the high
-// word); region_wait returns once every neighbour region (wave 0's lane
-// l < nnb watches region nbt) has too.  A region only ever reads its own
-// cells and its neighbours' cells, so once all of them have finished pass i
-// none can be a pass behind or ahead of it while it runs pass i + 1: nothing
-// it reads is being written.  The stores of the finished pass are written
-// back from the L2 (agent-scope release fence) before the flag is set, and
-// the caches are invalidated (agent-scope acquire fence) after the
-// neighbours' flags are seen.  The launch is cooperative, so every region is
-// resident.  The wait is bounded: past the bound the state's error word is
-// set and the wait returns, so every wave still ends.
-__device__ __forceinline__ void region_publish(unsigned long long* rflags, uint32_t tile, unsigned long long val) {
+// Device-scope (agent) accesses to the FP64 sites, the only data one region
+// writes and another reads within the apply: the XCDs have separate L2
+// caches, and these bypass them where needed.
+__device__ __forceinline__ uint64_t ld_dev(const void* p) {
+  return __hip_atomic_load((uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(void* p, uint64_t v) {
+  __hip_atomic_store((uint64_t*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double4 ld_dev_d4(const double4* p) {
+  const uint64_t* q = (const uint64_t*)p;
+  return make_double4(__longlong_as_double((long long)ld_dev(q)), __longlong_as_double((long long)ld_dev(q + 1)),
+                      __longlong_as_double((long long)ld_dev(q + 2)), __longlong_as_double((long long)ld_dev(q + 3)));
+}
+__device__ __forceinline__ void st_dev_d4(double4* p, double4 v) {
+  uint64_t* q = (uint64_t*)p;
+  st_dev(q, (uint64_t)__double_as_longlong(v.x));
+  st_dev(q + 1, (uint64_t)__double_as_longlong(v.y));
+  st_dev(q + 2, (uint64_t)__double_as_longlong(v.z));
+  st_dev(q + 3, (uint64_t)__double_as_longlong(v.w));
+}
+
+// Neighbour synchronisation of the persistent apply, through mailboxes: two
+// counters per region (one per pass parity), kMailboxStride words apart.
+// After pass i a region adds 1 to the counter i mod 2 of each of its 3^D - 1
+// neighbours (with multiplicity where the region grid wraps onto itself; the
+// relation is symmetric, so every region receives exactly 3^D - 1 per pass)
+// and then waits until its own counter reaches 3^D - 1, which it resets.  A
+// region reads only its own and its neighbours' cells, so once all of them
+// have finished pass i none can be a pass behind or ahead while it runs pass
+// i + 1: nothing it reads is being written.  A neighbour can therefore never
+// finish pass i + 2 (the next use of the same counter) before this region has
+// consumed pass i, and every counter is 0 again when an apply ends.  The
+// FP64 sites that neighbours read are written with device-scope stores, and
+// every wave waits for its stores to complete before the increments are sent
+// (no cache writeback or invalidation: those, issued by the regions that
+// finish first, stalled the ones still working); the neighbours read them
+// with device-scope loads.  One lane per region polls, on a counter of its
+// own.  The launch is cooperative, so every
+// region is resident.  The wait is bounded: past the bound the state's error
+// word is set and the wait returns, so every wave still ends.
+__device__ __forceinline__ void region_publish(uint32_t* mbox, uint32_t nbt, uint32_t nnb, uint32_t parity) {
   __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have completed
   __syncthreads();
-  if (threadIdx.x == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __hip_atomic_store(&rflags[tile * kRegionFlagStride], val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x < 64) {
+    if (threadIdx.x < nnb)
+      __hip_atomic_fetch_add(&mbox[nbt * kMailboxStride + parity], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
-__device__ __forceinline__ void region_wait(ApplyState* st, unsigned long long* rflags, uint32_t nbt, uint32_t nnb,
-                                            unsigned long long val) {
+__device__ __forceinline__ void region_wait(ApplyState* st, uint32_t* mbox, uint32_t tile, uint32_t nnb,
+                                            uint32_t parity) {
   if (threadIdx.x < 64) {
-    if (threadIdx.x < nnb) {
+    if (threadIdx.x == 0) {
+      uint32_t* box = &mbox[tile * kMailboxStride + parity];
       uint32_t spins = 0;
-      while (__hip_atomic_load(&rflags[nbt * kRegionFlagStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < val) {
+      while (__hip_atomic_load(box, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < nnb) {
         __builtin_amdgcn_s_sleep(1);
         if (++spins > kBarrierSpins) {
           atomicOr(&st->error, kErrBarrier);
           break;
         }
       }
+      __hip_atomic_store(box, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
 }
 
-// FP64 re-test of a band pair (rare; out of line), the site given by body.
+// LDS entry of the persistent apply from a site's FP64 copy: the image
+// nearest the box centre c, relative to c, computed in FP64 and rounded once
+// to FP32 (NaN for a site without images), body index in w.
+template <int D>
+__device__ __forceinline__ float4 stage_entry64(const Geometry& g, const double* c, double4 sp, uint32_t body) {
+  const double x[3] = {sp.x, sp.y, sp.z};
+  float r[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    double u = x[a] - c[a];
+    if (u >= g.half[a])
+      u -= g.L[a];
+    else if (u < -g.half[a])
+      u += g.L[a];
+    r[a] = (float)u;
+  }
+  if ((uint32_t)__double_as_longlong(sp.w) & kNoImages) r[0] = __int_as_float(0x7fc00000);
+  return make_float4(r[0], r[1], r[2], __int_as_float((int)body));
+}
+
+// FP64 re-test of a band pair (rare; out of line), the site given by body,
+// read with device-scope loads (another region may have written it).
 template <int D>
 __device__ __noinline__ bool retest64_body(const Geometry* gp, const double4* spos, uint32_t body, double t0,
                                            double t1, double t2) {
   const double t[3] = {t0, t1, t2};
-  return overlap64<D>(*gp, t, spos[body]);
+  return overlap64<D>(*gp, t, ld_dev_d4(&spos[body]));
 }
 
-// The trial of space s (slots [s0, s0 + n)) from global memory, for a region
-// whose box holds more bodies than its LDS copy (rare).  Same proposal and
-// decision as every other path.
+// The trial of space s (slots [s0, s0 + n)) from global memory with exact FP64
+// pair tests on the sites, for a region whose box holds more bodies than its
+// LDS copy (rare).  Same proposal and decision as every other path.
 template <int D>
 __device__ __noinline__ TrialOut trial_global_dev(const Geometry* gp, ApplyState* st, double4* bpos, double4* spos,
-                                                  const uint32_t* cbody, float4* pos32, const uint32_t* start,
+                                                  const uint32_t* cbody, const uint32_t* start,
                                                   hmc_trace_record* trace, unsigned long long trace_cap,
                                                   const double* zx, const double* zf, uint32_t s, uint32_t s0,
                                                   uint32_t n, uint32_t colour, uint32_t substep) {
@@ -1340,8 +1392,6 @@ __device__ __noinline__ TrialOut trial_global_dev(const Geometry* gp, ApplyState
   if (space_of<D>(g, st->origin, x) == s) {
     double t[3] = {0.0, 0.0, 0.0};
     for (int a = 0; a < D; ++a) t[a] = wrap1(x[a], g.L[a]);
-    float t32[3] = {0.0f, 0.0f, 0.0f};
-    for (int a = 0; a < D; ++a) t32[a] = (float)t[a];
     bool hit = false;
     for (int i0 = -1; i0 <= 1 && !hit; ++i0)
       for (int i1 = -1; i1 <= 1 && !hit; ++i1)
@@ -1357,12 +1407,7 @@ __device__ __noinline__ TrialOut trial_global_dev(const Geometry* gp, ApplyState
           const uint32_t q0 = start[ns], q1 = start[ns + 1];
           for (uint32_t q = q0; q < q1 && !hit; ++q) {
             if (q == slot) continue;
-            int r = overlap32<D>(g, t32, pos32[q]);
-            if (r == 2) {
-              ++o.retests;
-              r = retest64_body<D>(gp, spos, cbody[q], t[0], t[1], t[2]) ? 1 : 0;
-            }
-            hit = r != 0;
+            hit = overlap64<D>(g, t, ld_dev_d4(&spos[cbody[q]]));
           }
         }
     accept = !hit;
@@ -1371,8 +1416,7 @@ __device__ __noinline__ TrialOut trial_global_dev(const Geometry* gp, ApplyState
       for (int a = 0; a < D; ++a) nsp[a] = wrap1(t[a] + 0.0, g.L[a]);
       const uint32_t fl = ghost_flags<D>(g, nsp);
       bpos[body] = make_double4(t[0], t[1], t[2], 0.0);
-      spos[body] = pack_site(nsp, fl);
-      pos32[slot] = pack_site32(nsp, fl);
+      st_dev_d4(&spos[body], pack_site(nsp, fl));
     }
   }
   o.accepted = accept ? 1u : 0u;
@@ -1393,7 +1437,7 @@ __device__ __noinline__ TrialOut trial_global_dev(const Geometry* gp, ApplyState
 }
 
 template <int D>
-__global__ void __launch_bounds__(kTileThreads) k_apply_tile(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
+__global__ void __launch_bounds__(kTileThreads, 2) k_apply_tile(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
                                                              TileCfg tc) {
   using VT = float4;
   constexpr int AL = D - 1;  // the row axis (stride 1)
@@ -1412,7 +1456,6 @@ __global__ void __launch_bounds__(kTileThreads) k_apply_tile(Geometry g, ApplySt
   const uint32_t wid = tid >> 6, lane = tid & 63u;
   const uint32_t cycles = st->cycles, substep0 = st->substep0;
   if (!live) return;  // padding workgroups take part in nothing
-  const unsigned long long epoch = (unsigned long long)st->epoch << 32;
   const uint64_t step = st->step;
   const uint32_t seed = st->seed;
   const double max_move = st->max_move;
@@ -1437,11 +1480,11 @@ __global__ void __launch_bounds__(kTileThreads) k_apply_tile(Geometry g, ApplySt
   const int32_t SL = g.shape[AL], ml = mlo[AL];
   const uint32_t p = ml < 0 ? 1u : (uint32_t)min((int32_t)EL, SL - ml);
   const uint32_t m0 = (uint32_t)(ml < 0 ? ml + SL : ml);
-  float c32[3] = {0.0f, 0.0f, 0.0f};
+  double c64[3] = {0.0, 0.0, 0.0};
   for (int a = 0; a < D; ++a) {
     double c = origin[a] + ((double)mlo[a] + 0.5 * (double)tc.E[a]) * g.width[a];
     c -= g.L[a] * floor((c + g.half[a]) / g.L[a]);
-    c32[a] = (float)c;
+    c64[a] = c;
   }
   // this thread's trial digits (one space of each colour)
   uint32_t hh[3] = {0, 0, 0};
@@ -1453,9 +1496,9 @@ __global__ void __launch_bounds__(kTileThreads) k_apply_tile(Geometry g, ApplySt
     }
   }
   const bool has_trial = live && tid < tc.trials;
-  // wave 0's lane l < nnb watches neighbour region l (3^D - 1 of them, with
-  // the periodic wrap of the region grid; duplicates and the region itself
-  // where the grid has fewer than three regions on an axis are harmless)
+  // wave 0's lane l < nnb notifies neighbour region l (3^D - 1 of them, with
+  // the periodic wrap of the region grid: duplicates, and the region itself,
+  // where the grid has fewer than three regions on an axis)
   const uint32_t nnb = D == 3 ? 26u : 8u;
   uint32_t nbt = tile;
   if (wid == 0 && lane < nnb) {
@@ -1558,11 +1601,10 @@ __global__ void __launch_bounds__(kTileThreads) k_apply_tile(Geometry g, ApplySt
   const bool overflow = total > tc.maxb;  // uniform over the workgroup
   if (live && !overflow) {
     const uint32_t hw = tid >> 5, hl = tid & 31u, NH = NT >> 5;
-    for (uint32_t r0 = hw; r0 < R; r0 += NH * 8u) {
-      float4 tmp[8];
-      uint32_t dst[8], src[8], bod[8];
+    for (uint32_t r0 = hw; r0 < R; r0 += NH * kStageRows) {
+      uint32_t dst[kStageRows], src[kStageRows], bod[kStageRows];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < kStageRows; ++u) {
         const uint32_t r = r0 + (uint32_t)u * NH;
         dst[u] = 0xffffffffu;
         src[u] = 0u;
@@ -1574,24 +1616,26 @@ __global__ void __launch_bounds__(kTileThreads) k_apply_tile(Geometry g, ApplySt
             dst[u] = q.x + hl;
           }
         }
-        if (dst[u] != 0xffffffffu) {
-          tmp[u] = C.pos32[src[u]];
-          bod[u] = C.body[src[u]];
-        }
+        if (dst[u] != 0xffffffffu) bod[u] = C.body[src[u]];
       }
+      double4 tmp[kStageRows];
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (dst[u] != 0xffffffffu) lb[dst[u]] = stage_entry<D, HMC_MIXED>(g, c32, tmp[u], bod[u]);
+      for (int u = 0; u < kStageRows; ++u)
+        if (dst[u] != 0xffffffffu) tmp[u] = B.spos[bod[u]];
+#pragma unroll
+      for (int u = 0; u < kStageRows; ++u)
+        if (dst[u] != 0xffffffffu) lb[dst[u]] = stage_entry64<D>(g, c64, tmp[u], bod[u]);
       // entries past the first 32 of a row (rare)
 #pragma unroll 1
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < kStageRows; ++u) {
         const uint32_t r = r0 + (uint32_t)u * NH;
         if (r >= R) continue;
         const uint4 q = ri[r];
         const uint32_t len = ri[r + 1].x - q.x;
         for (uint32_t e = 32u + hl; e < len; e += 32u) {
           const uint32_t sl = e < q.z ? q.y + e : q.w + (e - q.z);
-          lb[q.x + e] = stage_entry<D, HMC_MIXED>(g, c32, C.pos32[sl], C.body[sl]);
+          const uint32_t bd = C.body[sl];
+          lb[q.x + e] = stage_entry64<D>(g, c64, B.spos[bd], bd);
         }
       }
     }
@@ -1606,12 +1650,6 @@ __global__ void __launch_bounds__(kTileThreads) k_apply_tile(Geometry g, ApplySt
   }
   __syncthreads();
 
-  // slot of LDS entry e of row r
-  auto slot_of = [&](uint32_t r, uint32_t e) -> uint32_t {
-    const uint4 q = ri[r];
-    const uint32_t o = e - q.x;
-    return o < q.z ? q.y + o : q.w + (o - q.z);
-  };
 
   unsigned retests = 0, exp_slow = 0;
   // The proposal of pass i (cycle i / 2^D, colour i mod 2^D) depends only on
@@ -1671,7 +1709,7 @@ __global__ void __launch_bounds__(kTileThreads) k_apply_tile(Geometry g, ApplySt
       }
       const uint32_t s0 = w.start[s], n = w.start[s + 1] - s0;
       if (active && n) {
-        const TrialOut o = trial_global_dev<D>(g.self, st, B.bpos, B.spos, C.body, C.pos32, w.start,
+        const TrialOut o = trial_global_dev<D>(g.self, st, B.bpos, B.spos, C.body, w.start,
                                                w.tracing ? w.trace : nullptr, w.trace_cap, zx, zf, s, s0, n,
                                                colour, substep);
         accepted = o.accepted;
@@ -1701,30 +1739,49 @@ __global__ void __launch_bounds__(kTileThreads) k_apply_tile(Geometry g, ApplySt
         uint32_t rounds = 0;
 #pragma unroll
         for (int r = 0; r < NR; ++r) rounds = max(rounds, rhi[r] - rlo[r]);
-        float t32[3] = {0.0f, 0.0f, 0.0f}, tr[3] = {0.0f, 0.0f, 0.0f};
-        for (int a = 0; a < D; ++a) t32[a] = (float)t[a];
-        box_relative<D>(g, c32, t32, tr);
+        float tr[3] = {0.0f, 0.0f, 0.0f};
+        {
+          const float4 te = stage_entry64<D>(g, c64, make_double4(t[0], t[1], D == 3 ? t[2] : 0.0, 0.0), 0u);
+          tr[0] = te.x;
+          tr[1] = te.y;
+          tr[2] = te.z;
+        }
         const float lo = tc.band_lo, hi = tc.band_hi;
-        for (uint32_t kk = 0; kk < rounds && !hit; ++kk) {
+        // rows of three cells; only the centre row (the trial's own cell in
+        // its middle) can hold the trial's own site.  Two rounds per step:
+        // 2 NR LDS reads in flight together.  The FP32 distance is fused
+        // (three roundings, within the band's 4 u relative term).
+        constexpr int RC = NR / 2;
+        uint32_t len[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) len[r] = rhi[r] - rlo[r];
+        for (uint32_t kk = 0; kk < rounds && !hit; kk += 2) {
           uint32_t amb = 0;
 #pragma unroll
-          for (int r = 0; r < NR; ++r) {
-            const uint32_t idx = rlo[r] + kk;
-            const bool valid = idx < rhi[r] && idx != self;
-            const float4 q = lb[valid ? idx : self];
-            const float dx = q.x - tr[0], dy = q.y - tr[1];
-            float d2 = dx * dx + dy * dy;
-            if (D == 3) {
-              const float dz = q.z - tr[2];
-              d2 = d2 + dz * dz;
+          for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+              const uint32_t k2 = kk + (uint32_t)h;
+              const uint32_t idx = rlo[r] + k2;
+              const bool valid = k2 < len[r] && (r != RC || idx != self);
+              const float4 q = lb[valid ? idx : self];
+              const float dx = q.x - tr[0], dy = q.y - tr[1];
+              float d2;
+              if (D == 3) {
+                const float dz = q.z - tr[2];
+                d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+              } else {
+                d2 = fmaf(dx, dx, dy * dy);
+              }
+              hit |= valid && d2 < lo;
+              if (valid && !(d2 < lo) && !(d2 >= hi)) amb |= 1u << (h * NR + r);
             }
-            hit |= valid && d2 < lo;
-            if (valid && !(d2 < lo) && !(d2 >= hi)) amb |= 1u << r;
           }
           while (amb && !hit) {
-            const int r = __ffs(amb) - 1;
+            const int b2 = __ffs(amb) - 1;
             amb &= amb - 1;
-            const float4 q = lb[rlo[r] + kk];
+            const int h = b2 / NR, r = b2 % NR;
+            const float4 q = lb[rlo[r] + kk + (uint32_t)h];
             ++retests;
             hit |= retest64_body<D>(g.self, B.spos, (uint32_t)__float_as_int(q.w), t[0], t[1], t[2]);
           }
@@ -1734,12 +1791,10 @@ __global__ void __launch_bounds__(kTileThreads) k_apply_tile(Geometry g, ApplySt
           double nsp[3] = {0.0, 0.0, 0.0};
           for (int a = 0; a < D; ++a) nsp[a] = wrap1(t[a] + 0.0, g.L[a]);
           const uint32_t fl = ghost_flags<D>(g, nsp);
-          const float4 s32 = pack_site32(nsp, fl);
-          const uint32_t slot = slot_of(pcrow, self);
+          const double4 sp = pack_site(nsp, fl);
           B.bpos[body] = make_double4(t[0], t[1], t[2], 0.0);
-          B.spos[body] = pack_site(nsp, fl);
-          C.pos32[slot] = s32;
-          lb[self] = stage_entry<D, HMC_MIXED>(g, c32, s32, body);
+          st_dev_d4(&B.spos[body], sp);
+          lb[self] = stage_entry64<D>(g, c64, sp, body);
           accepted = 1;
         }
       }
@@ -1760,16 +1815,16 @@ __global__ void __launch_bounds__(kTileThreads) k_apply_tile(Geometry g, ApplySt
     const unsigned long long mk = __ballot(accepted != 0);
     if (__lane_id() == 0 && mk) atomicAdd(&blk_acc, (unsigned)__popcll(mk));
     if (i + 1 == npass) break;  // no later pass reads the halo
-    region_publish(w.rflags, tile, epoch | (i + 1));
+    region_publish(w.mbox, nbt, nnb, i & 1u);
     // the next proposal reads only static cell data and this region's own
     // bodies, none of which pass i changed: drawn while the neighbours finish
     propose(i + 1);
-    region_wait(st, w.rflags, nbt, nnb, epoch | (i + 1));
+    region_wait(st, w.mbox, tile, nnb, i & 1u);
     // ---- reload the halo cells of this colour: box cells b with b = 1 - off
     // (mod 2) per axis whose space lies outside the region (the first
     // kRefreshHeld entries of a thread are loaded together)
     uint32_t hdst[kRefreshHeld];
-    float4 hval[kRefreshHeld];
+    double4 hval[kRefreshHeld];
     uint32_t nheld = 0;
     if (live && !overflow) {
       uint32_t nc = 1, cnt[3] = {1, 1, 1};
@@ -1791,7 +1846,8 @@ __global__ void __launch_bounds__(kTileThreads) k_apply_tile(Geometry g, ApplySt
         const uint32_t row = D == 3 ? bc[0] * tc.E[1] + bc[1] : bc[0];
         const uint32_t ea = coff[row * CW + bc[AL]], eb = coff[row * CW + bc[AL] + 1];
         for (uint32_t e = ea; e < eb; ++e) {
-          const float4 v = C.pos32[slot_of(row, e)];
+          const uint32_t bd = (uint32_t)__float_as_int(lb[e].w);
+          const double4 v = ld_dev_d4(&B.spos[bd]);
           if (nheld < (uint32_t)kRefreshHeld) {
 #pragma unroll
             for (int h = 0; h < kRefreshHeld; ++h)
@@ -1801,14 +1857,14 @@ __global__ void __launch_bounds__(kTileThreads) k_apply_tile(Geometry g, ApplySt
               }
             ++nheld;
           } else {
-            lb[e] = stage_entry<D, HMC_MIXED>(g, c32, v, (uint32_t)__float_as_int(lb[e].w));
+            lb[e] = stage_entry64<D>(g, c64, v, bd);
           }
         }
       }
     }
 #pragma unroll
     for (int h = 0; h < kRefreshHeld; ++h)
-      if ((uint32_t)h < nheld) lb[hdst[h]] = stage_entry<D, HMC_MIXED>(g, c32, hval[h], (uint32_t)__float_as_int(lb[hdst[h]].w));
+      if ((uint32_t)h < nheld) lb[hdst[h]] = stage_entry64<D>(g, c64, hval[h], (uint32_t)__float_as_int(lb[hdst[h]].w));
     __syncthreads();
   }
   lane_add(&st->band_retests, retests);
@@ -3404,10 +3460,16 @@ int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
         hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, c->device) == hipSuccess && coop) {
       TileCfg pc{};
       if (choose_regions(g, D, (double)cfg->n_bodies / (double)spaces, cus, &pc)) {
+        // Band of the persistent layout, whose LDS entries are the FP64
+        // box-relative positions rounded once to FP32.  Per axis: that
+        // rounding for the site and for the trial site (2^-24 R each, R the
+        // largest box-relative coordinate), the rounding of their FP32
+        // difference (2^-24 |d|), and the FP64 roundings of x - c and of the
+        // image shift on both sides as well as of the reference's own
+        // displacement (2^-53 (4 L + 2 |d|)).
         double R = 0.0;
         for (int a = 0; a < D; ++a) R = std::max(R, 0.5 * pc.E[a] * g.width[a] + g.width[a]);
-        const double Et =
-            (0x1p-24 * (5.1 * lmax + 2.0 * R + 2.0 * sigma) + 0x1p-53 * (1.5 * lmax + 2.0 * sigma)) * 1.01;
+        const double Et = (0x1p-24 * (2.0 * R + 2.0 * sigma) + 0x1p-53 * (4.0 * lmax + 2.0 * sigma)) * 1.01;
         const void* fn = D == 2 ? (const void*)k_apply_tile<2> : (const void*)k_apply_tile<3>;
         const uint32_t threads = ((pc.trials + 63) / 64) * 64;
         int per_cu = 0;
@@ -3419,8 +3481,8 @@ int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
           c->pc = pc;
           c->pgrid = pc.xcd_chunk * 8u;
           c->pthreads = threads;
-          const size_t fbytes = (size_t)pc.ntiles * kRegionFlagStride * sizeof(unsigned long long);
-          if (hipMalloc((void**)&c->w.rflags, fbytes) != hipSuccess || hipMemset(c->w.rflags, 0, fbytes) != hipSuccess) {
+          const size_t fbytes = (size_t)pc.ntiles * kMailboxStride * sizeof(uint32_t);
+          if (hipMalloc((void**)&c->w.mbox, fbytes) != hipSuccess || hipMemset(c->w.mbox, 0, fbytes) != hipSuccess) {
             delete c;
             return HMC_ERR_DEVICE;
           }
@@ -3887,7 +3949,7 @@ void hmc_destroy(hmc_ctx* c) {
                   (void*)c->gdev,
                   (void*)c->cells.pos64, (void*)c->w.cnt, (void*)c->w.start, (void*)c->w.elem_space,
                   (void*)c->w.elem_rank, (void*)c->w.tmp_body, (void*)c->w.block_sums, (void*)c->w.block_ne,
-                  (void*)c->w.acc_slots, (void*)c->w.trace, (void*)c->w.rflags, (void*)c->st})
+                  (void*)c->w.acc_slots, (void*)c->w.trace, (void*)c->w.mbox, (void*)c->st})
     if (p) (void)hipFree(p);
   collect_events(c, c->pass_events, c->pass_ms, c->pass_n);
   collect_events(c, c->build_events, c->build_ms, c->build_n);
