@@ -349,9 +349,14 @@ __global__ void __launch_bounds__(1024) k_scan_top(Geometry g, ApplyState* st, u
   }
 }
 
-__global__ void k_scan_add(uint32_t* start, const uint32_t* block_sums, uint32_t n) {
+// Adds the block offsets; with cnt given, also clears the per-space counts
+// (their last reader is k_scan_blocks) for the next build.
+__global__ void k_scan_add(uint32_t* start, const uint32_t* block_sums, uint32_t* cnt, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) start[i] += block_sums[i / kScanBlock];
+  if (i < n) {
+    start[i] += block_sums[i / kScanBlock];
+    if (cnt) cnt[i] = 0u;
+  }
 }
 
 __global__ void k_scatter(Geometry g, Work w) {
@@ -360,8 +365,9 @@ __global__ void k_scatter(Geometry g, Work w) {
   w.tmp_body[w.start[w.elem_space[b]] + w.elem_rank[b]] = b;
 }
 
-// Order each space by body index and write the cell arrays.
-template <int PREC>
+// Order each space by body index and write the cell arrays (COPY: also the
+// per-slot site copies the per-colour passes read).
+template <int PREC, bool COPY = true>
 __global__ void __launch_bounds__(kBlock) k_place(Geometry g, Bodies B, Cells C, Work w) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= g.n_bodies) return;
@@ -372,6 +378,7 @@ __global__ void __launch_bounds__(kBlock) k_place(Geometry g, Bodies B, Cells C,
   for (uint32_t q = s0; q < s1; ++q) rank += w.tmp_body[q] < b ? 1u : 0u;
   const uint32_t dst = s0 + rank;
   C.body[dst] = b;
+  if (!COPY) return;
   if (C.quat64) C.quat64[dst] = B.squat[b];
   const double4 sp = B.spos[b];
   if (PREC == HMC_MIXED)
@@ -2866,14 +2873,18 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
     (void)hipEventRecord(e0, s);
   }
   k_setup<<<1, 64, 0, s>>>(g, c->st, step, substep, seed, kT, max_move, from_args, advance, move);
-  k_fill_u32<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.cnt, g.n_spaces, 0u);
+  // (the per-space counts are zero: cleared at creation and by k_scan_add)
   if (N) k_bin<D><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
   k_scan_blocks<<<c->n_scan_blocks, 256, 0, s>>>(c->w.cnt, c->w.start, c->w.block_sums, c->w.block_ne, g.n_spaces);
   k_scan_top<<<1, 1024, 0, s>>>(g, c->st, c->w.block_sums, c->w.block_ne, c->n_scan_blocks, c->w.start, 1);
-  k_scan_add<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.start, c->w.block_sums, g.n_spaces);
+  k_scan_add<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.start, c->w.block_sums, c->w.cnt, g.n_spaces);
   if (N) {
     k_scatter<<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->w);
-    k_place<PREC><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->bodies, c->cells, c->w);
+    // the persistent apply stages its sites from the FP64 copies
+    if (c->persist)
+      k_place<PREC, false><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->bodies, c->cells, c->w);
+    else
+      k_place<PREC><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->bodies, c->cells, c->w);
   }
   if (prof) {
     e1 = take_event(c);
@@ -3056,7 +3067,7 @@ void launch_batch_step(hmc_batch* b, int from_args, int advance, uint32_t launch
   k_scan_blocks<<<b->n_scan_blocks, 256, 0, s>>>(b->w.cnt, b->w.start, b->w.block_sums, b->w.block_ne, b->s_total);
   k_scan_top<<<1, 1024, 0, s>>>(b->gtot, b->sts + b->R, b->w.block_sums, b->w.block_ne, b->n_scan_blocks, b->w.start,
                                 0);
-  k_scan_add<<<grid_for(b->s_total, 256), 256, 0, s>>>(b->w.start, b->w.block_sums, b->s_total);
+  k_scan_add<<<grid_for(b->s_total, 256), 256, 0, s>>>(b->w.start, b->w.block_sums, nullptr, b->s_total);
   k_bnonempty<<<b->R, 256, 0, s>>>(b->reps, b->sts, b->w.cnt);
   k_scatter<<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->gtot, b->w);
   k_place<PREC><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->gtot, b->bodies, b->cells, b->w);
@@ -3520,7 +3531,8 @@ int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
     return HMC_ERR_DEVICE;
   }
   if (hipMemset(c->st, 0, sizeof(ApplyState)) != hipSuccess ||
-      hipMemset(c->w.acc_slots, 0, kAccSlots * sizeof(unsigned long long)) != hipSuccess) {
+      hipMemset(c->w.acc_slots, 0, kAccSlots * sizeof(unsigned long long)) != hipSuccess ||
+      hipMemset(c->w.cnt, 0, (size_t)std::max<uint32_t>(g.n_spaces, 1) * sizeof(uint32_t)) != hipSuccess) {
     hmc_destroy(c);
     return HMC_ERR_DEVICE;
   }
@@ -3794,14 +3806,13 @@ int hmc_total_energy(hmc_ctx* c, double* energy, double* bound) {
   HIPCHK(c, hipMallocAsync((void**)&part, (4 * (size_t)nblk + 4) * sizeof(double), s));
   // cells for the neighbour search: the checkerboard of the current counter
   k_setup<<<1, 64, 0, s>>>(g, c->st, 0, 0, 0, 1.0, 1.0, 0, 0, 0u);
-  k_fill_u32<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.cnt, g.n_spaces, 0u);
   if (g.dim == 2)
     k_bin<2><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
   else
     k_bin<3><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
   k_scan_blocks<<<c->n_scan_blocks, 256, 0, s>>>(c->w.cnt, c->w.start, c->w.block_sums, c->w.block_ne, g.n_spaces);
   k_scan_top<<<1, 1024, 0, s>>>(g, c->st, c->w.block_sums, c->w.block_ne, c->n_scan_blocks, c->w.start, 0);
-  k_scan_add<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.start, c->w.block_sums, g.n_spaces);
+  k_scan_add<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.start, c->w.block_sums, c->w.cnt, g.n_spaces);
   k_scatter<<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->w);
   if (g.dim == 2)
     k_energy<2><<<nblk, kEnergyBlock, 0, s>>>(g, c->st, c->bodies, c->w, part);
