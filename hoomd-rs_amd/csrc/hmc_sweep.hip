@@ -1287,12 +1287,18 @@ __device__ __forceinline__ double4 ld_dev_d4(const double4* p) {
   return make_double4(__longlong_as_double((long long)ld_dev(q)), __longlong_as_double((long long)ld_dev(q + 1)),
                       __longlong_as_double((long long)ld_dev(q + 2)), __longlong_as_double((long long)ld_dev(q + 3)));
 }
+// Two 16-byte device-scope (sc1) stores.  Their completion is waited for
+// explicitly (s_waitcnt 0) before the region publishes the pass.
 __device__ __forceinline__ void st_dev_d4(double4* p, double4 v) {
-  uint64_t* q = (uint64_t*)p;
-  st_dev(q, (uint64_t)__double_as_longlong(v.x));
-  st_dev(q + 1, (uint64_t)__double_as_longlong(v.y));
-  st_dev(q + 2, (uint64_t)__double_as_longlong(v.z));
-  st_dev(q + 3, (uint64_t)__double_as_longlong(v.w));
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  const unsigned long long a = (unsigned long long)__double_as_longlong(v.x),
+                           b = (unsigned long long)__double_as_longlong(v.y),
+                           c = (unsigned long long)__double_as_longlong(v.z),
+                           d = (unsigned long long)__double_as_longlong(v.w);
+  const u32x4 lo = {(unsigned)a, (unsigned)(a >> 32), (unsigned)b, (unsigned)(b >> 32)};
+  const u32x4 hi = {(unsigned)c, (unsigned)(c >> 32), (unsigned)d, (unsigned)(d >> 32)};
+  asm volatile("global_store_dwordx4 %0, %1, off sc1" ::"v"(p), "v"(lo) : "memory");
+  asm volatile("global_store_dwordx4 %0, %1, off offset:16 sc1" ::"v"(p), "v"(hi) : "memory");
 }
 
 // Neighbour synchronisation of the persistent apply, through mailboxes: two
