@@ -2366,11 +2366,139 @@ __device__ void trial_convex(const Geometry& g, const ApplyState* st, const Bodi
   }
 }
 
+// The same trial with one wavefront: every lane draws the same proposal, the
+// candidate sites (those of the de-duplicated neighbour spaces, as segments
+// of slots) are spread over the lanes 64 at a time, and the trial stops at
+// the first round that finds an overlap (any overlap rejects: the order of
+// the pair tests does not change the decision).
+__device__ void trial_convex_wave(const Geometry& g, const ApplyState* st, const Bodies& B, const Cells& C,
+                                  const Work& w, const double* zx, const double* zf, const double* vx, uint32_t s,
+                                  const int32_t* m, uint32_t s0, uint32_t n, uint32_t colour, uint32_t substep,
+                                  TrialOut& o) {
+  const uint32_t lane = __lane_id();
+  Sfc64 rng = counter_rng(st->step, substep, st->seed, s);
+  const uint32_t pick = choose_index(rng, n);
+  const uint32_t slot = s0 + pick;
+  const uint32_t body = C.body[slot];
+  const double4 bp = B.bpos[body];
+  const double4 bq = B.bquat[body];
+  double x[3] = {bp.x, bp.y, bp.z};
+  quat q = {bq.x, {bq.y, bq.z, bq.w}};
+  unsigned exp_slow = 0;
+  if (st->move == 0) {
+    double v[3] = {0.0, 0.0, 0.0};
+    ball_sample<3>(rng, st->max_move, zx, zf, v, &exp_slow);
+    for (int a = 0; a < 3; ++a) x[a] += v[a];
+  } else {
+    q = qmul(versor_displacement(rng, st->max_move, zx, zf, &exp_slow), q);
+  }
+  o.exp_slow = lane == 0 ? exp_slow : 0u;
+  for (int a = 0; a < 3; ++a) x[a] = wrap1(x[a], g.L[a]);  // wrap(propose(..))
+  bool accept = false;
+  if (space_of<3>(g, st->origin, x) == s) {
+    const v3 r0 = qrotate(q, v3{0.0, 0.0, 0.0});
+    const double t[3] = {wrap1(x[0] + r0.x, g.L[0]), wrap1(x[1] + r0.y, g.L[1]), wrap1(x[2] + r0.z, g.L[2])};
+    const quat qt = qmul(q, quat{1.0, {0.0, 0.0, 0.0}});
+    const v3 tv = {t[0], t[1], t[2]};
+    uint32_t nb[3][5];
+    uint32_t nn[3] = {1, 1, 1};
+    for (int a = 0; a < 3; ++a) {
+      const int32_t sh = g.shape[a], r = g.reach[a];
+      uint32_t k = 0;
+      for (int off = -r; off <= r; ++off) {
+        const int32_t mm = ((m[a] + off) % sh + sh) % sh;
+        bool seen = false;
+        for (uint32_t j = 0; j < k; ++j) seen |= nb[a][j] == (uint32_t)mm * g.stride[a];
+        if (!seen) nb[a][k++] = (uint32_t)mm * g.stride[a];
+      }
+      nn[a] = k;
+    }
+    const uint32_t NS = nn[0] * nn[1] * nn[2];  // at most 125
+    uint32_t seg_lo[2] = {0, 0}, seg_pre[2] = {0, 0};
+    uint32_t base = 0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t j = lane + 64u * h;
+      uint32_t cnt = 0, lo = 0;
+      if (j < NS) {
+        const uint32_t i2 = j % nn[2], i1 = (j / nn[2]) % nn[1], i0 = j / (nn[2] * nn[1]);
+        const uint32_t ns = nb[0][i0] + nb[1][i1] + nb[2][i2];
+        lo = w.start[ns];
+        cnt = w.start[ns + 1] - lo;
+      }
+      const uint32_t incl = wave_incl_scan(cnt);
+      seg_lo[h] = lo;
+      seg_pre[h] = base + incl - cnt;
+      base += __shfl(incl, 63, 64);
+    }
+    const uint32_t total = base;
+    bool hit = false;  // uniform over the wavefront
+    for (uint32_t c0 = 0; c0 < total && !hit; c0 += 64) {
+      const uint32_t c = c0 + lane;
+      // segment of candidate c (as in trial_lj): fixed-depth binary search
+      // over the segment prefixes held in two registers per lane
+      uint32_t jlo = 0;
+#pragma unroll
+      for (uint32_t stepw = 64; stepw >= 1; stepw >>= 1) {
+        const uint32_t mid = jlo + stepw;
+        const uint32_t pa = __shfl(seg_pre[0], (int)(mid & 63u), 64);
+        const uint32_t pb = __shfl(seg_pre[1], (int)(mid & 63u), 64);
+        if (mid < NS && (mid < 64 ? pa : pb) <= c) jlo = mid;
+      }
+      const uint32_t pa = __shfl(seg_pre[0], (int)(jlo & 63u), 64), pb = __shfl(seg_pre[1], (int)(jlo & 63u), 64);
+      const uint32_t la = __shfl(seg_lo[0], (int)(jlo & 63u), 64), lb2 = __shfl(seg_lo[1], (int)(jlo & 63u), 64);
+      const uint32_t qs = (jlo < 64 ? la : lb2) + (c - (jlo < 64 ? pa : pb));
+      bool h = false;
+      if (c < total && qs != slot) {
+        v3 rj;
+        const double d2 = convex_image(g, t, C.pos64[qs], &rj);
+        if (d2 < g.cv_rcut2) {
+          const double4 qj4 = C.quat64[qs];
+          h = convex_overlap(vx, g.nverts, g.cv_rcut2, d2, tv, qt, rj, quat{qj4.x, {qj4.y, qj4.z, qj4.w}});
+        }
+      }
+      hit = __ballot(h) != 0ull;
+    }
+    accept = !hit;  // hard shapes: dH is 0 or +inf (parallel_sweep.rs:267-273)
+    if (accept && lane == 0) {
+      // update_body_properties (microstate.rs:985-1034): body = wrap(props),
+      // site = wrap(body.transform(site))
+      double nb3[3], ns3[3];
+      for (int a = 0; a < 3; ++a) nb3[a] = wrap1(x[a], g.L[a]);
+      const double r0a[3] = {r0.x, r0.y, r0.z};
+      for (int a = 0; a < 3; ++a) ns3[a] = wrap1(nb3[a] + r0a[a], g.L[a]);
+      const uint32_t fl = ghost_flags<3>(g, ns3);
+      B.bpos[body] = make_double4(nb3[0], nb3[1], nb3[2], 0.0);
+      B.bquat[body] = make_double4(q.s, q.v.x, q.v.y, q.v.z);
+      const double4 sp = pack_site(ns3, fl);
+      const double4 sq = make_double4(qt.s, qt.v.x, qt.v.y, qt.v.z);
+      B.spos[body] = sp;
+      B.squat[body] = sq;
+      C.pos64[slot] = sp;
+      C.quat64[slot] = sq;
+    }
+  }
+  o.accepted = (accept && lane == 0) ? 1u : 0u;
+  if (w.tracing && lane == 0) {
+    unsigned long long ti = atomicAdd((unsigned long long*)&st->trace_n, 1ull);
+    if (ti < w.trace_cap) {
+      hmc_trace_record rec;
+      rec.substep = substep;
+      rec.color = colour;
+      rec.space = s;
+      rec.body = (int64_t)body;
+      rec.status = accept ? 1 : 0;
+      rec.pad = 0;
+      w.trace[ti] = rec;
+    }
+  }
+}
+
 __device__ __forceinline__ void stage_verts(const Geometry& g, double* vx) {
   for (uint32_t i = threadIdx.x; i < 3 * g.nverts; i += blockDim.x) vx[i] = g.verts[i];
 }
 
-// One colour of one cycle for convex polyhedra: one lane per space.
+// One colour of one cycle for convex polyhedra: one wavefront per space.
 __global__ void __launch_bounds__(kBlock) k_pass_convex(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
                                                         uint32_t colour, uint32_t cycle, uint32_t xcd_chunk) {
   if (cycle >= st->cycles) return;  // uniform over the grid
@@ -2383,17 +2511,16 @@ __global__ void __launch_bounds__(kBlock) k_pass_convex(Geometry g, ApplyState* 
   __syncthreads();
   const uint32_t b = blockIdx.x;
   const uint32_t lb = (b & 7u) * xcd_chunk + (b >> 3);
-  const uint32_t t = lb * kBlock + threadIdx.x;
+  const uint32_t t = lb * (kBlock / 64) + (threadIdx.x >> 6);  // one space per wavefront
   TrialOut o{0, 0, 0};
   if (t < g.per_colour) {
     int32_t m[3] = {0, 0, 0};
     const uint32_t s = colour_space<3>(g, colour, t, m);
     const uint32_t s0 = w.start[s];
     const uint32_t n = w.start[s + 1] - s0;
-    if (n) trial_convex(g, st, B, C, w, zx, zf, vx, s, m, s0, n, colour, st->substep0 + cycle, o);
+    if (n) trial_convex_wave(g, st, B, C, w, zx, zf, vx, s, m, s0, n, colour, st->substep0 + cycle, o);
   }
-  const unsigned long long mk = __ballot(o.accepted != 0);
-  if (__lane_id() == 0 && mk) atomicAdd(&blk_acc, (unsigned)__popcll(mk));
+  if (o.accepted) atomicAdd(&blk_acc, 1u);
   lane_add(&st->exp_slow, o.exp_slow);
   __syncthreads();
   if (threadIdx.x == 0 && blk_acc) atomicAdd(&w.acc_slots[b & (kAccSlots - 1)], (unsigned long long)blk_acc);
@@ -2939,7 +3066,8 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
       if (lj)
         k_pass_lj<D><<<lj_chunk * 8, kBlock, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k, lj_chunk);
       else if (convex)
-        k_pass_convex<<<grid, kBlock, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k, chunk);
+        k_pass_convex<<<lj_chunk * 8, kBlock, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k,
+                                                      lj_chunk);
       else if (c->tiled)
         k_pass_tile<D, PREC><<<c->tc.xcd_chunk * 8, tile_threads, c->tc.lds_bytes, s>>>(
             g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k, c->tc);
