@@ -2366,101 +2366,228 @@ __device__ void trial_convex(const Geometry& g, const ApplyState* st, const Bodi
   }
 }
 
-// The same trial with one wavefront: every lane draws the same proposal, the
-// candidate sites (those of the de-duplicated neighbour spaces, as segments
-// of slots) are spread over the lanes 64 at a time, and the trial stops at
-// the first round that finds an overlap (any overlap rejects: the order of
-// the pair tests does not change the decision).
-__device__ void trial_convex_wave(const Geometry& g, const ApplyState* st, const Bodies& B, const Cells& C,
-                                  const Work& w, const double* zx, const double* zf, const double* vx, uint32_t s,
-                                  const int32_t* m, uint32_t s0, uint32_t n, uint32_t colour, uint32_t substep,
-                                  TrialOut& o) {
+// Convex pass layout: kCvTrials trials per wavefront, one per group of
+// kCvLanes lanes (the group draws its trial's proposal and lists its
+// neighbour spaces); the candidate sites of all the wavefront's trials are
+// tested against the circumsphere cutoff 64 at a time, the pairs within it
+// are gathered in a per-wavefront list, and XenoCollide runs over that list
+// 64 pairs at a time.  Any overlap rejects a hard-shape trial, so the order
+// of the pair tests does not change a decision (parallel_sweep.rs:267-273).
+constexpr uint32_t kCvTrials = 4;
+constexpr uint32_t kCvLanes = 64 / kCvTrials;
+constexpr uint32_t kCvSegs = 128;   // neighbour spaces of one trial: at most 5^3
+constexpr uint32_t kCvPairs = 128;  // pending pair tests of one wavefront
+struct CvWave {
+  uint32_t pre[kCvTrials][kCvSegs];  // index of the first candidate of each neighbour space, per trial
+  uint32_t lo[kCvTrials][kCvSegs];   // its first slot
+  double rx[kCvPairs], ry[kCvPairs], rz[kCvPairs];  // image of the pair's neighbour site
+  uint32_t qs[kCvPairs];                            // the neighbour's slot
+  uint32_t tk[kCvPairs];                            // the pair's trial
+  double t[kCvTrials][3];                           // trial site position
+  double qt[kCvTrials][4];                          // trial site orientation
+};
+
+// Orders the LDS accesses of one wavefront (lanes read what other lanes of
+// the same wavefront wrote; LDS operations of a wavefront complete in order).
+__device__ __forceinline__ void wave_lds_order() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// XenoCollide over the np listed pairs; returns the trials (bits) that
+// found an overlap, skipping pairs of trials already in `hits`.
+__device__ __forceinline__ uint32_t cv_pair_tests(const Geometry& g, const Cells& C, const CvWave& W,
+                                                  const double* vx, uint32_t np, uint32_t hits) {
   const uint32_t lane = __lane_id();
-  Sfc64 rng = counter_rng(st->step, substep, st->seed, s);
-  const uint32_t pick = choose_index(rng, n);
-  const uint32_t slot = s0 + pick;
-  const uint32_t body = C.body[slot];
-  const double4 bp = B.bpos[body];
-  const double4 bq = B.bquat[body];
-  double x[3] = {bp.x, bp.y, bp.z};
-  quat q = {bq.x, {bq.y, bq.z, bq.w}};
-  unsigned exp_slow = 0;
-  if (st->move == 0) {
-    double v[3] = {0.0, 0.0, 0.0};
-    ball_sample<3>(rng, st->max_move, zx, zf, v, &exp_slow);
-    for (int a = 0; a < 3; ++a) x[a] += v[a];
-  } else {
-    q = qmul(versor_displacement(rng, st->max_move, zx, zf, &exp_slow), q);
+  for (uint32_t p0 = 0; p0 < np; p0 += 64) {
+    const uint32_t p = p0 + lane;
+    bool h = false;
+    uint32_t kp = 0;
+    if (p < np) {
+      kp = W.tk[p];
+      if (!((hits >> kp) & 1u)) {
+        const double4 qj4 = C.quat64[W.qs[p]];
+        const v3 rj = {W.rx[p], W.ry[p], W.rz[p]};
+        const v3 tv = {W.t[kp][0], W.t[kp][1], W.t[kp][2]};
+        const quat qt = {W.qt[kp][0], {W.qt[kp][1], W.qt[kp][2], W.qt[kp][3]}};
+        // listed pairs are within the cutoff: 0 stands for their distance
+        h = convex_overlap(vx, g.nverts, g.cv_rcut2, 0.0, tv, qt, rj, quat{qj4.x, {qj4.y, qj4.z, qj4.w}});
+      }
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < kCvTrials; ++i)
+      if (__ballot(h && kp == i)) hits |= 1u << i;
   }
-  o.exp_slow = lane == 0 ? exp_slow : 0u;
-  for (int a = 0; a < 3; ++a) x[a] = wrap1(x[a], g.L[a]);  // wrap(propose(..))
-  bool accept = false;
-  if (space_of<3>(g, st->origin, x) == s) {
-    const v3 r0 = qrotate(q, v3{0.0, 0.0, 0.0});
-    const double t[3] = {wrap1(x[0] + r0.x, g.L[0]), wrap1(x[1] + r0.y, g.L[1]), wrap1(x[2] + r0.z, g.L[2])};
-    const quat qt = qmul(q, quat{1.0, {0.0, 0.0, 0.0}});
-    const v3 tv = {t[0], t[1], t[2]};
-    uint32_t nb[3][5];
-    uint32_t nn[3] = {1, 1, 1};
+  return hits;
+}
+
+__device__ __forceinline__ void stage_verts(const Geometry& g, double* vx) {
+  for (uint32_t i = threadIdx.x; i < 3 * g.nverts; i += blockDim.x) vx[i] = g.verts[i];
+}
+
+// One colour of one cycle for convex polyhedra: kCvTrials spaces per
+// wavefront.  Each trial is the one of trial_convex (parallel_sweep.rs:234-277
+// with Translate or Rotate<Versor>).
+__global__ void __launch_bounds__(kBlock) k_pass_convex(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
+                                                        uint32_t colour, uint32_t cycle, uint32_t xcd_chunk) {
+  if (cycle >= st->cycles) return;  // uniform over the grid
+  __shared__ unsigned blk_acc;
+  __shared__ double zx[257], zf[257];
+  __shared__ double vx[3 * kMaxVertices];
+  __shared__ CvWave cw[kBlock / 64];
+  if (threadIdx.x == 0) blk_acc = 0;
+  stage_zig(zx, zf);
+  stage_verts(g, vx);
+  __syncthreads();
+  const uint32_t b = blockIdx.x;
+  const uint32_t lb = (b & 7u) * xcd_chunk + (b >> 3);
+  const uint32_t lane = __lane_id(), k = lane / kCvLanes, kl = lane % kCvLanes;
+  CvWave& W = cw[threadIdx.x >> 6];
+  const uint32_t tix = (lb * (kBlock / 64) + (threadIdx.x >> 6)) * kCvTrials + k;
+  const uint32_t substep = st->substep0 + cycle;
+  // the trial of lane group k (all its lanes draw the same proposal)
+  bool tried = false, active = false;
+  uint32_t s = 0, slot = 0xFFFFFFFFu, body = 0, exp_slow = 0;
+  int32_t m[3] = {0, 0, 0};
+  double x[3] = {0.0, 0.0, 0.0};
+  quat q = {1.0, {0.0, 0.0, 0.0}};
+  if (tix < g.per_colour) {
+    s = colour_space<3>(g, colour, tix, m);
+    const uint32_t s0 = w.start[s];
+    const uint32_t n = w.start[s + 1] - s0;
+    if (n) {
+      tried = true;
+      Sfc64 rng = counter_rng(st->step, substep, st->seed, s);
+      slot = s0 + choose_index(rng, n);
+      body = C.body[slot];
+      const double4 bp = B.bpos[body];
+      const double4 bq = B.bquat[body];
+      x[0] = bp.x;
+      x[1] = bp.y;
+      x[2] = bp.z;
+      q = {bq.x, {bq.y, bq.z, bq.w}};
+      if (st->move == 0) {
+        double v[3] = {0.0, 0.0, 0.0};
+        ball_sample<3>(rng, st->max_move, zx, zf, v, &exp_slow);
+        for (int a = 0; a < 3; ++a) x[a] += v[a];
+      } else {
+        q = qmul(versor_displacement(rng, st->max_move, zx, zf, &exp_slow), q);
+      }
+      for (int a = 0; a < 3; ++a) x[a] = wrap1(x[a], g.L[a]);  // wrap(propose(..))
+      active = space_of<3>(g, st->origin, x) == s;
+    }
+  }
+  // the trial site: wrap(x + rotate(q, 0)) with orientation q * identity
+  // (property/oriented_point.rs:107-113)
+  const v3 r0 = qrotate(q, v3{0.0, 0.0, 0.0});
+  const quat qt = qmul(q, quat{1.0, {0.0, 0.0, 0.0}});
+  if (active && kl == 0) {
+    W.t[k][0] = wrap1(x[0] + r0.x, g.L[0]);
+    W.t[k][1] = wrap1(x[1] + r0.y, g.L[1]);
+    W.t[k][2] = wrap1(x[2] + r0.z, g.L[2]);
+    W.qt[k][0] = qt.s;
+    W.qt[k][1] = qt.v.x;
+    W.qt[k][2] = qt.v.y;
+    W.qt[k][3] = qt.v.z;
+  }
+  // the trial's neighbour spaces (distinct ones, as the reference's cell
+  // list visits each once) and the first candidate index of each
+  uint32_t nb[3][5];
+  uint32_t nn[3] = {0, 0, 0};
+  if (active) {
     for (int a = 0; a < 3; ++a) {
       const int32_t sh = g.shape[a], r = g.reach[a];
-      uint32_t k = 0;
+      uint32_t c = 0;
       for (int off = -r; off <= r; ++off) {
         const int32_t mm = ((m[a] + off) % sh + sh) % sh;
         bool seen = false;
-        for (uint32_t j = 0; j < k; ++j) seen |= nb[a][j] == (uint32_t)mm * g.stride[a];
-        if (!seen) nb[a][k++] = (uint32_t)mm * g.stride[a];
+        for (uint32_t j = 0; j < c; ++j) seen |= nb[a][j] == (uint32_t)mm * g.stride[a];
+        if (!seen) nb[a][c++] = (uint32_t)mm * g.stride[a];
       }
-      nn[a] = k;
+      nn[a] = c;
     }
-    const uint32_t NS = nn[0] * nn[1] * nn[2];  // at most 125
-    uint32_t seg_lo[2] = {0, 0}, seg_pre[2] = {0, 0};
-    uint32_t base = 0;
+  }
+  const uint32_t NS = nn[0] * nn[1] * nn[2];  // at most 125
+  uint32_t tot = 0;
+  for (uint32_t j0 = 0; __ballot(j0 < NS) != 0ull; j0 += kCvLanes) {
+    const uint32_t j = j0 + kl;
+    uint32_t cnt = 0, lo = 0;
+    if (j < NS) {
+      const uint32_t i2 = j % nn[2], i1 = (j / nn[2]) % nn[1], i0 = j / (nn[2] * nn[1]);
+      const uint32_t ns = nb[0][i0] + nb[1][i1] + nb[2][i2];
+      lo = w.start[ns];
+      cnt = w.start[ns + 1] - lo;
+    }
+    uint32_t incl = cnt;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const uint32_t j = lane + 64u * h;
-      uint32_t cnt = 0, lo = 0;
-      if (j < NS) {
-        const uint32_t i2 = j % nn[2], i1 = (j / nn[2]) % nn[1], i0 = j / (nn[2] * nn[1]);
-        const uint32_t ns = nb[0][i0] + nb[1][i1] + nb[2][i2];
-        lo = w.start[ns];
-        cnt = w.start[ns + 1] - lo;
-      }
-      const uint32_t incl = wave_incl_scan(cnt);
-      seg_lo[h] = lo;
-      seg_pre[h] = base + incl - cnt;
-      base += __shfl(incl, 63, 64);
+    for (uint32_t d = 1; d < kCvLanes; d <<= 1) {
+      const uint32_t u = __shfl_up(incl, d, kCvLanes);
+      if (kl >= d) incl += u;
     }
-    const uint32_t total = base;
-    bool hit = false;  // uniform over the wavefront
-    for (uint32_t c0 = 0; c0 < total && !hit; c0 += 64) {
-      const uint32_t c = c0 + lane;
-      // segment of candidate c (as in trial_lj): fixed-depth binary search
-      // over the segment prefixes held in two registers per lane
+    if (j < NS) {
+      W.pre[k][j] = tot + incl - cnt;
+      W.lo[k][j] = lo;
+    }
+    tot += __shfl(incl, (int)(lane | (kCvLanes - 1)), 64);
+  }
+  uint32_t base[kCvTrials + 1];
+  base[0] = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < kCvTrials; ++i) base[i + 1] = base[i] + __shfl(tot, (int)(i * kCvLanes), 64);
+  const uint32_t total = base[kCvTrials];
+  wave_lds_order();
+  uint32_t hits = 0;  // trials with an overlap (uniform over the wavefront)
+  uint32_t np = 0;    // listed pairs (uniform)
+  for (uint32_t c0 = 0; c0 < total; c0 += 64) {
+    const uint32_t c = c0 + lane;
+    uint32_t kc = 0, bk = 0;
+#pragma unroll
+    for (uint32_t i = 1; i < kCvTrials; ++i)
+      if (c >= base[i]) {
+        kc = i;
+        bk = base[i];
+      }
+    const uint32_t nsk = __shfl(NS, (int)(kc * kCvLanes), 64);
+    const uint32_t slk = __shfl(slot, (int)(kc * kCvLanes), 64);
+    bool inr = false;
+    uint32_t qs = 0;
+    v3 rj = {0.0, 0.0, 0.0};
+    if (c < total && !((hits >> kc) & 1u)) {
+      // neighbour space of candidate c: last one whose first index is <= c
+      const uint32_t cl = c - bk;
       uint32_t jlo = 0;
 #pragma unroll
-      for (uint32_t stepw = 64; stepw >= 1; stepw >>= 1) {
-        const uint32_t mid = jlo + stepw;
-        const uint32_t pa = __shfl(seg_pre[0], (int)(mid & 63u), 64);
-        const uint32_t pb = __shfl(seg_pre[1], (int)(mid & 63u), 64);
-        if (mid < NS && (mid < 64 ? pa : pb) <= c) jlo = mid;
+      for (uint32_t sw = kCvSegs / 2; sw >= 1; sw >>= 1) {
+        const uint32_t mid = jlo + sw;
+        if (mid < nsk && W.pre[kc][mid] <= cl) jlo = mid;
       }
-      const uint32_t pa = __shfl(seg_pre[0], (int)(jlo & 63u), 64), pb = __shfl(seg_pre[1], (int)(jlo & 63u), 64);
-      const uint32_t la = __shfl(seg_lo[0], (int)(jlo & 63u), 64), lb2 = __shfl(seg_lo[1], (int)(jlo & 63u), 64);
-      const uint32_t qs = (jlo < 64 ? la : lb2) + (c - (jlo < 64 ? pa : pb));
-      bool h = false;
-      if (c < total && qs != slot) {
-        v3 rj;
-        const double d2 = convex_image(g, t, C.pos64[qs], &rj);
-        if (d2 < g.cv_rcut2) {
-          const double4 qj4 = C.quat64[qs];
-          h = convex_overlap(vx, g.nverts, g.cv_rcut2, d2, tv, qt, rj, quat{qj4.x, {qj4.y, qj4.z, qj4.w}});
-        }
+      qs = W.lo[kc][jlo] + (cl - W.pre[kc][jlo]);
+      if (qs != slk) {
+        const double tq[3] = {W.t[kc][0], W.t[kc][1], W.t[kc][2]};
+        inr = convex_image(g, tq, C.pos64[qs], &rj) < g.cv_rcut2;
       }
-      hit = __ballot(h) != 0ull;
     }
-    accept = !hit;  // hard shapes: dH is 0 or +inf (parallel_sweep.rs:267-273)
-    if (accept && lane == 0) {
+    const unsigned long long mk = __ballot(inr);
+    if (inr) {
+      const uint32_t p = np + (uint32_t)__popcll(mk & ((1ull << lane) - 1ull));
+      W.rx[p] = rj.x;
+      W.ry[p] = rj.y;
+      W.rz[p] = rj.z;
+      W.qs[p] = qs;
+      W.tk[p] = kc;
+    }
+    np += (uint32_t)__popcll(mk);
+    if (np + 64 > kCvPairs || c0 + 64 >= total) {  // uniform
+      wave_lds_order();
+      hits = cv_pair_tests(g, C, W, vx, np, hits);
+      np = 0;
+      wave_lds_order();
+    }
+  }
+  const bool accept = active && !((hits >> k) & 1u);  // hard shapes: dH is 0 or +inf
+  if (kl == 0 && tried) {
+    if (accept) {
       // update_body_properties (microstate.rs:985-1034): body = wrap(props),
       // site = wrap(body.transform(site))
       double nb3[3], ns3[3];
@@ -2476,52 +2603,23 @@ __device__ void trial_convex_wave(const Geometry& g, const ApplyState* st, const
       B.squat[body] = sq;
       C.pos64[slot] = sp;
       C.quat64[slot] = sq;
+      atomicAdd(&blk_acc, 1u);
+    }
+    if (w.tracing) {
+      unsigned long long ti = atomicAdd((unsigned long long*)&st->trace_n, 1ull);
+      if (ti < w.trace_cap) {
+        hmc_trace_record rec;
+        rec.substep = substep;
+        rec.color = colour;
+        rec.space = s;
+        rec.body = (int64_t)body;
+        rec.status = accept ? 1 : 0;
+        rec.pad = 0;
+        w.trace[ti] = rec;
+      }
     }
   }
-  o.accepted = (accept && lane == 0) ? 1u : 0u;
-  if (w.tracing && lane == 0) {
-    unsigned long long ti = atomicAdd((unsigned long long*)&st->trace_n, 1ull);
-    if (ti < w.trace_cap) {
-      hmc_trace_record rec;
-      rec.substep = substep;
-      rec.color = colour;
-      rec.space = s;
-      rec.body = (int64_t)body;
-      rec.status = accept ? 1 : 0;
-      rec.pad = 0;
-      w.trace[ti] = rec;
-    }
-  }
-}
-
-__device__ __forceinline__ void stage_verts(const Geometry& g, double* vx) {
-  for (uint32_t i = threadIdx.x; i < 3 * g.nverts; i += blockDim.x) vx[i] = g.verts[i];
-}
-
-// One colour of one cycle for convex polyhedra: one wavefront per space.
-__global__ void __launch_bounds__(kBlock) k_pass_convex(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
-                                                        uint32_t colour, uint32_t cycle, uint32_t xcd_chunk) {
-  if (cycle >= st->cycles) return;  // uniform over the grid
-  __shared__ unsigned blk_acc;
-  __shared__ double zx[257], zf[257];
-  __shared__ double vx[3 * kMaxVertices];
-  if (threadIdx.x == 0) blk_acc = 0;
-  stage_zig(zx, zf);
-  stage_verts(g, vx);
-  __syncthreads();
-  const uint32_t b = blockIdx.x;
-  const uint32_t lb = (b & 7u) * xcd_chunk + (b >> 3);
-  const uint32_t t = lb * (kBlock / 64) + (threadIdx.x >> 6);  // one space per wavefront
-  TrialOut o{0, 0, 0};
-  if (t < g.per_colour) {
-    int32_t m[3] = {0, 0, 0};
-    const uint32_t s = colour_space<3>(g, colour, t, m);
-    const uint32_t s0 = w.start[s];
-    const uint32_t n = w.start[s + 1] - s0;
-    if (n) trial_convex_wave(g, st, B, C, w, zx, zf, vx, s, m, s0, n, colour, st->substep0 + cycle, o);
-  }
-  if (o.accepted) atomicAdd(&blk_acc, 1u);
-  lane_add(&st->exp_slow, o.exp_slow);
+  lane_add(&st->exp_slow, kl == 0 ? exp_slow : 0u);
   __syncthreads();
   if (threadIdx.x == 0 && blk_acc) atomicAdd(&w.acc_slots[b & (kAccSlots - 1)], (unsigned long long)blk_acc);
 }
@@ -3056,6 +3154,7 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
   const uint32_t launched = c->cycles_guess + (lj ? 1u : 0u);
   const uint32_t lj_blocks = grid_for(g.per_colour, kBlock / 64);
   const uint32_t lj_chunk = (lj_blocks + 7) / 8;
+  const uint32_t cv_chunk = (grid_for(g.per_colour, (kBlock / 64) * kCvTrials) + 7) / 8;
   for (uint32_t k = 0; k < launched; ++k) {
     for (int col = 0; col < (1 << D); ++col) {
       hipEvent_t p0 = nullptr, p1 = nullptr;
@@ -3066,8 +3165,8 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
       if (lj)
         k_pass_lj<D><<<lj_chunk * 8, kBlock, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k, lj_chunk);
       else if (convex)
-        k_pass_convex<<<lj_chunk * 8, kBlock, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k,
-                                                      lj_chunk);
+        k_pass_convex<<<cv_chunk * 8, kBlock, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k,
+                                                      cv_chunk);
       else if (c->tiled)
         k_pass_tile<D, PREC><<<c->tc.xcd_chunk * 8, tile_threads, c->tc.lds_bytes, s>>>(
             g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k, c->tc);
