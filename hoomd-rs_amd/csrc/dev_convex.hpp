@@ -69,11 +69,15 @@ __device__ __forceinline__ v3 mrot_t(const mat3& m, const v3& x) {
 
 // ConvexPolytope::support_mapping (shape/convex_polytope.rs:220-244): the
 // vertex maximising (x n.x + y n.y) + z n.z, first maximum wins.  vx holds
-// nv vertices as x, y, z triples.
+// nv vertices as x, y, z triples; NV > 0 fixes nv at compile time (the scan
+// is unrolled, with the vertex reads at constant offsets).
+template <int NV>
 __device__ __forceinline__ v3 support(const double* vx, uint32_t nv, const v3& n) {
+  const uint32_t cnt = NV > 0 ? (uint32_t)NV : nv;
   uint32_t best = 0;
   double maxv = -__builtin_inf();
-  for (uint32_t i = 0; i < nv; ++i) {
+#pragma unroll
+  for (uint32_t i = 0; i < cnt; ++i) {
     const double x = (vx[3 * i] * n.x + vx[3 * i + 1] * n.y) + vx[3 * i + 2] * n.z;
     if (x > maxv) {
       best = i;
@@ -84,18 +88,21 @@ __device__ __forceinline__ v3 support(const double* vx, uint32_t nv, const v3& n
 }
 
 // xenocollide.rs:320-329: S(n) = R sup(R^-1 n) + v_ij - sup(-n)
+template <int NV>
 __device__ __forceinline__ v3 minkowski(const double* vx, uint32_t nv, const mat3& R, const v3& vij, const v3& n) {
-  const v3 sb = add3(mrot(R, support(vx, nv, mrot_t(R, n))), vij);
-  return sub3(sb, support(vx, nv, neg3(n)));
+  const v3 sb = add3(mrot(R, support<NV>(vx, nv, mrot_t(R, n))), vij);
+  return sub3(sb, support<NV>(vx, nv, neg3(n)));
 }
 
 __device__ __forceinline__ bool tiny3(const v3& a, double tol) {
   return fabs(a.x) < tol && fabs(a.y) < tol && fabs(a.z) < tol;
 }
 
-// collide3d for a polytope with itself (xenocollide.rs:157-292, 378-420).
-__device__ __noinline__ bool collide3d(const double* vx, uint32_t nv, double vx0, double vy0, double vz0, double qs,
-                                       double qx, double qy, double qz) {
+// collide3d for a polytope with itself (xenocollide.rs:157-292, 378-420);
+// NV as for support().
+template <int NV>
+__device__ __forceinline__ bool collide3d_n(const double* vx, uint32_t nv, double vx0, double vy0, double vz0,
+                                            double qs, double qx, double qy, double qz) {
   const v3 vij = {vx0, vy0, vz0};
   const quat qij = {qs, {qx, qy, qz}};
   const double TOL = 2e-12;
@@ -103,11 +110,11 @@ __device__ __noinline__ bool collide3d(const double* vx, uint32_t nv, double vx0
   const mat3 R = rotmat(qij);
   const v3 v0 = vij;
   if (tiny3(v0, TOL)) return true;
-  v3 v1 = minkowski(vx, nv, R, vij, neg3(v0));
+  v3 v1 = minkowski<NV>(vx, nv, R, vij, neg3(v0));
   if (dot3(v1, v0) > 0.0) return false;
   v3 n = cross3(v1, v0);
   if (tiny3(n, TOL)) return true;
-  v3 v2 = minkowski(vx, nv, R, vij, n);
+  v3 v2 = minkowski<NV>(vx, nv, R, vij, n);
   if (dot3(v2, n) < 0.0) return false;
   n = cross3(sub3(v1, v0), sub3(v2, v0));
   if (dot3(n, v0) > 0.0) {
@@ -121,7 +128,7 @@ __device__ __noinline__ bool collide3d(const double* vx, uint32_t nv, double vx0
   for (;;) {
     count += 1;
     if (count >= MAX_ITER) return true;
-    v3p = minkowski(vx, nv, R, vij, n);
+    v3p = minkowski<NV>(vx, nv, R, vij, n);
     if (dot3(v3p, n) <= 0.0) return false;
     if (dot3(cross3(v1, v3p), v0) < 0.0) {
       v2 = v3p;
@@ -142,7 +149,7 @@ __device__ __noinline__ bool collide3d(const double* vx, uint32_t nv, double vx0
     v3 nrm = cross3(sub3(p1, p0), sub3(p2, p0));
     if (dot3(sub3(p0, v0), nrm) < 0.0) nrm = neg3(nrm);
     if (dot3(p0, nrm) >= 0.0) return true;
-    const v3 vn = minkowski(vx, nv, R, vij, nrm);
+    const v3 vn = minkowski<NV>(vx, nv, R, vij, nrm);
     if (dot3(vn, nrm) < 0.0) return false;
     const double tol = TOL * sqrt(dot3(nrm, nrm));
     if (fabs(dot3(sub3(vn, p0), nrm)) < tol) return false;
@@ -161,10 +168,18 @@ __device__ __noinline__ bool collide3d(const double* vx, uint32_t nv, double vx0
   }
 }
 
+__device__ __noinline__ bool collide3d(const double* vx, uint32_t nv, double vx0, double vy0, double vz0, double qs,
+                                       double qx, double qy, double qz) {
+  return collide3d_n<0>(vx, nv, vx0, vy0, vz0, qs, qx, qy, qz);
+}
+
 // Convex::intersects_at_global (hoomd-geometry/src/convex.rs:154-170): the
 // circumsphere test on the squared distance d2 (computed by the caller from
 // the same image), then pair_system_to_local (hoomd-vector/src/lib.rs:878-888)
 // and collide3d.
+// NV > 0: collide3d inlined with that vertex count; otherwise the out-of-line
+// one for any vertex count.
+template <int NV = 0>
 __device__ __forceinline__ bool convex_overlap(const double* vx, uint32_t nv, double rcut2, double d2, const v3& ri,
                                                const quat& qi, const v3& rj, const quat& qj) {
   if (d2 >= rcut2) return false;
@@ -172,7 +187,10 @@ __device__ __forceinline__ bool convex_overlap(const double* vx, uint32_t nv, do
   const quat inv = {qi.s, neg3(qi.v)};
   const v3 v = qrotate(inv, rab);
   const quat o = qmul(inv, qj);
-  return collide3d(vx, nv, v.x, v.y, v.z, o.s, o.v.x, o.v.y, o.v.z);
+  if constexpr (NV > 0)
+    return collide3d_n<NV>(vx, nv, v.x, v.y, v.z, o.s, o.v.x, o.v.y, o.v.z);
+  else
+    return collide3d(vx, nv, v.x, v.y, v.z, o.s, o.v.x, o.v.y, o.v.z);
 }
 
 // VersorDisplacement::sample (hoomd-mc/src/rotate/versor.rs:39-72).

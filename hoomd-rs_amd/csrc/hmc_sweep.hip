@@ -2369,22 +2369,23 @@ __device__ void trial_convex(const Geometry& g, const ApplyState* st, const Bodi
 // Convex pass layout: kCvTrials trials per wavefront, one per group of
 // kCvLanes lanes (the group draws its trial's proposal and lists its
 // neighbour spaces); the candidate sites of all the wavefront's trials are
-// tested against the circumsphere cutoff 64 at a time, the pairs within it
-// are gathered in a per-wavefront list, and XenoCollide runs over that list
-// 64 pairs at a time.  Any overlap rejects a hard-shape trial, so the order
-// of the pair tests does not change a decision (parallel_sweep.rs:267-273).
+// tested against the circumsphere cutoff 64 at a time (kCvBatch rounds with
+// their loads in flight together), the pairs within it are gathered in a
+// per-wavefront list, and XenoCollide runs over that list 64 pairs at a time.
+// Any overlap rejects a hard-shape trial, so the order of the pair tests does
+// not change a decision (parallel_sweep.rs:267-273).
 constexpr uint32_t kCvTrials = 4;
 constexpr uint32_t kCvLanes = 64 / kCvTrials;
-constexpr uint32_t kCvSegs = 128;   // neighbour spaces of one trial: at most 5^3
-constexpr uint32_t kCvPairs = 128;  // pending pair tests of one wavefront
+constexpr uint32_t kCvSegs = 128;  // neighbour spaces of one trial: at most 5^3
+constexpr uint32_t kCvBatch = 4;
+constexpr uint32_t kCvPairs = 384;  // listed pairs; tested once more than kCvPairs - 64 kCvBatch are pending
 struct CvWave {
   uint32_t pre[kCvTrials][kCvSegs];  // index of the first candidate of each neighbour space, per trial
   uint32_t lo[kCvTrials][kCvSegs];   // its first slot
-  double rx[kCvPairs], ry[kCvPairs], rz[kCvPairs];  // image of the pair's neighbour site
-  uint32_t qs[kCvPairs];                            // the neighbour's slot
-  uint32_t tk[kCvPairs];                            // the pair's trial
-  double t[kCvTrials][3];                           // trial site position
-  double qt[kCvTrials][4];                          // trial site orientation
+  uint32_t qs[kCvPairs];             // the listed pair's neighbour slot
+  uint32_t tk[kCvPairs];             // and trial
+  double t[kCvTrials][3];            // trial site position
+  double qt[kCvTrials][4];           // trial site orientation
 };
 
 // Orders the LDS accesses of one wavefront (lanes read what other lanes of
@@ -2396,7 +2397,9 @@ __device__ __forceinline__ void wave_lds_order() {
 }
 
 // XenoCollide over the np listed pairs; returns the trials (bits) that
-// found an overlap, skipping pairs of trials already in `hits`.
+// found an overlap, skipping pairs of trials already in `hits`.  NV: the
+// vertex count when it is one of the specialised ones, else 0.
+template <int NV>
 __device__ __forceinline__ uint32_t cv_pair_tests(const Geometry& g, const Cells& C, const CvWave& W,
                                                   const double* vx, uint32_t np, uint32_t hits) {
   const uint32_t lane = __lane_id();
@@ -2407,12 +2410,16 @@ __device__ __forceinline__ uint32_t cv_pair_tests(const Geometry& g, const Cells
     if (p < np) {
       kp = W.tk[p];
       if (!((hits >> kp) & 1u)) {
-        const double4 qj4 = C.quat64[W.qs[p]];
-        const v3 rj = {W.rx[p], W.ry[p], W.rz[p]};
-        const v3 tv = {W.t[kp][0], W.t[kp][1], W.t[kp][2]};
+        const uint32_t qs = W.qs[p];
+        const double4 sj = C.pos64[qs];
+        const double4 qj4 = C.quat64[qs];
+        const double tq[3] = {W.t[kp][0], W.t[kp][1], W.t[kp][2]};
+        v3 rj;
+        (void)convex_image(g, tq, sj, &rj);  // the image found when the pair was listed
         const quat qt = {W.qt[kp][0], {W.qt[kp][1], W.qt[kp][2], W.qt[kp][3]}};
         // listed pairs are within the cutoff: 0 stands for their distance
-        h = convex_overlap(vx, g.nverts, g.cv_rcut2, 0.0, tv, qt, rj, quat{qj4.x, {qj4.y, qj4.z, qj4.w}});
+        h = convex_overlap<NV>(vx, g.nverts, g.cv_rcut2, 0.0, v3{tq[0], tq[1], tq[2]}, qt, rj,
+                           quat{qj4.x, {qj4.y, qj4.z, qj4.w}});
       }
     }
 #pragma unroll
@@ -2452,50 +2459,24 @@ __global__ void __launch_bounds__(kBlock) k_pass_convex(Geometry g, ApplyState* 
   int32_t m[3] = {0, 0, 0};
   double x[3] = {0.0, 0.0, 0.0};
   quat q = {1.0, {0.0, 0.0, 0.0}};
+  Sfc64 rng = {0, 0, 0, 0};
   if (tix < g.per_colour) {
     s = colour_space<3>(g, colour, tix, m);
     const uint32_t s0 = w.start[s];
     const uint32_t n = w.start[s + 1] - s0;
     if (n) {
       tried = true;
-      Sfc64 rng = counter_rng(st->step, substep, st->seed, s);
+      rng = counter_rng(st->step, substep, st->seed, s);
       slot = s0 + choose_index(rng, n);
       body = C.body[slot];
-      const double4 bp = B.bpos[body];
-      const double4 bq = B.bquat[body];
-      x[0] = bp.x;
-      x[1] = bp.y;
-      x[2] = bp.z;
-      q = {bq.x, {bq.y, bq.z, bq.w}};
-      if (st->move == 0) {
-        double v[3] = {0.0, 0.0, 0.0};
-        ball_sample<3>(rng, st->max_move, zx, zf, v, &exp_slow);
-        for (int a = 0; a < 3; ++a) x[a] += v[a];
-      } else {
-        q = qmul(versor_displacement(rng, st->max_move, zx, zf, &exp_slow), q);
-      }
-      for (int a = 0; a < 3; ++a) x[a] = wrap1(x[a], g.L[a]);  // wrap(propose(..))
-      active = space_of<3>(g, st->origin, x) == s;
     }
   }
-  // the trial site: wrap(x + rotate(q, 0)) with orientation q * identity
-  // (property/oriented_point.rs:107-113)
-  const v3 r0 = qrotate(q, v3{0.0, 0.0, 0.0});
-  const quat qt = qmul(q, quat{1.0, {0.0, 0.0, 0.0}});
-  if (active && kl == 0) {
-    W.t[k][0] = wrap1(x[0] + r0.x, g.L[0]);
-    W.t[k][1] = wrap1(x[1] + r0.y, g.L[1]);
-    W.t[k][2] = wrap1(x[2] + r0.z, g.L[2]);
-    W.qt[k][0] = qt.s;
-    W.qt[k][1] = qt.v.x;
-    W.qt[k][2] = qt.v.y;
-    W.qt[k][3] = qt.v.z;
-  }
   // the trial's neighbour spaces (distinct ones, as the reference's cell
-  // list visits each once) and the first candidate index of each
+  // list visits each once) and the first candidate index of each; listed
+  // before the proposal is drawn so that their loads overlap the body's
   uint32_t nb[3][5];
   uint32_t nn[3] = {0, 0, 0};
-  if (active) {
+  if (tried) {
     for (int a = 0; a < 3; ++a) {
       const int32_t sh = g.shape[a], r = g.reach[a];
       uint32_t c = 0;
@@ -2531,6 +2512,47 @@ __global__ void __launch_bounds__(kBlock) k_pass_convex(Geometry g, ApplyState* 
     }
     tot += __shfl(incl, (int)(lane | (kCvLanes - 1)), 64);
   }
+  // the proposal (the random draws overlap the loads of the body)
+  double4 bp = make_double4(0.0, 0.0, 0.0, 0.0), bq = make_double4(1.0, 0.0, 0.0, 0.0);
+  if (tried) {
+    bp = B.bpos[body];
+    bq = B.bquat[body];
+  }
+  double v[3] = {0.0, 0.0, 0.0};
+  quat dq = {1.0, {0.0, 0.0, 0.0}};
+  if (tried) {
+    if (st->move == 0)
+      ball_sample<3>(rng, st->max_move, zx, zf, v, &exp_slow);
+    else
+      dq = versor_displacement(rng, st->max_move, zx, zf, &exp_slow);
+  }
+  if (tried) {
+    x[0] = bp.x;
+    x[1] = bp.y;
+    x[2] = bp.z;
+    q = {bq.x, {bq.y, bq.z, bq.w}};
+    if (st->move == 0) {
+      for (int a = 0; a < 3; ++a) x[a] += v[a];
+    } else {
+      q = qmul(dq, q);
+    }
+    for (int a = 0; a < 3; ++a) x[a] = wrap1(x[a], g.L[a]);  // wrap(propose(..))
+    active = space_of<3>(g, st->origin, x) == s;
+  }
+  if (!active) tot = 0;  // a trial outside its space is rejected without pair tests
+  // the trial site: wrap(x + rotate(q, 0)) with orientation q * identity
+  // (property/oriented_point.rs:107-113)
+  const v3 r0 = qrotate(q, v3{0.0, 0.0, 0.0});
+  const quat qt = qmul(q, quat{1.0, {0.0, 0.0, 0.0}});
+  if (active && kl == 0) {
+    W.t[k][0] = wrap1(x[0] + r0.x, g.L[0]);
+    W.t[k][1] = wrap1(x[1] + r0.y, g.L[1]);
+    W.t[k][2] = wrap1(x[2] + r0.z, g.L[2]);
+    W.qt[k][0] = qt.s;
+    W.qt[k][1] = qt.v.x;
+    W.qt[k][2] = qt.v.y;
+    W.qt[k][3] = qt.v.z;
+  }
   uint32_t base[kCvTrials + 1];
   base[0] = 0;
 #pragma unroll
@@ -2539,48 +2561,67 @@ __global__ void __launch_bounds__(kBlock) k_pass_convex(Geometry g, ApplyState* 
   wave_lds_order();
   uint32_t hits = 0;  // trials with an overlap (uniform over the wavefront)
   uint32_t np = 0;    // listed pairs (uniform)
-  for (uint32_t c0 = 0; c0 < total; c0 += 64) {
-    const uint32_t c = c0 + lane;
-    uint32_t kc = 0, bk = 0;
+  for (uint32_t c0 = 0; c0 < total; c0 += 64 * kCvBatch) {
+    uint32_t qsv[kCvBatch], kcv[kCvBatch];
+    bool ok[kCvBatch];
 #pragma unroll
-    for (uint32_t i = 1; i < kCvTrials; ++i)
-      if (c >= base[i]) {
-        kc = i;
-        bk = base[i];
-      }
-    const uint32_t nsk = __shfl(NS, (int)(kc * kCvLanes), 64);
-    const uint32_t slk = __shfl(slot, (int)(kc * kCvLanes), 64);
-    bool inr = false;
-    uint32_t qs = 0;
-    v3 rj = {0.0, 0.0, 0.0};
-    if (c < total && !((hits >> kc) & 1u)) {
-      // neighbour space of candidate c: last one whose first index is <= c
-      const uint32_t cl = c - bk;
-      uint32_t jlo = 0;
+    for (uint32_t r = 0; r < kCvBatch; ++r) {
+      const uint32_t c = c0 + 64 * r + lane;
+      uint32_t kc = 0, bk = 0;
 #pragma unroll
-      for (uint32_t sw = kCvSegs / 2; sw >= 1; sw >>= 1) {
-        const uint32_t mid = jlo + sw;
-        if (mid < nsk && W.pre[kc][mid] <= cl) jlo = mid;
+      for (uint32_t i = 1; i < kCvTrials; ++i)
+        if (c >= base[i]) {
+          kc = i;
+          bk = base[i];
+        }
+      const uint32_t nsk = __shfl(NS, (int)(kc * kCvLanes), 64);
+      const uint32_t slk = __shfl(slot, (int)(kc * kCvLanes), 64);
+      ok[r] = c < total && !((hits >> kc) & 1u);
+      uint32_t qs = 0;
+      if (ok[r]) {
+        // neighbour space of candidate c: last one whose first index is <= c
+        const uint32_t cl = c - bk;
+        uint32_t jlo = 0;
+#pragma unroll
+        for (uint32_t sw = kCvSegs / 2; sw >= 1; sw >>= 1) {
+          const uint32_t mid = jlo + sw;
+          if (mid < nsk && W.pre[kc][mid] <= cl) jlo = mid;
+        }
+        qs = W.lo[kc][jlo] + (cl - W.pre[kc][jlo]);
+        ok[r] = qs != slk;
       }
-      qs = W.lo[kc][jlo] + (cl - W.pre[kc][jlo]);
-      if (qs != slk) {
-        const double tq[3] = {W.t[kc][0], W.t[kc][1], W.t[kc][2]};
-        inr = convex_image(g, tq, C.pos64[qs], &rj) < g.cv_rcut2;
+      qsv[r] = qs;
+      kcv[r] = kc;
+    }
+    double4 sj[kCvBatch];
+#pragma unroll
+    for (uint32_t r = 0; r < kCvBatch; ++r) sj[r] = ok[r] ? C.pos64[qsv[r]] : make_double4(0.0, 0.0, 0.0, 0.0);
+#pragma unroll
+    for (uint32_t r = 0; r < kCvBatch; ++r) {
+      bool inr = false;
+      if (ok[r]) {
+        const double tq[3] = {W.t[kcv[r]][0], W.t[kcv[r]][1], W.t[kcv[r]][2]};
+        v3 rj;
+        inr = convex_image(g, tq, sj[r], &rj) < g.cv_rcut2;
       }
+      const unsigned long long mk = __ballot(inr);
+      if (inr) {
+        const uint32_t p = np + (uint32_t)__popcll(mk & ((1ull << lane) - 1ull));
+        W.qs[p] = qsv[r];
+        W.tk[p] = kcv[r];
+      }
+      np += (uint32_t)__popcll(mk);
     }
-    const unsigned long long mk = __ballot(inr);
-    if (inr) {
-      const uint32_t p = np + (uint32_t)__popcll(mk & ((1ull << lane) - 1ull));
-      W.rx[p] = rj.x;
-      W.ry[p] = rj.y;
-      W.rz[p] = rj.z;
-      W.qs[p] = qs;
-      W.tk[p] = kc;
-    }
-    np += (uint32_t)__popcll(mk);
-    if (np + 64 > kCvPairs || c0 + 64 >= total) {  // uniform
+    if (np + 64 * kCvBatch > kCvPairs || c0 + 64 * kCvBatch >= total) {  // uniform
       wave_lds_order();
-      hits = cv_pair_tests(g, C, W, vx, np, hits);
+      if (g.nverts == 8)
+        hits = cv_pair_tests<8>(g, C, W, vx, np, hits);
+      else if (g.nverts == 6)
+        hits = cv_pair_tests<6>(g, C, W, vx, np, hits);
+      else if (g.nverts == 4)
+        hits = cv_pair_tests<4>(g, C, W, vx, np, hits);
+      else
+        hits = cv_pair_tests<0>(g, C, W, vx, np, hits);
       np = 0;
       wave_lds_order();
     }
