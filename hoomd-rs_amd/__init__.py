@@ -33,7 +33,7 @@ EXPORTS = ["hmc_create", "hmc_set_state", "hmc_get_state", "hmc_apply_translate"
            "hmc_trace_enable", "hmc_trace_read", "hmc_get_stats", "hmc_profile_enable", "hmc_profile_read",
            "hmc_debug_math", "hmc_debug_normals", "hmc_debug_convex", "hmc_last_error", "hmc_destroy",
            "hmc_batch_create", "hmc_batch_set_state", "hmc_batch_get_state", "hmc_batch_run_translate",
-           "hmc_batch_last_error", "hmc_batch_destroy"]
+           "hmc_batch_last_error", "hmc_batch_destroy", "hmc_sdf", "hmc_batch_sdf"]
 
 
 class HmcError(RuntimeError):
@@ -92,6 +92,9 @@ def lib():
                                        C.POINTER(Count), C.POINTER(C.c_uint32)]
         L.hmc_set_orientations.argtypes = [vp, D_P, D_P]
         L.hmc_total_energy.argtypes = [vp, D_P, D_P]
+        u64p = C.POINTER(C.c_uint64)
+        L.hmc_sdf.argtypes = [vp, C.c_double, C.c_double, C.c_uint32, u64p, u64p]
+        L.hmc_batch_sdf.argtypes = [vp, C.c_double, C.c_double, C.c_uint32, u64p, u64p]
         L.hmc_batch_create.argtypes = [C.c_uint32, C.POINTER(Config), C.POINTER(vp)]
         L.hmc_batch_set_state.argtypes = [vp, C.c_uint32, D_P, D_P]
         L.hmc_batch_get_state.argtypes = [vp, C.c_uint32, D_P, D_P]
@@ -220,6 +223,16 @@ class GpuParallelSweep:
         b = C.c_double()
         self._check(lib().hmc_total_energy(self.h, C.byref(e), C.byref(b)))
         return e.value, b.value
+
+    def sdf(self, xmax=0.02, dx=1e-4, nbins=None):
+        """(hist, overlaps): the scale distribution function histogram of the
+        current state (hmc_sdf), nbins = ceil(xmax / dx) bins by default."""
+        nbins = sdf_nbins(xmax, dx) if nbins is None else int(nbins)
+        hist = np.zeros(nbins, np.uint64)
+        ov = C.c_uint64()
+        self._check(lib().hmc_sdf(self.h, xmax, dx, nbins, hist.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                  C.byref(ov)))
+        return hist, int(ov.value)
 
     def orientations(self):
         b = np.zeros((self.n, 4))
@@ -365,6 +378,15 @@ class GpuBatch:
         self._check(lib().hmc_batch_run_translate(self.h, st, sd, kT, _p(md), n_steps, 1 if use_graph else 0, out))
         return [(int(c.accepted), int(c.rejected)) for c in out]
 
+    def sdf(self, xmax=0.02, dx=1e-4, nbins=None):
+        """(hist[R, nbins], overlaps[R]): hmc_sdf of every replica."""
+        nbins = sdf_nbins(xmax, dx) if nbins is None else int(nbins)
+        hist = np.zeros((self.R, nbins), np.uint64)
+        ov = np.zeros(self.R, np.uint64)
+        u64p = C.POINTER(C.c_uint64)
+        self._check(lib().hmc_batch_sdf(self.h, xmax, dx, nbins, hist.ctypes.data_as(u64p), ov.ctypes.data_as(u64p)))
+        return hist, ov
+
     def close(self):
         if getattr(self, "h", None):
             lib().hmc_batch_destroy(self.h)
@@ -372,3 +394,20 @@ class GpuBatch:
 
     def __del__(self):
         self.close()
+
+
+def sdf_nbins(xmax, dx):
+    """Bins of width dx covering [0, xmax)."""
+    return int(np.ceil(xmax / dx))
+
+
+def sdf_beta_p_over_rho(hist, n_bodies, dx, dim, n_frames=1, degree=5):
+    """beta P / rho = 1 + s(0+) / (2 dim) from SDF counts summed over n_frames
+    frames of n_bodies bodies, as HOOMD-blue's hpmc compute.SDF.betaP: s =
+    counts / (n_frames n_bodies dx) at the bin centres, a least-squares
+    polynomial of the given degree through them, evaluated at x = 0."""
+    hist = np.asarray(hist, np.float64)
+    s = hist / (n_frames * n_bodies * dx)
+    x = (np.arange(len(hist)) + 0.5) * dx
+    s0 = np.polyval(np.polyfit(x, s, degree), 0.0)
+    return 1.0 + s0 / (2.0 * dim)

@@ -66,6 +66,7 @@ struct Geometry {
   uint32_t half_shape[3];  // spaces of one colour along each axis
   int32_t reach[3];        // neighbour spaces scanned on each side (1, or 2 when width ~ sigma)
   double sigma2;
+  double sigma;     // hard-sphere diameter (the interaction range)
   double edge2;     // sigma^2 (1 - 2^-30): at or above it the spatial-structure rule is applied
   double vc_width;  // VecCell cell width of the reference microstate
   double vc_k;      // VecCell stencil radius ceil(sigma / vc_width)
@@ -3016,6 +3017,79 @@ __global__ void k_energy_final(const double* part, uint32_t nblocks, double* out
   out[3] = inf;
 }
 
+// ------------------------------------------------ scale distribution function --
+// The SDF of hard spheres/disks as HOOMD-blue's hpmc compute.SDF defines it
+// (no counterpart in the reference): for each body, the smallest compression
+// x = 1 - sigma / r over the other bodies (r the minimum-image distance
+// between sites) at which the scaled system would first overlap, counted in
+// bin floor(x / dx) when 0 <= x < xmax and the bin is below nbins; x < 0 (an
+// overlap) is counted apart.  x is monotone in r^2, so the minimum is taken
+// over r^2 = (d_x^2 + d_y^2) + d_z^2 and x is formed once.  Bodies within
+// r_max = sigma / (1 - xmax) of a body lie in its neighbour spaces: one space
+// on each side when the space width is at least r_max (1 + 2^-30), else two
+// (the width is at least sigma and xmax < 1/2).  reps/boff describe one or
+// several systems laid out as in a batch.
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_sdf(const BatchRep* reps, const uint32_t* boff, uint32_t R,
+                                                uint32_t n_total, Bodies B, Work w, double xmax, double dx,
+                                                uint32_t nbins, uint32_t* hist, uint32_t* ovl) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n_total) return;
+  const uint32_t r = R > 1 ? find_replica(boff, R, b) : 0u;
+  const Geometry& g = reps[r].g;
+  const uint32_t soff = reps[r].space_off;
+  const uint32_t s = w.elem_space[b] - soff;
+  const double4 si = B.spos[b];
+  const double t[3] = {si.x, si.y, si.z};
+  const double rmax = g.sigma / (1.0 - xmax);
+  uint32_t nb[3][5];
+  int nn[3] = {1, 1, 1};
+  for (int a = 0; a < D; ++a) {
+    const int32_t sh = g.shape[a];
+    const int32_t m = (int32_t)((s / g.stride[a]) % (uint32_t)sh);
+    const int32_t reach = g.width[a] >= rmax * (1.0 + 0x1p-30) ? 1 : 2;
+    int k = 0;
+    for (int off = -reach; off <= reach; ++off) {
+      const int32_t mm = ((m + off) % sh + sh) % sh;
+      bool seen = false;
+      for (int j = 0; j < k; ++j) seen |= nb[a][j] == (uint32_t)mm * g.stride[a];
+      if (!seen) nb[a][k++] = (uint32_t)mm * g.stride[a];
+    }
+    nn[a] = k;
+  }
+  double best = __builtin_inf();
+  for (int i0 = 0; i0 < nn[0]; ++i0)
+    for (int i1 = 0; i1 < nn[1]; ++i1)
+      for (int i2 = 0; i2 < (D == 3 ? nn[2] : 1); ++i2) {
+        const uint32_t ns = soff + nb[0][i0] + nb[1][i1] + (D == 3 ? nb[2][i2] : 0u);
+        const uint32_t q1 = w.start[ns + 1];
+        for (uint32_t q = w.start[ns]; q < q1; ++q) {
+          const uint32_t j = w.tmp_body[q];
+          if (j == b) continue;
+          const double4 sj = B.spos[j];
+          const double u[3] = {sj.x, sj.y, sj.z};
+          double r2 = 0.0;
+          for (int a = 0; a < D; ++a) {
+            double d = t[a] - u[a];
+            if (d > g.half[a])
+              d -= g.L[a];
+            else if (d < -g.half[a])
+              d += g.L[a];
+            r2 = r2 + d * d;
+          }
+          best = fmin(best, r2);
+        }
+      }
+  if (!(best < __builtin_inf())) return;  // no other body nearby
+  const double x = 1.0 - g.sigma / sqrt(best);
+  if (x < 0.0) {
+    atomicAdd(&ovl[r], 1u);
+  } else if (x < xmax) {
+    const double k = floor(x / dx);
+    if (k < (double)nbins) atomicAdd(&hist[(size_t)r * nbins + (uint32_t)k], 1u);
+  }
+}
+
 // Site positions (given, or wrap(body + 0)) and their ghost flags.
 template <int D>
 __global__ void k_init_sites(Geometry g, Bodies B, int have_sites) {
@@ -3648,6 +3722,7 @@ static int make_geometry(const hmc_config* cfg, Geometry& g, bool* fast, double*
   *fast = true;
   for (int a = 0; a < D; ++a) *fast = *fast && g.reach[a] == 1 && g.shape[a] >= 3;
   g.sigma2 = sigma * sigma;
+  g.sigma = sigma;
   g.edge2 = g.sigma2 * (1.0 - 0x1p-30);
   g.model = cfg->model;
   if (convex) {
@@ -4063,6 +4138,77 @@ int hmc_run_translate(hmc_ctx* c, uint64_t step0, uint32_t seed, double kT, doub
   return HMC_OK;
 }
 
+// Arguments of the SDF for one system: hard spheres/disks, 0 < xmax < 1/2,
+// dx > 0, a box wider than 2 r_max (one image within r_max) and spaces at
+// least r_max / 2 wide (two neighbour spaces on each side suffice).
+static const char* sdf_check(const Geometry& g, double xmax, double dx, uint32_t nbins) {
+  if (g.model != HMC_HARD_SPHERE) return "the SDF is built for hard spheres and disks";
+  if (!(xmax > 0.0 && xmax < 0.5) || !(dx > 0.0) || !std::isfinite(dx) || nbins == 0 || nbins > (1u << 24))
+    return "SDF needs 0 < xmax < 1/2, dx > 0 and 0 < nbins <= 2^24";
+  const double rmax = g.sigma / (1.0 - xmax);
+  for (int a = 0; a < g.dim; ++a) {
+    if (!(g.L[a] > 2.0 * rmax * (1.0 + 0x1p-30))) return "box edge below 2 sigma / (1 - xmax)";
+    if (!(2.0 * g.width[a] >= rmax * (1.0 + 0x1p-30))) return "checkerboard spaces narrower than r_max / 2";
+  }
+  return nullptr;
+}
+
+int hmc_sdf(hmc_ctx* c, double xmax, double dx, uint32_t nbins, uint64_t* hist, uint64_t* overlaps) {
+  if (!c || !hist) return HMC_ERR_INVALID;
+  if (!c->have_state) return fail(c, HMC_ERR_STATE, "hmc_set_state was not called");
+  const Geometry& g = c->g;
+  if (const char* why = sdf_check(g, xmax, dx, nbins))
+    return fail(c, g.model != HMC_HARD_SPHERE ? HMC_ERR_UNSUPPORTED : HMC_ERR_INVALID, why);
+  const uint32_t N = g.n_bodies;
+  HIPCHK(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  // device: the system as a one-entry batch table, body offsets, counts
+  const size_t bytes = sizeof(BatchRep) + 2 * sizeof(uint32_t) + ((size_t)nbins + 1) * sizeof(uint32_t);
+  char* buf = nullptr;
+  HIPCHK(c, hipMallocAsync((void**)&buf, bytes, s));
+  BatchRep* drep = (BatchRep*)buf;
+  uint32_t* dboff = (uint32_t*)(buf + sizeof(BatchRep));
+  uint32_t* dh = dboff + 2;
+  const BatchRep hrep{g, 0u, 0u, 0u, 0u};
+  const uint32_t hboff[2] = {0u, N};
+  hipError_t e = hipMemcpyAsync(drep, &hrep, sizeof hrep, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(dboff, hboff, sizeof hboff, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    k_fill_u32<<<grid_for(nbins + 1, 256), 256, 0, s>>>(dh, nbins + 1, 0u);
+    if (N) {
+      // cells of the current state (the checkerboard of the current counter)
+      k_setup<<<1, 64, 0, s>>>(g, c->st, 0, 0, 0, 1.0, 1.0, 0, 0, 0u);
+      if (g.dim == 2)
+        k_bin<2><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
+      else
+        k_bin<3><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
+      k_scan_blocks<<<c->n_scan_blocks, 256, 0, s>>>(c->w.cnt, c->w.start, c->w.block_sums, c->w.block_ne,
+                                                     g.n_spaces);
+      k_scan_top<<<1, 1024, 0, s>>>(g, c->st, c->w.block_sums, c->w.block_ne, c->n_scan_blocks, c->w.start, 0);
+      k_scan_add<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.start, c->w.block_sums, c->w.cnt, g.n_spaces);
+      k_scatter<<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->w);
+      if (g.dim == 2)
+        k_sdf<2><<<grid_for(N, kBlock), kBlock, 0, s>>>(drep, dboff, 1u, N, c->bodies, c->w, xmax, dx, nbins, dh,
+                                                        dh + nbins);
+      else
+        k_sdf<3><<<grid_for(N, kBlock), kBlock, 0, s>>>(drep, dboff, 1u, N, c->bodies, c->w, xmax, dx, nbins, dh,
+                                                        dh + nbins);
+    }
+    e = hipGetLastError();
+  }
+  std::vector<uint32_t> h(nbins + 1, 0u);
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), dh, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFreeAsync(buf, s);
+  if (e != hipSuccess) return fail(c, HMC_ERR_DEVICE, std::string("sdf: ") + hipGetErrorString(e));
+  ApplyState st;  // a body outside the checkerboard is reported here
+  int rc = read_state(c, &st);
+  if (rc) return rc;
+  for (uint32_t k = 0; k < nbins; ++k) hist[k] = h[k];
+  if (overlaps) *overlaps = h[nbins];
+  return HMC_OK;
+}
+
 int hmc_total_energy(hmc_ctx* c, double* energy, double* bound) {
   if (!c || !energy) return HMC_ERR_INVALID;
   if (!c->have_state) return fail(c, HMC_ERR_STATE, "hmc_set_state was not called");
@@ -4435,6 +4581,55 @@ int hmc_batch_run_translate(hmc_batch* b, const uint64_t* step0, const uint32_t*
     }
   }
   if (n_steps) b->cycles_guess = most;
+  return HMC_OK;
+}
+
+int hmc_batch_sdf(hmc_batch* b, double xmax, double dx, uint32_t nbins, uint64_t* hist, uint64_t* overlaps) {
+  if (!b || !hist) return HMC_ERR_INVALID;
+  for (uint32_t r = 0; r < b->R; ++r)
+    if (const char* why = sdf_check(b->hreps[r].g, xmax, dx, nbins)) return bfail(b, HMC_ERR_INVALID, why);
+  if ((uint64_t)b->R * (nbins + 1ull) >= (1ull << 31)) return bfail(b, HMC_ERR_INVALID, "n_replicas * nbins too large");
+  BHIPCHK(b, hipSetDevice(b->device));
+  hipStream_t s = b->stream;
+  const size_t cells = (size_t)b->R * nbins + b->R;
+  uint32_t* dh = nullptr;
+  BHIPCHK(b, hipMallocAsync((void**)&dh, cells * sizeof(uint32_t), s));
+  k_fill_u32<<<grid_for((uint32_t)cells, 256), 256, 0, s>>>(dh, (uint32_t)cells, 0u);
+  if (b->n_total) {
+    // cells of the current states (the checkerboards of the current counters)
+    k_bsetup<<<grid_for(b->R, 64), 64, 0, s>>>(b->reps, b->sts, b->R, b->dstep, b->dseed, b->kT, b->dmove, 0, 0);
+    k_fill_u32<<<grid_for(b->s_total, 256), 256, 0, s>>>(b->w.cnt, b->s_total, 0u);
+    if (b->dim == 2)
+      k_bbin<2><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->reps, b->sts, b->dboff, b->R, b->n_total,
+                                                                b->bodies, b->w);
+    else
+      k_bbin<3><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->reps, b->sts, b->dboff, b->R, b->n_total,
+                                                                b->bodies, b->w);
+    k_scan_blocks<<<b->n_scan_blocks, 256, 0, s>>>(b->w.cnt, b->w.start, b->w.block_sums, b->w.block_ne, b->s_total);
+    k_scan_top<<<1, 1024, 0, s>>>(b->gtot, b->sts + b->R, b->w.block_sums, b->w.block_ne, b->n_scan_blocks,
+                                  b->w.start, 0);
+    k_scan_add<<<grid_for(b->s_total, 256), 256, 0, s>>>(b->w.start, b->w.block_sums, nullptr, b->s_total);
+    k_scatter<<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->gtot, b->w);
+    if (b->dim == 2)
+      k_sdf<2><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->reps, b->dboff, b->R, b->n_total, b->bodies, b->w,
+                                                               xmax, dx, nbins, dh, dh + (size_t)b->R * nbins);
+    else
+      k_sdf<3><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->reps, b->dboff, b->R, b->n_total, b->bodies, b->w,
+                                                               xmax, dx, nbins, dh, dh + (size_t)b->R * nbins);
+  }
+  std::vector<uint32_t> h(cells, 0u);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), dh, cells * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFreeAsync(dh, s);
+  if (e != hipSuccess) return bfail(b, HMC_ERR_DEVICE, std::string("batch sdf: ") + hipGetErrorString(e));
+  std::vector<ApplyState> st;
+  std::vector<unsigned long long> acc;
+  int rc = batch_read(b, st, acc);  // a body outside its checkerboard is reported here
+  if (rc) return rc;
+  for (size_t k = 0; k < (size_t)b->R * nbins; ++k) hist[k] = h[k];
+  if (overlaps)
+    for (uint32_t r = 0; r < b->R; ++r) overlaps[r] = h[(size_t)b->R * nbins + r];
   return HMC_OK;
 }
 

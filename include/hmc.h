@@ -177,6 +177,18 @@ int hmc_run_translate(hmc_ctx* ctx, uint64_t step0, uint32_t seed, double kT, do
  * two summation orders of the same n terms. */
 int hmc_total_energy(hmc_ctx* ctx, double* energy, double* bound);
 
+/* Scale distribution function (SDF) of the current state, hard spheres and
+ * disks: for every body the smallest compression x = 1 - sigma / r over the
+ * other bodies (r the minimum-image site distance) at which the isotropically
+ * scaled system would first overlap.  hist[k] (k < nbins) counts the bodies
+ * with 0 <= x < xmax and floor(x / dx) = k; *overlaps (may be NULL) counts
+ * those with x < 0.  Needs 0 < xmax < 1/2 and box edges above
+ * 2 sigma / (1 - xmax); HMC_ERR_UNSUPPORTED for other models.  No counterpart
+ * in the reference, which has no pressure compute: the definition is
+ * HOOMD-blue's hpmc compute.SDF, from which beta P / rho = 1 + s(0+) / (2 d)
+ * with s = hist / (N dx) extrapolated to x = 0. */
+int hmc_sdf(hmc_ctx* ctx, double xmax, double dx, uint32_t nbins, uint64_t* hist, uint64_t* overlaps);
+
 /* Record every non-empty trial of the following applies (testing/replay). */
 int hmc_trace_enable(hmc_ctx* ctx, int on, uint64_t capacity);
 int hmc_trace_read(hmc_ctx* ctx, hmc_trace_record* buf, uint64_t cap, uint64_t* n);
@@ -228,6 +240,9 @@ int hmc_batch_get_state(hmc_batch* batch, uint32_t replica, double* body_pos, do
  * summed counts.  use_graph = 0 launches every step directly (same results). */
 int hmc_batch_run_translate(hmc_batch* batch, const uint64_t* step0, const uint32_t* seeds, double kT,
                             const double* max_distance, uint32_t n_steps, int use_graph, hmc_count* out);
+/* hmc_sdf of every replica: hist holds n_replicas * nbins counts (replica
+ * major), overlaps (may be NULL) n_replicas counts. */
+int hmc_batch_sdf(hmc_batch* batch, double xmax, double dx, uint32_t nbins, uint64_t* hist, uint64_t* overlaps);
 const char* hmc_batch_last_error(const hmc_batch* batch);
 void hmc_batch_destroy(hmc_batch* batch);
 

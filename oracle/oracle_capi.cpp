@@ -12,6 +12,7 @@
 #include "geometry.hpp"
 #include "mc.hpp"
 #include "rng.hpp"
+#include "sdf.hpp"
 #include "vec_cell.hpp"
 
 using namespace oracle;
@@ -511,6 +512,17 @@ uint64_t oracle_convex_time(void* h, double d, double a, double seconds, double*
   }
   *steps = n;
   return total;
+}
+
+// ---- scale distribution function (oracle/sdf.hpp) ----------------------------
+int oracle_sdf(int dim, const double* L, const double* pos, uint64_t n, double sigma, double xmax, double dx,
+               uint32_t nbins, int all_pairs, uint64_t* hist, uint64_t* overlaps) {
+  if ((dim != 2 && dim != 3) || !(sigma > 0.0) || !(xmax > 0.0 && xmax < 1.0) || !(dx > 0.0)) return -1;
+  if (dim == 2)
+    sdf_histogram<2>(pos, n, L, sigma, xmax, dx, nbins, all_pairs != 0, hist, overlaps);
+  else
+    sdf_histogram<3>(pos, n, L, sigma, xmax, dx, nbins, all_pairs != 0, hist, overlaps);
+  return 0;
 }
 
 }  // extern "C"

@@ -352,6 +352,25 @@ def trig_counters():
     return int(out[0]), int(out[1])
 
 
+def sdf(L, positions, sigma, xmax, dx, nbins, all_pairs=False):
+    """oracle/sdf.hpp: (hist, overlaps) of one configuration of site positions;
+    all_pairs selects the all-pairs neighbour search instead of the cell grid."""
+    L = _d(L)
+    dim = len(L)
+    pos = _d(positions).reshape(-1, dim)
+    hist = np.zeros(int(nbins), np.uint64)
+    ov = np.zeros(1, np.uint64)
+    f = lib().oracle_sdf
+    f.argtypes = [C.c_int, D_P, D_P, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_uint32, C.c_int,
+                  U64_P, U64_P]
+    f.restype = C.c_int
+    rc = f(dim, _p(L, D_P), _p(pos, D_P), len(pos), sigma, xmax, dx, int(nbins), 1 if all_pairs else 0,
+           _p(hist, U64_P), _p(ov, U64_P))
+    if rc:
+        raise ValueError("invalid SDF arguments")
+    return hist, int(ov[0])
+
+
 class ConvexOracle:
     """The restated Microstate<OrientedPoint> + HardShape<Convex> + ParallelSweep."""
 
