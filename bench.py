@@ -78,7 +78,7 @@ def pass_bytes(precision, n, n_spaces, nonempty, accepted_per_pass, dim=3):
     return 4 * (n_spaces + 1) + site * n + 36 * active + write * accepted_per_pass
 
 
-def run_gpu(hmc, L, body, site, precision, steps, warmup, seed, kT, d, period, persistent=False):
+def run_gpu(hmc, L, body, site, precision, steps, warmup, seed, kT, d, period, persistent=False, sdf=False):
     sim = hmc.GpuParallelSweep(L, body, sites=site, precision=precision, persistent=persistent)
     # warm-up applies: steps 0 .. warmup-1
     sim.run(0, seed, kT, d, warmup)
@@ -89,9 +89,19 @@ def run_gpu(hmc, L, body, site, precision, steps, warmup, seed, kT, d, period, p
     pass_ms, pass_n, build_ms, build_n = sim.read_profile()
     stats = sim.stats()
     final = sim.positions()
+    sdf_out = None
+    if sdf:
+        # SDF of the final state (x_max 0.02, dx 1e-4): one call, then timed calls
+        hist, ov = sim.sdf(0.02, 1e-4)
+        t2 = time.perf_counter()
+        for _ in range(5):
+            hist, ov = sim.sdf(0.02, 1e-4)
+        ms = (time.perf_counter() - t2) / 5 * 1e3
+        sdf_out = {"ms_per_call": ms, "bodies_per_s": sim.n / (ms * 1e-3), "counted": int(hist.sum()),
+                   "overlaps": ov, "beta_p_over_rho_one_frame": hmc.sdf_beta_p_over_rho(hist, sim.n, 1e-4, 3)}
     sim.close()
     return dict(elapsed=t1 - t0, accepted=acc, rejected=rej, pass_ms=pass_ms, pass_n=pass_n,
-                build_ms=build_ms, build_n=build_n, stats=stats, final=final)
+                build_ms=build_ms, build_n=build_n, stats=stats, final=final, sdf=sdf_out)
 
 
 def measured_traffic(kernel):
@@ -170,6 +180,17 @@ def other_workloads(hmc, wl):
     v, t = timed(lambda s0, k: sum(a + r for a, r in batch.run(s0, seeds, 1.0, 0.08, k)), 2, 40)
     res["C5_replicas_256"] = {"value": v, "unit": "trial moves/s", "steps": 40, "precision": "mixed",
                               "bodies": int(sum(len(p) for _, p in systems))}
+    # SDF of every replica (x_max 0.02, dx 1e-4): one call after the run,
+    # then timed calls; beta P / rho from the single frame
+    nb = sum(len(p) for _, p in systems)
+    hist, ov = batch.sdf(0.02, 1e-4)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        hist, ov = batch.sdf(0.02, 1e-4)
+    ms = (time.perf_counter() - t0) / 5 * 1e3
+    z = [hmc.sdf_beta_p_over_rho(hist[r], len(p), 1e-4, 3) for r, (_L, p) in enumerate(systems)]
+    res["C5_replicas_256"]["sdf"] = {"ms_per_call": ms, "bodies_per_s": nb / (ms * 1e-3), "overlaps": int(ov.sum()),
+                                     "beta_p_over_rho_one_frame": {"phi_0.30": z[0], "phi_0.54": z[-1]}}
     batch.close()
     return res
 
@@ -206,7 +227,7 @@ def main():
     body, site = wl.wrap(L, pos), None
 
     mixed = run_gpu(hmc, L, body, site, "mixed", args.steps, args.warmup, args.seed, kT, d, args.profile_period,
-                    persistent=args.pass_layout == "persistent")
+                    persistent=args.pass_layout == "persistent", sdf=True)
     trials = mixed["accepted"] + mixed["rejected"]
     value = trials / mixed["elapsed"]
     out = {
@@ -241,6 +262,7 @@ def main():
                            "kernel": kernel, "avg_launch_ms": avg_ms,
                            "algorithmic_bytes_per_launch": b, "launches": mixed["pass_n"],
                            "build_ms_per_apply": mixed["build_ms"] / max(mixed["build_n"], 1)}
+    out["sdf"] = mixed["sdf"]  # the scale distribution function of the final state (hmc_sdf), timed apart
     out["band_retests"] = mixed["stats"]["band_retests"]  # FP64 re-tests over warm-up + timed applies
     out["exp_slow_path"] = mixed["stats"]["exp_slow_path"]
     if not args.no_fp64:
