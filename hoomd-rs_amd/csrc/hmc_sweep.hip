@@ -34,6 +34,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/hmc.h"
@@ -77,6 +78,7 @@ struct Geometry {
   double lj_rc2;           // r_cut^2 (isotropic.rs:151)
   double lj_rc2_edge;      // r_cut^2 (1 - 2^-30): at or above it the spatial-structure rule is applied
   double lj_shift;         // E(r_shift) for the shifted potential, else 0
+  int lj_mixed;            // LJ pair energies in FP32 with an FP64 re-test inside the error bound
   const double* verts;     // convex polyhedron: vertices (x, y, z triples), device memory
   uint32_t nverts;
   double cv_rcut2;         // (R + R)^2, R the circumsphere radius (convex.rs:163-166)
@@ -812,6 +814,14 @@ constexpr int kRowsPerLane = 4;         // rows per lane of wave 0: rows <= 256
 template <int PREC>
 constexpr int kRowBatch = PREC == HMC_MIXED ? 16 : 8;
 constexpr size_t kTileLds = 72 * 1024;  // dynamic LDS per workgroup
+
+// Orders the LDS accesses of one wavefront (lanes read what other lanes of
+// the same wavefront wrote; LDS operations of a wavefront complete in order).
+__device__ __forceinline__ void wave_lds_order() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // Inclusive scan over the 64 lanes of a wavefront.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
@@ -1971,6 +1981,25 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
   return v;
 }
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+// FP32 pair energy of the mixed-precision LJ path: s2 = sigma^2 / r2 as
+// fl(sigma^2) rcp(fl(r2)) (v_rcp_f32, within 1 ulp), then the reference's
+// powers and form (lennard_jones.rs:107-111; shifted.rs:92-94).  Relative
+// errors: s2 5u, s4 11u, s8 23u, s12 35u, s6 17u (u = 2^-24), so
+// |e - e_exact| <= 40 u A with A = 4 eps (s12 + s6) + |shift|, and the
+// computed *a (A in FP32) gives |e - e_exact| <= 48 u *a with room to spare.
+__device__ __forceinline__ float lj_energy32(float sig2, float eps4, float shift, double r2, float* a) {
+  const float s2 = sig2 * __builtin_amdgcn_rcpf((float)r2);
+  const float s4 = s2 * s2, s8 = s4 * s4;
+  const float s12 = s4 * s8, s6 = s2 * s4;
+  *a = eps4 * (s12 + s6) + fabsf(shift);
+  return eps4 * (s12 - s6) - shift;
+}
 
 // Metropolis decision of parallel_sweep.rs:267-273 for dH known to within
 // +-delta of the reference's value (same terms, other summation order):
@@ -2001,80 +2030,48 @@ struct LjOut {
   unsigned accepted;
   unsigned ambiguous;
   unsigned exp_slow;
+  unsigned retests;  // mixed precision: decisions the FP32 bound left open, re-taken in FP64
 };
 
-// One LJ trial (parallel_sweep.rs:234-277) by a whole wavefront: the random
-// proposal is drawn identically in every lane, the lanes share out the
-// candidate sites of the 3^D (or 5^D) neighbour spaces, compute the final
-// and initial pair energies of each in FP64 exactly as the reference does,
-// and reduce.  The reference sums in its VecCell order; any two summation
-// orders of n terms differ by at most 2 gamma_{n-1} sum|terms|, so the
-// decision is exact whenever that bound cannot change it (counted otherwise).
-template <int D>
-__device__ void trial_lj(const Geometry& g, const ApplyState* st, const Bodies& B, const Cells& C, const Work& w,
-                         const double* zx, const double* zf, uint32_t s, const int32_t* m, uint32_t s0, uint32_t n,
-                         uint32_t colour, uint32_t substep, LjOut& o) {
+constexpr uint32_t kLjBatch = 4;  // candidate rounds of an LJ trial with their loads in flight together
+constexpr uint32_t kLjTab = 1024;  // candidate slots of an LJ trial laid out in LDS (else searched)
+
+// Sums over the candidate sites of one LJ trial (reduced over the
+// wavefront) of the final and initial pair energies and of their
+// magnitudes: FP64 exactly as the reference forms each term, or (F32) with
+// lj_energy32, whose magnitudes are the A of its error bound.  The
+// candidates are the slots of the neighbour-space segments (seg_lo, seg_pre:
+// entry j in register j / 64 of lane j % 64).  The membership test r^2 <
+// r_cut^2 is the FP64 one in both cases.  tab (when not null) holds the
+// slot of every candidate and replaces the segment search.
+template <int D, bool F32>
+__device__ __forceinline__ WaveSums lj_trial_sums(const Geometry& g, const Cells& C, const uint32_t* seg_lo,
+                                                  const uint32_t* seg_pre, uint32_t NS, uint32_t total,
+                                                  uint32_t slot, const double* t, const double* xi,
+                                                  const uint32_t* tab) {
+  using acc_t = typename std::conditional<F32, float, double>::type;
   const uint32_t lane = __lane_id();
-  Sfc64 rng = counter_rng(st->step, substep, st->seed, s);
-  const uint32_t pick = choose_index(rng, n);
-  double v[3] = {0.0, 0.0, 0.0};
-  unsigned exp_slow = 0;
-  ball_sample<D>(rng, st->max_move, zx, zf, v, &exp_slow);
-  o.exp_slow = lane == 0 ? exp_slow : 0u;
-  const uint32_t slot = s0 + pick;
-  const uint32_t body = C.body[slot];
-  const double4 bp = B.bpos[body];
-  const double4 sp = B.spos[body];
-  double x[3] = {bp.x, bp.y, bp.z};
-  for (int a = 0; a < D; ++a) x[a] += v[a];
-  for (int a = 0; a < D; ++a) x[a] = wrap1(x[a], g.L[a]);  // generate_trial_moves: wrap(propose(..))
-  bool accept = false;
-  if (space_of<D>(g, st->origin, x) == s) {
-    double t[3] = {0.0, 0.0, 0.0};  // wrap(x + 0) == wrap(x)
-    for (int a = 0; a < D; ++a) t[a] = wrap1(x[a], g.L[a]);
-    const double xi[3] = {sp.x, sp.y, sp.z};
-    // neighbour spaces (de-duplicated per axis), as segments of slots
-    uint32_t nb[3][5];
-    uint32_t nn[3] = {1, 1, 1};
-    for (int a = 0; a < D; ++a) {
-      const int32_t sh = g.shape[a], r = g.reach[a];
-      uint32_t k = 0;
-      for (int off = -r; off <= r; ++off) {
-        const int32_t mm = ((m[a] + off) % sh + sh) % sh;
-        bool seen = false;
-        for (uint32_t j = 0; j < k; ++j) seen |= nb[a][j] == (uint32_t)mm * g.stride[a];
-        if (!seen) nb[a][k++] = (uint32_t)mm * g.stride[a];
-      }
-      nn[a] = k;
-    }
-    const uint32_t NS = nn[0] * nn[1] * nn[2];  // at most 125
-    uint32_t seg_lo[2] = {0, 0}, seg_pre[2] = {0, 0};
-    uint32_t base = 0;
+  const float sig2 = (float)(g.lj_sigma * g.lj_sigma), eps4 = (float)g.lj_eps4, shift = (float)g.lj_shift;
+  acc_t ef = 0, af = 0, ei = 0, ai = 0;
+  uint32_t nf = 0, ni = 0;
+  // every lane runs the same number of rounds: the shuffles below read
+  // segment data held by other lanes, which must all be active.  kLjBatch
+  // rounds of 64 candidates at a time: their segment searches interleave and
+  // their site loads are in flight together.
+  for (uint32_t c0 = 0; c0 < total; c0 += 64 * kLjBatch) {
+    uint32_t qv[kLjBatch];
+    bool ok[kLjBatch];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const uint32_t j = lane + 64u * h;
-      uint32_t cnt = 0, lo = 0;
-      if (j < NS) {
-        const uint32_t i2 = j % nn[2], i1 = (j / nn[2]) % nn[1], i0 = j / (nn[2] * nn[1]);
-        const uint32_t ns = nb[0][i0] + (D > 1 ? nb[1][i1] : 0u) + (D > 2 ? nb[2][i2] : 0u);
-        lo = w.start[ns];
-        cnt = w.start[ns + 1] - lo;
+    for (uint32_t r = 0; r < kLjBatch; ++r) {
+      const uint32_t c = c0 + 64 * r + lane;
+      if (tab) {  // uniform
+        qv[r] = c < total ? tab[c] : 0u;
+        ok[r] = c < total && qv[r] != slot;
+        continue;
       }
-      const uint32_t incl = wave_incl_scan(cnt);
-      seg_lo[h] = lo;
-      seg_pre[h] = base + incl - cnt;
-      base += __shfl(incl, 63, 64);
-    }
-    const uint32_t total = base;
-    WaveSums ws{0.0, 0.0, 0.0, 0.0, 0u, 0u};
-    // every lane runs the same number of rounds: the shuffles below read
-    // segment data held by other lanes, which must all be active
-    for (uint32_t c0 = 0; c0 < total; c0 += 64) {
-      const uint32_t c = c0 + lane;
       // segment of candidate c: the last j with pre[j] <= c, by a fixed-depth
-      // binary search over pre[0..NS) (pre[0] = 0, NS <= 128)
-      // (entry j lives in register j / 64 of lane j % 64; both registers are
-      // shuffled and the reader selects)
+      // binary search over pre[0..NS) (pre[0] = 0, NS <= 128; both registers
+      // are shuffled and the reader selects)
       uint32_t jlo = 0;
 #pragma unroll
       for (uint32_t stepw = 64; stepw >= 1; stepw >>= 1) {
@@ -2087,35 +2084,163 @@ __device__ void trial_lj(const Geometry& g, const ApplyState* st, const Bodies& 
       const uint32_t la = __shfl(seg_lo[0], (int)(jlo & 63u), 64), lb2 = __shfl(seg_lo[1], (int)(jlo & 63u), 64);
       const uint32_t plo = jlo < 64 ? pa : pb;
       const uint32_t llo = jlo < 64 ? la : lb2;
-      const uint32_t q = llo + (c - plo);
-      // skip lanes past the end and the body's own site (filter body_tag != site_j.body_tag)
-      if (c >= total || q == slot) continue;
-      const double4 sj = C.pos64[q];
+      qv[r] = llo + (c - plo);
+      // lanes past the end and the body's own site (filter body_tag != site_j.body_tag)
+      ok[r] = c < total && qv[r] != slot;
+    }
+    double4 sjv[kLjBatch];
+#pragma unroll
+    for (uint32_t r = 0; r < kLjBatch; ++r) sjv[r] = ok[r] ? C.pos64[qv[r]] : make_double4(0.0, 0.0, 0.0, 0.0);
+#pragma unroll
+    for (uint32_t r = 0; r < kLjBatch; ++r) {
+      if (!ok[r]) continue;
+      const double4 sj = sjv[r];
       const double r2f = lj_r2<D>(g, t, sj);
       if (r2f < g.lj_rc2) {
-        const double e = lj_energy(g, r2f);
-        ws.ef += e;
-        ws.af += fabs(e);
-        ws.nf += 1;
+        if constexpr (F32) {
+          float a;
+          ef += lj_energy32(sig2, eps4, shift, r2f, &a);
+          af += a;
+        } else {
+          const double e = lj_energy(g, r2f);
+          ef += e;
+          af += fabs(e);
+        }
+        nf += 1;
       }
       const double r2i = lj_r2<D>(g, xi, sj);
       if (r2i < g.lj_rc2) {
-        const double e = lj_energy(g, r2i);
-        ws.ei += e;
-        ws.ai += fabs(e);
-        ws.ni += 1;
+        if constexpr (F32) {
+          float a;
+          ei += lj_energy32(sig2, eps4, shift, r2i, &a);
+          ai += a;
+        } else {
+          const double e = lj_energy(g, r2i);
+          ei += e;
+          ai += fabs(e);
+        }
+        ni += 1;
       }
     }
-    const double ef = wave_sum(ws.ef), af = wave_sum(ws.af);
-    const double ei = wave_sum(ws.ei), ai = wave_sum(ws.ai);
-    const uint32_t nf = wave_sum(ws.nf), ni = wave_sum(ws.ni);
-    // energy_final - energy_initial (pairwise_cutoff.rs:919-941)
-    const double dH = ef - ei;
-    const double nmax = (double)max(max(nf, ni), 1u);
-    const double gam = nmax * 0x1p-53 / (1.0 - nmax * 0x1p-53);
-    const double delta = (2.0 * gam * (af + ai) + 0x1p-52 * fabs(dH)) * (1.0 + 0x1p-20) + 0x1p-1070;
+  }
+  WaveSums ws;
+  ws.ef = (double)wave_sum(ef);
+  ws.af = (double)wave_sum(af);
+  ws.ei = (double)wave_sum(ei);
+  ws.ai = (double)wave_sum(ai);
+  ws.nf = wave_sum(nf);
+  ws.ni = wave_sum(ni);
+  return ws;
+}
+
+// One LJ trial (parallel_sweep.rs:234-277) by a whole wavefront: the random
+// proposal is drawn identically in every lane, the lanes share out the
+// candidate sites of the 3^D (or 5^D) neighbour spaces, compute the final
+// and initial pair energies of each in FP64 exactly as the reference does,
+// and reduce.  The reference sums in its VecCell order; any two summation
+// orders of n terms differ by at most 2 gamma_{n-1} sum|terms|, so the
+// decision is exact whenever that bound cannot change it (counted otherwise).
+// tab: kLjTab words of LDS for this wavefront (or null), where the
+// candidates' slots are laid out when there are at most kLjTab of them.  The neighbour
+// segments are listed before the proposal is drawn, so that their loads and
+// the body's overlap the random draws.
+template <int D, int PREC>
+__device__ void trial_lj(const Geometry& g, const ApplyState* st, const Bodies& B, const Cells& C, const Work& w,
+                         const double* zx, const double* zf, uint32_t s, const int32_t* m, uint32_t s0, uint32_t n,
+                         uint32_t colour, uint32_t substep, LjOut& o, uint32_t* tab) {
+  const uint32_t lane = __lane_id();
+  Sfc64 rng = counter_rng(st->step, substep, st->seed, s);
+  const uint32_t pick = choose_index(rng, n);
+  const uint32_t slot = s0 + pick;
+  const uint32_t body = C.body[slot];
+  // neighbour spaces (de-duplicated per axis), as segments of slots
+  uint32_t nb[3][5];
+  uint32_t nn[3] = {1, 1, 1};
+  for (int a = 0; a < D; ++a) {
+    const int32_t sh = g.shape[a], r = g.reach[a];
+    uint32_t k = 0;
+    for (int off = -r; off <= r; ++off) {
+      const int32_t mm = ((m[a] + off) % sh + sh) % sh;
+      bool seen = false;
+      for (uint32_t j = 0; j < k; ++j) seen |= nb[a][j] == (uint32_t)mm * g.stride[a];
+      if (!seen) nb[a][k++] = (uint32_t)mm * g.stride[a];
+    }
+    nn[a] = k;
+  }
+  const uint32_t NS = nn[0] * nn[1] * nn[2];  // at most 125
+  uint32_t seg_lo[2] = {0, 0}, seg_pre[2] = {0, 0}, seg_cnt[2] = {0, 0};
+  uint32_t base = 0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t j = lane + 64u * h;
+    uint32_t cnt = 0, lo = 0;
+    if (j < NS) {
+      const uint32_t i2 = j % nn[2], i1 = (j / nn[2]) % nn[1], i0 = j / (nn[2] * nn[1]);
+      const uint32_t ns = nb[0][i0] + (D > 1 ? nb[1][i1] : 0u) + (D > 2 ? nb[2][i2] : 0u);
+      lo = w.start[ns];
+      cnt = w.start[ns + 1] - lo;
+    }
+    const uint32_t incl = wave_incl_scan(cnt);
+    seg_lo[h] = lo;
+    seg_pre[h] = base + incl - cnt;
+    seg_cnt[h] = cnt;
+    base += __shfl(incl, 63, 64);
+  }
+  const uint32_t total = base;
+  // the candidates' slots in LDS (a run of consecutive slots per segment)
+  const bool tabbed = tab != nullptr && total <= kLjTab;
+  if (tabbed) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+      for (uint32_t k = 0; k < seg_cnt[h]; ++k) tab[seg_pre[h] + k] = seg_lo[h] + k;
+    wave_lds_order();
+  }
+  const double4 bp = B.bpos[body];
+  const double4 sp = B.spos[body];
+  double v[3] = {0.0, 0.0, 0.0};
+  unsigned exp_slow = 0;
+  ball_sample<D>(rng, st->max_move, zx, zf, v, &exp_slow);
+  o.exp_slow = lane == 0 ? exp_slow : 0u;
+  double x[3] = {bp.x, bp.y, bp.z};
+  for (int a = 0; a < D; ++a) x[a] += v[a];
+  for (int a = 0; a < D; ++a) x[a] = wrap1(x[a], g.L[a]);  // generate_trial_moves: wrap(propose(..))
+  bool accept = false;
+  if (space_of<D>(g, st->origin, x) == s) {
+    double t[3] = {0.0, 0.0, 0.0};  // wrap(x + 0) == wrap(x)
+    for (int a = 0; a < D; ++a) t[a] = wrap1(x[a], g.L[a]);
+    const double xi[3] = {sp.x, sp.y, sp.z};
+    const uint32_t* tb = tabbed ? tab : nullptr;
+    const uint32_t total = base;
     bool amb = false;
-    accept = lj_metropolis(rng, dH, delta, st->kT, &amb) != 0;
+    bool decided = false;
+    if constexpr (PREC == HMC_MIXED) {
+      // FP32 energies: the difference of the two FP32 sums is exact in FP64;
+      // |dH32 - dH| <= (48 u + gamma_(rounds + 6)) (A_f + A_i) for the exact
+      // dH (each lane sums at most `rounds` terms, the reduction adds 6
+      // levels), plus 2^-40 (A_f + A_i) for the reference's own roundings.
+      const WaveSums w32 = lj_trial_sums<D, true>(g, C, seg_lo, seg_pre, NS, total, slot, t, xi, tb);
+      const double dH32 = w32.ef - w32.ei;
+      const double rounds = (double)((total + 63) / 64);
+      const double delta32 = ((48.0 + rounds + 8.0) * 0x1p-24 + 0x1p-40) * (w32.af + w32.ai) * (1.0 + 0x1p-10) +
+                             0x1p-1070;
+      const Sfc64 rng0 = rng;
+      bool open = false;
+      accept = lj_metropolis(rng, dH32, delta32, st->kT, &open) != 0;
+      decided = !open;
+      if (open) {
+        rng = rng0;  // the FP64 decision draws the same u
+        o.retests = lane == 0 ? 1u : 0u;
+      }
+    }
+    if (!decided) {
+      const WaveSums ws = lj_trial_sums<D, false>(g, C, seg_lo, seg_pre, NS, total, slot, t, xi, tb);
+      // energy_final - energy_initial (pairwise_cutoff.rs:919-941)
+      const double dH = ws.ef - ws.ei;
+      const double nmax = (double)max(max(ws.nf, ws.ni), 1u);
+      const double gam = nmax * 0x1p-53 / (1.0 - nmax * 0x1p-53);
+      const double delta = (2.0 * gam * (ws.af + ws.ai) + 0x1p-52 * fabs(dH)) * (1.0 + 0x1p-20) + 0x1p-1070;
+      accept = lj_metropolis(rng, dH, delta, st->kT, &amb) != 0;
+    }
     o.ambiguous = (amb && lane == 0) ? 1u : 0u;
     if (accept && lane == 0) {
       // update_body_properties (microstate.rs:985-1034)
@@ -2145,28 +2270,30 @@ __device__ void trial_lj(const Geometry& g, const ApplyState* st, const Bodies& 
 }
 
 // One colour of one cycle for LJ: one wavefront per space of the colour.
-template <int D>
+template <int D, int PREC>
 __global__ void __launch_bounds__(kBlock) k_pass_lj(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
                                                     uint32_t colour, uint32_t cycle, uint32_t xcd_chunk) {
   if (cycle >= st->cycles) return;  // uniform over the grid
   __shared__ double zx[257], zf[257];
   __shared__ unsigned blk_acc;
+  __shared__ uint32_t tab[kBlock / 64][kLjTab];
   if (D == 3) stage_zig(zx, zf);
   if (threadIdx.x == 0) blk_acc = 0;
   __syncthreads();
   const uint32_t b = blockIdx.x;
   const uint32_t lb = (b & 7u) * xcd_chunk + (b >> 3);
   const uint32_t t = lb * (kBlock / 64) + (threadIdx.x >> 6);  // one space per wavefront
-  LjOut o{0, 0, 0};
+  LjOut o{0, 0, 0, 0};
   if (t < g.per_colour) {
     int32_t m[3] = {0, 0, 0};
     const uint32_t s = colour_space<D>(g, colour, t, m);
     const uint32_t s0 = w.start[s];
     const uint32_t n = w.start[s + 1] - s0;
-    if (n) trial_lj<D>(g, st, B, C, w, zx, zf, s, m, s0, n, colour, st->substep0 + cycle, o);
+    if (n) trial_lj<D, PREC>(g, st, B, C, w, zx, zf, s, m, s0, n, colour, st->substep0 + cycle, o, tab[threadIdx.x >> 6]);
   }
   if (o.accepted) atomicAdd(&blk_acc, 1u);
   lane_add(&st->ambiguous, o.ambiguous);
+  lane_add(&st->band_retests, o.retests);
   lane_add(&st->exp_slow, o.exp_slow);
   __syncthreads();
   if (threadIdx.x == 0 && blk_acc) atomicAdd(&w.acc_slots[b & (kAccSlots - 1)], (unsigned long long)blk_acc);
@@ -2185,12 +2312,12 @@ __global__ void __launch_bounds__(kGuardThreads) k_guard_lj(Geometry g, ApplySta
   for (uint32_t k = first_cycle; k < cycles; ++k) {
     for (uint32_t c = 0; c < (1u << D); ++c) {
       for (uint32_t t = wid; t < g.per_colour; t += waves) {
-        LjOut o{0, 0, 0};
+        LjOut o{0, 0, 0, 0};
         int32_t m[3] = {0, 0, 0};
         const uint32_t s = colour_space<D>(g, c, t, m);
         const uint32_t s0 = w.start[s];
         const uint32_t n = w.start[s + 1] - s0;
-        if (n) trial_lj<D>(g, st, B, C, w, zx, zf, s, m, s0, n, c, st->substep0 + k, o);
+        if (n) trial_lj<D, HMC_FP64>(g, st, B, C, w, zx, zf, s, m, s0, n, c, st->substep0 + k, o, nullptr);
         if (o.accepted) atomicAdd(&w.acc_slots[0], 1ull);
         lane_add(&st->ambiguous, o.ambiguous);
         lane_add(&st->exp_slow, o.exp_slow);
@@ -2388,14 +2515,6 @@ struct CvWave {
   double t[kCvTrials][3];            // trial site position
   double qt[kCvTrials][4];           // trial site orientation
 };
-
-// Orders the LDS accesses of one wavefront (lanes read what other lanes of
-// the same wavefront wrote; LDS operations of a wavefront complete in order).
-__device__ __forceinline__ void wave_lds_order() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // XenoCollide over the np listed pairs; returns the trials (bits) that
 // found an overlap, skipping pairs of trials already in `hits`.  NV: the
@@ -3278,7 +3397,14 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
         (void)hipEventRecord(p0, s);
       }
       if (lj)
-        k_pass_lj<D><<<lj_chunk * 8, kBlock, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k, lj_chunk);
+      {
+        if (g.lj_mixed)
+          k_pass_lj<D, HMC_MIXED><<<lj_chunk * 8, kBlock, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, (uint32_t)col,
+                                                                  k, lj_chunk);
+        else
+          k_pass_lj<D, HMC_FP64><<<lj_chunk * 8, kBlock, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, (uint32_t)col,
+                                                                 k, lj_chunk);
+      }
       else if (convex)
         k_pass_convex<<<cv_chunk * 8, kBlock, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k,
                                                       cv_chunk);
@@ -3635,7 +3761,7 @@ static int make_geometry(const hmc_config* cfg, Geometry& g, bool* fast, double*
   const bool convex = cfg->model == HMC_CONVEX_POLYHEDRON;
   if (cfg->model != HMC_HARD_SPHERE && !lj && !convex) return HMC_ERR_UNSUPPORTED;
   if (cfg->precision != HMC_FP64 && cfg->precision != HMC_MIXED) return HMC_ERR_INVALID;
-  if ((lj || convex) && cfg->precision != HMC_FP64) return HMC_ERR_UNSUPPORTED;  // FP64 only
+  if (convex && cfg->precision != HMC_FP64) return HMC_ERR_UNSUPPORTED;  // FP64 only
   if (convex && cfg->dim != 3) return HMC_ERR_UNSUPPORTED;
   if (convex && (!cfg->vertices || cfg->n_vertices < 1 || cfg->n_vertices > (uint32_t)kMaxVertices))
     return HMC_ERR_INVALID;
@@ -3761,9 +3887,16 @@ static int make_geometry(const hmc_config* cfg, Geometry& g, bool* fast, double*
   return HMC_OK;
 }
 
-int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
-  if (!cfg || !out) return HMC_ERR_INVALID;
+int hmc_create(const hmc_config* cfg_in, hmc_ctx** out) {
+  if (!cfg_in || !out) return HMC_ERR_INVALID;
   *out = nullptr;
+  // LJ in mixed precision keeps the FP64 data layout and passes; only its
+  // pair energies run in FP32 (Geometry::lj_mixed)
+  hmc_config cfg_local = *cfg_in;
+  const bool lj_mixed = (cfg_in->model == HMC_LJ || cfg_in->model == HMC_LJ_SHIFTED) &&
+                        cfg_in->precision == HMC_MIXED;
+  if (lj_mixed) cfg_local.precision = HMC_FP64;
+  const hmc_config* cfg = &cfg_local;
   Geometry g0;
   bool fast0 = false;
   double sigma = 0.0, lmax = 0.0;
@@ -3779,6 +3912,7 @@ int hmc_create(const hmc_config* cfg, hmc_ctx** out) {
   c->cfg = *cfg;
   c->device = cfg->device;
   c->g = g0;
+  c->g.lj_mixed = lj_mixed ? 1 : 0;
   c->fast = fast0;
   Geometry& g = c->g;
   if (hipSetDevice(c->device) != hipSuccess) {

@@ -5,7 +5,10 @@ Bar: per-apply counts, sorted per-trial traces and final FP64 positions bit
 for bit, and no ambiguous decision.  The GPU sums the pair energies in
 another order than the reference's VecCell order; every decision is taken
 only when the rigorous bound on that difference cannot change it, and the
-library counts the ones where it could (expected: none).
+library counts the ones where it could (expected: none).  In mixed
+precision the pair energies are FP32 first; decisions the FP32 error bound
+leaves open are re-taken in FP64 (counted as band re-tests), so the bar is
+the same.
 """
 import ctypes as C
 import sys
@@ -40,10 +43,10 @@ def wl():
     return _pkg.workloads()
 
 
-def _replay_lj(hmc, L, pos, model, params, kT, d, n_applies, seed=42, trace=True):
+def _replay_lj(hmc, L, pos, model, params, kT, d, n_applies, seed=42, trace=True, precision="fp64"):
     ref = O.OracleSystem(L, pos, model=model, params=params, seed=seed)
     b, s = ref.positions()
-    gpu = hmc.GpuParallelSweep(L, b, sites=s, model=model, params=params)
+    gpu = hmc.GpuParallelSweep(L, b, sites=s, model=model, params=params, precision=precision)
     cap = 64 * len(pos) + 1024
     if trace:
         gpu.trace(True, capacity=cap)
@@ -74,37 +77,45 @@ def _replay_lj(hmc, L, pos, model, params, kT, d, n_applies, seed=42, trace=True
     return st, ref
 
 
-def test_lj_shifted_fcc_3d(hmc, wl):
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lj_shifted_fcc_3d(hmc, wl, precision):
     # 864 LJ particles, FCC at number density 0.8, kT 1.2, r_cut = r_shift = 2.5
     L, pos = wl.fcc_number_density((6, 6, 6), 0.8)
     t = time.time()
-    st, _ = _replay_lj(hmc, L, pos, "lj_shifted", (1.0, 1.0, 2.5, 2.5), 1.2, 0.1, 10)
-    log(f"LJ shifted N=864: 10 applies bit-identical ({time.time() - t:.1f}s) {st}")
+    st, _ = _replay_lj(hmc, L, pos, "lj_shifted", (1.0, 1.0, 2.5, 2.5), 1.2, 0.1, 10, precision=precision)
+    log(f"LJ shifted N=864 {precision}: 10 applies bit-identical ({time.time() - t:.1f}s) {st}")
 
 
-def test_lj_plain_square_2d(hmc, wl):
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lj_plain_square_2d(hmc, wl, precision):
     # 2D LJ without shift on a 24 x 24 square lattice at density 0.7
     L, pos = wl.square_lattice_disks(n_side=24, packing_fraction=0.7 * np.pi / 4)
-    st, _ = _replay_lj(hmc, L, pos, "lj", (1.0, 1.0, 2.5), 1.0, 0.15, 10, seed=3)
-    log(f"LJ 2D N=576: 10 applies bit-identical {st}")
+    st, _ = _replay_lj(hmc, L, pos, "lj", (1.0, 1.0, 2.5), 1.0, 0.15, 10, seed=3, precision=precision)
+    log(f"LJ 2D N=576 {precision}: 10 applies bit-identical {st}")
 
 
-def test_lj_space_width_equal_to_cutoff(hmc, wl):
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lj_space_width_equal_to_cutoff(hmc, wl, precision):
     # box 10 with r_cut 2.5: four spaces of width exactly r_cut per axis, so the
     # neighbour scan reaches two spaces out (and wraps onto itself), and the
     # FCC second shell sits exactly at the cutoff
     L, pos = wl.fcc_at_volume((4, 4, 4), 1000.0)
-    st, _ = _replay_lj(hmc, L, pos, "lj_shifted", (1.0, 1.0, 2.5, 2.5), 1.5, 0.2, 8, seed=9)
-    log(f"LJ reach-2 N=256: 8 applies bit-identical {st}")
+    st, _ = _replay_lj(hmc, L, pos, "lj_shifted", (1.0, 1.0, 2.5, 2.5), 1.5, 0.2, 8, seed=9, precision=precision)
+    log(f"LJ reach-2 N=256 {precision}: 8 applies bit-identical {st}")
 
 
-def test_lj_config_c4_full_size(hmc, wl):
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lj_config_c4_full_size(hmc, wl, precision):
     # N = 524,288 LJ particles: FCC 64 x 64 x 32 cells at number density 0.8
     # (box 109.44 x 109.44 x 54.72), kT 1.2, shifted at r_cut 2.5, d = 0.1
     L, pos = wl.fcc_number_density((64, 64, 32), 0.8)
     t = time.time()
-    st, ref = _replay_lj(hmc, L, pos, "lj_shifted", (1.0, 1.0, 2.5, 2.5), 1.2, 0.1, 2, trace=False)
-    log(f"C4 N={len(pos)}: 2 applies bit-identical ({time.time() - t:.1f}s) {st}")
+    st, ref = _replay_lj(hmc, L, pos, "lj_shifted", (1.0, 1.0, 2.5, 2.5), 1.2, 0.1, 2, trace=False,
+                         precision=precision)
+    log(f"C4 N={len(pos)} {precision}: 2 applies bit-identical ({time.time() - t:.1f}s) {st}")
+    if precision == "mixed":
+        # the FP32 bound leaves few decisions open
+        assert st["band_retests"] < 0.01 * 2 * len(pos), st
 
 
 def test_total_energy_matches_oracle(hmc, wl):
