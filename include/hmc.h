@@ -44,19 +44,22 @@ enum {
 enum {
   HMC_HARD_SPHERE = 0, /* HardSphere (pairwise/hard_shape.rs:127-174): params[0] = diameter */
   HMC_LJ = 1,          /* Isotropic<LennardJones<12,6>> (isotropic.rs, univariate/lennard_jones.rs):
-                          params = {epsilon, sigma, r_cut, -}; HMC_FP64 only */
+                          params = {epsilon, sigma, r_cut, -} */
   HMC_LJ_SHIFTED = 2,  /* Isotropic<Shifted<LennardJones<12,6>>> (univariate/shifted.rs):
-                          params = {epsilon, sigma, r_cut, r_shift}; HMC_FP64 only */
+                          params = {epsilon, sigma, r_cut, r_shift} */
   HMC_CONVEX_POLYHEDRON = 3 /* HardShape<Convex<ConvexPolyhedron>> (pairwise/hard_shape.rs:79-118,
                                hoomd-geometry/src/convex.rs:146-170) on OrientedPoint<Cartesian<3>,
                                Versor> bodies: hmc_config.vertices; dim 3, HMC_FP64 only */
 };
 
-/* Arithmetic of the overlap test. Both modes make identical decisions. */
+/* Arithmetic of the pair test. Both modes make identical decisions. */
 enum {
   HMC_FP64 = 0,  /* every pair in FP64, exactly as the reference */
-  HMC_MIXED = 1  /* FP32 pair test; pairs inside its rigorous error band are
-                    re-tested in FP64 */
+  HMC_MIXED = 1  /* hard spheres: FP32 overlap test, pairs inside its rigorous
+                    error band re-tested in FP64.  LJ: FP32 pair energies
+                    (FP64 displacements and cutoff test), the trial's
+                    Metropolis decision re-taken in FP64 when the FP32 error
+                    bound on dH could change it */
 };
 
 typedef struct {
@@ -109,7 +112,7 @@ typedef struct {
   uint64_t applies;         /* apply calls since creation */
   uint64_t cycles;          /* colour cycles (= substeps consumed) */
   uint64_t pair_tests;      /* not maintained (0) */
-  uint64_t band_retests;    /* MIXED: pairs re-tested in FP64 */
+  uint64_t band_retests;    /* MIXED: hard-sphere pairs, or LJ trials, re-tested in FP64 */
   uint64_t exp_slow_path;   /* exp comparisons decided by the correctly rounded exp */
   uint64_t guard_cycles;    /* cycles run by the single-workgroup overflow guard */
   uint64_t tile_overflows;  /* LDS tiles whose box exceeded the staging capacity
