@@ -203,6 +203,12 @@ def other_workloads(hmc, wl):
     res["C5_replicas_256"]["sdf"] = {"ms_per_call": ms, "bodies_per_s": nb / (ms * 1e-3), "overlaps": int(ov.sum()),
                                      "beta_p_over_rho_one_frame": {"phi_0.30": z[0], "phi_0.54": z[-1]}}
     batch.close()
+    # the same batch in plain FP64 (identical decisions)
+    batch = hmc.GpuBatch([(L, wl.wrap(L, p), None) for L, p in systems], precision="fp64")
+    v64, _ = timed(lambda s0, k: sum(a + r for a, r in batch.run(s0, seeds, 1.0, 0.08, k)), 2, 40)
+    res["C5_replicas_256"]["fp64_value"] = v64
+    res["C5_replicas_256"]["mixed_over_fp64"] = res["C5_replicas_256"]["value"] / v64
+    batch.close()
     return res
 
 

@@ -3176,29 +3176,62 @@ __global__ void __launch_bounds__(kBlock) k_sdf(const BatchRep* reps, const uint
     }
     nn[a] = k;
   }
+  auto r2_to = [&](const double4& sj) {
+    const double u[3] = {sj.x, sj.y, sj.z};
+    double r2 = 0.0;
+    for (int a = 0; a < D; ++a) {
+      double d = t[a] - u[a];
+      if (d > g.half[a])
+        d -= g.L[a];
+      else if (d < -g.half[a])
+        d += g.L[a];
+      r2 = r2 + d * d;
+    }
+    return r2;
+  };
   double best = __builtin_inf();
-  for (int i0 = 0; i0 < nn[0]; ++i0)
-    for (int i1 = 0; i1 < nn[1]; ++i1)
-      for (int i2 = 0; i2 < (D == 3 ? nn[2] : 1); ++i2) {
-        const uint32_t ns = soff + nb[0][i0] + nb[1][i1] + (D == 3 ? nb[2][i2] : 0u);
-        const uint32_t q1 = w.start[ns + 1];
-        for (uint32_t q = w.start[ns]; q < q1; ++q) {
-          const uint32_t j = w.tmp_body[q];
-          if (j == b) continue;
-          const double4 sj = B.spos[j];
-          const double u[3] = {sj.x, sj.y, sj.z};
-          double r2 = 0.0;
-          for (int a = 0; a < D; ++a) {
-            double d = t[a] - u[a];
-            if (d > g.half[a])
-              d -= g.L[a];
-            else if (d < -g.half[a])
-              d += g.L[a];
-            r2 = r2 + d * d;
-          }
-          best = fmin(best, r2);
-        }
+  constexpr int NC = D == 3 ? 27 : 9;
+  if (nn[0] == 3 && nn[1] == 3 && (D == 2 || nn[2] == 3)) {
+    // 3^D distinct neighbour spaces: their ranges first, then rounds (the
+    // k-th slot of every space), each with its loads issued before any test
+    uint32_t lo[NC], hi[NC];
+    uint32_t rounds = 0;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const uint32_t ns = soff + nb[0][c / (D == 3 ? 9 : 3)] + nb[1][(c / (D == 3 ? 3 : 1)) % 3] +
+                          (D == 3 ? nb[2][c % 3] : 0u);
+      lo[c] = w.start[ns];
+      hi[c] = w.start[ns + 1];
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) rounds = max(rounds, hi[c] - lo[c]);
+    constexpr int CH = 9;
+    for (uint32_t k = 0; k < rounds; ++k) {
+#pragma unroll
+      for (int c0 = 0; c0 < NC; c0 += CH) {
+        uint32_t jv[CH];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) jv[j] = lo[c0 + j] + k < hi[c0 + j] ? w.tmp_body[lo[c0 + j] + k] : b;
+        double4 sv[CH];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) sv[j] = B.spos[jv[j]];
+#pragma unroll
+        for (int j = 0; j < CH; ++j)
+          if (jv[j] != b) best = fmin(best, r2_to(sv[j]));
       }
+    }
+  } else {
+    for (int i0 = 0; i0 < nn[0]; ++i0)
+      for (int i1 = 0; i1 < nn[1]; ++i1)
+        for (int i2 = 0; i2 < (D == 3 ? nn[2] : 1); ++i2) {
+          const uint32_t ns = soff + nb[0][i0] + nb[1][i1] + (D == 3 ? nb[2][i2] : 0u);
+          const uint32_t q1 = w.start[ns + 1];
+          for (uint32_t q = w.start[ns]; q < q1; ++q) {
+            const uint32_t j = w.tmp_body[q];
+            if (j != b) best = fmin(best, r2_to(B.spos[j]));
+          }
+        }
+  }
   if (!(best < __builtin_inf())) return;  // no other body nearby
   const double x = 1.0 - g.sigma / sqrt(best);
   if (x < 0.0) {
@@ -4287,12 +4320,38 @@ static const char* sdf_check(const Geometry& g, double xmax, double dx, uint32_t
   return nullptr;
 }
 
+// The SDF's own cells: the checkerboard with spaces at least r_max (1 + 2^-29)
+// wide (never more spaces than the sweep's, whose buffers it reuses), so
+// that one neighbour space on each side covers r_max.
+static Geometry sdf_geometry(const Geometry& g, double xmax) {
+  Geometry s = g;
+  const double rmax = g.sigma / (1.0 - xmax);
+  uint64_t spaces = 1;
+  for (int a = 0; a < g.dim; ++a) {
+    int64_t k = (int64_t)std::floor(g.L[a] / (rmax * (1.0 + 0x1p-29)));
+    k = std::max<int64_t>(1, std::min<int64_t>(k, g.shape[a]));
+    s.shape[a] = (int32_t)k;
+    s.half_shape[a] = (uint32_t)std::max<int64_t>(1, k / 2);
+    s.width[a] = g.L[a] / (double)k;
+    spaces *= (uint64_t)k;
+  }
+  uint32_t str = 1;
+  for (int a = g.dim - 1; a >= 0; --a) {
+    s.stride[a] = str;
+    str *= (uint32_t)s.shape[a];
+  }
+  s.n_spaces = (uint32_t)spaces;
+  s.per_colour = s.n_spaces >> g.dim;
+  return s;
+}
+
 int hmc_sdf(hmc_ctx* c, double xmax, double dx, uint32_t nbins, uint64_t* hist, uint64_t* overlaps) {
   if (!c || !hist) return HMC_ERR_INVALID;
   if (!c->have_state) return fail(c, HMC_ERR_STATE, "hmc_set_state was not called");
-  const Geometry& g = c->g;
-  if (const char* why = sdf_check(g, xmax, dx, nbins))
-    return fail(c, g.model != HMC_HARD_SPHERE ? HMC_ERR_UNSUPPORTED : HMC_ERR_INVALID, why);
+  if (const char* why = sdf_check(c->g, xmax, dx, nbins))
+    return fail(c, c->g.model != HMC_HARD_SPHERE ? HMC_ERR_UNSUPPORTED : HMC_ERR_INVALID, why);
+  const Geometry g = sdf_geometry(c->g, xmax);
+  const uint32_t nsb = grid_for(g.n_spaces, kScanBlock);
   const uint32_t N = g.n_bodies;
   HIPCHK(c, hipSetDevice(c->device));
   hipStream_t s = c->stream;
@@ -4310,15 +4369,15 @@ int hmc_sdf(hmc_ctx* c, double xmax, double dx, uint32_t nbins, uint64_t* hist, 
   if (e == hipSuccess) {
     k_fill_u32<<<grid_for(nbins + 1, 256), 256, 0, s>>>(dh, nbins + 1, 0u);
     if (N) {
-      // cells of the current state (the checkerboard of the current counter)
+      // cells of the current state on the SDF's checkerboard (origin drawn
+      // from the current counter; the next apply rebuilds its own)
       k_setup<<<1, 64, 0, s>>>(g, c->st, 0, 0, 0, 1.0, 1.0, 0, 0, 0u);
       if (g.dim == 2)
         k_bin<2><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
       else
         k_bin<3><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
-      k_scan_blocks<<<c->n_scan_blocks, 256, 0, s>>>(c->w.cnt, c->w.start, c->w.block_sums, c->w.block_ne,
-                                                     g.n_spaces);
-      k_scan_top<<<1, 1024, 0, s>>>(g, c->st, c->w.block_sums, c->w.block_ne, c->n_scan_blocks, c->w.start, 0);
+      k_scan_blocks<<<nsb, 256, 0, s>>>(c->w.cnt, c->w.start, c->w.block_sums, c->w.block_ne, g.n_spaces);
+      k_scan_top<<<1, 1024, 0, s>>>(g, c->st, c->w.block_sums, c->w.block_ne, nsb, c->w.start, 0);
       k_scan_add<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.start, c->w.block_sums, c->w.cnt, g.n_spaces);
       k_scatter<<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->w);
       if (g.dim == 2)
@@ -4725,37 +4784,52 @@ int hmc_batch_sdf(hmc_batch* b, double xmax, double dx, uint32_t nbins, uint64_t
   if ((uint64_t)b->R * (nbins + 1ull) >= (1ull << 31)) return bfail(b, HMC_ERR_INVALID, "n_replicas * nbins too large");
   BHIPCHK(b, hipSetDevice(b->device));
   hipStream_t s = b->stream;
+  // the replicas laid out on the SDF's checkerboards (fewer, wider spaces)
+  std::vector<BatchRep> hr(b->R);
+  uint32_t stot = 0;
+  for (uint32_t r = 0; r < b->R; ++r) {
+    hr[r] = b->hreps[r];
+    hr[r].g = sdf_geometry(b->hreps[r].g, xmax);
+    hr[r].space_off = stot;
+    stot += hr[r].g.n_spaces;
+  }
+  Geometry gt = b->gtot;
+  gt.n_spaces = stot;
+  const uint32_t nsb = grid_for(stot, kScanBlock);
   const size_t cells = (size_t)b->R * nbins + b->R;
-  uint32_t* dh = nullptr;
-  BHIPCHK(b, hipMallocAsync((void**)&dh, cells * sizeof(uint32_t), s));
+  char* buf = nullptr;
+  BHIPCHK(b, hipMallocAsync((void**)&buf, b->R * sizeof(BatchRep) + cells * sizeof(uint32_t), s));
+  BatchRep* dreps = (BatchRep*)buf;
+  uint32_t* dh = (uint32_t*)(buf + b->R * sizeof(BatchRep));
+  hipError_t e = hipMemcpyAsync(dreps, hr.data(), b->R * sizeof(BatchRep), hipMemcpyHostToDevice, s);
   k_fill_u32<<<grid_for((uint32_t)cells, 256), 256, 0, s>>>(dh, (uint32_t)cells, 0u);
-  if (b->n_total) {
-    // cells of the current states (the checkerboards of the current counters)
-    k_bsetup<<<grid_for(b->R, 64), 64, 0, s>>>(b->reps, b->sts, b->R, b->dstep, b->dseed, b->kT, b->dmove, 0, 0);
-    k_fill_u32<<<grid_for(b->s_total, 256), 256, 0, s>>>(b->w.cnt, b->s_total, 0u);
+  if (e == hipSuccess && b->n_total) {
+    // cells of the current states (origins drawn from the current counters;
+    // the next step rebuilds the sweep's own)
+    k_bsetup<<<grid_for(b->R, 64), 64, 0, s>>>(dreps, b->sts, b->R, b->dstep, b->dseed, b->kT, b->dmove, 0, 0);
+    k_fill_u32<<<grid_for(stot, 256), 256, 0, s>>>(b->w.cnt, stot, 0u);
     if (b->dim == 2)
-      k_bbin<2><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->reps, b->sts, b->dboff, b->R, b->n_total,
+      k_bbin<2><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(dreps, b->sts, b->dboff, b->R, b->n_total,
                                                                 b->bodies, b->w);
     else
-      k_bbin<3><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->reps, b->sts, b->dboff, b->R, b->n_total,
+      k_bbin<3><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(dreps, b->sts, b->dboff, b->R, b->n_total,
                                                                 b->bodies, b->w);
-    k_scan_blocks<<<b->n_scan_blocks, 256, 0, s>>>(b->w.cnt, b->w.start, b->w.block_sums, b->w.block_ne, b->s_total);
-    k_scan_top<<<1, 1024, 0, s>>>(b->gtot, b->sts + b->R, b->w.block_sums, b->w.block_ne, b->n_scan_blocks,
-                                  b->w.start, 0);
-    k_scan_add<<<grid_for(b->s_total, 256), 256, 0, s>>>(b->w.start, b->w.block_sums, nullptr, b->s_total);
+    k_scan_blocks<<<nsb, 256, 0, s>>>(b->w.cnt, b->w.start, b->w.block_sums, b->w.block_ne, stot);
+    k_scan_top<<<1, 1024, 0, s>>>(gt, b->sts + b->R, b->w.block_sums, b->w.block_ne, nsb, b->w.start, 0);
+    k_scan_add<<<grid_for(stot, 256), 256, 0, s>>>(b->w.start, b->w.block_sums, nullptr, stot);
     k_scatter<<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->gtot, b->w);
     if (b->dim == 2)
-      k_sdf<2><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->reps, b->dboff, b->R, b->n_total, b->bodies, b->w,
+      k_sdf<2><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(dreps, b->dboff, b->R, b->n_total, b->bodies, b->w,
                                                                xmax, dx, nbins, dh, dh + (size_t)b->R * nbins);
     else
-      k_sdf<3><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->reps, b->dboff, b->R, b->n_total, b->bodies, b->w,
+      k_sdf<3><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(dreps, b->dboff, b->R, b->n_total, b->bodies, b->w,
                                                                xmax, dx, nbins, dh, dh + (size_t)b->R * nbins);
   }
   std::vector<uint32_t> h(cells, 0u);
-  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(h.data(), dh, cells * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  (void)hipFreeAsync(dh, s);
+  (void)hipFreeAsync(buf, s);
   if (e != hipSuccess) return bfail(b, HMC_ERR_DEVICE, std::string("batch sdf: ") + hipGetErrorString(e));
   std::vector<ApplyState> st;
   std::vector<unsigned long long> acc;
