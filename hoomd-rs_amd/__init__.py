@@ -33,7 +33,7 @@ EXPORTS = ["hmc_create", "hmc_set_state", "hmc_get_state", "hmc_apply_translate"
            "hmc_trace_enable", "hmc_trace_read", "hmc_get_stats", "hmc_profile_enable", "hmc_profile_read",
            "hmc_debug_math", "hmc_debug_normals", "hmc_debug_convex", "hmc_last_error", "hmc_destroy",
            "hmc_batch_create", "hmc_batch_set_state", "hmc_batch_get_state", "hmc_batch_run_translate",
-           "hmc_batch_last_error", "hmc_batch_destroy", "hmc_sdf", "hmc_batch_sdf"]
+           "hmc_batch_last_error", "hmc_batch_destroy", "hmc_sdf", "hmc_batch_sdf", "hmc_batch_get_stats"]
 
 
 class HmcError(RuntimeError):
@@ -95,6 +95,7 @@ def lib():
         u64p = C.POINTER(C.c_uint64)
         L.hmc_sdf.argtypes = [vp, C.c_double, C.c_double, C.c_uint32, u64p, u64p]
         L.hmc_batch_sdf.argtypes = [vp, C.c_double, C.c_double, C.c_uint32, u64p, u64p]
+        L.hmc_batch_get_stats.argtypes = [vp, C.c_uint32, C.POINTER(Stats)]
         L.hmc_batch_create.argtypes = [C.c_uint32, C.POINTER(Config), C.POINTER(vp)]
         L.hmc_batch_set_state.argtypes = [vp, C.c_uint32, D_P, D_P]
         L.hmc_batch_get_state.argtypes = [vp, C.c_uint32, D_P, D_P]
@@ -328,8 +329,10 @@ class GpuBatch:
     """Independent hard-sphere/disk replicas advanced together (one set of
     launches per step, replayed from a captured hipGraph)."""
 
-    def __init__(self, systems, *, diameter=1.0, precision="fp64", device=0):
-        """systems: list of (box_edges, body_positions, site_positions or None)."""
+    def __init__(self, systems, *, diameter=1.0, precision="fp64", device=0, lds_tiles=True):
+        """systems: list of (box_edges, body_positions, site_positions or None).
+        lds_tiles=False runs every pass a lane per space from global memory
+        instead of on LDS tiles (same decisions)."""
         self.R = len(systems)
         cfgs = (Config * self.R)()
         self.dims = []
@@ -345,6 +348,7 @@ class GpuBatch:
             c.params[0] = diameter
             c.precision = {"fp64": HMC_FP64, "mixed": HMC_MIXED}[precision]
             c.device = device
+            c.flags = 0 if lds_tiles else FLAG_GLOBAL_PASS
             self.dims.append((pos.shape[0], pos.shape[1]))
         h = C.c_void_p()
         rc = lib().hmc_batch_create(self.R, cfgs, C.byref(h))
@@ -377,6 +381,11 @@ class GpuBatch:
         out = (Count * self.R)()
         self._check(lib().hmc_batch_run_translate(self.h, st, sd, kT, _p(md), n_steps, 1 if use_graph else 0, out))
         return [(int(c.accepted), int(c.rejected)) for c in out]
+
+    def stats(self, r):
+        s = Stats()
+        self._check(lib().hmc_batch_get_stats(self.h, r, C.byref(s)))
+        return {k: int(getattr(s, k)) for k, _ in Stats._fields_ if k != "tile_shape"}
 
     def sdf(self, xmax=0.02, dx=1e-4, nbins=None):
         """(hist[R, nbins], overlaps[R]): hmc_sdf of every replica."""

@@ -879,15 +879,17 @@ __device__ __forceinline__ TrialOut trial_from_global(const Geometry& g, const A
   return o;
 }
 
+// The pass of one tile (the body of k_pass_tile and k_bpass_tile): g and tc
+// are the system's, w.start its spaces' offsets; the block's accepted moves
+// are added to *acc.
 template <int D, int PREC>
-__global__ void __launch_bounds__(kTileThreads) k_pass_tile(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
-                                                            uint32_t colour, uint32_t cycle, TileCfg tc) {
+__device__ __forceinline__ void pass_tile(const Geometry& g, ApplyState* st, const Bodies& B, const Cells& C,
+                                          const Work& w, uint32_t colour, uint32_t cycle, const TileCfg& tc,
+                                          uint32_t tile, unsigned long long* acc) {
   using VT = typename CellCopy<PREC>::type;
   constexpr int AL = D - 1;  // the row axis (stride 1)
   if (cycle >= st->cycles) return;  // uniform over the grid
-  const uint32_t b = blockIdx.x;
-  const uint32_t tile = (b & 7u) * tc.xcd_chunk + (b >> 3);  // XCD-aware tile order
-  if (tile >= tc.ntiles) return;                              // uniform over the block
+  if (tile >= tc.ntiles) return;    // uniform over the block
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t R = tc.rows, EL = tc.E[AL], CW = EL + 1;
   // per row: LDS offset, first run's start slot and length, second run's
@@ -1257,7 +1259,15 @@ __global__ void __launch_bounds__(kTileThreads) k_pass_tile(Geometry g, ApplySta
   lane_add(&st->exp_slow, o.exp_slow);
   if (tid == 0 && overflow) atomicAdd(&st->tile_overflows, 1ull);
   __syncthreads();
-  if (tid == 0 && blk_acc) atomicAdd(&w.acc_slots[b & (kAccSlots - 1)], (unsigned long long)blk_acc);
+  if (tid == 0 && blk_acc) atomicAdd(acc, (unsigned long long)blk_acc);
+}
+
+template <int D, int PREC>
+__global__ void __launch_bounds__(kTileThreads) k_pass_tile(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
+                                                            uint32_t colour, uint32_t cycle, TileCfg tc) {
+  const uint32_t b = blockIdx.x;
+  const uint32_t tile = (b & 7u) * tc.xcd_chunk + (b >> 3);  // XCD-aware tile order
+  pass_tile<D, PREC>(g, st, B, C, w, colour, cycle, tc, tile, &w.acc_slots[b & (kAccSlots - 1)]);
 }
 
 // ------------------------------------------------ persistent apply (mixed) --
@@ -2984,6 +2994,34 @@ __global__ void __launch_bounds__(kBlock) k_bpass(const BatchRep* reps, ApplySta
   lane_add(&st->exp_slow, o.exp_slow);
 }
 
+// One colour of one cycle of every replica on LDS tiles: block b takes tile
+// b' = (b mod 8) chunk + b / 8 of the concatenated tile list (toff: the
+// replicas' first tiles), copies that replica's geometry and tile shape into
+// LDS and runs pass_tile on its spaces.
+template <int D, int PREC>
+__global__ void __launch_bounds__(kTileThreads) k_bpass_tile(const BatchRep* reps, const TileCfg* tcs,
+                                                             const uint32_t* toff, ApplyState* sts, uint32_t R,
+                                                             uint32_t ntiles, uint32_t xcd_chunk, Bodies B,
+                                                             Cells C, Work w, uint32_t colour, uint32_t cycle) {
+  __shared__ Geometry sg;
+  __shared__ TileCfg stc;
+  __shared__ uint32_t srep;
+  const uint32_t b = blockIdx.x;
+  const uint32_t t = (b & 7u) * xcd_chunk + (b >> 3);
+  if (t >= ntiles) return;  // uniform over the block
+  if (threadIdx.x == 0) srep = find_replica(toff, R, t);
+  __syncthreads();
+  const uint32_t r = srep;
+  const uint32_t* gsrc = (const uint32_t*)&reps[r].g;
+  for (uint32_t i = threadIdx.x; i < sizeof(Geometry) / 4; i += blockDim.x) ((uint32_t*)&sg)[i] = gsrc[i];
+  const uint32_t* tsrc = (const uint32_t*)&tcs[r];
+  for (uint32_t i = threadIdx.x; i < sizeof(TileCfg) / 4; i += blockDim.x) ((uint32_t*)&stc)[i] = tsrc[i];
+  __syncthreads();
+  Work wr = w;
+  wr.start = w.start + reps[r].space_off;
+  pass_tile<D, PREC>(sg, sts + r, B, C, wr, colour, cycle, stc, t - toff[r], &w.acc_slots[r]);
+}
+
 // Cycles of one replica beyond the ones launched (one workgroup per replica).
 template <int D, int PREC>
 __global__ void __launch_bounds__(kGuardThreads) k_bguard(const BatchRep* reps, ApplyState* sts, Bodies B, Cells C,
@@ -3545,6 +3583,11 @@ struct hmc_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   uint32_t graph_cycles = 0;
+  // LDS-tiled passes (k_bpass_tile): per-replica tile shapes, first tiles
+  bool tiled = false;
+  TileCfg* dtcs = nullptr;
+  uint32_t* dtoff = nullptr;
+  uint32_t ntiles = 0, tile_threads = 0, tile_lds = 0;
   std::string err;
 };
 
@@ -3580,9 +3623,13 @@ void launch_batch_step(hmc_batch* b, int from_args, int advance, uint32_t launch
   k_place<PREC><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->gtot, b->bodies, b->cells, b->w);
   const uint32_t blocks = grid_for(b->pc_total, kBlock);
   const uint32_t chunk = (blocks + 7) / 8;
+  const uint32_t tchunk = (b->ntiles + 7) / 8;
   for (uint32_t k = 0; k < launched; ++k)
     for (int col = 0; col < (1 << D); ++col) {
-      if (b->fast)
+      if (b->tiled)
+        k_bpass_tile<D, PREC><<<tchunk * 8, b->tile_threads, b->tile_lds, s>>>(
+            b->reps, b->dtcs, b->dtoff, b->sts, b->R, b->ntiles, tchunk, b->bodies, b->cells, b->w, (uint32_t)col, k);
+      else if (b->fast)
         k_bpass<D, PREC, true><<<chunk * 8, kBlock, 0, s>>>(b->reps, b->sts, b->dcoff, b->R, b->pc_total, b->bodies,
                                                             b->cells, b->w, (uint32_t)col, k, chunk);
       else
@@ -4671,6 +4718,65 @@ int hmc_batch_create(uint32_t n_replicas, const hmc_config* cfgs, hmc_batch** ou
     const Geometry& g = b->hreps[r].g;
     b->cycles_guess = std::max<uint32_t>(b->cycles_guess, (g.n_bodies + g.n_spaces - 1) / std::max<uint32_t>(g.n_spaces, 1));
   }
+  // LDS tiles for every replica (else the lane-per-space pass): the largest
+  // tiles that fit (the batch fills the device by itself), one workgroup
+  // size for all, and each replica's FP32 band of the box-relative test
+  if (b->fast && !(cfgs[0].flags & HMC_FLAG_GLOBAL_PASS)) {
+    const size_t vt = b->precision == HMC_MIXED ? sizeof(float4) : sizeof(double4);
+    std::vector<TileCfg> tcs(R);
+    std::vector<uint32_t> toff(R + 1, 0);
+    bool tiled = true;
+    uint32_t threads = 64, lds = 0;
+    for (uint32_t r = 0; r < R && tiled; ++r) {
+      const Geometry& g = b->hreps[r].g;
+      const int D = b->dim;
+      tiled = choose_tiles(g, D, (double)g.n_bodies / (double)g.n_spaces, vt, 1, &tcs[r]);
+      if (tiled && b->precision == HMC_MIXED) {
+        const double sigma = std::sqrt(g.sigma2);
+        double lmax = 0.0, Rb = 0.0;
+        for (int a = 0; a < D; ++a) {
+          lmax = std::max(lmax, g.L[a]);
+          Rb = std::max(Rb, 0.5 * tcs[r].E[a] * g.width[a] + g.width[a]);
+        }
+        // as for a context's tiled pass
+        const double Et = (0x1p-24 * (5.1 * lmax + 2.0 * Rb + 2.0 * sigma) + 0x1p-53 * (1.5 * lmax + 2.0 * sigma)) * 1.01;
+        tiled = contact_band(Et, sigma, g.sigma2, g.edge2, D, &tcs[r].band_lo, &tcs[r].band_hi);
+      }
+      if (tiled) {
+        threads = std::max<uint32_t>(threads, ((tcs[r].trials + 63) / 64) * 64);
+        lds = std::max<uint32_t>(lds, tcs[r].lds_bytes);
+        toff[r + 1] = toff[r] + tcs[r].ntiles;
+      }
+    }
+    if (tiled) {
+      // the row-start walk steps by the batch's workgroup size
+      for (uint32_t r = 0; r < R; ++r) {
+        const int D = b->dim;
+        uint32_t q = threads;
+        for (int a = D - 1; a > 0; --a) {
+          const uint32_t rad = a == D - 1 ? tcs[r].E[a] + 2 : tcs[r].E[a];
+          tcs[r].step[a] = q % rad;
+          q /= rad;
+        }
+        tcs[r].step[0] = q;
+      }
+      const void* fn = b->dim == 2 ? (b->precision == HMC_MIXED ? (const void*)k_bpass_tile<2, HMC_MIXED>
+                                                                 : (const void*)k_bpass_tile<2, HMC_FP64>)
+                                   : (b->precision == HMC_MIXED ? (const void*)k_bpass_tile<3, HMC_MIXED>
+                                                                 : (const void*)k_bpass_tile<3, HMC_FP64>);
+      if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess ||
+          dalloc(&b->dtcs, R) != hipSuccess || dalloc(&b->dtoff, R + 1) != hipSuccess ||
+          hipMemcpy(b->dtcs, tcs.data(), R * sizeof(TileCfg), hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(b->dtoff, toff.data(), (R + 1) * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
+        hmc_batch_destroy(b);
+        return HMC_ERR_DEVICE;
+      }
+      b->tiled = true;
+      b->ntiles = toff[R];
+      b->tile_threads = threads;
+      b->tile_lds = lds;
+    }
+  }
   *out = b;
   return HMC_OK;
 }
@@ -4841,6 +4947,23 @@ int hmc_batch_sdf(hmc_batch* b, double xmax, double dx, uint32_t nbins, uint64_t
   return HMC_OK;
 }
 
+int hmc_batch_get_stats(hmc_batch* b, uint32_t replica, hmc_stats* out) {
+  if (!b || !out || replica >= b->R) return HMC_ERR_INVALID;
+  std::vector<ApplyState> st;
+  std::vector<unsigned long long> acc;
+  const int rc = batch_read(b, st, acc);
+  if (rc) return rc;
+  const ApplyState& s = st[replica];
+  std::memset(out, 0, sizeof *out);
+  out->cycles = s.cycles_total;
+  out->band_retests = s.band_retests;
+  out->exp_slow_path = s.exp_slow;
+  out->guard_cycles = s.guard_cycles;
+  out->tile_overflows = s.tile_overflows;
+  out->tiled = b->tiled ? 1u : 0u;
+  return HMC_OK;
+}
+
 const char* hmc_batch_last_error(const hmc_batch* b) { return b ? b->err.c_str() : "null batch"; }
 
 void hmc_batch_destroy(hmc_batch* b) {
@@ -4853,7 +4976,7 @@ void hmc_batch_destroy(hmc_batch* b) {
                   (void*)b->dmove, (void*)b->bodies.bpos, (void*)b->bodies.spos, (void*)b->cells.body,
                   (void*)b->cells.pos32, (void*)b->cells.pos64, (void*)b->w.cnt, (void*)b->w.start,
                   (void*)b->w.elem_space, (void*)b->w.elem_rank, (void*)b->w.tmp_body, (void*)b->w.block_sums,
-                  (void*)b->w.block_ne, (void*)b->w.acc_slots})
+                  (void*)b->w.block_ne, (void*)b->w.acc_slots, (void*)b->dtcs, (void*)b->dtoff})
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;

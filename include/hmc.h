@@ -243,6 +243,10 @@ int hmc_batch_get_state(hmc_batch* batch, uint32_t replica, double* body_pos, do
  * summed counts.  use_graph = 0 launches every step directly (same results). */
 int hmc_batch_run_translate(hmc_batch* batch, const uint64_t* step0, const uint32_t* seeds, double kT,
                             const double* max_distance, uint32_t n_steps, int use_graph, hmc_count* out);
+/* Statistics of one replica (applies and ambiguous are not kept for
+ * batches: 0); tiled = 1 when the batch's passes run on LDS tiles (set
+ * HMC_FLAG_GLOBAL_PASS in cfgs[0].flags for the lane-per-space pass). */
+int hmc_batch_get_stats(hmc_batch* batch, uint32_t replica, hmc_stats* out);
 /* hmc_sdf of every replica: hist holds n_replicas * nbins counts (replica
  * major), overlaps (may be NULL) n_replicas counts. */
 int hmc_batch_sdf(hmc_batch* batch, double xmax, double dx, uint32_t nbins, uint64_t* hist, uint64_t* overlaps);

@@ -47,16 +47,19 @@ def _oracle_run(L, pos, seed, d, steps):
     return (b, s), tuple(tot), ref.positions()
 
 
+@pytest.mark.parametrize("layout", ["tiles", "lanes"])
 @pytest.mark.parametrize("use_graph", [True, False], ids=["graph", "direct"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_batch_replicas_match_their_own_oracles(hmc, wl, precision, use_graph):
-    systems = wl.batched_replicas(n_replicas=6, k_range=(3, 6), phi_range=(0.30, 0.54))
+def test_batch_replicas_match_their_own_oracles(hmc, wl, precision, use_graph, layout):
+    # 864 .. 2,916 bodies: at least 5 spaces per axis, so the tile layout applies
+    systems = wl.batched_replicas(n_replicas=6, k_range=(6, 9), phi_range=(0.30, 0.54))
     seeds = [42 + r for r in range(6)]
     d = [0.08, 0.1, 0.12, 0.05, 0.08, 0.2]
     steps = 5
     refs = [_oracle_run(L, pos, seeds[r], d[r], steps) for r, (L, pos) in enumerate(systems)]
     batch = hmc.GpuBatch([(L, refs[r][0][0], refs[r][0][1]) for r, (L, _) in enumerate(systems)],
-                         precision=precision)
+                         precision=precision, lds_tiles=layout == "tiles")
+    assert batch.stats(0)["tiled"] == (1 if layout == "tiles" else 0)
     counts = batch.run(0, seeds, 1.0, d, steps, use_graph=use_graph)
     for r in range(6):
         assert counts[r] == refs[r][1], (r, counts[r], refs[r][1])
@@ -95,6 +98,7 @@ def test_batch_config_c5_full(hmc, wl):
     t = time.time()
     # the microstates' stored (wrapped) bodies; sites follow as wrap(body + 0)
     batch = hmc.GpuBatch([(L, O.wrap(L, p), None) for L, p in systems], precision="mixed")
+    assert batch.stats(0)["tiled"] == 1
     counts = batch.run(0, seeds, 1.0, 0.08, 3)
     log(f"C5: {len(systems)} replicas, N total {n_total}: 3 steps ({time.time() - t:.1f}s)")
     for r in (0, 101, 255):
