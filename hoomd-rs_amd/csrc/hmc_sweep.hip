@@ -2280,8 +2280,9 @@ __device__ void trial_lj(const Geometry& g, const ApplyState* st, const Bodies& 
 }
 
 // One colour of one cycle for LJ: one wavefront per space of the colour.
+// (four wavefronts per SIMD: measured faster than three, five spills)
 template <int D, int PREC>
-__global__ void __launch_bounds__(kBlock) k_pass_lj(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
+__global__ void __launch_bounds__(kBlock, 4) k_pass_lj(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
                                                     uint32_t colour, uint32_t cycle, uint32_t xcd_chunk) {
   if (cycle >= st->cycles) return;  // uniform over the grid
   __shared__ double zx[257], zf[257];
@@ -2566,7 +2567,9 @@ __device__ __forceinline__ void stage_verts(const Geometry& g, double* vx) {
 // One colour of one cycle for convex polyhedra: kCvTrials spaces per
 // wavefront.  Each trial is the one of trial_convex (parallel_sweep.rs:234-277
 // with Translate or Rotate<Versor>).
-__global__ void __launch_bounds__(kBlock) k_pass_convex(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
+// (three wavefronts per SIMD: 22% faster on C3 than the two its free
+// register use allows)
+__global__ void __launch_bounds__(kBlock, 3) k_pass_convex(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
                                                         uint32_t colour, uint32_t cycle, uint32_t xcd_chunk) {
   if (cycle >= st->cycles) return;  // uniform over the grid
   __shared__ unsigned blk_acc;
