@@ -2938,32 +2938,36 @@ __global__ void __launch_bounds__(kBlock) k_bbin(const BatchRep* reps, ApplyStat
   }
   const uint32_t gs = reps[r].space_off + s;
   w.elem_space[b] = gs;
-  w.elem_rank[b] = atomicAdd(&w.cnt[gs], 1u);
+  const uint32_t rank = atomicAdd(&w.cnt[gs], 1u);
+  w.elem_rank[b] = rank;
+  // a space's first body makes it non-empty: counted per wavefront when the
+  // wavefront's first bodies all belong to one replica (bodies are in
+  // replica order), else per lane
+  const unsigned long long fm = __ballot(rank == 0);
+  if (fm) {
+    const int lead = __ffsll((long long)fm) - 1;
+    const uint32_t rl = __shfl(r, lead, 64);
+    if (__ballot(rank == 0 && r != rl) == 0ull) {
+      if ((int)__lane_id() == lead) atomicAdd(&sts[rl].nonempty, (unsigned)__popcll(fm));
+    } else if (rank == 0) {
+      atomicAdd(&sts[r].nonempty, 1u);
+    }
+  }
 }
 
-// Non-empty spaces and colour cycles of each replica (one workgroup per replica).
-__global__ void __launch_bounds__(256) k_bnonempty(const BatchRep* reps, ApplyState* sts, const uint32_t* cnt) {
-  __shared__ uint32_t red[256];
-  const uint32_t r = blockIdx.x;
-  const Geometry& g = reps[r].g;
-  const uint32_t s0 = reps[r].space_off;
-  uint32_t ne = 0;
-  for (uint32_t s = threadIdx.x; s < g.n_spaces; s += 256) ne += cnt[s0 + s] ? 1u : 0u;
-  red[threadIdx.x] = ne;
-  __syncthreads();
-  for (int h = 128; h >= 1; h >>= 1) {
-    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    ApplyState* st = sts + r;
-    const uint32_t ne_total = red[0];
-    st->nonempty = ne_total;
-    const uint32_t cycles = ne_total ? (g.n_bodies + ne_total - 1) / ne_total : 0;
-    st->cycles = cycles;
-    st->cycles_total += cycles;
-    st->trials_total += (unsigned long long)cycles * ne_total;
-  }
+// Colour cycles of each replica from its non-empty spaces (counted by k_bbin):
+// ParallelSweep::apply loops until count.total() >= N (parallel_sweep.rs:402),
+// each cycle one trial per non-empty space.
+__global__ void k_bcycles(const BatchRep* reps, ApplyState* sts, uint32_t R) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  ApplyState* st = sts + r;
+  const uint32_t ne_total = st->nonempty;
+  const uint32_t n = reps[r].g.n_bodies;
+  const uint32_t cycles = ne_total ? (n + ne_total - 1) / ne_total : 0;
+  st->cycles = cycles;
+  st->cycles_total += cycles;
+  st->trials_total += (unsigned long long)cycles * ne_total;
 }
 
 // One colour of one cycle of every replica: one lane per space.
@@ -3621,7 +3625,7 @@ void launch_batch_step(hmc_batch* b, int from_args, int advance, uint32_t launch
   k_scan_top<<<1, 1024, 0, s>>>(b->gtot, b->sts + b->R, b->w.block_sums, b->w.block_ne, b->n_scan_blocks, b->w.start,
                                 0);
   k_scan_add<<<grid_for(b->s_total, 256), 256, 0, s>>>(b->w.start, b->w.block_sums, nullptr, b->s_total);
-  k_bnonempty<<<b->R, 256, 0, s>>>(b->reps, b->sts, b->w.cnt);
+  k_bcycles<<<grid_for(b->R, 64), 64, 0, s>>>(b->reps, b->sts, b->R);
   k_scatter<<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->gtot, b->w);
   k_place<PREC><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->gtot, b->bodies, b->cells, b->w);
   const uint32_t blocks = grid_for(b->pc_total, kBlock);
