@@ -188,10 +188,12 @@ def other_workloads(hmc, wl):
     systems = wl.batched_replicas()
     batch = hmc.GpuBatch([(L, wl.wrap(L, p), None) for L, p in systems], precision="mixed")
     seeds = [42 + r for r in range(len(systems))]
+    tune_ms, policy = batch.autotune(0, seeds, 1.0, 0.08, 2)  # tile layout (state left unchanged)
     v, t = timed(lambda s0, k: sum(a + r for a, r in batch.run(s0, seeds, 1.0, 0.08, k)), 2, 40)
     res["C5_replicas_256"] = {"value": v, "unit": "trial moves/s", "steps": 40, "precision": "mixed",
                               "bodies": int(sum(len(p) for _, p in systems)),
-                              "pass_layout": "tiles" if batch.stats(0)["tiled"] else "lanes"}
+                              "pass_layout": "tiles" if batch.stats(0)["tiled"] else "lanes",
+                              "autotuned_policy_cus": policy}
     # SDF of every replica (x_max 0.02, dx 1e-4): one call after the run,
     # then timed calls; beta P / rho from the single frame
     nb = sum(len(p) for _, p in systems)

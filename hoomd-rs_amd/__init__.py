@@ -33,7 +33,7 @@ EXPORTS = ["hmc_create", "hmc_set_state", "hmc_get_state", "hmc_apply_translate"
            "hmc_trace_enable", "hmc_trace_read", "hmc_get_stats", "hmc_profile_enable", "hmc_profile_read",
            "hmc_debug_math", "hmc_debug_normals", "hmc_debug_convex", "hmc_last_error", "hmc_destroy",
            "hmc_batch_create", "hmc_batch_set_state", "hmc_batch_get_state", "hmc_batch_run_translate",
-           "hmc_batch_last_error", "hmc_batch_destroy", "hmc_sdf", "hmc_batch_sdf", "hmc_batch_get_stats"]
+           "hmc_batch_last_error", "hmc_batch_destroy", "hmc_sdf", "hmc_batch_sdf", "hmc_batch_get_stats", "hmc_batch_autotune"]
 
 
 class HmcError(RuntimeError):
@@ -96,6 +96,8 @@ def lib():
         L.hmc_sdf.argtypes = [vp, C.c_double, C.c_double, C.c_uint32, u64p, u64p]
         L.hmc_batch_sdf.argtypes = [vp, C.c_double, C.c_double, C.c_uint32, u64p, u64p]
         L.hmc_batch_get_stats.argtypes = [vp, C.c_uint32, C.POINTER(Stats)]
+        L.hmc_batch_autotune.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), C.c_double, D_P,
+                                         C.c_uint32, D_P, C.POINTER(C.c_uint32)]
         L.hmc_batch_create.argtypes = [C.c_uint32, C.POINTER(Config), C.POINTER(vp)]
         L.hmc_batch_set_state.argtypes = [vp, C.c_uint32, D_P, D_P]
         L.hmc_batch_get_state.argtypes = [vp, C.c_uint32, D_P, D_P]
@@ -381,6 +383,18 @@ class GpuBatch:
         out = (Count * self.R)()
         self._check(lib().hmc_batch_run_translate(self.h, st, sd, kT, _p(md), n_steps, 1 if use_graph else 0, out))
         return [(int(c.accepted), int(c.rejected)) for c in out]
+
+    def autotune(self, step0, seeds, kT, max_distance, n_steps=3):
+        """Pick the fastest tile layout for these run parameters
+        (hmc_batch_autotune; the batch's state is left as it was).
+        Returns (ms per step, policy)."""
+        st = (C.c_uint64 * self.R)(*[int(x) for x in np.broadcast_to(step0, (self.R,))])
+        sd = (C.c_uint32 * self.R)(*[int(x) for x in np.broadcast_to(seeds, (self.R,))])
+        md = np.ascontiguousarray(np.broadcast_to(np.asarray(max_distance, np.float64), (self.R,)))
+        ms = np.zeros(1)
+        pol = C.c_uint32()
+        self._check(lib().hmc_batch_autotune(self.h, st, sd, kT, _p(md), n_steps, _p(ms), C.byref(pol)))
+        return float(ms[0]), int(pol.value)
 
     def stats(self, r):
         s = Stats()

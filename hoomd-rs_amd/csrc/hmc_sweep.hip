@@ -3598,6 +3598,12 @@ struct hmc_batch {
   std::string err;
 };
 
+// defined with the configuration checks below
+extern "C" {
+static bool contact_band(double E, double sigma, double sigma2, double edge2, int D, float* lo, float* hi);
+static bool choose_tiles(const Geometry& g, int D, double occ, size_t vt, int cus, TileCfg* out);
+}
+
 namespace {
 
 int bfail(hmc_batch* b, int code, const std::string& msg) {
@@ -3666,6 +3672,83 @@ int batch_read(hmc_batch* b, std::vector<ApplyState>& st, std::vector<unsigned l
   BHIPCHK(b, hipStreamSynchronize(b->stream));
   for (uint32_t r = 0; r < b->R; ++r)
     if (st[r].error) return bfail(b, HMC_ERR_COVER, "a body lies outside the checkerboard (replica " + std::to_string(r) + ")");
+  return HMC_OK;
+}
+
+// Tile shapes of every replica for the LDS-tiled batch pass: choose_tiles
+// with `cus` compute units assumed per replica (1: the largest tiles that
+// fit), one workgroup size for all (the row-start walk re-stepped for it)
+// and each replica's FP32 band of the box-relative test.  False when a
+// replica has no tile shape.
+struct BatchTiles {
+  std::vector<TileCfg> tcs;
+  std::vector<uint32_t> toff;
+  uint32_t threads = 64, lds = 0;
+};
+
+bool batch_tiles(const hmc_batch* b, int cus, BatchTiles& bt) {
+  const uint32_t R = b->R;
+  const int D = b->dim;
+  const size_t vt = b->precision == HMC_MIXED ? sizeof(float4) : sizeof(double4);
+  bt.tcs.assign(R, TileCfg{});
+  bt.toff.assign(R + 1, 0);
+  bt.threads = 64;
+  bt.lds = 0;
+  for (uint32_t r = 0; r < R; ++r) {
+    const Geometry& g = b->hreps[r].g;
+    TileCfg& tc = bt.tcs[r];
+    if (!choose_tiles(g, D, (double)g.n_bodies / (double)g.n_spaces, vt, cus, &tc)) return false;
+    if (b->precision == HMC_MIXED) {
+      const double sigma = std::sqrt(g.sigma2);
+      double lmax = 0.0, Rb = 0.0;
+      for (int a = 0; a < D; ++a) {
+        lmax = std::max(lmax, g.L[a]);
+        Rb = std::max(Rb, 0.5 * tc.E[a] * g.width[a] + g.width[a]);
+      }
+      // as for a context's tiled pass
+      const double Et = (0x1p-24 * (5.1 * lmax + 2.0 * Rb + 2.0 * sigma) + 0x1p-53 * (1.5 * lmax + 2.0 * sigma)) * 1.01;
+      if (!contact_band(Et, sigma, g.sigma2, g.edge2, D, &tc.band_lo, &tc.band_hi)) return false;
+    }
+    bt.threads = std::max<uint32_t>(bt.threads, ((tc.trials + 63) / 64) * 64);
+    bt.lds = std::max<uint32_t>(bt.lds, tc.lds_bytes);
+    bt.toff[r + 1] = bt.toff[r] + tc.ntiles;
+  }
+  // the row-start walk steps by the batch's workgroup size
+  for (uint32_t r = 0; r < R; ++r) {
+    uint32_t q = bt.threads;
+    for (int a = D - 1; a > 0; --a) {
+      const uint32_t rad = a == D - 1 ? bt.tcs[r].E[a] + 2 : bt.tcs[r].E[a];
+      bt.tcs[r].step[a] = q % rad;
+      q /= rad;
+    }
+    bt.tcs[r].step[0] = q;
+  }
+  return true;
+}
+
+// Makes bt the batch's tile layout (the captured step graph is dropped).
+int batch_set_tiles(hmc_batch* b, const BatchTiles& bt) {
+  const uint32_t R = b->R;
+  const void* fn = b->dim == 2 ? (b->precision == HMC_MIXED ? (const void*)k_bpass_tile<2, HMC_MIXED>
+                                                             : (const void*)k_bpass_tile<2, HMC_FP64>)
+                               : (b->precision == HMC_MIXED ? (const void*)k_bpass_tile<3, HMC_MIXED>
+                                                             : (const void*)k_bpass_tile<3, HMC_FP64>);
+  if (!b->dtcs && (dalloc(&b->dtcs, R) != hipSuccess || dalloc(&b->dtoff, R + 1) != hipSuccess))
+    return HMC_ERR_DEVICE;
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bt.lds) != hipSuccess ||
+      hipMemcpyAsync(b->dtcs, bt.tcs.data(), R * sizeof(TileCfg), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+      hipMemcpyAsync(b->dtoff, bt.toff.data(), (R + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(b->stream) != hipSuccess)
+    return HMC_ERR_DEVICE;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->tiled = true;
+  b->ntiles = bt.toff[R];
+  b->tile_threads = bt.threads;
+  b->tile_lds = bt.lds;
   return HMC_OK;
 }
 
@@ -4726,62 +4809,13 @@ int hmc_batch_create(uint32_t n_replicas, const hmc_config* cfgs, hmc_batch** ou
     b->cycles_guess = std::max<uint32_t>(b->cycles_guess, (g.n_bodies + g.n_spaces - 1) / std::max<uint32_t>(g.n_spaces, 1));
   }
   // LDS tiles for every replica (else the lane-per-space pass): the largest
-  // tiles that fit (the batch fills the device by itself), one workgroup
-  // size for all, and each replica's FP32 band of the box-relative test
+  // tiles that fit (the batch fills the device by itself); hmc_batch_autotune
+  // may pick another shape later
   if (b->fast && !(cfgs[0].flags & HMC_FLAG_GLOBAL_PASS)) {
-    const size_t vt = b->precision == HMC_MIXED ? sizeof(float4) : sizeof(double4);
-    std::vector<TileCfg> tcs(R);
-    std::vector<uint32_t> toff(R + 1, 0);
-    bool tiled = true;
-    uint32_t threads = 64, lds = 0;
-    for (uint32_t r = 0; r < R && tiled; ++r) {
-      const Geometry& g = b->hreps[r].g;
-      const int D = b->dim;
-      tiled = choose_tiles(g, D, (double)g.n_bodies / (double)g.n_spaces, vt, 1, &tcs[r]);
-      if (tiled && b->precision == HMC_MIXED) {
-        const double sigma = std::sqrt(g.sigma2);
-        double lmax = 0.0, Rb = 0.0;
-        for (int a = 0; a < D; ++a) {
-          lmax = std::max(lmax, g.L[a]);
-          Rb = std::max(Rb, 0.5 * tcs[r].E[a] * g.width[a] + g.width[a]);
-        }
-        // as for a context's tiled pass
-        const double Et = (0x1p-24 * (5.1 * lmax + 2.0 * Rb + 2.0 * sigma) + 0x1p-53 * (1.5 * lmax + 2.0 * sigma)) * 1.01;
-        tiled = contact_band(Et, sigma, g.sigma2, g.edge2, D, &tcs[r].band_lo, &tcs[r].band_hi);
-      }
-      if (tiled) {
-        threads = std::max<uint32_t>(threads, ((tcs[r].trials + 63) / 64) * 64);
-        lds = std::max<uint32_t>(lds, tcs[r].lds_bytes);
-        toff[r + 1] = toff[r] + tcs[r].ntiles;
-      }
-    }
-    if (tiled) {
-      // the row-start walk steps by the batch's workgroup size
-      for (uint32_t r = 0; r < R; ++r) {
-        const int D = b->dim;
-        uint32_t q = threads;
-        for (int a = D - 1; a > 0; --a) {
-          const uint32_t rad = a == D - 1 ? tcs[r].E[a] + 2 : tcs[r].E[a];
-          tcs[r].step[a] = q % rad;
-          q /= rad;
-        }
-        tcs[r].step[0] = q;
-      }
-      const void* fn = b->dim == 2 ? (b->precision == HMC_MIXED ? (const void*)k_bpass_tile<2, HMC_MIXED>
-                                                                 : (const void*)k_bpass_tile<2, HMC_FP64>)
-                                   : (b->precision == HMC_MIXED ? (const void*)k_bpass_tile<3, HMC_MIXED>
-                                                                 : (const void*)k_bpass_tile<3, HMC_FP64>);
-      if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess ||
-          dalloc(&b->dtcs, R) != hipSuccess || dalloc(&b->dtoff, R + 1) != hipSuccess ||
-          hipMemcpy(b->dtcs, tcs.data(), R * sizeof(TileCfg), hipMemcpyHostToDevice) != hipSuccess ||
-          hipMemcpy(b->dtoff, toff.data(), (R + 1) * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
-        hmc_batch_destroy(b);
-        return HMC_ERR_DEVICE;
-      }
-      b->tiled = true;
-      b->ntiles = toff[R];
-      b->tile_threads = threads;
-      b->tile_lds = lds;
+    BatchTiles bt;
+    if (batch_tiles(b, 1, bt) && batch_set_tiles(b, bt) != HMC_OK) {
+      hmc_batch_destroy(b);
+      return HMC_ERR_DEVICE;
     }
   }
   *out = b;
@@ -4951,6 +4985,81 @@ int hmc_batch_sdf(hmc_batch* b, double xmax, double dx, uint32_t nbins, uint64_t
   for (size_t k = 0; k < (size_t)b->R * nbins; ++k) hist[k] = h[k];
   if (overlaps)
     for (uint32_t r = 0; r < b->R; ++r) overlaps[r] = h[(size_t)b->R * nbins + r];
+  return HMC_OK;
+}
+
+int hmc_batch_autotune(hmc_batch* b, const uint64_t* step0, const uint32_t* seeds, double kT,
+                       const double* max_distance, uint32_t n_steps, double* best_ms, uint32_t* best_policy) {
+  if (!b || !step0 || !seeds || !max_distance || !(kT > 0.0) || n_steps == 0) return HMC_ERR_INVALID;
+  for (uint32_t r = 0; r < b->R; ++r)
+    if (!(max_distance[r] > 0.0)) return bfail(b, HMC_ERR_INVALID, "max_distance must be > 0");
+  if (!b->tiled) return bfail(b, HMC_ERR_UNSUPPORTED, "the batch does not run on LDS tiles");
+  BHIPCHK(b, hipSetDevice(b->device));
+  hipStream_t s = b->stream;
+  const uint32_t R = b->R;
+  b->kT = kT;
+  BHIPCHK(b, hipMemcpyAsync(b->dstep, step0, R * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+  BHIPCHK(b, hipMemcpyAsync(b->dseed, seeds, R * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  BHIPCHK(b, hipMemcpyAsync(b->dmove, max_distance, R * sizeof(double), hipMemcpyHostToDevice, s));
+  // the state every trial run starts from and leaves behind
+  const size_t nb = (size_t)b->n_total * sizeof(double4), ns = (R + 1) * sizeof(ApplyState),
+               na = kAccSlots * sizeof(unsigned long long);
+  char* save = nullptr;
+  BHIPCHK(b, hipMalloc((void**)&save, 2 * nb + ns + na));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = HMC_OK;
+  auto copy_state = [&](bool restore) -> bool {
+    void* dst[4] = {save, save + nb, save + 2 * nb, save + 2 * nb + ns};
+    void* src[4] = {b->bodies.bpos, b->bodies.spos, b->sts, b->w.acc_slots};
+    const size_t len[4] = {nb, nb, ns, na};
+    for (int i = 0; i < 4; ++i)
+      if (len[i] && hipMemcpyAsync(restore ? src[i] : dst[i], restore ? dst[i] : src[i], len[i],
+                                   hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return false;
+    return hipStreamSynchronize(s) == hipSuccess;
+  };
+  const int policies[] = {1, 4, 16, 64, 256};
+  double best = std::numeric_limits<double>::infinity();
+  int chosen = -1;
+  BatchTiles best_bt;
+  std::vector<std::vector<uint32_t>> seen;
+  if (!copy_state(false) || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = HMC_ERR_DEVICE;
+  for (int pi = 0; pi < 5 && rc == HMC_OK; ++pi) {
+    BatchTiles bt;
+    if (!batch_tiles(b, policies[pi], bt)) continue;
+    std::vector<uint32_t> sig;
+    for (const TileCfg& tc : bt.tcs) sig.insert(sig.end(), {tc.T[0], tc.T[1], tc.T[2]});
+    if (std::find(seen.begin(), seen.end(), sig) != seen.end()) continue;  // same layout as an earlier policy
+    seen.push_back(sig);
+    if (batch_set_tiles(b, bt) != HMC_OK) {
+      rc = HMC_ERR_DEVICE;
+      break;
+    }
+    // one untimed step from (step0, seeds), then n_steps timed ones
+    batch_step(b, 1, 0, b->cycles_guess);
+    float ms = 0.0f;
+    bool ok = hipEventRecord(e0, s) == hipSuccess;
+    for (uint32_t i = 0; i < n_steps; ++i) batch_step(b, 0, 1, b->cycles_guess);
+    ok = ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+         hipEventElapsedTime(&ms, e0, e1) == hipSuccess && hipGetLastError() == hipSuccess;
+    ok = ok && copy_state(true);
+    if (!ok) {
+      rc = HMC_ERR_DEVICE;
+      break;
+    }
+    if (ms / n_steps < best) {
+      best = ms / n_steps;
+      chosen = policies[pi];
+      best_bt = bt;
+    }
+  }
+  if (rc == HMC_OK && chosen > 0 && batch_set_tiles(b, best_bt) != HMC_OK) rc = HMC_ERR_DEVICE;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(save);
+  if (rc) return bfail(b, rc, "autotune: device error");
+  if (best_ms) *best_ms = best;
+  if (best_policy) *best_policy = chosen > 0 ? (uint32_t)chosen : 0u;
   return HMC_OK;
 }
 

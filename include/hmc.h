@@ -243,6 +243,18 @@ int hmc_batch_get_state(hmc_batch* batch, uint32_t replica, double* body_pos, do
  * summed counts.  use_graph = 0 launches every step directly (same results). */
 int hmc_batch_run_translate(hmc_batch* batch, const uint64_t* step0, const uint32_t* seeds, double kT,
                             const double* max_distance, uint32_t n_steps, int use_graph, hmc_count* out);
+/* Autotuner of the LDS-tiled batch pass (the tile size, i.e. spaces per
+ * workgroup, of every replica).  For tile-size policies from the largest
+ * tiles that fit down to small ones (tile shapes chosen as if each replica
+ * had 1, 4, 16, 64 or 256 compute units to fill), it runs one step and then
+ * n_steps timed steps of every replica from (step0, seeds, kT,
+ * max_distance), restores the batch's positions, counters and counts after
+ * each, and keeps the fastest layout: *best_ms (may be NULL) its time per
+ * step, *best_policy (may be NULL) its compute-unit count.  The batch's
+ * results do not depend on the layout.  HMC_ERR_UNSUPPORTED when the batch
+ * does not run on LDS tiles.  No counterpart in the reference. */
+int hmc_batch_autotune(hmc_batch* batch, const uint64_t* step0, const uint32_t* seeds, double kT,
+                       const double* max_distance, uint32_t n_steps, double* best_ms, uint32_t* best_policy);
 /* Statistics of one replica (applies and ambiguous are not kept for
  * batches: 0); tiled = 1 when the batch's passes run on LDS tiles (set
  * HMC_FLAG_GLOBAL_PASS in cfgs[0].flags for the lane-per-space pass). */

@@ -111,3 +111,26 @@ def test_batch_config_c5_full(hmc, wl):
         _, gs = batch.positions(r)
         assert O.OracleSystem(systems[r][0], gs).total_energy() == 0.0, r
     batch.close()
+
+
+def test_batch_autotune_leaves_state_and_results_unchanged(hmc, wl):
+    # the autotuner times several tile layouts from the current state and
+    # restores it after each; the chosen layout makes the same decisions
+    systems = wl.batched_replicas(n_replicas=12, k_range=(6, 12))
+    seeds = [7 + r for r in range(12)]
+    a = hmc.GpuBatch([(L, p, None) for L, p in systems], precision="mixed")
+    b = hmc.GpuBatch([(L, p, None) for L, p in systems], precision="mixed")
+    ms, policy = a.autotune(0, seeds, 1.0, 0.08, n_steps=2)
+    log(f"autotune: {ms:.3f} ms per step with policy {policy}")
+    assert ms > 0.0 and policy in (1, 4, 16, 64, 256)
+    assert a.stats(0)["tiled"] == 1
+    assert all(a.stats(r)["cycles"] == 0 for r in range(12))
+    for r in range(12):
+        assert np.array_equal(a.positions(r)[0], b.positions(r)[0]), r
+    ca = a.run(0, seeds, 1.0, 0.08, 4)
+    cb = b.run(0, seeds, 1.0, 0.08, 4)
+    assert ca == cb
+    for r in range(12):
+        assert np.array_equal(a.positions(r)[1], b.positions(r)[1]), r
+    a.close()
+    b.close()
