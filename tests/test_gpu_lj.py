@@ -118,6 +118,29 @@ def test_lj_config_c4_full_size(hmc, wl, precision):
         assert st["band_retests"] < 0.01 * 2 * len(pos), st
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lj_more_candidates_than_the_slot_table(hmc, wl, precision):
+    # box 15 with r_cut 2.5: six spaces of width r_cut per axis, so the scan
+    # reaches two spaces out (5^3 spaces, about 1,200 candidates per trial):
+    # more than the LDS slot table holds, so the segment search runs
+    L, pos = wl.fcc_at_volume((8, 8, 8), 15.0**3)
+    st, _ = _replay_lj(hmc, L, pos, "lj_shifted", (1.0, 1.0, 2.5, 2.5), 1.2, 0.15, 4, seed=21, precision=precision)
+    log(f"LJ reach-2 dense N={len(pos)} {precision}: 4 applies bit-identical {st}")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lj_nearly_coincident_pair(hmc, wl, precision):
+    # two sites 4e-4 apart: (sigma / r)^12 overflows FP32 (the mixed path
+    # must re-take those decisions in FP64) while FP64 stays finite
+    L, pos = wl.fcc_number_density((6, 6, 6), 0.8)
+    pos = pos.copy()
+    pos[1] = pos[0] + np.array([4e-4, 0.0, 0.0])
+    st, _ = _replay_lj(hmc, L, pos, "lj_shifted", (1.0, 1.0, 2.5, 2.5), 1.2, 0.1, 6, seed=5, precision=precision)
+    log(f"LJ near-coincident pair {precision}: 6 applies bit-identical {st}")
+    if precision == "mixed":
+        assert st["band_retests"] > 0
+
+
 def test_total_energy_matches_oracle(hmc, wl):
     # PairwiseCutoff::total_energy on the device: within the summation-order
     # bound of the oracle's VecCell-order sum, before and after moves
