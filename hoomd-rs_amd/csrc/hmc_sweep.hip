@@ -1713,19 +1713,15 @@ __global__ void __launch_bounds__(kTileThreads, 2) k_apply_tile(Geometry g, Appl
     pn = coff[pcrow * CW + pcb[AL] + 1] - pe0;
     if (!pn) return;
     pgo = true;
-    // positions of the space's first bodies, in flight during the draw
-    const uint32_t pb0 = __float_as_int(lb[pe0].w);
-    const uint32_t pb1 = pn > 1 ? (uint32_t)__float_as_int(lb[pe0 + 1].w) : pb0;
-    const double4 q0 = B.bpos[pb0];
-    const double4 q1 = B.bpos[pb1];
+    // the choice first (a few draws), then the chosen body's position in
+    // flight during the displacement draw
     Sfc64 rng = counter_rng(step, substep, seed, s);
     ppick = choose_index(rng, pn);
+    pbody = (uint32_t)__float_as_int(lb[pe0 + ppick].w);
+    pbp = B.bpos[pbody];
     unsigned es = 0;
     ball_sample<D>(rng, max_move, zx, zf, pv, &es);
     exp_slow += es;
-    pbody = ppick == 0 ? pb0 : (ppick == 1 ? pb1 : (uint32_t)__float_as_int(lb[pe0 + ppick].w));
-    pbp = ppick == 0 ? q0 : q1;
-    if (ppick > 1) pbp = B.bpos[pbody];
   };
 
   const uint32_t npass = cycles * NCOL;
