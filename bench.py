@@ -164,6 +164,14 @@ def other_workloads(hmc, wl):
     v64, _ = timed(lambda s0, k: sum(sim.run(s0, 42, 1.2, 0.1, k)), 1, 4)
     res["C4_shifted_lj_524288"]["fp64_value"] = v64
     sim.close()
+    # FP32 error of dH against FP64 (every trial of two applies, diagnostic run)
+    sim = hmc.GpuParallelSweep(L, wl.wrap(L, pos), model="lj_shifted", params=(1.0, 1.0, 2.5, 2.5),
+                               precision="mixed", fp32_error=True)
+    sim.run(0, 42, 1.2, 0.1, 2)
+    st = sim.stats()
+    res["C4_shifted_lj_524288"]["fp32_dH_error"] = {"max_abs": st["fp32_error_max"],
+                                                    "max_over_bound": st["fp32_error_bound"]}
+    sim.close()
     # C3: unit cubes, simple cubic 64^3 at packing fraction 0.55; a step is a
     # translate apply (d = 0.05) and a rotate apply (0.1), as the reference's
     # polyhedron benchmark; counts are read back after every apply

@@ -53,6 +53,7 @@ class Config(C.Structure):
 FLAG_GLOBAL_PASS = 1
 FLAG_NO_GRAPH = 2
 FLAG_PER_COLOUR = 4
+FLAG_FP32_ERROR = 8
 
 
 class Count(C.Structure):
@@ -63,7 +64,7 @@ class Stats(C.Structure):
     _fields_ = [("applies", C.c_uint64), ("cycles", C.c_uint64), ("pair_tests", C.c_uint64),
                 ("band_retests", C.c_uint64), ("exp_slow_path", C.c_uint64), ("guard_cycles", C.c_uint64),
                 ("tile_overflows", C.c_uint64), ("tiled", C.c_uint32), ("tile_shape", C.c_uint32 * 3),
-                ("ambiguous", C.c_uint64)]
+                ("ambiguous", C.c_uint64), ("fp32_error_max", C.c_double), ("fp32_error_bound", C.c_double)]
 
 
 TRACE_DTYPE = np.dtype([("substep", "<u4"), ("color", "<u4"), ("space", "<u8"),
@@ -135,7 +136,7 @@ class GpuParallelSweep:
     def __init__(self, box_edges, positions, *, sites=None, diameter=1.0, boundary_range=None,
                  body_interaction_range=None, spatial_cell_width=None, precision="fp64", device=0,
                  lds_tiles=True, model="hard_sphere", params=None, vertices=None, orientations=None,
-                 site_orientations=None, graphs=True, persistent=True):
+                 site_orientations=None, graphs=True, persistent=True, fp32_error=False):
         """positions / sites: the microstate's body and site positions (already
         wrapped into the box, as Microstate::add_body stores them); sites=None
         derives them as wrap(body + 0).  lds_tiles=False runs every trial from
@@ -145,6 +146,9 @@ class GpuParallelSweep:
         colour pass instead of one persistent launch per apply (same results;
         the persistent launch needs cooperative launch support and a region
         shape whose workgroups all fit on the device at once).
+        fp32_error=True (mixed LJ) also forms every trial's dH in FP64 and
+        records the largest FP32 error and its ratio to the error bound
+        (stats()["fp32_error_max"], ["fp32_error_bound"]).
 
         model: "hard_sphere" (params = (diameter,), default (diameter,)),
         "lj" (params = (epsilon, sigma, r_cut)) or "lj_shifted"
@@ -183,7 +187,7 @@ class GpuParallelSweep:
         cfg.precision = {"fp64": HMC_FP64, "mixed": HMC_MIXED}[precision]
         cfg.device = device
         cfg.flags = ((0 if lds_tiles else FLAG_GLOBAL_PASS) | (0 if graphs else FLAG_NO_GRAPH) |
-                     (0 if persistent else FLAG_PER_COLOUR))
+                     (0 if persistent else FLAG_PER_COLOUR) | (FLAG_FP32_ERROR if fp32_error else 0))
         h = C.c_void_p()
         rc = lib().hmc_create(C.byref(cfg), C.byref(h))
         if rc != HMC_OK:
@@ -278,7 +282,8 @@ class GpuParallelSweep:
         self._check(lib().hmc_get_stats(self.h, C.byref(s)))
         out = {k: getattr(s, k) for k, _ in Stats._fields_}
         out["tile_shape"] = tuple(int(v) for v in s.tile_shape)
-        return {k: v if k == "tile_shape" else int(v) for k, v in out.items()}
+        return {k: v if k in ("tile_shape", "fp32_error_max", "fp32_error_bound") else int(v)
+                for k, v in out.items()}
 
     def profile(self, on=True, period=1):
         """Time every period-th apply's launches with HIP events (on=False: off)."""
@@ -399,7 +404,8 @@ class GpuBatch:
     def stats(self, r):
         s = Stats()
         self._check(lib().hmc_batch_get_stats(self.h, r, C.byref(s)))
-        return {k: int(getattr(s, k)) for k, _ in Stats._fields_ if k != "tile_shape"}
+        return {k: getattr(s, k) if k.startswith("fp32") else int(getattr(s, k))
+                for k, _ in Stats._fields_ if k != "tile_shape"}
 
     def sdf(self, xmax=0.02, dx=1e-4, nbins=None):
         """(hist[R, nbins], overlaps[R]): hmc_sdf of every replica."""

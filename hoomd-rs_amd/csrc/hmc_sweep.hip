@@ -79,6 +79,7 @@ struct Geometry {
   double lj_rc2_edge;      // r_cut^2 (1 - 2^-30): at or above it the spatial-structure rule is applied
   double lj_shift;         // E(r_shift) for the shifted potential, else 0
   int lj_mixed;            // LJ pair energies in FP32 with an FP64 re-test inside the error bound
+  int lj_fp32_error;       // mixed LJ: also form every trial's dH in FP64 and record the FP32 error
   const double* verts;     // convex polyhedron: vertices (x, y, z triples), device memory
   uint32_t nverts;
   double cv_rcut2;         // (R + R)^2, R the circumsphere radius (convex.rs:163-166)
@@ -112,6 +113,8 @@ struct ApplyState {
   unsigned long long tile_overflows;
   unsigned long long ambiguous;  // decisions the error bound could not settle
   unsigned long long trace_n;
+  unsigned long long fp32_err_max;    // bits of the largest |dH32 - dH64| (HMC_FLAG_FP32_ERROR)
+  unsigned long long fp32_err_ratio;  // bits of the largest |dH32 - dH64| / bound
 };
 
 struct Bodies {
@@ -2037,6 +2040,7 @@ struct LjOut {
   unsigned ambiguous;
   unsigned exp_slow;
   unsigned retests;  // mixed precision: decisions the FP32 bound left open, re-taken in FP64
+  double err, ratio; // with lj_fp32_error: |dH32 - dH64| and its ratio to the bound
 };
 
 constexpr uint32_t kLjBatch = 4;  // candidate rounds of an LJ trial with their loads in flight together
@@ -2229,6 +2233,11 @@ __device__ void trial_lj(const Geometry& g, const ApplyState* st, const Bodies& 
       const double rounds = (double)((total + 63) / 64);
       const double delta32 = ((48.0 + rounds + 8.0) * 0x1p-24 + 0x1p-40) * (w32.af + w32.ai) * (1.0 + 0x1p-10) +
                              0x1p-1070;
+      if (g.lj_fp32_error && isfinite(dH32) && isfinite(delta32)) {
+        const WaveSums w64 = lj_trial_sums<D, false>(g, C, seg_lo, seg_pre, NS, total, slot, t, xi, tb);
+        o.err = fabs(dH32 - (w64.ef - w64.ei));
+        o.ratio = o.err / delta32;
+      }
       const Sfc64 rng0 = rng;
       bool open = false;
       accept = lj_metropolis(rng, dH32, delta32, st->kT, &open) != 0;
@@ -2290,7 +2299,7 @@ __global__ void __launch_bounds__(kBlock, 4) k_pass_lj(Geometry g, ApplyState* s
   const uint32_t b = blockIdx.x;
   const uint32_t lb = (b & 7u) * xcd_chunk + (b >> 3);
   const uint32_t t = lb * (kBlock / 64) + (threadIdx.x >> 6);  // one space per wavefront
-  LjOut o{0, 0, 0, 0};
+  LjOut o{0, 0, 0, 0, 0.0, 0.0};
   if (t < g.per_colour) {
     int32_t m[3] = {0, 0, 0};
     const uint32_t s = colour_space<D>(g, colour, t, m);
@@ -2301,6 +2310,10 @@ __global__ void __launch_bounds__(kBlock, 4) k_pass_lj(Geometry g, ApplyState* s
   if (o.accepted) atomicAdd(&blk_acc, 1u);
   lane_add(&st->ambiguous, o.ambiguous);
   lane_add(&st->band_retests, o.retests);
+  if (g.lj_fp32_error && __lane_id() == 0 && o.err > 0.0) {  // non-negative doubles order as their bits
+    atomicMax(&st->fp32_err_max, (unsigned long long)__double_as_longlong(o.err));
+    atomicMax(&st->fp32_err_ratio, (unsigned long long)__double_as_longlong(o.ratio));
+  }
   lane_add(&st->exp_slow, o.exp_slow);
   __syncthreads();
   if (threadIdx.x == 0 && blk_acc) atomicAdd(&w.acc_slots[b & (kAccSlots - 1)], (unsigned long long)blk_acc);
@@ -2319,7 +2332,7 @@ __global__ void __launch_bounds__(kGuardThreads) k_guard_lj(Geometry g, ApplySta
   for (uint32_t k = first_cycle; k < cycles; ++k) {
     for (uint32_t c = 0; c < (1u << D); ++c) {
       for (uint32_t t = wid; t < g.per_colour; t += waves) {
-        LjOut o{0, 0, 0, 0};
+        LjOut o{0, 0, 0, 0, 0.0, 0.0};
         int32_t m[3] = {0, 0, 0};
         const uint32_t s = colour_space<D>(g, c, t, m);
         const uint32_t s0 = w.start[s];
@@ -4079,6 +4092,7 @@ int hmc_create(const hmc_config* cfg_in, hmc_ctx** out) {
   c->device = cfg->device;
   c->g = g0;
   c->g.lj_mixed = lj_mixed ? 1 : 0;
+  c->g.lj_fp32_error = lj_mixed && (cfg_in->flags & HMC_FLAG_FP32_ERROR) ? 1 : 0;
   c->fast = fast0;
   Geometry& g = c->g;
   if (hipSetDevice(c->device) != hipSuccess) {
@@ -4632,6 +4646,8 @@ int hmc_get_stats(hmc_ctx* c, hmc_stats* out) {
   out->guard_cycles = s.guard_cycles;
   out->tile_overflows = s.tile_overflows;
   out->ambiguous = s.ambiguous;
+  std::memcpy(&out->fp32_error_max, &s.fp32_err_max, sizeof(double));
+  std::memcpy(&out->fp32_error_bound, &s.fp32_err_ratio, sizeof(double));
   out->tiled = c->persist ? 2u : (c->tiled ? 1u : 0u);
   for (int a = 0; a < 3; ++a) out->tile_shape[a] = c->persist ? c->pc.T[a] : (c->tiled ? c->tc.T[a] : 0u);
   return HMC_OK;

@@ -92,9 +92,13 @@ enum {
   HMC_FLAG_NO_GRAPH = 2,    /* hmc_run_translate launches every apply directly
                                instead of replaying a captured hipGraph
                                (same results) */
-  HMC_FLAG_PER_COLOUR = 4   /* MIXED hard shapes: one launch per colour pass
+  HMC_FLAG_PER_COLOUR = 4,  /* MIXED hard shapes: one launch per colour pass
                                instead of the default single persistent launch
                                per apply (same results) */
+  HMC_FLAG_FP32_ERROR = 8   /* MIXED LJ: also evaluate every trial's dH in
+                               FP64 and record the largest FP32 error and its
+                               ratio to the error bound (hmc_stats; same
+                               decisions, slower) */
 };
 
 /* One trial of a trace: replaces the per-space BodyTrial status of
@@ -126,6 +130,9 @@ typedef struct {
                                summation-order difference could not settle
                                (decided from the GPU's own sum; 0 means every
                                decision provably equals the reference's) */
+  double fp32_error_max;    /* HMC_FLAG_FP32_ERROR: largest |dH32 - dH64| of an LJ trial */
+  double fp32_error_bound;  /* HMC_FLAG_FP32_ERROR: largest |dH32 - dH64| / bound
+                               (at most 1 when the bound holds) */
 } hmc_stats;
 
 /* Allocate device state for one system.  Replaces building a ParallelSweep and

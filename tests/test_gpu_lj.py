@@ -141,6 +141,29 @@ def test_lj_nearly_coincident_pair(hmc, wl, precision):
         assert st["band_retests"] > 0
 
 
+def test_lj_fp32_error_stays_within_its_bound(hmc, wl):
+    # with the diagnostic on, every trial also forms dH in FP64: the largest
+    # FP32 error must lie within the rigorous bound (ratio <= 1), and the
+    # decisions stay bit-identical to the oracle's
+    L, pos = wl.fcc_number_density((6, 6, 6), 0.8)
+    ref = O.OracleSystem(L, pos, model="lj_shifted", params=(1.0, 1.0, 2.5, 2.5), seed=42)
+    b, s = ref.positions()
+    gpu = hmc.GpuParallelSweep(L, b, sites=s, model="lj_shifted", params=(1.0, 1.0, 2.5, 2.5),
+                               precision="mixed", fp32_error=True)
+    for _ in range(6):
+        step, sub = ref.counter()
+        ga, gr, used = gpu.apply(step, sub, 42, 1.2, 0.1)
+        oa, orj = ref.parallel_apply(1.2, 0.1)
+        assert (ga, gr) == (oa, orj)
+        ref.increment_step()
+    st = gpu.stats()
+    log(f"LJ FP32 dH error: max {st['fp32_error_max']:.3g}, max error / bound {st['fp32_error_bound']:.3g}")
+    assert 0.0 < st["fp32_error_max"] and 0.0 < st["fp32_error_bound"] <= 1.0
+    gb, _ = gpu.positions()
+    assert np.array_equal(gb, ref.positions()[0])
+    gpu.close()
+
+
 def test_total_energy_matches_oracle(hmc, wl):
     # PairwiseCutoff::total_energy on the device: within the summation-order
     # bound of the oracle's VecCell-order sum, before and after moves
