@@ -1303,9 +1303,6 @@ constexpr int kStageRows = 4;                 // rows per half-wave staged toget
 __device__ __forceinline__ uint64_t ld_dev(const void* p) {
   return __hip_atomic_load((uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_dev(void* p, uint64_t v) {
-  __hip_atomic_store((uint64_t*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ __forceinline__ double4 ld_dev_d4(const double4* p) {
   const uint64_t* q = (const uint64_t*)p;
   return make_double4(__longlong_as_double((long long)ld_dev(q)), __longlong_as_double((long long)ld_dev(q + 1)),
