@@ -3008,23 +3008,21 @@ __global__ void __launch_bounds__(kBlock) k_bpass(const BatchRep* reps, ApplySta
 }
 
 // One colour of one cycle of every replica on LDS tiles: block b takes tile
-// b' = (b mod 8) chunk + b / 8 of the concatenated tile list (toff: the
-// replicas' first tiles), copies that replica's geometry and tile shape into
-// LDS and runs pass_tile on its spaces.
+// b' = (b mod 8) chunk + b / 8 of the concatenated tile list (trep: each
+// tile's replica, toff: the replicas' first tiles), copies that replica's
+// geometry and tile shape into LDS and runs pass_tile on its spaces.
 template <int D, int PREC>
 __global__ void __launch_bounds__(kTileThreads) k_bpass_tile(const BatchRep* reps, const TileCfg* tcs,
-                                                             const uint32_t* toff, ApplyState* sts, uint32_t R,
-                                                             uint32_t ntiles, uint32_t xcd_chunk, Bodies B,
-                                                             Cells C, Work w, uint32_t colour, uint32_t cycle) {
+                                                             const uint32_t* toff, const uint32_t* trep,
+                                                             ApplyState* sts, uint32_t ntiles, uint32_t xcd_chunk,
+                                                             Bodies B, Cells C, Work w, uint32_t colour,
+                                                             uint32_t cycle) {
   __shared__ Geometry sg;
   __shared__ TileCfg stc;
-  __shared__ uint32_t srep;
   const uint32_t b = blockIdx.x;
   const uint32_t t = (b & 7u) * xcd_chunk + (b >> 3);
   if (t >= ntiles) return;  // uniform over the block
-  if (threadIdx.x == 0) srep = find_replica(toff, R, t);
-  __syncthreads();
-  const uint32_t r = srep;
+  const uint32_t r = trep[t];
   const uint32_t* gsrc = (const uint32_t*)&reps[r].g;
   for (uint32_t i = threadIdx.x; i < sizeof(Geometry) / 4; i += blockDim.x) ((uint32_t*)&sg)[i] = gsrc[i];
   const uint32_t* tsrc = (const uint32_t*)&tcs[r];
@@ -3600,6 +3598,8 @@ struct hmc_batch {
   bool tiled = false;
   TileCfg* dtcs = nullptr;
   uint32_t* dtoff = nullptr;
+  uint32_t* dtrep = nullptr;  // replica of each tile
+  uint32_t dtrep_cap = 0;
   uint32_t ntiles = 0, tile_threads = 0, tile_lds = 0;
   std::string err;
 };
@@ -3647,7 +3647,8 @@ void launch_batch_step(hmc_batch* b, int from_args, int advance, uint32_t launch
     for (int col = 0; col < (1 << D); ++col) {
       if (b->tiled)
         k_bpass_tile<D, PREC><<<tchunk * 8, b->tile_threads, b->tile_lds, s>>>(
-            b->reps, b->dtcs, b->dtoff, b->sts, b->R, b->ntiles, tchunk, b->bodies, b->cells, b->w, (uint32_t)col, k);
+            b->reps, b->dtcs, b->dtoff, b->dtrep, b->sts, b->ntiles, tchunk, b->bodies, b->cells, b->w,
+            (uint32_t)col, k);
       else if (b->fast)
         k_bpass<D, PREC, true><<<chunk * 8, kBlock, 0, s>>>(b->reps, b->sts, b->dcoff, b->R, b->pc_total, b->bodies,
                                                             b->cells, b->w, (uint32_t)col, k, chunk);
@@ -3740,6 +3741,21 @@ int batch_set_tiles(hmc_batch* b, const BatchTiles& bt) {
                                : (b->precision == HMC_MIXED ? (const void*)k_bpass_tile<3, HMC_MIXED>
                                                              : (const void*)k_bpass_tile<3, HMC_FP64>);
   if (!b->dtcs && (dalloc(&b->dtcs, R) != hipSuccess || dalloc(&b->dtoff, R + 1) != hipSuccess))
+    return HMC_ERR_DEVICE;
+  const uint32_t nt = bt.toff[R];
+  std::vector<uint32_t> trep(nt);
+  for (uint32_t r = 0; r < R; ++r)
+    for (uint32_t t = bt.toff[r]; t < bt.toff[r + 1]; ++t) trep[t] = r;
+  if (nt > b->dtrep_cap) {
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return HMC_ERR_DEVICE;
+    if (b->dtrep) (void)hipFree(b->dtrep);
+    b->dtrep = nullptr;
+    b->dtrep_cap = 0;
+    if (dalloc(&b->dtrep, nt) != hipSuccess) return HMC_ERR_DEVICE;
+    b->dtrep_cap = nt;
+  }
+  if (nt && hipMemcpyAsync(b->dtrep, trep.data(), nt * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream) !=
+                hipSuccess)
     return HMC_ERR_DEVICE;
   if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bt.lds) != hipSuccess ||
       hipMemcpyAsync(b->dtcs, bt.tcs.data(), R * sizeof(TileCfg), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
@@ -5101,7 +5117,7 @@ void hmc_batch_destroy(hmc_batch* b) {
                   (void*)b->dmove, (void*)b->bodies.bpos, (void*)b->bodies.spos, (void*)b->cells.body,
                   (void*)b->cells.pos32, (void*)b->cells.pos64, (void*)b->w.cnt, (void*)b->w.start,
                   (void*)b->w.elem_space, (void*)b->w.elem_rank, (void*)b->w.tmp_body, (void*)b->w.block_sums,
-                  (void*)b->w.block_ne, (void*)b->w.acc_slots, (void*)b->dtcs, (void*)b->dtoff})
+                  (void*)b->w.block_ne, (void*)b->w.acc_slots, (void*)b->dtcs, (void*)b->dtoff, (void*)b->dtrep})
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
