@@ -46,6 +46,8 @@ namespace hmc {
 
 constexpr int kBlock = 256;
 constexpr int kScanBlock = 1024;  // elements per scan block (256 threads x 4)
+constexpr uint32_t kBarrierSpins = 1u << 22;  // bound of a device-side wait (about 0.2 s of sleeps)
+constexpr uint32_t kErrBarrier = 2u;          // ApplyState::error: a device-side wait timed out
 constexpr int kGuardThreads = 1024;
 constexpr int kAccSlots = 1024;
 constexpr int kMaxVertices = 32;  // ConvexPolyhedron = ConvexPolytope<3, 32>
@@ -144,6 +146,7 @@ struct Work {
   uint64_t trace_cap;
   int tracing;
   uint32_t* mbox;  // persistent apply: per region two pass counters (by parity), kMailboxStride words apart
+  unsigned long long* scan_status;  // single-pass scan: per block its tagged aggregate or inclusive prefix
 };
 
 // ----------------------------------------------------------- small helpers --
@@ -352,6 +355,106 @@ __global__ void __launch_bounds__(1024) k_scan_top(Geometry g, ApplyState* st, u
     st->cycles = cycles;
     st->cycles_total += cycles;
     st->trials_total += (unsigned long long)cycles * ne_total;
+  }
+}
+
+// Single-pass scan of the per-space counts (decoupled look-back): block b
+// scans its kScanBlock counts, publishes its aggregate, adds its
+// predecessors' aggregates back to the first inclusive prefix and publishes
+// its own.  Status word: bits 56-63 the apply's epoch (older words read as
+// not ready), 54-55 the state (1 aggregate, 2 inclusive prefix), 27-53 the
+// count sum, 0-26 the non-empty spaces (so n and the body count stay below
+// 2^27).  Also clears the counts for the next build and, in the last block,
+// forms the totals and the colour cycles as k_scan_top does.  Every
+// predecessor of a waiting block was dispatched before it and never waits
+// on a later block, so the waits end; a bounded spin reports
+// kErrBarrier all the same.
+constexpr uint32_t kLookbackBits = 27;
+__global__ void __launch_bounds__(256) k_scan_lookback(Geometry g, ApplyState* st, uint32_t* cnt, uint32_t* start,
+                                                       unsigned long long* status, uint32_t n, uint32_t nblocks) {
+  __shared__ uint32_t lds[256];
+  __shared__ uint32_t pre[2];
+  const uint32_t b = blockIdx.x;
+  const unsigned long long tag = (unsigned long long)(st->epoch & 0xffu) << 56;
+  const unsigned long long field = (1ull << kLookbackBits) - 1ull;
+  const uint32_t base = b * kScanBlock + threadIdx.x * 4;
+  uint32_t v[4];
+  uint32_t sum = 0, ne = 0;
+  for (int k = 0; k < 4; ++k) {
+    v[k] = (base + k < n) ? cnt[base + k] : 0;
+    sum += v[k];
+    ne += v[k] ? 1u : 0u;
+  }
+  uint32_t total, ne_total;
+  const uint32_t off = block_exclusive_scan(sum, lds, &total);
+  block_exclusive_scan(ne, lds, &ne_total);
+  if (threadIdx.x == 0 && b > 0)
+    __hip_atomic_store(&status[b], tag | (1ull << 54) | ((unsigned long long)total << kLookbackBits) | ne_total,
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x < 64) {
+    // the first wavefront looks back over 64 predecessors at a time: lane l
+    // reads block hi - l; the window's words up to the nearest inclusive
+    // prefix are summed once they are all ready
+    const uint32_t lane = threadIdx.x;
+    uint32_t ps = 0, pn = 0, spins = 0;
+    int32_t hi = (int32_t)b - 1;
+    while (hi >= 0) {
+      const int32_t j = hi - (int32_t)lane;
+      unsigned long long w = tag | (2ull << 54);  // before block 0: an inclusive prefix of 0
+      if (j >= 0) w = __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t state = (uint32_t)(w >> 54) & 3u;
+      const bool ready = (w & 0xff00000000000000ull) == tag && state != 0u;
+      const unsigned long long incl = __ballot(ready && state == 2u);
+      const unsigned long long notready = __ballot(!ready);
+      const uint32_t first = incl ? (uint32_t)__ffsll((long long)incl) - 1u : 64u;  // nearest inclusive
+      const unsigned long long need = first >= 63u ? ~0ull : ((2ull << first) - 1ull);  // lanes 0..first
+      if (notready & need) {
+        if (++spins > kBarrierSpins) {
+          if (lane == 0) atomicOr(&st->error, kErrBarrier);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        continue;  // read the same window again
+      }
+      uint32_t a = lane <= first ? (uint32_t)((w >> kLookbackBits) & field) : 0u;
+      uint32_t c = lane <= first ? (uint32_t)(w & field) : 0u;
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1) {
+        a += __shfl_xor(a, d, 64);
+        c += __shfl_xor(c, d, 64);
+      }
+      ps += a;
+      pn += c;
+      if (first < 64u) break;
+      hi -= 64;
+    }
+    if (lane == 0) {
+      __hip_atomic_store(&status[b],
+                         tag | (2ull << 54) | ((unsigned long long)(ps + total) << kLookbackBits) | (pn + ne_total),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      pre[0] = ps;
+      pre[1] = pn;
+    }
+  }
+  __syncthreads();
+  uint32_t o = pre[0] + off;
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < n) {
+      start[base + k] = o;
+      cnt[base + k] = 0;  // the next build counts from zero
+    }
+    o += v[k];
+  }
+  if (b == nblocks - 1 && threadIdx.x == 0) {
+    const uint32_t all = pre[0] + total, ne_all = pre[1] + ne_total;
+    start[n] = all;
+    st->nonempty = ne_all;
+    // ParallelSweep::apply loops colour cycles until count.total() >= N
+    // (parallel_sweep.rs:402); each cycle counts one trial per non-empty space.
+    const uint32_t cycles = ne_all ? (g.n_bodies + ne_all - 1) / ne_all : 0;
+    st->cycles = cycles;
+    st->cycles_total += cycles;
+    st->trials_total += (unsigned long long)cycles * ne_all;
   }
 }
 
@@ -1291,8 +1394,6 @@ __global__ void __launch_bounds__(kTileThreads) k_pass_tile(Geometry g, ApplySta
 //
 // An LDS entry holds the site relative to the box centre in FP32 (NaN for a
 // site without images, as in the per-colour kernel) with the body index in w.
-constexpr uint32_t kBarrierSpins = 1u << 22;  // about 0.2 s of 128-cycle sleeps
-constexpr uint32_t kErrBarrier = 2u;          // ApplyState::error: a wait for neighbour regions timed out
 constexpr uint32_t kMailboxStride = 64;       // words between region mailboxes (256 B: separate channels)
 constexpr int kRefreshHeld = 2;               // halo entries per thread loaded together in a refresh
 constexpr int kStageRows = 4;                 // rows per half-wave staged together by the persistent apply
@@ -3422,9 +3523,15 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
   k_setup<<<1, 64, 0, s>>>(g, c->st, step, substep, seed, kT, max_move, from_args, advance, move);
   // (the per-space counts are zero: cleared at creation and by k_scan_add)
   if (N) k_bin<D><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
-  k_scan_blocks<<<c->n_scan_blocks, 256, 0, s>>>(c->w.cnt, c->w.start, c->w.block_sums, c->w.block_ne, g.n_spaces);
-  k_scan_top<<<1, 1024, 0, s>>>(g, c->st, c->w.block_sums, c->w.block_ne, c->n_scan_blocks, c->w.start, 1);
-  k_scan_add<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.start, c->w.block_sums, c->w.cnt, g.n_spaces);
+  if (N < (1u << kLookbackBits) && g.n_spaces < (1u << kLookbackBits)) {
+    k_scan_lookback<<<c->n_scan_blocks, 256, 0, s>>>(g, c->st, c->w.cnt, c->w.start, c->w.scan_status, g.n_spaces,
+                                                     c->n_scan_blocks);
+  } else {
+    k_scan_blocks<<<c->n_scan_blocks, 256, 0, s>>>(c->w.cnt, c->w.start, c->w.block_sums, c->w.block_ne,
+                                                   g.n_spaces);
+    k_scan_top<<<1, 1024, 0, s>>>(g, c->st, c->w.block_sums, c->w.block_ne, c->n_scan_blocks, c->w.start, 1);
+    k_scan_add<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.start, c->w.block_sums, c->w.cnt, g.n_spaces);
+  }
   if (N) {
     k_scatter<<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->w);
     // the persistent apply stages its sites from the FP64 copies
@@ -4190,7 +4297,7 @@ int hmc_create(const hmc_config* cfg_in, hmc_ctx** out) {
        dalloc(&c->w.elem_space, N) == hipSuccess && dalloc(&c->w.elem_rank, N) == hipSuccess &&
        dalloc(&c->w.tmp_body, N) == hipSuccess && dalloc(&c->w.block_sums, c->n_scan_blocks) == hipSuccess &&
        dalloc(&c->w.block_ne, c->n_scan_blocks) == hipSuccess && dalloc(&c->st, 1) == hipSuccess &&
-       dalloc(&c->w.acc_slots, kAccSlots) == hipSuccess;
+       dalloc(&c->w.acc_slots, kAccSlots) == hipSuccess && dalloc(&c->w.scan_status, c->n_scan_blocks) == hipSuccess;
   if (ok && convex) {
     // orientations start at the identity; vertices live on the device
     ok = dalloc(&c->bodies.bquat, N) == hipSuccess && dalloc(&c->bodies.squat, N) == hipSuccess &&
@@ -4207,6 +4314,7 @@ int hmc_create(const hmc_config* cfg_in, hmc_ctx** out) {
     return HMC_ERR_DEVICE;
   }
   if (hipMemset(c->st, 0, sizeof(ApplyState)) != hipSuccess ||
+      hipMemset(c->w.scan_status, 0, c->n_scan_blocks * sizeof(unsigned long long)) != hipSuccess ||
       hipMemset(c->w.acc_slots, 0, kAccSlots * sizeof(unsigned long long)) != hipSuccess ||
       hipMemset(c->w.cnt, 0, (size_t)std::max<uint32_t>(g.n_spaces, 1) * sizeof(uint32_t)) != hipSuccess) {
     hmc_destroy(c);
@@ -4735,7 +4843,8 @@ void hmc_destroy(hmc_ctx* c) {
                   (void*)c->gdev,
                   (void*)c->cells.pos64, (void*)c->w.cnt, (void*)c->w.start, (void*)c->w.elem_space,
                   (void*)c->w.elem_rank, (void*)c->w.tmp_body, (void*)c->w.block_sums, (void*)c->w.block_ne,
-                  (void*)c->w.acc_slots, (void*)c->w.trace, (void*)c->w.mbox, (void*)c->st})
+                  (void*)c->w.acc_slots, (void*)c->w.trace, (void*)c->w.mbox, (void*)c->w.scan_status,
+                  (void*)c->st})
     if (p) (void)hipFree(p);
   collect_events(c, c->pass_events, c->pass_ms, c->pass_n);
   collect_events(c, c->build_events, c->build_ms, c->build_n);
