@@ -223,9 +223,9 @@ def other_workloads(hmc, wl):
     return res
 
 
-def cpu_baseline(L, pos, seconds, seed, kT, d):
+def cpu_baseline(L, pos, seconds, seed, kT, d, threads=None):
     import _oracle as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    threads = threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
     O.set_threads(threads)
     sys_ = O.OracleSystem(L, pos, seed=seed)
     sys_.time_applies(True, kT, d, 0.5)  # warm caches
@@ -310,6 +310,8 @@ def main():
         cb = cpu_baseline(L, pos, args.cpu_seconds, args.seed, kT, d)
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = value / cb["value"]
+        # the same restatement on one core (a shorter sample)
+        out["cpu_baseline_1core"] = cpu_baseline(L, pos, args.cpu_seconds / 3.0, args.seed, kT, d, threads=1)
     print(json.dumps(out), flush=True)
 
 
