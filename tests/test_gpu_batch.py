@@ -134,3 +134,14 @@ def test_batch_autotune_leaves_state_and_results_unchanged(hmc, wl):
         assert np.array_equal(a.positions(r)[1], b.positions(r)[1]), r
     a.close()
     b.close()
+
+
+def test_batch_autotune_needs_the_tile_layout(hmc, wl):
+    # a batch on the lane-per-space pass has no tile sizes to tune
+    systems = wl.batched_replicas(n_replicas=3, k_range=(6, 7))
+    b = hmc.GpuBatch([(L, p, None) for L, p in systems], lds_tiles=False)
+    assert b.stats(0)["tiled"] == 0
+    with pytest.raises(hmc.HmcError) as e:
+        b.autotune(0, [1, 2, 3], 1.0, 0.08)
+    assert e.value.code == hmc.HMC_ERR_UNSUPPORTED
+    b.close()
