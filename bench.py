@@ -291,6 +291,14 @@ def main():
                            "algorithmic_bytes_per_launch": b, "launches": mixed["pass_n"],
                            "build_ms_per_apply": mixed["build_ms"] / max(mixed["build_n"], 1)}
     out["sdf"] = mixed["sdf"]  # the scale distribution function of the final state (hmc_sdf), timed apart
+    # FP32 error of the pair test against FP64: every pair within 1.01 sigma
+    # of four applies measured both ways (a diagnostic run on the per-colour
+    # tile passes, not timed)
+    diag = hmc.GpuParallelSweep(L, body, sites=site, precision="mixed", fp32_error=True)
+    diag.run(0, args.seed, kT, d, 4)
+    dst = diag.stats()
+    out["fp32_d2_error"] = {"max_abs": dst["fp32_error_max"], "max_over_band_bound": dst["fp32_error_bound"]}
+    diag.close()
     out["band_retests"] = mixed["stats"]["band_retests"]  # FP64 re-tests over warm-up + timed applies
     out["exp_slow_path"] = mixed["stats"]["exp_slow_path"]
     if not args.no_fp64:

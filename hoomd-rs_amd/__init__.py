@@ -146,9 +146,11 @@ class GpuParallelSweep:
         colour pass instead of one persistent launch per apply (same results;
         the persistent launch needs cooperative launch support and a region
         shape whose workgroups all fit on the device at once).
-        fp32_error=True (mixed LJ) also forms every trial's dH in FP64 and
-        records the largest FP32 error and its ratio to the error bound
-        (stats()["fp32_error_max"], ["fp32_error_bound"]).
+        fp32_error=True (mixed precision) also measures in FP64 -- LJ: every
+        trial's dH; hard spheres: every pair within 1.01 sigma, on the
+        per-colour tile passes -- and records the largest FP32 error and its
+        ratio to the error bound (stats()["fp32_error_max"],
+        ["fp32_error_bound"]).
 
         model: "hard_sphere" (params = (diameter,), default (diameter,)),
         "lj" (params = (epsilon, sigma, r_cut)) or "lj_shifted"

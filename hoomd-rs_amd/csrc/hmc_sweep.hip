@@ -81,7 +81,8 @@ struct Geometry {
   double lj_rc2_edge;      // r_cut^2 (1 - 2^-30): at or above it the spatial-structure rule is applied
   double lj_shift;         // E(r_shift) for the shifted potential, else 0
   int lj_mixed;            // LJ pair energies in FP32 with an FP64 re-test inside the error bound
-  int lj_fp32_error;       // mixed LJ: also form every trial's dH in FP64 and record the FP32 error
+  int lj_fp32_error;       // mixed: also measure in FP64 (LJ: every trial's dH; hard spheres: near-contact
+                           // pairs of the per-colour tile pass) and record the largest FP32 error
   const double* verts;     // convex polyhedron: vertices (x, y, z triples), device memory
   uint32_t nverts;
   double cv_rcut2;         // (R + R)^2, R the circumsphere radius (convex.rs:163-166)
@@ -594,6 +595,26 @@ __device__ __forceinline__ bool overlap64(const Geometry& g, const double* t, do
   return overlap64_edge<D>(g, t, sj);
 }
 
+// The squared distance of overlap64 (the image its test uses), for the FP32
+// error report.
+template <int D>
+__device__ __forceinline__ double dist2_64(const Geometry& g, const double* t, double4 sj) {
+  const uint32_t fl = (uint32_t)__double_as_longlong(sj.w);
+  const double xj[3] = {sj.x, sj.y, sj.z};
+  double d2 = 0.0;
+  for (int a = 0; a < D; ++a) {
+    double d = t[a] - xj[a];
+    const uint32_t f = (fl >> (2 * a)) & 3u;
+    if (f) {
+      const double gx = xj[a] + (f == 1u ? g.L[a] : -g.L[a]);
+      const double dg = t[a] - gx;
+      if (fabs(dg) < fabs(d)) d = dg;
+    }
+    d2 = d2 + d * d;
+  }
+  return d2;
+}
+
 // FP32 filter: 1 overlap, 0 no overlap, 2 inside the error band.
 template <int D>
 __device__ __forceinline__ int overlap32(const Geometry& g, const float* t, float4 sj) {
@@ -987,8 +1008,10 @@ __device__ __forceinline__ TrialOut trial_from_global(const Geometry& g, const A
 
 // The pass of one tile (the body of k_pass_tile and k_bpass_tile): g and tc
 // are the system's, w.start its spaces' offsets; the block's accepted moves
-// are added to *acc.
-template <int D, int PREC>
+// are added to *acc.  FP32_ERROR (mixed): every pair within 1.01 sigma is
+// also measured in FP64 and the largest |d2_32 - d2_64| and its ratio to the
+// band's error bound are recorded in the state.
+template <int D, int PREC, bool FP32_ERROR = false>
 __device__ __forceinline__ void pass_tile(const Geometry& g, ApplyState* st, const Bodies& B, const Cells& C,
                                           const Work& w, uint32_t colour, uint32_t cycle, const TileCfg& tc,
                                           uint32_t tile, unsigned long long* acc) {
@@ -1255,6 +1278,7 @@ __device__ __forceinline__ void pass_tile(const Geometry& g, ApplyState* st, con
 
   // (C) the trials
   TrialOut o{0, 0, exp_slow};
+  double err_max = 0.0, err_ratio = 0.0;  // FP32_ERROR
   if (go && overflow) {
     o = trial_from_global<D, PREC>(g, st, B, C, w, zx, zf, s, s0, n, colour, substep);
   } else if (go) {
@@ -1306,6 +1330,16 @@ __device__ __forceinline__ void pass_tile(const Geometry& g, ApplyState* st, con
             }
             hit |= valid && d2 < lo;
             if (valid && !(d2 < lo) && !(d2 >= hi)) amb |= 1u << r;
+            if constexpr (FP32_ERROR) {
+              if (valid) {
+                const double e2 = dist2_64<D>(g, t, B.spos[C.body[(uint32_t)__float_as_int(p.w)]]);
+                if (e2 < 1.0201 * g.sigma2) {
+                  const double err = fabs((double)d2 - e2);
+                  err_max = fmax(err_max, err);
+                  err_ratio = fmax(err_ratio, err / (g.sigma2 - (double)lo));
+                }
+              }
+            }
           }
           // FP64 re-test of the band pairs (rare): one call site
           while (amb && !hit) {
@@ -1364,16 +1398,22 @@ __device__ __forceinline__ void pass_tile(const Geometry& g, ApplyState* st, con
   lane_add(&st->band_retests, o.retests);
   lane_add(&st->exp_slow, o.exp_slow);
   if (tid == 0 && overflow) atomicAdd(&st->tile_overflows, 1ull);
+  if constexpr (FP32_ERROR) {
+    if (err_max > 0.0) {  // non-negative doubles order as their bits
+      atomicMax(&st->fp32_err_max, (unsigned long long)__double_as_longlong(err_max));
+      atomicMax(&st->fp32_err_ratio, (unsigned long long)__double_as_longlong(err_ratio));
+    }
+  }
   __syncthreads();
   if (tid == 0 && blk_acc) atomicAdd(acc, (unsigned long long)blk_acc);
 }
 
-template <int D, int PREC>
+template <int D, int PREC, bool FP32_ERROR = false>
 __global__ void __launch_bounds__(kTileThreads) k_pass_tile(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
                                                             uint32_t colour, uint32_t cycle, TileCfg tc) {
   const uint32_t b = blockIdx.x;
   const uint32_t tile = (b & 7u) * tc.xcd_chunk + (b >> 3);  // XCD-aware tile order
-  pass_tile<D, PREC>(g, st, B, C, w, colour, cycle, tc, tile, &w.acc_slots[b & (kAccSlots - 1)]);
+  pass_tile<D, PREC, FP32_ERROR>(g, st, B, C, w, colour, cycle, tc, tile, &w.acc_slots[b & (kAccSlots - 1)]);
 }
 
 // ------------------------------------------------ persistent apply (mixed) --
@@ -3597,6 +3637,9 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
       else if (convex)
         k_pass_convex<<<cv_chunk * 8, kBlock, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k,
                                                       cv_chunk);
+      else if (c->tiled && PREC == HMC_MIXED && g.lj_fp32_error)  // the FP32 error report
+        k_pass_tile<D, PREC, true><<<c->tc.xcd_chunk * 8, tile_threads, c->tc.lds_bytes, s>>>(
+            g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k, c->tc);
       else if (c->tiled)
         k_pass_tile<D, PREC><<<c->tc.xcd_chunk * 8, tile_threads, c->tc.lds_bytes, s>>>(
             g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k, c->tc);
@@ -4212,7 +4255,7 @@ int hmc_create(const hmc_config* cfg_in, hmc_ctx** out) {
   c->device = cfg->device;
   c->g = g0;
   c->g.lj_mixed = lj_mixed ? 1 : 0;
-  c->g.lj_fp32_error = lj_mixed && (cfg_in->flags & HMC_FLAG_FP32_ERROR) ? 1 : 0;
+  c->g.lj_fp32_error = (cfg_in->flags & HMC_FLAG_FP32_ERROR) && cfg_in->precision == HMC_MIXED ? 1 : 0;
   c->fast = fast0;
   Geometry& g = c->g;
   if (hipSetDevice(c->device) != hipSuccess) {
@@ -4244,13 +4287,16 @@ int hmc_create(const hmc_config* cfg_in, hmc_ctx** out) {
         fn = cfg->precision == HMC_MIXED ? (const void*)k_pass_tile<2, HMC_MIXED> : (const void*)k_pass_tile<2, HMC_FP64>;
       else
         fn = cfg->precision == HMC_MIXED ? (const void*)k_pass_tile<3, HMC_MIXED> : (const void*)k_pass_tile<3, HMC_FP64>;
-      if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->tc.lds_bytes) != hipSuccess) {
+      const void* fe = D == 2 ? (const void*)k_pass_tile<2, HMC_MIXED, true> : (const void*)k_pass_tile<3, HMC_MIXED, true>;
+      if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->tc.lds_bytes) != hipSuccess ||
+          hipFuncSetAttribute(fe, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->tc.lds_bytes) != hipSuccess) {
         delete c;
         return HMC_ERR_DEVICE;
       }
     }
     int coop = 0;
-    if (c->tiled && cfg->precision == HMC_MIXED && !(cfg->flags & HMC_FLAG_PER_COLOUR) &&
+    // (the FP32 error report runs the per-colour passes)
+    if (c->tiled && cfg->precision == HMC_MIXED && !(cfg->flags & (HMC_FLAG_PER_COLOUR | HMC_FLAG_FP32_ERROR)) &&
         hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, c->device) == hipSuccess && coop) {
       TileCfg pc{};
       if (choose_regions(g, D, (double)cfg->n_bodies / (double)spaces, cus, &pc)) {

@@ -95,10 +95,12 @@ enum {
   HMC_FLAG_PER_COLOUR = 4,  /* MIXED hard shapes: one launch per colour pass
                                instead of the default single persistent launch
                                per apply (same results) */
-  HMC_FLAG_FP32_ERROR = 8   /* MIXED LJ: also evaluate every trial's dH in
-                               FP64 and record the largest FP32 error and its
-                               ratio to the error bound (hmc_stats; same
-                               decisions, slower) */
+  HMC_FLAG_FP32_ERROR = 8   /* MIXED: also measure in FP64 and record the
+                               largest FP32 error and its ratio to the error
+                               bound (hmc_stats; same decisions, slower).
+                               LJ: every trial's dH; hard spheres: every pair
+                               within 1.01 sigma (runs the per-colour tile
+                               passes) */
 };
 
 /* One trial of a trace: replaces the per-space BodyTrial status of
@@ -130,9 +132,10 @@ typedef struct {
                                summation-order difference could not settle
                                (decided from the GPU's own sum; 0 means every
                                decision provably equals the reference's) */
-  double fp32_error_max;    /* HMC_FLAG_FP32_ERROR: largest |dH32 - dH64| of an LJ trial */
-  double fp32_error_bound;  /* HMC_FLAG_FP32_ERROR: largest |dH32 - dH64| / bound
-                               (at most 1 when the bound holds) */
+  double fp32_error_max;    /* HMC_FLAG_FP32_ERROR: largest |dH32 - dH64| of an LJ
+                               trial, or |d2_32 - d2_64| of a hard-sphere pair */
+  double fp32_error_bound;  /* HMC_FLAG_FP32_ERROR: the largest such error divided
+                               by its bound (at most 1 when the bound holds) */
 } hmc_stats;
 
 /* Allocate device state for one system.  Replaces building a ParallelSweep and

@@ -106,6 +106,8 @@ def _sorted_trace(t):
 # falls back to the per-colour tiles), one LDS-tiled launch per colour, or
 # every trial from global memory
 MODES = {"persist": dict(), "percolour": dict(persistent=False), "global": dict(lds_tiles=False)}
+# layouts not in the general parametrisations
+EXTRA_MODES = {"fp32err": dict(fp32_error=True)}
 
 
 def expected_tiled(mode, precision):
@@ -115,7 +117,7 @@ def expected_tiled(mode, precision):
 def _mirror(hmc, ref, L, precision, mode="persist"):
     """GPU copy of the oracle microstate (its wrapped bodies and sites)."""
     b, s = ref.positions()
-    return hmc.GpuParallelSweep(L, b, sites=s, precision=precision, **MODES[mode])
+    return hmc.GpuParallelSweep(L, b, sites=s, precision=precision, **{**MODES, **EXTRA_MODES}[mode])
 
 
 def _replay(hmc, L, pos, d, n_applies, precision, seed=42, trace=True, step0=0, mode="persist"):
@@ -164,6 +166,17 @@ def test_replay_hard_spheres(hmc, wl, precision, mode):
     assert st["tiled"] == expected_tiled(mode, precision) and st["tile_overflows"] == 0
     log(f"hard spheres N=4000 {precision}: 12 applies bit-identical ({time.time() - t:.1f}s) {st} "
         f"libm divergences (log, exp) {libm_divergences()}")
+
+
+def test_fp32_error_within_the_contact_band(hmc, wl):
+    # with the FP32 error report on (per-colour tile passes), every pair within
+    # 1.01 sigma is also measured in FP64: the largest |d2_32 - d2_64| lies
+    # within the band's error bound, and the replay stays bit-identical
+    L, pos = wl.fcc_spheres((10, 10, 10), packing_fraction=0.50)
+    st = _replay(hmc, L, pos, 0.08, 8, "mixed", mode="fp32err")
+    log(f"hard spheres FP32 d2 error: max {st['fp32_error_max']:.3g}, max error / bound {st['fp32_error_bound']:.3g}")
+    assert st["tiled"] == 1
+    assert 0.0 < st["fp32_error_max"] and 0.0 < st["fp32_error_bound"] <= 1.0
 
 
 @pytest.mark.parametrize("precision,mode", [("fp64", "percolour"), ("mixed", "percolour"), ("mixed", "persist")])
