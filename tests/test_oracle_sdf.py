@@ -1,6 +1,6 @@
 """Scale distribution function (SDF): the oracle's histogram (oracle/sdf.hpp)
 pinned by its all-pairs neighbour search, closed forms (two bodies, a perfect
-FCC lattice), overlap counting, and the hard-sphere equation of state
+FCC lattice, a square lattice of disks), overlap counting, and the hard-sphere equation of state
 (Carnahan-Starling) through the package's beta P / rho extrapolation on an
 oracle trajectory.  The reference has no pressure compute; the SDF follows
 HOOMD-blue's hpmc compute.SDF definition."""
@@ -96,6 +96,21 @@ def test_fcc_lattice_closed_form():
         assert ov == 0 and h.sum() == len(pos) and h[int(math.floor(k))] == len(pos)
         return
     pytest.fail("no packing fraction away from a bin edge")
+
+
+def test_square_lattice_closed_form():
+    # every disk of a square lattice has its nearest neighbours at the
+    # lattice constant a, so x = 1 - sigma / a for all of them
+    for phi in (0.77, 0.76, 0.75):
+        L, pos = wl.square_lattice_disks(n_side=12, packing_fraction=phi)
+        a = L[0] / 12
+        k = (1.0 - 1.0 / a) / DX
+        if not 0.0 <= k < NBINS or abs(k - round(k)) < 1e-3:
+            continue
+        h, ov = O.sdf(L, pos, 1.0, XMAX, DX, NBINS)
+        assert ov == 0 and h.sum() == len(pos) and h[int(math.floor(k))] == len(pos)
+        return
+    pytest.fail("no packing fraction inside the histogram and away from a bin edge")
 
 
 def test_empty_and_single_body():
