@@ -24,6 +24,7 @@ HMC_HARD_SPHERE = 0
 HMC_LJ = 1
 HMC_LJ_SHIFTED = 2
 HMC_CONVEX_POLYHEDRON = 3
+HMC_OVERLAP_PENALTY = 4
 HMC_FP64 = 0
 HMC_MIXED = 1
 
@@ -156,8 +157,12 @@ class GpuParallelSweep:
         "lj" (params = (epsilon, sigma, r_cut)) or "lj_shifted"
         (params = (epsilon, sigma, r_cut, r_shift)) or "convex" (hard convex
         polyhedra: vertices (k x 3), orientations (n x 4, s x y z; identity by
-        default)); the ranges default to the model's interaction range
-        (diameter, r_cut or twice the circumsphere radius)."""
+        default)) or "overlap_penalty" (the placement Hamiltonian
+        Isotropic { Expanded { delta, OverlapPenalty }, r_cut: delta }:
+        params = (k, maximum_allowed_overlap, epsilon_shoulder, delta),
+        default OverlapPenalty::default() with delta 1; FP64 only); the
+        ranges default to the model's interaction range (diameter, r_cut,
+        twice the circumsphere radius or delta)."""
         pos = np.ascontiguousarray(positions, np.float64)
         self.dim = pos.shape[1]
         self.n = pos.shape[0]
@@ -167,9 +172,10 @@ class GpuParallelSweep:
         for a in range(self.dim):
             cfg.box_edges[a] = float(box_edges[a])
         model_id = {"hard_sphere": HMC_HARD_SPHERE, "lj": HMC_LJ, "lj_shifted": HMC_LJ_SHIFTED,
-                    "convex": HMC_CONVEX_POLYHEDRON}[model]
+                    "convex": HMC_CONVEX_POLYHEDRON, "overlap_penalty": HMC_OVERLAP_PENALTY}[model]
         if params is None:
-            params = (diameter,) if model == "hard_sphere" else (1.0, 1.0, 2.5, 2.5)
+            params = {"hard_sphere": (diameter,),
+                      "overlap_penalty": (10000.0, 0.2, 100.0, 1.0)}.get(model, (1.0, 1.0, 2.5, 2.5))
         if model == "convex":
             self._verts = np.ascontiguousarray(vertices, np.float64).reshape(-1, 3)
             cfg.vertices = _p(self._verts)
@@ -177,7 +183,8 @@ class GpuParallelSweep:
             rng = 2.0 * float(np.sqrt(np.max(np.sum(self._verts**2, axis=1))))
             params = ()
         else:
-            rng = params[0] if model == "hard_sphere" else params[2]
+            rng = {"hard_sphere": 0, "overlap_penalty": 3}.get(model, 2)
+            rng = params[rng]
         cfg.boundary_range = rng if boundary_range is None else boundary_range
         cfg.body_interaction_range = rng if body_interaction_range is None else body_interaction_range
         # VecCell::with_search_radius(maximum interaction range), as the

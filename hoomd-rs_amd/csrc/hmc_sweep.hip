@@ -74,12 +74,14 @@ struct Geometry {
   double vc_width;  // VecCell cell width of the reference microstate
   double vc_k;      // VecCell stencil radius ceil(sigma / vc_width)
   float band_lo, band_hi;  // mixed mode: d2 < band_lo overlaps, d2 >= band_hi does not
-  int model;               // HMC_HARD_SPHERE, HMC_LJ, HMC_LJ_SHIFTED
+  int model;               // HMC_HARD_SPHERE, HMC_LJ, HMC_LJ_SHIFTED, HMC_OVERLAP_PENALTY, ...
   double lj_eps4;          // 4 epsilon (lennard_jones.rs:110)
   double lj_sigma;
   double lj_rc2;           // r_cut^2 (isotropic.rs:151)
   double lj_rc2_edge;      // r_cut^2 (1 - 2^-30): at or above it the spatial-structure rule is applied
   double lj_shift;         // E(r_shift) for the shifted potential, else 0
+  double op_half_k;        // overlap penalty: 0.5 k, maximum_allowed_overlap, epsilon_shoulder, delta
+  double op_dmax, op_eps, op_delta;
   int lj_mixed;            // LJ pair energies in FP32 with an FP64 re-test inside the error bound
   int lj_fp32_error;       // mixed: also measure in FP64 (LJ: every trial's dH; hard spheres: near-contact
                            // pairs of the per-colour tile pass) and record the largest FP32 error
@@ -92,6 +94,12 @@ struct Geometry {
   // every lane copy the whole struct to scratch memory at kernel entry.
   const Geometry* self;
 };
+
+// The isotropic pair energies, which run one trial per wavefront with a
+// Metropolis decision on dH (the others are hard overlap tests).
+__host__ __device__ inline bool is_isotropic(int model) {
+  return model == HMC_LJ || model == HMC_LJ_SHIFTED || model == HMC_OVERLAP_PENALTY;
+}
 
 // Device-resident per-apply state.
 struct ApplyState {
@@ -2084,8 +2092,17 @@ __global__ void __launch_bounds__(kGuardThreads) k_guard(Geometry g, ApplyState*
 // r = sqrt(r^2), s = sigma / r, E = (4 eps) (s^12 - s^6) - E(r_shift), with
 // s^12 = s^4 s^8 and s^6 = s^2 s^4 as powi's square chain evaluates them
 // (lj_shift is 0 for the unshifted potential, which subtracts exactly).
+//
+// Isotropic<Expanded<OverlapPenalty>> (overlap_penalty.rs:109-115,
+// expanded.rs:62-66): x = r - delta, E = inf for x < -d_max,
+// (0.5 k) x^2 + eps_shoulder for x < 0, else 0 (r_cut = delta).
 __device__ __forceinline__ double lj_energy(const Geometry& g, double r2) {
   const double r = sqrt(r2);
+  if (g.model == HMC_OVERLAP_PENALTY) {
+    const double x = r - g.op_delta;
+    if (x < -g.op_dmax) return INFINITY;
+    return x < 0.0 ? g.op_half_k * (x * x) + g.op_eps : 0.0;
+  }
   const double sr = g.lj_sigma / r;
   const double s2 = sr * sr, s4 = s2 * s2, s8 = s4 * s4;
   const double s12 = s4 * s8, s6 = s2 * s4;
@@ -2387,12 +2404,20 @@ __device__ void trial_lj(const Geometry& g, const ApplyState* st, const Bodies& 
     }
     if (!decided) {
       const WaveSums ws = lj_trial_sums<D, false>(g, C, seg_lo, seg_pre, NS, total, slot, t, xi, tb);
-      // energy_final - energy_initial (pairwise_cutoff.rs:919-941)
-      const double dH = ws.ef - ws.ei;
-      const double nmax = (double)max(max(ws.nf, ws.ni), 1u);
-      const double gam = nmax * 0x1p-53 / (1.0 - nmax * 0x1p-53);
-      const double delta = (2.0 * gam * (ws.af + ws.ai) + 0x1p-52 * fabs(dH)) * (1.0 + 0x1p-20) + 0x1p-1070;
-      accept = lj_metropolis(rng, dH, delta, st->kT, &amb) != 0;
+      // energy_final - energy_initial (pairwise_cutoff.rs:919-941): an
+      // infinite final energy is returned as is and rejected; an infinite
+      // initial one makes dH = -inf, accepted
+      if (ws.ef == INFINITY) {
+        accept = false;
+      } else if (ws.ei == INFINITY) {
+        accept = true;
+      } else {
+        const double dH = ws.ef - ws.ei;
+        const double nmax = (double)max(max(ws.nf, ws.ni), 1u);
+        const double gam = nmax * 0x1p-53 / (1.0 - nmax * 0x1p-53);
+        const double delta = (2.0 * gam * (ws.af + ws.ai) + 0x1p-52 * fabs(dH)) * (1.0 + 0x1p-20) + 0x1p-1070;
+        accept = lj_metropolis(rng, dH, delta, st->kT, &amb) != 0;
+      }
     }
     o.ambiguous = (amb && lane == 0) ? 1u : 0u;
     if (accept && lane == 0) {
@@ -3281,6 +3306,10 @@ __global__ void __launch_bounds__(kEnergyBlock) k_energy(Geometry g, const Apply
               const double r2 = lj_r2<D>(g, t, sj);
               if (r2 < g.lj_rc2) {
                 const double one = lj_energy(g, r2);
+                if (one == INFINITY) {  // pairwise_cutoff.rs:720-745 returns at once
+                  inf = true;
+                  break;
+                }
                 e += one;
                 a += fabs(one);
                 cnt += 1.0;
@@ -3610,7 +3639,7 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
   const uint32_t chunk = (blocks + 7) / 8;
   const uint32_t grid = chunk * 8;
   const uint32_t tile_threads = ((c->tc.trials + 63) / 64) * 64;
-  const bool lj = g.model == HMC_LJ || g.model == HMC_LJ_SHIFTED;
+  const bool lj = is_isotropic(g.model);
   const bool convex = g.model == HMC_CONVEX_POLYHEDRON;
   // LJ needs many cycles per apply and its guard is slow: launch one spare
   // cycle (its passes exit at once when not needed)
@@ -4099,10 +4128,12 @@ static bool choose_regions(const Geometry& g, int D, double occ, int cus, TileCf
 // replica of a batch.
 static int make_geometry(const hmc_config* cfg, Geometry& g, bool* fast, double* sigma_out, double* lmax_out) {
   if (cfg->dim != 2 && cfg->dim != 3) return HMC_ERR_UNSUPPORTED;
-  const bool lj = cfg->model == HMC_LJ || cfg->model == HMC_LJ_SHIFTED;
+  const bool lj = is_isotropic(cfg->model);
   const bool convex = cfg->model == HMC_CONVEX_POLYHEDRON;
+  const bool op = cfg->model == HMC_OVERLAP_PENALTY;
   if (cfg->model != HMC_HARD_SPHERE && !lj && !convex) return HMC_ERR_UNSUPPORTED;
   if (cfg->precision != HMC_FP64 && cfg->precision != HMC_MIXED) return HMC_ERR_INVALID;
+  if (op && cfg->precision != HMC_FP64) return HMC_ERR_UNSUPPORTED;  // FP64 only
   if (convex && cfg->precision != HMC_FP64) return HMC_ERR_UNSUPPORTED;  // FP64 only
   if (convex && cfg->dim != 3) return HMC_ERR_UNSUPPORTED;
   if (convex && (!cfg->vertices || cfg->n_vertices < 1 || cfg->n_vertices > (uint32_t)kMaxVertices))
@@ -4123,12 +4154,16 @@ static int make_geometry(const hmc_config* cfg, Geometry& g, bool* fast, double*
   }
   // interaction range: the diameter (hard_shape.rs:170-174), r_cut (isotropic.rs)
   // or twice the circumsphere radius (hard_shape.rs:111-118)
-  const double sigma = lj ? cfg->params[2] : convex ? cv_radius * 2.0 : cfg->params[0];
+  const double sigma = op ? cfg->params[3] : lj ? cfg->params[2] : convex ? cv_radius * 2.0 : cfg->params[0];
   if (!(sigma > 0.0) || !(cfg->body_interaction_range > 0.0) || !(cfg->boundary_range >= 0.0) ||
       !(cfg->spatial_cell_width > 0.0))
     return HMC_ERR_INVALID;
-  if (lj && (!(cfg->params[0] >= 0.0) || !(cfg->params[1] > 0.0) || !std::isfinite(cfg->params[0]) ||
-             !std::isfinite(cfg->params[1])))
+  if (lj && !op && (!(cfg->params[0] >= 0.0) || !(cfg->params[1] > 0.0) || !std::isfinite(cfg->params[0]) ||
+                   !std::isfinite(cfg->params[1])))
+    return HMC_ERR_INVALID;
+  if (op && (!(cfg->params[0] >= 0.0) || !(cfg->params[1] >= 0.0) || !(cfg->params[2] >= 0.0) ||
+             !std::isfinite(cfg->params[0]) || !std::isfinite(cfg->params[1]) || !std::isfinite(cfg->params[2]) ||
+             !std::isfinite(cfg->params[3])))
     return HMC_ERR_INVALID;
   if (cfg->model == HMC_LJ_SHIFTED && !(cfg->params[3] > 0.0)) return HMC_ERR_INVALID;
   double lmin = cfg->box_edges[0];
@@ -4198,7 +4233,15 @@ static int make_geometry(const hmc_config* cfg, Geometry& g, bool* fast, double*
     g.cv_rcut2 = (cv_radius + cv_radius) * (cv_radius + cv_radius);
     g.cv_edge2 = g.cv_rcut2 * (1.0 - 0x1p-30);
   }
-  if (lj) {
+  if (op) {
+    // overlap_penalty.rs:109-115 (0.5 k as the reference forms it), r_cut = delta
+    g.op_half_k = 0.5 * cfg->params[0];
+    g.op_dmax = cfg->params[1];
+    g.op_eps = cfg->params[2];
+    g.op_delta = cfg->params[3];
+    g.lj_rc2 = sigma * sigma;
+    g.lj_rc2_edge = g.lj_rc2 * (1.0 - 0x1p-30);
+  } else if (lj) {
     // lennard_jones.rs:107-111, shifted.rs:92-94, isotropic.rs:147-156
     const double eps = cfg->params[0], lsig = cfg->params[1], rc = cfg->params[2];
     g.lj_eps4 = 4.0 * eps;
@@ -4247,7 +4290,7 @@ int hmc_create(const hmc_config* cfg_in, hmc_ctx** out) {
     if (rc) return rc;
   }
   const int D = (int)cfg->dim;
-  const bool lj = cfg->model == HMC_LJ || cfg->model == HMC_LJ_SHIFTED;
+  const bool lj = is_isotropic(cfg->model);
   const bool convex = cfg->model == HMC_CONVEX_POLYHEDRON;
   const uint64_t spaces = g0.n_spaces;
   auto* c = new hmc_ctx;
@@ -4572,7 +4615,7 @@ int hmc_run_translate(hmc_ctx* c, uint64_t step0, uint32_t seed, double kT, doub
   ApplyState before;
   int rc = read_state(c, &before);
   if (rc) return rc;
-  const bool lj = c->g.model == HMC_LJ || c->g.model == HMC_LJ_SHIFTED;
+  const bool lj = is_isotropic(c->g.model);
   // (a cooperative launch is not captured: the persistent apply launches directly)
   const bool graphs = !(c->cfg.flags & HMC_FLAG_NO_GRAPH) && !c->persist;
   for (uint32_t i = 0; i < n_steps; ++i) {

@@ -47,9 +47,14 @@ enum {
                           params = {epsilon, sigma, r_cut, -} */
   HMC_LJ_SHIFTED = 2,  /* Isotropic<Shifted<LennardJones<12,6>>> (univariate/shifted.rs):
                           params = {epsilon, sigma, r_cut, r_shift} */
-  HMC_CONVEX_POLYHEDRON = 3 /* HardShape<Convex<ConvexPolyhedron>> (pairwise/hard_shape.rs:79-118,
-                               hoomd-geometry/src/convex.rs:146-170) on OrientedPoint<Cartesian<3>,
-                               Versor> bodies: hmc_config.vertices; dim 3, HMC_FP64 only */
+  HMC_CONVEX_POLYHEDRON = 3, /* HardShape<Convex<ConvexPolyhedron>> (pairwise/hard_shape.rs:79-118,
+                                hoomd-geometry/src/convex.rs:146-170) on OrientedPoint<Cartesian<3>,
+                                Versor> bodies: hmc_config.vertices; dim 3, HMC_FP64 only */
+  HMC_OVERLAP_PENALTY = 4    /* Isotropic { Expanded { delta, OverlapPenalty }, r_cut: delta }
+                                (univariate/overlap_penalty.rs:109-115, univariate/expanded.rs:62-66),
+                                the placement Hamiltonian of benchmarks/src/mc/hard_sphere.rs:192-198:
+                                params = {k, maximum_allowed_overlap, epsilon_shoulder, delta};
+                                HMC_FP64 only */
 };
 
 /* Arithmetic of the pair test. Both modes make identical decisions. */
@@ -73,7 +78,8 @@ typedef struct {
   double spatial_cell_width;     /* cell width of the microstate's VecCell
                                     (hoomd-spatial/src/vec_cell.rs:286-299); decides
                                     which images are visited at exact contact */
-  int32_t model;                 /* HMC_HARD_SPHERE, HMC_LJ, HMC_LJ_SHIFTED or HMC_CONVEX_POLYHEDRON */
+  int32_t model;                 /* HMC_HARD_SPHERE, HMC_LJ, HMC_LJ_SHIFTED, HMC_CONVEX_POLYHEDRON
+                                    or HMC_OVERLAP_PENALTY */
   double params[4];
   int32_t precision;             /* HMC_FP64 or HMC_MIXED */
   int32_t device;                /* HIP device ordinal */

@@ -68,11 +68,14 @@ struct Site {
   Vec<D> pos{};
 };
 
-// Pair energies.  kind: 0 HardSphere, 1 Isotropic<LennardJones>, 2 Isotropic<Shifted<LennardJones>>.
+// Pair energies.  kind: 0 HardSphere, 1 Isotropic<LennardJones>, 2 Isotropic<Shifted<LennardJones>>,
+// 4 Isotropic<Expanded<OverlapPenalty>> with r_cut = delta (the placement
+// Hamiltonian of benchmarks/src/mc/hard_sphere.rs:192-198).
 struct PairModel {
   int kind = 0;
   double diameter = 1.0;          // HardSphere (hard_shape.rs:127-174)
   double epsilon = 1.0, sigma = 1.0, r_cut = 2.5, r_shift = 2.5;  // LJ
+  double k = 10000.0, max_overlap = 0.2, eps_shoulder = 100.0, delta = 1.0;  // OverlapPenalty
 
   double range() const { return kind == 0 ? diameter : r_cut; }
   bool only_infinite_or_zero() const { return kind == 0; }
@@ -84,12 +87,21 @@ struct PairModel {
     double s12 = s4 * s8, s6 = s2 * s4;
     return 4.0 * epsilon * (s12 - s6);
   }
+  // univariate/overlap_penalty.rs:109-115 at x = r - delta (expanded.rs:62-66);
+  // powi(2) is x * x
+  double overlap_penalty(double r) const {
+    double x = r - delta;
+    if (x < -max_overlap) return INF;
+    if (x < 0.0) return 0.5 * k * (x * x) + eps_shoulder;
+    return 0.0;
+  }
   template <int D>
   double pair(const Vec<D>& a, const Vec<D>& b) const {
     double r2 = distance_squared<D>(a, b);
     if (kind == 0) return r2 < diameter * diameter ? INF : 0.0;  // hard_shape.rs:143-153
     if (r2 >= r_cut * r_cut) return 0.0;                          // isotropic.rs:147-156
     double r = std::sqrt(r2);
+    if (kind == 4) return overlap_penalty(r);
     if (kind == 1) return lj(r);
     return lj(r) - lj(r_shift);                                    // shifted.rs:92-94
   }

@@ -30,6 +30,12 @@ void fill_model(System<D>& s, int model, const double* params) {
   s.model.kind = model;
   if (model == 0) {
     s.model.diameter = params[0];
+  } else if (model == 4) {
+    s.model.k = params[0];
+    s.model.max_overlap = params[1];
+    s.model.eps_shoulder = params[2];
+    s.model.delta = params[3];
+    s.model.r_cut = params[3];
   } else {
     s.model.epsilon = params[0];
     s.model.sigma = params[1];
@@ -45,7 +51,7 @@ System<D>* make(const double* L, double boundary_range, double body_range, int m
   for (int i = 0; i < D; ++i) edges[i] = L[i];
   PairModel pm;
   pm.kind = model;
-  double radius = model == 0 ? params[0] : params[2];
+  double radius = model == 0 ? params[0] : model == 4 ? params[3] : params[2];
   auto* s = new System<D>(edges, boundary_range, radius);
   fill_model(*s, model, params);
   s->body_range = body_range;
@@ -268,7 +274,8 @@ void oracle_cell_index(int dim, double w, const double* p, int64_t* out) {
 }
 
 // ---- full systems ------------------------------------------------------------
-// model: 0 HardSphere(params[0]=diameter), 1 LJ, 2 shifted LJ (params eps, sigma, r_cut, r_shift)
+// model: 0 HardSphere(params[0]=diameter), 1 LJ, 2 shifted LJ (params eps, sigma, r_cut, r_shift),
+// 4 overlap penalty (params k, maximum_allowed_overlap, epsilon_shoulder, delta = r_cut)
 void* oracle_system_create(int dim, const double* L, double boundary_range, double body_range, int model,
                            const double* params, uint64_t step, uint32_t seed, const double* pos, uint64_t n) {
   auto* h = new Handle;

@@ -193,7 +193,7 @@ def cell_index(w, p):
 class OracleSystem:
     """The restated Microstate + PairwiseCutoff + ParallelSweep/Sweep."""
 
-    MODELS = {"hard_sphere": 0, "lj": 1, "lj_shifted": 2}
+    MODELS = {"hard_sphere": 0, "lj": 1, "lj_shifted": 2, "overlap_penalty": 4}
 
     def __init__(self, L, positions, *, model="hard_sphere", params=(1.0,), boundary_range=None,
                  body_range=None, step=0, seed=0):
@@ -203,7 +203,7 @@ class OracleSystem:
         self.n = len(pos)
         prm = np.zeros(4)
         prm[: len(params)] = params
-        rng = params[0] if model == "hard_sphere" else params[2]
+        rng = {"hard_sphere": params[0], "overlap_penalty": prm[3]}.get(model, prm[2])
         boundary_range = rng if boundary_range is None else boundary_range
         body_range = rng if body_range is None else body_range
         self._keep = (L, pos, prm)
