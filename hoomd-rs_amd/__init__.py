@@ -29,7 +29,8 @@ HMC_FP64 = 0
 HMC_MIXED = 1
 
 # every entry point declared in include/hmc.h
-EXPORTS = ["hmc_create", "hmc_set_state", "hmc_get_state", "hmc_apply_translate", "hmc_apply_rotate",
+EXPORTS = ["hmc_create", "hmc_set_state", "hmc_get_state", "hmc_clone_with_boundary", "hmc_apply_translate",
+           "hmc_apply_rotate",
            "hmc_set_orientations", "hmc_get_orientations", "hmc_run_translate", "hmc_total_energy",
            "hmc_trace_enable", "hmc_trace_read", "hmc_get_stats", "hmc_profile_enable", "hmc_profile_read",
            "hmc_debug_math", "hmc_debug_normals", "hmc_debug_convex", "hmc_last_error", "hmc_destroy",
@@ -86,6 +87,7 @@ def lib():
         L.hmc_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
         L.hmc_set_state.argtypes = [vp, D_P, D_P]
         L.hmc_get_state.argtypes = [vp, D_P, D_P]
+        L.hmc_clone_with_boundary.argtypes = [vp, D_P, C.POINTER(vp)]
         L.hmc_apply_translate.argtypes = [vp, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
                                           C.POINTER(Count), C.POINTER(C.c_uint32)]
         L.hmc_run_translate.argtypes = [vp, C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_uint32,
@@ -166,6 +168,7 @@ class GpuParallelSweep:
         pos = np.ascontiguousarray(positions, np.float64)
         self.dim = pos.shape[1]
         self.n = pos.shape[0]
+        self.box_edges = [float(box_edges[a]) for a in range(self.dim)]
         cfg = Config()
         cfg.dim = self.dim
         cfg.n_bodies = self.n
@@ -217,6 +220,23 @@ class GpuParallelSweep:
 
     def __del__(self):
         self.close()
+
+    def clone_with_boundary(self, box_edges):
+        """Microstate::clone_with_boundary(new_boundary, |_| true)
+        (hoomd-microstate/src/microstate.rs:1529-1565) onto new box edges: a
+        new GpuParallelSweep with every body mapped by Hypercuboid::map_point
+        (hoomd-geometry/src/shape/cuboid.rs:553-567) on the device; this one is
+        unchanged.  HmcError(HMC_ERR_INVALID) when the interaction ranges do
+        not fit the new edges."""
+        e = np.zeros(3)
+        e[: self.dim] = [float(x) for x in box_edges[: self.dim]]
+        h = C.c_void_p()
+        self._check(lib().hmc_clone_with_boundary(self.h, _p(e), C.byref(h)))
+        new = object.__new__(type(self))
+        new.__dict__.update({k: v for k, v in self.__dict__.items() if k != "h"})
+        new.h = h
+        new.box_edges = [float(x) for x in e[: self.dim]]
+        return new
 
     def apply(self, step, substep, seed, kT, max_distance):
         cnt = Count()
@@ -432,6 +452,14 @@ class GpuBatch:
 
     def __del__(self):
         self.close()
+
+
+def scale_volume(box_edges, v):
+    """Edges of a hypercuboid whose volume is scaled by v
+    (Hypercuboid::scale_volume, hoomd-geometry/src/shape/cuboid.rs:514-518):
+    every edge times v^(1/D), as the reference forms it."""
+    f = float(v) ** (1.0 / len(box_edges))
+    return [float(e) * f for e in box_edges]
 
 
 def sdf_nbins(xmax, dx):

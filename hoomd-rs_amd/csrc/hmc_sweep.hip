@@ -3480,6 +3480,38 @@ __global__ void k_init_sites(Geometry g, Bodies B, int have_sites) {
   B.spos[e] = pack_site(s, ghost_flags<D>(g, s));
 }
 
+// Microstate::clone_with_boundary with every body mapped (microstate.rs:1529-1565):
+// Hypercuboid::map_point (cuboid.rs:553-567), p' = clamp(p (L'/L), -L'/2,
+// next_down(L'/2)) per axis, then update_body_properties (microstate.rs:985-1034):
+// body = wrap(p'), site = wrap(body + 0) with its ghost flags.  g is the new
+// geometry; a body outside the old box (map_point's error, on which the
+// reference panics) sets *outside.
+template <int D>
+__global__ void k_map_bodies(Geometry g, double3 lold, const double4* __restrict__ src, Bodies B,
+                             uint32_t* outside) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= g.n_bodies) return;
+  const double4 b = src[e];
+  const double x[3] = {b.x, b.y, b.z};
+  const double lo[3] = {lold.x, lold.y, lold.z};
+  double p[3] = {0.0, 0.0, 0.0};
+  bool inside = true;
+  for (int a = 0; a < D; ++a) {
+    inside = inside && -lo[a] / 2.0 <= x[a] && x[a] < lo[a] / 2.0;  // cuboid.rs:449-454
+    const double scale = g.L[a] / lo[a];
+    const double low = -g.L[a] / 2.0;
+    const double high = __longlong_as_double(__double_as_longlong(g.L[a] / 2.0) - 1);  // next_down of a positive
+    const double v = scale * x[a];
+    p[a] = v < low ? low : (v > high ? high : v);  // f64::clamp
+  }
+  if (!inside) atomicOr(outside, 1u);
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int a = 0; a < D; ++a) p[a] = wrap1(p[a], g.L[a]);
+  for (int a = 0; a < D; ++a) s[a] = wrap1(p[a] + 0.0, g.L[a]);
+  B.bpos[e] = make_double4(p[0], p[1], p[2], 0.0);
+  B.spos[e] = pack_site(s, ghost_flags<D>(g, s));
+}
+
 // ------------------------------------------------------------- debug hooks --
 __global__ void k_debug_math(int kind, const double* in, double* out, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -3512,6 +3544,8 @@ using namespace hmc;
 
 struct hmc_ctx {
   hmc_config cfg;
+  hmc_config cfg_user;              // as given to hmc_create (vertices -> verts_host)
+  std::vector<double> verts_host;   // convex: the vertex table, for hmc_clone_with_boundary
   Geometry g;
   int device;
   hipStream_t stream = nullptr;
@@ -4295,6 +4329,9 @@ int hmc_create(const hmc_config* cfg_in, hmc_ctx** out) {
   const uint64_t spaces = g0.n_spaces;
   auto* c = new hmc_ctx;
   c->cfg = *cfg;
+  c->cfg_user = *cfg_in;
+  if (convex) c->verts_host.assign(cfg_in->vertices, cfg_in->vertices + 3 * (size_t)cfg_in->n_vertices);
+  c->cfg_user.vertices = nullptr;
   c->device = cfg->device;
   c->g = g0;
   c->g.lj_mixed = lj_mixed ? 1 : 0;
@@ -4544,6 +4581,62 @@ int hmc_debug_convex(int device, int kind, const double* vertices, uint32_t n_ve
   if (di) (void)hipFree(di);
   if (dout) (void)hipFree(dout);
   return rc;
+}
+
+int hmc_clone_with_boundary(hmc_ctx* c, const double* new_edges, hmc_ctx** out) {
+  if (!c || !new_edges || !out) return HMC_ERR_INVALID;
+  *out = nullptr;
+  if (!c->have_state) return fail(c, HMC_ERR_STATE, "no state");
+  const int D = c->g.dim;
+  const uint32_t N = c->g.n_bodies;
+  hmc_config cfg = c->cfg_user;
+  for (int a = 0; a < D; ++a) cfg.box_edges[a] = new_edges[a];
+  if (!c->verts_host.empty()) cfg.vertices = c->verts_host.data();
+  // the same ranges on the new edges: hmc_create applies Periodic::new's
+  // criterion (the scale_volume assertion, periodic.rs:288-300) and the
+  // checkerboard's
+  hmc_ctx* d = nullptr;
+  const int rc = hmc_create(&cfg, &d);
+  if (rc) return fail(c, rc, "the new boundary does not fit the configuration's interaction ranges");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  uint32_t outside = 0;
+  hipError_t e = hipSuccess;
+  if (N) {
+    uint32_t* flag = nullptr;
+    e = hipMallocAsync((void**)&flag, sizeof(uint32_t), d->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(flag, 0, sizeof(uint32_t), d->stream);
+    if (e == hipSuccess) {
+      const double3 lold = make_double3(c->g.L[0], c->g.L[1], c->g.L[2]);
+      if (D == 2)
+        k_map_bodies<2><<<grid_for(N, 256), 256, 0, d->stream>>>(d->g, lold, c->bodies.bpos, d->bodies, flag);
+      else
+        k_map_bodies<3><<<grid_for(N, 256), 256, 0, d->stream>>>(d->g, lold, c->bodies.bpos, d->bodies, flag);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess && c->g.model == HMC_CONVEX_POLYHEDRON) {
+      // orientations are kept; the sites' are rebuilt from the bodies'
+      e = hipMemcpyAsync(d->bodies.bquat, c->bodies.bquat, N * sizeof(double4), hipMemcpyDeviceToDevice, d->stream);
+      if (e == hipSuccess) {
+        k_init_site_quats<<<grid_for(N, 256), 256, 0, d->stream>>>(d->g, d->bodies, 0);
+        e = hipGetLastError();
+      }
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&outside, flag, sizeof(uint32_t), hipMemcpyDeviceToHost, d->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+    if (flag) (void)hipFreeAsync(flag, d->stream);
+  }
+  if (e != hipSuccess) {
+    hmc_destroy(d);
+    return fail(c, HMC_ERR_DEVICE, std::string("clone_with_boundary: ") + hipGetErrorString(e));
+  }
+  if (outside) {
+    hmc_destroy(d);
+    return fail(c, HMC_ERR_INVALID, "a body lies outside the boundary (map_point: PointOutsideShape)");
+  }
+  d->have_state = true;
+  *out = d;
+  return HMC_OK;
 }
 
 int hmc_get_state(hmc_ctx* c, double* body_pos, double* site_pos) {

@@ -158,6 +158,19 @@ int hmc_set_state(hmc_ctx* ctx, const double* body_pos, const double* site_pos);
 /* Download body and site positions in body index order (site_pos may be NULL). */
 int hmc_get_state(hmc_ctx* ctx, double* body_pos, double* site_pos);
 
+/* Microstate::clone_with_boundary(new_boundary, |_| true)
+ * (hoomd-microstate/src/microstate.rs:1529-1565) for a Periodic<Hypercuboid<D>>
+ * with new edge lengths, such as Periodic::scale_volume makes
+ * (boundary/periodic.rs:288-300, hoomd-geometry/src/shape/cuboid.rs:514-518):
+ * a new context with the same configuration on new_edges (first dim used),
+ * each body mapped by Hypercuboid::map_point (cuboid.rs:553-567) and rebuilt by
+ * update_body_properties (microstate.rs:985-1034).  ctx is left unchanged, as
+ * QuickCompress keeps the initial microstate (hoomd-mc/src/quick_compress.rs,
+ * Compressing state).  HMC_ERR_INVALID when the ranges do not fit the new
+ * edges or a body lies outside the old box; free the new context with
+ * hmc_destroy. */
+int hmc_clone_with_boundary(hmc_ctx* ctx, const double* new_edges, hmc_ctx** out);
+
 /* One ParallelSweep::apply with Translate moves (parallel_sweep.rs:390-421).
  * (step, substep, seed) = microstate.counter(); kT = macrostate.temperature();
  * max_distance = Translate::maximum_distance.  On return *substeps_used holds

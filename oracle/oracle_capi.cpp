@@ -532,4 +532,34 @@ int oracle_sdf(int dim, const double* L, const double* pos, uint64_t n, double s
   return 0;
 }
 
+// Hypercuboid::map_point (hoomd-geometry/src/shape/cuboid.rs:553-567) of n
+// points from edges Lold to Lnew: p' = clamp(p (Lnew / Lold), -Lnew/2,
+// next_down(Lnew/2)) per axis.  A point outside the old box (the
+// PointOutsideShape error; is_point_inside, cuboid.rs:449-454) is counted and
+// left unmapped.  Returns the number of such points.
+uint64_t oracle_map_points(int dim, const double* Lold, const double* Lnew, const double* pts, uint64_t n,
+                           double* out) {
+  uint64_t outside = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    bool inside = true;
+    for (int a = 0; a < dim; ++a) {
+      const double x = pts[i * dim + a];
+      inside = inside && -Lold[a] / 2.0 <= x && x < Lold[a] / 2.0;
+    }
+    for (int a = 0; a < dim; ++a) {
+      const double x = pts[i * dim + a];
+      if (!inside) {
+        out[i * dim + a] = x;
+        continue;
+      }
+      const double scale = Lnew[a] / Lold[a];
+      const double lo = -Lnew[a] / 2.0, hi = std::nextafter(Lnew[a] / 2.0, -INF);
+      const double v = scale * x;
+      out[i * dim + a] = v < lo ? lo : (v > hi ? hi : v);  // f64::clamp
+    }
+    outside += inside ? 0 : 1;
+  }
+  return outside;
+}
+
 }  // extern "C"

@@ -436,3 +436,18 @@ class ConvexOracle:
         st = np.zeros(1, np.uint64)
         moves = _convex_lib().oracle_convex_time(self.h, d, a, seconds, _p(el, D_P), _p(st, U64_P))
         return int(moves), float(el[0]), int(st[0])
+
+
+def map_points(L_old, L_new, pts):
+    """Hypercuboid::map_point of every point (oracle_map_points): (mapped,
+    number outside the old box, left unmapped)."""
+    L_old = np.ascontiguousarray(L_old, np.float64)
+    L_new = np.ascontiguousarray(L_new, np.float64)
+    dim = len(L_old)
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, dim)
+    out = np.zeros_like(pts)
+    f = lib().oracle_map_points
+    f.argtypes = [C.c_int, D_P, D_P, D_P, C.c_uint64, D_P]
+    f.restype = C.c_uint64
+    n_out = f(dim, _p(L_old, D_P), _p(L_new, D_P), _p(pts, D_P), len(pts), _p(out, D_P))
+    return out, int(n_out)
