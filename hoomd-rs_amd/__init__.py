@@ -30,6 +30,7 @@ HMC_MIXED = 1
 
 # every entry point declared in include/hmc.h
 EXPORTS = ["hmc_create", "hmc_set_state", "hmc_get_state", "hmc_clone_with_boundary", "hmc_apply_translate",
+           "hmc_quick_compress_init", "hmc_quick_compress_apply",
            "hmc_apply_rotate",
            "hmc_set_orientations", "hmc_get_orientations", "hmc_run_translate", "hmc_total_energy",
            "hmc_trace_enable", "hmc_trace_read", "hmc_get_stats", "hmc_profile_enable", "hmc_profile_read",
@@ -88,6 +89,10 @@ def lib():
         L.hmc_set_state.argtypes = [vp, D_P, D_P]
         L.hmc_get_state.argtypes = [vp, D_P, D_P]
         L.hmc_clone_with_boundary.argtypes = [vp, D_P, C.POINTER(vp)]
+        L.hmc_quick_compress_init.argtypes = [C.POINTER(QuickCompressState), C.c_double]
+        L.hmc_quick_compress_init.restype = None
+        L.hmc_quick_compress_apply.argtypes = [C.POINTER(QuickCompressState), C.POINTER(vp), C.c_uint64,
+                                               C.c_uint32, C.c_uint32]
         L.hmc_apply_translate.argtypes = [vp, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
                                           C.POINTER(Count), C.POINTER(C.c_uint32)]
         L.hmc_run_translate.argtypes = [vp, C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_uint32,
@@ -452,6 +457,50 @@ class GpuBatch:
 
     def __del__(self):
         self.close()
+
+
+class QuickCompressState(C.Structure):
+    _fields_ = [("target_volume", C.c_double), ("maximum_energy_per_site", C.c_double),
+                ("maximum_delta", C.c_double), ("state", C.c_int32), ("is_final", C.c_int32),
+                ("boundary", C.c_double * 3), ("ambiguous", C.c_uint64)]
+
+
+QC_STATES = ("startup", "waiting", "compressing", "complete")
+
+
+class QuickCompress:
+    """QuickCompress (hoomd-mc/src/quick_compress.rs:19-444) over a
+    GpuParallelSweep: hmc_quick_compress_apply runs the state machine on the
+    host, the trial microstate (hmc_clone_with_boundary) and the energies
+    (hmc_total_energy) on the device."""
+
+    def __init__(self, target_volume, maximum_energy_per_site=1000.0, maximum_delta=0.05):
+        self.st = QuickCompressState()
+        lib().hmc_quick_compress_init(C.byref(self.st), float(target_volume))
+        self.st.maximum_energy_per_site = float(maximum_energy_per_site)
+        self.st.maximum_delta = float(maximum_delta)
+
+    def apply(self, sim, step, substep, seed):
+        """QuickCompress::apply with (step, substep, seed) = the microstate's
+        counter; it consumes one substep.  An accepted compression replaces
+        sim's device state by the trial microstate (sim.box_edges follow)."""
+        before = sim.h.value
+        rc = lib().hmc_quick_compress_apply(C.byref(self.st), C.byref(sim.h), step, substep, seed)
+        if rc != HMC_OK:
+            raise HmcError(rc, lib().hmc_last_error(sim.h).decode())
+        if sim.h.value != before:
+            sim.box_edges = [float(self.st.boundary[a]) for a in range(sim.dim)]
+
+    def is_complete(self):
+        return self.st.state == 3
+
+    @property
+    def state(self):
+        return QC_STATES[self.st.state]
+
+    @property
+    def ambiguous(self):
+        return int(self.st.ambiguous)
 
 
 def scale_volume(box_edges, v):

@@ -171,6 +171,35 @@ int hmc_get_state(hmc_ctx* ctx, double* body_pos, double* site_pos);
  * hmc_destroy. */
 int hmc_clone_with_boundary(hmc_ctx* ctx, const double* new_edges, hmc_ctx** out);
 
+/* QuickCompress (hoomd-mc/src/quick_compress.rs:19-444): compress (or expand)
+ * the box toward a target volume in steps the Hamiltonian tolerates, waiting
+ * for the sweeps in between to remove the overlaps (total energy <= 0).  The
+ * state machine runs on the host; the trial microstate is
+ * hmc_clone_with_boundary and the energies hmc_total_energy. */
+enum { HMC_QC_STARTUP = 0, HMC_QC_WAITING = 1, HMC_QC_COMPRESSING = 2, HMC_QC_COMPLETE = 3 };
+typedef struct {
+  double target_volume;           /* QuickCompress::with_target_volume */
+  double maximum_energy_per_site; /* default 1000 */
+  double maximum_delta;           /* default 0.05, in (0, 1) */
+  int32_t state;                  /* HMC_QC_* */
+  int32_t is_final;               /* Waiting: the last compression reached the target */
+  double boundary[3];             /* Waiting: last known edges; Complete: final edges */
+  uint64_t ambiguous;             /* energy comparisons inside the summation-order bound
+                                     (decided by the GPU value; expected 0) */
+} hmc_quick_compress;
+
+/* QuickCompress::with_target_volume: defaults, Startup state. */
+void hmc_quick_compress_init(hmc_quick_compress* qc, double target_volume);
+
+/* QuickCompress::apply(microstate, hamiltonian, |_| true) with (step,
+ * substep, seed) = microstate.counter(); it consumes one substep (the
+ * caller's counter then calls increment_substep).  When a compression is
+ * accepted, *ctx is replaced by the trial context and the old one destroyed.
+ * HMC_ERR_INVALID when a trial box is too small for the interaction ranges
+ * (the reference panics in Periodic::scale_volume). */
+int hmc_quick_compress_apply(hmc_quick_compress* qc, hmc_ctx** ctx, uint64_t step, uint32_t substep,
+                             uint32_t seed);
+
 /* One ParallelSweep::apply with Translate moves (parallel_sweep.rs:390-421).
  * (step, substep, seed) = microstate.counter(); kT = macrostate.temperature();
  * max_distance = Translate::maximum_distance.  On return *substeps_used holds

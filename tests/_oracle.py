@@ -201,6 +201,7 @@ class OracleSystem:
         self.dim = len(L)
         pos = np.ascontiguousarray(positions, np.float64).reshape(-1, self.dim)
         self.n = len(pos)
+        self.seed = seed
         prm = np.zeros(4)
         prm[: len(params)] = params
         rng = {"hard_sphere": params[0], "overlap_penalty": prm[3]}.get(model, prm[2])
@@ -451,3 +452,78 @@ def map_points(L_old, L_new, pts):
     f.restype = C.c_uint64
     n_out = f(dim, _p(L_old, D_P), _p(L_new, D_P), _p(pts, D_P), len(pts), _p(out, D_P))
     return out, int(n_out)
+
+
+def uniform_f64(bits, low, high):
+    """rand 0.10 Uniform::new(low, high).sample for f64 (a third-party crate,
+    not in the reference tree; parity unpinned): scale = high - low, lowered
+    one ulp at a time while low + scale (1 - 2^-52) >= high; a sample is
+    (float with mantissa bits >> 12 in [1, 2) - 1) scale + low."""
+    import math
+    import struct
+    scale = high - low
+    while scale * (1.0 - 2.0**-52) + low >= high:
+        scale = math.nextafter(scale, 0.0)
+    v12 = struct.unpack("<d", struct.pack("<Q", (int(bits) >> 12) | 0x3FF0000000000000))[0]
+    return (v12 - 1.0) * scale + low
+
+
+class OracleQuickCompress:
+    """QuickCompress::apply (hoomd-mc/src/quick_compress.rs:19-444) over
+    OracleSystem microstates: the trial microstate is a new OracleSystem on
+    the scaled edges (Hypercuboid::scale_volume, cuboid.rs:514-518) holding
+    the bodies mapped by map_points (clone_with_boundary, microstate.rs:
+    1529-1565); delta_energy_total follows pairwise_cutoff.rs:793-840."""
+
+    def __init__(self, target_volume, maximum_energy_per_site=1000.0, maximum_delta=0.05):
+        self.target = float(target_volume)
+        self.max_e = float(maximum_energy_per_site)
+        self.max_delta = float(maximum_delta)
+        self.state = "startup"
+        self.is_final = False
+        self.boundary = None
+
+    def apply(self, sysm, L, seed, model="hard_sphere", params=(1.0,)):
+        """Returns the (possibly new) system and its edges; the counter's
+        substep advances by one."""
+        step, sub = sysm.counter()
+        bits = counter_stream(step, sub, seed, 0, 1)[0]
+        sysm.set_counter(step, sub + 1)
+        L = [float(x) for x in L]
+        if self.state == "startup":
+            self.state, self.is_final, self.boundary = "waiting", False, L
+        elif self.state == "complete":
+            if self.boundary != L:
+                self.state, self.is_final, self.boundary = "waiting", False, L
+        elif self.state == "waiting":
+            if sysm.total_energy() <= 0.0:
+                if self.is_final and self.boundary == L:
+                    self.state, self.boundary = "complete", L
+                else:
+                    self.state = "compressing"
+        else:
+            delta = uniform_f64(bits, 0.0, self.max_delta)
+            current = 1.0
+            for x in L:
+                current *= x
+            if self.target > current:
+                trial = min(current * (1.0 + delta), self.target)
+            else:
+                trial = max(current * (1.0 - delta), self.target)
+            f = (trial / current) ** (1.0 / len(L))
+            L2 = [x * f for x in L]
+            body, _site = sysm.positions()
+            mapped, n_out = map_points(L, L2, body)
+            assert n_out == 0
+            new = OracleSystem(L2, mapped, model=model, params=params, seed=sysm.seed, step=step)
+            new.set_counter(step, sub + 1)
+            ef = new.total_energy()
+            if ef == float("inf"):
+                de = ef
+            else:
+                ei = 0.0 if model == "hard_sphere" else sysm.total_energy()
+                de = -ei if ei == float("inf") else ef - ei
+            if not de > self.max_e * len(body):
+                self.state, self.is_final, self.boundary = "waiting", trial == self.target, L2
+                return new, L2
+        return sysm, L
