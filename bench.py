@@ -220,6 +220,40 @@ def other_workloads(hmc, wl):
     res["C5_replicas_256"]["fp64_value"] = v64
     res["C5_replicas_256"]["mixed_over_fp64"] = res["C5_replicas_256"]["value"] / v64
     batch.close()
+    # Placement (benchmarks/src/place.rs:75-99 after QuickInsert): FCC 4*63^3
+    # spheres at packing fraction 0.45 compressed to 0.50 by QuickCompress under
+    # the overlap penalty (OverlapPenalty::default(), delta 1; FP64), one
+    # QuickCompress call every 20 steps and a translate sweep (d 0.1) each step
+    L, pos = wl.fcc_spheres(packing_fraction=0.45)
+    n = len(pos)
+    vt = n * (np.pi / 6.0) / 0.50
+    sim = hmc.GpuParallelSweep(L, wl.wrap(L, pos), model="overlap_penalty")
+    qc = hmc.QuickCompress(vt)
+    qc_s = sweep_s = 0.0
+    calls = trials = 0
+    step = 0
+    t0 = time.perf_counter()
+    while not qc.is_complete() and step < 1000:
+        sub = 0
+        if step % 20 == 0:
+            t1 = time.perf_counter()
+            qc.apply(sim, step, 0, 42)
+            qc_s += time.perf_counter() - t1
+            calls += 1
+            sub = 1
+        t1 = time.perf_counter()
+        a, r, _ = sim.apply(step, sub, 42, 1.0, 0.1)
+        sweep_s += time.perf_counter() - t1
+        trials += a + r
+        step += 1
+    total = time.perf_counter() - t0
+    res["placement_quick_compress_1M"] = {
+        "bodies": n, "from_packing_fraction": 0.45, "to_packing_fraction": 0.50,
+        "complete": qc.is_complete(), "steps": step, "seconds": total,
+        "final_packing_fraction": n * (np.pi / 6.0) / float(np.prod(sim.box_edges)),
+        "quick_compress_calls": calls, "ms_per_quick_compress_call": qc_s / calls * 1e3,
+        "overlap_penalty_sweep_trial_moves_per_s": trials / sweep_s, "ambiguous": qc.ambiguous}
+    sim.close()
     return res
 
 
