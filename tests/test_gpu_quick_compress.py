@@ -47,6 +47,7 @@ def bits(a):
     (3, 400, 0.42, 0.6, "overlap_penalty", P, 20, 0.1, 3000),
     (2, 300, 0.30, 0.45, "hard_sphere", (1.0,), 5, 0.15, 600),
     (3, 500, 0.30, 0.25, "hard_sphere", (1.0,), 10, 0.1, 400),  # expansion
+    (3, 500, 0.55, 0.70, "lj_shifted", (1.0, 1.0, 2.5, 2.5), 10, 0.1, 300),  # negative energies
 ])
 def test_quick_compress_matches_oracle(hmc, wl, dim, n, rho0, rho_t, model, params, every, d, max_steps):
     log(f"QuickCompress: D={dim} N={n} {rho0} -> {rho_t} {model}")
@@ -55,7 +56,8 @@ def test_quick_compress_matches_oracle(hmc, wl, dim, n, rho0, rho_t, model, para
     seed = 5
     ref = O.OracleSystem(L, wl.random_dilute(n, L, dim, 1.0, 3), model=model, params=params, seed=seed)
     b, s = ref.positions()
-    gpu = hmc.GpuParallelSweep(L, b, sites=s, model=model, params=params)
+    precision = "mixed" if model == "lj_shifted" else "fp64"
+    gpu = hmc.GpuParallelSweep(L, b, sites=s, model=model, params=params, precision=precision)
     oqc = O.OracleQuickCompress(vt)
     gqc = hmc.QuickCompress(vt)
     compressions = 0
