@@ -42,6 +42,8 @@ def parse():
                     help="mixed precision: one launch per colour pass, or one persistent launch per apply")
     ap.add_argument("--profile-period", type=int, default=8,
                     help="time the kernels of every k-th timed apply with HIP events")
+    ap.add_argument("--placement-only", action="store_true",
+                    help="only run the QuickCompress placement of 1M spheres and print its figures")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (HBM counter calibration)")
     return ap.parse_args()
@@ -220,10 +222,15 @@ def other_workloads(hmc, wl):
     res["C5_replicas_256"]["fp64_value"] = v64
     res["C5_replicas_256"]["mixed_over_fp64"] = res["C5_replicas_256"]["value"] / v64
     batch.close()
-    # Placement (benchmarks/src/place.rs:75-99 after QuickInsert): FCC 4*63^3
-    # spheres at packing fraction 0.45 compressed to 0.50 by QuickCompress under
-    # the overlap penalty (OverlapPenalty::default(), delta 1; FP64), one
-    # QuickCompress call every 20 steps and a translate sweep (d 0.1) each step
+    res["placement_quick_compress_1M"] = placement(hmc, wl)
+    return res
+
+
+def placement(hmc, wl):
+    """Placement (benchmarks/src/place.rs:75-99 after QuickInsert): FCC 4*63^3
+    spheres at packing fraction 0.45 compressed to 0.50 by QuickCompress under
+    the overlap penalty (OverlapPenalty::default(), delta 1; FP64), one
+    QuickCompress call every 20 steps and a translate sweep (d 0.1) each step."""
     L, pos = wl.fcc_spheres(packing_fraction=0.45)
     n = len(pos)
     vt = n * (np.pi / 6.0) / 0.50
@@ -247,7 +254,7 @@ def other_workloads(hmc, wl):
         trials += a + r
         step += 1
     total = time.perf_counter() - t0
-    res["placement_quick_compress_1M"] = {
+    res = {
         "bodies": n, "from_packing_fraction": 0.45, "to_packing_fraction": 0.50,
         "complete": qc.is_complete(), "steps": step, "seconds": total,
         "final_packing_fraction": n * (np.pi / 6.0) / float(np.prod(sim.box_edges)),
@@ -281,6 +288,9 @@ def main():
     import _pkg
     hmc = _pkg.pkg()
     wl = _pkg.workloads()
+    if args.placement_only:
+        print(json.dumps({"placement_quick_compress_1M": placement(hmc, wl)}), flush=True)
+        return
     kT, d = 1.0, 0.08
     L, pos = wl.fcc_spheres((63, 63, 63), packing_fraction=0.50)
     n = len(pos)

@@ -3339,20 +3339,39 @@ __global__ void __launch_bounds__(kEnergyBlock) k_energy(Geometry g, const Apply
   }
 }
 
-// Sum of the block partials in block order (one thread; the partial count is small).
-__global__ void k_energy_final(const double* part, uint32_t nblocks, double* out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Sum of the block partials by one workgroup of kEnergyFinal threads: thread t
+// adds partials t, t + kEnergyFinal, ... in order, then a fixed tree in LDS
+// (a deterministic order; the energy's bound holds for any order of its terms).
+constexpr int kEnergyFinal = 256;
+__global__ void __launch_bounds__(kEnergyFinal) k_energy_final(const double* part, uint32_t nblocks, double* out) {
+  __shared__ double r[4][kEnergyFinal];
   double e = 0.0, a = 0.0, n = 0.0, inf = 0.0;
-  for (uint32_t b = 0; b < nblocks; ++b) {
+  for (uint32_t b = threadIdx.x; b < nblocks; b += kEnergyFinal) {
     e += part[4 * b];
     a += part[4 * b + 1];
     n += part[4 * b + 2];
     inf = fmax(inf, part[4 * b + 3]);
   }
-  out[0] = e;
-  out[1] = a;
-  out[2] = n;
-  out[3] = inf;
+  r[0][threadIdx.x] = e;
+  r[1][threadIdx.x] = a;
+  r[2][threadIdx.x] = n;
+  r[3][threadIdx.x] = inf;
+  __syncthreads();
+  for (int h = kEnergyFinal / 2; h >= 1; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      r[0][threadIdx.x] += r[0][threadIdx.x + h];
+      r[1][threadIdx.x] += r[1][threadIdx.x + h];
+      r[2][threadIdx.x] += r[2][threadIdx.x + h];
+      r[3][threadIdx.x] = fmax(r[3][threadIdx.x], r[3][threadIdx.x + h]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = r[0][0];
+    out[1] = r[1][0];
+    out[2] = r[2][0];
+    out[3] = r[3][0];
+  }
 }
 
 // ------------------------------------------------ scale distribution function --
@@ -5016,7 +5035,7 @@ int hmc_total_energy(hmc_ctx* c, double* energy, double* bound) {
     k_energy<2><<<nblk, kEnergyBlock, 0, s>>>(g, c->st, c->bodies, c->w, part);
   else
     k_energy<3><<<nblk, kEnergyBlock, 0, s>>>(g, c->st, c->bodies, c->w, part);
-  k_energy_final<<<1, 64, 0, s>>>(part, nblk, part + 4 * (size_t)nblk);
+  k_energy_final<<<1, kEnergyFinal, 0, s>>>(part, nblk, part + 4 * (size_t)nblk);
   double out[4] = {0, 0, 0, 0};
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(out, part + 4 * (size_t)nblk, sizeof out, hipMemcpyDeviceToHost, s);
