@@ -43,20 +43,21 @@ def bits(a):
     return np.ascontiguousarray(a).view(np.uint64)
 
 
-@pytest.mark.parametrize("dim,n,rho0,rho_t,model,params,every,d,max_steps", [
-    (3, 400, 0.42, 0.6, "overlap_penalty", P, 20, 0.1, 3000),
-    (2, 300, 0.30, 0.45, "hard_sphere", (1.0,), 5, 0.15, 600),
-    (3, 500, 0.30, 0.25, "hard_sphere", (1.0,), 10, 0.1, 400),  # expansion
-    (3, 500, 0.55, 0.70, "lj_shifted", (1.0, 1.0, 2.5, 2.5), 10, 0.1, 300),  # negative energies
+@pytest.mark.parametrize("dim,n,rho0,rho_t,model,params,every,d,max_steps,precision", [
+    (3, 400, 0.42, 0.6, "overlap_penalty", P, 20, 0.1, 3000, "fp64"),
+    (2, 300, 0.30, 0.45, "hard_sphere", (1.0,), 5, 0.15, 600, "fp64"),
+    (2, 300, 0.30, 0.45, "hard_sphere", (1.0,), 5, 0.15, 600, "mixed"),  # persistent apply
+    (3, 500, 0.30, 0.25, "hard_sphere", (1.0,), 10, 0.1, 400, "mixed"),  # expansion
+    (3, 500, 0.55, 0.70, "lj_shifted", (1.0, 1.0, 2.5, 2.5), 10, 0.1, 300, "mixed"),  # negative energies
 ])
-def test_quick_compress_matches_oracle(hmc, wl, dim, n, rho0, rho_t, model, params, every, d, max_steps):
-    log(f"QuickCompress: D={dim} N={n} {rho0} -> {rho_t} {model}")
+def test_quick_compress_matches_oracle(hmc, wl, dim, n, rho0, rho_t, model, params, every, d, max_steps,
+                                       precision):
+    log(f"QuickCompress: D={dim} N={n} {rho0} -> {rho_t} {model} {precision}")
     L = [(n / rho0) ** (1.0 / dim)] * dim
     vt = n / rho_t
     seed = 5
     ref = O.OracleSystem(L, wl.random_dilute(n, L, dim, 1.0, 3), model=model, params=params, seed=seed)
     b, s = ref.positions()
-    precision = "mixed" if model == "lj_shifted" else "fp64"
     gpu = hmc.GpuParallelSweep(L, b, sites=s, model=model, params=params, precision=precision)
     oqc = O.OracleQuickCompress(vt)
     gqc = hmc.QuickCompress(vt)
@@ -77,7 +78,8 @@ def test_quick_compress_matches_oracle(hmc, wl, dim, n, rho0, rho_t, model, para
         oa, orj = ref.parallel_apply(1.0, d)
         assert (ga, gr) == (oa, orj), step
         ref.increment_step()
-    log(f"  {compressions} box changes, state {oqc.state}, density {n / np.prod(L):.4f}")
+    log(f"  {compressions} box changes, state {oqc.state}, density {n / np.prod(L):.4f}, "
+        f"layout {gpu.stats()['tiled']}")
     assert compressions > 0 and gqc.ambiguous == 0
     gb, gs = gpu.positions()
     rb, rs = ref.positions()
