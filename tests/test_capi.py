@@ -43,6 +43,8 @@ def _cfg(**kw):
     c.spatial_cell_width = kw.get("cell_width", 1.0)
     c.model = kw.get("model", 0)
     c.params[0] = kw.get("diameter", 1.0)
+    for i, v in enumerate(kw.get("params", ())):
+        c.params[i] = v
     c.precision = kw.get("precision", 0)
     c.device = 0
     return c
@@ -57,6 +59,10 @@ def _cfg(**kw):
     (dict(body_range=0.5), -1),                        # checkerboard narrower than the interaction
     (dict(L=(1.5, 10.0, 10.0), boundary_range=0.7, diameter=0.7, body_range=0.8), -1),  # < 2 spaces
     (dict(cell_width=0.0), -1),                        # the microstate's VecCell width is required
+    (dict(model=4, params=(1e4, 0.2, 100.0, 1.0), precision=1), -2),  # overlap penalty: FP64 only
+    (dict(model=4, params=(-1.0, 0.2, 100.0, 1.0)), -1),              # k < 0
+    (dict(model=4, params=(1e4, 0.2, float("nan"), 1.0)), -1),        # epsilon_shoulder not finite
+    (dict(model=4, params=(1e4, 0.2, 100.0, 1.5)), -1),               # delta = r_cut beyond the ranges
 ])
 def test_create_rejects(kw, code):
     pkg = _pkg()
