@@ -264,11 +264,11 @@ def placement(hmc, wl):
     return res
 
 
-def cpu_baseline(L, pos, seconds, seed, kT, d, threads=None):
+def cpu_baseline(L, pos, seconds, seed, kT, d, threads=None, model="hard_sphere", params=(1.0,)):
     import _oracle as O
     threads = threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
     O.set_threads(threads)
-    sys_ = O.OracleSystem(L, pos, seed=seed)
+    sys_ = O.OracleSystem(L, pos, model=model, params=params, seed=seed)
     sys_.time_applies(True, kT, d, 0.5)  # warm caches
     moves, elapsed, applies = sys_.time_applies(True, kT, d, seconds)
     return dict(value=moves / elapsed, unit="trial moves/s", cores=threads, kind="port",
@@ -364,6 +364,16 @@ def main():
         out["gpu_over_cpu"] = value / cb["value"]
         # the same restatement on one core (a shorter sample)
         out["cpu_baseline_1core"] = cpu_baseline(L, pos, args.cpu_seconds / 3.0, args.seed, kT, d, threads=1)
+        if not args.no_other:
+            # the overlap-penalty sweeps of the placement (FCC 4*63^3 at packing
+            # fraction 0.45, d 0.1) on the same host cores, a shorter sample
+            Lp, pp = wl.fcc_spheres(packing_fraction=0.45)
+            pc = cpu_baseline(Lp, wl.wrap(Lp, pp), args.cpu_seconds / 3.0, args.seed, 1.0, 0.1,
+                              model="overlap_penalty", params=(1e4, 0.2, 100.0, 1.0))
+            pc["sample"] = pc["sample"].replace("ParallelSweep applies", "overlap-penalty ParallelSweep applies")
+            pl = out["other_workloads"]["placement_quick_compress_1M"]
+            pl["cpu_baseline_sweeps"] = pc
+            pl["sweeps_gpu_over_cpu"] = pl["overlap_penalty_sweep_trial_moves_per_s"] / pc["value"]
     print(json.dumps(out), flush=True)
 
 
