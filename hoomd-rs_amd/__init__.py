@@ -30,7 +30,7 @@ HMC_MIXED = 1
 
 # every entry point declared in include/hmc.h
 EXPORTS = ["hmc_create", "hmc_set_state", "hmc_get_state", "hmc_clone_with_boundary", "hmc_apply_translate",
-           "hmc_quick_compress_init", "hmc_quick_compress_apply",
+           "hmc_quick_compress_init", "hmc_quick_compress_apply", "hmc_delta_energy_insert",
            "hmc_apply_rotate",
            "hmc_set_orientations", "hmc_get_orientations", "hmc_run_translate", "hmc_total_energy",
            "hmc_trace_enable", "hmc_trace_read", "hmc_get_stats", "hmc_profile_enable", "hmc_profile_read",
@@ -89,6 +89,7 @@ def lib():
         L.hmc_set_state.argtypes = [vp, D_P, D_P]
         L.hmc_get_state.argtypes = [vp, D_P, D_P]
         L.hmc_clone_with_boundary.argtypes = [vp, D_P, C.POINTER(vp)]
+        L.hmc_delta_energy_insert.argtypes = [vp, D_P, C.c_uint64, D_P, D_P]
         L.hmc_quick_compress_init.argtypes = [C.POINTER(QuickCompressState), C.c_double]
         L.hmc_quick_compress_init.restype = None
         L.hmc_quick_compress_apply.argtypes = [C.POINTER(QuickCompressState), C.POINTER(vp), C.c_uint64,
@@ -225,6 +226,17 @@ class GpuParallelSweep:
 
     def __del__(self):
         self.close()
+
+    def delta_energy_insert(self, bodies):
+        """(energy, bound) arrays: PairwiseCutoff::delta_energy_insert
+        (pairwise_cutoff.rs:1017-1025) of each candidate body position alone
+        against the current state, with a bound on the summation-order
+        difference."""
+        b = np.ascontiguousarray(bodies, np.float64).reshape(-1, self.dim)
+        e = np.zeros(len(b))
+        bd = np.zeros(len(b))
+        self._check(lib().hmc_delta_energy_insert(self.h, _p(b), len(b), _p(e), _p(bd)))
+        return e, bd
 
     def clone_with_boundary(self, box_edges):
         """Microstate::clone_with_boundary(new_boundary, |_| true)

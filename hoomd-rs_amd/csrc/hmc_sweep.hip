@@ -3374,6 +3374,75 @@ __global__ void __launch_bounds__(kEnergyFinal) k_energy_final(const double* par
   }
 }
 
+// PairwiseCutoff::delta_energy_insert (pairwise_cutoff.rs:1017-1025 with
+// :420-488): the energy of a new body's site wrap(body + 0) with every site of
+// the microstate (the new body has no tag yet, so nothing is filtered),
+// infinite on a hard overlap or an infinite pair term.  One thread per
+// candidate over the 3^D (or 5^D) neighbour spaces of the current cells;
+// out[2 k] is the energy, out[2 k + 1] the bound 2 gamma_n sum |terms| on its
+// difference from the reference's summation order.
+template <int D>
+__global__ void __launch_bounds__(kEnergyBlock) k_insert_energy(Geometry g, const ApplyState* st, Bodies B, Work w,
+                                                                const double* __restrict__ cand, uint32_t n,
+                                                                double* __restrict__ out, uint32_t* outside) {
+  const uint32_t k = blockIdx.x * kEnergyBlock + threadIdx.x;
+  if (k >= n) return;
+  double t[3] = {0.0, 0.0, 0.0};
+  for (int a = 0; a < D; ++a) t[a] = wrap1(cand[(size_t)k * D + a] + 0.0, g.L[a]);
+  const uint32_t s = space_of<D>(g, st->origin, t);
+  if (s == UINT32_MAX) {  // outside the checkerboard: reported as an error by the host
+    atomicOr(outside, 1u);
+    out[2 * (size_t)k] = 0.0;
+    out[2 * (size_t)k + 1] = 0.0;
+    return;
+  }
+  int32_t m[3] = {0, 0, 0};
+  for (int ax = 0; ax < D; ++ax) m[ax] = (int32_t)((s / g.stride[ax]) % (uint32_t)g.shape[ax]);
+  uint32_t nb[3][5];
+  int nn[3] = {1, 1, 1};
+  for (int ax = 0; ax < D; ++ax) {
+    const int32_t sh = g.shape[ax], r = g.reach[ax];
+    int kk = 0;
+    for (int off = -r; off <= r; ++off) {
+      const int32_t mm = ((m[ax] + off) % sh + sh) % sh;
+      bool seen = false;
+      for (int j = 0; j < kk; ++j) seen |= nb[ax][j] == (uint32_t)mm * g.stride[ax];
+      if (!seen) nb[ax][kk++] = (uint32_t)mm * g.stride[ax];
+    }
+    nn[ax] = kk;
+  }
+  double e = 0.0, a = 0.0, cnt = 0.0;
+  bool inf = false;
+  for (int i0 = 0; i0 < nn[0] && !inf; ++i0)
+    for (int i1 = 0; i1 < nn[1] && !inf; ++i1)
+      for (int i2 = 0; i2 < (D == 3 ? nn[2] : 1) && !inf; ++i2) {
+        const uint32_t ns = nb[0][i0] + nb[1][i1] + (D == 3 ? nb[2][i2] : 0u);
+        const uint32_t q0 = w.start[ns], q1 = w.start[ns + 1];
+        for (uint32_t q = q0; q < q1 && !inf; ++q) {
+          const double4 sj = B.spos[w.tmp_body[q]];
+          if (g.model == HMC_HARD_SPHERE) {
+            inf = overlap64<D>(g, t, sj);
+          } else {
+            const double r2 = lj_r2<D>(g, t, sj);
+            if (r2 < g.lj_rc2) {
+              const double one = lj_energy(g, r2);
+              if (one == INFINITY) {
+                inf = true;
+                break;
+              }
+              e += one;
+              a += fabs(one);
+              cnt += 1.0;
+            }
+          }
+        }
+      }
+  const double nt = fmax(cnt, 1.0);
+  const double gam = nt * 0x1p-53 / (1.0 - nt * 0x1p-53);
+  out[2 * (size_t)k] = inf ? INFINITY : e;
+  out[2 * (size_t)k + 1] = inf ? 0.0 : 2.0 * gam * a;
+}
+
 // ------------------------------------------------ scale distribution function --
 // The SDF of hard spheres/disks as HOOMD-blue's hpmc compute.SDF defines it
 // (no counterpart in the reference): for each body, the smallest compression
@@ -5003,6 +5072,72 @@ int hmc_sdf(hmc_ctx* c, double xmax, double dx, uint32_t nbins, uint64_t* hist, 
   if (rc) return rc;
   for (uint32_t k = 0; k < nbins; ++k) hist[k] = h[k];
   if (overlaps) *overlaps = h[nbins];
+  return HMC_OK;
+}
+
+int hmc_delta_energy_insert(hmc_ctx* c, const double* body_pos, uint64_t n, double* energy, double* bound) {
+  if (!c || (n && (!body_pos || !energy))) return HMC_ERR_INVALID;
+  if (!c->have_state) return fail(c, HMC_ERR_STATE, "hmc_set_state was not called");
+  const Geometry& g = c->g;
+  if (g.model == HMC_CONVEX_POLYHEDRON)
+    return fail(c, HMC_ERR_UNSUPPORTED, "insertion energies of oriented bodies are not offloaded");
+  if (n >= 0xffffffffull) return fail(c, HMC_ERR_INVALID, "too many candidates");
+  if (n == 0) return HMC_OK;
+  const int D = g.dim;
+  const uint32_t N = g.n_bodies;
+  if (N == 0) {  // no sites: every insertion energy is 0
+    for (uint64_t k = 0; k < n; ++k) {
+      energy[k] = 0.0;
+      if (bound) bound[k] = 0.0;
+    }
+    return HMC_OK;
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  double* dc = nullptr;
+  double* dout = nullptr;
+  uint32_t* flag = nullptr;
+  hipError_t e = hipMallocAsync((void**)&dc, (size_t)n * D * sizeof(double), s);
+  if (e == hipSuccess) e = hipMallocAsync((void**)&dout, 2 * (size_t)n * sizeof(double), s);
+  if (e == hipSuccess) e = hipMallocAsync((void**)&flag, sizeof(uint32_t), s);
+  if (e == hipSuccess) e = hipMemcpyAsync(dc, body_pos, (size_t)n * D * sizeof(double), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemsetAsync(flag, 0, sizeof(uint32_t), s);
+  if (e == hipSuccess) {
+    // cells of the current state, as for the total energy
+    k_setup<<<1, 64, 0, s>>>(g, c->st, 0, 0, 0, 1.0, 1.0, 0, 0, 0u);
+    if (D == 2)
+      k_bin<2><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
+    else
+      k_bin<3><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
+    k_scan_blocks<<<c->n_scan_blocks, 256, 0, s>>>(c->w.cnt, c->w.start, c->w.block_sums, c->w.block_ne, g.n_spaces);
+    k_scan_top<<<1, 1024, 0, s>>>(g, c->st, c->w.block_sums, c->w.block_ne, c->n_scan_blocks, c->w.start, 0);
+    k_scan_add<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.start, c->w.block_sums, c->w.cnt, g.n_spaces);
+    k_scatter<<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->w);
+    if (D == 2)
+      k_insert_energy<2><<<grid_for(n, kEnergyBlock), kEnergyBlock, 0, s>>>(g, c->st, c->bodies, c->w, dc,
+                                                                           (uint32_t)n, dout, flag);
+    else
+      k_insert_energy<3><<<grid_for(n, kEnergyBlock), kEnergyBlock, 0, s>>>(g, c->st, c->bodies, c->w, dc,
+                                                                           (uint32_t)n, dout, flag);
+    e = hipGetLastError();
+  }
+  std::vector<double> hout(2 * (size_t)n);
+  uint32_t outside = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(hout.data(), dout, 2 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(&outside, flag, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (dc) (void)hipFreeAsync(dc, s);
+  if (dout) (void)hipFreeAsync(dout, s);
+  if (flag) (void)hipFreeAsync(flag, s);
+  if (e != hipSuccess) return fail(c, HMC_ERR_DEVICE, std::string("insertion energy: ") + hipGetErrorString(e));
+  ApplyState st;
+  int rc = read_state(c, &st);
+  if (rc) return rc;
+  if (outside) return fail(c, HMC_ERR_COVER, "a candidate lies outside the checkerboard");
+  for (uint64_t k = 0; k < n; ++k) {
+    energy[k] = hout[2 * k];
+    if (bound) bound[k] = hout[2 * k + 1];
+  }
   return HMC_OK;
 }
 

@@ -238,6 +238,16 @@ int hmc_run_translate(hmc_ctx* ctx, uint64_t step0, uint32_t seed, double kT, do
  * two summation orders of the same n terms. */
 int hmc_total_energy(hmc_ctx* ctx, double* energy, double* bound);
 
+/* PairwiseCutoff::delta_energy_insert (hoomd-interaction/src/pairwise_cutoff.rs:1017-1025,
+ * with :420-488) of n candidate bodies (n x dim positions), each evaluated
+ * alone against the current microstate: the energy of the site wrap(body + 0)
+ * with every site, +inf on an overlap or an infinite term.  bound (may be
+ * NULL) receives, per candidate, a bound on the difference from the
+ * reference's summation order (0 when infinite); two calls may differ in the
+ * last bits within it.  The microstate is unchanged.  HMC_ERR_UNSUPPORTED for
+ * convex polyhedra (oriented bodies). */
+int hmc_delta_energy_insert(hmc_ctx* ctx, const double* body_pos, uint64_t n, double* energy, double* bound);
+
 /* Scale distribution function (SDF) of the current state, hard spheres and
  * disks: for every body the smallest compression x = 1 - sigma / r over the
  * other bodies (r the minimum-image site distance) at which the isotropically

@@ -263,6 +263,12 @@ struct System {
     return ef - ei;
   }
 
+  // pairwise_cutoff.rs:1017-1025: the new body has no tag, nothing is filtered
+  double delta_energy_insert(const Vec<D>& body) const {
+    Vec<D> si = boundary.wrap(add_site_local(body));
+    return site_energy(si, UINT64_MAX, false);
+  }
+
   // pairwise_cutoff.rs:720-745
   double total_energy() const {
     double total = 0.0;

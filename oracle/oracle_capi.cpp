@@ -343,6 +343,11 @@ double oracle_total_energy(void* h) {
   DISPATCH(h, e = S.total_energy());
   return e;
 }
+double oracle_delta_energy_insert(void* h, const double* body) {
+  auto* H = (Handle*)h;
+  if (H->dim == 2) return H->s2->delta_energy_insert({body[0], body[1]});
+  return H->s3->delta_energy_insert({body[0], body[1], body[2]});
+}
 double oracle_delta_energy_one(void* h, uint64_t body, const double* trial) {
   double e = 0;
   auto* H = (Handle*)h;

@@ -77,6 +77,8 @@ def lib():
         L.oracle_total_energy.restype = C.c_double
         L.oracle_delta_energy_one.argtypes = [C.c_void_p, C.c_uint64, D_P]
         L.oracle_delta_energy_one.restype = C.c_double
+        L.oracle_delta_energy_insert.argtypes = [C.c_void_p, D_P]
+        L.oracle_delta_energy_insert.restype = C.c_double
         L.oracle_trace_enable.argtypes = [C.c_void_p, C.c_int]
         L.oracle_trace_size.argtypes = [C.c_void_p]
         L.oracle_trace_size.restype = C.c_uint64
@@ -260,6 +262,11 @@ class OracleSystem:
     def delta_energy_one(self, body, trial):
         t = np.ascontiguousarray(trial, np.float64)
         return lib().oracle_delta_energy_one(self.h, body, _p(t, D_P))
+
+    def delta_energy_insert(self, body):
+        """PairwiseCutoff::delta_energy_insert of a new body at `body`."""
+        t = np.ascontiguousarray(body, np.float64)
+        return lib().oracle_delta_energy_insert(self.h, _p(t, D_P))
 
     def trace(self, on=True):
         lib().oracle_trace_enable(self.h, 1 if on else 0)

@@ -15,6 +15,7 @@ import _oracle as O
 import _pkg
 
 wl = _pkg.workloads()
+P = (1e4, 0.2, 100.0, 1.0)  # OverlapPenalty::default(), delta 1
 
 
 def two_body_energy(params, x, dim=3):
@@ -109,3 +110,17 @@ def test_parallel_sweep_relaxes_overlaps():
         s.parallel_apply(1.0, 0.1)
         s.increment_step()
     assert s.total_energy() < 0.2 * e0
+
+
+def test_delta_energy_insert_known_answers():
+    # pairwise_cutoff.rs:990-1012 (hard circle, diameter 1, a body at the
+    # origin): inserting at (0.4, 0) is infinite, at (1.5, 0) zero
+    h = O.OracleSystem([8.0, 8.0], np.zeros((1, 2)))
+    assert h.delta_energy_insert([0.4, 0.0]) == math.inf
+    assert h.delta_energy_insert([1.5, 0.0]) == 0.0
+    # the overlap penalty: one penalised pair, then beyond the maximum overlap
+    s = O.OracleSystem([8.0] * 3, np.array([[0.0, 0.0, 0.0], [3.0, 0.0, 0.0]]), model="overlap_penalty", params=P)
+    x = 0.875 - 1.0
+    assert s.delta_energy_insert([0.875, 0.0, 0.0]) == 0.5 * 1e4 * (x * x) + 100.0
+    assert s.delta_energy_insert([0.5, 0.0, 0.0]) == math.inf
+    assert s.delta_energy_insert([1.5, 0.0, 0.0]) == 0.0
