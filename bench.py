@@ -344,7 +344,6 @@ def main():
     out["fp32_d2_error"] = {"max_abs": dst["fp32_error_max"], "max_over_band_bound": dst["fp32_error_bound"]}
     diag.close()
     out["band_retests"] = mixed["stats"]["band_retests"]  # FP64 re-tests over warm-up + timed applies
-    out["exp_slow_path"] = mixed["stats"]["exp_slow_path"]
     if not args.no_fp64:
         f64 = run_gpu(hmc, L, body, site, "fp64", args.steps, args.warmup, args.seed, kT, d, args.profile_period)
         same = (f64["accepted"], f64["rejected"]) == (mixed["accepted"], mixed["rejected"])

@@ -65,7 +65,7 @@ class Count(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("applies", C.c_uint64), ("cycles", C.c_uint64), ("pair_tests", C.c_uint64),
-                ("band_retests", C.c_uint64), ("exp_slow_path", C.c_uint64), ("guard_cycles", C.c_uint64),
+                ("band_retests", C.c_uint64), ("guard_cycles", C.c_uint64),
                 ("tile_overflows", C.c_uint64), ("tiled", C.c_uint32), ("tile_shape", C.c_uint32 * 3),
                 ("ambiguous", C.c_uint64), ("fp32_error_max", C.c_double), ("fp32_error_bound", C.c_double)]
 

@@ -194,15 +194,15 @@ __device__ __forceinline__ bool convex_overlap(const double* vx, uint32_t nv, do
 }
 
 // VersorDisplacement::sample (hoomd-mc/src/rotate/versor.rs:39-72).
-__device__ __forceinline__ quat versor_displacement(Sfc64& rng, double sigma, const double* zx, const double* zf,
-                                                    unsigned* exp_slow) {
+__device__ __forceinline__ quat versor_displacement(Sfc64& rng, double sigma, const double* zx,
+                                                    const double* zf) {
   const double PI = 3.141592653589793;
   for (;;) {
     // three draws through one call site (a shift register keeps them in registers)
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll 1
     for (int i = 0; i < 3; ++i) {
-      const double z = 0.0 + sigma * standard_normal(rng, zx, zf, exp_slow);
+      const double z = 0.0 + sigma * standard_normal(rng, zx, zf);
       s0 = s1;
       s1 = s2;
       s2 = z;
@@ -211,8 +211,11 @@ __device__ __forceinline__ quat versor_displacement(Sfc64& rng, double sigma, co
     const double theta = sqrt(dot3(sv, sv));
     if (theta > PI) continue;
     const double half = 0.5 * theta;
-    const double w = hmc_cos_cr(half);
-    double f = hmc_sin_cr(half) / theta;
+    // half.cos() and half.sin() (versor.rs:59-61): one sincos call, as LLVM
+    // lowers the pair
+    double sn, w;
+    glibc_sincos(half, &sn, &w);
+    double f = sn / theta;
     if (!isfinite(f)) f = 0.5;
     const v3 v = {sv.x * f, sv.y * f, sv.z * f};
     const double mag = sqrt(w * w + dot3(v, v));

@@ -65,12 +65,11 @@ __device__ __forceinline__ uint32_t choose_index(Sfc64& r, uint32_t len) {
   return result;
 }
 
-// StandardNormal by the symmetric 256-layer ziggurat with the Marsaglia tail.
-// The wedge test uses an exact exp comparison; the tail uses a correctly
-// rounded log.  zx / zf: the layer tables (kZigX / kZigF or a copy of them in
-// LDS).  *ambiguous counts slow-path exp decisions.
-__device__ __forceinline__ double standard_normal(Sfc64& r, const double* zx, const double* zf,
-                                                  unsigned* ambiguous) {
+// StandardNormal by the symmetric 256-layer ziggurat with the Marsaglia tail
+// (rand_distr 0.6).  The tail's log and the wedge test's exp are glibc's,
+// as Rust's f64::ln / f64::exp call them on Linux.  zx / zf: the layer
+// tables (kZigX / kZigF or a copy of them in LDS).
+__device__ __forceinline__ double standard_normal(Sfc64& r, const double* zx, const double* zf) {
   for (;;) {
     uint64_t bits = r.next();
     int i = (int)(bits & 0xff);
@@ -85,14 +84,14 @@ __device__ __forceinline__ double standard_normal(Sfc64& r, const double* zx, co
       while (-2.0 * ty < tx * tx) {
         double x_ = open01(r);
         double y_ = open01(r);
-        tx = hmc_log_cr(x_) / R;
-        ty = hmc_log_cr(y_);
+        tx = glibc_log(x_) / R;
+        ty = glibc_log(y_);
       }
       return (u < 0.0) ? tx - R : R - tx;
     }
     double f0 = zf[i], f1 = zf[i + 1];
     double lhs = f1 + (f0 - f1) * uniform01(r);
-    if (hmc_lt_exp(lhs, -x * x / 2.0, ambiguous)) return x;
+    if (lhs < glibc_exp(-x * x / 2.0)) return x;
   }
 }
 

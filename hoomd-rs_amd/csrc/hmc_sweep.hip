@@ -113,13 +113,12 @@ struct ApplyState {
   uint32_t nonempty;
   uint32_t error;
   uint32_t move;      // 0 Translate, 1 Rotate<Versor>
-  uint32_t epoch;     // applies since creation
+  uint32_t epoch;     // applies since creation: the tag of the look-back scan's status words
   uint32_t pad;
   unsigned long long accepted_total;  // filled on the host from the slot counters
   unsigned long long trials_total;
   unsigned long long cycles_total;
   unsigned long long band_retests;
-  unsigned long long exp_slow;
   unsigned long long guard_cycles;
   unsigned long long tile_overflows;
   unsigned long long ambiguous;  // decisions the error bound could not settle
@@ -240,8 +239,12 @@ __device__ __forceinline__ void lane_add(unsigned long long* dst, unsigned v) {
 }
 
 // ----------------------------------------------------------------- kernels --
+// new_epoch: 1 for an apply (its build tags the look-back scan's status words
+// with the new epoch), 0 for the other users of the build (energies, SDF,
+// insertion energies), which scan with three kernels and leave the words and
+// the epoch alone: a word tagged by an older apply must never match.
 __global__ void k_setup(Geometry g, ApplyState* st, uint64_t step, uint32_t substep, uint32_t seed, double kT,
-                        double max_move, int from_args, int advance, uint32_t move) {
+                        double max_move, int from_args, int advance, uint32_t move, int new_epoch) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   st->move = move;
   if (from_args) {
@@ -262,7 +265,7 @@ __global__ void k_setup(Geometry g, ApplyState* st, uint64_t step, uint32_t subs
   }
   st->cycles = 0;
   st->nonempty = 0;
-  st->epoch += 1;
+  if (new_epoch) st->epoch += 1;
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
@@ -508,8 +511,7 @@ __global__ void __launch_bounds__(kBlock) k_place(Geometry g, Bodies B, Cells C,
 // Ball::sample (hoomd-vector/src/distribution.rs:39-70): the displacement of
 // Translate::propose (hoomd-mc/src/translate/cartesian.rs:39-52).
 template <int D>
-__device__ __forceinline__ void ball_sample(Sfc64& rng, double r, const double* zx, const double* zf, double* v,
-                                            unsigned* exp_slow) {
+__device__ __forceinline__ void ball_sample(Sfc64& rng, double r, const double* zx, const double* zf, double* v) {
   if (D == 2) {
     for (;;) {
       double c0 = uniform01(rng);
@@ -524,7 +526,7 @@ __device__ __forceinline__ void ball_sample(Sfc64& rng, double r, const double* 
     double p0 = 0.0, p1 = 0.0, p2 = 0.0;
 #pragma unroll 1
     for (int i = 0; i < 3; ++i) {
-      double z = standard_normal(rng, zx, zf, exp_slow);
+      double z = standard_normal(rng, zx, zf);
       p0 = p1;
       p1 = p2;
       p2 = z;
@@ -672,7 +674,6 @@ __device__ __forceinline__ bool pair_hits(const Geometry& g, const Bodies& B, co
 struct TrialOut {
   unsigned accepted;
   unsigned retests;
-  unsigned exp_slow;
 };
 
 template <int PREC>
@@ -745,9 +746,7 @@ __device__ __forceinline__ void trial(const Geometry& g, const ApplyState* st, c
   Sfc64 rng = counter_rng(st->step, substep, st->seed, s);
   const uint32_t pick = choose_index(rng, n);
   double v[3] = {0.0, 0.0, 0.0};
-  unsigned exp_slow = 0;
-  ball_sample<D>(rng, st->max_move, zx, zf, v, &exp_slow);
-  o.exp_slow = exp_slow;
+  ball_sample<D>(rng, st->max_move, zx, zf, v);
   const uint32_t slot = s0 + pick;
   uint32_t body = body0;
   double4 bp = bp0;
@@ -877,7 +876,7 @@ __global__ void __launch_bounds__(kBlock) k_pass(Geometry g, ApplyState* st, Bod
   const uint32_t b = blockIdx.x;
   const uint32_t lb = (b & 7u) * xcd_chunk + (b >> 3);
   const uint32_t t = lb * kBlock + threadIdx.x;
-  TrialOut o{0, 0, 0};
+  TrialOut o{0, 0};
   if (t < g.per_colour) {
     int32_t m[3] = {0, 0, 0};
     const uint32_t s = colour_space<D>(g, colour, t, m);
@@ -888,7 +887,6 @@ __global__ void __launch_bounds__(kBlock) k_pass(Geometry g, ApplyState* st, Bod
   const unsigned long long mk = __ballot(o.accepted != 0);
   if (__lane_id() == 0 && mk) atomicAdd(&blk_acc, (unsigned)__popcll(mk));
   lane_add(&st->band_retests, o.retests);
-  lane_add(&st->exp_slow, o.exp_slow);
   __syncthreads();
   if (threadIdx.x == 0 && blk_acc) atomicAdd(&w.acc_slots[b & (kAccSlots - 1)], (unsigned long long)blk_acc);
 }
@@ -1009,7 +1007,7 @@ __device__ __forceinline__ TrialOut trial_from_global(const Geometry& g, const A
                                                    uint32_t substep) {
   int32_t m[3] = {0, 0, 0};
   for (int a = 0; a < D; ++a) m[a] = (int32_t)((s / g.stride[a]) % (uint32_t)g.shape[a]);
-  TrialOut o{0, 0, 0};
+  TrialOut o{0, 0};
   trial<D, PREC, false>(g, st, B, C, w, zx, zf, s, m, s0, n, colour, substep, o);
   return o;
 }
@@ -1256,10 +1254,9 @@ __device__ __forceinline__ void pass_tile(const Geometry& g, ApplyState* st, con
   Sfc64 rng = counter_rng(step, substep, seed, s);
   uint32_t pick = 0;
   double v[3] = {0.0, 0.0, 0.0};
-  unsigned exp_slow = 0;
   if (go && !overflow) {
     pick = choose_index(rng, n);
-    ball_sample<D>(rng, max_move, zx, zf, v, &exp_slow);
+    ball_sample<D>(rng, max_move, zx, zf, v);
   }
   uint32_t body = pick == 0 ? ob0 : (pick == 1 ? ob1 : ob2);
   double4 bp = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -1285,7 +1282,7 @@ __device__ __forceinline__ void pass_tile(const Geometry& g, ApplyState* st, con
   __syncthreads();
 
   // (C) the trials
-  TrialOut o{0, 0, exp_slow};
+  TrialOut o{0, 0};
   double err_max = 0.0, err_ratio = 0.0;  // FP32_ERROR
   if (go && overflow) {
     o = trial_from_global<D, PREC>(g, st, B, C, w, zx, zf, s, s0, n, colour, substep);
@@ -1404,7 +1401,6 @@ __device__ __forceinline__ void pass_tile(const Geometry& g, ApplyState* st, con
   const unsigned long long mk = __ballot(o.accepted != 0);
   if (__lane_id() == 0 && mk) atomicAdd(&blk_acc, (unsigned)__popcll(mk));
   lane_add(&st->band_retests, o.retests);
-  lane_add(&st->exp_slow, o.exp_slow);
   if (tid == 0 && overflow) atomicAdd(&st->tile_overflows, 1ull);
   if constexpr (FP32_ERROR) {
     if (err_max > 0.0) {  // non-negative doubles order as their bits
@@ -1556,15 +1552,13 @@ __device__ __noinline__ TrialOut trial_global_dev(const Geometry* gp, ApplyState
                                                   const double* zx, const double* zf, uint32_t s, uint32_t s0,
                                                   uint32_t n, uint32_t colour, uint32_t substep) {
   const Geometry& g = *gp;
-  TrialOut o{0, 0, 0};
+  TrialOut o{0, 0};
   int32_t m[3] = {0, 0, 0};
   for (int a = 0; a < D; ++a) m[a] = (int32_t)((s / g.stride[a]) % (uint32_t)g.shape[a]);
   Sfc64 rng = counter_rng(st->step, substep, st->seed, s);
   const uint32_t pick = choose_index(rng, n);
   double v[3] = {0.0, 0.0, 0.0};
-  unsigned exp_slow = 0;
-  ball_sample<D>(rng, st->max_move, zx, zf, v, &exp_slow);
-  o.exp_slow = exp_slow;
+  ball_sample<D>(rng, st->max_move, zx, zf, v);
   const uint32_t slot = s0 + pick;
   const uint32_t body = cbody[slot];
   const double4 bp = bpos[body];
@@ -1834,7 +1828,7 @@ __global__ void __launch_bounds__(kTileThreads, 2) k_apply_tile(Geometry g, Appl
   __syncthreads();
 
 
-  unsigned retests = 0, exp_slow = 0;
+  unsigned retests = 0;
   // The proposal of pass i (cycle i / 2^D, colour i mod 2^D) depends only on
   // the counter, the space's body count and the chosen body's position, which
   // no other colour changes; it is drawn while the neighbour regions finish
@@ -1868,9 +1862,7 @@ __global__ void __launch_bounds__(kTileThreads, 2) k_apply_tile(Geometry g, Appl
     ppick = choose_index(rng, pn);
     pbody = (uint32_t)__float_as_int(lb[pe0 + ppick].w);
     pbp = B.bpos[pbody];
-    unsigned es = 0;
-    ball_sample<D>(rng, max_move, zx, zf, pv, &es);
-    exp_slow += es;
+    ball_sample<D>(rng, max_move, zx, zf, pv);
   };
 
   const uint32_t npass = cycles * NCOL;
@@ -1893,7 +1885,6 @@ __global__ void __launch_bounds__(kTileThreads, 2) k_apply_tile(Geometry g, Appl
                                                colour, substep);
         accepted = o.accepted;
         retests += o.retests;
-        exp_slow += o.exp_slow;
       }
     } else if (pgo) {
       // ---- the trial of this pass from LDS
@@ -2047,7 +2038,6 @@ __global__ void __launch_bounds__(kTileThreads, 2) k_apply_tile(Geometry g, Appl
     __syncthreads();
   }
   lane_add(&st->band_retests, retests);
-  lane_add(&st->exp_slow, exp_slow);
   if (tid == 0 && live && overflow) atomicAdd(&st->tile_overflows, 1ull);
   __syncthreads();
   if (tid == 0 && blk_acc) atomicAdd(&w.acc_slots[b & (kAccSlots - 1)], (unsigned long long)blk_acc);
@@ -2068,7 +2058,7 @@ __global__ void __launch_bounds__(kGuardThreads) k_guard(Geometry g, ApplyState*
     for (int c = 0; c < ncol; ++c) {
       for (uint32_t r = 0; r < rounds; ++r) {
         const uint32_t t = r * kGuardThreads + threadIdx.x;
-        TrialOut o{0, 0, 0};
+        TrialOut o{0, 0};
         if (t < g.per_colour) {
           int32_t m[3] = {0, 0, 0};
           const uint32_t s = colour_space<D>(g, (uint32_t)c, t, m);
@@ -2078,7 +2068,6 @@ __global__ void __launch_bounds__(kGuardThreads) k_guard(Geometry g, ApplyState*
         }
         wave_count(&w.acc_slots[0], o.accepted != 0);
         lane_add(&st->band_retests, o.retests);
-        lane_add(&st->exp_slow, o.exp_slow);
       }
       __syncthreads();
     }
@@ -2168,8 +2157,11 @@ __device__ __forceinline__ float lj_energy32(float sig2, float eps4, float shift
 // Metropolis decision of parallel_sweep.rs:267-273 for dH known to within
 // +-delta of the reference's value (same terms, other summation order):
 // accept iff dH <= 0 or u < exp(-dH / kT), u drawn only when dH > 0.
-// Returns 1 accept, 0 reject; *ambiguous is set when the interval
-// [dH - delta, dH + delta] admits both answers (then dH itself decides).
+// Returns 1 accept, 0 reject.  A u clear of exp(q) for every q of the
+// interval [dH - delta, dH + delta] is decided by the device's fast exp;
+// otherwise the reference's own comparison u < exp(-dH / kT), with glibc's
+// exp, decides at the GPU's dH, and *ambiguous is set when delta > 0 (the
+// reference's dH may then lie on the other side).
 __device__ __forceinline__ int lj_metropolis(Sfc64& rng, double dH, double delta, double kT, bool* ambiguous) {
   const double hi = dH + delta, lo = dH - delta;
   *ambiguous = false;
@@ -2185,15 +2177,14 @@ __device__ __forceinline__ int lj_metropolis(Sfc64& rng, double dH, double delta
     const double eb = exp(qb);
     if (u >= eb * (1.0 + 0x1p-45 * (1.0 + fabs(qb)))) return 0;
   }
-  *ambiguous = true;
+  *ambiguous = delta > 0.0;
   if (dH <= 0.0) return 1;
-  return u < hmc_exp_cr(-dH / kT) ? 1 : 0;
+  return u < glibc_exp(-dH / kT) ? 1 : 0;
 }
 
 struct LjOut {
   unsigned accepted;
   unsigned ambiguous;
-  unsigned exp_slow;
   unsigned retests;  // mixed precision: decisions the FP32 bound left open, re-taken in FP64
   double err, ratio; // with lj_fp32_error: |dH32 - dH64| and its ratio to the bound
 };
@@ -2363,9 +2354,7 @@ __device__ void trial_lj(const Geometry& g, const ApplyState* st, const Bodies& 
   const double4 bp = B.bpos[body];
   const double4 sp = B.spos[body];
   double v[3] = {0.0, 0.0, 0.0};
-  unsigned exp_slow = 0;
-  ball_sample<D>(rng, st->max_move, zx, zf, v, &exp_slow);
-  o.exp_slow = lane == 0 ? exp_slow : 0u;
+  ball_sample<D>(rng, st->max_move, zx, zf, v);
   double x[3] = {bp.x, bp.y, bp.z};
   for (int a = 0; a < D; ++a) x[a] += v[a];
   for (int a = 0; a < D; ++a) x[a] = wrap1(x[a], g.L[a]);  // generate_trial_moves: wrap(propose(..))
@@ -2462,7 +2451,7 @@ __global__ void __launch_bounds__(kBlock, 4) k_pass_lj(Geometry g, ApplyState* s
   const uint32_t b = blockIdx.x;
   const uint32_t lb = (b & 7u) * xcd_chunk + (b >> 3);
   const uint32_t t = lb * (kBlock / 64) + (threadIdx.x >> 6);  // one space per wavefront
-  LjOut o{0, 0, 0, 0, 0.0, 0.0};
+  LjOut o{0, 0, 0, 0.0, 0.0};
   if (t < g.per_colour) {
     int32_t m[3] = {0, 0, 0};
     const uint32_t s = colour_space<D>(g, colour, t, m);
@@ -2477,7 +2466,6 @@ __global__ void __launch_bounds__(kBlock, 4) k_pass_lj(Geometry g, ApplyState* s
     atomicMax(&st->fp32_err_max, (unsigned long long)__double_as_longlong(o.err));
     atomicMax(&st->fp32_err_ratio, (unsigned long long)__double_as_longlong(o.ratio));
   }
-  lane_add(&st->exp_slow, o.exp_slow);
   __syncthreads();
   if (threadIdx.x == 0 && blk_acc) atomicAdd(&w.acc_slots[b & (kAccSlots - 1)], (unsigned long long)blk_acc);
 }
@@ -2495,7 +2483,7 @@ __global__ void __launch_bounds__(kGuardThreads) k_guard_lj(Geometry g, ApplySta
   for (uint32_t k = first_cycle; k < cycles; ++k) {
     for (uint32_t c = 0; c < (1u << D); ++c) {
       for (uint32_t t = wid; t < g.per_colour; t += waves) {
-        LjOut o{0, 0, 0, 0, 0.0, 0.0};
+        LjOut o{0, 0, 0, 0.0, 0.0};
         int32_t m[3] = {0, 0, 0};
         const uint32_t s = colour_space<D>(g, c, t, m);
         const uint32_t s0 = w.start[s];
@@ -2503,7 +2491,6 @@ __global__ void __launch_bounds__(kGuardThreads) k_guard_lj(Geometry g, ApplySta
         if (n) trial_lj<D, HMC_FP64>(g, st, B, C, w, zx, zf, s, m, s0, n, c, st->substep0 + k, o, nullptr);
         if (o.accepted) atomicAdd(&w.acc_slots[0], 1ull);
         lane_add(&st->ambiguous, o.ambiguous);
-        lane_add(&st->exp_slow, o.exp_slow);
       }
       __syncthreads();
     }
@@ -2595,15 +2582,13 @@ __device__ void trial_convex(const Geometry& g, const ApplyState* st, const Bodi
   const double4 bq = B.bquat[body];
   double x[3] = {bp.x, bp.y, bp.z};
   quat q = {bq.x, {bq.y, bq.z, bq.w}};
-  unsigned exp_slow = 0;
   if (st->move == 0) {
     double v[3] = {0.0, 0.0, 0.0};
-    ball_sample<3>(rng, st->max_move, zx, zf, v, &exp_slow);
+    ball_sample<3>(rng, st->max_move, zx, zf, v);
     for (int a = 0; a < 3; ++a) x[a] += v[a];
   } else {
-    q = qmul(versor_displacement(rng, st->max_move, zx, zf, &exp_slow), q);
+    q = qmul(versor_displacement(rng, st->max_move, zx, zf), q);
   }
-  o.exp_slow = exp_slow;
   for (int a = 0; a < 3; ++a) x[a] = wrap1(x[a], g.L[a]);  // wrap(propose(..))
   bool accept = false;
   if (space_of<3>(g, st->origin, x) == s) {
@@ -2760,7 +2745,7 @@ __global__ void __launch_bounds__(kBlock, 3) k_pass_convex(Geometry g, ApplyStat
   const uint32_t substep = st->substep0 + cycle;
   // the trial of lane group k (all its lanes draw the same proposal)
   bool tried = false, active = false;
-  uint32_t s = 0, slot = 0xFFFFFFFFu, body = 0, exp_slow = 0;
+  uint32_t s = 0, slot = 0xFFFFFFFFu, body = 0;
   int32_t m[3] = {0, 0, 0};
   double x[3] = {0.0, 0.0, 0.0};
   quat q = {1.0, {0.0, 0.0, 0.0}};
@@ -2827,9 +2812,9 @@ __global__ void __launch_bounds__(kBlock, 3) k_pass_convex(Geometry g, ApplyStat
   quat dq = {1.0, {0.0, 0.0, 0.0}};
   if (tried) {
     if (st->move == 0)
-      ball_sample<3>(rng, st->max_move, zx, zf, v, &exp_slow);
+      ball_sample<3>(rng, st->max_move, zx, zf, v);
     else
-      dq = versor_displacement(rng, st->max_move, zx, zf, &exp_slow);
+      dq = versor_displacement(rng, st->max_move, zx, zf);
   }
   if (tried) {
     x[0] = bp.x;
@@ -2965,7 +2950,6 @@ __global__ void __launch_bounds__(kBlock, 3) k_pass_convex(Geometry g, ApplyStat
       }
     }
   }
-  lane_add(&st->exp_slow, kl == 0 ? exp_slow : 0u);
   __syncthreads();
   if (threadIdx.x == 0 && blk_acc) atomicAdd(&w.acc_slots[b & (kAccSlots - 1)], (unsigned long long)blk_acc);
 }
@@ -2985,7 +2969,7 @@ __global__ void __launch_bounds__(kGuardThreads) k_guard_convex(Geometry g, Appl
     for (uint32_t c = 0; c < 8; ++c) {
       for (uint32_t r = 0; r < rounds; ++r) {
         const uint32_t t = r * kGuardThreads + threadIdx.x;
-        TrialOut o{0, 0, 0};
+        TrialOut o{0, 0};
         if (t < g.per_colour) {
           int32_t m[3] = {0, 0, 0};
           const uint32_t s = colour_space<3>(g, c, t, m);
@@ -2994,7 +2978,6 @@ __global__ void __launch_bounds__(kGuardThreads) k_guard_convex(Geometry g, Appl
           if (n) trial_convex(g, st, B, C, w, zx, zf, vx, s, m, s0, n, c, st->substep0 + k, o);
         }
         wave_count(&w.acc_slots[0], o.accepted != 0);
-        lane_add(&st->exp_slow, o.exp_slow);
       }
       __syncthreads();
     }
@@ -3034,8 +3017,7 @@ __global__ void k_debug_convex(int kind, const double* vx, uint32_t nv, double r
   } else {
     const double* p = in + 5 * i;
     Sfc64 rng = counter_rng((uint64_t)p[0], (uint32_t)p[1], (uint32_t)p[2], (uint64_t)p[3]);
-    unsigned slow = 0;
-    const quat q = versor_displacement(rng, p[4], kZigX, kZigF, &slow);
+    const quat q = versor_displacement(rng, p[4], kZigX, kZigF);
     out[4 * i] = q.s;
     out[4 * i + 1] = q.v.x;
     out[4 * i + 2] = q.v.y;
@@ -3092,7 +3074,6 @@ __global__ void k_bsetup(const BatchRep* reps, ApplyState* sts, uint32_t R, cons
   }
   st->cycles = 0;
   st->nonempty = 0;
-  st->epoch += 1;
 }
 
 template <int D>
@@ -3166,11 +3147,10 @@ __global__ void __launch_bounds__(kBlock) k_bpass(const BatchRep* reps, ApplySta
   const uint32_t s0 = wr.start[s];
   const uint32_t n = wr.start[s + 1] - s0;
   if (!n) return;
-  TrialOut o{0, 0, 0};
+  TrialOut o{0, 0};
   trial<D, PREC, FAST>(g, st, B, C, wr, zx, zf, s, m, s0, n, colour, st->substep0 + cycle, o);
   if (o.accepted) atomicAdd(&w.acc_slots[r], 1ull);
   lane_add(&st->band_retests, o.retests);
-  lane_add(&st->exp_slow, o.exp_slow);
 }
 
 // One colour of one cycle of every replica on LDS tiles: block b takes tile
@@ -3218,7 +3198,7 @@ __global__ void __launch_bounds__(kGuardThreads) k_bguard(const BatchRep* reps, 
     for (int c = 0; c < (1 << D); ++c) {
       for (uint32_t q = 0; q < rounds; ++q) {
         const uint32_t t = q * kGuardThreads + threadIdx.x;
-        TrialOut o{0, 0, 0};
+        TrialOut o{0, 0};
         if (t < g.per_colour) {
           int32_t m[3] = {0, 0, 0};
           const uint32_t s = colour_space<D>(g, (uint32_t)c, t, m);
@@ -3228,7 +3208,6 @@ __global__ void __launch_bounds__(kGuardThreads) k_bguard(const BatchRep* reps, 
         }
         wave_count(&w.acc_slots[r], o.accepted != 0);
         lane_add(&st->band_retests, o.retests);
-        lane_add(&st->exp_slow, o.exp_slow);
       }
       __syncthreads();
     }
@@ -3607,11 +3586,19 @@ __global__ void k_debug_math(int kind, const double* in, double* out, uint64_t n
   double r = 0.0;
   switch (kind) {
     case 0: r = hmc_cbrt(in[i]); break;
-    case 1: r = hmc_log_cr(in[i]); break;
-    case 2: r = hmc_exp_cr(in[i]); break;
+    case 1: r = glibc_log(in[i]); break;
+    case 2: r = glibc_exp(in[i]); break;
     case 3: r = sqrt(in[i]); break;
-    case 5: r = hmc_sin_cr(in[i]); break;
-    case 6: r = hmc_cos_cr(in[i]); break;
+    case 5: {
+      double c;
+      glibc_sincos(in[i], &r, &c);
+      break;
+    }
+    case 6: {
+      double sn;
+      glibc_sincos(in[i], &sn, &r);
+      break;
+    }
     default: r = in[2 * i] / in[2 * i + 1]; break;
   }
   out[i] = r;
@@ -3621,8 +3608,7 @@ __global__ void k_debug_normals(uint64_t step, uint32_t substep, uint32_t seed, 
                                 uint64_t n) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   Sfc64 r = counter_rng(step, substep, seed, index);
-  unsigned slow = 0;
-  for (uint64_t i = 0; i < n; ++i) out[i] = standard_normal(r, kZigX, kZigF, &slow);
+  for (uint64_t i = 0; i < n; ++i) out[i] = standard_normal(r, kZigX, kZigF);
 }
 
 }  // namespace hmc
@@ -3711,7 +3697,7 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
     e0 = take_event(c);
     (void)hipEventRecord(e0, s);
   }
-  k_setup<<<1, 64, 0, s>>>(g, c->st, step, substep, seed, kT, max_move, from_args, advance, move);
+  k_setup<<<1, 64, 0, s>>>(g, c->st, step, substep, seed, kT, max_move, from_args, advance, move, 1);
   // (the per-space counts are zero: cleared at creation and by k_scan_add)
   if (N) k_bin<D><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
   if (N < (1u << kLookbackBits) && g.n_spaces < (1u << kLookbackBits)) {
@@ -5044,7 +5030,7 @@ int hmc_sdf(hmc_ctx* c, double xmax, double dx, uint32_t nbins, uint64_t* hist, 
     if (N) {
       // cells of the current state on the SDF's checkerboard (origin drawn
       // from the current counter; the next apply rebuilds its own)
-      k_setup<<<1, 64, 0, s>>>(g, c->st, 0, 0, 0, 1.0, 1.0, 0, 0, 0u);
+      k_setup<<<1, 64, 0, s>>>(g, c->st, 0, 0, 0, 1.0, 1.0, 0, 0, 0u, 0);
       if (g.dim == 2)
         k_bin<2><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
       else
@@ -5104,7 +5090,7 @@ int hmc_delta_energy_insert(hmc_ctx* c, const double* body_pos, uint64_t n, doub
   if (e == hipSuccess) e = hipMemsetAsync(flag, 0, sizeof(uint32_t), s);
   if (e == hipSuccess) {
     // cells of the current state, as for the total energy
-    k_setup<<<1, 64, 0, s>>>(g, c->st, 0, 0, 0, 1.0, 1.0, 0, 0, 0u);
+    k_setup<<<1, 64, 0, s>>>(g, c->st, 0, 0, 0, 1.0, 1.0, 0, 0, 0u, 0);
     if (D == 2)
       k_bin<2><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
     else
@@ -5157,7 +5143,7 @@ int hmc_total_energy(hmc_ctx* c, double* energy, double* bound) {
   double* part = nullptr;
   HIPCHK(c, hipMallocAsync((void**)&part, (4 * (size_t)nblk + 4) * sizeof(double), s));
   // cells for the neighbour search: the checkerboard of the current counter
-  k_setup<<<1, 64, 0, s>>>(g, c->st, 0, 0, 0, 1.0, 1.0, 0, 0, 0u);
+  k_setup<<<1, 64, 0, s>>>(g, c->st, 0, 0, 0, 1.0, 1.0, 0, 0, 0u, 0);
   if (g.dim == 2)
     k_bin<2><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
   else
@@ -5234,7 +5220,6 @@ int hmc_get_stats(hmc_ctx* c, hmc_stats* out) {
   out->cycles = s.cycles_total;
   out->pair_tests = 0;
   out->band_retests = s.band_retests;
-  out->exp_slow_path = s.exp_slow;
   out->guard_cycles = s.guard_cycles;
   out->tile_overflows = s.tile_overflows;
   out->ambiguous = s.ambiguous;
@@ -5678,7 +5663,6 @@ int hmc_batch_get_stats(hmc_batch* b, uint32_t replica, hmc_stats* out) {
   std::memset(out, 0, sizeof *out);
   out->cycles = s.cycles_total;
   out->band_retests = s.band_retests;
-  out->exp_slow_path = s.exp_slow;
   out->guard_cycles = s.guard_cycles;
   out->tile_overflows = s.tile_overflows;
   out->tiled = b->tiled ? 1u : 0u;

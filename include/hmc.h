@@ -125,7 +125,6 @@ typedef struct {
   uint64_t cycles;          /* colour cycles (= substeps consumed) */
   uint64_t pair_tests;      /* not maintained (0) */
   uint64_t band_retests;    /* MIXED: hard-sphere pairs, or LJ trials, re-tested in FP64 */
-  uint64_t exp_slow_path;   /* exp comparisons decided by the correctly rounded exp */
   uint64_t guard_cycles;    /* cycles run by the single-workgroup overflow guard */
   uint64_t tile_overflows;  /* LDS tiles whose box exceeded the staging capacity
                                (their trials ran from global memory) */
@@ -274,10 +273,11 @@ int hmc_profile_enable(hmc_ctx* ctx, int period);
 int hmc_profile_read(hmc_ctx* ctx, double* pass_ms_total, uint64_t* pass_launches, double* build_ms_total,
                      uint64_t* builds);
 
-/* Evaluate the library's device math on n inputs (kind 0 cbrt, 1 correctly
- * rounded log, 2 correctly rounded exp, 3 sqrt, 4 division in[2i]/in[2i+1],
- * 5 correctly rounded sin, 6 correctly rounded cos).
- * Test hook for bit-exactness checks of the device arithmetic. */
+/* Evaluate the library's device math on n inputs: kind 0 cbrt, 1 log,
+ * 2 exp, 5 the sine and 6 the cosine of one sincos call (the device's
+ * restatements of the glibc functions Rust's std calls on Linux), 3 sqrt,
+ * 4 division in[2i]/in[2i+1].  Test hook for bit-exactness checks of the
+ * device arithmetic against the host C library. */
 int hmc_debug_math(int device, int kind, const double* in, double* out, uint64_t n);
 
 /* Evaluate the device convex-polyhedron routines on n inputs: kind 0

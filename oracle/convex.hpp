@@ -12,9 +12,9 @@
 //   intersects_at_global -- pinned by the reference's boolean test tables
 //       (xenocollide.rs, convex_polytope.rs tests transcribed in the tests)
 //       and by an independent separating-axis test away from contact;
-//   VersorDisplacement -- follows rotate/versor.rs:39-72; its Normal and
-//       sin/cos come from rand_distr 0.6 / glibc: PARITY UNPINNED (as the
-//       other rand adapters), with sin/cos optionally correctly rounded.
+//   VersorDisplacement -- follows rotate/versor.rs:39-72; its Normal comes
+//       from rand_distr 0.6 (PARITY UNPINNED, as the other rand adapters) and
+//       its sin/cos from the host library's sincos (libm.hpp).
 #pragma once
 #include <cmath>
 #include <cstdint>
@@ -215,8 +215,9 @@ inline Quat versor_displacement(Sfc64& rng, double sigma) {
     const double theta = vnorm(s);
     if (theta > PI) continue;
     const double half = 0.5 * theta;
-    const double w = lib_cos(half);
-    double f = lib_sin(half) / theta;
+    double sn, w;
+    lib_sincos(half, &sn, &w);  // half.cos(), half.sin(): one sincos call
+    double f = sn / theta;
     if (!std::isfinite(f)) f = 0.5;
     const V3 v = vscale(s, f);
     const double mag = std::sqrt(w * w + dot<3>(v, v));

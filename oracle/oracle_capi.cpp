@@ -73,31 +73,22 @@ System<D>* make(const double* L, double boundary_range, double body_range, int m
 
 extern "C" {
 
-// ---- libm mode -----------------------------------------------------------------
-void oracle_set_libm(int cr) { libm().mode = cr; }
-void oracle_libm_counters(uint64_t* out) {
-  out[0] = libm().log_calls.load();
-  out[1] = libm().log_divergent.load();
-  out[2] = libm().exp_calls.load();
-  out[3] = libm().exp_divergent.load();
+// ---- host C library ---------------------------------------------------------
+// The library functions the reference calls, evaluated in bulk on the host:
+// kind 0 cbrt, 1 log, 2 exp, 5 and 6 the sine and cosine of one sincos call.
+void oracle_libm_eval(int kind, const double* in, double* out, uint64_t n) {
+  for (uint64_t i = 0; i < n; ++i) {
+    double s, c;
+    switch (kind) {
+      case 0: out[i] = std::cbrt(in[i]); break;
+      case 1: out[i] = std::log(in[i]); break;
+      case 2: out[i] = std::exp(in[i]); break;
+      case 5: lib_sincos(in[i], &s, &c); out[i] = s; break;
+      case 6: lib_sincos(in[i], &s, &c); out[i] = c; break;
+      default: out[i] = NAN; break;
+    }
+  }
 }
-void oracle_libm_reset() {
-  libm().log_calls = 0;
-  libm().log_divergent = 0;
-  libm().exp_calls = 0;
-  libm().exp_divergent = 0;
-  libm().trig_calls = 0;
-  libm().trig_divergent = 0;
-}
-// sin/cos calls, and those where glibc and the correctly rounded value differ
-void oracle_trig_counters(uint64_t* out) {
-  out[0] = libm().trig_calls.load();
-  out[1] = libm().trig_divergent.load();
-}
-double oracle_cr_log(double x) { return cr_log(x); }
-double oracle_cr_exp(double y) { return cr_exp(y); }
-double oracle_cr_sin(double x) { return cr_sin(x); }
-double oracle_cr_cos(double x) { return cr_cos(x); }
 
 // ---- RNG and adapter probes -------------------------------------------------
 void oracle_sfc64_stream(uint64_t a, uint64_t b, uint64_t c, uint64_t ctr, int warm, uint64_t* out, uint64_t n) {

@@ -321,7 +321,6 @@ def _convex_lib():
         L.oracle_convex_trace_read.argtypes = [C.c_void_p, C.c_void_p]
         L.oracle_convex_time.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, D_P, U64_P]
         L.oracle_convex_time.restype = C.c_uint64
-        L.oracle_trig_counters.argtypes = [U64_P]
         L._convex_ready = True
     return L
 
@@ -354,10 +353,20 @@ def versor_displacement(step, substep, seed, index, sigma, n):
     return out
 
 
-def trig_counters():
-    out = np.zeros(2, np.uint64)
-    _convex_lib().oracle_trig_counters(_p(out, U64_P))
-    return int(out[0]), int(out[1])
+LIBM_KINDS = {"cbrt": 0, "log": 1, "exp": 2, "sin": 5, "cos": 6}
+
+
+def libm_eval(kind, x):
+    """The host C library's cbrt / log / exp, or the sine or cosine of its
+    sincos, on every element of x (the functions the reference calls)."""
+    L = lib()
+    if not getattr(L, "_libm_ready", False):
+        L.oracle_libm_eval.argtypes = [C.c_int, D_P, D_P, C.c_uint64]
+        L._libm_ready = True
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.empty_like(x)
+    L.oracle_libm_eval(LIBM_KINDS[kind], _p(x, D_P), _p(out, D_P), x.size)
+    return out
 
 
 def sdf(L, positions, sigma, xmax, dx, nbins, all_pairs=False):

@@ -25,10 +25,7 @@ def log(msg):
 def hmc():
     p = _pkg.pkg()
     p.lib()
-    O.lib().oracle_set_libm.argtypes = [C.c_int]
-    O.lib().oracle_set_libm(1)
-    yield p
-    O.lib().oracle_set_libm(0)
+    return p
 
 
 @pytest.fixture(scope="module")
