@@ -1,0 +1,550 @@
+// The C ABI of a batch of replicas: creation, state, hipGraph-captured
+// steps, SDF and the tile autotuner.
+#include "host.hpp"
+
+namespace {
+
+int bfail(hmc_batch* b, int code, const std::string& msg) {
+  if (b) b->err = msg;
+  return code;
+}
+
+#define BHIPCHK(b, expr)                                                                     \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) return bfail((b), HMC_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// The launches of one step of every replica (ParallelSweep::apply with
+// Translate, then increment_step when advance is set).
+template <int D, int PREC>
+void launch_batch_step(hmc_batch* b, int from_args, int advance, uint32_t launched) {
+  hipStream_t s = b->stream;
+  k_bsetup<<<grid_for(b->R, 64), 64, 0, s>>>(b->reps, b->sts, b->R, b->dstep, b->dseed, b->kT, b->dmove, from_args,
+                                             advance);
+  k_fill_u32<<<grid_for(b->s_total, 256), 256, 0, s>>>(b->w.cnt, b->s_total, 0u);
+  k_bbin<D><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->reps, b->sts, b->dboff, b->R, b->n_total, b->bodies,
+                                                            b->w);
+  k_scan_blocks<<<b->n_scan_blocks, 256, 0, s>>>(b->w.cnt, b->w.start, b->w.block_sums, b->w.block_ne, b->s_total);
+  k_scan_top<<<1, 1024, 0, s>>>(b->gtot, b->sts + b->R, b->w.block_sums, b->w.block_ne, b->n_scan_blocks, b->w.start,
+                                0);
+  k_scan_add<<<grid_for(b->s_total, 256), 256, 0, s>>>(b->w.start, b->w.block_sums, nullptr, b->s_total);
+  k_bcycles<<<grid_for(b->R, 64), 64, 0, s>>>(b->reps, b->sts, b->R);
+  k_scatter<<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->gtot, b->w);
+  k_place<PREC><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->gtot, b->bodies, b->cells, b->w);
+  const uint32_t blocks = grid_for(b->pc_total, kBlock);
+  const uint32_t chunk = (blocks + 7) / 8;
+  const uint32_t tchunk = (b->ntiles + 7) / 8;
+  for (uint32_t k = 0; k < launched; ++k)
+    for (int col = 0; col < (1 << D); ++col) {
+      if (b->tiled)
+        k_bpass_tile<D, PREC><<<tchunk * 8, b->tile_threads, b->tile_lds, s>>>(
+            b->reps, b->dtcs, b->dtoff, b->dtrep, b->sts, b->ntiles, tchunk, b->bodies, b->cells, b->w,
+            (uint32_t)col, k);
+      else if (b->fast)
+        k_bpass<D, PREC, true><<<chunk * 8, kBlock, 0, s>>>(b->reps, b->sts, b->dcoff, b->R, b->pc_total, b->bodies,
+                                                            b->cells, b->w, (uint32_t)col, k, chunk);
+      else
+        k_bpass<D, PREC, false><<<chunk * 8, kBlock, 0, s>>>(b->reps, b->sts, b->dcoff, b->R, b->pc_total,
+                                                             b->bodies, b->cells, b->w, (uint32_t)col, k, chunk);
+    }
+  k_bguard<D, PREC><<<b->R, kGuardThreads, 0, s>>>(b->reps, b->sts, b->bodies, b->cells, b->w, launched);
+}
+
+void batch_step(hmc_batch* b, int from_args, int advance, uint32_t launched) {
+  if (b->dim == 2 && b->precision == HMC_FP64)
+    launch_batch_step<2, HMC_FP64>(b, from_args, advance, launched);
+  else if (b->dim == 2)
+    launch_batch_step<2, HMC_MIXED>(b, from_args, advance, launched);
+  else if (b->precision == HMC_FP64)
+    launch_batch_step<3, HMC_FP64>(b, from_args, advance, launched);
+  else
+    launch_batch_step<3, HMC_MIXED>(b, from_args, advance, launched);
+}
+
+int batch_read(hmc_batch* b, std::vector<ApplyState>& st, std::vector<unsigned long long>& acc) {
+  st.resize(b->R);
+  acc.resize(b->R);
+  BHIPCHK(b, hipMemcpyAsync(st.data(), b->sts, b->R * sizeof(ApplyState), hipMemcpyDeviceToHost, b->stream));
+  BHIPCHK(b, hipMemcpyAsync(acc.data(), b->w.acc_slots, b->R * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                            b->stream));
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  for (uint32_t r = 0; r < b->R; ++r)
+    if (st[r].error) return bfail(b, HMC_ERR_COVER, "a body lies outside the checkerboard (replica " + std::to_string(r) + ")");
+  return HMC_OK;
+}
+
+// Tile shapes of every replica for the LDS-tiled batch pass: choose_tiles
+// with `cus` compute units assumed per replica (1: the largest tiles that
+// fit), one workgroup size for all (the row-start walk re-stepped for it)
+// and each replica's FP32 band of the box-relative test.  False when a
+// replica has no tile shape.
+struct BatchTiles {
+  std::vector<TileCfg> tcs;
+  std::vector<uint32_t> toff;
+  uint32_t threads = 64, lds = 0;
+};
+
+bool batch_tiles(const hmc_batch* b, int cus, BatchTiles& bt) {
+  const uint32_t R = b->R;
+  const int D = b->dim;
+  const size_t vt = b->precision == HMC_MIXED ? sizeof(float4) : sizeof(double4);
+  bt.tcs.assign(R, TileCfg{});
+  bt.toff.assign(R + 1, 0);
+  bt.threads = 64;
+  bt.lds = 0;
+  for (uint32_t r = 0; r < R; ++r) {
+    const Geometry& g = b->hreps[r].g;
+    TileCfg& tc = bt.tcs[r];
+    if (!choose_tiles(g, D, (double)g.n_bodies / (double)g.n_spaces, vt, cus, &tc)) return false;
+    if (b->precision == HMC_MIXED) {
+      const double sigma = std::sqrt(g.sigma2);
+      double lmax = 0.0, Rb = 0.0;
+      for (int a = 0; a < D; ++a) {
+        lmax = std::max(lmax, g.L[a]);
+        Rb = std::max(Rb, 0.5 * tc.E[a] * g.width[a] + g.width[a]);
+      }
+      // as for a context's tiled pass
+      const double Et = (0x1p-24 * (5.1 * lmax + 2.0 * Rb + 2.0 * sigma) + 0x1p-53 * (1.5 * lmax + 2.0 * sigma)) * 1.01;
+      if (!contact_band(Et, sigma, g.sigma2, g.edge2, D, &tc.band_lo, &tc.band_hi)) return false;
+    }
+    bt.threads = std::max<uint32_t>(bt.threads, ((tc.trials + 63) / 64) * 64);
+    bt.lds = std::max<uint32_t>(bt.lds, tc.lds_bytes);
+    bt.toff[r + 1] = bt.toff[r] + tc.ntiles;
+  }
+  // the row-start walk steps by the batch's workgroup size
+  for (uint32_t r = 0; r < R; ++r) {
+    uint32_t q = bt.threads;
+    for (int a = D - 1; a > 0; --a) {
+      const uint32_t rad = a == D - 1 ? bt.tcs[r].E[a] + 2 : bt.tcs[r].E[a];
+      bt.tcs[r].step[a] = q % rad;
+      q /= rad;
+    }
+    bt.tcs[r].step[0] = q;
+  }
+  return true;
+}
+
+// Makes bt the batch's tile layout (the captured step graph is dropped).
+int batch_set_tiles(hmc_batch* b, const BatchTiles& bt) {
+  const uint32_t R = b->R;
+  const void* fn = b->dim == 2 ? (b->precision == HMC_MIXED ? (const void*)k_bpass_tile<2, HMC_MIXED>
+                                                             : (const void*)k_bpass_tile<2, HMC_FP64>)
+                               : (b->precision == HMC_MIXED ? (const void*)k_bpass_tile<3, HMC_MIXED>
+                                                             : (const void*)k_bpass_tile<3, HMC_FP64>);
+  if (!b->dtcs && (dalloc(&b->dtcs, R) != hipSuccess || dalloc(&b->dtoff, R + 1) != hipSuccess))
+    return HMC_ERR_DEVICE;
+  const uint32_t nt = bt.toff[R];
+  std::vector<uint32_t> trep(nt);
+  for (uint32_t r = 0; r < R; ++r)
+    for (uint32_t t = bt.toff[r]; t < bt.toff[r + 1]; ++t) trep[t] = r;
+  if (nt > b->dtrep_cap) {
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return HMC_ERR_DEVICE;
+    if (b->dtrep) (void)hipFree(b->dtrep);
+    b->dtrep = nullptr;
+    b->dtrep_cap = 0;
+    if (dalloc(&b->dtrep, nt) != hipSuccess) return HMC_ERR_DEVICE;
+    b->dtrep_cap = nt;
+  }
+  if (nt && hipMemcpyAsync(b->dtrep, trep.data(), nt * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream) !=
+                hipSuccess)
+    return HMC_ERR_DEVICE;
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bt.lds) != hipSuccess ||
+      hipMemcpyAsync(b->dtcs, bt.tcs.data(), R * sizeof(TileCfg), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+      hipMemcpyAsync(b->dtoff, bt.toff.data(), (R + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(b->stream) != hipSuccess)
+    return HMC_ERR_DEVICE;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->tiled = true;
+  b->ntiles = bt.toff[R];
+  b->tile_threads = bt.threads;
+  b->tile_lds = bt.lds;
+  return HMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ------------------------------------------------------------------ batches --
+int hmc_batch_create(uint32_t n_replicas, const hmc_config* cfgs, hmc_batch** out) {
+  if (!out || !cfgs || n_replicas < 1 || n_replicas > (uint32_t)kAccSlots) return HMC_ERR_INVALID;
+  *out = nullptr;
+  auto* b = new hmc_batch;
+  b->R = n_replicas;
+  b->device = cfgs[0].device;
+  b->dim = (int)cfgs[0].dim;
+  b->precision = cfgs[0].precision;
+  b->hreps.resize(n_replicas);
+  b->boff.assign(n_replicas + 1, 0);
+  b->coff.assign(n_replicas + 1, 0);
+  uint64_t nb = 0, ns = 0, npc = 0;
+  for (uint32_t r = 0; r < n_replicas; ++r) {
+    const hmc_config* cfg = cfgs + r;
+    if (cfg->model != HMC_HARD_SPHERE || (int)cfg->dim != b->dim || cfg->precision != b->precision ||
+        cfg->device != b->device) {
+      delete b;
+      return HMC_ERR_UNSUPPORTED;
+    }
+    bool fast = false;
+    double sigma = 0.0, lmax = 0.0;
+    BatchRep& rep = b->hreps[r];
+    std::memset(&rep, 0, sizeof rep);
+    const int rc = make_geometry(cfg, rep.g, &fast, &sigma, &lmax);
+    if (rc) {
+      delete b;
+      return rc;
+    }
+    b->fast = b->fast && fast;
+    rep.body_off = (uint32_t)nb;
+    rep.space_off = (uint32_t)ns;
+    rep.colour_off = (uint32_t)npc;
+    b->boff[r] = (uint32_t)nb;
+    b->coff[r] = (uint32_t)npc;
+    nb += rep.g.n_bodies;
+    ns += rep.g.n_spaces;
+    npc += rep.g.per_colour;
+  }
+  if (nb >= 0x7fffffffull || ns >= 0x7fffffffull || ns > (uint64_t)kScanBlock * 1024 * 16) {
+    delete b;
+    return HMC_ERR_UNSUPPORTED;
+  }
+  b->boff[n_replicas] = (uint32_t)nb;
+  b->coff[n_replicas] = (uint32_t)npc;
+  b->n_total = (uint32_t)nb;
+  b->s_total = (uint32_t)ns;
+  b->pc_total = (uint32_t)npc;
+  std::memset(&b->gtot, 0, sizeof b->gtot);
+  b->gtot.dim = b->dim;
+  b->gtot.n_bodies = b->n_total;
+  b->gtot.n_spaces = b->s_total;
+  if (hipSetDevice(b->device) != hipSuccess) {
+    delete b;
+    return HMC_ERR_DEVICE;
+  }
+  const uint32_t N = b->n_total, R = b->R;
+  b->n_scan_blocks = grid_for(b->s_total, kScanBlock);
+  bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess;
+  ok = ok && dalloc(&b->reps, R) == hipSuccess && dalloc(&b->sts, R + 1) == hipSuccess &&
+       dalloc(&b->dboff, R + 1) == hipSuccess && dalloc(&b->dcoff, R + 1) == hipSuccess &&
+       dalloc(&b->dstep, R) == hipSuccess && dalloc(&b->dseed, R) == hipSuccess && dalloc(&b->dmove, R) == hipSuccess;
+  ok = ok && dalloc(&b->bodies.bpos, N) == hipSuccess && dalloc(&b->bodies.spos, N) == hipSuccess &&
+       dalloc(&b->cells.body, N) == hipSuccess;
+  if (b->precision == HMC_MIXED)
+    ok = ok && dalloc(&b->cells.pos32, N) == hipSuccess;
+  else
+    ok = ok && dalloc(&b->cells.pos64, N) == hipSuccess;
+  ok = ok && dalloc(&b->w.cnt, b->s_total) == hipSuccess && dalloc(&b->w.start, b->s_total + 1) == hipSuccess &&
+       dalloc(&b->w.elem_space, N) == hipSuccess && dalloc(&b->w.elem_rank, N) == hipSuccess &&
+       dalloc(&b->w.tmp_body, N) == hipSuccess && dalloc(&b->w.block_sums, b->n_scan_blocks) == hipSuccess &&
+       dalloc(&b->w.block_ne, b->n_scan_blocks) == hipSuccess && dalloc(&b->w.acc_slots, kAccSlots) == hipSuccess;
+  if (ok)
+    for (uint32_t r = 0; r < R; ++r) b->hreps[r].g.self = &b->reps[r].g;
+  ok = ok && hipMemcpy(b->reps, b->hreps.data(), R * sizeof(BatchRep), hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(b->dboff, b->boff.data(), (R + 1) * sizeof(uint32_t), hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(b->dcoff, b->coff.data(), (R + 1) * sizeof(uint32_t), hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemset(b->sts, 0, (R + 1) * sizeof(ApplyState)) == hipSuccess &&
+       hipMemset(b->w.acc_slots, 0, kAccSlots * sizeof(unsigned long long)) == hipSuccess;
+  if (!ok) {
+    hmc_batch_destroy(b);
+    return HMC_ERR_DEVICE;
+  }
+  b->cycles_guess = 1;
+  for (uint32_t r = 0; r < R; ++r) {
+    const Geometry& g = b->hreps[r].g;
+    b->cycles_guess = std::max<uint32_t>(b->cycles_guess, (g.n_bodies + g.n_spaces - 1) / std::max<uint32_t>(g.n_spaces, 1));
+  }
+  // LDS tiles for every replica (else the lane-per-space pass): the largest
+  // tiles that fit (the batch fills the device by itself); hmc_batch_autotune
+  // may pick another shape later
+  if (b->fast && !(cfgs[0].flags & HMC_FLAG_GLOBAL_PASS)) {
+    BatchTiles bt;
+    if (batch_tiles(b, 1, bt) && batch_set_tiles(b, bt) != HMC_OK) {
+      hmc_batch_destroy(b);
+      return HMC_ERR_DEVICE;
+    }
+  }
+  *out = b;
+  return HMC_OK;
+}
+
+int hmc_batch_set_state(hmc_batch* b, uint32_t replica, const double* body_pos, const double* site_pos) {
+  if (!b || replica >= b->R) return HMC_ERR_INVALID;
+  const Geometry& g = b->hreps[replica].g;
+  const uint32_t N = g.n_bodies, off = b->boff[replica];
+  const int D = b->dim;
+  if (!body_pos && N) return HMC_ERR_INVALID;
+  std::vector<double4> hb(N), hs(site_pos ? N : 0);
+  for (uint32_t i = 0; i < N; ++i) {
+    double v[3] = {0, 0, 0}, u[3] = {0, 0, 0};
+    for (int a = 0; a < D; ++a) v[a] = body_pos[(size_t)i * D + a];
+    hb[i] = make_double4(v[0], v[1], v[2], 0.0);
+    if (site_pos) {
+      for (int a = 0; a < D; ++a) u[a] = site_pos[(size_t)i * D + a];
+      hs[i] = make_double4(u[0], u[1], u[2], 0.0);
+    }
+  }
+  BHIPCHK(b, hipSetDevice(b->device));
+  if (N) {
+    BHIPCHK(b, hipMemcpyAsync(b->bodies.bpos + off, hb.data(), N * sizeof(double4), hipMemcpyHostToDevice, b->stream));
+    if (site_pos)
+      BHIPCHK(b, hipMemcpyAsync(b->bodies.spos + off, hs.data(), N * sizeof(double4), hipMemcpyHostToDevice,
+                                b->stream));
+    Bodies sub{b->bodies.bpos + off, b->bodies.spos + off, nullptr, nullptr};
+    if (D == 2)
+      k_init_sites<2><<<grid_for(N, 256), 256, 0, b->stream>>>(g, sub, site_pos ? 1 : 0);
+    else
+      k_init_sites<3><<<grid_for(N, 256), 256, 0, b->stream>>>(g, sub, site_pos ? 1 : 0);
+    BHIPCHK(b, hipGetLastError());
+  }
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  return HMC_OK;
+}
+
+int hmc_batch_get_state(hmc_batch* b, uint32_t replica, double* body_pos, double* site_pos) {
+  if (!b || replica >= b->R) return HMC_ERR_INVALID;
+  const uint32_t N = b->hreps[replica].g.n_bodies, off = b->boff[replica];
+  const int D = b->dim;
+  std::vector<double4> hb(N), hs(N);
+  BHIPCHK(b, hipSetDevice(b->device));
+  if (N) {
+    BHIPCHK(b, hipMemcpyAsync(hb.data(), b->bodies.bpos + off, N * sizeof(double4), hipMemcpyDeviceToHost, b->stream));
+    BHIPCHK(b, hipMemcpyAsync(hs.data(), b->bodies.spos + off, N * sizeof(double4), hipMemcpyDeviceToHost, b->stream));
+  }
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  for (uint32_t i = 0; i < N; ++i) {
+    const double bv[3] = {hb[i].x, hb[i].y, hb[i].z}, sv[3] = {hs[i].x, hs[i].y, hs[i].z};
+    for (int a = 0; a < D; ++a) {
+      if (body_pos) body_pos[(size_t)i * D + a] = bv[a];
+      if (site_pos) site_pos[(size_t)i * D + a] = sv[a];
+    }
+  }
+  return HMC_OK;
+}
+
+int hmc_batch_run_translate(hmc_batch* b, const uint64_t* step0, const uint32_t* seeds, double kT,
+                            const double* max_distance, uint32_t n_steps, int use_graph, hmc_count* out) {
+  if (!b || !step0 || !seeds || !max_distance || !(kT > 0.0)) return HMC_ERR_INVALID;
+  for (uint32_t r = 0; r < b->R; ++r)
+    if (!(max_distance[r] > 0.0)) return bfail(b, HMC_ERR_INVALID, "max_distance must be > 0");
+  BHIPCHK(b, hipSetDevice(b->device));
+  b->kT = kT;
+  BHIPCHK(b, hipMemcpyAsync(b->dstep, step0, b->R * sizeof(uint64_t), hipMemcpyHostToDevice, b->stream));
+  BHIPCHK(b, hipMemcpyAsync(b->dseed, seeds, b->R * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
+  BHIPCHK(b, hipMemcpyAsync(b->dmove, max_distance, b->R * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  std::vector<ApplyState> st0, st1;
+  std::vector<unsigned long long> a0, a1;
+  int rc = batch_read(b, st0, a0);
+  if (rc) return rc;
+  const uint32_t launched = b->cycles_guess;
+  for (uint32_t i = 0; i < n_steps; ++i) {
+    if (i == 0 || !use_graph) {
+      batch_step(b, i == 0 ? 1 : 0, i == 0 ? 0 : 1, launched);
+      continue;
+    }
+    if (!b->exec || b->graph_cycles != launched) {
+      if (b->exec) (void)hipGraphExecDestroy(b->exec);
+      if (b->graph) (void)hipGraphDestroy(b->graph);
+      b->exec = nullptr;
+      b->graph = nullptr;
+      BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      batch_step(b, 0, 1, launched);
+      BHIPCHK(b, hipStreamEndCapture(b->stream, &b->graph));
+      BHIPCHK(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+      b->graph_cycles = launched;
+    }
+    BHIPCHK(b, hipGraphLaunch(b->exec, b->stream));
+  }
+  BHIPCHK(b, hipGetLastError());
+  rc = batch_read(b, st1, a1);
+  if (rc) return rc;
+  uint32_t most = 1;
+  for (uint32_t r = 0; r < b->R; ++r) {
+    most = std::max<uint32_t>(most, st1[r].cycles);
+    if (out) {
+      out[r].accepted = a1[r] - a0[r];
+      out[r].rejected = (st1[r].trials_total - st0[r].trials_total) - out[r].accepted;
+    }
+  }
+  if (n_steps) b->cycles_guess = most;
+  return HMC_OK;
+}
+
+int hmc_batch_sdf(hmc_batch* b, double xmax, double dx, uint32_t nbins, uint64_t* hist, uint64_t* overlaps) {
+  if (!b || !hist) return HMC_ERR_INVALID;
+  for (uint32_t r = 0; r < b->R; ++r)
+    if (const char* why = sdf_check(b->hreps[r].g, xmax, dx, nbins)) return bfail(b, HMC_ERR_INVALID, why);
+  if ((uint64_t)b->R * (nbins + 1ull) >= (1ull << 31)) return bfail(b, HMC_ERR_INVALID, "n_replicas * nbins too large");
+  BHIPCHK(b, hipSetDevice(b->device));
+  hipStream_t s = b->stream;
+  // the replicas laid out on the SDF's checkerboards (fewer, wider spaces)
+  std::vector<BatchRep> hr(b->R);
+  uint32_t stot = 0;
+  for (uint32_t r = 0; r < b->R; ++r) {
+    hr[r] = b->hreps[r];
+    hr[r].g = sdf_geometry(b->hreps[r].g, xmax);
+    hr[r].space_off = stot;
+    stot += hr[r].g.n_spaces;
+  }
+  Geometry gt = b->gtot;
+  gt.n_spaces = stot;
+  const uint32_t nsb = grid_for(stot, kScanBlock);
+  const size_t cells = (size_t)b->R * nbins + b->R;
+  char* buf = nullptr;
+  BHIPCHK(b, hipMallocAsync((void**)&buf, b->R * sizeof(BatchRep) + cells * sizeof(uint32_t), s));
+  BatchRep* dreps = (BatchRep*)buf;
+  uint32_t* dh = (uint32_t*)(buf + b->R * sizeof(BatchRep));
+  hipError_t e = hipMemcpyAsync(dreps, hr.data(), b->R * sizeof(BatchRep), hipMemcpyHostToDevice, s);
+  k_fill_u32<<<grid_for((uint32_t)cells, 256), 256, 0, s>>>(dh, (uint32_t)cells, 0u);
+  if (e == hipSuccess && b->n_total) {
+    // cells of the current states (origins drawn from the current counters;
+    // the next step rebuilds the sweep's own)
+    k_bsetup<<<grid_for(b->R, 64), 64, 0, s>>>(dreps, b->sts, b->R, b->dstep, b->dseed, b->kT, b->dmove, 0, 0);
+    k_fill_u32<<<grid_for(stot, 256), 256, 0, s>>>(b->w.cnt, stot, 0u);
+    if (b->dim == 2)
+      k_bbin<2><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(dreps, b->sts, b->dboff, b->R, b->n_total,
+                                                                b->bodies, b->w);
+    else
+      k_bbin<3><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(dreps, b->sts, b->dboff, b->R, b->n_total,
+                                                                b->bodies, b->w);
+    k_scan_blocks<<<nsb, 256, 0, s>>>(b->w.cnt, b->w.start, b->w.block_sums, b->w.block_ne, stot);
+    k_scan_top<<<1, 1024, 0, s>>>(gt, b->sts + b->R, b->w.block_sums, b->w.block_ne, nsb, b->w.start, 0);
+    k_scan_add<<<grid_for(stot, 256), 256, 0, s>>>(b->w.start, b->w.block_sums, nullptr, stot);
+    k_scatter<<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->gtot, b->w);
+    if (b->dim == 2)
+      k_sdf<2><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(dreps, b->dboff, b->R, b->n_total, b->bodies, b->w,
+                                                               xmax, dx, nbins, dh, dh + (size_t)b->R * nbins);
+    else
+      k_sdf<3><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(dreps, b->dboff, b->R, b->n_total, b->bodies, b->w,
+                                                               xmax, dx, nbins, dh, dh + (size_t)b->R * nbins);
+  }
+  std::vector<uint32_t> h(cells, 0u);
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), dh, cells * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFreeAsync(buf, s);
+  if (e != hipSuccess) return bfail(b, HMC_ERR_DEVICE, std::string("batch sdf: ") + hipGetErrorString(e));
+  std::vector<ApplyState> st;
+  std::vector<unsigned long long> acc;
+  int rc = batch_read(b, st, acc);  // a body outside its checkerboard is reported here
+  if (rc) return rc;
+  for (size_t k = 0; k < (size_t)b->R * nbins; ++k) hist[k] = h[k];
+  if (overlaps)
+    for (uint32_t r = 0; r < b->R; ++r) overlaps[r] = h[(size_t)b->R * nbins + r];
+  return HMC_OK;
+}
+
+int hmc_batch_autotune(hmc_batch* b, const uint64_t* step0, const uint32_t* seeds, double kT,
+                       const double* max_distance, uint32_t n_steps, double* best_ms, uint32_t* best_policy) {
+  if (!b || !step0 || !seeds || !max_distance || !(kT > 0.0) || n_steps == 0) return HMC_ERR_INVALID;
+  for (uint32_t r = 0; r < b->R; ++r)
+    if (!(max_distance[r] > 0.0)) return bfail(b, HMC_ERR_INVALID, "max_distance must be > 0");
+  if (!b->tiled) return bfail(b, HMC_ERR_UNSUPPORTED, "the batch does not run on LDS tiles");
+  BHIPCHK(b, hipSetDevice(b->device));
+  hipStream_t s = b->stream;
+  const uint32_t R = b->R;
+  b->kT = kT;
+  BHIPCHK(b, hipMemcpyAsync(b->dstep, step0, R * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+  BHIPCHK(b, hipMemcpyAsync(b->dseed, seeds, R * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  BHIPCHK(b, hipMemcpyAsync(b->dmove, max_distance, R * sizeof(double), hipMemcpyHostToDevice, s));
+  // the state every trial run starts from and leaves behind
+  const size_t nb = (size_t)b->n_total * sizeof(double4), ns = (R + 1) * sizeof(ApplyState),
+               na = kAccSlots * sizeof(unsigned long long);
+  char* save = nullptr;
+  BHIPCHK(b, hipMalloc((void**)&save, 2 * nb + ns + na));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = HMC_OK;
+  auto copy_state = [&](bool restore) -> bool {
+    void* dst[4] = {save, save + nb, save + 2 * nb, save + 2 * nb + ns};
+    void* src[4] = {b->bodies.bpos, b->bodies.spos, b->sts, b->w.acc_slots};
+    const size_t len[4] = {nb, nb, ns, na};
+    for (int i = 0; i < 4; ++i)
+      if (len[i] && hipMemcpyAsync(restore ? src[i] : dst[i], restore ? dst[i] : src[i], len[i],
+                                   hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return false;
+    return hipStreamSynchronize(s) == hipSuccess;
+  };
+  const int policies[] = {1, 4, 16, 64, 256};
+  double best = std::numeric_limits<double>::infinity();
+  int chosen = -1;
+  BatchTiles best_bt;
+  std::vector<std::vector<uint32_t>> seen;
+  if (!copy_state(false) || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = HMC_ERR_DEVICE;
+  for (int pi = 0; pi < 5 && rc == HMC_OK; ++pi) {
+    BatchTiles bt;
+    if (!batch_tiles(b, policies[pi], bt)) continue;
+    std::vector<uint32_t> sig;
+    for (const TileCfg& tc : bt.tcs) sig.insert(sig.end(), {tc.T[0], tc.T[1], tc.T[2]});
+    if (std::find(seen.begin(), seen.end(), sig) != seen.end()) continue;  // same layout as an earlier policy
+    seen.push_back(sig);
+    if (batch_set_tiles(b, bt) != HMC_OK) {
+      rc = HMC_ERR_DEVICE;
+      break;
+    }
+    // one untimed step from (step0, seeds), then n_steps timed ones
+    batch_step(b, 1, 0, b->cycles_guess);
+    float ms = 0.0f;
+    bool ok = hipEventRecord(e0, s) == hipSuccess;
+    for (uint32_t i = 0; i < n_steps; ++i) batch_step(b, 0, 1, b->cycles_guess);
+    ok = ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+         hipEventElapsedTime(&ms, e0, e1) == hipSuccess && hipGetLastError() == hipSuccess;
+    ok = ok && copy_state(true);
+    if (!ok) {
+      rc = HMC_ERR_DEVICE;
+      break;
+    }
+    if (ms / n_steps < best) {
+      best = ms / n_steps;
+      chosen = policies[pi];
+      best_bt = bt;
+    }
+  }
+  if (rc == HMC_OK && chosen > 0 && batch_set_tiles(b, best_bt) != HMC_OK) rc = HMC_ERR_DEVICE;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(save);
+  if (rc) return bfail(b, rc, "autotune: device error");
+  if (best_ms) *best_ms = best;
+  if (best_policy) *best_policy = chosen > 0 ? (uint32_t)chosen : 0u;
+  return HMC_OK;
+}
+
+int hmc_batch_get_stats(hmc_batch* b, uint32_t replica, hmc_stats* out) {
+  if (!b || !out || replica >= b->R) return HMC_ERR_INVALID;
+  std::vector<ApplyState> st;
+  std::vector<unsigned long long> acc;
+  const int rc = batch_read(b, st, acc);
+  if (rc) return rc;
+  const ApplyState& s = st[replica];
+  std::memset(out, 0, sizeof *out);
+  out->cycles = s.cycles_total;
+  out->band_retests = s.band_retests;
+  out->guard_cycles = s.guard_cycles;
+  out->tile_overflows = s.tile_overflows;
+  out->tiled = b->tiled ? 1u : 0u;
+  return HMC_OK;
+}
+
+const char* hmc_batch_last_error(const hmc_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+void hmc_batch_destroy(hmc_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (void* p : {(void*)b->reps, (void*)b->sts, (void*)b->dboff, (void*)b->dcoff, (void*)b->dstep, (void*)b->dseed,
+                  (void*)b->dmove, (void*)b->bodies.bpos, (void*)b->bodies.spos, (void*)b->cells.body,
+                  (void*)b->cells.pos32, (void*)b->cells.pos64, (void*)b->w.cnt, (void*)b->w.start,
+                  (void*)b->w.elem_space, (void*)b->w.elem_rank, (void*)b->w.tmp_body, (void*)b->w.block_sums,
+                  (void*)b->w.block_ne, (void*)b->w.acc_slots, (void*)b->dtcs, (void*)b->dtoff, (void*)b->dtrep})
+    if (p) (void)hipFree(p);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+}  // extern "C"

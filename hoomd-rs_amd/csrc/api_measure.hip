@@ -1,0 +1,222 @@
+// Measurements through the C ABI: the SDF, insertion energies and the
+// total energy.
+#include "host.hpp"
+
+namespace hmc {
+
+// Arguments of the SDF for one system: hard spheres/disks, 0 < xmax < 1/2,
+// dx > 0, a box wider than 2 r_max (one image within r_max) and spaces at
+// least r_max / 2 wide (two neighbour spaces on each side suffice).
+const char* sdf_check(const Geometry& g, double xmax, double dx, uint32_t nbins) {
+  if (g.model != HMC_HARD_SPHERE) return "the SDF is built for hard spheres and disks";
+  if (!(xmax > 0.0 && xmax < 0.5) || !(dx > 0.0) || !std::isfinite(dx) || nbins == 0 || nbins > (1u << 24))
+    return "SDF needs 0 < xmax < 1/2, dx > 0 and 0 < nbins <= 2^24";
+  const double rmax = g.sigma / (1.0 - xmax);
+  for (int a = 0; a < g.dim; ++a) {
+    if (!(g.L[a] > 2.0 * rmax * (1.0 + 0x1p-30))) return "box edge below 2 sigma / (1 - xmax)";
+    if (!(2.0 * g.width[a] >= rmax * (1.0 + 0x1p-30))) return "checkerboard spaces narrower than r_max / 2";
+  }
+  return nullptr;
+}
+
+// The SDF's own cells: the checkerboard with spaces at least r_max (1 + 2^-29)
+// wide (never more spaces than the sweep's, whose buffers it reuses), so
+// that one neighbour space on each side covers r_max.
+Geometry sdf_geometry(const Geometry& g, double xmax) {
+  Geometry s = g;
+  const double rmax = g.sigma / (1.0 - xmax);
+  uint64_t spaces = 1;
+  for (int a = 0; a < g.dim; ++a) {
+    int64_t k = (int64_t)std::floor(g.L[a] / (rmax * (1.0 + 0x1p-29)));
+    k = std::max<int64_t>(1, std::min<int64_t>(k, g.shape[a]));
+    s.shape[a] = (int32_t)k;
+    s.half_shape[a] = (uint32_t)std::max<int64_t>(1, k / 2);
+    s.width[a] = g.L[a] / (double)k;
+    spaces *= (uint64_t)k;
+  }
+  uint32_t str = 1;
+  for (int a = g.dim - 1; a >= 0; --a) {
+    s.stride[a] = str;
+    str *= (uint32_t)s.shape[a];
+  }
+  s.n_spaces = (uint32_t)spaces;
+  s.per_colour = s.n_spaces >> g.dim;
+  return s;
+}
+
+}  // namespace hmc
+
+extern "C" {
+
+int hmc_sdf(hmc_ctx* c, double xmax, double dx, uint32_t nbins, uint64_t* hist, uint64_t* overlaps) {
+  if (!c || !hist) return HMC_ERR_INVALID;
+  if (!c->have_state) return fail(c, HMC_ERR_STATE, "hmc_set_state was not called");
+  if (const char* why = sdf_check(c->g, xmax, dx, nbins))
+    return fail(c, c->g.model != HMC_HARD_SPHERE ? HMC_ERR_UNSUPPORTED : HMC_ERR_INVALID, why);
+  const Geometry g = sdf_geometry(c->g, xmax);
+  const uint32_t nsb = grid_for(g.n_spaces, kScanBlock);
+  const uint32_t N = g.n_bodies;
+  HIPCHK(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  // device: the system as a one-entry batch table, body offsets, counts
+  const size_t bytes = sizeof(BatchRep) + 2 * sizeof(uint32_t) + ((size_t)nbins + 1) * sizeof(uint32_t);
+  char* buf = nullptr;
+  HIPCHK(c, hipMallocAsync((void**)&buf, bytes, s));
+  BatchRep* drep = (BatchRep*)buf;
+  uint32_t* dboff = (uint32_t*)(buf + sizeof(BatchRep));
+  uint32_t* dh = dboff + 2;
+  const BatchRep hrep{g, 0u, 0u, 0u, 0u};
+  const uint32_t hboff[2] = {0u, N};
+  hipError_t e = hipMemcpyAsync(drep, &hrep, sizeof hrep, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(dboff, hboff, sizeof hboff, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    k_fill_u32<<<grid_for(nbins + 1, 256), 256, 0, s>>>(dh, nbins + 1, 0u);
+    if (N) {
+      // cells of the current state on the SDF's checkerboard (origin drawn
+      // from the current counter; the next apply rebuilds its own)
+      k_setup<<<1, 64, 0, s>>>(g, c->st, 0, 0, 0, 1.0, 1.0, 0, 0, 0u, 0);
+      if (g.dim == 2)
+        k_bin<2><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
+      else
+        k_bin<3><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
+      k_scan_blocks<<<nsb, 256, 0, s>>>(c->w.cnt, c->w.start, c->w.block_sums, c->w.block_ne, g.n_spaces);
+      k_scan_top<<<1, 1024, 0, s>>>(g, c->st, c->w.block_sums, c->w.block_ne, nsb, c->w.start, 0);
+      k_scan_add<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.start, c->w.block_sums, c->w.cnt, g.n_spaces);
+      k_scatter<<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->w);
+      if (g.dim == 2)
+        k_sdf<2><<<grid_for(N, kBlock), kBlock, 0, s>>>(drep, dboff, 1u, N, c->bodies, c->w, xmax, dx, nbins, dh,
+                                                        dh + nbins);
+      else
+        k_sdf<3><<<grid_for(N, kBlock), kBlock, 0, s>>>(drep, dboff, 1u, N, c->bodies, c->w, xmax, dx, nbins, dh,
+                                                        dh + nbins);
+    }
+    e = hipGetLastError();
+  }
+  std::vector<uint32_t> h(nbins + 1, 0u);
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), dh, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFreeAsync(buf, s);
+  if (e != hipSuccess) return fail(c, HMC_ERR_DEVICE, std::string("sdf: ") + hipGetErrorString(e));
+  ApplyState st;  // a body outside the checkerboard is reported here
+  int rc = read_state(c, &st);
+  if (rc) return rc;
+  for (uint32_t k = 0; k < nbins; ++k) hist[k] = h[k];
+  if (overlaps) *overlaps = h[nbins];
+  return HMC_OK;
+}
+
+int hmc_delta_energy_insert(hmc_ctx* c, const double* body_pos, uint64_t n, double* energy, double* bound) {
+  if (!c || (n && (!body_pos || !energy))) return HMC_ERR_INVALID;
+  if (!c->have_state) return fail(c, HMC_ERR_STATE, "hmc_set_state was not called");
+  const Geometry& g = c->g;
+  if (g.model == HMC_CONVEX_POLYHEDRON)
+    return fail(c, HMC_ERR_UNSUPPORTED, "insertion energies of oriented bodies are not offloaded");
+  if (n >= 0xffffffffull) return fail(c, HMC_ERR_INVALID, "too many candidates");
+  if (n == 0) return HMC_OK;
+  const int D = g.dim;
+  const uint32_t N = g.n_bodies;
+  if (N == 0) {  // no sites: every insertion energy is 0
+    for (uint64_t k = 0; k < n; ++k) {
+      energy[k] = 0.0;
+      if (bound) bound[k] = 0.0;
+    }
+    return HMC_OK;
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  double* dc = nullptr;
+  double* dout = nullptr;
+  uint32_t* flag = nullptr;
+  hipError_t e = hipMallocAsync((void**)&dc, (size_t)n * D * sizeof(double), s);
+  if (e == hipSuccess) e = hipMallocAsync((void**)&dout, 2 * (size_t)n * sizeof(double), s);
+  if (e == hipSuccess) e = hipMallocAsync((void**)&flag, sizeof(uint32_t), s);
+  if (e == hipSuccess) e = hipMemcpyAsync(dc, body_pos, (size_t)n * D * sizeof(double), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemsetAsync(flag, 0, sizeof(uint32_t), s);
+  if (e == hipSuccess) {
+    // cells of the current state, as for the total energy
+    k_setup<<<1, 64, 0, s>>>(g, c->st, 0, 0, 0, 1.0, 1.0, 0, 0, 0u, 0);
+    if (D == 2)
+      k_bin<2><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
+    else
+      k_bin<3><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
+    k_scan_blocks<<<c->n_scan_blocks, 256, 0, s>>>(c->w.cnt, c->w.start, c->w.block_sums, c->w.block_ne, g.n_spaces);
+    k_scan_top<<<1, 1024, 0, s>>>(g, c->st, c->w.block_sums, c->w.block_ne, c->n_scan_blocks, c->w.start, 0);
+    k_scan_add<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.start, c->w.block_sums, c->w.cnt, g.n_spaces);
+    k_scatter<<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->w);
+    if (D == 2)
+      k_insert_energy<2><<<grid_for(n, kEnergyBlock), kEnergyBlock, 0, s>>>(g, c->st, c->bodies, c->w, dc,
+                                                                           (uint32_t)n, dout, flag);
+    else
+      k_insert_energy<3><<<grid_for(n, kEnergyBlock), kEnergyBlock, 0, s>>>(g, c->st, c->bodies, c->w, dc,
+                                                                           (uint32_t)n, dout, flag);
+    e = hipGetLastError();
+  }
+  std::vector<double> hout(2 * (size_t)n);
+  uint32_t outside = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(hout.data(), dout, 2 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(&outside, flag, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (dc) (void)hipFreeAsync(dc, s);
+  if (dout) (void)hipFreeAsync(dout, s);
+  if (flag) (void)hipFreeAsync(flag, s);
+  if (e != hipSuccess) return fail(c, HMC_ERR_DEVICE, std::string("insertion energy: ") + hipGetErrorString(e));
+  ApplyState st;
+  int rc = read_state(c, &st);
+  if (rc) return rc;
+  if (outside) return fail(c, HMC_ERR_COVER, "a candidate lies outside the checkerboard");
+  for (uint64_t k = 0; k < n; ++k) {
+    energy[k] = hout[2 * k];
+    if (bound) bound[k] = hout[2 * k + 1];
+  }
+  return HMC_OK;
+}
+
+int hmc_total_energy(hmc_ctx* c, double* energy, double* bound) {
+  if (!c || !energy) return HMC_ERR_INVALID;
+  if (!c->have_state) return fail(c, HMC_ERR_STATE, "hmc_set_state was not called");
+  const Geometry& g = c->g;
+  const uint32_t N = g.n_bodies;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (N == 0) {
+    *energy = 0.0;
+    if (bound) *bound = 0.0;
+    return HMC_OK;
+  }
+  hipStream_t s = c->stream;
+  const uint32_t nblk = grid_for(N, kEnergyBlock);
+  double* part = nullptr;
+  HIPCHK(c, hipMallocAsync((void**)&part, (4 * (size_t)nblk + 4) * sizeof(double), s));
+  // cells for the neighbour search: the checkerboard of the current counter
+  k_setup<<<1, 64, 0, s>>>(g, c->st, 0, 0, 0, 1.0, 1.0, 0, 0, 0u, 0);
+  if (g.dim == 2)
+    k_bin<2><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
+  else
+    k_bin<3><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
+  k_scan_blocks<<<c->n_scan_blocks, 256, 0, s>>>(c->w.cnt, c->w.start, c->w.block_sums, c->w.block_ne, g.n_spaces);
+  k_scan_top<<<1, 1024, 0, s>>>(g, c->st, c->w.block_sums, c->w.block_ne, c->n_scan_blocks, c->w.start, 0);
+  k_scan_add<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.start, c->w.block_sums, c->w.cnt, g.n_spaces);
+  k_scatter<<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->w);
+  if (g.dim == 2)
+    k_energy<2><<<nblk, kEnergyBlock, 0, s>>>(g, c->st, c->bodies, c->w, part);
+  else
+    k_energy<3><<<nblk, kEnergyBlock, 0, s>>>(g, c->st, c->bodies, c->w, part);
+  k_energy_final<<<1, kEnergyFinal, 0, s>>>(part, nblk, part + 4 * (size_t)nblk);
+  double out[4] = {0, 0, 0, 0};
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, part + 4 * (size_t)nblk, sizeof out, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFreeAsync(part, s);
+  if (e != hipSuccess) return fail(c, HMC_ERR_DEVICE, std::string("total energy: ") + hipGetErrorString(e));
+  // the next apply rebuilds the cells; the neighbour copies were not touched
+  ApplyState st;
+  int rc = read_state(c, &st);
+  if (rc) return rc;
+  // any two summation orders of n terms differ by at most 2 gamma_{n-1} sum|terms|
+  const double n = std::max(out[2], 1.0);
+  const double gam = n * 0x1p-53 / (1.0 - n * 0x1p-53);
+  *energy = out[3] != 0.0 ? std::numeric_limits<double>::infinity() : out[0];
+  if (bound) *bound = out[3] != 0.0 ? 0.0 : 2.0 * gam * out[1];
+  return HMC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,95 @@
+// Initial site positions and orientations, boundary maps and test hooks.
+#include "sweep_device.hpp"
+
+namespace hmc {
+
+// Site positions (given, or wrap(body + 0)) and their ghost flags.
+template <int D>
+__global__ void k_init_sites(Geometry g, Bodies B, int have_sites) {
+  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= g.n_bodies) return;
+  double4 b = B.bpos[e];
+  double x[3] = {b.x, b.y, b.z};
+  double s[3] = {0.0, 0.0, 0.0};
+  if (have_sites) {
+    double4 sp = B.spos[e];
+    s[0] = sp.x;
+    s[1] = sp.y;
+    s[2] = sp.z;
+  } else {
+    for (int a = 0; a < D; ++a) s[a] = wrap1(x[a] + 0.0, g.L[a]);
+  }
+  B.spos[e] = pack_site(s, ghost_flags<D>(g, s));
+}
+
+// Microstate::clone_with_boundary with every body mapped (microstate.rs:1529-1565):
+// Hypercuboid::map_point (cuboid.rs:553-567), p' = clamp(p (L'/L), -L'/2,
+// next_down(L'/2)) per axis, then update_body_properties (microstate.rs:985-1034):
+// body = wrap(p'), site = wrap(body + 0) with its ghost flags.  g is the new
+// geometry; a body outside the old box (map_point's error, on which the
+// reference panics) sets *outside.
+template <int D>
+__global__ void k_map_bodies(Geometry g, double3 lold, const double4* __restrict__ src, Bodies B,
+                             uint32_t* outside) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= g.n_bodies) return;
+  const double4 b = src[e];
+  const double x[3] = {b.x, b.y, b.z};
+  const double lo[3] = {lold.x, lold.y, lold.z};
+  double p[3] = {0.0, 0.0, 0.0};
+  bool inside = true;
+  for (int a = 0; a < D; ++a) {
+    inside = inside && -lo[a] / 2.0 <= x[a] && x[a] < lo[a] / 2.0;  // cuboid.rs:449-454
+    const double scale = g.L[a] / lo[a];
+    const double low = -g.L[a] / 2.0;
+    const double high = __longlong_as_double(__double_as_longlong(g.L[a] / 2.0) - 1);  // next_down of a positive
+    const double v = scale * x[a];
+    p[a] = v < low ? low : (v > high ? high : v);  // f64::clamp
+  }
+  if (!inside) atomicOr(outside, 1u);
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int a = 0; a < D; ++a) p[a] = wrap1(p[a], g.L[a]);
+  for (int a = 0; a < D; ++a) s[a] = wrap1(p[a] + 0.0, g.L[a]);
+  B.bpos[e] = make_double4(p[0], p[1], p[2], 0.0);
+  B.spos[e] = pack_site(s, ghost_flags<D>(g, s));
+}
+
+// ------------------------------------------------------------- debug hooks --
+__global__ void k_debug_math(int kind, const double* in, double* out, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double r = 0.0;
+  switch (kind) {
+    case 0: r = hmc_cbrt(in[i]); break;
+    case 1: r = glibc_log(in[i]); break;
+    case 2: r = glibc_exp(in[i]); break;
+    case 3: r = sqrt(in[i]); break;
+    case 5: {
+      double c;
+      glibc_sincos(in[i], &r, &c);
+      break;
+    }
+    case 6: {
+      double sn;
+      glibc_sincos(in[i], &sn, &r);
+      break;
+    }
+    default: r = in[2 * i] / in[2 * i + 1]; break;
+  }
+  out[i] = r;
+}
+
+__global__ void k_debug_normals(uint64_t step, uint32_t substep, uint32_t seed, uint64_t index, double* out,
+                                uint64_t n) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  Sfc64 r = counter_rng(step, substep, seed, index);
+  for (uint64_t i = 0; i < n; ++i) out[i] = standard_normal(r, kZigX, kZigF);
+}
+
+// explicit instantiations (launched from the host translation units)
+template __global__ void k_init_sites<2>(Geometry g, Bodies B, int have_sites);
+template __global__ void k_init_sites<3>(Geometry g, Bodies B, int have_sites);
+template __global__ void k_map_bodies<2>(Geometry g, double3 lold, const double4* __restrict__ src, Bodies B, uint32_t* outside);
+template __global__ void k_map_bodies<3>(Geometry g, double3 lold, const double4* __restrict__ src, Bodies B, uint32_t* outside);
+
+}  // namespace hmc

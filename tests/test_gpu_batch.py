@@ -87,7 +87,7 @@ def test_batch_graph_replays_equal_direct_launches(hmc, wl):
 
 def test_batch_config_c5_full(hmc, wl):
     # 256 replicas, N log-uniform over FCC counts 4 k^3 (k = 8..25), packing
-    # fractions 0.30 .. 0.54, seeds 42 + r, d = 0.08: 3 steps, three replicas
+    # fractions 0.30 .. 0.54, seeds 42 + r, d = 0.08: 10 steps, six replicas
     # checked against their own oracles, all replicas overlap-free
     systems = wl.batched_replicas()
     n_total = sum(len(p) for _, p in systems)
@@ -96,11 +96,11 @@ def test_batch_config_c5_full(hmc, wl):
     # the microstates' stored (wrapped) bodies; sites follow as wrap(body + 0)
     batch = hmc.GpuBatch([(L, O.wrap(L, p), None) for L, p in systems], precision="mixed")
     assert batch.stats(0)["tiled"] == 1
-    counts = batch.run(0, seeds, 1.0, 0.08, 3)
-    log(f"C5: {len(systems)} replicas, N total {n_total}: 3 steps ({time.time() - t:.1f}s)")
-    for r in (0, 101, 255):
+    counts = batch.run(0, seeds, 1.0, 0.08, 10)
+    log(f"C5: {len(systems)} replicas, N total {n_total}: 10 steps ({time.time() - t:.1f}s)")
+    for r in (0, 51, 101, 153, 204, 255):
         L, pos = systems[r]
-        _, tot, (ob, os_) = _oracle_run(L, pos, seeds[r], 0.08, 3)
+        _, tot, (ob, os_) = _oracle_run(L, pos, seeds[r], 0.08, 10)
         assert counts[r] == tot, (r, counts[r], tot)
         gb, gs = batch.positions(r)
         assert np.array_equal(gb, ob) and np.array_equal(gs, os_), r

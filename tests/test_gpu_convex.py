@@ -146,9 +146,9 @@ def test_replay_octahedra_large_rotations(hmc):
 
 def test_replay_config_c3_full(hmc):
     # N = 262,144 unit cubes, simple cubic 64^3 at packing fraction 0.55
-    # (box 78.11), aligned start, translate 0.05 + rotate 0.1 per step
+    # (box 78.11), aligned start, translate 0.05 + rotate 0.1 per step, 10 steps
     L, pos = _sc(64, 0.55)
     q = np.tile([1.0, 0.0, 0.0, 0.0], (len(pos), 1))
     t = time.time()
-    _replay(hmc, L, pos, q, O.CUBE, 0.05, 0.1, 2, trace=False)
-    log(f"C3 N={len(pos)}: 2 translate+rotate steps bit-identical ({time.time() - t:.1f}s)")
+    _replay(hmc, L, pos, q, O.CUBE, 0.05, 0.1, 10, trace=False)
+    log(f"C3 N={len(pos)}: 10 translate+rotate steps bit-identical ({time.time() - t:.1f}s)")

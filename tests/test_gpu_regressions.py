@@ -89,7 +89,8 @@ def test_insert_energy_after_applies(hmc, wl, case):
     log(f"insertion energies after applies: {case}")
     kT, d = 1.0, 0.08
     if case == "mixed_persistent":
-        L, pos = wl.fcc_spheres((10, 10, 10), packing_fraction=0.50)
+        # dilute enough that some candidates fit between the lattice sites
+        L, pos = wl.fcc_spheres((10, 10, 10), packing_fraction=0.15)
         model, params, precision = "hard_sphere", (1.0,), "mixed"
     elif case == "mixed_overflow":
         L, pos = _clustered(wl)
