@@ -33,7 +33,7 @@ EXPORTS = ["hmc_create", "hmc_set_state", "hmc_get_state", "hmc_clone_with_bound
            "hmc_quick_compress_init", "hmc_quick_compress_apply", "hmc_delta_energy_insert",
            "hmc_apply_rotate",
            "hmc_set_orientations", "hmc_get_orientations", "hmc_run_translate", "hmc_total_energy",
-           "hmc_trace_enable", "hmc_trace_read", "hmc_get_stats", "hmc_profile_enable", "hmc_profile_read",
+           "hmc_total_energy_noalloc", "hmc_trace_enable", "hmc_trace_read", "hmc_get_stats", "hmc_profile_enable", "hmc_profile_read",
            "hmc_debug_math", "hmc_debug_normals", "hmc_debug_convex", "hmc_last_error", "hmc_destroy",
            "hmc_batch_create", "hmc_batch_set_state", "hmc_batch_get_state", "hmc_batch_run_translate",
            "hmc_batch_last_error", "hmc_batch_destroy", "hmc_sdf", "hmc_batch_sdf", "hmc_batch_get_stats", "hmc_batch_autotune"]
@@ -102,6 +102,7 @@ def lib():
                                        C.POINTER(Count), C.POINTER(C.c_uint32)]
         L.hmc_set_orientations.argtypes = [vp, D_P, D_P]
         L.hmc_total_energy.argtypes = [vp, D_P, D_P]
+        L.hmc_total_energy_noalloc.argtypes = [vp, D_P, D_P]
         u64p = C.POINTER(C.c_uint64)
         L.hmc_sdf.argtypes = [vp, C.c_double, C.c_double, C.c_uint32, u64p, u64p]
         L.hmc_batch_sdf.argtypes = [vp, C.c_double, C.c_double, C.c_uint32, u64p, u64p]
@@ -275,6 +276,14 @@ class GpuParallelSweep:
         e = C.c_double()
         b = C.c_double()
         self._check(lib().hmc_total_energy(self.h, C.byref(e), C.byref(b)))
+        return e.value, b.value
+
+    def total_energy_noalloc(self):
+        """total_energy through hmc_total_energy_noalloc (no device allocation:
+        the partials live in the library's static scratch)."""
+        e = C.c_double()
+        b = C.c_double()
+        self._check(lib().hmc_total_energy_noalloc(self.h, C.byref(e), C.byref(b)))
         return e.value, b.value
 
     def sdf(self, xmax=0.02, dx=1e-4, nbins=None):

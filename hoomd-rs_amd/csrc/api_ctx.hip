@@ -538,6 +538,10 @@ int hmc_create(const hmc_config* cfg_in, hmc_ctx** out) {
     delete c;
     return HMC_ERR_DEVICE;
   }
+  if (noalloc_setup(c->device) != HMC_OK) {  // the static scratch of hmc_total_energy_noalloc
+    delete c;
+    return HMC_ERR_DEVICE;
+  }
   if (c->fast && !lj && !convex && !(cfg->flags & HMC_FLAG_GLOBAL_PASS)) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0) {

@@ -44,6 +44,80 @@ Geometry sdf_geometry(const Geometry& g, double xmax) {
   return s;
 }
 
+// PairwiseCutoff::total_energy of the current state with the block partials
+// in part (4 nblk + 4 doubles, nblk = ceil(N / kEnergyBlock)): the cells of
+// the current counter's checkerboard, k_energy per block, k_energy_final in
+// a fixed order.  Allocates nothing; reads back only the result and the
+// state's error word.
+int total_energy_into(hmc_ctx* c, double* part, double* energy, double* bound) {
+  const Geometry& g = c->g;
+  const uint32_t N = g.n_bodies;
+  if (N == 0) {
+    *energy = 0.0;
+    if (bound) *bound = 0.0;
+    return HMC_OK;
+  }
+  hipStream_t s = c->stream;
+  const uint32_t nblk = grid_for(N, kEnergyBlock);
+  // cells for the neighbour search: the checkerboard of the current counter
+  k_setup<<<1, 64, 0, s>>>(g, c->st, 0, 0, 0, 1.0, 1.0, 0, 0, 0u, 0);
+  if (g.dim == 2)
+    k_bin<2><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
+  else
+    k_bin<3><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
+  k_scan_blocks<<<c->n_scan_blocks, 256, 0, s>>>(c->w.cnt, c->w.start, c->w.block_sums, c->w.block_ne, g.n_spaces);
+  k_scan_top<<<1, 1024, 0, s>>>(g, c->st, c->w.block_sums, c->w.block_ne, c->n_scan_blocks, c->w.start, 0);
+  k_scan_add<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.start, c->w.block_sums, c->w.cnt, g.n_spaces);
+  k_scatter<<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->w);
+  if (g.dim == 2)
+    k_energy<2><<<nblk, kEnergyBlock, 0, s>>>(g, c->st, c->bodies, c->w, part);
+  else
+    k_energy<3><<<nblk, kEnergyBlock, 0, s>>>(g, c->st, c->bodies, c->w, part);
+  k_energy_final<<<1, kEnergyFinal, 0, s>>>(part, nblk, part + 4 * (size_t)nblk);
+  double out[4] = {0, 0, 0, 0};
+  uint32_t err = 0;
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, part + 4 * (size_t)nblk, sizeof out, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(&err, &c->st->error, sizeof err, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return fail(c, HMC_ERR_DEVICE, std::string("total energy: ") + hipGetErrorString(e));
+  // (the next apply rebuilds the cells; the neighbour copies were not touched)
+  if (err & kErrBarrier) return fail(c, HMC_ERR_DEVICE, "a device-side wait timed out");
+  if (err) return fail(c, HMC_ERR_COVER, "a body lies outside the checkerboard");
+  // any two summation orders of n terms differ by at most 2 gamma_{n-1} sum|terms|
+  const double n = std::max(out[2], 1.0);
+  const double gam = n * 0x1p-53 / (1.0 - n * 0x1p-53);
+  *energy = out[3] != 0.0 ? std::numeric_limits<double>::infinity() : out[0];
+  if (bound) *bound = out[3] != 0.0 ? 0.0 : 2.0 * gam * out[1];
+  return HMC_OK;
+}
+
+// ------------------------------------------- static scratch of the no-alloc --
+// hmc_total_energy_noalloc keeps the block partials of the total energy in
+// this array instead of a per-call allocation: 4 kNoAllocBlocks + 4 doubles
+// (1 MiB + 32 B), the partials of up to 8,388,608 bodies.  Each GPU the
+// library runs on holds its own copy until the process ends.  In host code
+// the array's name is only a host-side stand-in: kernels receive the device
+// address taken once per device by noalloc_setup.  Calls on one GPU share it
+// (each call writes every partial it reads): calls that overlap on one GPU
+// race.
+static __device__ __align__(256) double scratch[4 * kNoAllocBlocks + 4];
+static std::mutex g_scratch_mu;
+void* g_scratch[kNoAllocDevices];
+
+// The device address of scratch on `device`, taken once per device with that
+// device current (hmc_create calls it after selecting the context's device).
+int noalloc_setup(int device) {
+  if (device < 0 || device >= kNoAllocDevices) return HMC_ERR_INVALID;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return HMC_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) return HMC_ERR_DEVICE;
+  g_scratch[device] = p;
+  return HMC_OK;
+}
+
 }  // namespace hmc
 
 extern "C" {
@@ -174,49 +248,27 @@ int hmc_delta_energy_insert(hmc_ctx* c, const double* body_pos, uint64_t n, doub
 int hmc_total_energy(hmc_ctx* c, double* energy, double* bound) {
   if (!c || !energy) return HMC_ERR_INVALID;
   if (!c->have_state) return fail(c, HMC_ERR_STATE, "hmc_set_state was not called");
-  const Geometry& g = c->g;
-  const uint32_t N = g.n_bodies;
   HIPCHK(c, hipSetDevice(c->device));
-  if (N == 0) {
-    *energy = 0.0;
-    if (bound) *bound = 0.0;
-    return HMC_OK;
-  }
-  hipStream_t s = c->stream;
-  const uint32_t nblk = grid_for(N, kEnergyBlock);
+  if (c->g.n_bodies == 0) return total_energy_into(c, nullptr, energy, bound);
+  const uint32_t nblk = grid_for(c->g.n_bodies, kEnergyBlock);
   double* part = nullptr;
-  HIPCHK(c, hipMallocAsync((void**)&part, (4 * (size_t)nblk + 4) * sizeof(double), s));
-  // cells for the neighbour search: the checkerboard of the current counter
-  k_setup<<<1, 64, 0, s>>>(g, c->st, 0, 0, 0, 1.0, 1.0, 0, 0, 0u, 0);
-  if (g.dim == 2)
-    k_bin<2><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
-  else
-    k_bin<3><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
-  k_scan_blocks<<<c->n_scan_blocks, 256, 0, s>>>(c->w.cnt, c->w.start, c->w.block_sums, c->w.block_ne, g.n_spaces);
-  k_scan_top<<<1, 1024, 0, s>>>(g, c->st, c->w.block_sums, c->w.block_ne, c->n_scan_blocks, c->w.start, 0);
-  k_scan_add<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.start, c->w.block_sums, c->w.cnt, g.n_spaces);
-  k_scatter<<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->w);
-  if (g.dim == 2)
-    k_energy<2><<<nblk, kEnergyBlock, 0, s>>>(g, c->st, c->bodies, c->w, part);
-  else
-    k_energy<3><<<nblk, kEnergyBlock, 0, s>>>(g, c->st, c->bodies, c->w, part);
-  k_energy_final<<<1, kEnergyFinal, 0, s>>>(part, nblk, part + 4 * (size_t)nblk);
-  double out[4] = {0, 0, 0, 0};
-  hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipMemcpyAsync(out, part + 4 * (size_t)nblk, sizeof out, hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  (void)hipFreeAsync(part, s);
-  if (e != hipSuccess) return fail(c, HMC_ERR_DEVICE, std::string("total energy: ") + hipGetErrorString(e));
-  // the next apply rebuilds the cells; the neighbour copies were not touched
-  ApplyState st;
-  int rc = read_state(c, &st);
-  if (rc) return rc;
-  // any two summation orders of n terms differ by at most 2 gamma_{n-1} sum|terms|
-  const double n = std::max(out[2], 1.0);
-  const double gam = n * 0x1p-53 / (1.0 - n * 0x1p-53);
-  *energy = out[3] != 0.0 ? std::numeric_limits<double>::infinity() : out[0];
-  if (bound) *bound = out[3] != 0.0 ? 0.0 : 2.0 * gam * out[1];
-  return HMC_OK;
+  HIPCHK(c, hipMallocAsync((void**)&part, (4 * (size_t)nblk + 4) * sizeof(double), c->stream));
+  const int rc = total_energy_into(c, part, energy, bound);
+  (void)hipFreeAsync(part, c->stream);
+  return rc;
+}
+
+int hmc_total_energy_noalloc(hmc_ctx* c, double* energy, double* bound) {
+  if (!c || !energy) return HMC_ERR_INVALID;
+  if (!c->have_state) return fail(c, HMC_ERR_STATE, "hmc_set_state was not called");
+  if (grid_for(c->g.n_bodies, kEnergyBlock) > kNoAllocBlocks)
+    return fail(c, HMC_ERR_INVALID,
+                "hmc_total_energy_noalloc: more than 8,388,608 bodies (the static scratch holds the partials of "
+                "32,768 blocks of 256); use hmc_total_energy");
+  if (c->device < 0 || c->device >= kNoAllocDevices || !g_scratch[c->device])
+    return fail(c, HMC_ERR_DEVICE, "hmc_total_energy_noalloc: the scratch of this device was not set up");
+  HIPCHK(c, hipSetDevice(c->device));
+  return total_energy_into(c, (double*)g_scratch[c->device], energy, bound);
 }
 
 }  // extern "C"

@@ -1,6 +1,8 @@
 // Host-side state of a context and of a batch, and the helpers the C ABI's
 // translation units (api_*.hip) share.
 #pragma once
+#include <mutex>
+
 #include "kernels.hpp"
 
 using namespace hmc;
@@ -112,5 +114,14 @@ bool choose_regions(const Geometry& g, int D, double occ, int cus, TileCfg* out)
 int make_geometry(const hmc_config* cfg, Geometry& g, bool* fast, double* sigma_out, double* lmax_out);
 const char* sdf_check(const Geometry& g, double xmax, double dx, uint32_t nbins);
 Geometry sdf_geometry(const Geometry& g, double xmax);
+int total_energy_into(hmc_ctx* c, double* part, double* energy, double* bound);
+
+// The allocation-free total energy (hmc_total_energy_noalloc): its static
+// scratch holds the partials of kNoAllocBlocks energy blocks; noalloc_setup
+// records the scratch's device address per device (g_scratch).
+constexpr uint32_t kNoAllocBlocks = 32768;
+constexpr int kNoAllocDevices = 64;
+extern void* g_scratch[kNoAllocDevices];
+int noalloc_setup(int device);
 
 }  // namespace hmc

@@ -237,6 +237,16 @@ int hmc_run_translate(hmc_ctx* ctx, uint64_t step0, uint32_t seed, double kT, do
  * two summation orders of the same n terms. */
 int hmc_total_energy(hmc_ctx* ctx, double* energy, double* bound);
 
+/* hmc_total_energy without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same kernels and
+ * the same *energy and *bound, bit for bit; the block partials live in a
+ * static device array of the library (1 MiB per GPU, held until the process
+ * ends) instead of a per-call allocation.  HMC_ERR_INVALID above 8,388,608
+ * bodies.  Calls that overlap on one GPU (several contexts on several
+ * threads or streams) race on that array. */
+int hmc_total_energy_noalloc(hmc_ctx* ctx, double* energy, double* bound);
+
 /* PairwiseCutoff::delta_energy_insert (hoomd-interaction/src/pairwise_cutoff.rs:1017-1025,
  * with :420-488) of n candidate bodies (n x dim positions), each evaluated
  * alone against the current microstate: the energy of the site wrap(body + 0)
