@@ -33,7 +33,6 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=800, help="timed applies (800 = 200 sweeps x 4)")
     ap.add_argument("--warmup", type=int, default=40)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample length")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison run")
     ap.add_argument("--seed", type=int, default=42)
@@ -44,6 +43,8 @@ def parse():
                     help="time the kernels of every k-th timed apply with HIP events")
     ap.add_argument("--placement-only", action="store_true",
                     help="only run the QuickCompress placement of 1M spheres and print its figures")
+    ap.add_argument("--only", choices=["c1", "c3", "c4", "c5", "placement"],
+                    help="only run this other configuration (with its CPU baseline unless --no-cpu-baseline)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (HBM counter calibration)")
     return ap.parse_args()
@@ -131,9 +132,45 @@ def apply_bytes(n, n_spaces, trials, accepted):
     return 4 * (n_spaces + 1) + 20 * n + 32 * trials + 80 * accepted
 
 
-def other_workloads(hmc, wl):
-    """Short timed runs of the other configurations (trial moves/s; inputs
-    resident on the device; a few warm-up steps first)."""
+PEAK_FP64_TFLOPS = 78.6  # MI355X vector FP64 (no MFMA on this path)
+CPU_WARM_S, CPU_TIMED_S = 2.0, 4.0  # the reference harness (benchmarks/src/benchmark.rs:29-35, 46-89)
+
+
+def lj_trial_flops(n, n_spaces, density, r_cut, dim=3):
+    """Algorithmic flops of one LJ trial: the final and initial energies each
+    test every candidate of the 3^D neighbour spaces (3 sub, 3 mul-add and a
+    compare: 8 flops) and evaluate the pairs within r_cut (a division, 4
+    multiplies for s^12 and s^6, a subtraction, a multiply and the sum: 10)."""
+    cand = 3**dim * n / n_spaces
+    within = (4.0 / 3.0) * math.pi * r_cut**3 * density if dim == 3 else math.pi * r_cut**2 * density
+    return 2.0 * (8.0 * cand + 10.0 * within)
+
+
+def cpu_sample(make, kT, d, threads, parallel, what, warm=CPU_WARM_S, seconds=CPU_TIMED_S):
+    """Trial moves/s of the oracle (the C++ restatement of hoomd-rs 1.4.0) on
+    the host cores: `warm` seconds untimed, then `seconds` timed, as the
+    reference harness times its benchmarks."""
+    import _oracle as O
+    O.set_threads(threads)
+    sys_ = make()
+    sys_.time_applies(parallel, kT, d, warm)
+    moves, elapsed, applies = sys_.time_applies(parallel, kT, d, seconds)
+    del sys_
+    kind = "ParallelSweep applies (OpenMP over same-colour spaces)" if parallel else "serial Sweep applies"
+    return dict(value=moves / elapsed, unit="trial moves/s", cores=threads, kind="port",
+                sample=f"{applies} {kind} of {what} ({elapsed:.1f} s after {warm:.0f} s warm-up), "
+                       f"C++ restatement of hoomd-rs 1.4.0")
+
+
+def all_cores():
+    return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+
+
+def other_workloads(hmc, wl, cpu=True, only=None):
+    """Timed runs of the other configurations (trial moves/s; inputs resident
+    on the device; warm-up steps first), each with the roofline of its
+    dominant kernel and the oracle timed on the host cores beside it."""
+    import _oracle as O
     res = {}
 
     def timed(fn, warm, steps):
@@ -142,87 +179,198 @@ def other_workloads(hmc, wl):
         trials = fn(warm, steps)
         return trials / (time.perf_counter() - t0), trials
 
-    # C1: hard disks, 128 x 128 square lattice at packing fraction 0.70, d = 0.15
-    L, pos = wl.square_lattice_disks()
-    sim = hmc.GpuParallelSweep(L, wl.wrap(L, pos), precision="mixed")
-    v, t = timed(lambda s0, k: sum(sim.run(s0, 42, 1.0, 0.15, k)), 20, 400)
-    res["C1_hard_disks_16384"] = {"value": v, "unit": "trial moves/s", "steps": 400, "precision": "mixed",
-                                  "pass_layout": {2: "persistent", 1: "percolour", 0: "global"}[sim.stats()["tiled"]]}
-    sim.close()
-    # C4: shifted LJ, FCC 64 x 64 x 32 at number density 0.8, kT 1.2, r_cut 2.5, d = 0.1
-    L, pos = wl.fcc_number_density((64, 64, 32), 0.8)
-    sim = hmc.GpuParallelSweep(L, wl.wrap(L, pos), model="lj_shifted", params=(1.0, 1.0, 2.5, 2.5),
-                               precision="mixed")
-    sim.run(0, 42, 1.2, 0.1, 1)
-    r0 = sim.stats()["band_retests"]
-    t0 = time.perf_counter()
-    t = sum(sim.run(1, 42, 1.2, 0.1, 4))
-    v = t / (time.perf_counter() - t0)
-    res["C4_shifted_lj_524288"] = {"value": v, "unit": "trial moves/s", "steps": 4,
-                                   "precision": "mixed: FP32 pair energies, FP64 re-test inside the error bound",
-                                   "fp64_retest_fraction": (sim.stats()["band_retests"] - r0) / max(t, 1)}
-    sim.close()
-    sim = hmc.GpuParallelSweep(L, wl.wrap(L, pos), model="lj_shifted", params=(1.0, 1.0, 2.5, 2.5))
-    v64, _ = timed(lambda s0, k: sum(sim.run(s0, 42, 1.2, 0.1, k)), 1, 4)
-    res["C4_shifted_lj_524288"]["fp64_value"] = v64
-    sim.close()
-    # FP32 error of dH against FP64 (every trial of two applies, diagnostic run)
-    sim = hmc.GpuParallelSweep(L, wl.wrap(L, pos), model="lj_shifted", params=(1.0, 1.0, 2.5, 2.5),
-                               precision="mixed", fp32_error=True)
-    sim.run(0, 42, 1.2, 0.1, 2)
-    st = sim.stats()
-    res["C4_shifted_lj_524288"]["fp32_dH_error"] = {"max_abs": st["fp32_error_max"],
-                                                    "max_over_bound": st["fp32_error_bound"]}
-    sim.close()
-    # C3: unit cubes, simple cubic 64^3 at packing fraction 0.55; a step is a
-    # translate apply (d = 0.05) and a rotate apply (0.1), as the reference's
-    # polyhedron benchmark; counts are read back after every apply
-    L, pos = wl.simple_cubic()
-    sim = hmc.GpuParallelSweep(L, wl.wrap(L, pos), model="convex", vertices=wl.UNIT_CUBE,
-                               orientations=np.tile([1.0, 0.0, 0.0, 0.0], (len(pos), 1)))
-    state = {"sub": 0}
+    def want(k):
+        return only is None or only == k
 
-    def c3(s0, k):
-        tot = 0
-        for step in range(s0, s0 + k):
-            a, r, used = sim.apply(step, 0, 42, 1.0, 0.05)
-            a2, r2, _ = sim.apply_rotate(step, used, 42, 1.0, 0.1)
-            tot += a + r + a2 + r2
-        return tot
-    v, t = timed(c3, 1, 4)
-    res["C3_cubes_262144"] = {"value": v, "unit": "trial moves/s", "steps": 4, "precision": "fp64",
-                              "moves": "translate 0.05 + rotate 0.1 per step"}
-    sim.close()
-    # C5: 256 replicas of hard spheres (FCC, N 2,048 .. 62,500, packing
-    # fractions 0.30 .. 0.54), seeds 42 + r, d = 0.08, one graph per step
-    systems = wl.batched_replicas()
-    batch = hmc.GpuBatch([(L, wl.wrap(L, p), None) for L, p in systems], precision="mixed")
-    seeds = [42 + r for r in range(len(systems))]
-    tune_ms, policy = batch.autotune(0, seeds, 1.0, 0.08, 2)  # tile layout (state left unchanged)
-    v, t = timed(lambda s0, k: sum(a + r for a, r in batch.run(s0, seeds, 1.0, 0.08, k)), 2, 40)
-    res["C5_replicas_256"] = {"value": v, "unit": "trial moves/s", "steps": 40, "precision": "mixed",
-                              "bodies": int(sum(len(p) for _, p in systems)),
-                              "pass_layout": "tiles" if batch.stats(0)["tiled"] else "lanes",
-                              "autotuned_policy_cus": policy}
-    # SDF of every replica (x_max 0.02, dx 1e-4): one call after the run,
-    # then timed calls; beta P / rho from the single frame
-    nb = sum(len(p) for _, p in systems)
-    hist, ov = batch.sdf(0.02, 1e-4)
-    t0 = time.perf_counter()
-    for _ in range(5):
+    if want("c1"):
+        # C1: hard disks, 128 x 128 square lattice at packing fraction 0.70, d = 0.15
+        L, pos = wl.square_lattice_disks()
+        sim = hmc.GpuParallelSweep(L, wl.wrap(L, pos), precision="mixed")
+        sim.run(0, 42, 1.0, 0.15, 20)
+        sim.profile(True, 4)
+        t0 = time.perf_counter()
+        acc, rej = sim.run(20, 42, 1.0, 0.15, 400)
+        v = (acc + rej) / (time.perf_counter() - t0)
+        pm, pn, _bm, _bn = sim.read_profile()
+        st = sim.stats()
+        layout = {2: "persistent", 1: "percolour", 0: "global"}[st["tiled"]]
+        r = {"value": v, "unit": "trial moves/s", "steps": 400, "precision": "mixed", "pass_layout": layout}
+        if pn and st["tiled"] == 2:
+            ms = pm / pn
+            S = 134 * 134
+            b = apply_bytes(len(pos), S, (acc + rej) / 400, acc / 400)
+            r["roofline"] = {"bound": "hbm", "achieved": b / (ms * 1e-3) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                             "frac": b / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, "traffic": None,
+                             "kernel": "k_apply_tile<2>", "avg_launch_ms": ms, "algorithmic_bytes_per_launch": b}
+        sim.close()
+        if cpu:
+            mk = lambda: O.OracleSystem(L, wl.wrap(L, pos), seed=42)
+            r["cpu_baseline"] = cpu_sample(mk, 1.0, 0.15, 1, False, "the same 16,384 disks")
+            r["cpu_baseline_all_cores"] = cpu_sample(mk, 1.0, 0.15, all_cores(), True, "the same 16,384 disks")
+            r["gpu_over_cpu"] = v / r["cpu_baseline"]["value"]
+        res["C1_hard_disks_16384"] = r
+    if want("c4"):
+        # C4: shifted LJ, FCC 64 x 64 x 32 at number density 0.8, kT 1.2, r_cut 2.5, d = 0.1
+        prm = (1.0, 1.0, 2.5, 2.5)
+        L, pos = wl.fcc_number_density((64, 64, 32), 0.8)
+        sim = hmc.GpuParallelSweep(L, wl.wrap(L, pos), model="lj_shifted", params=prm, precision="mixed")
+        sim.run(0, 42, 1.2, 0.1, 2)
+        r0 = sim.stats()["band_retests"]
+        sim.profile(True, 1)
+        t0 = time.perf_counter()
+        acc, rej = sim.run(2, 42, 1.2, 0.1, 20)
+        t = acc + rej
+        v = t / (time.perf_counter() - t0)
+        pm, pn, bm, bn = sim.read_profile()
+        st = sim.stats()
+        e, _bound = sim.total_energy()
+        r = {"value": v, "unit": "trial moves/s", "steps": 20,
+             "precision": "mixed: FP32 pair energies, FP64 re-test inside the error bound",
+             "pass_layout": {2: "persistent", 1: "percolour", 0: "percolour"}[st["tiled"]],
+             "fp64_retest_fraction": (st["band_retests"] - r0) / max(t, 1),
+             "u_per_body_final": e / len(pos)}
+        shape = [int(np.floor(x / 2.5)) for x in L]
+        shape = [k - (k % 2) for k in shape]
+        n_spaces = int(np.prod(shape))
+        if pn:
+            ms = pm / pn
+            trials_per_launch = t / pn
+            fl = lj_trial_flops(len(pos), n_spaces, 0.8, 2.5) * trials_per_launch
+            ach = fl / (ms * 1e-3) / 1e12
+            r["roofline"] = {"bound": "valu_fp64", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                             "frac": ach / PEAK_FP64_TFLOPS, "traffic": None,
+                             "kernel": "k_apply_lj<3>" if st["tiled"] == 2 else "k_pass_lj<3,MIXED>",
+                             "avg_launch_ms": ms,
+                             "algorithmic_flops_per_launch": fl, "launches": pn,
+                             "flops_per_trial": lj_trial_flops(len(pos), n_spaces, 0.8, 2.5)}
+        sim.close()
+        sim = hmc.GpuParallelSweep(L, wl.wrap(L, pos), model="lj_shifted", params=prm)
+        v64, _ = timed(lambda s0, k: sum(sim.run(s0, 42, 1.2, 0.1, k)), 1, 4)
+        r["fp64_value"] = v64
+        sim.close()
+        # FP32 error of dH against FP64 (every trial of two applies, diagnostic run)
+        sim = hmc.GpuParallelSweep(L, wl.wrap(L, pos), model="lj_shifted", params=prm, precision="mixed",
+                                   fp32_error=True)
+        sim.run(0, 42, 1.2, 0.1, 2)
+        st = sim.stats()
+        r["fp32_dH_error"] = {"max_abs": st["fp32_error_max"], "max_over_bound": st["fp32_error_bound"]}
+        sim.close()
+        if cpu:
+            mk = lambda: O.OracleSystem(L, wl.wrap(L, pos), model="lj_shifted", params=prm, seed=42)
+            r["cpu_baseline"] = cpu_sample(mk, 1.2, 0.1, all_cores(), True, "the same 524,288 LJ bodies")
+            r["gpu_over_cpu"] = v / r["cpu_baseline"]["value"]
+        res["C4_shifted_lj_524288"] = r
+    if want("c3"):
+        # C3: unit cubes, simple cubic 64^3 at packing fraction 0.55; a step is a
+        # translate apply (d = 0.05) and a rotate apply (0.1), as the reference's
+        # polyhedron benchmark (hmc_run with both moves: no host reads between)
+        L, pos = wl.simple_cubic()
+        q0 = np.tile([1.0, 0.0, 0.0, 0.0], (len(pos), 1))
+        sim = hmc.GpuParallelSweep(L, wl.wrap(L, pos), model="convex", vertices=wl.UNIT_CUBE, orientations=q0)
+        moves = [("translate", 0.05), ("rotate", 0.1)]
+        sim.run_moves(0, 42, 1.0, moves, 1)
+        t0 = time.perf_counter()
+        cnt = sim.run_moves(1, 42, 1.0, moves, 4)
+        t = sum(a + b for a, b in cnt)
+        v = t / (time.perf_counter() - t0)
+        r = {"value": v, "unit": "trial moves/s", "steps": 4, "precision": "fp64",
+             "moves": "translate 0.05 + rotate 0.1 per step",
+             "acceptance": {"translate": cnt[0][0] / max(sum(cnt[0]), 1), "rotate": cnt[1][0] / max(sum(cnt[1]), 1)}}
+        sim.close()
+        if cpu:
+            def c3_cpu():
+                O.set_threads(all_cores())
+                ref = O.ConvexOracle(L, wl.wrap(L, pos), q0, wl.UNIT_CUBE, seed=42)
+                ref.time_steps(0.05, 0.1, CPU_WARM_S)
+                moves_, el, steps = ref.time_steps(0.05, 0.1, CPU_TIMED_S)
+                return dict(value=moves_ / el, unit="trial moves/s", cores=all_cores(), kind="port",
+                            sample=f"{steps} steps of ParallelSweep translate + rotate applies of the same 262,144 "
+                                   f"cubes ({el:.1f} s after {CPU_WARM_S:.0f} s warm-up), C++ restatement of "
+                                   f"hoomd-rs 1.4.0 with OpenMP over same-colour spaces")
+            r["cpu_baseline"] = c3_cpu()
+            r["gpu_over_cpu"] = v / r["cpu_baseline"]["value"]
+        res["C3_cubes_262144"] = r
+    if want("c5"):
+        # C5: 256 replicas of hard spheres (FCC, N 2,048 .. 62,500, packing
+        # fractions 0.30 .. 0.54), seeds 42 + r, d = 0.08, one graph per step
+        systems = wl.batched_replicas()
+        batch = hmc.GpuBatch([(L, wl.wrap(L, p), None) for L, p in systems], precision="mixed")
+        seeds = [42 + r for r in range(len(systems))]
+        tune_ms, policy = batch.autotune(0, seeds, 1.0, 0.08, 2)  # tile layout (state left unchanged)
+        batch.run(0, seeds, 1.0, 0.08, 2)
+        t0 = time.perf_counter()
+        cnt = batch.run(2, seeds, 1.0, 0.08, 40)
+        el = time.perf_counter() - t0
+        t = sum(a + b for a, b in cnt)
+        acc = sum(a for a, _ in cnt)
+        v = t / el
+        nb = int(sum(len(p) for _, p in systems))
+        r = {"value": v, "unit": "trial moves/s", "steps": 40, "precision": "mixed", "bodies": nb,
+             "pass_layout": "tiles" if batch.stats(0)["tiled"] else "lanes", "autotuned_policy_cus": policy}
+        # whole-step roofline: the build (bodies read twice, 4 index arrays
+        # written and read) and the passes (every colour pass reads every
+        # site copy and cell offset once) against the step's wall time
+        spaces = 0
+        for L_, p in systems:
+            k = [int(np.floor(x / 1.0)) for x in L_]
+            spaces += int(np.prod([x - (x % 2) for x in k]))
+        cycles = sum(batch.stats(i)["cycles"] for i in range(len(systems))) / len(systems)
+        passes = 8 * max(cycles, 1.0)
+        step_b = (2 * 32 * nb + 4 * 4 * nb + 3 * 4 * spaces) + passes * (4 * spaces + 16 * nb) + \
+            (36 * t + 80 * acc) / 40
+        ach = step_b / (el / 40) / 1e9
+        r["roofline"] = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": ach / PEAK_HBM_GBS, "traffic": None, "kernel": "whole step (build + passes)",
+                         "ms_per_step": el / 40 * 1e3, "algorithmic_bytes_per_step": step_b}
+        # SDF of every replica (x_max 0.02, dx 1e-4): one call after the run,
+        # then timed calls; beta P / rho from the single frame
         hist, ov = batch.sdf(0.02, 1e-4)
-    ms = (time.perf_counter() - t0) / 5 * 1e3
-    z = [hmc.sdf_beta_p_over_rho(hist[r], len(p), 1e-4, 3) for r, (_L, p) in enumerate(systems)]
-    res["C5_replicas_256"]["sdf"] = {"ms_per_call": ms, "bodies_per_s": nb / (ms * 1e-3), "overlaps": int(ov.sum()),
-                                     "beta_p_over_rho_one_frame": {"phi_0.30": z[0], "phi_0.54": z[-1]}}
-    batch.close()
-    # the same batch in plain FP64 (identical decisions)
-    batch = hmc.GpuBatch([(L, wl.wrap(L, p), None) for L, p in systems], precision="fp64")
-    v64, _ = timed(lambda s0, k: sum(a + r for a, r in batch.run(s0, seeds, 1.0, 0.08, k)), 2, 40)
-    res["C5_replicas_256"]["fp64_value"] = v64
-    res["C5_replicas_256"]["mixed_over_fp64"] = res["C5_replicas_256"]["value"] / v64
-    batch.close()
-    res["placement_quick_compress_1M"] = placement(hmc, wl)
+        t0 = time.perf_counter()
+        for _ in range(5):
+            hist, ov = batch.sdf(0.02, 1e-4)
+        ms = (time.perf_counter() - t0) / 5 * 1e3
+        z = [hmc.sdf_beta_p_over_rho(hist[i], len(p), 1e-4, 3) for i, (_L, p) in enumerate(systems)]
+        r["sdf"] = {"ms_per_call": ms, "bodies_per_s": nb / (ms * 1e-3), "overlaps": int(ov.sum()),
+                    "beta_p_over_rho_one_frame": {"phi_0.30": z[0], "phi_0.54": z[-1]}}
+        batch.close()
+        # the same batch in plain FP64 (identical decisions)
+        batch = hmc.GpuBatch([(L, wl.wrap(L, p), None) for L, p in systems], precision="fp64")
+        v64, _ = timed(lambda s0, k: sum(a + b for a, b in batch.run(s0, seeds, 1.0, 0.08, k)), 2, 40)
+        r["fp64_value"] = v64
+        r["mixed_over_fp64"] = v / v64
+        batch.close()
+        if cpu:
+            # every 16th replica, each for a share of the sample, one after
+            # another with all cores (the replicas are independent)
+            O.set_threads(all_cores())
+            moves = elapsed = 0.0
+            picks = list(range(0, len(systems), 16))
+            for i in picks:
+                L_, p = systems[i]
+                ref = O.OracleSystem(L_, wl.wrap(L_, p), seed=42 + i)
+                ref.time_applies(True, 1.0, 0.08, CPU_WARM_S / len(picks))
+                m_, e_, _n = ref.time_applies(True, 1.0, 0.08, CPU_TIMED_S / len(picks))
+                moves += m_
+                elapsed += e_
+                del ref
+            r["cpu_baseline"] = dict(value=moves / elapsed, unit="trial moves/s", cores=all_cores(), kind="port",
+                                     sample=f"{len(picks)} of the 256 replicas (every 16th), ParallelSweep applies "
+                                            f"of each in turn ({elapsed:.1f} s), C++ restatement of hoomd-rs 1.4.0 "
+                                            f"with OpenMP over same-colour spaces")
+            r["gpu_over_cpu"] = v / r["cpu_baseline"]["value"]
+        res["C5_replicas_256"] = r
+    if want("placement"):
+        res["placement_quick_compress_1M"] = placement(hmc, wl)
+        if cpu:
+            # the overlap-penalty sweeps of the placement (FCC 4*63^3 at packing
+            # fraction 0.45, d 0.1) on the same host cores
+            Lp, pp = wl.fcc_spheres(packing_fraction=0.45)
+            mk = lambda: O.OracleSystem(Lp, wl.wrap(Lp, pp), model="overlap_penalty", params=(1e4, 0.2, 100.0, 1.0),
+                                        seed=42)
+            pc = cpu_sample(mk, 1.0, 0.1, all_cores(), True, "the same 1,000,188 spheres under the overlap penalty")
+            pl = res["placement_quick_compress_1M"]
+            pl["cpu_baseline_sweeps"] = pc
+            pl["sweeps_gpu_over_cpu"] = pl["overlap_penalty_sweep_trial_moves_per_s"] / pc["value"]
     return res
 
 
@@ -264,18 +412,6 @@ def placement(hmc, wl):
     return res
 
 
-def cpu_baseline(L, pos, seconds, seed, kT, d, threads=None, model="hard_sphere", params=(1.0,)):
-    import _oracle as O
-    threads = threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
-    O.set_threads(threads)
-    sys_ = O.OracleSystem(L, pos, model=model, params=params, seed=seed)
-    sys_.time_applies(True, kT, d, 0.5)  # warm caches
-    moves, elapsed, applies = sys_.time_applies(True, kT, d, seconds)
-    return dict(value=moves / elapsed, unit="trial moves/s", cores=threads, kind="port",
-                sample=f"{applies} ParallelSweep applies of the same N=1,000,188 system ({elapsed:.1f} s), "
-                       f"C++ restatement of hoomd-rs 1.4.0 with OpenMP over same-colour spaces")
-
-
 def main():
     args = parse()
     if args.copy_calibration:
@@ -290,6 +426,9 @@ def main():
     wl = _pkg.workloads()
     if args.placement_only:
         print(json.dumps({"placement_quick_compress_1M": placement(hmc, wl)}), flush=True)
+        return
+    if args.only:
+        print(json.dumps(other_workloads(hmc, wl, cpu=not args.no_cpu_baseline, only=args.only)), flush=True)
         return
     kT, d = 1.0, 0.08
     L, pos = wl.fcc_spheres((63, 63, 63), packing_fraction=0.50)
@@ -355,24 +494,15 @@ def main():
                        "avg_pass_ms": f64["pass_ms"] / max(f64["pass_n"], 1),
                        "identical_decisions_and_positions": bool(same)}
         out["mixed_over_fp64"] = value / v64
-    if not args.no_other:
-        out["other_workloads"] = other_workloads(hmc, wl)
     if not args.no_cpu_baseline:
-        cb = cpu_baseline(L, pos, args.cpu_seconds, args.seed, kT, d)
-        out["cpu_baseline"] = cb
-        out["gpu_over_cpu"] = value / cb["value"]
-        # the same restatement on one core (a shorter sample)
-        out["cpu_baseline_1core"] = cpu_baseline(L, pos, args.cpu_seconds / 3.0, args.seed, kT, d, threads=1)
-        if not args.no_other:
-            # the overlap-penalty sweeps of the placement (FCC 4*63^3 at packing
-            # fraction 0.45, d 0.1) on the same host cores, a shorter sample
-            Lp, pp = wl.fcc_spheres(packing_fraction=0.45)
-            pc = cpu_baseline(Lp, wl.wrap(Lp, pp), args.cpu_seconds / 3.0, args.seed, 1.0, 0.1,
-                              model="overlap_penalty", params=(1e4, 0.2, 100.0, 1.0))
-            pc["sample"] = pc["sample"].replace("ParallelSweep applies", "overlap-penalty ParallelSweep applies")
-            pl = out["other_workloads"]["placement_quick_compress_1M"]
-            pl["cpu_baseline_sweeps"] = pc
-            pl["sweeps_gpu_over_cpu"] = pl["overlap_penalty_sweep_trial_moves_per_s"] / pc["value"]
+        import _oracle as O
+        mk = lambda: O.OracleSystem(L, pos, seed=args.seed)
+        what = "the same N=1,000,188 system"
+        out["cpu_baseline"] = cpu_sample(mk, kT, d, all_cores(), True, what)
+        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+        out["cpu_baseline_1core"] = cpu_sample(mk, kT, d, 1, False, what)
+    if not args.no_other:
+        out["other_workloads"] = other_workloads(hmc, wl, cpu=not args.no_cpu_baseline)
     print(json.dumps(out), flush=True)
 
 

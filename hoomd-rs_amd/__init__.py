@@ -5,8 +5,11 @@ include/hmc.h).  This module is a thin ctypes view of that ABI for Python
 callers (tests, benchmark).  It has no fallback: if libhmc.so is missing or
 no GPU is present the calls raise.
 """
+import atexit
 import ctypes as C
+import itertools
 import pathlib
+import weakref
 
 import numpy as np
 
@@ -33,10 +36,24 @@ EXPORTS = ["hmc_create", "hmc_set_state", "hmc_get_state", "hmc_clone_with_bound
            "hmc_quick_compress_init", "hmc_quick_compress_apply", "hmc_delta_energy_insert",
            "hmc_apply_rotate",
            "hmc_set_orientations", "hmc_get_orientations", "hmc_run_translate", "hmc_total_energy",
-           "hmc_total_energy_noalloc", "hmc_trace_enable", "hmc_trace_read", "hmc_get_stats", "hmc_profile_enable", "hmc_profile_read",
+           "hmc_total_energy_noalloc", "hmc_run", "hmc_moved_since", "hmc_get_bodies", "hmc_debug_regions",
+           "hmc_trace_enable", "hmc_trace_read", "hmc_get_stats", "hmc_profile_enable", "hmc_profile_read",
            "hmc_debug_math", "hmc_debug_normals", "hmc_debug_convex", "hmc_last_error", "hmc_destroy",
            "hmc_batch_create", "hmc_batch_set_state", "hmc_batch_get_state", "hmc_batch_run_translate",
            "hmc_batch_last_error", "hmc_batch_destroy", "hmc_sdf", "hmc_batch_sdf", "hmc_batch_get_stats", "hmc_batch_autotune"]
+
+
+# Contexts and batches still open when the interpreter exits are destroyed by
+# an atexit hook, in creation order, while the HIP runtime (and any profiler
+# attached to it) is still up -- not by __del__ during interpreter teardown.
+_LIVE = weakref.WeakValueDictionary()
+_LIVE_ID = itertools.count()
+
+
+@atexit.register
+def _close_all():
+    for _k, obj in sorted(list(_LIVE.items())):
+        obj.close()
 
 
 class HmcError(RuntimeError):
@@ -63,11 +80,21 @@ class Count(C.Structure):
     _fields_ = [("accepted", C.c_uint64), ("rejected", C.c_uint64)]
 
 
+class Move(C.Structure):
+    _fields_ = [("move", C.c_int32), ("pad", C.c_int32), ("size", C.c_double)]
+
+
+HMC_MOVE_TRANSLATE = 0
+HMC_MOVE_ROTATE = 1
+
+
 class Stats(C.Structure):
     _fields_ = [("applies", C.c_uint64), ("cycles", C.c_uint64), ("pair_tests", C.c_uint64),
                 ("band_retests", C.c_uint64), ("guard_cycles", C.c_uint64),
                 ("tile_overflows", C.c_uint64), ("tiled", C.c_uint32), ("tile_shape", C.c_uint32 * 3),
-                ("ambiguous", C.c_uint64), ("fp32_error_max", C.c_double), ("fp32_error_bound", C.c_double)]
+                ("ambiguous", C.c_uint64), ("fp32_error_max", C.c_double), ("fp32_error_bound", C.c_double),
+                ("regions", C.c_uint32), ("region_threads", C.c_uint32), ("regions_per_cu", C.c_uint32),
+                ("wait_timeout_at", C.c_uint32)]
 
 
 TRACE_DTYPE = np.dtype([("substep", "<u4"), ("color", "<u4"), ("space", "<u8"),
@@ -103,6 +130,12 @@ def lib():
         L.hmc_set_orientations.argtypes = [vp, D_P, D_P]
         L.hmc_total_energy.argtypes = [vp, D_P, D_P]
         L.hmc_total_energy_noalloc.argtypes = [vp, D_P, D_P]
+        L.hmc_run.argtypes = [vp, C.c_uint64, C.c_uint32, C.c_double, C.POINTER(Move), C.c_uint32, C.c_uint32,
+                              C.POINTER(Count)]
+        L.hmc_moved_since.argtypes = [vp, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
+                                      C.POINTER(C.c_uint64)]
+        L.hmc_get_bodies.argtypes = [vp, C.POINTER(C.c_uint32), C.c_uint64, D_P, D_P, D_P, D_P]
+        L.hmc_debug_regions.argtypes = [vp, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint64)]
         u64p = C.POINTER(C.c_uint64)
         L.hmc_sdf.argtypes = [vp, C.c_double, C.c_double, C.c_uint32, u64p, u64p]
         L.hmc_batch_sdf.argtypes = [vp, C.c_double, C.c_double, C.c_uint32, u64p, u64p]
@@ -212,6 +245,7 @@ class GpuParallelSweep:
         if rc != HMC_OK:
             raise HmcError(rc, "hmc_create failed")
         self.h = h
+        _LIVE[next(_LIVE_ID)] = self
         self.set_positions(pos, sites)
         if orientations is not None:
             self.set_orientations(orientations, site_orientations)
@@ -311,6 +345,55 @@ class GpuParallelSweep:
         cnt = Count()
         self._check(lib().hmc_run_translate(self.h, step0, seed, kT, max_distance, n_steps, C.byref(cnt)))
         return int(cnt.accepted), int(cnt.rejected)
+
+    def run_moves(self, step0, seed, kT, moves, n_steps):
+        """n_steps of {each (kind, size) of moves in order; increment_step}
+        (hmc_run); kind is "translate" or "rotate".  Returns one (accepted,
+        rejected) per move."""
+        arr = (Move * len(moves))()
+        for i, (kind, size) in enumerate(moves):
+            arr[i].move = {"translate": HMC_MOVE_TRANSLATE, "rotate": HMC_MOVE_ROTATE}[kind]
+            arr[i].size = size
+        out = (Count * len(moves))()
+        self._check(lib().hmc_run(self.h, step0, seed, kT, arr, len(moves), n_steps, out))
+        return [(int(c.accepted), int(c.rejected)) for c in out]
+
+    def debug_regions(self):
+        """Mailbox words of the persistent apply's regions, (regions, 6):
+        parity counters, probe counter, staged, passes published, waits passed."""
+        st = self.stats()
+        cap = 6 * max(st["regions"], 1)
+        out = np.zeros(cap, np.uint32)
+        n = C.c_uint64()
+        self._check(lib().hmc_debug_regions(self.h, out.ctypes.data_as(C.POINTER(C.c_uint32)), cap, C.byref(n)))
+        return out[: 6 * n.value].reshape(-1, 6)
+
+    def moved_since(self, since):
+        """(indices of the bodies moved by applies after epoch `since`, the
+        current epoch) -- hmc_moved_since."""
+        words = (self.n + 63) // 64
+        mask = np.zeros(max(words, 1), np.uint64)
+        now = C.c_uint32()
+        nm = C.c_uint64()
+        self._check(lib().hmc_moved_since(self.h, since, C.byref(now), mask.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                          C.byref(nm)))
+        bits = np.unpackbits(mask.view(np.uint8), bitorder="little")[: self.n]
+        idx = np.flatnonzero(bits).astype(np.uint32)
+        assert len(idx) == nm.value
+        return idx, int(now.value)
+
+    def get_bodies(self, index, orientations=False):
+        """Body and site positions (and, with orientations, body and site
+        quaternions) of the listed bodies (hmc_get_bodies)."""
+        idx = np.ascontiguousarray(index, np.uint32)
+        n = len(idx)
+        b = np.zeros((n, self.dim))
+        s = np.zeros((n, self.dim))
+        bq = np.zeros((n, 4)) if orientations else None
+        sq = np.zeros((n, 4)) if orientations else None
+        self._check(lib().hmc_get_bodies(self.h, idx.ctypes.data_as(C.POINTER(C.c_uint32)), n, _p(b), _p(s),
+                                         None if bq is None else _p(bq), None if sq is None else _p(sq)))
+        return (b, s, bq, sq) if orientations else (b, s)
 
     def positions(self):
         b = np.zeros((self.n, self.dim))
@@ -417,6 +500,7 @@ class GpuBatch:
         if rc != HMC_OK:
             raise HmcError(rc, "hmc_batch_create failed")
         self.h = h
+        _LIVE[next(_LIVE_ID)] = self
         for r, (_L, pos, site) in enumerate(systems):
             self.set_positions(r, pos, site)
 

@@ -52,26 +52,27 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
     (void)hipEventRecord(e1, s);
     c->build_events.push_back({e0, e1});
   }
-  if constexpr (PREC == HMC_MIXED) {
-    if (c->persist) {
-      hipEvent_t p0 = nullptr, p1 = nullptr;
-      if (prof) {
-        p0 = take_event(c);
-        (void)hipEventRecord(p0, s);
-      }
-      ApplyState* stp = c->st;
-      void* args[] = {(void*)&g, (void*)&stp, (void*)&c->bodies, (void*)&c->cells, (void*)&c->w, (void*)&c->pc};
-      const hipError_t le = hipLaunchCooperativeKernel((const void*)k_apply_tile<D>, dim3(c->pgrid),
-                                                       dim3(c->pthreads), args, c->pc.lds_bytes, s);
-      if (le != hipSuccess) return fail(c, HMC_ERR_DEVICE, std::string("cooperative launch: ") + hipGetErrorString(le));
-      if (prof) {
-        p1 = take_event(c);
-        (void)hipEventRecord(p1, s);
-        c->pass_events.push_back({p0, p1});
-      }
-      c->applies += 1;
-      return HMC_OK;
+  if (c->persist) {
+    // one persistent launch runs the apply: hard shapes (mixed precision) or
+    // isotropic pair energies (FP32 energies with the FP64 layout)
+    hipEvent_t p0 = nullptr, p1 = nullptr;
+    if (prof) {
+      p0 = take_event(c);
+      (void)hipEventRecord(p0, s);
     }
+    ApplyState* stp = c->st;
+    void* args[] = {(void*)&g, (void*)&stp, (void*)&c->bodies, (void*)&c->cells, (void*)&c->w, (void*)&c->pc};
+    const void* fn = is_isotropic(g.model) ? (const void*)k_apply_lj<D> : (const void*)k_apply_tile<D>;
+    const hipError_t le =
+        hipLaunchCooperativeKernel(fn, dim3(c->pgrid), dim3(c->pthreads), args, c->pc.lds_bytes, s);
+    if (le != hipSuccess) return fail(c, HMC_ERR_DEVICE, std::string("cooperative launch: ") + hipGetErrorString(le));
+    if (prof) {
+      p1 = take_event(c);
+      (void)hipEventRecord(p1, s);
+      c->pass_events.push_back({p0, p1});
+    }
+    c->applies += 1;
+    return HMC_OK;
   }
   const uint32_t blocks = grid_for(g.per_colour, kBlock);
   const uint32_t chunk = (blocks + 7) / 8;
@@ -166,7 +167,7 @@ void collect_events(hmc_ctx* c, std::vector<std::pair<hipEvent_t, hipEvent_t>>& 
   evs.clear();
 }
 
-int read_state(hmc_ctx* c, ApplyState* out) {
+int read_state(hmc_ctx* c, ApplyState* out, bool check_errors) {
   std::vector<unsigned long long> slots(kAccSlots);
   HIPCHK(c, hipMemcpyAsync(out, c->st, sizeof(ApplyState), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(slots.data(), c->w.acc_slots, kAccSlots * sizeof(unsigned long long),
@@ -178,6 +179,7 @@ int read_state(hmc_ctx* c, ApplyState* out) {
     collect_events(c, c->pass_events, c->pass_ms, c->pass_n);
     collect_events(c, c->build_events, c->build_ms, c->build_n);
   }
+  if (!check_errors) return HMC_OK;
   if (out->error & kErrBarrier) return fail(c, HMC_ERR_DEVICE, "a device-side wait (persistent apply neighbour sync or look-back scan) timed out");
   if (out->error) return fail(c, HMC_ERR_COVER, "a body lies outside the checkerboard");
   return HMC_OK;
@@ -324,6 +326,113 @@ bool choose_regions(const Geometry& g, int D, double occ, int cus, TileCfg* out)
           best_threads = threads;
           best_trials = trials;
           best_work = work;
+          for (int a = 0; a < 3; ++a) {
+            out->T[a] = a < D ? T[a] : 1;
+            out->nt[a] = a < D ? (g.half_shape[a] + T[a] - 1) / T[a] : 1;
+            out->E[a] = a < D ? 2 * T[a] + 2 : 1;
+          }
+          out->trials = trials;
+          out->ncell = ncell;
+          out->rows = rows;
+          out->maxb = maxb;
+          out->ntiles = ntiles;
+          out->xcd_chunk = (ntiles + 7) / 8;
+          out->lds_bytes = (uint32_t)lds;
+          uint32_t r = threads;
+          for (int a = D - 1; a > 0; --a) {
+            const uint32_t rad = a == D - 1 ? EL + 2 : out->E[a];
+            out->step[a] = r % rad;
+            r /= rad;
+          }
+          out->step[0] = r;
+          for (int a = D; a < 3; ++a) out->step[a] = 0;
+        }
+      }
+  return found;
+}
+
+// Whether every region of the context's persistent apply is resident at
+// once, checked on the device: the persistent kernel itself in probe mode
+// (region_probe), so the check sees the kernel's real register, LDS and
+// private-memory use, which the occupancy API may not all account for.  The
+// mailboxes and the state's error word are cleared afterwards.
+bool persist_probe(hmc_ctx* c) {
+  TileCfg pc = c->pc;
+  pc.probe = 1;
+  Geometry g = c->g;
+  ApplyState* stp = c->st;
+  void* args[] = {(void*)&g, (void*)&stp, (void*)&c->bodies, (void*)&c->cells, (void*)&c->w, (void*)&pc};
+  const int D = g.dim;
+  const void* fn = is_isotropic(g.model) ? (D == 2 ? (const void*)k_apply_lj<2> : (const void*)k_apply_lj<3>)
+                                         : (D == 2 ? (const void*)k_apply_tile<2> : (const void*)k_apply_tile<3>);
+  const size_t fbytes = (size_t)c->pc.ntiles * kMailboxStride * sizeof(uint32_t);
+  uint32_t err = 0;
+  bool ok = hipLaunchCooperativeKernel(fn, dim3(c->pgrid), dim3(c->pthreads), args, c->pc.lds_bytes, c->stream) ==
+                hipSuccess &&
+            hipMemcpyAsync(&err, &c->st->error, sizeof err, hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
+            hipStreamSynchronize(c->stream) == hipSuccess;
+  ok = ok && err == 0;
+  (void)hipMemsetAsync(c->w.mbox, 0, fbytes, c->stream);
+  (void)hipMemsetAsync(&c->st->error, 0, sizeof(uint32_t), c->stream);
+  (void)hipStreamSynchronize(c->stream);
+  return ok;
+}
+
+// Workgroup size of k_apply_lj for a region of `trials` trials: one
+// half-wavefront per trial up to eight, in 1, 2 or 4 wavefronts (a count
+// that divides the four SIMDs of a CU, so that the wavefronts of several
+// workgroups on one CU spread evenly; with five, two workgroups did not
+// always fit on a CU that the occupancy count said holds them).
+static uint32_t lj_region_threads(uint32_t trials) { return trials <= 2 ? 64u : (trials <= 4 ? 128u : 256u); }
+
+// Region shape of the persistent isotropic apply (k_apply_lj): as
+// choose_regions, with at most 12 trials (half-wavefronts taking one or two
+// each), at most 128 cells of a colour's halo, and a site capacity of 1.1x
+// the box's expected count plus 128.  Among the shapes whose workgroups all
+// fit on the device at once (two waves per SIMD, the kernel's register
+// budget; the host checks the real occupancy): the least staging over all
+// regions, then the fewest lanes.
+bool choose_regions_lj(const Geometry& g, int D, double occ, int cus, TileCfg* out) {
+  uint32_t lim[3] = {1, 1, 1};
+  for (int a = 0; a < D; ++a) {
+    if (g.shape[a] < 6 || g.reach[a] != 1) return false;
+    lim[a] = std::min<uint32_t>(std::min<uint32_t>(g.half_shape[a], 8u), (uint32_t)(g.shape[a] - 4) / 2);
+  }
+  // ziggurat tables, halo lists, 12 proposals of 104 bytes, 8 x 9 row offsets
+  const size_t static_lds = 2 * 257 * sizeof(double) + (2 * 128 + 1) * sizeof(uint32_t) + 12 * 104 + 8 * 9 * 4 + 64;
+  const size_t cu_lds = 160 * 1024;
+  const uint32_t cu_waves = 8;
+  bool found = false;
+  double best_work = 0.0;
+  uint32_t best_threads = 0;
+  for (uint32_t t0 = 1; t0 <= lim[0]; ++t0)
+    for (uint32_t t1 = 1; t1 <= lim[1]; ++t1)
+      for (uint32_t t2 = 1; t2 <= lim[2]; ++t2) {
+        const uint32_t T[3] = {t0, t1, t2};
+        const uint32_t trials = T[0] * T[1] * T[2];
+        if (trials > 12) continue;
+        const uint32_t threads = lj_region_threads(trials);
+        uint32_t ncell = 1, ntiles = 1, halo = 1;
+        for (int a = 0; a < D; ++a) {
+          ncell *= 2 * T[a] + 2;
+          ntiles *= (g.half_shape[a] + T[a] - 1) / T[a];
+          halo *= T[a] + 1;
+        }
+        if (halo > 128) continue;
+        const uint32_t EL = 2 * T[D - 1] + 2, rows = ncell / EL;
+        if (rows > 64u * kRowsPerLane || rows * (EL + 2) > threads * kValsPerThread) continue;
+        const uint32_t maxb = (uint32_t)std::ceil(1.1 * occ * ncell) + 128;  // >= 96: the overflow path's tables
+        const size_t lds = ((16 * ((size_t)rows + 1) + 4 * (size_t)rows * (EL + 1) + 15) & ~(size_t)15) +
+                           (size_t)maxb * sizeof(float4);
+        if (lds + static_lds > cu_lds) continue;
+        const uint32_t per_cu = std::min<uint32_t>((uint32_t)(cu_lds / (lds + static_lds)), cu_waves / (threads / 64));
+        const uint32_t grid = ((ntiles + 7) / 8) * 8;
+        if (per_cu == 0 || grid > per_cu * (uint32_t)cus) continue;
+        const double work = (double)ntiles * ncell;
+        if (!found || work < best_work || (work == best_work && threads < best_threads)) {
+          found = true;
+          best_work = work;
+          best_threads = threads;
           for (int a = 0; a < 3; ++a) {
             out->T[a] = a < D ? T[a] : 1;
             out->nt[a] = a < D ? (g.half_shape[a] + T[a] - 1) / T[a] : 1;
@@ -601,11 +710,61 @@ int hmc_create(const hmc_config* cfg_in, hmc_ctx** out) {
           c->pc = pc;
           c->pgrid = pc.xcd_chunk * 8u;
           c->pthreads = threads;
+          c->pper_cu = (uint32_t)per_cu;
           const size_t fbytes = (size_t)pc.ntiles * kMailboxStride * sizeof(uint32_t);
           if (hipMalloc((void**)&c->w.mbox, fbytes) != hipSuccess || hipMemset(c->w.mbox, 0, fbytes) != hipSuccess) {
             delete c;
             return HMC_ERR_DEVICE;
           }
+        }
+      }
+    }
+  }
+  // Isotropic pair energies in mixed precision: one persistent launch per
+  // apply (k_apply_lj) when every axis has at least six spaces and the
+  // neighbour scan reaches one space out.
+  if (lj_mixed && !(cfg_in->flags & (HMC_FLAG_PER_COLOUR | HMC_FLAG_FP32_ERROR | HMC_FLAG_GLOBAL_PASS))) {
+    int cus = 0, coop = 0;
+    TileCfg pc{};
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess && cus > 0 &&
+        hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, c->device) == hipSuccess && coop &&
+        choose_regions_lj(g, D, (double)cfg->n_bodies / (double)spaces, cus, &pc)) {
+      // Per-axis bound E on the FP32 displacement of two staged entries
+      // against the reference's FP64 displacement: the rounding of each
+      // box-relative coordinate to FP32 (2^-24 R, R the largest), the FP64
+      // roundings of x - c and of the image shift on both sides
+      // (2^-53 (L + R) each) and the reference's own displacement and image
+      // shift (2^-53 (r_cut + 1.5 L)).  The membership band around r_cut^2
+      // is the pair bound beta of k_apply_lj at |d| = r_cut, widened by 1%.
+      double R = 0.0;
+      for (int a = 0; a < D; ++a) R = std::max(R, 0.5 * pc.E[a] * g.width[a] + g.width[a]);
+      const double rc2 = g.lj_rc2, rc = std::sqrt(rc2) * (1.0 + 0x1p-20);
+      const double E = (0x1p-23 * R + 0x1p-53 * (3.5 * lmax + 2.0 * R + rc)) * 1.01;
+      const double beta = (E * (2.0 * std::sqrt((double)D) * rc + D * E) + rc * rc * (5.0 * 0x1p-24 + 0x1p-50)) * 1.01;
+      float lo = (float)(rc2 - beta), hi = (float)(rc2 + beta);
+      while ((double)lo > rc2 - beta) lo = std::nextafter(lo, 0.0f);
+      while ((double)hi < rc2 + beta) hi = std::nextafter(hi, INFINITY);
+      float pe = (float)E;
+      while ((double)pe < E) pe = std::nextafter(pe, INFINITY);
+      pc.band_lo = lo;
+      pc.band_hi = hi;
+      pc.perr = pe;
+      const void* fn = D == 2 ? (const void*)k_apply_lj<2> : (const void*)k_apply_lj<3>;
+      const uint32_t threads = lj_region_threads(pc.trials);
+      int per_cu = 0;
+      if (beta < 0.01 * rc2 &&
+          hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pc.lds_bytes) == hipSuccess &&
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, pc.lds_bytes) == hipSuccess &&
+          (uint64_t)per_cu * (uint64_t)cus >= (uint64_t)pc.xcd_chunk * 8u) {
+        c->persist = true;
+        c->pc = pc;
+        c->pgrid = pc.xcd_chunk * 8u;
+        c->pthreads = threads;
+        c->pper_cu = (uint32_t)per_cu;
+        const size_t fbytes = (size_t)pc.ntiles * kMailboxStride * sizeof(uint32_t);
+        if (hipMalloc((void**)&c->w.mbox, fbytes) != hipSuccess || hipMemset(c->w.mbox, 0, fbytes) != hipSuccess) {
+          delete c;
+          return HMC_ERR_DEVICE;
         }
       }
     }
@@ -658,6 +817,7 @@ int hmc_create(const hmc_config* cfg_in, hmc_ctx** out) {
     hmc_destroy(c);
     return HMC_ERR_DEVICE;
   }
+  if (c->persist && !persist_probe(c)) c->persist = false;  // the per-colour passes instead
   *out = c;
   return HMC_OK;
 }
@@ -899,35 +1059,65 @@ int hmc_apply_rotate(hmc_ctx* c, uint64_t step, uint32_t substep, uint32_t seed,
   return apply_one(c, step, substep, seed, kT, max_rotation, 1u, out, substeps_used);
 }
 
-int hmc_run_translate(hmc_ctx* c, uint64_t step0, uint32_t seed, double kT, double max_distance,
-                      uint32_t n_steps, hmc_count* out) {
+static bool same_moves(const std::vector<hmc_move>& a, const hmc_move* b, uint32_t n) {
+  if (a.size() != n) return false;
+  for (uint32_t i = 0; i < n; ++i)
+    if (a[i].move != b[i].move || a[i].size != b[i].size) return false;
+  return true;
+}
+
+int hmc_run(hmc_ctx* c, uint64_t step0, uint32_t seed, double kT, const hmc_move* moves, uint32_t n_moves,
+            uint32_t n_steps, hmc_count* out) {
   if (!c) return HMC_ERR_INVALID;
   if (!c->have_state) return fail(c, HMC_ERR_STATE, "hmc_set_state was not called");
-  if (!(max_distance > 0.0) || !(kT > 0.0)) return fail(c, HMC_ERR_INVALID, "max_distance and kT must be > 0");
+  if (!moves || n_moves < 1 || n_moves > (uint32_t)kMaxMoves)
+    return fail(c, HMC_ERR_INVALID, "hmc_run: 1 to 4 moves per step");
+  if (!(kT > 0.0)) return fail(c, HMC_ERR_INVALID, "kT must be > 0");
+  for (uint32_t m = 0; m < n_moves; ++m) {
+    if (moves[m].move != HMC_MOVE_TRANSLATE && moves[m].move != HMC_MOVE_ROTATE)
+      return fail(c, HMC_ERR_INVALID, "hmc_run: unknown move");
+    if (!(moves[m].size > 0.0)) return fail(c, HMC_ERR_INVALID, "the move sizes must be > 0");
+    if (moves[m].move == HMC_MOVE_ROTATE && c->g.model != HMC_CONVEX_POLYHEDRON)
+      return fail(c, HMC_ERR_UNSUPPORTED, "rotation moves need oriented bodies");
+  }
   HIPCHK(c, hipSetDevice(c->device));
   ApplyState before;
   int rc = read_state(c, &before);
   if (rc) return rc;
+  const bool tally = n_moves > 1;
+  if (tally) k_tally<<<1, kAccSlots, 0, c->stream>>>(c->st, c->w.acc_slots, (uint32_t)kMaxMoves);
   const bool lj = is_isotropic(c->g.model);
   // (a cooperative launch is not captured: the persistent apply launches directly)
   const bool graphs = !(c->cfg.flags & HMC_FLAG_NO_GRAPH) && !c->persist;
+  // one step: apply each move in order (the step's substeps follow on), then
+  // Microstate::increment_step before the next step
+  auto step_launches = [&](bool first) -> int {
+    for (uint32_t m = 0; m < n_moves; ++m) {
+      const int from_args = first && m == 0 ? 1 : 0;
+      const int advance = m > 0 ? 2 : (first ? 0 : 1);
+      const int r = dispatch_apply(c, from_args, advance, step0, 0, seed, kT, moves[m].size, (uint32_t)moves[m].move);
+      if (r) return r;
+      if (tally) k_tally<<<1, kAccSlots, 0, c->stream>>>(c->st, c->w.acc_slots, m);
+    }
+    return HMC_OK;
+  };
   for (uint32_t i = 0; i < n_steps; ++i) {
     const bool timed = c->profiling && c->applies % c->profile_period == 0;
     if (i == 0 || !graphs || timed) {
-      rc = dispatch_apply(c, i == 0 ? 1 : 0, i == 0 ? 0 : 1, step0, 0, seed, kT, max_distance, 0u);
+      rc = step_launches(i == 0);
       if (rc) return rc;
       continue;
     }
-    // later steps: replay one captured { apply; increment_step }
+    // later steps: replay one captured step
     const uint32_t launched = c->cycles_guess + (lj ? 1u : 0u);
-    if (!c->graph_exec || c->graph_kT != kT || c->graph_move_size != max_distance || c->graph_move != 0u ||
+    if (!c->graph_exec || c->graph_kT != kT || !same_moves(c->graph_moves, moves, n_moves) ||
         c->graph_launched != launched) {
       drop_graph(c);
       HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       c->capturing = true;
-      rc = dispatch_apply(c, 0, 1, step0, 0, seed, kT, max_distance, 0u);
+      rc = step_launches(false);
       c->capturing = false;
-      c->applies -= 1;  // counted when replayed
+      c->applies -= n_moves;  // counted when replayed
       hipGraph_t gr = nullptr;
       const hipError_t ce = hipStreamEndCapture(c->stream, &gr);
       if (rc) return rc;
@@ -935,22 +1125,118 @@ int hmc_run_translate(hmc_ctx* c, uint64_t step0, uint32_t seed, double kT, doub
       c->graph = gr;
       HIPCHK(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
       c->graph_kT = kT;
-      c->graph_move_size = max_distance;
-      c->graph_move = 0u;
+      c->graph_moves.assign(moves, moves + n_moves);
       c->graph_launched = launched;
     }
     HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
-    c->applies += 1;
+    c->applies += n_moves;
   }
   ApplyState after;
   rc = read_state(c, &after);
   if (rc) return rc;
   if (n_steps) c->cycles_guess = std::max<uint32_t>(after.cycles, 1);
-  uint64_t acc = after.accepted_total - before.accepted_total;
-  uint64_t tri = after.trials_total - before.trials_total;
   if (out) {
-    out->accepted = acc;
-    out->rejected = tri - acc;
+    if (tally) {
+      for (uint32_t m = 0; m < n_moves; ++m) {
+        out[m].accepted = after.move_acc[m];
+        out[m].rejected = after.move_trials[m] - after.move_acc[m];
+      }
+    } else {
+      const uint64_t acc = after.accepted_total - before.accepted_total;
+      const uint64_t tri = after.trials_total - before.trials_total;
+      out[0].accepted = acc;
+      out[0].rejected = tri - acc;
+    }
+  }
+  return HMC_OK;
+}
+
+int hmc_run_translate(hmc_ctx* c, uint64_t step0, uint32_t seed, double kT, double max_distance,
+                      uint32_t n_steps, hmc_count* out) {
+  if (!c) return HMC_ERR_INVALID;
+  if (!(max_distance > 0.0) || !(kT > 0.0)) return fail(c, HMC_ERR_INVALID, "max_distance and kT must be > 0");
+  hmc_move m{};
+  m.move = HMC_MOVE_TRANSLATE;
+  m.size = max_distance;
+  return hmc_run(c, step0, seed, kT, &m, 1, n_steps, out);
+}
+
+int hmc_moved_since(hmc_ctx* c, uint32_t since, uint32_t* now, uint64_t* mask, uint64_t* n_moved) {
+  if (!c || !mask) return HMC_ERR_INVALID;
+  if (!c->have_state) return fail(c, HMC_ERR_STATE, "hmc_set_state was not called");
+  HIPCHK(c, hipSetDevice(c->device));
+  const uint32_t N = c->g.n_bodies;
+  const size_t words = ((size_t)N + 63) / 64;
+  hipStream_t s = c->stream;
+  unsigned long long* d = nullptr;  // mask words, then the count
+  HIPCHK(c, hipMallocAsync((void**)&d, (words + 1) * sizeof(unsigned long long), s));
+  hipError_t e = hipMemsetAsync(d + words, 0, sizeof(unsigned long long), s);
+  if (e == hipSuccess && N) {
+    k_moved_mask<<<grid_for(N, 256), 256, 0, s>>>(c->bodies.bpos, N, since, d, d + words);
+    e = hipGetLastError();
+  }
+  unsigned long long cnt = 0;
+  if (e == hipSuccess && words) e = hipMemcpyAsync(mask, d, words * sizeof(uint64_t), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(&cnt, d + words, sizeof cnt, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFreeAsync(d, s);
+  if (e != hipSuccess) return fail(c, HMC_ERR_DEVICE, std::string("moved bodies: ") + hipGetErrorString(e));
+  ApplyState st;
+  const int rc = read_state(c, &st);
+  if (rc) return rc;
+  if (now) *now = st.epoch;
+  if (n_moved) *n_moved = cnt;
+  return HMC_OK;
+}
+
+int hmc_get_bodies(hmc_ctx* c, const uint32_t* index, uint64_t n, double* body_pos, double* site_pos,
+                   double* body_quat, double* site_quat) {
+  if (!c || (n && !index)) return HMC_ERR_INVALID;
+  if (!c->have_state) return fail(c, HMC_ERR_STATE, "hmc_set_state was not called");
+  const uint32_t N = c->g.n_bodies;
+  const int D = c->g.dim;
+  if ((body_quat || site_quat) && !c->bodies.bquat)
+    return fail(c, HMC_ERR_UNSUPPORTED, "the bodies have no orientations");
+  if (n >= 0xffffffffull) return fail(c, HMC_ERR_INVALID, "too many bodies");
+  for (uint64_t k = 0; k < n; ++k)
+    if (index[k] >= N) return fail(c, HMC_ERR_INVALID, "a body index is out of range");
+  if (n == 0) return HMC_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  uint32_t* di = nullptr;
+  double4* dout = nullptr;
+  hipError_t e = hipMallocAsync((void**)&di, n * sizeof(uint32_t), s);
+  if (e == hipSuccess) e = hipMallocAsync((void**)&dout, 4 * n * sizeof(double4), s);
+  if (e == hipSuccess) e = hipMemcpyAsync(di, index, n * sizeof(uint32_t), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    k_gather_bodies<<<grid_for(n, 256), 256, 0, s>>>(c->bodies, di, (uint32_t)n, dout);
+    e = hipGetLastError();
+  }
+  std::vector<double4> h(4 * n);
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), dout, 4 * n * sizeof(double4), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (di) (void)hipFreeAsync(di, s);
+  if (dout) (void)hipFreeAsync(dout, s);
+  if (e != hipSuccess) return fail(c, HMC_ERR_DEVICE, std::string("gather bodies: ") + hipGetErrorString(e));
+  for (uint64_t k = 0; k < n; ++k) {
+    const double4 b = h[4 * k], st = h[4 * k + 1], bq = h[4 * k + 2], sq = h[4 * k + 3];
+    const double bv[3] = {b.x, b.y, b.z}, sv[3] = {st.x, st.y, st.z};
+    for (int a = 0; a < D; ++a) {
+      if (body_pos) body_pos[k * D + a] = bv[a];
+      if (site_pos) site_pos[k * D + a] = sv[a];
+    }
+    if (body_quat) {
+      body_quat[4 * k] = bq.x;
+      body_quat[4 * k + 1] = bq.y;
+      body_quat[4 * k + 2] = bq.z;
+      body_quat[4 * k + 3] = bq.w;
+    }
+    if (site_quat) {
+      site_quat[4 * k] = sq.x;
+      site_quat[4 * k + 1] = sq.y;
+      site_quat[4 * k + 2] = sq.z;
+      site_quat[4 * k + 3] = sq.w;
+    }
   }
   return HMC_OK;
 }
@@ -994,7 +1280,7 @@ int hmc_trace_read(hmc_ctx* c, hmc_trace_record* buf, uint64_t cap, uint64_t* n)
 int hmc_get_stats(hmc_ctx* c, hmc_stats* out) {
   if (!c || !out) return HMC_ERR_INVALID;
   ApplyState s;
-  int rc = read_state(c, &s);
+  int rc = read_state(c, &s, false);  // (the stats also describe a failed apply)
   if (rc) return rc;
   out->applies = c->applies;
   out->cycles = s.cycles_total;
@@ -1007,6 +1293,26 @@ int hmc_get_stats(hmc_ctx* c, hmc_stats* out) {
   std::memcpy(&out->fp32_error_bound, &s.fp32_err_ratio, sizeof(double));
   out->tiled = c->persist ? 2u : (c->tiled ? 1u : 0u);
   for (int a = 0; a < 3; ++a) out->tile_shape[a] = c->persist ? c->pc.T[a] : (c->tiled ? c->tc.T[a] : 0u);
+  out->regions = c->persist ? c->pc.ntiles : 0u;
+  out->region_threads = c->persist ? c->pthreads : 0u;
+  out->regions_per_cu = c->persist ? c->pper_cu : 0u;
+  out->wait_timeout_at = s.wait_at;
+  return HMC_OK;
+}
+
+int hmc_debug_regions(hmc_ctx* c, uint32_t* out, uint64_t cap, uint64_t* n) {
+  if (!c || !out || !n) return HMC_ERR_INVALID;
+  *n = 0;
+  if (!c->persist || !c->w.mbox) return HMC_OK;
+  const uint64_t words = (uint64_t)c->pc.ntiles * kMailboxStride;
+  std::vector<uint32_t> h(words);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->w.mbox, words * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const uint64_t m = std::min<uint64_t>(cap / 6, c->pc.ntiles);
+  for (uint64_t r = 0; r < m; ++r)
+    for (int k = 0; k < 6; ++k) out[6 * r + k] = h[r * kMailboxStride + k];
+  *n = m;
   return HMC_OK;
 }
 

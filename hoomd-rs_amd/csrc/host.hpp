@@ -25,7 +25,7 @@ struct hmc_ctx {
   TileCfg tc{};
   bool persist = false;  // one cooperative launch per apply (k_apply_tile)
   TileCfg pc{};
-  uint32_t pgrid = 0, pthreads = 0;
+  uint32_t pgrid = 0, pthreads = 0, pper_cu = 0;
   double* verts = nullptr;  // convex polyhedron vertices (device)
   Geometry* gdev = nullptr;  // device copy of g (g.self)
   uint32_t n_scan_blocks = 0;
@@ -37,8 +37,9 @@ struct hmc_ctx {
   // one apply (with increment_step) captured as a graph, and what it depends on
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
-  double graph_kT = 0.0, graph_move_size = 0.0;
-  uint32_t graph_move = 0, graph_launched = 0;
+  double graph_kT = 0.0;
+  std::vector<hmc_move> graph_moves;  // the step the graph holds: one apply per move
+  uint32_t graph_launched = 0;
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pass_events, build_events;
   double pass_ms = 0.0, build_ms = 0.0;
@@ -107,10 +108,11 @@ int dispatch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32
 void drop_graph(hmc_ctx* c);
 void collect_events(hmc_ctx* c, std::vector<std::pair<hipEvent_t, hipEvent_t>>& evs, double& ms_total,
                     uint64_t& n);
-int read_state(hmc_ctx* c, ApplyState* out);
+int read_state(hmc_ctx* c, ApplyState* out, bool check_errors = true);
 bool contact_band(double E, double sigma, double sigma2, double edge2, int D, float* lo, float* hi);
 bool choose_tiles(const Geometry& g, int D, double occ, size_t vt, int cus, TileCfg* out);
 bool choose_regions(const Geometry& g, int D, double occ, int cus, TileCfg* out);
+bool choose_regions_lj(const Geometry& g, int D, double occ, int cus, TileCfg* out);
 int make_geometry(const hmc_config* cfg, Geometry& g, bool* fast, double* sigma_out, double* lmax_out);
 const char* sdf_check(const Geometry& g, double xmax, double dx, uint32_t nbins);
 Geometry sdf_geometry(const Geometry& g, double xmax);

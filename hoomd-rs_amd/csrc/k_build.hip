@@ -6,6 +6,8 @@
 namespace hmc {
 
 // ----------------------------------------------------------------- kernels --
+// advance: 0 keep the counter, 1 increment_step first, 2 continue after
+// the previous apply's substeps (the next trial of the same step).
 // new_epoch: 1 for an apply (its build tags the look-back scan's status words
 // with the new epoch), 0 for the other users of the build (energies, SDF,
 // insertion energies), which scan with three kernels and leave the words and
@@ -18,10 +20,14 @@ __global__ void k_setup(Geometry g, ApplyState* st, uint64_t step, uint32_t subs
     st->step = step;
     st->substep0 = substep;
     st->seed = seed;
-  } else if (advance) {
+  } else if (advance == 1) {
     // Microstate::increment_step (microstate.rs:412-415)
     st->step += 1;
     st->substep0 = 0;
+  } else if (advance == 2) {
+    // the next trial of the same step: the substeps the previous apply used
+    // (one per colour cycle, parallel_sweep.rs:414-419) follow on
+    st->substep0 += st->cycles;
   }
   st->kT = kT;
   st->max_move = max_move;
@@ -33,6 +39,31 @@ __global__ void k_setup(Geometry g, ApplyState* st, uint64_t step, uint32_t subs
   st->cycles = 0;
   st->nonempty = 0;
   if (new_epoch) st->epoch += 1;
+}
+
+// After an apply of move `m` of a several-move step (hmc_run): the accepted
+// moves (the sum of the accept slots) and trials since the last tally are
+// added to that move's counters (m = kMaxMoves: clear them and only record
+// the totals).  One workgroup of kAccSlots = 1024 threads.
+__global__ void __launch_bounds__(1024) k_tally(ApplyState* st, const unsigned long long* acc_slots, uint32_t m) {
+  __shared__ unsigned long long part[16];
+  unsigned long long v = acc_slots[threadIdx.x];
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long acc = 0;
+    for (int i = 0; i < 16; ++i) acc += part[i];
+    if (m < (uint32_t)kMaxMoves) {
+      st->move_acc[m] += acc - st->tally_acc;
+      st->move_trials[m] += st->trials_total - st->tally_trials;
+    } else {  // m = kMaxMoves: start of a run, the per-move counters start at 0
+      for (int i = 0; i < kMaxMoves; ++i) st->move_acc[i] = st->move_trials[i] = 0;
+    }
+    st->tally_acc = acc;
+    st->tally_trials = st->trials_total;
+  }
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {

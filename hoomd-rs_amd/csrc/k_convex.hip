@@ -72,7 +72,7 @@ __device__ void trial_convex(const Geometry& g, const ApplyState* st, const Bodi
       const double r0a[3] = {r0.x, r0.y, r0.z};
       for (int a = 0; a < 3; ++a) ns3[a] = wrap1(nb3[a] + r0a[a], g.L[a]);
       const uint32_t fl = ghost_flags<3>(g, ns3);
-      B.bpos[body] = make_double4(nb3[0], nb3[1], nb3[2], 0.0);
+      B.bpos[body] = make_double4(nb3[0], nb3[1], nb3[2], move_stamp(st->epoch));
       B.bquat[body] = make_double4(q.s, q.v.x, q.v.y, q.v.z);
       const double4 sp = pack_site(ns3, fl);
       const double4 sq = make_double4(qt.s, qt.v.x, qt.v.y, qt.v.z);
@@ -356,7 +356,7 @@ __global__ void __launch_bounds__(kBlock, 3) k_pass_convex(Geometry g, ApplyStat
       const double r0a[3] = {r0.x, r0.y, r0.z};
       for (int a = 0; a < 3; ++a) ns3[a] = wrap1(nb3[a] + r0a[a], g.L[a]);
       const uint32_t fl = ghost_flags<3>(g, ns3);
-      B.bpos[body] = make_double4(nb3[0], nb3[1], nb3[2], 0.0);
+      B.bpos[body] = make_double4(nb3[0], nb3[1], nb3[2], move_stamp(st->epoch));
       B.bquat[body] = make_double4(q.s, q.v.x, q.v.y, q.v.z);
       const double4 sp = pack_site(ns3, fl);
       const double4 sq = make_double4(qt.s, qt.v.x, qt.v.y, qt.v.z);

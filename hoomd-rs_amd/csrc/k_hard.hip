@@ -142,7 +142,7 @@ __device__ __forceinline__ void trial(const Geometry& g, const ApplyState* st, c
       double nsp[3] = {0.0, 0.0, 0.0};
       for (int a = 0; a < D; ++a) nsp[a] = wrap1(t[a] + 0.0, g.L[a]);
       const uint32_t fl = ghost_flags<D>(g, nsp);
-      B.bpos[body] = make_double4(t[0], t[1], t[2], 0.0);
+      B.bpos[body] = make_double4(t[0], t[1], t[2], move_stamp(st->epoch));
       const double4 sp = pack_site(nsp, fl);
       B.spos[body] = sp;
       if (PREC == HMC_MIXED)
@@ -573,7 +573,7 @@ __device__ __forceinline__ void pass_tile(const Geometry& g, ApplyState* st, con
         double nsp[3] = {0.0, 0.0, 0.0};
         for (int a = 0; a < D; ++a) nsp[a] = wrap1(t[a] + 0.0, g.L[a]);
         const uint32_t fl = ghost_flags<D>(g, nsp);
-        B.bpos[body] = make_double4(t[0], t[1], t[2], 0.0);
+        B.bpos[body] = make_double4(t[0], t[1], t[2], move_stamp(st->epoch));
         const double4 sp = pack_site(nsp, fl);
         B.spos[body] = sp;
         if (PREC == HMC_MIXED)

@@ -226,7 +226,7 @@ __device__ void trial_lj(const Geometry& g, const ApplyState* st, const Bodies& 
       double nsp[3] = {0.0, 0.0, 0.0};
       for (int a = 0; a < D; ++a) nsp[a] = wrap1(t[a] + 0.0, g.L[a]);
       const uint32_t fl = ghost_flags<D>(g, nsp);
-      B.bpos[body] = make_double4(t[0], t[1], t[2], 0.0);
+      B.bpos[body] = make_double4(t[0], t[1], t[2], move_stamp(st->epoch));
       const double4 nspk = pack_site(nsp, fl);
       B.spos[body] = nspk;
       C.pos64[slot] = nspk;

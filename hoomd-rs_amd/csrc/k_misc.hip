@@ -54,6 +54,34 @@ __global__ void k_map_bodies(Geometry g, double3 lold, const double4* __restrict
   B.spos[e] = pack_site(s, ghost_flags<D>(g, s));
 }
 
+// ------------------------------------------------------------ moved bodies --
+// Bit i % 64 of mask[i / 64]: body i's stamp (the epoch of the apply that last
+// moved it) is above `since`; *count receives the number of such bodies.
+__global__ void __launch_bounds__(256) k_moved_mask(const double4* __restrict__ bpos, uint32_t n, uint32_t since,
+                                                    unsigned long long* mask, unsigned long long* count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool moved = i < n && stamp_of(bpos[i].w) > since;
+  const unsigned long long b = __ballot(moved);
+  if (__lane_id() == 0 && i < n) {
+    mask[i >> 6] = b;
+    if (b) atomicAdd(count, (unsigned long long)__popcll(b));
+  }
+}
+
+// Body and site positions (and orientations of oriented bodies) of the
+// listed bodies: out row k = {body, site, body quat, site quat} as double4.
+__global__ void k_gather_bodies(Bodies B, const uint32_t* __restrict__ idx, uint32_t n, double4* __restrict__ out) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t i = idx[k];
+  out[4 * (size_t)k] = B.bpos[i];
+  out[4 * (size_t)k + 1] = B.spos[i];
+  if (B.bquat) {
+    out[4 * (size_t)k + 2] = B.bquat[i];
+    out[4 * (size_t)k + 3] = B.squat[i];
+  }
+}
+
 // ------------------------------------------------------------- debug hooks --
 __global__ void k_debug_math(int kind, const double* in, double* out, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -22,6 +22,8 @@ template <int D, int PREC, bool FP32_ERROR = false>
 __global__ void k_pass_tile(Geometry g, ApplyState* st, Bodies B, Cells C, Work w, uint32_t colour, uint32_t cycle, TileCfg tc);
 template <int D>
 __global__ void k_apply_tile(Geometry g, ApplyState* st, Bodies B, Cells C, Work w, TileCfg tc);
+template <int D>
+__global__ void k_apply_lj(Geometry g, ApplyState* st, Bodies B, Cells C, Work w, TileCfg tc);
 template <int D, int PREC>
 __global__ void k_guard(Geometry g, ApplyState* st, Bodies B, Cells C, Work w, uint32_t first_cycle);
 template <int D, int PREC>
@@ -53,6 +55,10 @@ template <int D>
 __global__ void k_init_sites(Geometry g, Bodies B, int have_sites);
 template <int D>
 __global__ void k_map_bodies(Geometry g, double3 lold, const double4* __restrict__ src, Bodies B, uint32_t* outside);
+__global__ void k_tally(ApplyState* st, const unsigned long long* acc_slots, uint32_t m);
+__global__ void k_moved_mask(const double4* __restrict__ bpos, uint32_t n, uint32_t since, unsigned long long* mask,
+                             unsigned long long* count);
+__global__ void k_gather_bodies(Bodies B, const uint32_t* __restrict__ idx, uint32_t n, double4* __restrict__ out);
 __global__ void k_debug_math(int kind, const double* in, double* out, uint64_t n);
 __global__ void k_debug_normals(uint64_t step, uint32_t substep, uint32_t seed, uint64_t index, double* out, uint64_t n);
 

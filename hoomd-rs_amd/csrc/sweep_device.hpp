@@ -87,6 +87,8 @@ __host__ __device__ inline bool is_isotropic(int model) {
   return model == HMC_LJ || model == HMC_LJ_SHIFTED || model == HMC_OVERLAP_PENALTY;
 }
 
+constexpr int kMaxMoves = 4;  // moves per step of hmc_run
+
 // Device-resident per-apply state.
 struct ApplyState {
   uint64_t step;
@@ -111,6 +113,13 @@ struct ApplyState {
   unsigned long long trace_n;
   unsigned long long fp32_err_max;    // bits of the largest |dH32 - dH64| (HMC_FLAG_FP32_ERROR)
   unsigned long long fp32_err_ratio;  // bits of the largest |dH32 - dH64| / bound
+  // hmc_run with several moves per step: accepted moves and trials per move
+  // (k_tally), and the totals at the last tally
+  unsigned long long move_acc[kMaxMoves];
+  unsigned long long move_trials[kMaxMoves];
+  unsigned long long tally_acc, tally_trials;
+  uint32_t wait_at;  // the first neighbour wait that timed out: (region << 12) | (pass + 1)
+  uint32_t pad2;
 };
 
 struct Bodies {
@@ -144,6 +153,12 @@ struct Work {
 };
 
 // ----------------------------------------------------------- small helpers --
+// The moved-body stamp an accepted move stores in the body position's w: the
+// epoch (applies since the context's creation) of the apply that last moved
+// the body; 0 for a body not moved since its state was set (hmc_moved_since).
+__device__ __forceinline__ double move_stamp(uint32_t epoch) { return __longlong_as_double((long long)epoch); }
+__device__ __forceinline__ uint32_t stamp_of(double w) { return (uint32_t)__double_as_longlong(w); }
+
 // Periodic<Hypercuboid>::wrap (hoomd-microstate/src/boundary/periodic/cuboid.rs:90-103)
 __device__ __forceinline__ double wrap1(double x, double L) {
   double lambda = x / L;
@@ -460,7 +475,9 @@ struct TileCfg {
   uint32_t ntiles;
   uint32_t xcd_chunk;
   uint32_t lds_bytes;
-  float band_lo, band_hi;  // FP32 band of the box-relative test
+  float band_lo, band_hi;  // FP32 band of the box-relative test (isotropic: of the r_cut membership test)
+  float perr;              // isotropic persistent apply: bound on the FP32 displacement error per axis
+  uint32_t probe;          // persistent apply: 1 = only check that every region is resident (region_probe)
   uint32_t step[3];        // the workgroup size in the mixed radix (E0, .., E[D-2], E[D-1] + 2) of the row starts
 };
 
@@ -541,6 +558,15 @@ __device__ __forceinline__ double4 ld_dev_d4(const double4* p) {
 }
 // Two 16-byte device-scope (sc1) stores.  Their completion is waited for
 // explicitly (s_waitcnt 0) before the region publishes the pass.
+//
+// A store of more than 8 bytes reads its data registers some cycles after it
+// issues, and a VALU write to those registers in that window corrupts the
+// stored value (two wait states on gfx940 and later).  The compiler inserts
+// those wait states after the stores it emits itself but cannot see into
+// inline asm, and takes the asm's inputs as consumed once the asm is issued.
+// So both stores sit in one asm statement that ends with an s_nop covering
+// the window (5 wait states) before any later instruction can overwrite
+// their data.
 __device__ __forceinline__ void st_dev_d4(double4* p, double4 v) {
   typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
   const unsigned long long a = (unsigned long long)__double_as_longlong(v.x),
@@ -549,8 +575,12 @@ __device__ __forceinline__ void st_dev_d4(double4* p, double4 v) {
                            d = (unsigned long long)__double_as_longlong(v.w);
   const u32x4 lo = {(unsigned)a, (unsigned)(a >> 32), (unsigned)b, (unsigned)(b >> 32)};
   const u32x4 hi = {(unsigned)c, (unsigned)(c >> 32), (unsigned)d, (unsigned)(d >> 32)};
-  asm volatile("global_store_dwordx4 %0, %1, off sc1" ::"v"(p), "v"(lo) : "memory");
-  asm volatile("global_store_dwordx4 %0, %1, off offset:16 sc1" ::"v"(p), "v"(hi) : "memory");
+  asm volatile(
+      "global_store_dwordx4 %0, %1, off sc1\n\t"
+      "global_store_dwordx4 %0, %2, off offset:16 sc1\n\t"
+      "s_nop 4" ::"v"(p),
+      "v"(lo), "v"(hi)
+      : "memory");
 }
 
 // ------------------------------------------------------------ Lennard-Jones --
@@ -626,6 +656,15 @@ __device__ __forceinline__ float wave_sum(float v) {
 // computed *a (A in FP32) gives |e - e_exact| <= 48 u *a with room to spare.
 __device__ __forceinline__ float lj_energy32(float sig2, float eps4, float shift, double r2, float* a) {
   const float s2 = sig2 * __builtin_amdgcn_rcpf((float)r2);
+  const float s4 = s2 * s2, s8 = s4 * s4;
+  const float s12 = s4 * s8, s6 = s2 * s4;
+  *a = eps4 * (s12 + s6) + fabsf(shift);
+  return eps4 * (s12 - s6) - shift;
+}
+// The same from an FP32 r^2 (the conversion of r^2 it skips is within the
+// bound above).
+__device__ __forceinline__ float lj_energy32f(float sig2, float eps4, float shift, float r2, float* a) {
+  const float s2 = sig2 * __builtin_amdgcn_rcpf(r2);
   const float s4 = s2 * s2, s8 = s4 * s4;
   const float s12 = s4 * s8, s6 = s2 * s4;
   *a = eps4 * (s12 + s6) + fabsf(shift);

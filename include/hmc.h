@@ -141,6 +141,10 @@ typedef struct {
                                trial, or |d2_32 - d2_64| of a hard-sphere pair */
   double fp32_error_bound;  /* HMC_FLAG_FP32_ERROR: the largest such error divided
                                by its bound (at most 1 when the bound holds) */
+  uint32_t regions;         /* persistent apply: regions (workgroups that take part) */
+  uint32_t region_threads;  /* persistent apply: lanes per region */
+  uint32_t regions_per_cu;  /* persistent apply: resident workgroups per CU (occupancy) */
+  uint32_t wait_timeout_at; /* first neighbour wait that timed out: (region << 12) | (pass + 1), or 0 */
 } hmc_stats;
 
 /* Allocate device state for one system.  Replaces building a ParallelSweep and
@@ -230,6 +234,44 @@ int hmc_get_orientations(hmc_ctx* ctx, double* body_quat, double* site_quat);
 int hmc_run_translate(hmc_ctx* ctx, uint64_t step0, uint32_t seed, double kT, double max_distance,
                       uint32_t n_steps, hmc_count* out);
 
+/* One trial move of a step: Translate (size = maximum_distance) or
+ * Rotate<Versor> (size = maximum_rotation, oriented bodies only). */
+enum { HMC_MOVE_TRANSLATE = 0, HMC_MOVE_ROTATE = 1 };
+typedef struct {
+  int32_t move; /* HMC_MOVE_* */
+  int32_t pad;
+  double size;
+} hmc_move;
+
+/* n_steps iterations of { for each move in order: ParallelSweep::apply of
+ * that move (the step's substeps follow on from one apply to the next);
+ * Microstate::increment_step } starting at (step0, substep 0), as a
+ * Simulation's advance applies several trials per step (e.g. translate then
+ * rotate, benchmarks/src/mc/octahedron.rs:109-134), with no host
+ * synchronisation.  1 to 4 moves.  out[m] (n_moves entries, may be NULL)
+ * receives the summed count of move m, as the reference keeps one Count per
+ * trial.  Graph replay and direct launches as hmc_run_translate, which is
+ * hmc_run with one Translate move. */
+int hmc_run(hmc_ctx* ctx, uint64_t step0, uint32_t seed, double kT, const hmc_move* moves, uint32_t n_moves,
+            uint32_t n_steps, hmc_count* out);
+
+/* The side effects of Trial::apply on the caller's microstate
+ * (Microstate::update_body_properties of every accepted body,
+ * hoomd-microstate/src/microstate.rs:985-1056, called by
+ * ParallelSweep::update_bodies, parallel_sweep.rs:283-314), synced lazily:
+ * every accepted move stamps its body with the apply's epoch (applies since
+ * hmc_create; hmc_set_state clears the stamps).  hmc_moved_since sets bit
+ * i % 64 of mask[i / 64] (ceil(n_bodies / 64) words) for every body moved by
+ * an apply after epoch `since`, *n_moved to their number and *now (may be
+ * NULL) to the current epoch, the `since` of the next sync.
+ * hmc_get_bodies copies the current body and site positions (n x dim) and,
+ * for oriented bodies, orientations (n x 4) of the n listed bodies; any
+ * output may be NULL.  The caller then runs update_body_properties on those
+ * bodies only. */
+int hmc_moved_since(hmc_ctx* ctx, uint32_t since, uint32_t* now, uint64_t* mask, uint64_t* n_moved);
+int hmc_get_bodies(hmc_ctx* ctx, const uint32_t* index, uint64_t n, double* body_pos, double* site_pos,
+                   double* body_quat, double* site_quat);
+
 /* PairwiseCutoff::total_energy (hoomd-interaction/src/pairwise_cutoff.rs:720-745)
  * of the current state: +inf on any hard overlap.  The reference sums in its
  * VecCell order, the device in a fixed order of its own; *bound (may be NULL)
@@ -282,6 +324,12 @@ int hmc_get_stats(hmc_ctx* ctx, hmc_stats* out);
 int hmc_profile_enable(hmc_ctx* ctx, int period);
 int hmc_profile_read(hmc_ctx* ctx, double* pass_ms_total, uint64_t* pass_launches, double* build_ms_total,
                      uint64_t* builds);
+
+/* Test hook: the mailbox words of every region of the persistent apply, six
+ * per region (the two pass-parity counters, the residency probe's counter in
+ * region 0, then the isotropic apply's progress: staged, passes published,
+ * waits passed). No reference counterpart. */
+int hmc_debug_regions(hmc_ctx* ctx, uint32_t* out, uint64_t cap, uint64_t* n);
 
 /* Evaluate the library's device math on n inputs: kind 0 cbrt, 1 log,
  * 2 exp, 5 the sine and 6 the cosine of one sincos call (the device's

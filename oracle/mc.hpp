@@ -145,6 +145,21 @@ struct System {
   bool tracing = false;
   std::vector<TraceRecord> trace;
 
+  // External(Harmonic) (hoomd-interaction/src/external_type.rs:341-375 with
+  // the Harmonic site energy of hoomd-mc/src/sweep.rs:399-405), added to the
+  // pair energies when ext_k > 0: 1.0 / 2.0 * K * (x - c).norm_squared(),
+  // norm_squared = dot = fold(0, fma) (hoomd-vector/src/cartesian.rs:293-296).
+  double ext_k = 0.0;
+  Vec<D> ext_c{};
+  double external(const Vec<D>& x) const {
+    double ns = 0.0;
+    for (int i = 0; i < D; ++i) {
+      const double d = x[i] - ext_c[i];
+      ns = std::fma(d, d, ns);
+    }
+    return 1.0 / 2.0 * ext_k * ns;
+  }
+
   System(const std::array<double, D>& L, double boundary_range, double spatial_radius)
       : spatial(spatial_radius) {
     boundary.L = L;
@@ -260,7 +275,12 @@ struct System {
       else
         ei += one;
     }
-    return ef - ei;
+    double dh = ef - ei;
+    if (ext_k > 0.0) {  // the sum of the two Hamiltonians' delta_energy_one
+      const Site<D>& s = sites.items[sites.indices[body_index]];
+      dh += external(boundary.wrap(add_site_local(trial_body))) - (0.0 + external(s.pos));
+    }
+    return dh;
   }
 
   // pairwise_cutoff.rs:1017-1025: the new body has no tag, nothing is filtered

@@ -306,6 +306,44 @@ void oracle_sweep_apply(void* h, double kt, double max_move, uint64_t* acc, uint
   DISPATCH(h, S.sweep_apply(kt, max_move, *acc, *rej));
 }
 void oracle_increment_step(void* h) { DISPATCH(h, S.increment_step()); }
+
+// Adds External(Harmonic) with spring constant k about centre c (dim values).
+}  // extern "C"
+template <int D>
+static void set_external(System<D>& S, double k, const double* c) {
+  S.ext_k = k;
+  for (int i = 0; i < D; ++i) S.ext_c[i] = c[i];
+}
+extern "C" {
+void oracle_set_external_harmonic(void* h, double k, const double* c) { DISPATCH(h, set_external(S, k, c)); }
+
+// n steps of {Sweep (parallel = 0) or ParallelSweep apply; accumulate body 0's
+// position and the external energy summed over sites; increment_step}, as the
+// reference's harmonic-oscillator test does (hoomd-mc/src/sweep.rs:420-461).
+}  // extern "C"
+template <int D>
+static void run_accumulate(System<D>& S, int parallel, double kt, double max_move, uint64_t n, double* pos_sum,
+                           double* energy_sum) {
+  for (int i = 0; i < D; ++i) pos_sum[i] = 0.0;
+  *energy_sum = 0.0;
+  for (uint64_t k = 0; k < n; ++k) {
+    uint64_t a = 0, r = 0;
+    if (parallel)
+      S.parallel_apply(kt, max_move, a, r);
+    else
+      S.sweep_apply(kt, max_move, a, r);
+    for (int i = 0; i < D; ++i) pos_sum[i] += S.bodies[0][i];
+    double e = 0.0;
+    for (const auto& st : S.sites.items) e += S.external(st.pos);
+    *energy_sum += e;
+    S.increment_step();
+  }
+}
+extern "C" {
+void oracle_run_accumulate(void* h, int parallel, double kt, double max_move, uint64_t n, double* pos_sum,
+                           double* energy_sum) {
+  DISPATCH(h, run_accumulate(S, parallel, kt, max_move, n, pos_sum, energy_sum));
+}
 void oracle_get_counter(void* h, uint64_t* step, uint32_t* substep) {
   DISPATCH(h, { *step = S.step; *substep = S.substep; });
 }

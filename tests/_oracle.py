@@ -83,6 +83,8 @@ def lib():
         L.oracle_trace_size.argtypes = [C.c_void_p]
         L.oracle_trace_size.restype = C.c_uint64
         L.oracle_trace_read.argtypes = [C.c_void_p, C.c_void_p]
+        L.oracle_set_external_harmonic.argtypes = [C.c_void_p, C.c_double, D_P]
+        L.oracle_run_accumulate.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_uint64, D_P, D_P]
         L.oracle_time_applies.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, D_P, U64_P]
         L.oracle_time_applies.restype = C.c_uint64
         _lib = L
@@ -276,6 +278,19 @@ class OracleSystem:
         out = np.zeros(n, TRACE_DTYPE)
         lib().oracle_trace_read(self.h, out.ctypes.data_as(C.c_void_p))
         return out
+
+    def set_external_harmonic(self, k, centre):
+        """Adds External(Harmonic): 1/2 k |x - centre|^2 per site."""
+        c = np.ascontiguousarray(centre, np.float64)
+        lib().oracle_set_external_harmonic(self.h, k, _p(c, D_P))
+
+    def run_accumulate(self, parallel, kt, max_move, n):
+        """n steps of apply + increment_step: (sum of body 0's position, sum of
+        the external energy)."""
+        ps = np.zeros(self.dim)
+        es = np.zeros(1)
+        lib().oracle_run_accumulate(self.h, 1 if parallel else 0, kt, max_move, n, _p(ps, D_P), _p(es, D_P))
+        return ps, float(es[0])
 
     def time_applies(self, parallel, kt, max_move, seconds):
         el = np.zeros(1)

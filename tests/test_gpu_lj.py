@@ -193,3 +193,66 @@ def test_total_energy_hard_shapes(hmc, wl):
     gpu.set_positions(ref2.positions()[0], ref2.positions()[1])
     assert gpu.total_energy()[0] == np.inf == ref2.total_energy()
     gpu.close()
+
+
+# ------------------------------------------------- persistent apply (mixed) --
+def _lj_persistent_replay(hmc, L, pos, model, params, kT, d, n_applies, seed=42, trace=True):
+    """Mixed precision (one persistent launch per apply) against the oracle
+    and against the per-colour FP64 passes; returns the mixed stats."""
+    st, _ = _replay_lj(hmc, L, pos, model, params, kT, d, n_applies, seed=seed, trace=trace, precision="mixed")
+    assert st["tiled"] == 2, st
+    return st
+
+
+def test_lj_persistent_shifted_3d(hmc, wl):
+    # 6,912 shifted-LJ bodies, FCC 12^3 at number density 0.8 (8 spaces of
+    # width 2.56 per axis), kT 1.2: one persistent launch per apply
+    L, pos = wl.fcc_number_density((12, 12, 12), 0.8)
+    t = time.time()
+    st = _lj_persistent_replay(hmc, L, pos, "lj_shifted", (1.0, 1.0, 2.5, 2.5), 1.2, 0.1, 12)
+    log(f"LJ persistent 3D N={len(pos)}: 12 applies bit-identical ({time.time() - t:.1f}s) {st}")
+    assert st["tile_overflows"] == 0
+
+
+def test_lj_persistent_plain_2d(hmc, wl):
+    # 4,096 plain LJ disks (square lattice, r_cut 2.5): the 2D persistent apply
+    L, pos = wl.square_lattice_disks(n_side=64, packing_fraction=0.55)
+    st = _lj_persistent_replay(hmc, L, pos, "lj", (1.0, 1.0, 2.5), 1.0, 0.15, 10)
+    log(f"LJ persistent 2D N={len(pos)}: 10 applies bit-identical {st}")
+
+
+def test_lj_persistent_near_coincident_pair(hmc, wl):
+    # two sites 4e-4 apart: FP32 cannot decide their trials (re-taken in FP64)
+    L, pos = wl.fcc_number_density((12, 12, 12), 0.8)
+    pos = pos.copy()
+    pos[1] = pos[0] + np.array([4e-4, 0.0, 0.0])
+    st = _lj_persistent_replay(hmc, L, pos, "lj_shifted", (1.0, 1.0, 2.5, 2.5), 1.2, 0.1, 6, seed=5)
+    log(f"LJ persistent near-coincident pair: 6 applies bit-identical {st}")
+    assert st["band_retests"] > 0
+
+
+def test_lj_persistent_overflowing_region(hmc, wl):
+    # a dense FCC block in one corner of a mostly empty box: the regions
+    # around it hold more sites than their LDS capacity and run their trials
+    # from the cell lists in global memory (L = 31: 12 spaces of width 2.58
+    # per axis, so the neighbour scan reaches one space out)
+    L1, block = wl.fcc_number_density((6, 6, 6), 0.8)
+    L = np.array([31.0, 31.0, 31.0])
+    pos = block - (-L1 / 2) + (-L / 2 + 0.6)
+    st = _lj_persistent_replay(hmc, L, pos, "lj_shifted", (1.0, 1.0, 2.5, 2.5), 1.2, 0.1, 6, seed=8)
+    log(f"LJ persistent overflow: 6 applies bit-identical {st}")
+    assert st["tile_overflows"] > 0
+
+
+def test_lj_persistent_equals_per_colour(hmc, wl):
+    # the same run with persistent=False (one launch per colour pass)
+    L, pos = wl.fcc_number_density((12, 12, 12), 0.8)
+    prm = (1.0, 1.0, 2.5, 2.5)
+    a = hmc.GpuParallelSweep(L, wl.wrap(L, pos), model="lj_shifted", params=prm, precision="mixed")
+    b = hmc.GpuParallelSweep(L, wl.wrap(L, pos), model="lj_shifted", params=prm, precision="mixed",
+                             persistent=False)
+    assert a.stats()["tiled"] == 2 and b.stats()["tiled"] != 2
+    assert a.run(0, 11, 1.2, 0.1, 8) == b.run(0, 11, 1.2, 0.1, 8)
+    assert np.array_equal(a.positions()[1].view(np.uint64), b.positions()[1].view(np.uint64))
+    a.close()
+    b.close()
