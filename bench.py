@@ -146,6 +146,19 @@ def lj_trial_flops(n, n_spaces, density, r_cut, dim=3):
     return 2.0 * (8.0 * cand + 10.0 * within)
 
 
+def convex_trial_flops(n, n_spaces, density):
+    """Algorithmic flops of one trial on unit cubes (an estimate from the
+    operation counts of the code path): the circumsphere test of every
+    candidate of the 27 neighbour spaces (8 flops), and XenoCollide for every
+    candidate within the circumsphere cutoff sqrt(3) (density x 21.8 of
+    them): pair_system_to_local and the rotation matrix (about 90 flops), two
+    discovery supports (76 each: R^T n, 8 vertex dot products, R s) and 1.6
+    refinement steps on average (116 each), about 450 flops per pair."""
+    cand = 27.0 * n / n_spaces
+    pairs = density * (4.0 / 3.0) * math.pi * math.sqrt(3.0) ** 3
+    return 8.0 * cand + 450.0 * pairs
+
+
 def cpu_sample(make, kT, d, threads, parallel, what, warm=CPU_WARM_S, seconds=CPU_TIMED_S):
     """Trial moves/s of the oracle (the C++ restatement of hoomd-rs 1.4.0) on
     the host cores: `warm` seconds untimed, then `seconds` timed, as the
@@ -166,7 +179,7 @@ def all_cores():
     return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
 
 
-def other_workloads(hmc, wl, cpu=True, only=None):
+def other_workloads(hmc, wl, cpu=True, only=None, fp64=True):
     """Timed runs of the other configurations (trial moves/s; inputs resident
     on the device; warm-up steps first), each with the roofline of its
     dominant kernel and the oracle timed on the host cores beside it."""
@@ -244,17 +257,19 @@ def other_workloads(hmc, wl, cpu=True, only=None):
                              "algorithmic_flops_per_launch": fl, "launches": pn,
                              "flops_per_trial": lj_trial_flops(len(pos), n_spaces, 0.8, 2.5)}
         sim.close()
-        sim = hmc.GpuParallelSweep(L, wl.wrap(L, pos), model="lj_shifted", params=prm)
-        v64, _ = timed(lambda s0, k: sum(sim.run(s0, 42, 1.2, 0.1, k)), 1, 4)
-        r["fp64_value"] = v64
-        sim.close()
-        # FP32 error of dH against FP64 (every trial of two applies, diagnostic run)
-        sim = hmc.GpuParallelSweep(L, wl.wrap(L, pos), model="lj_shifted", params=prm, precision="mixed",
-                                   fp32_error=True)
-        sim.run(0, 42, 1.2, 0.1, 2)
-        st = sim.stats()
-        r["fp32_dH_error"] = {"max_abs": st["fp32_error_max"], "max_over_bound": st["fp32_error_bound"]}
-        sim.close()
+        if fp64:
+            sim = hmc.GpuParallelSweep(L, wl.wrap(L, pos), model="lj_shifted", params=prm)
+            v64, _ = timed(lambda s0, k: sum(sim.run(s0, 42, 1.2, 0.1, k)), 1, 4)
+            r["fp64_value"] = v64
+            r["mixed_over_fp64"] = v / v64
+            sim.close()
+            # FP32 error of dH against FP64 (every trial of two applies, diagnostic run)
+            sim = hmc.GpuParallelSweep(L, wl.wrap(L, pos), model="lj_shifted", params=prm, precision="mixed",
+                                       fp32_error=True)
+            sim.run(0, 42, 1.2, 0.1, 2)
+            st = sim.stats()
+            r["fp32_dH_error"] = {"max_abs": st["fp32_error_max"], "max_over_bound": st["fp32_error_bound"]}
+            sim.close()
         if cpu:
             mk = lambda: O.OracleSystem(L, wl.wrap(L, pos), model="lj_shifted", params=prm, seed=42)
             r["cpu_baseline"] = cpu_sample(mk, 1.2, 0.1, all_cores(), True, "the same 524,288 LJ bodies")
@@ -273,9 +288,22 @@ def other_workloads(hmc, wl, cpu=True, only=None):
         cnt = sim.run_moves(1, 42, 1.0, moves, 4)
         t = sum(a + b for a, b in cnt)
         v = t / (time.perf_counter() - t0)
+        # pass timing: one more step with every launch timed by HIP events
+        sim.profile(True, 1)
+        t_prof = sum(a + b for a, b in sim.run_moves(5, 42, 1.0, moves, 1))
+        pm, pn, _bm, _bn = sim.read_profile()
         r = {"value": v, "unit": "trial moves/s", "steps": 4, "precision": "fp64",
              "moves": "translate 0.05 + rotate 0.1 per step",
              "acceptance": {"translate": cnt[0][0] / max(sum(cnt[0]), 1), "rotate": cnt[1][0] / max(sum(cnt[1]), 1)}}
+        if pn:
+            ms = pm / pn
+            fpt = convex_trial_flops(len(pos), 44**3, len(pos) / float(np.prod(L)))
+            fl = fpt * t_prof / pn
+            ach = fl / (ms * 1e-3) / 1e12
+            r["roofline"] = {"bound": "valu_fp64", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                             "frac": ach / PEAK_FP64_TFLOPS, "traffic": None, "kernel": "k_pass_convex",
+                             "avg_launch_ms": ms, "algorithmic_flops_per_launch": fl, "launches": pn,
+                             "flops_per_trial": fpt}
         sim.close()
         if cpu:
             def c3_cpu():
@@ -311,13 +339,15 @@ def other_workloads(hmc, wl, cpu=True, only=None):
         # written and read) and the passes (every colour pass reads every
         # site copy and cell offset once) against the step's wall time
         spaces = 0
-        for L_, p in systems:
+        pass_b = 0.0
+        for i, (L_, p) in enumerate(systems):
             k = [int(np.floor(x / 1.0)) for x in L_]
-            spaces += int(np.prod([x - (x % 2) for x in k]))
-        cycles = sum(batch.stats(i)["cycles"] for i in range(len(systems))) / len(systems)
-        passes = 8 * max(cycles, 1.0)
-        step_b = (2 * 32 * nb + 4 * 4 * nb + 3 * 4 * spaces) + passes * (4 * spaces + 16 * nb) + \
-            (36 * t + 80 * acc) / 40
+            s_i = int(np.prod([x - (x % 2) for x in k]))
+            spaces += s_i
+            st_i = batch.stats(i)
+            cycles_i = st_i["cycles"] / max(st_i["applies"], 1)  # colour cycles per step
+            pass_b += 8 * cycles_i * (4 * s_i + 16 * len(p))
+        step_b = (2 * 32 * nb + 4 * 4 * nb + 3 * 4 * spaces) + pass_b + (36 * t + 80 * acc) / 40
         ach = step_b / (el / 40) / 1e9
         r["roofline"] = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                          "frac": ach / PEAK_HBM_GBS, "traffic": None, "kernel": "whole step (build + passes)",
@@ -333,12 +363,13 @@ def other_workloads(hmc, wl, cpu=True, only=None):
         r["sdf"] = {"ms_per_call": ms, "bodies_per_s": nb / (ms * 1e-3), "overlaps": int(ov.sum()),
                     "beta_p_over_rho_one_frame": {"phi_0.30": z[0], "phi_0.54": z[-1]}}
         batch.close()
-        # the same batch in plain FP64 (identical decisions)
-        batch = hmc.GpuBatch([(L, wl.wrap(L, p), None) for L, p in systems], precision="fp64")
-        v64, _ = timed(lambda s0, k: sum(a + b for a, b in batch.run(s0, seeds, 1.0, 0.08, k)), 2, 40)
-        r["fp64_value"] = v64
-        r["mixed_over_fp64"] = v / v64
-        batch.close()
+        if fp64:
+            # the same batch in plain FP64 (identical decisions)
+            batch = hmc.GpuBatch([(L, wl.wrap(L, p), None) for L, p in systems], precision="fp64")
+            v64, _ = timed(lambda s0, k: sum(a + b for a, b in batch.run(s0, seeds, 1.0, 0.08, k)), 2, 40)
+            r["fp64_value"] = v64
+            r["mixed_over_fp64"] = v / v64
+            batch.close()
         if cpu:
             # every 16th replica, each for a share of the sample, one after
             # another with all cores (the replicas are independent)
@@ -428,7 +459,8 @@ def main():
         print(json.dumps({"placement_quick_compress_1M": placement(hmc, wl)}), flush=True)
         return
     if args.only:
-        print(json.dumps(other_workloads(hmc, wl, cpu=not args.no_cpu_baseline, only=args.only)), flush=True)
+        print(json.dumps(other_workloads(hmc, wl, cpu=not args.no_cpu_baseline, only=args.only,
+                                         fp64=not args.no_fp64)), flush=True)
         return
     kT, d = 1.0, 0.08
     L, pos = wl.fcc_spheres((63, 63, 63), packing_fraction=0.50)

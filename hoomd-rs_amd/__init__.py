@@ -33,7 +33,8 @@ HMC_MIXED = 1
 
 # every entry point declared in include/hmc.h
 EXPORTS = ["hmc_create", "hmc_set_state", "hmc_get_state", "hmc_clone_with_boundary", "hmc_apply_translate",
-           "hmc_quick_compress_init", "hmc_quick_compress_apply", "hmc_delta_energy_insert",
+           "hmc_quick_compress_init", "hmc_quick_compress_apply", "hmc_quick_insert_init",
+           "hmc_quick_insert_apply", "hmc_delta_energy_insert",
            "hmc_apply_rotate",
            "hmc_set_orientations", "hmc_get_orientations", "hmc_run_translate", "hmc_total_energy",
            "hmc_total_energy_noalloc", "hmc_run", "hmc_moved_since", "hmc_get_bodies", "hmc_debug_regions",
@@ -121,6 +122,10 @@ def lib():
         L.hmc_quick_compress_init.restype = None
         L.hmc_quick_compress_apply.argtypes = [C.POINTER(QuickCompressState), C.POINTER(vp), C.c_uint64,
                                                C.c_uint32, C.c_uint32]
+        L.hmc_quick_insert_init.argtypes = [C.POINTER(QuickInsertState), C.c_uint64]
+        L.hmc_quick_insert_init.restype = None
+        L.hmc_quick_insert_apply.argtypes = [C.POINTER(QuickInsertState), C.POINTER(vp), C.c_uint64,
+                                             C.c_uint32, C.c_uint32, C.POINTER(Count)]
         L.hmc_apply_translate.argtypes = [vp, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
                                           C.POINTER(Count), C.POINTER(C.c_uint32)]
         L.hmc_run_translate.argtypes = [vp, C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_uint32,
@@ -602,6 +607,48 @@ class QuickCompress:
     @property
     def state(self):
         return QC_STATES[self.st.state]
+
+    @property
+    def ambiguous(self):
+        return int(self.st.ambiguous)
+
+
+class QuickInsertState(C.Structure):
+    _fields_ = [("target", C.c_uint64), ("allowed_overlaps", C.c_uint64), ("inserted", C.c_uint64),
+                ("state", C.c_int32), ("pad", C.c_int32), ("ambiguous", C.c_uint64)]
+
+
+class QuickInsert:
+    """QuickInsert (hoomd-mc/src/quick_insert.rs:94-334) with UniformIn over
+    the periodic box, over a GpuParallelSweep: the candidates' insertion
+    energies against the current bodies on the device (one launch per call),
+    the candidates taken in order on the host (hmc_quick_insert_apply)."""
+
+    def __init__(self, target):
+        self.st = QuickInsertState()
+        lib().hmc_quick_insert_init(C.byref(self.st), int(target))
+
+    def apply(self, sim, step, substep, seed):
+        """QuickInsert::apply with (step, substep, seed) = the microstate's
+        counter.  Returns (accepted, rejected, substeps used: 0 or 1).  The
+        inserted bodies follow sim's bodies in order (sim.n grows)."""
+        cnt = Count()
+        rc = lib().hmc_quick_insert_apply(C.byref(self.st), C.byref(sim.h), step, substep, seed, C.byref(cnt))
+        if rc != HMC_OK:
+            raise HmcError(rc, lib().hmc_last_error(sim.h).decode())
+        sim.n += int(cnt.accepted)
+        return int(cnt.accepted), int(cnt.rejected), 0 if self.is_complete() else 1
+
+    def is_complete(self):
+        return self.st.state == 1
+
+    @property
+    def inserted(self):
+        return int(self.st.inserted)
+
+    @property
+    def target(self):
+        return int(self.st.target)
 
     @property
     def ambiguous(self):

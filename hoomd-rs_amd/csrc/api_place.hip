@@ -2,6 +2,8 @@
 // boundary clones.
 #include "host.hpp"
 
+#include <unordered_map>
+
 extern "C" {
 
 // ------------------------------------------------------------ QuickCompress --
@@ -31,6 +33,24 @@ static double host_uniform(uint64_t bits, double low, double high) {
   double v12;
   std::memcpy(&v12, &m, sizeof v12);
   return (v12 - 1.0) * scale + low;
+}
+
+// Counter(step, substep, seed).make_rng() as a stream (index 0).
+struct HostSfc64 {
+  uint64_t a, b, c, ctr;
+  uint64_t next() {
+    const uint64_t out = a + b + ctr;
+    a = b ^ (b >> 11);
+    b = c + (c << 3);
+    c = ((c << 24) | (c >> 40)) + out;
+    ctr += 1;
+    return out;
+  }
+};
+static HostSfc64 host_counter_rng(uint64_t step, uint32_t substep, uint32_t seed) {
+  HostSfc64 r{step, ((uint64_t)substep << 32) | (uint64_t)seed, 0, 0};
+  for (int i = 0; i < 12; ++i) (void)r.next();
+  return r;
 }
 
 static double box_volume(const Geometry& g) {  // Hypercuboid::volume: the product of the edges
@@ -137,6 +157,215 @@ int hmc_quick_compress_apply(hmc_quick_compress* qc, hmc_ctx** pc, uint64_t step
     default:
       return fail(c, HMC_ERR_INVALID, "QuickCompress: unknown state");
   }
+}
+
+// -------------------------------------------------------------- QuickInsert --
+// Host restatements, in the device's arithmetic, for the candidates of one
+// call against each other: Periodic::wrap (cuboid.rs:90-103), the ghost
+// flags of a site (cuboid.rs:108-161), the squared distance to a site over
+// its images (the per-axis nearest image), and the pair energy.
+static double host_wrap(double x, double L) {
+  double lambda = x / L;
+  lambda = lambda - std::round(lambda);
+  if (lambda == 0.5) lambda = -0.5;
+  return lambda * L;
+}
+static uint32_t host_ghost_flags(const Geometry& g, const double* x) {
+  uint32_t f = 0;
+  for (int a = 0; a < g.dim; ++a) {
+    if (x[a] < -g.half[a] || x[a] >= g.half[a]) return kNoImages;
+    if (x[a] > g.ghost_hi[a])
+      f |= 2u << (2 * a);
+    else if (x[a] < g.ghost_lo[a])
+      f |= 1u << (2 * a);
+  }
+  return f;
+}
+static double host_r2(const Geometry& g, const double* t, const double* xj, uint32_t fl) {
+  double d2 = 0.0;
+  for (int a = 0; a < g.dim; ++a) {
+    double d = t[a] - xj[a];
+    const uint32_t f = (fl >> (2 * a)) & 3u;
+    if (f && !(fl & kNoImages)) {
+      const double dg = t[a] - (xj[a] + (f == 1u ? g.L[a] : -g.L[a]));
+      if (std::fabs(dg) < std::fabs(d)) d = dg;
+    }
+    d2 = d2 + d * d;
+  }
+  return d2;
+}
+// Pair energy of two sites at squared distance d2 (hard_shape.rs:143-153,
+// isotropic.rs:147-156, lennard_jones.rs:107-111, shifted.rs:92-94,
+// overlap_penalty.rs:109-115); *edge when d2 lies within 2^-30 of the
+// cutoff, where the reference's VecCell may not visit the image.
+static double host_pair(const Geometry& g, double d2, bool* edge) {
+  if (g.model == HMC_HARD_SPHERE) {
+    if (d2 >= g.sigma2) return 0.0;
+    if (d2 >= g.edge2) *edge = true;
+    return INFINITY;
+  }
+  if (d2 >= g.lj_rc2) return 0.0;
+  if (d2 >= g.lj_rc2_edge) *edge = true;
+  const double r = std::sqrt(d2);
+  if (g.model == HMC_OVERLAP_PENALTY) {
+    const double x = r - g.op_delta;
+    if (x < -g.op_dmax) return INFINITY;
+    return x < 0.0 ? g.op_half_k * (x * x) + g.op_eps : 0.0;
+  }
+  const double sr = g.lj_sigma / r;
+  const double s2 = sr * sr, s4 = s2 * s2, s8 = s4 * s4;
+  const double s12 = s4 * s8, s6 = s2 * s4;
+  return g.lj_eps4 * (s12 - s6) - g.lj_shift;
+}
+
+void hmc_quick_insert_init(hmc_quick_insert* qi, uint64_t target) {
+  if (!qi) return;
+  std::memset(qi, 0, sizeof *qi);
+  qi->target = target;
+  qi->allowed_overlaps = std::max<uint64_t>(target / 8, 1);
+  qi->state = HMC_QI_RUNNING;
+}
+
+int hmc_quick_insert_apply(hmc_quick_insert* qi, hmc_ctx** pc, uint64_t step, uint32_t substep, uint32_t seed,
+                           hmc_count* count) {
+  if (!qi || !pc || !*pc || !count) return HMC_ERR_INVALID;
+  hmc_ctx* c = *pc;
+  count->accepted = count->rejected = 0;
+  if (!c->have_state) return fail(c, HMC_ERR_STATE, "no state");
+  const Geometry& g = c->g;
+  const int D = g.dim;
+  if (g.model == HMC_CONVEX_POLYHEDRON)
+    return fail(c, HMC_ERR_UNSUPPORTED, "QuickInsert of oriented bodies is not offloaded");
+  if (qi->state == HMC_QI_COMPLETE) return HMC_OK;
+  double e = 0.0, bound = 0.0;
+  int rc = hmc_total_energy(c, &e, &bound);
+  if (rc) return rc;
+  if (std::isfinite(e) && e != 0.0 && std::fabs(e) <= bound) qi->ambiguous += 1;
+  if (e <= 0.0 && qi->inserted >= qi->target) {
+    qi->state = HMC_QI_COMPLETE;
+    return HMC_OK;
+  }
+  if (!(e <= 0.0) || qi->target == 0) return HMC_OK;
+  const uint64_t T = qi->target;
+  if ((uint64_t)g.n_bodies + T >= 0xffffffffull) return fail(c, HMC_ERR_INVALID, "QuickInsert: too many bodies");
+  // every candidate the call can draw, in the stream's order
+  // (UniformIn::sample -> Hypercuboid's Distribution: one Uniform draw per axis)
+  std::vector<double> cand((size_t)T * D);
+  {
+    HostSfc64 rng = host_counter_rng(step, substep, seed);
+    for (uint64_t k = 0; k < T; ++k)
+      for (int a = 0; a < D; ++a) cand[(size_t)k * D + a] = host_uniform(rng.next(), -g.L[a] / 2.0, g.L[a] / 2.0);
+  }
+  // insertion energies against the bodies already present, all on the device
+  std::vector<double> es(T), eb(T);
+  rc = hmc_delta_energy_insert(c, cand.data(), T, es.data(), eb.data());
+  if (rc) return rc;
+  // the candidates in order, each against the ones inserted before it in this
+  // call (a hash grid of cells no narrower than the interaction range)
+  const double range = g.model == HMC_HARD_SPHERE ? g.sigma : std::sqrt(g.lj_rc2);
+  int32_t gs[3] = {1, 1, 1};
+  for (int a = 0; a < D; ++a) gs[a] = std::max<int32_t>(1, (int32_t)std::floor(g.L[a] / range));
+  auto cell_of = [&](const double* x, int32_t* m) {
+    for (int a = 0; a < D; ++a) {
+      int32_t k = (int32_t)std::floor((x[a] + g.half[a]) / g.L[a] * gs[a]);
+      m[a] = std::min(std::max(k, 0), gs[a] - 1);
+    }
+  };
+  std::unordered_map<uint64_t, std::vector<uint32_t>> grid;
+  auto key = [&](const int32_t* m) {  // one key per cell
+    return (uint64_t)m[0] + (uint64_t)gs[0] * ((uint64_t)m[1] + (uint64_t)gs[1] * (uint64_t)m[2]);
+  };
+  struct New {
+    double body[3], site[3];
+    uint32_t flags;
+  };
+  std::vector<New> added;
+  uint64_t overlapping = 0;
+  for (uint64_t k = 0; k < T; ++k) {
+    const double* p = &cand[(size_t)k * D];
+    double t[3] = {0.0, 0.0, 0.0};  // the candidate's site, as delta_energy_insert forms it
+    for (int a = 0; a < D; ++a) t[a] = host_wrap(p[a] + 0.0, g.L[a]);
+    double de = es[k], dbound = eb[k], mag = 0.0;
+    if (std::isfinite(de)) {
+      int32_t m[3] = {0, 0, 0};
+      cell_of(t, m);
+      int32_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+      for (int a = 0; a < D; ++a) {
+        lo[a] = gs[a] < 3 ? 0 : -1;
+        hi[a] = gs[a] < 3 ? gs[a] - 1 : 1;
+      }
+      for (int32_t o0 = lo[0]; o0 <= hi[0] && std::isfinite(de); ++o0)
+        for (int32_t o1 = lo[1]; o1 <= hi[1] && std::isfinite(de); ++o1)
+          for (int32_t o2 = (D == 3 ? lo[2] : 0); o2 <= (D == 3 ? hi[2] : 0) && std::isfinite(de); ++o2) {
+            int32_t n[3] = {0, 0, 0};
+            const int32_t off[3] = {o0, o1, o2};
+            for (int a = 0; a < D; ++a) {
+              n[a] = gs[a] < 3 ? off[a] : ((m[a] + off[a]) % gs[a] + gs[a]) % gs[a];
+            }
+            const auto it = grid.find(key(n));
+            if (it == grid.end()) continue;
+            for (uint32_t j : it->second) {
+              bool edge = false;
+              const double one = host_pair(g, host_r2(g, t, added[j].site, added[j].flags), &edge);
+              if (edge) qi->ambiguous += 1;
+              if (std::isinf(one)) {
+                de = INFINITY;
+                break;
+              }
+              de += one;
+              mag += std::fabs(one);
+            }
+          }
+    }
+    if (std::isfinite(de)) {
+      // Microstate::add_body (microstate.rs:777-835): body = wrap(p),
+      // site = wrap(body + 0)
+      New nb{};
+      for (int a = 0; a < D; ++a) {
+        nb.body[a] = host_wrap(p[a], g.L[a]);
+        nb.site[a] = host_wrap(nb.body[a] + 0.0, g.L[a]);
+      }
+      nb.flags = host_ghost_flags(g, nb.site);
+      int32_t m[3] = {0, 0, 0};
+      cell_of(nb.site, m);
+      grid[key(m)].push_back((uint32_t)added.size());
+      added.push_back(nb);
+      count->accepted += 1;
+      qi->inserted += 1;
+      // the sign of dE within the bound of the summation order (the
+      // reference sums in VecCell order)
+      const double b2 = dbound + mag * 0x1p-50;
+      if (de != 0.0 && std::fabs(de) <= b2) qi->ambiguous += 1;
+      if (de > 0.0) overlapping += 1;
+      if (qi->inserted == qi->target || overlapping >= qi->allowed_overlaps) break;
+    } else {
+      count->rejected += 1;
+    }
+  }
+  if (added.empty()) return HMC_OK;
+  // the new context: the old bodies, then the inserted ones in order
+  const uint64_t N = g.n_bodies, A = added.size();
+  std::vector<double> body((size_t)(N + A) * D), site((size_t)(N + A) * D);
+  rc = hmc_get_state(c, body.data(), site.data());
+  if (rc) return rc;
+  for (uint64_t k = 0; k < A; ++k)
+    for (int a = 0; a < D; ++a) {
+      body[(size_t)(N + k) * D + a] = added[k].body[a];
+      site[(size_t)(N + k) * D + a] = added[k].site[a];
+    }
+  hmc_config cfg = c->cfg_user;
+  cfg.n_bodies = N + A;
+  hmc_ctx* d = nullptr;
+  rc = hmc_create(&cfg, &d);
+  if (rc) return fail(c, rc, "QuickInsert: creating the context of the larger microstate failed");
+  rc = hmc_set_state(d, body.data(), site.data());
+  if (rc) {
+    hmc_destroy(d);
+    return fail(c, rc, "QuickInsert: setting the state of the larger microstate failed");
+  }
+  hmc_destroy(c);
+  *pc = d;
+  return HMC_OK;
 }
 
 }  // extern "C"

@@ -203,6 +203,42 @@ void hmc_quick_compress_init(hmc_quick_compress* qc, double target_volume);
 int hmc_quick_compress_apply(hmc_quick_compress* qc, hmc_ctx** ctx, uint64_t step, uint32_t substep,
                              uint32_t seed);
 
+/* QuickInsert (hoomd-mc/src/quick_insert.rs:94-334) with a UniformIn
+ * distribution over the periodic box (hoomd-mc/src/uniform_in.rs:99-110):
+ * while the total energy is <= 0, insert bodies at uniformly random positions,
+ * rejecting the insertions of infinite energy, until `target` bodies are in or
+ * `allowed_overlaps` insertions of positive energy were made in one call.
+ * Point bodies only (spheres, disks, LJ, OverlapPenalty). */
+enum { HMC_QI_RUNNING = 0, HMC_QI_COMPLETE = 1 };
+typedef struct {
+  uint64_t target;           /* bodies to insert (QuickInsert::new) */
+  uint64_t allowed_overlaps; /* max(target / 8, 1) */
+  uint64_t inserted;         /* insertions so far */
+  int32_t state;             /* HMC_QI_* */
+  int32_t pad;
+  uint64_t ambiguous;        /* sign tests of an energy, and candidate pairs within 2^-30 of the
+                                cutoff (VecCell's edge rule), inside their rounding bound:
+                                decided by the computed value; expected 0 */
+} hmc_quick_insert;
+
+/* QuickInsert::new(distribution, target): Running, nothing inserted. */
+void hmc_quick_insert_init(hmc_quick_insert* qi, uint64_t target);
+
+/* QuickInsert::apply(microstate, hamiltonian) with (step, substep, seed) =
+ * microstate.counter().  Every candidate of the call is drawn on the host from
+ * Counter(step, substep, seed).make_rng() (one Uniform(-L/2, L/2) per axis),
+ * its insertion energy against the current bodies is computed on the device
+ * (k_insert_energy, all candidates in one launch), and the candidates are then
+ * taken in order on the host, each against the bodies inserted before it in
+ * the same call.  When bodies were inserted, *ctx is replaced by a context
+ * holding the old bodies followed by the new ones (Microstate::add_body order)
+ * and the old context destroyed.  *count gets the inserted (accepted) and
+ * rejected candidates.  When qi->state is not HMC_QI_COMPLETE after the call,
+ * the caller then calls increment_substep, as the reference's apply does (a
+ * call that finds the work complete returns before it). */
+int hmc_quick_insert_apply(hmc_quick_insert* qi, hmc_ctx** ctx, uint64_t step, uint32_t substep, uint32_t seed,
+                           hmc_count* count);
+
 /* One ParallelSweep::apply with Translate moves (parallel_sweep.rs:390-421).
  * (step, substep, seed) = microstate.counter(); kT = macrostate.temperature();
  * max_distance = Translate::maximum_distance.  On return *substeps_used holds

@@ -431,6 +431,39 @@ struct System {
     substep += 1;
   }
 
+  // QuickInsert::apply (hoomd-mc/src/quick_insert.rs:286-334) with a
+  // UniformIn distribution over the periodic box (uniform_in.rs:99-110:
+  // one Uniform(-L/2, L/2) draw per axis, cuboid.rs:590-598), single-site
+  // point bodies.  qi = {target, allowed_overlaps, inserted, complete}.
+  void quick_insert_apply(uint64_t* qi, uint64_t& acc, uint64_t& rej) {
+    acc = rej = 0;
+    if (qi[3]) return;
+    const double energy = total_energy();
+    if (energy <= 0.0 && qi[2] >= qi[0]) {
+      qi[3] = 1;
+      return;
+    }
+    if (energy <= 0.0) {
+      Sfc64 rng = counter_rng(step, substep, seed, 0);
+      uint64_t overlapping = 0;
+      for (uint64_t k = 0; k < qi[0]; ++k) {
+        Vec<D> p;
+        for (int a = 0; a < D; ++a) p[a] = rand_uniform(rng, -boundary.L[a] / 2.0, boundary.L[a] / 2.0);
+        const double de = delta_energy_insert(p);
+        if (std::isfinite(de)) {  // Microstate::add_body on a periodic boundary cannot fail
+          add_body(p);
+          acc += 1;
+          qi[2] += 1;
+          if (de > 0.0) overlapping += 1;
+          if (qi[2] == qi[0] || overlapping >= qi[1]) break;
+        } else {
+          rej += 1;
+        }
+      }
+    }
+    substep += 1;
+  }
+
   // microstate.rs:412-415
   void increment_step() {
     step += 1;

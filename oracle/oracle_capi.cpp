@@ -372,6 +372,15 @@ double oracle_total_energy(void* h) {
   DISPATCH(h, e = S.total_energy());
   return e;
 }
+// qi = {target, allowed_overlaps, inserted, complete}, updated in place
+void oracle_quick_insert_apply(void* h, uint64_t* qi, uint64_t* acc, uint64_t* rej) {
+  DISPATCH(h, S.quick_insert_apply(qi, *acc, *rej));
+}
+uint64_t oracle_n_bodies(void* h) {
+  uint64_t n = 0;
+  DISPATCH(h, n = S.bodies.size());
+  return n;
+}
 double oracle_delta_energy_insert(void* h, const double* body) {
   auto* H = (Handle*)h;
   if (H->dim == 2) return H->s2->delta_energy_insert({body[0], body[1]});

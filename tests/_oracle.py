@@ -77,6 +77,10 @@ def lib():
         L.oracle_total_energy.restype = C.c_double
         L.oracle_delta_energy_one.argtypes = [C.c_void_p, C.c_uint64, D_P]
         L.oracle_delta_energy_one.restype = C.c_double
+        L.oracle_quick_insert_apply.argtypes = [C.c_void_p, U64_P, U64_P, U64_P]
+        L.oracle_quick_insert_apply.restype = None
+        L.oracle_n_bodies.argtypes = [C.c_void_p]
+        L.oracle_n_bodies.restype = C.c_uint64
         L.oracle_delta_energy_insert.argtypes = [C.c_void_p, D_P]
         L.oracle_delta_energy_insert.restype = C.c_double
         L.oracle_trace_enable.argtypes = [C.c_void_p, C.c_int]
@@ -264,6 +268,17 @@ class OracleSystem:
     def delta_energy_one(self, body, trial):
         t = np.ascontiguousarray(trial, np.float64)
         return lib().oracle_delta_energy_one(self.h, body, _p(t, D_P))
+
+    def quick_insert(self, qi):
+        """QuickInsert::apply (quick_insert.rs:286-334) with UniformIn over the
+        box; qi is the state [target, allowed_overlaps, inserted, complete]
+        (np.uint64, updated in place).  Returns (accepted, rejected); the
+        substep advances by one."""
+        acc = np.zeros(1, np.uint64)
+        rej = np.zeros(1, np.uint64)
+        lib().oracle_quick_insert_apply(self.h, _p(qi, U64_P), _p(acc, U64_P), _p(rej, U64_P))
+        self.n = int(lib().oracle_n_bodies(self.h))
+        return int(acc[0]), int(rej[0])
 
     def delta_energy_insert(self, body):
         """PairwiseCutoff::delta_energy_insert of a new body at `body`."""
