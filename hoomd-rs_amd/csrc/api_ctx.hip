@@ -398,8 +398,9 @@ bool choose_regions_lj(const Geometry& g, int D, double occ, int cus, TileCfg* o
     if (g.shape[a] < 6 || g.reach[a] != 1) return false;
     lim[a] = std::min<uint32_t>(std::min<uint32_t>(g.half_shape[a], 8u), (uint32_t)(g.shape[a] - 4) / 2);
   }
-  // ziggurat tables, halo lists, 12 proposals of 104 bytes, 8 x 9 row offsets
-  const size_t static_lds = 2 * 257 * sizeof(double) + (2 * 128 + 1) * sizeof(uint32_t) + 12 * 104 + 8 * 9 * 4 + 64;
+  // ziggurat tables, halo lists, 12 proposals of 160 bytes, the shared
+  // trials' partial sums (2 x 8 x 11 floats)
+  const size_t static_lds = 2 * 257 * sizeof(double) + (2 * 128 + 1) * sizeof(uint32_t) + 12 * 160 + 2 * 8 * 11 * 4 + 64;
   const size_t cu_lds = 160 * 1024;
   const uint32_t cu_waves = 8;
   bool found = false;
@@ -749,6 +750,18 @@ int hmc_create(const hmc_config* cfg_in, hmc_ctx** out) {
       pc.band_lo = lo;
       pc.band_hi = hi;
       pc.perr = pe;
+      // |r2_32 - r2_ref| <= E (2 sum|d_a| + D E) + (5 u + 2^-50) r2 for every
+      // pair counted (r2 < band_hi, so sum|d_a| <= sqrt(D band_hi)), and the
+      // first-order bound on the pair energy holds within 2% where that is
+      // at most r2 / 1024
+      const double bmax = (E * (2.0 * std::sqrt((double)D * (double)hi) + D * E) +
+                           (double)hi * (5.0 * 0x1p-24 + 0x1p-50)) * 1.01;
+      float bm = (float)bmax;
+      while ((double)bm < bmax) bm = std::nextafter(bm, INFINITY);
+      float ro = (float)(1024.0 * (double)bm);
+      while ((double)ro < 1024.0 * (double)bm) ro = std::nextafter(ro, INFINITY);
+      pc.beta_max = bm;
+      pc.r2_open = ro;
       const void* fn = D == 2 ? (const void*)k_apply_lj<2> : (const void*)k_apply_lj<3>;
       const uint32_t threads = lj_region_threads(pc.trials);
       int per_cu = 0;

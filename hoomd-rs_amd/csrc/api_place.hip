@@ -1,5 +1,7 @@
-// QuickCompress (hoomd-mc/src/quick_compress.rs) over device energies and
-// boundary clones.
+// Initial-state preparation through the C ABI: QuickCompress
+// (hoomd-mc/src/quick_compress.rs) over device energies and boundary clones,
+// and QuickInsert (hoomd-mc/src/quick_insert.rs) over device insertion
+// energies.
 #include "host.hpp"
 
 #include <unordered_map>

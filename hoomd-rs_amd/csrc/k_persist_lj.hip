@@ -1,9 +1,11 @@
 // The persistent apply for isotropic pair energies (LJ and shifted LJ, mixed
 // precision): one cooperative launch runs every cycle and colour of an apply,
-// as k_apply_tile does for hard shapes.  A trial runs on one half-wavefront
-// (32 lanes); half-wave h of G takes trials h, h + G, ... of its region.
-// Workgroups have 1, 2 or 4 wavefronts, so that the wavefronts of the
-// workgroups sharing a CU spread evenly over its four SIMDs.
+// as k_apply_tile does for hard shapes.  Trials 0 .. G - 1 of a region run
+// on one half-wavefront (32 lanes) each, G the half-waves of the workgroup;
+// a trial beyond them runs on the whole workgroup, its half-waves' sums
+// combined in LDS.  Workgroups have 1, 2 or 4 wavefronts, so that the
+// wavefronts of the workgroups sharing a CU spread evenly over its four
+// SIMDs.
 //
 // A trial's candidates are the staged sites of its 3^D neighbour cells in
 // LDS (box-relative FP32, body index in w).  For each candidate the final
@@ -23,13 +25,17 @@
 // to box-relative FP32 on both sides, and the reference's own roundings of
 // the displacement and image shift), with u = 2^-24:
 //   |r2_32 - r2_ref| <= beta = E (2 sum|d_a| + D E) + (5 u + 2^-50) r2_32
-//     (the FP32 difference, squares and sums; the reference's r^2),
-//   |e_32 - e(r2_32)| <= 48 u A   (lj_energy32, A = 4 eps (s12 + s6) + |shift|),
-//   |e(r2_32) - e(r2_ref)| <= |de/dr2| beta, |de/dr2| = 4 eps |6 s12 - 3 s6| / r2
-//     taken at r2_32 with a relative margin (2^-16 of 4 eps (6 s12 + 3 s6) / r2)
-//     and a factor 1.02 for the interval (beta <= r2 / 1024, else the pair is
-//     left open).
-// The sums over a half-wave add gamma_(rounds + 5) (A_f + A_i); the
+//     (the FP32 difference, squares and sums; the reference's r^2), and for
+//     every pair counted (r2_32 < band_hi) beta <= tc.beta_max;
+//   |e_32 - e(r2_32)| <= 48 u A   (A = 4 eps (s12 + s6) + |shift|; the FP32
+//     powers s12 and s6 are summed separately and combined in FP64),
+//   |e(r2_32) - e(r2_ref)| <= |de/dr2| beta <= 4 eps (6 s12 + 3 s6) / r2 x
+//     beta_max x 1.02 (de/dr2 varies by less than 2% over the interval when
+//     beta <= r2 / 1024, i.e. r2_32 >= tc.r2_open; closer pairs are left
+//     open).  W = (6 s12 + 3 s6) / r2 is summed in FP32 and the bound taken
+//     as 1.03 (1 + 2^-16) 4 eps beta_max sum W, the extra factors covering
+//     W's own roundings and its sum.
+// The energy sums over a half-wave add gamma_(rounds + 5) (A_f + A_i); the
 // reference's FP64 terms and sum differ from exact by less than
 // 2^-40 (A_f + A_i).  A trial the FP32 values leave open is re-taken in FP64
 // over the candidates whose FP32 r^2 (final or initial) lies below the top
@@ -41,51 +47,57 @@ namespace hmc {
 constexpr int kLjRegionThreads = 256;  // upper bound of the workgroup size: 8 half-waves
 constexpr int kLjMaxTrials = 12;       // trials of a region per pass
 
-// The proposal of a half-wave's next trial, kept in LDS across the wait for
-// the neighbour regions (in registers it would push the kernel past its
-// register budget).  Written by lane 0 of the half-wave.
+// The proposal of a trial of the next pass, kept in LDS across the wait for
+// the neighbour regions.  Lane k of the workgroup draws trial k's, and with
+// it everything of the trial that depends on the proposal alone: the trial
+// position, its space test, the FP32 images of the trial and initial sites,
+// and the site and LDS entry an acceptance writes.
 struct LjProp {
-  double v[3];   // displacement (Ball::sample)
-  double bp[3];  // the chosen body's position
   Sfc64 rng;     // the trial's stream after the proposal (Metropolis draws)
-  uint32_t s, e0, pick, body, go, pad;
+  double t[3];   // trial body position wrap(wrap(x + v)) (the site before wrap(.. + 0))
+  double4 sp;    // the accepted site: wrap(t + 0) with its ghost flags
+  float4 ent;    // its LDS entry
+  float tr[3];   // FP32 box-relative trial site
+  float xr[3];   // FP32 box-relative initial site (the body's LDS entry)
+  uint32_t s, self, body, go, in_space, pad;
 };
 
-static_assert(sizeof(LjProp) == 104, "proposal size counted in the host's LDS budget");
+static_assert(sizeof(LjProp) == 160, "proposal size counted in the host's LDS budget");
 
 struct Lj32 {
-  float e, a, b;  // sum of FP32 energies, of their magnitudes A, of the position-error bounds
+  // over the pairs within r_cut: the sums of s^12 and s^6 (s = sigma / r), of
+  // s^12 / r^2 and s^6 / r^2, and their count
+  float s12, s6, t12, t6, n;
 };
 
 // One candidate q (box-relative FP32 entry) against the FP32 site p: adds
-// its term to acc when within r_cut; sets *open when the membership or the
-// bound cannot be decided in FP32.
+// its powers to acc when within r_cut; sets *open when the membership cannot
+// be decided in FP32 or the pair is closer than the error bound allows.
+// The energy sum is 4 eps (S12 - S6) - n shift, its magnitude sum
+// 4 eps (S12 + S6) + n |shift|, and W = 6 T12 + 3 T6.
 template <int D>
-__device__ __forceinline__ void lj_pair32(const float* p, float4 q, float lo, float hi, float perr, float sig2,
-                                          float eps4, float shift, Lj32& acc, bool& open) {
+__device__ __forceinline__ void lj_pair32(const float* p, float4 q, float lo, float hi, float r2_open, float sig2,
+                                          Lj32& acc, bool& open) {
   const float dx = q.x - p[0], dy = q.y - p[1];
-  float r2, sabs;
+  float r2;
   if (D == 3) {
     const float dz = q.z - p[2];
     r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-    sabs = fabsf(dx) + fabsf(dy) + fabsf(dz);
   } else {
     r2 = fmaf(dx, dx, dy * dy);
-    sabs = fabsf(dx) + fabsf(dy);
   }
   const bool in = r2 < lo, out = r2 >= hi;
-  if (!(in || out)) open = true;  // also NaN: a site without images
+  if (!(in || out) || r2 < r2_open) open = true;  // also NaN: a site without images
   if (in) {
-    const float beta = perr * (2.0f * sabs + (float)D * perr) + r2 * (5.0f * 0x1p-24f + 0x1p-50f);
-    if (beta * 1024.0f > r2) open = true;
     const float rr = __builtin_amdgcn_rcpf(r2);
     const float s2 = sig2 * rr;
     const float s4 = s2 * s2, s8 = s4 * s4;
     const float s12 = s4 * s8, s6 = s2 * s4;
-    acc.e += eps4 * (s12 - s6) - shift;
-    acc.a += eps4 * (s12 + s6) + fabsf(shift);
-    const float dd = eps4 * (fabsf(6.0f * s12 - 3.0f * s6) + 0x1p-16f * (6.0f * s12 + 3.0f * s6)) * rr;
-    acc.b += 1.02f * dd * beta;
+    acc.s12 += s12;
+    acc.s6 += s6;
+    acc.t12 = fmaf(s12, rr, acc.t12);
+    acc.t6 = fmaf(s6, rr, acc.t6);
+    acc.n += 1.0f;
   }
 }
 
@@ -265,7 +277,7 @@ __global__ void __launch_bounds__(kLjRegionThreads, 2) k_apply_lj(Geometry g, Ap
   __shared__ unsigned blk_acc;
   __shared__ uint32_t hlo[kLjHalo], hpre[kLjHalo + 1];
   __shared__ LjProp prop[kLjMaxTrials];
-  __shared__ uint32_t radj[kLjRegionThreads / 32][NR];
+  __shared__ float shr[2][kLjRegionThreads / 32][11];  // shared trials: per half-wave partial sums, two buffers
   const uint32_t tid = threadIdx.x;
   const uint32_t wid = tid >> 6, lane = tid & 63u;
   const uint32_t hw = tid >> 5, hl = tid & 31u, G = blockDim.x >> 5;  // trials hw, hw + G, ...
@@ -278,7 +290,7 @@ __global__ void __launch_bounds__(kLjRegionThreads, 2) k_apply_lj(Geometry g, Ap
   for (int a = 0; a < D; ++a) origin[a] = st->origin[a];
   const Region rg = region_of<D>(g, tc, origin, tile);
   const float sig2 = (float)(g.lj_sigma * g.lj_sigma), eps4 = (float)g.lj_eps4, shift = (float)g.lj_shift;
-  const float blo = tc.band_lo, bhi = tc.band_hi, perr = tc.perr;
+  const float blo = tc.band_lo, bhi = tc.band_hi, r2_open = tc.r2_open;
   const uint32_t ntr = tc.trials;
   const uint32_t nnb = D == 3 ? 26u : 8u;
   const uint32_t nbt = wid == 0 && lane < nnb ? region_neighbour<D>(tc, rg, lane) : tile;
@@ -290,9 +302,9 @@ __global__ void __launch_bounds__(kLjRegionThreads, 2) k_apply_lj(Geometry g, Ap
 
   unsigned retests = 0, ambiguous = 0;
   // The proposals of pass i, drawn while the neighbour regions finish pass
-  // i - 1 (they read only static cell data and this region's own bodies);
-  // trial k's waits in prop[k].  Trial k of colour c is the space with digits
-  // 2 h + c_a of the region (h the digits of k in T).
+  // i - 1 (they read only static cell data and this region's own bodies),
+  // one lane per trial; trial k's waits in prop[k].  Trial k of colour c is
+  // the space with digits 2 h + c_a of the region (h the digits of k in T).
   auto trial_space = [&](uint32_t colour, uint32_t k, uint32_t* cb, bool* active) {
     uint32_t s = 0, hh[3] = {0, 0, 0};
     for (int a = D - 1; a >= 0; --a) {
@@ -313,46 +325,190 @@ __global__ void __launch_bounds__(kLjRegionThreads, 2) k_apply_lj(Geometry g, Ap
     uint32_t cb[3] = {0, 0, 0};
     bool active = false;
     const uint32_t s = trial_space(colour, k, cb, &active);
-    uint32_t go = 0, e0 = 0, pick = 0, body = 0;
-    double v[3] = {0.0, 0.0, 0.0};
-    double4 bp = make_double4(0.0, 0.0, 0.0, 0.0);
-    Sfc64 rng{0, 0, 0, 0};
+    LjProp& pr = prop[k];
+    pr.s = s;
+    pr.go = 0;
     if (active && !overflow) {
       const uint32_t crow = D == 3 ? cb[0] * tc.E[1] + cb[1] : cb[0];
-      e0 = coff[crow * CW + cb[AL]];
+      const uint32_t e0 = coff[crow * CW + cb[AL]];
       const uint32_t n = coff[crow * CW + cb[AL] + 1] - e0;
       if (n) {
-        go = 1;
-        rng = counter_rng(step, substep, seed, s);
-        pick = choose_index(rng, n);
-        body = (uint32_t)__float_as_int(lb[e0 + pick].w);
-        bp = B.bpos[body];
+        Sfc64 rng = counter_rng(step, substep, seed, s);
+        const uint32_t self = e0 + choose_index(rng, n);
+        const float4 se = lb[self];
+        const uint32_t body = (uint32_t)__float_as_int(se.w);
+        const double4 bp = B.bpos[body];
+        double v[3] = {0.0, 0.0, 0.0};
         ball_sample<D>(rng, max_move, zx, zf, v);
+        double x[3] = {bp.x, bp.y, bp.z};
+        for (int a = 0; a < D; ++a) x[a] += v[a];
+        for (int a = 0; a < D; ++a) x[a] = wrap1(x[a], g.L[a]);  // generate_trial_moves: wrap(propose(..))
+        double t[3] = {0.0, 0.0, 0.0};
+        for (int a = 0; a < D; ++a) t[a] = wrap1(x[a], g.L[a]);  // wrap(x + 0) == wrap(x)
+        // update_body_properties (microstate.rs:985-1034): site = wrap(body + 0)
+        double nsp[3] = {0.0, 0.0, 0.0};
+        for (int a = 0; a < D; ++a) nsp[a] = wrap1(t[a] + 0.0, g.L[a]);
+        const double4 sp = pack_site(nsp, ghost_flags<D>(g, nsp));
+        const float4 te = stage_entry64<D>(g, rg.c64, make_double4(t[0], t[1], t[2], 0.0), 0u);
+        pr.rng = rng;
+        for (int a = 0; a < 3; ++a) pr.t[a] = t[a];
+        pr.sp = sp;
+        pr.ent = stage_entry64<D>(g, rg.c64, sp, body);
+        pr.tr[0] = te.x;
+        pr.tr[1] = te.y;
+        pr.tr[2] = te.z;
+        pr.xr[0] = se.x;
+        pr.xr[1] = se.y;
+        pr.xr[2] = se.z;
+        pr.self = self;
+        pr.body = body;
+        pr.in_space = space_of<D>(g, origin, x) == s ? 1u : 0u;
+        pr.go = 1;
       }
     }
-    if (hl == 0) {
-      LjProp& pr = prop[k];
-      pr.s = s;
-      pr.go = go;
-      pr.e0 = e0;
-      pr.pick = pick;
-      pr.body = body;
-      pr.rng = rng;
-      for (int a = 0; a < 3; ++a) pr.v[a] = v[a];
-      pr.bp[0] = bp.x;
-      pr.bp[1] = bp.y;
-      pr.bp[2] = bp.z;
+  };
+
+  // The LDS trial of proposal pr (colour `colour`, trial k): the trial site
+  // and its FP32 image, and the rows of three cells around its cell -- the
+  // prefix of their lengths and the offset from a candidate's index to its
+  // LDS entry (that of row 0, then its change at each later row), in
+  // registers and the same in every lane.  A candidate's offset is summed
+  // from masks rather than selected, so that the compiler keeps the table
+  // in registers.
+  struct LdsTrial {
+    bool in_space;
+    uint32_t self, body, total;
+    double t[3];
+    float tr[3], xr[3];
+    uint32_t pre[NR + 1], dadj[NR];
+  };
+  auto lds_trial = [&](const LjProp& pr, uint32_t colour, uint32_t k) {
+    LdsTrial lt;
+    lt.self = pr.self;
+    lt.body = pr.body;
+    lt.in_space = pr.in_space != 0;
+    uint32_t pcb[3] = {0, 0, 0};
+    bool act = false;
+    (void)trial_space(colour, k, pcb, &act);
+    for (int a = 0; a < 3; ++a) {
+      lt.t[a] = pr.t[a];
+      lt.tr[a] = pr.tr[a];
+      lt.xr[a] = pr.xr[a];
+    }
+    lt.pre[0] = 0;
+    uint32_t adj_prev = 0;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int i0 = D == 3 ? r / 3 : r;
+      const int i1 = D == 3 ? r % 3 : 0;
+      const uint32_t row = D == 3 ? (pcb[0] + i0 - 1) * tc.E[1] + (pcb[1] + i1 - 1) : pcb[0] + i0 - 1;
+      const uint32_t lo_r = coff[row * CW + pcb[AL] - 1];
+      lt.pre[r + 1] = lt.pre[r] + (coff[row * CW + pcb[AL] + 2] - lo_r);
+      const uint32_t adj_r = lo_r - lt.pre[r];
+      lt.dadj[r] = adj_r - adj_prev;  // dadj[0] = the offset of row 0
+      adj_prev = adj_r;
+    }
+    lt.total = lt.pre[NR];
+    return lt;
+  };
+  auto slot_of = [&](const LdsTrial& lt, uint32_t c) -> uint32_t {
+    uint32_t off = lt.dadj[0];
+#pragma unroll
+    for (int k = 1; k < NR; ++k) off += lt.dadj[k] & (0u - (uint32_t)(c >= lt.pre[k]));
+    return c + off;
+  };
+  // FP32 sums over candidates gl, gl + GS, ... of the trial; the next
+  // candidate's entry is loaded while this one is summed (past the end, and
+  // for the trial's own body, the entry read is the body's own and skipped)
+  auto sums32 = [&](const LdsTrial& lt, uint32_t gl, uint32_t GS, Lj32& af, Lj32& ai, bool& open) {
+    uint32_t q = gl < lt.total ? slot_of(lt, gl) : lt.self;
+    float4 e = lb[q];
+    for (uint32_t c = gl; c < lt.total; c += GS) {
+      const uint32_t qn = c + GS < lt.total ? slot_of(lt, c + GS) : lt.self;
+      const float4 en = lb[qn];
+      if (q != lt.self) {
+        lj_pair32<D>(lt.tr, e, blo, bhi, r2_open, sig2, af, open);
+        lj_pair32<D>(lt.xr, e, blo, bhi, r2_open, sig2, ai, open);
+      }
+      q = qn;
+      e = en;
+    }
+  };
+  // The decision from the FP32 sums of both sides (combined in FP64 into
+  // energies, magnitudes and W) when their bound settles it; per-lane rounds
+  // of summation `rounds`
+  struct Sums {
+    double s12, s6, t12, t6, n;
+  };
+  auto decide32 = [&](Sfc64& rng, const Sums& f, const Sums& in, double rounds, bool* decided) {
+    const double e4 = (double)eps4, sh = (double)shift;
+    const double ef = e4 * (f.s12 - f.s6) - f.n * sh, ei = e4 * (in.s12 - in.s6) - in.n * sh;
+    const double sa = e4 * (f.s12 + f.s6 + in.s12 + in.s6) + (f.n + in.n) * fabs(sh);
+    const double sw = 6.0 * (f.t12 + in.t12) + 3.0 * (f.t6 + in.t6);
+    const double sb = 1.03 * (1.0 + 0x1p-16) * e4 * (double)tc.beta_max * sw;
+    const double dH32 = ef - ei;
+    const double delta = (((48.0 + rounds + 8.0) * 0x1p-24 + 0x1p-40) * sa + sb) * (1.0 + 0x1p-10) + 0x1p-1070;
+    bool accepted = false;
+    *decided = false;
+    if (isfinite(dH32) && isfinite(delta)) {
+      bool amb32 = false;
+      accepted = lj_metropolis(rng, dH32, delta, kT, &amb32) != 0;
+      *decided = !amb32;
+    }
+    return accepted;
+  };
+  // The exact FP64 re-take on one half-wave, with the same draws
+  auto retake64 = [&](const LdsTrial& lt, Sfc64& rng, bool* amb) {
+    const double4 sp = ld_dev_d4(&B.spos[lt.body]);
+    const double xi[3] = {sp.x, sp.y, sp.z};
+    const Lj64 s64 = lj_sums64<D>(g, B.spos, lt.t, xi, lt.total, [&](uint32_t c) -> uint32_t {
+      const uint32_t q = slot_of(lt, c);
+      if (q == lt.self) return UINT32_MAX;
+      const float4 e = lb[q];
+      float dfx = e.x - lt.tr[0], dfy = e.y - lt.tr[1], dix = e.x - lt.xr[0], diy = e.y - lt.xr[1];
+      float r2f = fmaf(dfx, dfx, dfy * dfy), r2i = fmaf(dix, dix, diy * diy);
+      if (D == 3) {
+        const float dfz = e.z - lt.tr[2], diz = e.z - lt.xr[2];
+        r2f = fmaf(dfz, dfz, r2f);
+        r2i = fmaf(diz, diz, r2i);
+      }
+      // both far beyond r_cut in FP32: no term of the reference's sums
+      return (r2f >= bhi && r2i >= bhi) ? UINT32_MAX : (uint32_t)__float_as_int(e.w);
+    });
+    return lj_decide64(rng, s64, kT, amb);
+  };
+  // update_body_properties (microstate.rs:985-1034) of an accepted trial
+  auto accept_move = [&](const LjProp& pr) {
+    B.bpos[pr.body] = make_double4(pr.t[0], pr.t[1], pr.t[2], stamp);
+    st_dev_d4(&B.spos[pr.body], pr.sp);
+    lb[pr.self] = pr.ent;
+  };
+  auto trace_trial = [&](uint32_t colour, uint32_t substep, uint32_t k, uint32_t body, bool accepted) {
+    unsigned long long ti = atomicAdd((unsigned long long*)&st->trace_n, 1ull);
+    if (ti < w.trace_cap) {
+      uint32_t cbt[3] = {0, 0, 0};
+      bool act = false;
+      hmc_trace_record rec;
+      rec.substep = substep;
+      rec.color = colour;
+      rec.space = trial_space(colour, k, cbt, &act);
+      rec.body = (int64_t)body;
+      rec.status = accepted ? 1 : 0;
+      rec.pad = 0;
+      w.trace[ti] = rec;
     }
   };
 
   const uint32_t npass = cycles * NCOL;
-  if (npass)
-    for (uint32_t k = hw; k < ntr; k += G) propose(0, k);
-  wave_lds_order();
+  if (npass && tid < ntr) propose(0, tid);
+  __syncthreads();
   unsigned nacc = 0;  // accepted trials (lane 0 of each half-wave)
   for (uint32_t i = 0; i < npass; ++i) {
     const uint32_t colour = i & (NCOL - 1), substep = substep0 + (i >> D);
-    for (uint32_t k = hw; k < ntr; k += G) {
+    // trials 0 .. G - 1: one half-wave each (in a region that overflows its
+    // LDS, every trial: half-wave h takes h, h + G, ...)
+    const uint32_t own_end = overflow ? ntr : min(ntr, G);
+    for (uint32_t k = hw; k < own_end; k += G) {
       const LjProp& pr = prop[k];
       bool accepted = false, traced = false;
       uint32_t tbody = 0;
@@ -433,135 +589,97 @@ __global__ void __launch_bounds__(kLjRegionThreads, 2) k_apply_lj(Geometry g, Ap
         }
       } else if (pr.go) {
         // ---- the trial of this pass from LDS
-        const uint32_t ps = pr.s, self = pr.e0 + pr.pick, body = pr.body;
-        uint32_t pcb[3] = {0, 0, 0};
-        bool act = false;
-        (void)trial_space(colour, k, pcb, &act);
+        const LdsTrial lt = lds_trial(pr, colour, k);
         traced = true;
-        tbody = body;
-        double x[3] = {pr.bp[0], pr.bp[1], pr.bp[2]};
-        for (int a = 0; a < D; ++a) x[a] += pr.v[a];
-        for (int a = 0; a < D; ++a) x[a] = wrap1(x[a], g.L[a]);  // generate_trial_moves: wrap(propose(..))
-        if (space_of<D>(g, origin, x) == ps) {
-          double t[3] = {0.0, 0.0, 0.0};  // wrap(x + 0) == wrap(x)
-          for (int a = 0; a < D; ++a) t[a] = wrap1(x[a], g.L[a]);
-          float tr[3] = {0.0f, 0.0f, 0.0f}, xr[3] = {0.0f, 0.0f, 0.0f};
-          {
-            const float4 te = stage_entry64<D>(g, rg.c64, make_double4(t[0], t[1], D == 3 ? t[2] : 0.0, 0.0), 0u);
-            tr[0] = te.x;
-            tr[1] = te.y;
-            tr[2] = te.z;
-            const float4 se = lb[self];
-            xr[0] = se.x;
-            xr[1] = se.y;
-            xr[2] = se.z;
-          }
-          // rows of three cells around the trial's cell: the prefix of their
-          // lengths in registers, and per row the offset from a candidate's
-          // index to its LDS entry in this half-wave's table (row r* of
-          // candidate c is the number of row starts at or below c; indexing a
-          // register array with it would put the array in scratch memory)
-          uint32_t pre[NR + 1];
-          pre[0] = 0;
-          uint32_t* adj = radj[hw];
-  #pragma unroll
-          for (int r = 0; r < NR; ++r) {
-            const int i0 = D == 3 ? r / 3 : r;
-            const int i1 = D == 3 ? r % 3 : 0;
-            const uint32_t row = D == 3 ? (pcb[0] + i0 - 1) * tc.E[1] + (pcb[1] + i1 - 1) : pcb[0] + i0 - 1;
-            const uint32_t lo_r = coff[row * CW + pcb[AL] - 1];
-            pre[r + 1] = pre[r] + (coff[row * CW + pcb[AL] + 2] - lo_r);
-            if (hl == (uint32_t)r) adj[r] = lo_r - pre[r];
-          }
-          wave_lds_order();
-          const uint32_t total = pre[NR];
-          auto slot_of = [&](uint32_t c) -> uint32_t {
-            uint32_t r = 0;
-  #pragma unroll
-            for (int k = 1; k < NR; ++k) r += c >= pre[k] ? 1u : 0u;
-            return c + adj[r];
-          };
-          Lj32 af{0.0f, 0.0f, 0.0f}, ai{0.0f, 0.0f, 0.0f};
+        tbody = lt.body;
+        if (lt.in_space) {
+          Lj32 af{0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, ai{0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
           bool open = false;
-          for (uint32_t c = hl; c < total; c += 32u) {
-            const uint32_t q = slot_of(c);
-            if (q == self) continue;
-            const float4 e = lb[q];
-            lj_pair32<D>(tr, e, blo, bhi, perr, sig2, eps4, shift, af, open);
-            lj_pair32<D>(xr, e, blo, bhi, perr, sig2, eps4, shift, ai, open);
-          }
+          sums32(lt, hl, 32u, af, ai, open);
           open = half_any(open);
           bool decided = false;
           Sfc64 rng = pr.rng;
           if (!open) {
-            const double ef = (double)half_sum(af.e), ei = (double)half_sum(ai.e);
-            const double sa = (double)half_sum(af.a) + (double)half_sum(ai.a);
-            const double sb = (double)half_sum(af.b) + (double)half_sum(ai.b);
-            const double rounds = (double)((total + 31u) / 32u);
-            const double dH32 = ef - ei;
-            const double delta = (((48.0 + rounds + 8.0) * 0x1p-24 + 0x1p-40) * sa + sb) * (1.0 + 0x1p-10) + 0x1p-1070;
-            if (isfinite(dH32) && isfinite(delta)) {
-              bool amb32 = false;
-              accepted = lj_metropolis(rng, dH32, delta, kT, &amb32) != 0;
-              decided = !amb32;
-            }
+            const Sums f{half_sum(af.s12), half_sum(af.s6), half_sum(af.t12), half_sum(af.t6), half_sum(af.n)};
+            const Sums in{half_sum(ai.s12), half_sum(ai.s6), half_sum(ai.t12), half_sum(ai.t6), half_sum(ai.n)};
+            accepted = decide32(rng, f, in, (double)((lt.total + 31u) / 32u), &decided);
           }
           if (!decided) {
-            // exact FP64 re-take with the same draws
             rng = pr.rng;
             retests += hl == 0 ? 1u : 0u;
-            const double4 sp = ld_dev_d4(&B.spos[body]);
-            const double xi[3] = {sp.x, sp.y, sp.z};
-            const Lj64 s64 = lj_sums64<D>(g, B.spos, t, xi, total, [&](uint32_t c) -> uint32_t {
-              const uint32_t q = slot_of(c);
-              if (q == self) return UINT32_MAX;
-              const float4 e = lb[q];
-              float dfx = e.x - tr[0], dfy = e.y - tr[1], dix = e.x - xr[0], diy = e.y - xr[1];
-              float r2f = fmaf(dfx, dfx, dfy * dfy), r2i = fmaf(dix, dix, diy * diy);
-              if (D == 3) {
-                const float dfz = e.z - tr[2], diz = e.z - xr[2];
-                r2f = fmaf(dfz, dfz, r2f);
-                r2i = fmaf(diz, diz, r2i);
-              }
-              // both far beyond r_cut in FP32: no term of the reference's sums
-              return (r2f >= bhi && r2i >= bhi) ? UINT32_MAX : (uint32_t)__float_as_int(e.w);
-            });
             bool amb = false;
-            accepted = lj_decide64(rng, s64, kT, &amb);
+            accepted = retake64(lt, rng, &amb);
             ambiguous += (amb && hl == 0) ? 1u : 0u;
           }
-          if (accepted && hl == 0) {
-            // update_body_properties (microstate.rs:985-1034)
-            double nsp[3] = {0.0, 0.0, 0.0};
-            for (int a = 0; a < D; ++a) nsp[a] = wrap1(t[a] + 0.0, g.L[a]);
-            const double4 sp = pack_site(nsp, ghost_flags<D>(g, nsp));
-            B.bpos[body] = make_double4(t[0], t[1], t[2], stamp);
-            st_dev_d4(&B.spos[body], sp);
-            lb[self] = stage_entry64<D>(g, rg.c64, sp, body);
-          }
+          if (accepted && hl == 0) accept_move(pr);
         }
       }
-      if (traced && hl == 0 && w.tracing) {
-        unsigned long long ti = atomicAdd((unsigned long long*)&st->trace_n, 1ull);
-        if (ti < w.trace_cap) {
-          uint32_t cbt[3] = {0, 0, 0};
-          bool act = false;
-          hmc_trace_record rec;
-          rec.substep = substep;
-          rec.color = colour;
-          rec.space = trial_space(colour, k, cbt, &act);
-          rec.body = (int64_t)tbody;
-          rec.status = accepted ? 1 : 0;
-          rec.pad = 0;
-          w.trace[ti] = rec;
-        }
-      }
+      if (traced && hl == 0 && w.tracing) trace_trial(colour, substep, k, tbody, accepted);
       nacc += (accepted && hl == 0) ? 1u : 0u;
+    }
+    // trials G .. ntr - 1: the whole workgroup on each, its half-waves' sums
+    // combined in LDS (in FP64, in half-wave order) and the decision taken
+    // by half-wave 0 (the FP64 re-take too)
+    for (uint32_t k = G; k < ntr && !overflow; ++k) {
+      const LjProp& pr = prop[k];
+      bool accepted = false, run = false;
+      LdsTrial lt{};
+      if (pr.go) {  // uniform over the workgroup
+        lt = lds_trial(pr, colour, k);
+        run = lt.in_space;
+      }
+      if (run) {
+        Lj32 af{0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, ai{0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        bool open = false;
+        sums32(lt, tid, blockDim.x, af, ai, open);
+        float* red = shr[k & 1u][hw];
+        const bool hopen = half_any(open);
+        const float part[11] = {half_sum(af.s12), half_sum(af.s6), half_sum(af.t12), half_sum(af.t6), half_sum(af.n),
+                                half_sum(ai.s12), half_sum(ai.s6), half_sum(ai.t12), half_sum(ai.t6), half_sum(ai.n),
+                                hopen ? 1.0f : 0.0f};
+        if (hl == 0)
+          for (int q = 0; q < 11; ++q) red[q] = part[q];
+        __syncthreads();
+        if (hw == 0) {
+          Sums f{0.0, 0.0, 0.0, 0.0, 0.0}, in{0.0, 0.0, 0.0, 0.0, 0.0};
+          bool any_open = false;
+          for (uint32_t h = 0; h < G; ++h) {
+            const float* r = shr[k & 1u][h];
+            f.s12 += (double)r[0];
+            f.s6 += (double)r[1];
+            f.t12 += (double)r[2];
+            f.t6 += (double)r[3];
+            f.n += (double)r[4];
+            in.s12 += (double)r[5];
+            in.s6 += (double)r[6];
+            in.t12 += (double)r[7];
+            in.t6 += (double)r[8];
+            in.n += (double)r[9];
+            any_open |= r[10] != 0.0f;
+          }
+          bool decided = false;
+          Sfc64 rng = pr.rng;
+          const double rounds = (double)((lt.total + blockDim.x - 1) / blockDim.x);
+          if (!any_open) accepted = decide32(rng, f, in, rounds, &decided);
+          if (!decided) {
+            rng = pr.rng;
+            retests += hl == 0 ? 1u : 0u;
+            bool amb = false;
+            accepted = retake64(lt, rng, &amb);
+            ambiguous += (amb && hl == 0) ? 1u : 0u;
+          }
+          if (accepted && hl == 0) accept_move(pr);
+        }
+      }
+      if (tid == 0) {
+        if (pr.go && w.tracing) trace_trial(colour, substep, k, lt.body, accepted);
+        nacc += accepted ? 1u : 0u;
+      }
     }
     if (i + 1 == npass) break;  // no later pass reads the halo
     region_publish(w.mbox, nbt, nnb, i & 1u);
     if (tid == 0) w.mbox[tile * kMailboxStride + 4] = i + 1;  // progress: passes published
-    for (uint32_t k = hw; k < ntr; k += G) propose(i + 1, k);
+    if (tid < ntr) propose(i + 1, tid);
     region_wait(st, w.mbox, tile, nnb, i);
     if (tid == 0) w.mbox[tile * kMailboxStride + 5] = i + 1;  // progress: waits passed
     if (!overflow)

@@ -477,6 +477,8 @@ struct TileCfg {
   uint32_t lds_bytes;
   float band_lo, band_hi;  // FP32 band of the box-relative test (isotropic: of the r_cut membership test)
   float perr;              // isotropic persistent apply: bound on the FP32 displacement error per axis
+  float beta_max;          // isotropic persistent apply: bound on |r2_32 - r2_ref| of a pair below band_hi
+  float r2_open;           // isotropic persistent apply: pairs with FP32 r2 below it are left to FP64
   uint32_t probe;          // persistent apply: 1 = only check that every region is resident (region_probe)
   uint32_t step[3];        // the workgroup size in the mixed radix (E0, .., E[D-2], E[D-1] + 2) of the row starts
 };
