@@ -101,10 +101,24 @@ __device__ __forceinline__ void lj_pair32(const float* p, float4 q, float lo, fl
   }
 }
 
+// A DPP move of v (quad_perm 0x00-0xff, row_mirror 0x140, row_half_mirror
+// 0x141, row_bcast15 0x142); rows outside ROW_MASK read 0.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
+}
+// Sum over the 32 lanes of each half-wave, in every lane of it: within
+// quads, half-rows and rows by DPP moves (no LDS round trips), then row 1
+// (3) adds row 0's (2's) total and lane 31 (63) is read back.
 __device__ __forceinline__ float half_sum(float v) {
-#pragma unroll
-  for (int d = 16; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-  return v;
+  v += dpp_f32<0xb1>(v);  // quad_perm [1, 0, 3, 2]
+  v += dpp_f32<0x4e>(v);  // quad_perm [2, 3, 0, 1]
+  v += dpp_f32<0x141>(v);
+  v += dpp_f32<0x140>(v);
+  v += dpp_f32<0x142, 0xa>(v);
+  const float lo = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 31));
+  const float hi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+  return (__lane_id() & 32u) ? hi : lo;
 }
 __device__ __forceinline__ double half_sum(double v) {
 #pragma unroll
