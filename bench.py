@@ -326,6 +326,7 @@ def other_workloads(hmc, wl, cpu=True, only=None, fp64=True):
         seeds = [42 + r for r in range(len(systems))]
         tune_ms, policy = batch.autotune(0, seeds, 1.0, 0.08, 2)  # tile layout (state left unchanged)
         batch.run(0, seeds, 1.0, 0.08, 2)
+        cyc0 = [batch.stats(i)["cycles"] for i in range(len(systems))]
         t0 = time.perf_counter()
         cnt = batch.run(2, seeds, 1.0, 0.08, 40)
         el = time.perf_counter() - t0
@@ -344,8 +345,7 @@ def other_workloads(hmc, wl, cpu=True, only=None, fp64=True):
             k = [int(np.floor(x / 1.0)) for x in L_]
             s_i = int(np.prod([x - (x % 2) for x in k]))
             spaces += s_i
-            st_i = batch.stats(i)
-            cycles_i = st_i["cycles"] / max(st_i["applies"], 1)  # colour cycles per step
+            cycles_i = (batch.stats(i)["cycles"] - cyc0[i]) / 40.0  # colour cycles per step
             pass_b += 8 * cycles_i * (4 * s_i + 16 * len(p))
         step_b = (2 * 32 * nb + 4 * 4 * nb + 3 * 4 * spaces) + pass_b + (36 * t + 80 * acc) / 40
         ach = step_b / (el / 40) / 1e9

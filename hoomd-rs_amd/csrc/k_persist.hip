@@ -139,12 +139,16 @@ __global__ void __launch_bounds__(kTileThreads, 2) k_apply_tile(Geometry g, Appl
   // The proposal of pass i (cycle i / 2^D, colour i mod 2^D) depends only on
   // the counter, the space's body count and the chosen body's position, which
   // no other colour changes; it is drawn while the neighbour regions finish
-  // the previous pass.
-  bool pgo = false;
+  // the previous pass, and with it everything of the trial that depends on
+  // the proposal alone: the wrapped trial position and its space test, its
+  // FP32 image, and the site and LDS entry an acceptance writes.
+  bool pgo = false, pin = false;
   uint32_t ps = 0, pcrow = 0, pe0 = 0, pn = 0, ppick = 0, pbody = 0;
   uint32_t pcb[3] = {0, 0, 0};
-  double pv[3] = {0.0, 0.0, 0.0};
-  double4 pbp = make_double4(0.0, 0.0, 0.0, 0.0);
+  double pt[3] = {0.0, 0.0, 0.0};
+  float ptr[3] = {0.0f, 0.0f, 0.0f};
+  double4 psp = make_double4(0.0, 0.0, 0.0, 0.0);
+  float4 pent = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   auto propose = [&](uint32_t i) {
     const uint32_t colour = i & (NCOL - 1), substep = substep0 + (i >> D);
     bool active = has_trial && !overflow;
@@ -168,8 +172,23 @@ __global__ void __launch_bounds__(kTileThreads, 2) k_apply_tile(Geometry g, Appl
     Sfc64 rng = counter_rng(step, substep, seed, s);
     ppick = choose_index(rng, pn);
     pbody = (uint32_t)__float_as_int(lb[pe0 + ppick].w);
-    pbp = B.bpos[pbody];
-    ball_sample<D>(rng, max_move, zx, zf, pv);
+    const double4 bp = B.bpos[pbody];
+    double v[3] = {0.0, 0.0, 0.0};
+    ball_sample<D>(rng, max_move, zx, zf, v);
+    double x[3] = {bp.x, bp.y, bp.z};
+    for (int a = 0; a < D; ++a) x[a] += v[a];
+    for (int a = 0; a < D; ++a) x[a] = wrap1(x[a], g.L[a]);  // generate_trial_moves: wrap(propose(..))
+    pin = space_of<D>(g, origin, x) == s;
+    for (int a = 0; a < D; ++a) pt[a] = wrap1(x[a], g.L[a]);
+    const float4 te = stage_entry64<D>(g, c64, make_double4(pt[0], pt[1], D == 3 ? pt[2] : 0.0, 0.0), 0u);
+    ptr[0] = te.x;
+    ptr[1] = te.y;
+    ptr[2] = te.z;
+    // update_body_properties (microstate.rs:985-1034): site = wrap(body + 0)
+    double nsp[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < D; ++a) nsp[a] = wrap1(pt[a] + 0.0, g.L[a]);
+    psp = pack_site(nsp, ghost_flags<D>(g, nsp));
+    pent = stage_entry64<D>(g, c64, psp, pbody);
   };
 
   const uint32_t npass = cycles * NCOL;
@@ -196,10 +215,7 @@ __global__ void __launch_bounds__(kTileThreads, 2) k_apply_tile(Geometry g, Appl
     } else if (pgo) {
       // ---- the trial of this pass from LDS
       const uint32_t s = ps, self = pe0 + ppick, body = pbody;
-      double x[3] = {pbp.x, pbp.y, pbp.z};
-      for (int a = 0; a < D; ++a) x[a] += pv[a];
-      for (int a = 0; a < D; ++a) x[a] = wrap1(x[a], g.L[a]);  // generate_trial_moves: wrap(propose(..))
-      if (space_of<D>(g, origin, x) == s) {
+      if (pin) {
         constexpr int NR = D == 3 ? 9 : 3;
         uint32_t rlo[NR], rhi[NR];
 #pragma unroll
@@ -210,19 +226,12 @@ __global__ void __launch_bounds__(kTileThreads, 2) k_apply_tile(Geometry g, Appl
           rlo[r] = coff[row * CW + pcb[AL] - 1];
           rhi[r] = coff[row * CW + pcb[AL] + 2];
         }
-        double t[3] = {0.0, 0.0, 0.0};
-        for (int a = 0; a < D; ++a) t[a] = wrap1(x[a], g.L[a]);
+        const double* t = pt;
         bool hit = false;
         uint32_t rounds = 0;
 #pragma unroll
         for (int r = 0; r < NR; ++r) rounds = max(rounds, rhi[r] - rlo[r]);
-        float tr[3] = {0.0f, 0.0f, 0.0f};
-        {
-          const float4 te = stage_entry64<D>(g, c64, make_double4(t[0], t[1], D == 3 ? t[2] : 0.0, 0.0), 0u);
-          tr[0] = te.x;
-          tr[1] = te.y;
-          tr[2] = te.z;
-        }
+        const float* tr = ptr;
         const float lo = tc.band_lo, hi = tc.band_hi;
         // rows of three cells; only the centre row (the trial's own cell in
         // its middle) can hold the trial's own site.  Two rounds per step:
@@ -264,14 +273,10 @@ __global__ void __launch_bounds__(kTileThreads, 2) k_apply_tile(Geometry g, Appl
           }
         }
         if (!hit) {
-          // update_body_properties (microstate.rs:985-1034)
-          double nsp[3] = {0.0, 0.0, 0.0};
-          for (int a = 0; a < D; ++a) nsp[a] = wrap1(t[a] + 0.0, g.L[a]);
-          const uint32_t fl = ghost_flags<D>(g, nsp);
-          const double4 sp = pack_site(nsp, fl);
+          // update_body_properties (microstate.rs:985-1034), formed with the proposal
           B.bpos[body] = make_double4(t[0], t[1], t[2], stamp);
-          st_dev_d4(&B.spos[body], sp);
-          lb[self] = stage_entry64<D>(g, c64, sp, body);
+          st_dev_d4(&B.spos[body], psp);
+          lb[self] = pent;
           accepted = 1;
         }
       }
