@@ -43,7 +43,7 @@ def parse():
                     help="time the kernels of every k-th timed apply with HIP events")
     ap.add_argument("--placement-only", action="store_true",
                     help="only run the QuickCompress placement of 1M spheres and print its figures")
-    ap.add_argument("--only", choices=["c1", "c3", "c4", "c5", "placement"],
+    ap.add_argument("--only", choices=["c1", "c3", "c4", "c5", "placement", "gsd"],
                     help="only run this other configuration (with its CPU baseline unless --no-cpu-baseline)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (HBM counter calibration)")
@@ -390,6 +390,8 @@ def other_workloads(hmc, wl, cpu=True, only=None, fp64=True):
                                             f"with OpenMP over same-colour spaces")
             r["gpu_over_cpu"] = v / r["cpu_baseline"]["value"]
         res["C5_replicas_256"] = r
+    if want("gsd"):
+        res["gsd_frames_1M"] = gsd_frames(hmc, wl, cpu)
     if want("placement"):
         res["placement_quick_compress_1M"] = placement(hmc, wl)
         if cpu:
@@ -403,6 +405,49 @@ def other_workloads(hmc, wl, cpu=True, only=None, fp64=True):
             pl["cpu_baseline_sweeps"] = pc
             pl["sweeps_gpu_over_cpu"] = pl["overlap_penalty_sweep_trial_moves_per_s"] / pc["value"]
     return res
+
+
+def gsd_frames(hmc, wl, cpu=True):
+    """HOOMD GSD frames of the C2 system (N=1,000,188) written from the
+    device state (hmc_gsd_append_frame: sites packed to FP32 on the device,
+    copied back, appended with the index): frames/s and MB/s of file bytes,
+    beside the reference's path timed on the host (the FP64 sites converted
+    to FP32 with numpy and written with the same file layer)."""
+    import tempfile
+    L, pos = wl.fcc_spheres()
+    sim = hmc.GpuParallelSweep(L, wl.wrap(L, pos), precision="mixed")
+    sim.run(0, 42, 1.0, 0.08, 4)
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        f = hmc.GsdFile.create(f"{d}/gpu.gsd")
+        f.append_frame(sim, 0)
+        n = 10
+        t0 = time.perf_counter()
+        for k in range(n):
+            f.append_frame(sim, k + 1)
+        el = time.perf_counter() - t0
+        f.close()
+        frame_bytes = 12 * len(pos) + 8 + 24 + 1 + 4
+        out = {"bodies": len(pos), "frames": n, "ms_per_frame": el / n * 1e3, "frames_per_s": n / el,
+               "file_MB_per_s": frame_bytes * n / el / 1e6}
+        if cpu:
+            sites = sim.positions()[1]
+            g = hmc.GsdFile.create(f"{d}/cpu.gsd")
+            t0 = time.perf_counter()
+            for k in range(n):
+                g.write_chunk("configuration/step", np.array([k], np.uint64))
+                g.write_chunk("configuration/box", np.array([L[0], L[1], L[2], 0, 0, 0], np.float32))
+                g.write_chunk("configuration/dimensions", np.array([3], np.uint8))
+                g.write_chunk("particles/N", np.array([len(pos)], np.uint32))
+                g.write_chunk("particles/position", sites.astype(np.float32))
+                g.end_frame()
+            el_c = time.perf_counter() - t0
+            g.close()
+            out["cpu_baseline"] = dict(value=n / el_c, unit="frames/s", cores=1, kind="port",
+                                       sample=f"{n} frames of the host's FP64 sites converted with numpy and "
+                                              f"written by the same file layer ({el_c:.2f} s)")
+    sim.close()
+    return out
 
 
 def placement(hmc, wl):

@@ -34,7 +34,9 @@ HMC_MIXED = 1
 # every entry point declared in include/hmc.h
 EXPORTS = ["hmc_create", "hmc_set_state", "hmc_get_state", "hmc_clone_with_boundary", "hmc_apply_translate",
            "hmc_quick_compress_init", "hmc_quick_compress_apply", "hmc_quick_insert_init",
-           "hmc_quick_insert_apply", "hmc_delta_energy_insert",
+           "hmc_quick_insert_apply", "hmc_delta_energy_insert", "hmc_gsd_create", "hmc_gsd_open",
+           "hmc_gsd_write_chunk", "hmc_gsd_end_frame", "hmc_gsd_append_frame", "hmc_gsd_n_frames",
+           "hmc_gsd_read_chunk", "hmc_gsd_header", "hmc_gsd_close",
            "hmc_apply_rotate",
            "hmc_set_orientations", "hmc_get_orientations", "hmc_run_translate", "hmc_total_energy",
            "hmc_total_energy_noalloc", "hmc_run", "hmc_moved_since", "hmc_get_bodies", "hmc_debug_regions",
@@ -126,6 +128,17 @@ def lib():
         L.hmc_quick_insert_init.restype = None
         L.hmc_quick_insert_apply.argtypes = [C.POINTER(QuickInsertState), C.POINTER(vp), C.c_uint64,
                                              C.c_uint32, C.c_uint32, C.POINTER(Count)]
+        L.hmc_gsd_create.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint32, C.c_uint32, C.POINTER(vp)]
+        L.hmc_gsd_open.argtypes = [C.c_char_p, C.c_int, C.POINTER(vp)]
+        L.hmc_gsd_write_chunk.argtypes = [vp, C.c_char_p, C.c_uint8, C.c_uint64, C.c_uint32, vp]
+        L.hmc_gsd_end_frame.argtypes = [vp]
+        L.hmc_gsd_append_frame.argtypes = [vp, vp, C.c_uint64]
+        L.hmc_gsd_n_frames.argtypes = [vp]
+        L.hmc_gsd_n_frames.restype = C.c_uint64
+        L.hmc_gsd_read_chunk.argtypes = [vp, C.c_uint64, C.c_char_p, vp, C.c_uint64, C.POINTER(C.c_uint64),
+                                         C.POINTER(C.c_uint32), C.POINTER(C.c_uint8)]
+        L.hmc_gsd_header.argtypes = [vp, C.c_char_p, C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.hmc_gsd_close.argtypes = [vp]
         L.hmc_apply_translate.argtypes = [vp, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
                                           C.POINTER(Count), C.POINTER(C.c_uint32)]
         L.hmc_run_translate.argtypes = [vp, C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_uint32,
@@ -653,6 +666,91 @@ class QuickInsert:
     @property
     def ambiguous(self):
         return int(self.st.ambiguous)
+
+
+GSD_TYPES = {1: np.uint8, 2: np.uint16, 3: np.uint32, 4: np.uint64, 5: np.int8, 6: np.int16, 7: np.int32,
+             8: np.int64, 9: np.float32, 10: np.float64, 11: np.uint8}
+
+
+class GsdFile:
+    """A GSD 2.1 file (hoomd-gsd/src/file_layer.rs) through hmc_gsd_*:
+    GsdFile.create(path) makes a HOOMD trajectory (schema "hoomd" 1.4, as
+    HoomdGsdFile::create), GsdFile.open(path, writable) opens one.
+    append_frame(sim, step) writes a GpuParallelSweep's state as a HOOMD
+    frame; write_chunk / end_frame write any chunks; read(frame, name)
+    returns a chunk as an array of shape (rows, columns)."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    @classmethod
+    def create(cls, path, application="hoomd-rs_amd", schema="hoomd", schema_version=(1, 4)):
+        h = C.c_void_p()
+        rc = lib().hmc_gsd_create(str(path).encode(), application.encode(), schema.encode(),
+                                  schema_version[0], schema_version[1], C.byref(h))
+        if rc != HMC_OK:
+            raise HmcError(rc, f"cannot create {path}")
+        return cls(h)
+
+    @classmethod
+    def open(cls, path, writable=False):
+        h = C.c_void_p()
+        rc = lib().hmc_gsd_open(str(path).encode(), 1 if writable else 0, C.byref(h))
+        if rc != HMC_OK:
+            raise HmcError(rc, f"cannot open {path}")
+        return cls(h)
+
+    def write_chunk(self, name, array):
+        a = np.ascontiguousarray(array)
+        if a.ndim == 0:
+            a = a.reshape(1, 1)
+        elif a.ndim == 1:
+            a = a.reshape(-1, 1)
+        code = {v: k for k, v in GSD_TYPES.items() if k != 11}[a.dtype.type]
+        rc = lib().hmc_gsd_write_chunk(self.h, name.encode(), code, a.shape[0], a.shape[1], a.ctypes.data)
+        if rc != HMC_OK:
+            raise HmcError(rc, f"cannot write chunk {name}")
+
+    def end_frame(self):
+        rc = lib().hmc_gsd_end_frame(self.h)
+        if rc != HMC_OK:
+            raise HmcError(rc, "cannot end the frame")
+
+    def append_frame(self, sim, step):
+        rc = lib().hmc_gsd_append_frame(self.h, sim.h, step)
+        if rc != HMC_OK:
+            raise HmcError(rc, lib().hmc_last_error(sim.h).decode())
+
+    @property
+    def n_frames(self):
+        return int(lib().hmc_gsd_n_frames(self.h))
+
+    def header(self):
+        app = C.create_string_buffer(64)
+        sch = C.create_string_buffer(64)
+        ma, mi = C.c_uint32(), C.c_uint32()
+        lib().hmc_gsd_header(self.h, app, sch, C.byref(ma), C.byref(mi))
+        return app.value.decode(), sch.value.decode(), (ma.value, mi.value)
+
+    def read(self, frame, name):
+        rows, cols, typ = C.c_uint64(), C.c_uint32(), C.c_uint8()
+        rc = lib().hmc_gsd_read_chunk(self.h, frame, name.encode(), None, 0, C.byref(rows), C.byref(cols),
+                                      C.byref(typ))
+        if rc != HMC_OK:
+            raise KeyError(f"chunk {name} not in frame {frame}")
+        out = np.zeros((rows.value, cols.value), GSD_TYPES[typ.value])
+        rc = lib().hmc_gsd_read_chunk(self.h, frame, name.encode(), out.ctypes.data, out.nbytes, C.byref(rows),
+                                      C.byref(cols), C.byref(typ))
+        if rc != HMC_OK:
+            raise HmcError(rc, f"cannot read chunk {name}")
+        return out
+
+    def close(self):
+        if self.h:
+            rc = lib().hmc_gsd_close(self.h)
+            self.h = None
+            if rc != HMC_OK:
+                raise HmcError(rc, "closing the file failed")
 
 
 def scale_volume(box_edges, v):

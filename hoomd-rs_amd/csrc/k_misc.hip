@@ -82,6 +82,27 @@ __global__ void k_gather_bodies(Bodies B, const uint32_t* __restrict__ idx, uint
   }
 }
 
+// A HOOMD GSD frame's particle arrays from the sites (tag order == body
+// order): positions as FP32 triples (z = 0 in 2D; Rust's `as f32` rounds to
+// nearest like the conversion here) and, for oriented bodies, orientations as
+// FP32 (s, x, y, z) (hoomd-gsd/src/hoomd.rs particles_position and
+// particles_orientation).
+__global__ void k_pack_frame(Bodies B, uint32_t n, int dim, float* __restrict__ pos, float* __restrict__ quat) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 s = B.spos[i];
+  pos[3 * (size_t)i] = (float)s.x;
+  pos[3 * (size_t)i + 1] = (float)s.y;
+  pos[3 * (size_t)i + 2] = dim == 3 ? (float)s.z : 0.0f;
+  if (quat) {
+    const double4 q = B.squat[i];
+    quat[4 * (size_t)i] = (float)q.x;
+    quat[4 * (size_t)i + 1] = (float)q.y;
+    quat[4 * (size_t)i + 2] = (float)q.z;
+    quat[4 * (size_t)i + 3] = (float)q.w;
+  }
+}
+
 // ------------------------------------------------------------- debug hooks --
 __global__ void k_debug_math(int kind, const double* in, double* out, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;

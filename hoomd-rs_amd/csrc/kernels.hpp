@@ -59,6 +59,7 @@ __global__ void k_tally(ApplyState* st, const unsigned long long* acc_slots, uin
 __global__ void k_moved_mask(const double4* __restrict__ bpos, uint32_t n, uint32_t since, unsigned long long* mask,
                              unsigned long long* count);
 __global__ void k_gather_bodies(Bodies B, const uint32_t* __restrict__ idx, uint32_t n, double4* __restrict__ out);
+__global__ void k_pack_frame(Bodies B, uint32_t n, int dim, float* __restrict__ pos, float* __restrict__ quat);
 __global__ void k_debug_math(int kind, const double* in, double* out, uint64_t n);
 __global__ void k_debug_normals(uint64_t step, uint32_t substep, uint32_t seed, uint64_t index, double* out, uint64_t n);
 

@@ -389,6 +389,64 @@ int hmc_debug_normals(int device, uint64_t step, uint32_t substep, uint32_t seed
 const char* hmc_last_error(const hmc_ctx* ctx);
 void hmc_destroy(hmc_ctx* ctx);
 
+/* ---- GSD trajectory frames -------------------------------------------------
+ * The GSD 2.1 file layer (hoomd-gsd/src/file_layer.rs: a 256-byte header,
+ * an index of 32-byte entries sorted by (frame, id), a list of 64-byte chunk
+ * names, and the chunk data) and the HOOMD schema frames of
+ * hoomd-gsd/src/hoomd.rs.  Host code, except that hmc_gsd_append_frame packs
+ * a context's sites to FP32 on the device before copying them back.  Every
+ * frame is written through to the file (index and names included) when it
+ * ends. */
+typedef struct hmc_gsd hmc_gsd;
+enum {
+  HMC_GSD_U8 = 1, HMC_GSD_U16 = 2, HMC_GSD_U32 = 3, HMC_GSD_U64 = 4, HMC_GSD_I8 = 5, HMC_GSD_I16 = 6,
+  HMC_GSD_I32 = 7, HMC_GSD_I64 = 8, HMC_GSD_F32 = 9, HMC_GSD_F64 = 10, HMC_GSD_STRING = 11
+};
+
+/* GsdFile::create (file_layer.rs:1008-1030): a new empty file (truncating),
+ * with the application and schema names (< 64 bytes each) and the schema
+ * version (major, minor); schema "hoomd" (1, 4) is HoomdGsdFile::create
+ * (hoomd.rs:186-198).  HMC_ERR_INVALID for names too long or I/O errors. */
+int hmc_gsd_create(const char* path, const char* application, const char* schema, uint32_t schema_major,
+                   uint32_t schema_minor, hmc_gsd** out);
+
+/* GsdFile::open (file_layer.rs:966-980): an existing file, for reading
+ * (writable = 0) or for appending frames (writable = 1). */
+int hmc_gsd_open(const char* path, int writable, hmc_gsd** out);
+
+/* GsdFile::write_arrays / write_scalars (file_layer.rs): one chunk of `rows`
+ * x `columns` values of GSD type `type` in the current frame; a name may
+ * appear once per frame. */
+int hmc_gsd_write_chunk(hmc_gsd* f, const char* name, uint8_t type, uint64_t rows, uint32_t columns,
+                        const void* data);
+
+/* GsdFile::end_frame (file_layer.rs:2043-2053) followed by sync_all
+ * (file_layer.rs:2269-2330): the frame's data, new names and index entries
+ * are written to the file. */
+int hmc_gsd_end_frame(hmc_gsd* f);
+
+/* HoomdGsdFile::append_frame(step) with configuration/box, configuration/
+ * dimensions, particles/position (the sites in tag order as FP32, z = 0 in
+ * 2D) and, for oriented bodies, particles/orientation, then Frame::end
+ * (hoomd.rs:343-1013), from a context's current state. */
+int hmc_gsd_append_frame(hmc_gsd* f, hmc_ctx* ctx, uint64_t step);
+
+/* Frames in the file (GsdFile::n_frames). */
+uint64_t hmc_gsd_n_frames(const hmc_gsd* f);
+
+/* GsdFile::read of chunk `name` in `frame`: its rows, columns and type, and
+ * its bytes copied to `out` when capacity_bytes holds them (out may be null
+ * to query the shape).  HMC_ERR_INVALID when the chunk does not exist. */
+int hmc_gsd_read_chunk(const hmc_gsd* f, uint64_t frame, const char* name, void* out, uint64_t capacity_bytes,
+                       uint64_t* rows, uint32_t* columns, uint8_t* type);
+
+/* The header's application, schema and schema version. */
+int hmc_gsd_header(const hmc_gsd* f, char* application, char* schema, uint32_t* schema_major,
+                   uint32_t* schema_minor);
+
+/* Closes the file (a frame in progress is ended first, as Frame's Drop does). */
+int hmc_gsd_close(hmc_gsd* f);
+
 /* ---- replica batches ------------------------------------------------------
  * Many independent hard-sphere/disk systems (own box, checkerboard, counter,
  * seed and maximum distance each; same dim and precision) advanced together:
