@@ -283,17 +283,23 @@ def other_workloads(hmc, wl, cpu=True, only=None, fp64=True):
         q0 = np.tile([1.0, 0.0, 0.0, 0.0], (len(pos), 1))
         sim = hmc.GpuParallelSweep(L, wl.wrap(L, pos), model="convex", vertices=wl.UNIT_CUBE, orientations=q0)
         moves = [("translate", 0.05), ("rotate", 0.1)]
-        sim.run_moves(0, 42, 1.0, moves, 1)
+        # orientations decorrelated from the lattice before timing: 30 steps
+        # of translate 0.05 + rotate 0.5, then one step of the timed moves
+        sim.run_moves(0, 42, 1.0, [("translate", 0.05), ("rotate", 0.5)], 30)
+        sim.run_moves(30, 42, 1.0, moves, 1)
         t0 = time.perf_counter()
-        cnt = sim.run_moves(1, 42, 1.0, moves, 4)
+        cnt = sim.run_moves(31, 42, 1.0, moves, 4)
         t = sum(a + b for a, b in cnt)
         v = t / (time.perf_counter() - t0)
         # pass timing: one more step with every launch timed by HIP events
         sim.profile(True, 1)
-        t_prof = sum(a + b for a, b in sim.run_moves(5, 42, 1.0, moves, 1))
+        t_prof = sum(a + b for a, b in sim.run_moves(35, 42, 1.0, moves, 1))
         pm, pn, _bm, _bn = sim.read_profile()
+        bq_now, sq = sim.orientations()
+        bp_now = sim.positions()[0]
         r = {"value": v, "unit": "trial moves/s", "steps": 4, "precision": "fp64",
-             "moves": "translate 0.05 + rotate 0.1 per step",
+             "moves": "translate 0.05 + rotate 0.1 per step, after 30 steps of translate 0.05 + rotate 0.5",
+             "orientation_mean_abs_s": float(np.mean(np.abs(sq[:, 0]))),
              "acceptance": {"translate": cnt[0][0] / max(sum(cnt[0]), 1), "rotate": cnt[1][0] / max(sum(cnt[1]), 1)}}
         if pn:
             ms = pm / pn
@@ -308,13 +314,14 @@ def other_workloads(hmc, wl, cpu=True, only=None, fp64=True):
         if cpu:
             def c3_cpu():
                 O.set_threads(all_cores())
-                ref = O.ConvexOracle(L, wl.wrap(L, pos), q0, wl.UNIT_CUBE, seed=42)
+                # from the GPU's decorrelated state
+                ref = O.ConvexOracle(L, bp_now, bq_now, wl.UNIT_CUBE, seed=42)
                 ref.time_steps(0.05, 0.1, CPU_WARM_S)
                 moves_, el, steps = ref.time_steps(0.05, 0.1, CPU_TIMED_S)
                 return dict(value=moves_ / el, unit="trial moves/s", cores=all_cores(), kind="port",
                             sample=f"{steps} steps of ParallelSweep translate + rotate applies of the same 262,144 "
-                                   f"cubes ({el:.1f} s after {CPU_WARM_S:.0f} s warm-up), C++ restatement of "
-                                   f"hoomd-rs 1.4.0 with OpenMP over same-colour spaces")
+                                   f"cubes from the GPU's decorrelated state ({el:.1f} s after {CPU_WARM_S:.0f} s "
+                                   f"warm-up), C++ restatement of hoomd-rs 1.4.0 with OpenMP over same-colour spaces")
             r["cpu_baseline"] = c3_cpu()
             r["gpu_over_cpu"] = v / r["cpu_baseline"]["value"]
         res["C3_cubes_262144"] = r
@@ -571,6 +578,17 @@ def main():
                        "avg_pass_ms": f64["pass_ms"] / max(f64["pass_n"], 1),
                        "identical_decisions_and_positions": bool(same)}
         out["mixed_over_fp64"] = value / v64
+        # the layout's share and the precision's share of that ratio: mixed
+        # precision in the same per-colour layout as the FP64 run
+        if args.pass_layout == "persistent":
+            pc = run_gpu(hmc, L, body, site, "mixed", args.steps // 4, args.warmup, args.seed, kT, d,
+                         args.profile_period, persistent=False)
+            vpc = (pc["accepted"] + pc["rejected"]) / pc["elapsed"]
+            out["mixed_percolour"] = {"value": vpc, "ms_per_step": 1e3 * pc["elapsed"] / (args.steps // 4),
+                                      "avg_pass_ms": pc["pass_ms"] / max(pc["pass_n"], 1),
+                                      "steps": args.steps // 4}
+            out["mixed_over_fp64_same_layout"] = vpc / v64
+            out["persistent_over_percolour_mixed"] = value / vpc
     if not args.no_cpu_baseline:
         import _oracle as O
         mk = lambda: O.OracleSystem(L, pos, seed=args.seed)
