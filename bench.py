@@ -297,6 +297,7 @@ def other_workloads(hmc, wl, cpu=True, only=None, fp64=True):
         pm, pn, _bm, _bn = sim.read_profile()
         bq_now, sq = sim.orientations()
         bp_now = sim.positions()[0]
+        sim_state = sim.positions() + sim.orientations()
         r = {"value": v, "unit": "trial moves/s", "steps": 4, "precision": "fp64",
              "moves": "translate 0.05 + rotate 0.1 per step, after 30 steps of translate 0.05 + rotate 0.5",
              "orientation_mean_abs_s": float(np.mean(np.abs(sq[:, 0]))),
@@ -311,6 +312,26 @@ def other_workloads(hmc, wl, cpu=True, only=None, fp64=True):
                              "avg_launch_ms": ms, "algorithmic_flops_per_launch": fl, "launches": pn,
                              "flops_per_trial": fpt}
         sim.close()
+        if fp64:
+            # the mixed mode (FP32 separating axes, FP64 XenoCollide inside
+            # their margin) over the same steps: the same decisions, so the
+            # same final state bit for bit
+            sm = hmc.GpuParallelSweep(L, wl.wrap(L, pos), model="convex", vertices=wl.UNIT_CUBE, orientations=q0,
+                                      precision="mixed")
+            sm.run_moves(0, 42, 1.0, [("translate", 0.05), ("rotate", 0.5)], 30)
+            sm.run_moves(30, 42, 1.0, moves, 1)
+            b0 = sm.stats()["band_retests"]
+            t0 = time.perf_counter()
+            cm = sm.run_moves(31, 42, 1.0, moves, 4)
+            vm = sum(a + b for a, b in cm) / (time.perf_counter() - t0)
+            band = sm.stats()["band_retests"] - b0
+            sm.run_moves(35, 42, 1.0, moves, 1)
+            same = cm == cnt and all(np.array_equal(x.view(np.uint64), y.view(np.uint64))
+                                     for x, y in zip(sm.positions() + sm.orientations(),
+                                                     sim_state))
+            r["mixed"] = {"value": vm, "mixed_over_fp64": vm / v, "identical_to_fp64": bool(same),
+                          "fp64_pairs_per_trial": band / max(sum(a + b for a, b in cm), 1)}
+            sm.close()
         if cpu:
             def c3_cpu():
                 O.set_threads(all_cores())

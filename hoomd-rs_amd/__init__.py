@@ -41,7 +41,8 @@ EXPORTS = ["hmc_create", "hmc_set_state", "hmc_get_state", "hmc_clone_with_bound
            "hmc_set_orientations", "hmc_get_orientations", "hmc_run_translate", "hmc_total_energy",
            "hmc_total_energy_noalloc", "hmc_run", "hmc_moved_since", "hmc_get_bodies", "hmc_debug_regions",
            "hmc_trace_enable", "hmc_trace_read", "hmc_get_stats", "hmc_profile_enable", "hmc_profile_read",
-           "hmc_debug_math", "hmc_debug_normals", "hmc_debug_convex", "hmc_last_error", "hmc_destroy",
+           "hmc_debug_math", "hmc_debug_normals", "hmc_debug_convex", "hmc_debug_convex_axes", "hmc_last_error",
+           "hmc_destroy",
            "hmc_batch_create", "hmc_batch_set_state", "hmc_batch_get_state", "hmc_batch_run_translate",
            "hmc_batch_last_error", "hmc_batch_destroy", "hmc_sdf", "hmc_batch_sdf", "hmc_batch_get_stats", "hmc_batch_autotune"]
 
@@ -170,6 +171,7 @@ def lib():
         L.hmc_batch_destroy.argtypes = [vp]
         L.hmc_get_orientations.argtypes = [vp, D_P, D_P]
         L.hmc_debug_convex.argtypes = [C.c_int, C.c_int, D_P, C.c_uint32, D_P, D_P, C.c_uint64]
+        L.hmc_debug_convex_axes.argtypes = [D_P, C.c_uint32, C.POINTER(C.c_uint32), D_P, D_P]
         L.hmc_trace_enable.argtypes = [vp, C.c_int, C.c_uint64]
         L.hmc_trace_read.argtypes = [vp, vp, C.c_uint64, C.POINTER(C.c_uint64)]
         L.hmc_get_stats.argtypes = [vp, C.POINTER(Stats)]
@@ -475,9 +477,10 @@ def debug_normals(step, substep, seed, index, n, device=0):
 
 def debug_convex(kind, vertices, rows, device=0):
     """Device convex routines on rows of inputs: "collide" (v_ij[3], q_ij[4]),
-    "global" (r_i[3], q_i[4], r_j[3], q_j[4]) or "rotate" (step, substep,
-    seed, index, sigma)."""
-    k = {"collide": 0, "global": 1, "rotate": 2}[kind]
+    "global" (r_i[3], q_i[4], r_j[3], q_j[4]), "sat" (as "global"; the mixed
+    mode's FP32 class: 0 apart, 1 overlap, 2 left to FP64) or "rotate" (step,
+    substep, seed, index, sigma)."""
+    k = {"collide": 0, "global": 1, "rotate": 2, "sat": 3}[kind]
     v = np.ascontiguousarray(vertices, np.float64).reshape(-1, 3)
     x = np.ascontiguousarray(rows, np.float64)
     n = x.shape[0]
@@ -486,6 +489,23 @@ def debug_convex(kind, vertices, rows, device=0):
     if rc != HMC_OK:
         raise HmcError(rc, "hmc_debug_convex failed")
     return out
+
+
+def convex_axes(vertices):
+    """The mixed mode's separating-axis data of a polytope (host code): dict
+    of the counts (faces, edges, scanned vertices, sym, use, use_edges), the
+    face axes (unit normal, hmin, hmax) and the edge directions, as FP32
+    values."""
+    v = np.ascontiguousarray(vertices, np.float64).reshape(-1, 3)
+    cnt = (C.c_uint32 * 6)()
+    faces = np.zeros((16, 5))
+    edges = np.zeros((8, 3))
+    rc = lib().hmc_debug_convex_axes(_p(v), len(v), cnt, _p(faces), _p(edges))
+    if rc != HMC_OK:
+        raise HmcError(rc, "hmc_debug_convex_axes failed")
+    nf, ne, nh, sym, use, use_edges = (int(x) for x in cnt)
+    return dict(n_faces=nf, n_edges=ne, n_scanned=nh, sym=bool(sym), use=bool(use), use_edges=bool(use_edges),
+                faces=faces[:min(nf, 16)], edges=edges[:min(ne, 8)])
 
 
 class GpuBatch:

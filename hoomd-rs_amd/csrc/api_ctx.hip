@@ -102,9 +102,12 @@ int launch_apply(hmc_ctx* c, int from_args, int advance, uint64_t step, uint32_t
           k_pass_lj<D, HMC_FP64><<<lj_chunk * 8, kBlock, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, (uint32_t)col,
                                                                  k, lj_chunk);
       }
+      else if (convex && g.cv_mixed)
+        k_pass_convex<true><<<cv_chunk * 8, kBlock, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k,
+                                                            cv_chunk, c->sat);
       else if (convex)
-        k_pass_convex<<<cv_chunk * 8, kBlock, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k,
-                                                      cv_chunk);
+        k_pass_convex<false><<<cv_chunk * 8, kBlock, 0, s>>>(g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k,
+                                                             cv_chunk, c->sat);
       else if (c->tiled && PREC == HMC_MIXED && g.lj_fp32_error)  // the FP32 error report
         k_pass_tile<D, PREC, true><<<c->tc.xcd_chunk * 8, tile_threads, c->tc.lds_bytes, s>>>(
             g, c->st, c->bodies, c->cells, c->w, (uint32_t)col, k, c->tc);
@@ -459,6 +462,125 @@ bool choose_regions_lj(const Geometry& g, int D, double occ, int cus, TileCfg* o
   return found;
 }
 
+// The FP32 separating-axis data of a polytope for the mixed-precision convex
+// pass: the face normals and edge directions of its convex hull, found by
+// brute force over vertex triples (at most 32 vertices) and kept once per
+// +-pair, the slab [hmin, hmax] of every face axis, and the scanned vertices
+// (one of each antipodal pair when the vertex set is closed under negation).
+// A flat or degenerate vertex set, or one with more face axes than
+// kSatFaces, gets use = 0: all its pairs go to XenoCollide in FP64.
+static void sat_build(const double* vt, uint32_t nv, SatShape* S) {
+  std::memset(S, 0, sizeof *S);
+  using d3 = std::array<double, 3>;
+  auto V = [&](uint32_t i) { return d3{vt[3 * i], vt[3 * i + 1], vt[3 * i + 2]}; };
+  auto sub = [](const d3& a, const d3& b) { return d3{a[0] - b[0], a[1] - b[1], a[2] - b[2]}; };
+  auto dot = [](const d3& a, const d3& b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+  auto cross = [](const d3& a, const d3& b) {
+    return d3{a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+  };
+  double R2 = 0.0;
+  for (uint32_t i = 0; i < nv; ++i) R2 = std::max(R2, dot(V(i), V(i)));
+  const double R = std::sqrt(R2);
+  if (!(R > 0.0)) return;
+  const double eps = 1e-12 * R;
+  std::vector<d3> pn;       // oriented face planes: outward unit normal
+  std::vector<double> ph;   // and offset
+  for (uint32_t i = 0; i < nv; ++i)
+    for (uint32_t j = i + 1; j < nv; ++j)
+      for (uint32_t k = j + 1; k < nv; ++k) {
+        d3 n = cross(sub(V(j), V(i)), sub(V(k), V(i)));
+        const double nn = std::sqrt(dot(n, n));
+        if (!(nn > 1e-12 * R2)) continue;  // (nearly) collinear
+        for (double& x : n) x /= nn;
+        double h = dot(n, V(i));
+        bool pos = false, neg = false;
+        for (uint32_t m = 0; m < nv; ++m) {
+          const double d = dot(n, V(m)) - h;
+          pos |= d > eps;
+          neg |= d < -eps;
+        }
+        if (pos && neg) continue;
+        if (pos) {
+          for (double& x : n) x = -x;
+          h = -h;
+        }
+        bool seen = false;
+        for (size_t q = 0; q < pn.size() && !seen; ++q) seen = dot(pn[q], n) > 1.0 - 1e-12 && std::fabs(ph[q] - h) <= eps;
+        if (!seen) {
+          pn.push_back(n);
+          ph.push_back(h);
+        }
+      }
+  if (pn.size() < 4) return;  // flat or degenerate
+  auto unique_axis = [&](std::vector<d3>& set, const d3& a) {
+    for (const d3& b : set)
+      if (std::fabs(dot(a, b)) > 1.0 - 1e-12) return;
+    set.push_back(a);
+  };
+  std::vector<d3> faces, edges;
+  for (const d3& n : pn) unique_axis(faces, n);
+  for (size_t q = 0; q < pn.size(); ++q) {
+    std::vector<uint32_t> on;
+    for (uint32_t m = 0; m < nv; ++m)
+      if (std::fabs(dot(pn[q], V(m)) - ph[q]) <= eps) on.push_back(m);
+    for (size_t a = 0; a < on.size(); ++a)
+      for (size_t b = a + 1; b < on.size(); ++b) {
+        d3 e = sub(V(on[b]), V(on[a]));
+        const double el = std::sqrt(dot(e, e));
+        if (!(el > eps)) continue;
+        const d3 m = cross(pn[q], e);
+        bool pos = false, neg = false;
+        for (size_t r = 0; r < on.size(); ++r) {
+          const double d = dot(m, sub(V(on[r]), V(on[a])));
+          pos |= d > eps * el;
+          neg |= d < -eps * el;
+        }
+        if (pos && neg) continue;  // a diagonal of the face
+        for (double& x : e) x /= el;
+        unique_axis(edges, e);
+      }
+  }
+  S->nf = (uint32_t)faces.size();
+  S->ne = (uint32_t)edges.size();
+  if (S->nf > kSatFaces) return;
+  for (uint32_t f = 0; f < S->nf; ++f) {
+    double lo = INFINITY, hi = -INFINITY;
+    for (uint32_t m = 0; m < nv; ++m) {
+      lo = std::min(lo, dot(faces[f], V(m)));
+      hi = std::max(hi, dot(faces[f], V(m)));
+    }
+    for (int a = 0; a < 3; ++a) S->face[f][a] = (float)faces[f][a];
+    S->face[f][3] = (float)hi;
+    S->fmin[f] = (float)lo;
+  }
+  if (S->ne <= kSatEdges)
+    for (uint32_t e = 0; e < S->ne; ++e)
+      for (int a = 0; a < 3; ++a) S->edge[e][a] = (float)edges[e][a];
+  // the scanned vertices
+  bool sym = true;
+  for (uint32_t i = 0; i < nv && sym; ++i) {
+    bool found = false;
+    for (uint32_t j = 0; j < nv && !found; ++j)
+      found = V(j)[0] == -V(i)[0] && V(j)[1] == -V(i)[1] && V(j)[2] == -V(i)[2];
+    sym = found;
+  }
+  std::vector<bool> covered(nv, false);
+  uint32_t nh = 0;
+  for (uint32_t i = 0; i < nv; ++i) {
+    if (sym && covered[i]) continue;
+    for (int a = 0; a < 3; ++a) S->hv[nh][a] = (float)vt[3 * i + a];
+    ++nh;
+    if (sym)
+      for (uint32_t j = 0; j < nv; ++j)
+        if (V(j)[0] == -V(i)[0] && V(j)[1] == -V(i)[1] && V(j)[2] == -V(i)[2]) covered[j] = true;
+  }
+  S->nh = nh;
+  S->sym = sym ? 1u : 0u;
+  S->tol = (float)std::ldexp(R, -12);
+  S->use = 1;
+  S->use_edges = S->ne >= 1 && S->ne <= kSatEdges ? 1u : 0u;
+}
+
 // Geometry of one system: validation, checkerboard dimensions, model
 // constants and the FP32 contact band.  Shared by a context and by every
 // replica of a batch.
@@ -621,6 +743,10 @@ int hmc_create(const hmc_config* cfg_in, hmc_ctx** out) {
   const bool lj_mixed = (cfg_in->model == HMC_LJ || cfg_in->model == HMC_LJ_SHIFTED) &&
                         cfg_in->precision == HMC_MIXED;
   if (lj_mixed) cfg_local.precision = HMC_FP64;
+  // so do convex polyhedra: their listed pairs go through FP32 separating
+  // axes first (Geometry::cv_mixed)
+  const bool cv_mixed = cfg_in->model == HMC_CONVEX_POLYHEDRON && cfg_in->precision == HMC_MIXED;
+  if (cv_mixed) cfg_local.precision = HMC_FP64;
   const hmc_config* cfg = &cfg_local;
   Geometry g0;
   bool fast0 = false;
@@ -641,6 +767,8 @@ int hmc_create(const hmc_config* cfg_in, hmc_ctx** out) {
   c->device = cfg->device;
   c->g = g0;
   c->g.lj_mixed = lj_mixed ? 1 : 0;
+  c->g.cv_mixed = cv_mixed ? 1 : 0;
+  if (cv_mixed) sat_build(cfg_in->vertices, cfg_in->n_vertices, &c->sat);
   c->g.lj_fp32_error = (cfg_in->flags & HMC_FLAG_FP32_ERROR) && cfg_in->precision == HMC_MIXED ? 1 : 0;
   c->fast = fast0;
   Geometry& g = c->g;
@@ -923,10 +1051,12 @@ int hmc_get_orientations(hmc_ctx* c, double* body_quat, double* site_quat) {
 int hmc_debug_convex(int device, int kind, const double* vertices, uint32_t n_vertices, const double* in,
                      double* out, uint64_t n) {
   if (!vertices || n_vertices < 1 || n_vertices > (uint32_t)kMaxVertices || (n && (!in || !out)) || kind < 0 ||
-      kind > 2)
+      kind > 3)
     return HMC_ERR_INVALID;
   if (hipSetDevice(device) != hipSuccess) return HMC_ERR_DEVICE;
-  const size_t w_in = kind == 0 ? 7 : kind == 1 ? 14 : 5, w_out = kind == 2 ? 4 : 1;
+  const size_t w_in = kind == 0 ? 7 : (kind == 1 || kind == 3) ? 14 : 5, w_out = kind == 2 ? 4 : 1;
+  SatShape sat;
+  sat_build(vertices, n_vertices, &sat);
   double m2 = 0.0;
   for (uint32_t i = 0; i < n_vertices; ++i) {
     const double* v = vertices + 3 * i;
@@ -944,7 +1074,7 @@ int hmc_debug_convex(int device, int kind, const double* vertices, uint32_t n_ve
         hipMemcpy(di, in, n * w_in * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
       rc = HMC_ERR_DEVICE;
     if (!rc) {
-      k_debug_convex<<<grid_for(n, 64), 64>>>(kind, dv, n_vertices, (R + R) * (R + R), di, dout, n);
+      k_debug_convex<<<grid_for(n, 64), 64>>>(kind, dv, n_vertices, (R + R) * (R + R), di, dout, n, sat);
       if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
           hipMemcpy(out, dout, n * w_out * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
         rc = HMC_ERR_DEVICE;
@@ -954,6 +1084,25 @@ int hmc_debug_convex(int device, int kind, const double* vertices, uint32_t n_ve
   if (di) (void)hipFree(di);
   if (dout) (void)hipFree(dout);
   return rc;
+}
+
+int hmc_debug_convex_axes(const double* vertices, uint32_t n_vertices, uint32_t* counts, double* faces,
+                          double* edges) {
+  if (!vertices || n_vertices < 1 || n_vertices > (uint32_t)kMaxVertices || !counts) return HMC_ERR_INVALID;
+  SatShape S;
+  sat_build(vertices, n_vertices, &S);
+  const uint32_t v[7] = {S.nf, S.ne, S.nh, S.sym, S.use, S.use_edges, 0};
+  for (int i = 0; i < 6; ++i) counts[i] = v[i];
+  if (faces)
+    for (uint32_t f = 0; f < std::min(S.nf, kSatFaces); ++f) {
+      for (int a = 0; a < 3; ++a) faces[5 * f + a] = S.face[f][a];
+      faces[5 * f + 3] = S.fmin[f];
+      faces[5 * f + 4] = S.face[f][3];
+    }
+  if (edges)
+    for (uint32_t e = 0; e < std::min(S.ne, kSatEdges); ++e)
+      for (int a = 0; a < 3; ++a) edges[3 * e + a] = S.edge[e][a];
+  return HMC_OK;
 }
 
 int hmc_clone_with_boundary(hmc_ctx* c, const double* new_edges, hmc_ctx** out) {

@@ -1,6 +1,7 @@
 // Host-side state of a context and of a batch, and the helpers the C ABI's
 // translation units (api_*.hip) share.
 #pragma once
+#include <array>
 #include <mutex>
 
 #include "kernels.hpp"
@@ -27,6 +28,7 @@ struct hmc_ctx {
   TileCfg pc{};
   uint32_t pgrid = 0, pthreads = 0, pper_cu = 0;
   double* verts = nullptr;  // convex polyhedron vertices (device)
+  SatShape sat{};           // convex, mixed precision: the FP32 separating axes (else use = 0)
   Geometry* gdev = nullptr;  // device copy of g (g.self)
   uint32_t n_scan_blocks = 0;
   uint64_t applies = 0;

@@ -98,6 +98,177 @@ __device__ void trial_convex(const Geometry& g, const ApplyState* st, const Bodi
   }
 }
 
+// ------------------------------------------------ FP32 separating axes --
+// Mixed precision for hard convex polyhedra.  Shape B (the neighbour) is
+// placed in the frame of shape A (the trial) as XenoCollide places it
+// (convex.rs:154-170): centre v = conj(q_i)(r_j - r_i), rotation R =
+// rotmat(conj(q_i) q_j), both formed in FP32.  Every FP32 quantity then lies
+// within 500 u R of its exact value (u = 2^-24, R the circumsphere radius,
+// |v| < 2R for a listed pair, unit axes), and every decision below keeps a
+// margin tol = 2^-12 R = 4096 u R:
+// - apart: the extents of A and B along some axis w are separated by more
+//   than tol (any such w is a separating direction, so A and B are disjoint by
+//   more than XenoCollide's 2e-12 tolerance and its FP64 rounding);
+// - overlap: a vertex of one shape lies inside every face slab of the other
+//   by more than tol, or every separating-axis candidate (the face normals of
+//   A and B and the cross products of their edge directions, which include
+//   every facet normal of the Minkowski difference) shows an overlap depth
+//   above tol; the origin is then inside the Minkowski difference by a margin
+//   that no FP64 rounding in XenoCollide reaches, so XenoCollide reports an
+//   overlap;
+// - anything else (near contact, near-parallel edges) is left to FP64
+//   XenoCollide.
+// So every decision equals the FP64 one.
+
+// Extent [*lo, *hi] of the polytope along w (in its own frame); with IN,
+// bit k of the result is set where off + w.v_k lies in (a, b), bit 32 + k
+// where off - w.v_k does (the antipode of a scanned vertex of a symmetric
+// polytope).
+template <int NH, bool SYM, bool IN>
+__device__ __forceinline__ uint64_t sat_scan(const SatShape& S, float wx, float wy, float wz, float off, float a,
+                                             float b, float* lo, float* hi) {
+  float mx = -INFINITY, mn = INFINITY;
+  uint64_t keep = 0;
+  if constexpr (NH > 0) {
+#pragma unroll
+    for (int k = 0; k < NH; ++k) {
+      const float x = fmaf(S.hv[k][2], wz, fmaf(S.hv[k][1], wy, S.hv[k][0] * wx));
+      mx = fmaxf(mx, x);
+      mn = fminf(mn, x);
+      if (IN) {
+        const float p = off + x, q = off - x;
+        keep |= (uint64_t)(p > a && p < b) << k;
+        if (SYM) keep |= (uint64_t)(q > a && q < b) << (32 + k);
+      }
+    }
+  } else {
+    for (uint32_t k = 0; k < S.nh; ++k) {
+      const float x = fmaf(S.hv[k][2], wz, fmaf(S.hv[k][1], wy, S.hv[k][0] * wx));
+      mx = fmaxf(mx, x);
+      mn = fminf(mn, x);
+      if (IN) {
+        const float p = off + x, q = off - x;
+        keep |= (uint64_t)(p > a && p < b) << k;
+        keep |= (uint64_t)(q > a && q < b) << (32 + k);
+      }
+    }
+  }
+  const bool sym = NH > 0 ? SYM : S.sym != 0;
+  if (sym) {
+    mx = fmaxf(mx, -mn);
+    mn = -mx;
+  }
+  *lo = mn;
+  *hi = mx;
+  return keep;
+}
+
+// 0: apart, 1: overlap, 2: undecided (XenoCollide in FP64).  Lanes with act
+// false take part in the wave-uniform stage skips only.  NH, NF, NE > 0: the
+// counts of S fixed at compile time (0: read from S).
+template <int NH, int NF, int NE, bool SYM>
+__device__ __forceinline__ uint32_t sat_classify(const SatShape& S, bool act, const float* v, const float* R) {
+  const uint32_t nf = NF > 0 ? (uint32_t)NF : S.nf, ne = NE > 0 ? (uint32_t)NE : S.ne;
+  const bool sym = NH > 0 ? SYM : S.sym != 0;
+  const uint32_t nh = NH > 0 ? (uint32_t)NH : S.nh;
+  const float tol = S.tol;
+  bool sep = false, deep = true;
+  uint64_t bin = ~0ull, ain = ~0ull;  // B's vertices inside A, A's vertices inside B
+  for (uint32_t f = 0; f < nf; ++f) {
+    const float nx = S.face[f][0], ny = S.face[f][1], nz = S.face[f][2];
+    const float hmax = S.face[f][3], hmin = S.fmin[f];
+    // A's face axis n: B's extent along it from R^T n
+    {
+      const float s = fmaf(nz, v[2], fmaf(ny, v[1], nx * v[0]));
+      const float bx = fmaf(R[6], nz, fmaf(R[3], ny, R[0] * nx));
+      const float by = fmaf(R[7], nz, fmaf(R[4], ny, R[1] * nx));
+      const float bz = fmaf(R[8], nz, fmaf(R[5], ny, R[2] * nx));
+      float lo, hi;
+      bin &= sat_scan<NH, SYM, true>(S, bx, by, bz, s, hmin + tol, hmax - tol, &lo, &hi);
+      const float gap = fmaxf((s + lo) - hmax, hmin - (s + hi));
+      sep |= gap > tol;
+      deep &= -gap > tol;
+    }
+    // B's face axis w = R n: A's extent along it
+    {
+      const float wx = fmaf(R[2], nz, fmaf(R[1], ny, R[0] * nx));
+      const float wy = fmaf(R[5], nz, fmaf(R[4], ny, R[3] * nx));
+      const float wz = fmaf(R[8], nz, fmaf(R[7], ny, R[6] * nx));
+      const float s = fmaf(wz, v[2], fmaf(wy, v[1], wx * v[0]));
+      float lo, hi;
+      ain &= sat_scan<NH, SYM, true>(S, wx, wy, wz, 0.0f, s + hmin + tol, s + hmax - tol, &lo, &hi);
+      const float gap = fmaxf((s + hmin) - hi, lo - (s + hmax));
+      sep |= gap > tol;
+      deep &= -gap > tol;
+    }
+  }
+  const uint64_t valid = (nh >= 32 ? 0xffffffffull : ((1ull << nh) - 1ull)) * (sym ? 0x100000001ull : 1ull);
+  uint32_t r = sep ? 0u : (((bin | ain) & valid) ? 1u : 2u);
+  if (nf == 0) r = 2u;
+  if (!S.use_edges || __ballot(act && r == 2u) == 0ull) return r;
+  // the edge-pair axes w = e_a x R e_b; B's extent along it from
+  // R^T w = (R^T e_a) x e_b
+  for (uint32_t ia = 0; ia < ne; ++ia) {
+    const float ax = S.edge[ia][0], ay = S.edge[ia][1], az = S.edge[ia][2];
+    const float tx = fmaf(R[6], az, fmaf(R[3], ay, R[0] * ax));
+    const float ty = fmaf(R[7], az, fmaf(R[4], ay, R[1] * ax));
+    const float tz = fmaf(R[8], az, fmaf(R[5], ay, R[2] * ax));
+    for (uint32_t ib = 0; ib < ne; ++ib) {
+      const float ex = S.edge[ib][0], ey = S.edge[ib][1], ez = S.edge[ib][2];
+      const float px = fmaf(R[2], ez, fmaf(R[1], ey, R[0] * ex));
+      const float py = fmaf(R[5], ez, fmaf(R[4], ey, R[3] * ex));
+      const float pz = fmaf(R[8], ez, fmaf(R[7], ey, R[6] * ex));
+      const float wx = ay * pz - az * py, wy = az * px - ax * pz, wz = ax * py - ay * px;
+      const float ux = ty * ez - tz * ey, uy = tz * ex - tx * ez, uz = tx * ey - ty * ex;
+      const float s = fmaf(wz, v[2], fmaf(wy, v[1], wx * v[0]));
+      float alo, ahi, blo, bhi;
+      (void)sat_scan<NH, SYM, false>(S, wx, wy, wz, 0.0f, 0.0f, 0.0f, &alo, &ahi);
+      (void)sat_scan<NH, SYM, false>(S, ux, uy, uz, 0.0f, 0.0f, 0.0f, &blo, &bhi);
+      const float gap = fmaxf((s + blo) - ahi, alo - (s + bhi));
+      sep |= gap > tol;
+      deep &= -gap > tol;
+    }
+  }
+  if (r == 2u) r = sep ? 0u : (deep ? 1u : 2u);
+  return r;
+}
+
+// B's centre and rotation in A's frame, in FP32, from the FP64 site data.
+__device__ __forceinline__ void sat_frame(const double* ti, const double* qi, const v3& rj, const double4& qj,
+                                          float* v, float* R) {
+  const float dx = (float)(rj.x - ti[0]), dy = (float)(rj.y - ti[1]), dz = (float)(rj.z - ti[2]);
+  const float a = (float)qi[0], b = (float)qi[1], c = (float)qi[2], d = (float)qi[3];
+  // rows of rotmat(q_i) (quaternion.rs:665-690); v = rotmat(q_i)^T (r_j - r_i)
+  const float m0 = a * a + b * b - c * c - d * d, m1 = 2.0f * (b * c - a * d), m2 = 2.0f * (b * d + a * c);
+  const float m3 = 2.0f * (b * c + a * d), m4 = a * a - b * b + c * c - d * d, m5 = 2.0f * (c * d - a * b);
+  const float m6 = 2.0f * (b * d - a * c), m7 = 2.0f * (c * d + a * b), m8 = a * a - b * b - c * c + d * d;
+  v[0] = fmaf(m6, dz, fmaf(m3, dy, m0 * dx));
+  v[1] = fmaf(m7, dz, fmaf(m4, dy, m1 * dx));
+  v[2] = fmaf(m8, dz, fmaf(m5, dy, m2 * dx));
+  // o = conj(q_i) q_j (quaternion.rs:405-418 with the conjugate)
+  const float js = (float)qj.x, jx = (float)qj.y, jy = (float)qj.z, jz = (float)qj.w;
+  const float os = a * js + b * jx + c * jy + d * jz;
+  const float ox = a * jx - js * b + (-c * jz + d * jy);
+  const float oy = a * jy - js * c + (-d * jx + b * jz);
+  const float oz = a * jz - js * d + (-b * jy + c * jx);
+  R[0] = os * os + ox * ox - oy * oy - oz * oz;
+  R[1] = 2.0f * (ox * oy - os * oz);
+  R[2] = 2.0f * (ox * oz + os * oy);
+  R[3] = 2.0f * (ox * oy + os * oz);
+  R[4] = os * os - ox * ox + oy * oy - oz * oz;
+  R[5] = 2.0f * (oy * oz - os * ox);
+  R[6] = 2.0f * (ox * oz - os * oy);
+  R[7] = 2.0f * (oy * oz + os * ox);
+  R[8] = os * os - ox * ox - oy * oy + oz * oz;
+}
+
+__device__ __forceinline__ uint32_t sat_dispatch(const SatShape& S, bool act, const float* v, const float* R) {
+  // unit cubes and other centrally symmetric hexahedra: 4 scanned vertices,
+  // 3 face axes, 3 edge directions
+  if (S.sym && S.nh == 4 && S.nf == 3 && S.ne == 3) return sat_classify<4, 3, 3, true>(S, act, v, R);
+  return sat_classify<0, 0, 0, false>(S, act, v, R);
+}
+
 // Convex pass layout: kCvTrials trials per wavefront, one per group of
 // kCvLanes lanes (the group draws its trial's proposal and lists its
 // neighbour spaces); the candidate sites of all the wavefront's trials are
@@ -151,13 +322,59 @@ __device__ __forceinline__ uint32_t cv_pair_tests(const Geometry& g, const Cells
   return hits;
 }
 
+// Mixed precision: the FP32 separating-axis tests over the np listed pairs.
+// Certain overlaps set their trial's bit in *hits; the undecided pairs are
+// moved to the front of the list (in order) for XenoCollide in FP64, and
+// their number is returned.
+__device__ __forceinline__ uint32_t cv_sat_filter(const Geometry& g, const Cells& C, CvWave& W, const SatShape& S,
+                                                  uint32_t np, uint32_t* hits) {
+  const uint32_t lane = __lane_id();
+  uint32_t nb = 0;
+  for (uint32_t p0 = 0; p0 < np; p0 += 64) {
+    const uint32_t p = p0 + lane;
+    bool act = false;
+    uint32_t kp = 0, qs = 0;
+    float v[3] = {0.0f, 0.0f, 0.0f}, R[9] = {1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+    if (p < np) {
+      kp = W.tk[p];
+      if (!((*hits >> kp) & 1u)) {
+        act = true;
+        qs = W.qs[p];
+        const double4 sj = C.pos64[qs];
+        const double4 qj4 = C.quat64[qs];
+        const double tq[3] = {W.t[kp][0], W.t[kp][1], W.t[kp][2]};
+        v3 rj;
+        (void)convex_image(g, tq, sj, &rj);  // the image found when the pair was listed
+        sat_frame(tq, W.qt[kp], rj, qj4, v, R);
+      }
+    }
+    const uint32_t r = sat_dispatch(S, act, v, R);
+#pragma unroll
+    for (uint32_t i = 0; i < kCvTrials; ++i)
+      if (__ballot(act && r == 1u && kp == i)) *hits |= 1u << i;
+    const bool band = act && r == 2u;
+    const unsigned long long mk = __ballot(band);
+    if (band) {
+      const uint32_t o = nb + (uint32_t)__popcll(mk & ((1ull << lane) - 1ull));
+      W.qs[o] = qs;  // o <= p: entries not yet read are never overwritten
+      W.tk[o] = kp;
+    }
+    nb += (uint32_t)__popcll(mk);
+  }
+  return nb;
+}
+
 // One colour of one cycle for convex polyhedra: kCvTrials spaces per
 // wavefront.  Each trial is the one of trial_convex (parallel_sweep.rs:234-277
 // with Translate or Rotate<Versor>).
 // (three wavefronts per SIMD: 22% faster on C3 than the two its free
 // register use allows)
+// MIXED: the listed pairs go through the FP32 separating-axis tests first
+// (cv_sat_filter); only the undecided ones run XenoCollide in FP64.
+template <bool MIXED>
 __global__ void __launch_bounds__(kBlock, 3) k_pass_convex(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
-                                                        uint32_t colour, uint32_t cycle, uint32_t xcd_chunk) {
+                                                        uint32_t colour, uint32_t cycle, uint32_t xcd_chunk,
+                                                        SatShape sat) {
   if (cycle >= st->cycles) return;  // uniform over the grid
   __shared__ unsigned blk_acc;
   __shared__ double zx[257], zf[257];
@@ -281,6 +498,7 @@ __global__ void __launch_bounds__(kBlock, 3) k_pass_convex(Geometry g, ApplyStat
   wave_lds_order();
   uint32_t hits = 0;  // trials with an overlap (uniform over the wavefront)
   uint32_t np = 0;    // listed pairs (uniform)
+  uint32_t nband = 0; // MIXED: pairs left to FP64 XenoCollide (uniform)
   for (uint32_t c0 = 0; c0 < total; c0 += 64 * kCvBatch) {
     uint32_t qsv[kCvBatch], kcv[kCvBatch];
     bool ok[kCvBatch];
@@ -334,7 +552,16 @@ __global__ void __launch_bounds__(kBlock, 3) k_pass_convex(Geometry g, ApplyStat
     }
     if (np + 64 * kCvBatch > kCvPairs || c0 + 64 * kCvBatch >= total) {  // uniform
       wave_lds_order();
-      if (g.nverts == 8)
+      if (MIXED && sat.use) {
+        np = cv_sat_filter(g, C, W, sat, np, &hits);
+        nband += np;
+        wave_lds_order();
+      }
+      // (the pairs left by the FP32 tests run the inlined XenoCollide too: a
+      // pass lasts as long as its slowest wavefront, and the out-of-line one
+      // made those wavefronts slower)
+      if (np == 0) {
+      } else if (g.nverts == 8)
         hits = cv_pair_tests<8>(g, C, W, vx, np, hits);
       else if (g.nverts == 6)
         hits = cv_pair_tests<6>(g, C, W, vx, np, hits);
@@ -346,6 +573,7 @@ __global__ void __launch_bounds__(kBlock, 3) k_pass_convex(Geometry g, ApplyStat
       wave_lds_order();
     }
   }
+  if (MIXED && lane == 0 && nband) atomicAdd(&st->band_retests, (unsigned long long)nband);
   const bool accept = active && !((hits >> k) & 1u);  // hard shapes: dH is 0 or +inf
   if (kl == 0 && tried) {
     if (accept) {
@@ -428,8 +656,21 @@ __global__ void k_init_site_quats(Geometry g, Bodies B, int have_sites) {
 // test (kind 1, in = r_i, q_i, r_j, q_j: 14 values) or the rotation sampler
 // (kind 2, in = step, substep, seed, index, sigma: out 4 values).
 __global__ void k_debug_convex(int kind, const double* vx, uint32_t nv, double rcut2, const double* in,
-                               double* out, uint64_t n) {
+                               double* out, uint64_t n, SatShape sat) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (kind == 3) {
+    // the FP32 separating-axis class of a pair (in as for kind 1): 0 apart,
+    // 1 overlap, 2 undecided; every lane of the wavefront takes part
+    const bool act = i < n;
+    float v[3] = {0.0f, 0.0f, 0.0f}, R[9] = {1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+    if (act) {
+      const double* p = in + 14 * i;
+      sat_frame(p, p + 3, v3{p[7], p[8], p[9]}, make_double4(p[10], p[11], p[12], p[13]), v, R);
+    }
+    const uint32_t r = sat.use ? sat_dispatch(sat, act, v, R) : 2u;
+    if (act) out[i] = (double)r;
+    return;
+  }
   if (i >= n) return;
   if (kind == 0) {
     const double* p = in + 7 * i;
@@ -456,5 +697,9 @@ __global__ void k_debug_convex(int kind, const double* vx, uint32_t nv, double r
 }
 
 // explicit instantiations (launched from the host translation units)
+template __global__ void k_pass_convex<false>(Geometry, ApplyState*, Bodies, Cells, Work, uint32_t, uint32_t, uint32_t,
+                                              SatShape);
+template __global__ void k_pass_convex<true>(Geometry, ApplyState*, Bodies, Cells, Work, uint32_t, uint32_t, uint32_t,
+                                             SatShape);
 
 }  // namespace hmc

@@ -30,10 +30,11 @@ template <int D, int PREC>
 __global__ void k_pass_lj(Geometry g, ApplyState* st, Bodies B, Cells C, Work w, uint32_t colour, uint32_t cycle, uint32_t xcd_chunk);
 template <int D>
 __global__ void k_guard_lj(Geometry g, ApplyState* st, Bodies B, Cells C, Work w, uint32_t first_cycle);
-__global__ void k_pass_convex(Geometry g, ApplyState* st, Bodies B, Cells C, Work w, uint32_t colour, uint32_t cycle, uint32_t xcd_chunk);
+template <bool MIXED>
+__global__ void k_pass_convex(Geometry g, ApplyState* st, Bodies B, Cells C, Work w, uint32_t colour, uint32_t cycle, uint32_t xcd_chunk, SatShape sat);
 __global__ void k_guard_convex(Geometry g, ApplyState* st, Bodies B, Cells C, Work w, uint32_t first_cycle);
 __global__ void k_init_site_quats(Geometry g, Bodies B, int have_sites);
-__global__ void k_debug_convex(int kind, const double* vx, uint32_t nv, double rcut2, const double* in, double* out, uint64_t n);
+__global__ void k_debug_convex(int kind, const double* vx, uint32_t nv, double rcut2, const double* in, double* out, uint64_t n, SatShape sat);
 __global__ void k_bsetup(const BatchRep* reps, ApplyState* sts, uint32_t R, const uint64_t* step, const uint32_t* seed, double kT, const double* max_move, int from_args, int advance);
 template <int D>
 __global__ void k_bbin(const BatchRep* reps, ApplyState* sts, const uint32_t* boff, uint32_t R, uint32_t n_total, Bodies B, Work w);

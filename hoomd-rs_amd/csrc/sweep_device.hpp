@@ -75,10 +75,29 @@ struct Geometry {
   uint32_t nverts;
   double cv_rcut2;         // (R + R)^2, R the circumsphere radius (convex.rs:163-166)
   double cv_edge2;         // cv_rcut2 (1 - 2^-30): at or above it the spatial-structure rule is applied
+  int cv_mixed;            // convex: FP32 separating-axis decisions with an FP64 XenoCollide band
   // A copy of this struct in device memory.  Out-of-line helpers receive this
   // pointer instead of the address of a kernel argument, which would make
   // every lane copy the whole struct to scratch memory at kernel entry.
   const Geometry* self;
+};
+
+// Mixed-precision hard convex polyhedra: the polytope's separating axes in
+// FP32, built on the host from its vertices and passed to the convex pass by
+// value (so that the pass reads them through the scalar cache).  Face axes
+// and edge directions are kept once per +-pair.
+constexpr uint32_t kSatFaces = 16;  // face axes of a polytope that the FP32 tests take
+constexpr uint32_t kSatEdges = 8;   // edge directions that the edge-pair axes take
+struct SatShape {
+  float face[kSatFaces][4];   // unit face normal n and hmax = max over the vertices of n.v
+  float fmin[kSatFaces];      // hmin = min over the vertices of n.v
+  float edge[kSatEdges][4];   // unit edge direction
+  float hv[kMaxVertices][4];  // the scanned vertices: all of them, or one of each antipodal pair (sym)
+  uint32_t nf, ne, nh, sym;   // counts; sym: the vertex set is closed under negation
+  float tol;                  // margin of every FP32 decision: 2^-12 R, R the circumsphere radius
+  uint32_t use;               // 0: no FP32 tests (every pair goes to XenoCollide in FP64)
+  uint32_t use_edges;         // the edge-pair axes are complete (ne <= kSatEdges)
+  uint32_t pad;
 };
 
 // The isotropic pair energies, which run one trial per wavefront with a

@@ -64,7 +64,10 @@ enum {
                     error band re-tested in FP64.  LJ: FP32 pair energies
                     (FP64 displacements and cutoff test), the trial's
                     Metropolis decision re-taken in FP64 when the FP32 error
-                    bound on dH could change it */
+                    bound on dH could change it.  Convex polyhedra: FP32
+                    separating-axis tests (face normals, edge-pair cross
+                    products, vertex containment) with a margin of 2^-12 R;
+                    pairs inside it run XenoCollide in FP64 */
 };
 
 typedef struct {
@@ -378,9 +381,19 @@ int hmc_debug_math(int device, int kind, const double* in, double* out, uint64_t
  * collide3d of the polytope with itself (in: v_ij[3], q_ij[4]; out 0/1),
  * kind 1 the global pair test (in: r_i[3], q_i[4], r_j[3], q_j[4]; out 0/1),
  * kind 2 the Rotate<Versor> displacement (in: step, substep, seed, index,
- * sigma; out: q[4]).  Test hook. */
+ * sigma; out: q[4]), kind 3 the mixed mode's FP32 separating-axis class of
+ * a pair (in as for kind 1; out 0 apart, 1 overlap, 2 left to FP64).
+ * Test hook. */
 int hmc_debug_convex(int device, int kind, const double* vertices, uint32_t n_vertices, const double* in,
                      double* out, uint64_t n);
+
+/* The mixed mode's separating-axis data of a polytope (host only):
+ * counts = {face axes, edge directions, scanned vertices, symmetric,
+ * FP32 tests used, edge axes used}; faces (16 x 5: unit normal, hmin, hmax)
+ * and edges (8 x 3: unit direction) in FP32 values, either may be NULL.
+ * Test hook; no reference counterpart. */
+int hmc_debug_convex_axes(const double* vertices, uint32_t n_vertices, uint32_t* counts, double* faces,
+                          double* edges);
 
 /* Draw n standard normals (ziggurat) from stream (step, substep, seed, index). */
 int hmc_debug_normals(int device, uint64_t step, uint32_t substep, uint32_t seed, uint64_t index, double* out,
