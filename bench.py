@@ -43,7 +43,7 @@ def parse():
                     help="time the kernels of every k-th timed apply with HIP events")
     ap.add_argument("--placement-only", action="store_true",
                     help="only run the QuickCompress placement of 1M spheres and print its figures")
-    ap.add_argument("--only", choices=["c1", "c3", "c4", "c5", "placement", "gsd"],
+    ap.add_argument("--only", choices=["c1", "c3", "c4", "c5", "placement", "gsd", "tune"],
                     help="only run this other configuration (with its CPU baseline unless --no-cpu-baseline)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (HBM counter calibration)")
@@ -420,6 +420,8 @@ def other_workloads(hmc, wl, cpu=True, only=None, fp64=True):
         res["C5_replicas_256"] = r
     if want("gsd"):
         res["gsd_frames_1M"] = gsd_frames(hmc, wl, cpu)
+    if want("tune"):
+        res["tune_local_c2"] = tune_bench(hmc, wl, cpu)
     if want("placement"):
         res["placement_quick_compress_1M"] = placement(hmc, wl)
         if cpu:
@@ -433,6 +435,36 @@ def other_workloads(hmc, wl, cpu=True, only=None, fp64=True):
             pl["cpu_baseline_sweeps"] = pc
             pl["sweeps_gpu_over_cpu"] = pl["overlap_penalty_sweep_trial_moves_per_s"] / pc["value"]
     return res
+
+
+def tune_bench(hmc, wl, cpu=True):
+    """ParallelSweep::tune with Translate and TuneOptions::default() (target
+    0.2, 8,000 samples, 32 rounds) on the C2 system (N=1,000,188 hard
+    spheres at packing fraction 0.50), from d = 0.08: the GPU's time and
+    the oracle's serial time for the same rounds, and whether the tuned
+    sizes agree bit for bit."""
+    L, pos = wl.fcc_spheres((63, 63, 63), packing_fraction=0.50)
+    body = wl.wrap(L, pos)
+    sim = hmc.GpuParallelSweep(L, body)
+    sim.tune_translate(0, 0, 42, 1.0, 0.2, 8000, 2, 0.08)  # warm-up
+    t0 = time.perf_counter()
+    m, cnt, amb = sim.tune_translate(0, 0, 42, 1.0, 0.2, 8000, 32, 0.08)
+    el = time.perf_counter() - t0
+    sim.close()
+    r = {"value": 32 * 8000 / el, "unit": "tuning samples/s", "seconds": el, "maximum_distance": [0.08, m],
+         "last_round": cnt[-1], "ambiguous": amb}
+    if cpu:
+        import _oracle as O
+        O.set_threads(1)
+        ref = O.OracleSystem(L, body, seed=42)
+        t0 = time.perf_counter()
+        m2, c2 = ref.tune_translate(1.0, 0.2, 8000, 32, 0.08)
+        el2 = time.perf_counter() - t0
+        r["cpu_baseline"] = dict(value=32 * 8000 / el2, unit="tuning samples/s", cores=1, kind="port",
+                                 sample="the same 32 rounds of 8,000 samples, serial as tune_local_trial is "
+                                        f"({el2:.2f} s), C++ restatement of hoomd-rs 1.4.0")
+        r["identical_to_cpu"] = bool(m2 == m and c2 == cnt)
+    return r
 
 
 def gsd_frames(hmc, wl, cpu=True):

@@ -34,7 +34,7 @@ HMC_MIXED = 1
 # every entry point declared in include/hmc.h
 EXPORTS = ["hmc_create", "hmc_set_state", "hmc_get_state", "hmc_clone_with_boundary", "hmc_apply_translate",
            "hmc_quick_compress_init", "hmc_quick_compress_apply", "hmc_quick_insert_init",
-           "hmc_quick_insert_apply", "hmc_delta_energy_insert", "hmc_gsd_create", "hmc_gsd_open",
+           "hmc_quick_insert_apply", "hmc_delta_energy_insert", "hmc_tune_translate", "hmc_grand_canonical_apply", "hmc_gsd_create", "hmc_gsd_open",
            "hmc_gsd_write_chunk", "hmc_gsd_end_frame", "hmc_gsd_append_frame", "hmc_gsd_n_frames",
            "hmc_gsd_read_chunk", "hmc_gsd_header", "hmc_gsd_close",
            "hmc_apply_rotate",
@@ -84,6 +84,11 @@ class Count(C.Structure):
     _fields_ = [("accepted", C.c_uint64), ("rejected", C.c_uint64)]
 
 
+class GcCount(C.Structure):
+    _fields_ = [("insert_accepted", C.c_uint64), ("insert_rejected", C.c_uint64),
+                ("remove_accepted", C.c_uint64), ("remove_rejected", C.c_uint64)]
+
+
 class Move(C.Structure):
     _fields_ = [("move", C.c_int32), ("pad", C.c_int32), ("size", C.c_double)]
 
@@ -129,6 +134,10 @@ def lib():
         L.hmc_quick_insert_init.restype = None
         L.hmc_quick_insert_apply.argtypes = [C.POINTER(QuickInsertState), C.POINTER(vp), C.c_uint64,
                                              C.c_uint32, C.c_uint32, C.POINTER(Count)]
+        L.hmc_tune_translate.argtypes = [vp, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_uint64,
+                                         C.c_uint32, D_P, C.POINTER(Count), C.POINTER(C.c_uint64)]
+        L.hmc_grand_canonical_apply.argtypes = [C.POINTER(vp), C.c_uint64, C.c_uint32, C.c_uint32, C.c_double,
+                                                C.c_double, C.POINTER(GcCount), C.POINTER(C.c_uint64)]
         L.hmc_gsd_create.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint32, C.c_uint32, C.POINTER(vp)]
         L.hmc_gsd_open.argtypes = [C.c_char_p, C.c_int, C.POINTER(vp)]
         L.hmc_gsd_write_chunk.argtypes = [vp, C.c_char_p, C.c_uint8, C.c_uint64, C.c_uint32, vp]
@@ -365,6 +374,35 @@ class GpuParallelSweep:
         cnt = Count()
         self._check(lib().hmc_run_translate(self.h, step0, seed, kT, max_distance, n_steps, C.byref(cnt)))
         return int(cnt.accepted), int(cnt.rejected)
+
+    def tune_translate(self, step, substep, seed, kT, target, samples, steps, max_move):
+        """ParallelSweep::tune with Translate (tune_local.rs:21-89) against the
+        current state, from Counter(step, substep, seed).make_rng(): returns
+        (the tuned maximum distance, [(accepted, rejected)] per step,
+        ambiguous samples).  The state is not modified."""
+        m = C.c_double(max_move)
+        cnt = (Count * max(int(steps), 1))()
+        amb = C.c_uint64()
+        self._check(lib().hmc_tune_translate(self.h, step, substep, seed, kT, target, int(samples), int(steps),
+                                             C.byref(m), cnt, C.byref(amb)))
+        return m.value, [(int(c.accepted), int(c.rejected)) for c in cnt[:steps]], int(amb.value)
+
+    def grand_canonical(self, step, substep, seed, kT, fugacity):
+        """GrandCanonical(UniformIn over the box)::apply (grand_canonical.rs:145-205)
+        with (step, substep, seed) = the microstate's counter: returns
+        (insert accepted, insert rejected, remove accepted, remove rejected,
+        ambiguous).  An accepted move changes the body list (a new body last,
+        or the last body moved to the removed index) and self.n; the caller
+        then increments the substep."""
+        cnt = GcCount()
+        amb = C.c_uint64()
+        rc = lib().hmc_grand_canonical_apply(C.byref(self.h), step, substep, seed, kT, fugacity, C.byref(cnt),
+                                             C.byref(amb))
+        if rc != HMC_OK:
+            raise HmcError(rc, lib().hmc_last_error(self.h).decode())
+        self.n += int(cnt.insert_accepted) - int(cnt.remove_accepted)
+        return (int(cnt.insert_accepted), int(cnt.insert_rejected), int(cnt.remove_accepted),
+                int(cnt.remove_rejected), int(amb.value))
 
     def run_moves(self, step0, seed, kT, moves, n_steps):
         """n_steps of {each (kind, size) of moves in order; increment_step}

@@ -10,7 +10,7 @@ FLAGS=(-O3 -std=c++17 --offload-arch=gfx950 -fPIC -ffp-contract=off -fno-fast-ma
        -Wall -Wno-unused-function -Wno-unused-parameter -Wno-unused-result)
 SRCS=(csrc/k_build.hip csrc/k_hard.hip csrc/k_persist.hip csrc/k_persist_lj.hip csrc/k_lj.hip csrc/k_convex.hip
       csrc/k_measure.hip csrc/k_misc.hip csrc/api_ctx.hip csrc/api_batch.hip csrc/api_measure.hip
-      csrc/api_place.hip csrc/api_gsd.hip)
+      csrc/api_place.hip csrc/api_gsd.hip csrc/k_tune.hip)
 mkdir -p _build
 pids=()
 objs=()

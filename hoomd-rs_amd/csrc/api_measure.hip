@@ -179,6 +179,25 @@ int hmc_sdf(hmc_ctx* c, double xmax, double dx, uint32_t nbins, uint64_t* hist, 
   return HMC_OK;
 }
 
+}  // extern "C"
+
+void hmc::launch_cell_lists(hmc_ctx* c) {
+  const Geometry& g = c->g;
+  const uint32_t N = g.n_bodies;
+  hipStream_t s = c->stream;
+  k_setup<<<1, 64, 0, s>>>(g, c->st, 0, 0, 0, 1.0, 1.0, 0, 0, 0u, 0);
+  if (g.dim == 2)
+    k_bin<2><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
+  else
+    k_bin<3><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
+  k_scan_blocks<<<c->n_scan_blocks, 256, 0, s>>>(c->w.cnt, c->w.start, c->w.block_sums, c->w.block_ne, g.n_spaces);
+  k_scan_top<<<1, 1024, 0, s>>>(g, c->st, c->w.block_sums, c->w.block_ne, c->n_scan_blocks, c->w.start, 0);
+  k_scan_add<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.start, c->w.block_sums, c->w.cnt, g.n_spaces);
+  k_scatter<<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->w);
+}
+
+extern "C" {
+
 int hmc_delta_energy_insert(hmc_ctx* c, const double* body_pos, uint64_t n, double* energy, double* bound) {
   if (!c || (n && (!body_pos || !energy))) return HMC_ERR_INVALID;
   if (!c->have_state) return fail(c, HMC_ERR_STATE, "hmc_set_state was not called");
@@ -207,16 +226,7 @@ int hmc_delta_energy_insert(hmc_ctx* c, const double* body_pos, uint64_t n, doub
   if (e == hipSuccess) e = hipMemcpyAsync(dc, body_pos, (size_t)n * D * sizeof(double), hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemsetAsync(flag, 0, sizeof(uint32_t), s);
   if (e == hipSuccess) {
-    // cells of the current state, as for the total energy
-    k_setup<<<1, 64, 0, s>>>(g, c->st, 0, 0, 0, 1.0, 1.0, 0, 0, 0u, 0);
-    if (D == 2)
-      k_bin<2><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
-    else
-      k_bin<3><<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->st, c->bodies, c->w);
-    k_scan_blocks<<<c->n_scan_blocks, 256, 0, s>>>(c->w.cnt, c->w.start, c->w.block_sums, c->w.block_ne, g.n_spaces);
-    k_scan_top<<<1, 1024, 0, s>>>(g, c->st, c->w.block_sums, c->w.block_ne, c->n_scan_blocks, c->w.start, 0);
-    k_scan_add<<<grid_for(g.n_spaces, 256), 256, 0, s>>>(c->w.start, c->w.block_sums, c->w.cnt, g.n_spaces);
-    k_scatter<<<grid_for(N, kBlock), kBlock, 0, s>>>(g, c->w);
+    launch_cell_lists(c);  // cells of the current state, as for the total energy
     if (D == 2)
       k_insert_energy<2><<<grid_for(n, kEnergyBlock), kEnergyBlock, 0, s>>>(g, c->st, c->bodies, c->w, dc,
                                                                            (uint32_t)n, dout, flag);

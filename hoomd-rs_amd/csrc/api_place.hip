@@ -1,7 +1,9 @@
 // Initial-state preparation through the C ABI: QuickCompress
 // (hoomd-mc/src/quick_compress.rs) over device energies and boundary clones,
-// and QuickInsert (hoomd-mc/src/quick_insert.rs) over device insertion
-// energies.
+// QuickInsert (hoomd-mc/src/quick_insert.rs) over device insertion energies,
+// the move-size tuner (hoomd-mc/src/tune_local.rs) over device trial
+// evaluations, and GrandCanonical (hoomd-mc/src/grand_canonical.rs) over
+// device insertion and removal energies.
 #include "host.hpp"
 
 #include <unordered_map>
@@ -228,6 +230,40 @@ void hmc_quick_insert_init(hmc_quick_insert* qi, uint64_t target) {
   qi->state = HMC_QI_RUNNING;
 }
 
+// The context of a changed body list (bodies and sites in index order, D
+// per row): a new context with the same configuration replaces *pc.
+static int rebuild_with(hmc_ctx** pc, const std::vector<double>& body, const std::vector<double>& site, uint64_t n,
+                        const char* who) {
+  hmc_ctx* c = *pc;
+  hmc_config cfg = c->cfg_user;
+  cfg.n_bodies = n;
+  hmc_ctx* d = nullptr;
+  int rc = hmc_create(&cfg, &d);
+  if (rc) return fail(c, rc, std::string(who) + ": creating the context of the new microstate failed");
+  rc = hmc_set_state(d, body.data(), site.data());
+  if (rc) {
+    hmc_destroy(d);
+    return fail(c, rc, std::string(who) + ": setting the state of the new microstate failed");
+  }
+  hmc_destroy(c);
+  *pc = d;
+  return HMC_OK;
+}
+
+// random_range(0..len) for len < 2^32 (rand 0.10 samples a usize range that
+// fits in u32 as a u32): Canon's widening multiply with at most one extra
+// draw, as choose_index on the device.
+static uint32_t host_range_u32(HostSfc64& r, uint32_t len) {
+  const uint64_t m = (uint64_t)(uint32_t)r.next() * (uint64_t)len;
+  uint32_t result = (uint32_t)(m >> 32);
+  const uint32_t lo = (uint32_t)m;
+  if (lo > (uint32_t)(0u - len)) {
+    const uint32_t hi2 = (uint32_t)(((uint64_t)(uint32_t)r.next() * (uint64_t)len) >> 32);
+    if ((uint64_t)lo + (uint64_t)hi2 > 0xffffffffull) result += 1;
+  }
+  return result;
+}
+
 int hmc_quick_insert_apply(hmc_quick_insert* qi, hmc_ctx** pc, uint64_t step, uint32_t substep, uint32_t seed,
                            hmc_count* count) {
   if (!qi || !pc || !*pc || !count) return HMC_ERR_INVALID;
@@ -355,19 +391,238 @@ int hmc_quick_insert_apply(hmc_quick_insert* qi, hmc_ctx** pc, uint64_t step, ui
       body[(size_t)(N + k) * D + a] = added[k].body[a];
       site[(size_t)(N + k) * D + a] = added[k].site[a];
     }
-  hmc_config cfg = c->cfg_user;
-  cfg.n_bodies = N + A;
-  hmc_ctx* d = nullptr;
-  rc = hmc_create(&cfg, &d);
-  if (rc) return fail(c, rc, "QuickInsert: creating the context of the larger microstate failed");
-  rc = hmc_set_state(d, body.data(), site.data());
-  if (rc) {
-    hmc_destroy(d);
-    return fail(c, rc, "QuickInsert: setting the state of the larger microstate failed");
+  return rebuild_with(pc, body, site, N + A, "QuickInsert");
+}
+
+// ------------------------------------------------------------ tune_local --
+int hmc_tune_translate(hmc_ctx* c, uint64_t step, uint32_t substep, uint32_t seed, double kT, double target,
+                       uint64_t samples, uint32_t steps, double* maximum_distance, hmc_count* counts,
+                       uint64_t* ambiguous) {
+  if (!c || !maximum_distance) return HMC_ERR_INVALID;
+  if (!c->have_state) return fail(c, HMC_ERR_STATE, "no state");
+  const Geometry& g = c->g;
+  const int D = g.dim;
+  if (!is_isotropic(g.model) && g.model != HMC_HARD_SPHERE)
+    return fail(c, HMC_ERR_UNSUPPORTED, "tuning of oriented bodies is not offloaded");
+  if (!(target > 0.0 && target < 1.0) || !(*maximum_distance > 0.0) || !std::isfinite(*maximum_distance) ||
+      !(kT > 0.0))
+    return fail(c, HMC_ERR_INVALID, "tune: target in (0, 1), maximum distance and kT > 0");
+  if (g.n_bodies == 0) return fail(c, HMC_ERR_INVALID, "tune: no bodies to sample");
+  if (samples > (1ull << 26)) return fail(c, HMC_ERR_INVALID, "tune: at most 2^26 samples per step");
+  if (ambiguous) *ambiguous = 0;
+  HIPCHK(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  launch_cell_lists(c);  // the state is not modified: one set of cells serves every step
+  HostSfc64 rng = host_counter_rng(step, substep, seed);
+  uint64_t M = samples * (D == 3 ? 7 : 6) + 4096, cap = 0;
+  uint64_t* dx = nullptr;
+  uint32_t* dn = nullptr;
+  uint8_t* df = nullptr;
+  std::vector<uint64_t> hx;
+  std::vector<uint32_t> hn;
+  std::vector<uint8_t> hf;
+  auto release = [&]() {
+    if (dx) (void)hipFreeAsync(dx, s);
+    if (dn) (void)hipFreeAsync(dn, s);
+    if (df) (void)hipFreeAsync(df, s);
+    dx = nullptr;
+    dn = nullptr;
+    df = nullptr;
+  };
+  int rc = HMC_OK;
+  uint64_t amb = 0;
+  for (uint32_t t = 0; t < steps && !rc; ++t) {
+    uint64_t acc = 0, rej = 0, a = 0, p = 0;
+    for (;;) {
+      if (M > 0x7fffffffull) {
+        rc = fail(c, HMC_ERR_DEVICE, "tune: the sample chain outgrew its stream");
+        break;
+      }
+      if (M > cap) {
+        release();
+        hipError_t e = hipMallocAsync((void**)&dx, M * sizeof(uint64_t), s);
+        if (e == hipSuccess) e = hipMallocAsync((void**)&dn, M * sizeof(uint32_t), s);
+        if (e == hipSuccess) e = hipMallocAsync((void**)&df, M, s);
+        if (e != hipSuccess) {
+          rc = fail(c, HMC_ERR_DEVICE, std::string("tune: ") + hipGetErrorString(e));
+          break;
+        }
+        cap = M;
+        hx.resize(M);
+        hn.resize(M);
+        hf.resize(M);
+      }
+      // the step's stream from the generator's current state
+      HostSfc64 r = rng;
+      for (uint64_t i = 0; i < M; ++i) hx[i] = r.next();
+      hipError_t e = hipMemcpyAsync(dx, hx.data(), M * sizeof(uint64_t), hipMemcpyHostToDevice, s);
+      if (e == hipSuccess) {
+        if (D == 2)
+          k_tune_eval<2><<<grid_for(M, kEnergyBlock), kEnergyBlock, 0, s>>>(g, c->st, c->bodies, c->w, dx,
+                                                                             (uint32_t)M, *maximum_distance, kT,
+                                                                             dn, df);
+        else
+          k_tune_eval<3><<<grid_for(M, kEnergyBlock), kEnergyBlock, 0, s>>>(g, c->st, c->bodies, c->w, dx,
+                                                                             (uint32_t)M, *maximum_distance, kT,
+                                                                             dn, df);
+        e = hipGetLastError();
+      }
+      if (e == hipSuccess) e = hipMemcpyAsync(hn.data(), dn, M * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipMemcpyAsync(hf.data(), df, M, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e != hipSuccess) {
+        rc = fail(c, HMC_ERR_DEVICE, std::string("tune: ") + hipGetErrorString(e));
+        break;
+      }
+      // the samples of this step: the chain from offset 0
+      acc = rej = a = p = 0;
+      bool ok = true;
+      for (uint64_t k = 0; k < samples; ++k) {
+        const uint32_t q = p < M ? hn[p] : 0xFFFFFFFFu;
+        if (q == 0xFFFFFFFFu) {
+          ok = false;
+          break;
+        }
+        if (hf[p] & 4u) {
+          rc = fail(c, HMC_ERR_STATE, "tune: a site outside the checkerboard");
+          break;
+        }
+        if (hf[p] & 1u)
+          ++acc;
+        else
+          ++rej;
+        a += (hf[p] >> 1) & 1u;
+        p = q;
+      }
+      if (rc || ok) break;
+      M *= 2;  // a longer stream for this step
+    }
+    if (rc) break;
+    for (uint64_t i = 0; i < p; ++i) (void)rng.next();  // the generator after the step's samples
+    amb += a;
+    if (counts) {
+      counts[t].accepted = acc;
+      counts[t].rejected = rej;
+    }
+    // tune_by_scaling (hoomd-mc/src/lib.rs:514-524), Translate::adjust
+    // (translate.rs:87-92)
+    if (acc + rej > 0) {
+      const double ratio = (double)acc / (double)(acc + rej);
+      const double scale = (ratio + 1.5) / (target + 1.5);
+      *maximum_distance *= scale;
+    }
   }
-  hmc_destroy(c);
-  *pc = d;
-  return HMC_OK;
+  release();
+  if (ambiguous) *ambiguous = amb;
+  return rc;
+}
+
+// --------------------------------------------------------- GrandCanonical --
+int hmc_grand_canonical_apply(hmc_ctx** pc, uint64_t step, uint32_t substep, uint32_t seed, double kT,
+                              double fugacity, hmc_gc_count* count, uint64_t* ambiguous) {
+  if (!pc || !*pc || !count) return HMC_ERR_INVALID;
+  hmc_ctx* c = *pc;
+  std::memset(count, 0, sizeof *count);
+  if (ambiguous) *ambiguous = 0;
+  if (!c->have_state) return fail(c, HMC_ERR_STATE, "no state");
+  const Geometry& g = c->g;
+  const int D = g.dim;
+  if (!is_isotropic(g.model) && g.model != HMC_HARD_SPHERE)
+    return fail(c, HMC_ERR_UNSUPPORTED, "GrandCanonical of oriented bodies is not offloaded");
+  if (!(kT > 0.0) || !std::isfinite(fugacity)) return fail(c, HMC_ERR_INVALID, "GrandCanonical: kT > 0, finite f");
+  const uint64_t n = g.n_bodies;
+  const double volume = box_volume(g);
+  if (n == 0 && fugacity <= 0.0) return HMC_OK;  // no move can be accepted
+  HostSfc64 rng = host_counter_rng(step, substep, seed);
+  const bool insert = (int32_t)(uint32_t)rng.next() < 0;  // StandardUniform bool: the sign bit of a u32
+  auto u01 = [&]() { return (double)(rng.next() >> 11) * 0x1p-53; };
+  std::vector<double> body, site;
+  auto state = [&](uint64_t cap) {
+    body.assign((size_t)cap * D, 0.0);
+    site.assign((size_t)cap * D, 0.0);
+    return n ? hmc_get_state(c, body.data(), site.data()) : HMC_OK;
+  };
+  int rc = HMC_OK;
+  if (insert) {
+    // UniformIn over the box (uniform_in.rs:99-110): one Uniform draw per axis
+    double p[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < D; ++a) p[a] = host_uniform(rng.next(), -g.L[a] / 2.0, g.L[a] / 2.0);
+    double de = 0.0, db = 0.0;
+    rc = hmc_delta_energy_insert(c, p, 1, &de, &db);
+    if (rc) return rc;
+    if (!std::isfinite(de)) {
+      count->insert_rejected = 1;
+      return HMC_OK;
+    }
+    const double p_insert = (volume * fugacity * std::exp(-de / kT)) / ((double)n + 1.0);
+    const double u = u01();
+    if (ambiguous && db > 0.0) {
+      const double plo = (volume * fugacity * std::exp(-(de + db) / kT)) / ((double)n + 1.0);
+      const double phi = (volume * fugacity * std::exp(-(de - db) / kT)) / ((double)n + 1.0);
+      if ((plo > u) != (phi > u)) *ambiguous = 1;
+    }
+    if (!(p_insert > u)) {
+      count->insert_rejected = 1;
+      return HMC_OK;
+    }
+    // Microstate::add_body (microstate.rs:777-835): body = wrap(p), site = wrap(body + 0)
+    rc = state(n + 1);
+    if (rc) return rc;
+    for (int a = 0; a < D; ++a) {
+      body[(size_t)n * D + a] = host_wrap(p[a], g.L[a]);
+      site[(size_t)n * D + a] = host_wrap(body[(size_t)n * D + a] + 0.0, g.L[a]);
+    }
+    rc = rebuild_with(pc, body, site, n + 1, "GrandCanonical");
+    if (!rc) count->insert_accepted = 1;
+    return rc;
+  }
+  if (n == 0) return HMC_OK;
+  const uint32_t index = host_range_u32(rng, (uint32_t)n);
+  double out[2] = {0.0, 0.0};
+  {
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    double* d = nullptr;
+    hipError_t e = hipMallocAsync((void**)&d, 2 * sizeof(double), s);
+    if (e == hipSuccess) {
+      launch_cell_lists(c);
+      if (D == 2)
+        k_remove_energy<2><<<1, 64, 0, s>>>(g, c->st, c->bodies, c->w, index, d);
+      else
+        k_remove_energy<3><<<1, 64, 0, s>>>(g, c->st, c->bodies, c->w, index, d);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipMemcpyAsync(out, d, sizeof out, hipMemcpyDeviceToHost, s);
+      (void)hipFreeAsync(d, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
+    if (e != hipSuccess) return fail(c, HMC_ERR_DEVICE, std::string("GrandCanonical: ") + hipGetErrorString(e));
+    if (out[1] < 0.0) return fail(c, HMC_ERR_STATE, "GrandCanonical: a site outside the checkerboard");
+  }
+  const double de = out[0], db = out[1];
+  const double p_remove = fugacity <= 0.0 ? 1.0 : ((double)n * std::exp(-de / kT)) / (volume * fugacity);
+  const double u = u01();
+  if (ambiguous && db > 0.0 && fugacity > 0.0) {
+    const double plo = ((double)n * std::exp(-(de + db) / kT)) / (volume * fugacity);
+    const double phi = ((double)n * std::exp(-(de - db) / kT)) / (volume * fugacity);
+    if ((plo > u) != (phi > u)) *ambiguous = 1;
+  }
+  if (!(p_remove > u)) {
+    count->remove_rejected = 1;
+    return HMC_OK;
+  }
+  // Microstate::remove_body (microstate.rs:917-943): the last body takes the
+  // removed one's index (swap_remove)
+  rc = state(n);
+  if (rc) return rc;
+  for (int a = 0; a < D; ++a) {
+    body[(size_t)index * D + a] = body[(size_t)(n - 1) * D + a];
+    site[(size_t)index * D + a] = site[(size_t)(n - 1) * D + a];
+  }
+  body.resize((size_t)(n - 1) * D);
+  site.resize((size_t)(n - 1) * D);
+  rc = rebuild_with(pc, body, site, n - 1, "GrandCanonical");
+  if (!rc) count->remove_accepted = 1;
+  return rc;
 }
 
 }  // extern "C"

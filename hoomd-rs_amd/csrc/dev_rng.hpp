@@ -43,18 +43,24 @@ __device__ __forceinline__ Sfc64 counter_rng(uint64_t step, uint32_t substep, ui
   return r;
 }
 
+// The samplers below take any generator with next() / next_u32(): Sfc64,
+// or a pre-drawn stream (the tuner reads one stream from every offset).
+
 // StandardUniform f64: 53 high bits times 2^-53, in [0, 1).
-__device__ __forceinline__ double uniform01(Sfc64& r) { return (double)(r.next() >> 11) * 0x1p-53; }
+template <class R>
+__device__ __forceinline__ double uniform01(R& r) { return (double)(r.next() >> 11) * 0x1p-53; }
 
 // Open01 f64: 52 high bits as the mantissa of [1, 2) minus (1 - 2^-53).
-__device__ __forceinline__ double open01(Sfc64& r) {
+template <class R>
+__device__ __forceinline__ double open01(R& r) {
   uint64_t bits = (r.next() >> 12) | 0x3ff0000000000000ull;
   return __longlong_as_double((long long)bits) - (1.0 - 0x1p-53);
 }
 
 // Index in [0, len), len <= 2^32 - 1: u32 sample with Canon's widening multiply
 // and at most one extra draw (choose on a slice).
-__device__ __forceinline__ uint32_t choose_index(Sfc64& r, uint32_t len) {
+template <class R>
+__device__ __forceinline__ uint32_t choose_index(R& r, uint32_t len) {
   uint64_t m = (uint64_t)r.next_u32() * (uint64_t)len;
   uint32_t result = (uint32_t)(m >> 32);
   uint32_t lo = (uint32_t)m;
@@ -69,7 +75,8 @@ __device__ __forceinline__ uint32_t choose_index(Sfc64& r, uint32_t len) {
 // (rand_distr 0.6).  The tail's log and the wedge test's exp are glibc's,
 // as Rust's f64::ln / f64::exp call them on Linux.  zx / zf: the layer
 // tables (kZigX / kZigF or a copy of them in LDS).
-__device__ __forceinline__ double standard_normal(Sfc64& r, const double* zx, const double* zf) {
+template <class G>
+__device__ __forceinline__ double standard_normal(G& r, const double* zx, const double* zf) {
   for (;;) {
     uint64_t bits = r.next();
     int i = (int)(bits & 0xff);

@@ -119,6 +119,9 @@ int make_geometry(const hmc_config* cfg, Geometry& g, bool* fast, double* sigma_
 const char* sdf_check(const Geometry& g, double xmax, double dx, uint32_t nbins);
 Geometry sdf_geometry(const Geometry& g, double xmax);
 int total_energy_into(hmc_ctx* c, double* part, double* energy, double* bound);
+// Cell lists of the current state on c's stream (checkerboard spaces at
+// origin 0, bodies of each space in body order: w.start, w.tmp_body).
+void launch_cell_lists(hmc_ctx* c);
 
 // The allocation-free total energy (hmc_total_energy_noalloc): its static
 // scratch holds the partials of kNoAllocBlocks energy blocks; noalloc_setup

@@ -51,6 +51,10 @@ __global__ void k_energy_final(const double* part, uint32_t nblocks, double* out
 template <int D>
 __global__ void k_insert_energy(Geometry g, const ApplyState* st, Bodies B, Work w, const double* __restrict__ cand, uint32_t n, double* __restrict__ out, uint32_t* outside);
 template <int D>
+__global__ void k_tune_eval(Geometry g, const ApplyState* st, Bodies B, Work w, const uint64_t* __restrict__ xs, uint32_t M, double max_move, double kT, uint32_t* __restrict__ next, uint8_t* __restrict__ flags);
+template <int D>
+__global__ void k_remove_energy(Geometry g, const ApplyState* st, Bodies B, Work w, uint32_t body, double* out);
+template <int D>
 __global__ void k_sdf(const BatchRep* reps, const uint32_t* boff, uint32_t R, uint32_t n_total, Bodies B, Work w, double xmax, double dx, uint32_t nbins, uint32_t* hist, uint32_t* ovl);
 template <int D>
 __global__ void k_init_sites(Geometry g, Bodies B, int have_sites);

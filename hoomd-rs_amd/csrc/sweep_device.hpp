@@ -262,8 +262,8 @@ constexpr uint32_t kLookbackBits = 27;  // count bits of the look-back scan's st
 
 // Ball::sample (hoomd-vector/src/distribution.rs:39-70): the displacement of
 // Translate::propose (hoomd-mc/src/translate/cartesian.rs:39-52).
-template <int D>
-__device__ __forceinline__ void ball_sample(Sfc64& rng, double r, const double* zx, const double* zf, double* v) {
+template <int D, class R>
+__device__ __forceinline__ void ball_sample(R& rng, double r, const double* zx, const double* zf, double* v) {
   if (D == 2) {
     for (;;) {
       double c0 = uniform01(rng);

@@ -242,6 +242,41 @@ void hmc_quick_insert_init(hmc_quick_insert* qi, uint64_t target);
 int hmc_quick_insert_apply(hmc_quick_insert* qi, hmc_ctx** ctx, uint64_t step, uint32_t substep, uint32_t seed,
                            hmc_count* count);
 
+/* ParallelSweep::tune / Sweep::tune with Translate (hoomd-mc/src/tune_local.rs:21-89,
+ * parallel_sweep.rs:486-506): `steps` rounds of `samples` trial moves of
+ * random bodies against the unchanged microstate, from one stream
+ * Counter(step, substep, seed).make_rng(); after each round
+ * tune_by_scaling (lib.rs:514-524) multiplies *maximum_distance by
+ * (ratio + 1.5) / (target + 1.5).  The state is not modified and no substep
+ * is consumed.  counts (optional, `steps` entries) get each round's accepted
+ * and rejected samples; *ambiguous (optional) the samples whose draw or
+ * decision the bound on the summation order could not fix (decided by the
+ * GPU's dH; expected 0).  Every sample of a round is evaluated on the device
+ * at once: the round's stream is drawn on the host, one lane takes a sample
+ * from every offset of it (k_tune_eval), and the samples are the chain of
+ * offsets from 0.  Point bodies only. */
+int hmc_tune_translate(hmc_ctx* ctx, uint64_t step, uint32_t substep, uint32_t seed, double kT, double target,
+                       uint64_t samples, uint32_t steps, double* maximum_distance, hmc_count* counts,
+                       uint64_t* ambiguous);
+
+/* GrandCanonical(UniformIn over the periodic box)::apply
+ * (hoomd-mc/src/grand_canonical.rs:145-205): one insertion or removal trial,
+ * chosen by a bool draw of Counter(step, substep, seed).make_rng().  An
+ * insertion at a uniform position is accepted when
+ * V f exp(-dE/kT) / (n + 1) > u, a removal of body random_range(0..n) when
+ * n exp(-dE/kT) / (V f) > u (1 > u when f <= 0); the insertion and removal
+ * energies come from the device.  An accepted move replaces *ctx by a
+ * context of the new body list (add_body appends; remove_body moves the last
+ * body to the removed index) and destroys the old one.  The caller then
+ * increments the substep, as the reference's apply does.  *ambiguous
+ * (optional): 1 when the bound on the summation order could flip the
+ * decision (decided by the GPU's dE).  Point bodies only. */
+typedef struct {
+  uint64_t insert_accepted, insert_rejected, remove_accepted, remove_rejected;
+} hmc_gc_count;
+int hmc_grand_canonical_apply(hmc_ctx** ctx, uint64_t step, uint32_t substep, uint32_t seed, double kT,
+                              double fugacity, hmc_gc_count* count, uint64_t* ambiguous);
+
 /* One ParallelSweep::apply with Translate moves (parallel_sweep.rs:390-421).
  * (step, substep, seed) = microstate.counter(); kT = macrostate.temperature();
  * max_distance = Translate::maximum_distance.  On return *substeps_used holds

@@ -16,6 +16,8 @@ namespace oracle {
 
 inline double lib_log(double x) { return std::log(x); }
 
+inline double lib_exp(double x) { return std::exp(x); }
+
 // lhs < exp(y)
 inline bool lib_lt_exp(double lhs, double y) { return lhs < std::exp(y); }
 

@@ -464,6 +464,122 @@ struct System {
     substep += 1;
   }
 
+  // tune_local_trial (hoomd-mc/src/tune_local.rs:21-89) with Translate, all
+  // bodies eligible (parallel_sweep.rs:486-506), then tune_by_scaling
+  // (lib.rs:514-524: GAMMA = 1.5) and Translate::adjust (translate.rs:87-92).
+  // One stream for every step; the microstate is not modified.  body =
+  // random_range(0..n): rand samples a usize range that fits in u32 as a
+  // u32 (rand_index).  counts: accepted, rejected per step.
+  void tune_translate(double kt, double target, uint64_t samples, uint32_t steps, double& max_move,
+                      uint64_t* counts) const {
+    Sfc64 rng = counter_rng(step, substep, seed, 0);
+    const uint64_t n = bodies.size();
+    for (uint32_t t = 0; t < steps; ++t) {
+      uint64_t acc = 0, rej = 0;
+      while (acc + rej < samples) {
+        const uint64_t bi = rand_index(rng, (uint32_t)n);
+        const Vec<D> np = boundary.wrap(propose_translate(rng, bodies[bi], max_move));
+        const double dh = delta_energy_one(bi, np);
+        if (dh != INF && (dh <= 0.0 || lib_lt_exp(rand_f64(rng), -dh / kt)))
+          ++acc;
+        else
+          ++rej;
+      }
+      counts[2 * t] = acc;
+      counts[2 * t + 1] = rej;
+      if (acc + rej > 0) {
+        const double ratio = (double)acc / (double)(acc + rej);
+        max_move *= (ratio + 1.5) / (target + 1.5);
+      }
+    }
+  }
+
+  // pairwise_cutoff.rs:1097-1109 with :458-488: minus the initial energy of
+  // the body's site with the other bodies' sites (0 for hard shapes)
+  double delta_energy_remove(uint64_t body_index) const {
+    double e = 0.0;
+    if (!model.only_infinite_or_zero()) {
+      const Site<D>& s = sites.items[sites.indices[body_index]];
+      const double one = site_energy(s.pos, body_index, true);
+      if (one == INF) return -INF;
+      e += one;
+    }
+    return -e;
+  }
+
+  // The microstate of another body list, kept as stored (no wrap: wrap is
+  // not idempotent in floating point).  Tags are renumbered in index order;
+  // the reference reuses freed tags, which enter no decision here.
+  void reset_bodies(const std::vector<Vec<D>>& nb, const std::vector<Vec<D>>& ns) {
+    bodies.clear();
+    sites = VecWithTags<Site<D>>{};
+    sites_ghosts.clear();
+    ghosts = VecWithTags<Site<D>>{};
+    spatial = VecCell<D>(spatial.w);
+    for (size_t k = 0; k < nb.size(); ++k) {
+      Site<D> s;
+      s.site_tag = k;
+      s.body_tag = k;
+      s.pos = ns[k];
+      spatial.insert(key_primary(k), s.pos);
+      sites.push(s);
+      sites_ghosts.emplace_back();
+      bodies.push_back(nb[k]);
+      update_site_ghosts((int64_t)k);
+    }
+  }
+
+  // GrandCanonical(UniformIn over the periodic box)::apply
+  // (hoomd-mc/src/grand_canonical.rs:145-205).  The bool draw is rand's
+  // StandardUniform bool, the sign bit of a u32 (rand 0.10.4, PARITY
+  // UNPINNED).  cnt = {insert accepted, insert rejected, remove accepted,
+  // remove rejected}, incremented.
+  void grand_canonical_apply(double kt, double fugacity, uint64_t* cnt) {
+    const uint64_t n = bodies.size();
+    double volume = 1.0;
+    for (int i = 0; i < D; ++i) volume *= boundary.L[i];
+    Sfc64 rng = counter_rng(step, substep, seed, 0);
+    if (n == 0 && fugacity <= 0.0) {
+      substep += 1;
+      return;
+    }
+    const bool insert = (int32_t)rng.next_u32() < 0;
+    if (insert) {
+      Vec<D> p;
+      for (int a = 0; a < D; ++a) p[a] = rand_uniform(rng, -boundary.L[a] / 2.0, boundary.L[a] / 2.0);
+      const double de = delta_energy_insert(p);
+      if (std::isfinite(de)) {
+        const double p_insert = (volume * fugacity * lib_exp(-de / kt)) / ((double)n + 1.0);
+        if (p_insert > rand_f64(rng)) {
+          add_body(p);
+          cnt[0] += 1;
+        } else {
+          cnt[1] += 1;
+        }
+      } else {
+        cnt[1] += 1;
+      }
+    } else if (n > 0) {
+      const uint64_t index = rand_index(rng, (uint32_t)n);
+      const double de = delta_energy_remove(index);
+      const double p_remove = fugacity <= 0.0 ? 1.0 : ((double)n * lib_exp(-de / kt)) / (volume * fugacity);
+      if (p_remove > rand_f64(rng)) {
+        // Microstate::remove_body (microstate.rs:917-943): swap_remove
+        std::vector<Vec<D>> nb = bodies, ns(n);
+        for (uint64_t k = 0; k < n; ++k) ns[k] = sites.items[sites.indices[k]].pos;
+        nb[index] = nb[n - 1];
+        ns[index] = ns[n - 1];
+        nb.pop_back();
+        ns.pop_back();
+        reset_bodies(nb, ns);
+        cnt[2] += 1;
+      } else {
+        cnt[3] += 1;
+      }
+    }
+    substep += 1;
+  }
+
   // microstate.rs:412-415
   void increment_step() {
     step += 1;

@@ -376,6 +376,15 @@ double oracle_total_energy(void* h) {
 void oracle_quick_insert_apply(void* h, uint64_t* qi, uint64_t* acc, uint64_t* rej) {
   DISPATCH(h, S.quick_insert_apply(qi, *acc, *rej));
 }
+// tune_local with Translate: counts = 2 per step; *max_move updated
+void oracle_tune_translate(void* h, double kt, double target, uint64_t samples, uint32_t steps, double* max_move,
+                           uint64_t* counts) {
+  DISPATCH(h, S.tune_translate(kt, target, samples, steps, *max_move, counts));
+}
+// GrandCanonical::apply; cnt = {insert acc, insert rej, remove acc, remove rej}, incremented
+void oracle_grand_canonical_apply(void* h, double kt, double fugacity, uint64_t* cnt) {
+  DISPATCH(h, S.grand_canonical_apply(kt, fugacity, cnt));
+}
 uint64_t oracle_n_bodies(void* h) {
   uint64_t n = 0;
   DISPATCH(h, n = S.bodies.size());

@@ -79,6 +79,10 @@ def lib():
         L.oracle_delta_energy_one.restype = C.c_double
         L.oracle_quick_insert_apply.argtypes = [C.c_void_p, U64_P, U64_P, U64_P]
         L.oracle_quick_insert_apply.restype = None
+        L.oracle_tune_translate.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_uint64, C.c_uint32, D_P, U64_P]
+        L.oracle_tune_translate.restype = None
+        L.oracle_grand_canonical_apply.argtypes = [C.c_void_p, C.c_double, C.c_double, U64_P]
+        L.oracle_grand_canonical_apply.restype = None
         L.oracle_n_bodies.argtypes = [C.c_void_p]
         L.oracle_n_bodies.restype = C.c_uint64
         L.oracle_delta_energy_insert.argtypes = [C.c_void_p, D_P]
@@ -279,6 +283,24 @@ class OracleSystem:
         lib().oracle_quick_insert_apply(self.h, _p(qi, U64_P), _p(acc, U64_P), _p(rej, U64_P))
         self.n = int(lib().oracle_n_bodies(self.h))
         return int(acc[0]), int(rej[0])
+
+    def tune_translate(self, kt, target, samples, steps, max_move):
+        """Sweep::tune / ParallelSweep::tune with Translate (tune_local.rs:21-89):
+        returns (the tuned maximum distance, [(accepted, rejected)] per step).
+        The microstate and its counter are unchanged."""
+        m = np.array([max_move], np.float64)
+        cnt = np.zeros(2 * steps, np.uint64)
+        lib().oracle_tune_translate(self.h, kt, target, samples, steps, _p(m, D_P), _p(cnt, U64_P))
+        return float(m[0]), [(int(cnt[2 * t]), int(cnt[2 * t + 1])) for t in range(steps)]
+
+    def grand_canonical(self, kt, fugacity):
+        """GrandCanonical(UniformIn over the box)::apply (grand_canonical.rs:145-205):
+        returns (insert accepted, insert rejected, remove accepted, remove
+        rejected); the substep advances by one."""
+        cnt = np.zeros(4, np.uint64)
+        lib().oracle_grand_canonical_apply(self.h, kt, fugacity, _p(cnt, U64_P))
+        self.n = int(lib().oracle_n_bodies(self.h))
+        return tuple(int(x) for x in cnt)
 
     def delta_energy_insert(self, body):
         """PairwiseCutoff::delta_energy_insert of a new body at `body`."""
