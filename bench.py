@@ -120,6 +120,19 @@ def measured_traffic(kernel):
     return None
 
 
+def model_apply_bytes(n, n_spaces, cycles, accepted, retests, dim=3, mixed=True):
+    """Compulsory-traffic model of one apply, the per-trial figure the
+    roofline uses: every colour pass (cycles x 2^D of them) reads every body
+    position once (P = 4 B per FP32 coordinate of the cell-relative copy in
+    mixed mode, 8 B per FP64 coordinate otherwise) with its cell index (4 B),
+    and the cell offsets (4 (S + 1) B); each accepted move writes its FP64
+    position (8 B per coordinate) and, in mixed mode, its FP32 copy; each
+    FP64 band re-test reads one FP64 position."""
+    p = (4 if mixed else 8) * dim
+    w = 8 * dim + (4 * dim if mixed else 0)
+    return cycles * 2**dim * (n * (p + 4) + 4 * (n_spaces + 1)) + w * accepted + 8 * dim * retests
+
+
 def apply_bytes(n, n_spaces, trials, accepted):
     """Algorithmic HBM bytes of one persistent apply launch (every colour pass
     of every cycle).
@@ -211,10 +224,14 @@ def other_workloads(hmc, wl, cpu=True, only=None, fp64=True):
         if pn and st["tiled"] == 2:
             ms = pm / pn
             S = 134 * 134
-            b = apply_bytes(len(pos), S, (acc + rej) / 400, acc / 400)
+            cyc = st["cycles"] / max(st["applies"], 1)
+            b = model_apply_bytes(len(pos), S, cyc, acc / 400, st["band_retests"] / 420, dim=2)
+            bp = apply_bytes(len(pos), S, (acc + rej) / 400, acc / 400)
             r["roofline"] = {"bound": "hbm", "achieved": b / (ms * 1e-3) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                              "frac": b / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, "traffic": None,
-                             "kernel": "k_apply_tile<2>", "avg_launch_ms": ms, "algorithmic_bytes_per_launch": b}
+                             "kernel": "k_apply_tile<2>", "avg_launch_ms": ms, "algorithmic_bytes_per_launch": b,
+                             "persistent_model_bytes_per_launch": bp,
+                             "persistent_model_frac": bp / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
         sim.close()
         if cpu:
             mk = lambda: O.OracleSystem(L, wl.wrap(L, pos), seed=42)
@@ -598,8 +615,13 @@ def main():
     acc_per_pass = mixed["accepted"] / passes
     if mixed["pass_n"]:
         avg_ms = mixed["pass_ms"] / mixed["pass_n"]
+        extra = {}
         if mixed["stats"]["tiled"] == 2:
-            b = apply_bytes(n, n_spaces, trials / args.steps, mixed["accepted"] / args.steps)
+            retests = mixed["stats"]["band_retests"] / max(mixed["stats"]["applies"], 1)
+            b = model_apply_bytes(n, n_spaces, cycles, mixed["accepted"] / args.steps, retests)
+            bp = apply_bytes(n, n_spaces, trials / args.steps, mixed["accepted"] / args.steps)
+            extra = {"persistent_model_bytes_per_launch": bp,
+                     "persistent_model_frac": bp / (avg_ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
             kernel = "k_apply_tile<3>"
         else:
             b = pass_bytes("mixed", n, n_spaces, nonempty, acc_per_pass)
@@ -609,7 +631,7 @@ def main():
                            "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(kernel),
                            "kernel": kernel, "avg_launch_ms": avg_ms,
                            "algorithmic_bytes_per_launch": b, "launches": mixed["pass_n"],
-                           "build_ms_per_apply": mixed["build_ms"] / max(mixed["build_n"], 1)}
+                           "build_ms_per_apply": mixed["build_ms"] / max(mixed["build_n"], 1), **extra}
     out["sdf"] = mixed["sdf"]  # the scale distribution function of the final state (hmc_sdf), timed apart
     # FP32 error of the pair test against FP64: every pair within 1.01 sigma
     # of four applies measured both ways (a diagnostic run on the per-colour

@@ -69,6 +69,38 @@ def test_insert_energy_matches_oracle(hmc, wl, dim, n, L, model, params, precisi
     gpu.close()
 
 
+@pytest.mark.parametrize("model,params,precision", [
+    ("hard_sphere", (1.0,), "mixed"),
+    ("lj_shifted", (1.0, 1.0, 2.5, 2.5), "mixed"),
+])
+def test_insert_energy_after_applies(hmc, wl, model, params, precision):
+    # insertion energies of a state the device has moved: the mixed
+    # persistent applies write the FP64 sites with device-scope stores
+    L = [16.0, 16.0, 16.0]
+    pos = wl.random_dilute(1500, L, 3, 1.0, 5)
+    ref = O.OracleSystem(L, pos, model=model, params=params, seed=6)
+    b, s = ref.positions()
+    gpu = hmc.GpuParallelSweep(L, b, sites=s, model=model, params=params, precision=precision)
+    kT = 1.0 if model == "hard_sphere" else 1.5
+    for _ in range(4):
+        step, sub = ref.counter()
+        g = gpu.apply(step, sub, 6, kT, 0.15)
+        assert g[:2] == ref.parallel_apply(kT, 0.15)
+        ref.increment_step()
+    assert all(np.array_equal(x.view(np.uint64), y.view(np.uint64)) for x, y in zip(gpu.positions(),
+                                                                                 ref.positions()))
+    rng = np.random.default_rng(3)
+    cand = rng.uniform(-8.0, 8.0, size=(3000, 3))
+    e, bound = gpu.delta_energy_insert(cand)
+    want = np.array([ref.delta_energy_insert(c) for c in cand])
+    inf = np.isinf(want)
+    assert np.array_equal(np.isinf(e), inf)
+    assert (np.abs(e[~inf] - want[~inf]) <= bound[~inf]).all()
+    if model == "hard_sphere":
+        assert np.array_equal(e, want)
+    gpu.close()
+
+
 def test_insert_energy_rejects_convex_and_handles_empty(hmc, wl):
     L = [12.0, 12.0, 12.0]
     pos = wl.random_dilute(100, L, 3, 1.8, 4)

@@ -88,9 +88,11 @@ def test_tune_c2_default_options(hmc, wl):
     sim.close()
 
 
-@pytest.mark.parametrize("kind,fugacity", [("hs3d", 0.5), ("lj3d", 0.03)])
-def test_grand_canonical_with_sweeps_matches_oracle(hmc, wl, kind, fugacity):
+@pytest.mark.parametrize("kind,fugacity,precision", [("hs3d", 0.5, "fp64"), ("hs3d", 0.5, "mixed"),
+                                                      ("lj3d", 0.03, "fp64")])
+def test_grand_canonical_with_sweeps_matches_oracle(hmc, wl, kind, fugacity, precision):
     L, pos, gk, ok, kT = _system(wl, kind)
+    gk = dict(gk, precision=precision)
     body = wl.wrap(L, pos)
     if kind == "hs3d":  # a fluid at packing fraction 0.1
         L, pos = wl.fcc_spheres((4, 4, 4), packing_fraction=0.10)
