@@ -34,7 +34,7 @@ HMC_MIXED = 1
 # every entry point declared in include/hmc.h
 EXPORTS = ["hmc_create", "hmc_set_state", "hmc_get_state", "hmc_clone_with_boundary", "hmc_apply_translate",
            "hmc_quick_compress_init", "hmc_quick_compress_apply", "hmc_quick_insert_init",
-           "hmc_quick_insert_apply", "hmc_delta_energy_insert", "hmc_tune_translate", "hmc_grand_canonical_apply", "hmc_gsd_create", "hmc_gsd_open",
+           "hmc_quick_insert_apply", "hmc_delta_energy_insert", "hmc_tune_local", "hmc_grand_canonical_apply", "hmc_gsd_create", "hmc_gsd_open",
            "hmc_gsd_write_chunk", "hmc_gsd_end_frame", "hmc_gsd_append_frame", "hmc_gsd_n_frames",
            "hmc_gsd_read_chunk", "hmc_gsd_header", "hmc_gsd_close",
            "hmc_apply_rotate",
@@ -134,8 +134,8 @@ def lib():
         L.hmc_quick_insert_init.restype = None
         L.hmc_quick_insert_apply.argtypes = [C.POINTER(QuickInsertState), C.POINTER(vp), C.c_uint64,
                                              C.c_uint32, C.c_uint32, C.POINTER(Count)]
-        L.hmc_tune_translate.argtypes = [vp, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_uint64,
-                                         C.c_uint32, D_P, C.POINTER(Count), C.POINTER(C.c_uint64)]
+        L.hmc_tune_local.argtypes = [vp, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
+                                     C.c_uint64, C.c_uint32, D_P, C.POINTER(Count), C.POINTER(C.c_uint64)]
         L.hmc_grand_canonical_apply.argtypes = [C.POINTER(vp), C.c_uint64, C.c_uint32, C.c_uint32, C.c_double,
                                                 C.c_double, C.POINTER(GcCount), C.POINTER(C.c_uint64)]
         L.hmc_gsd_create.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_uint32, C.c_uint32, C.POINTER(vp)]
@@ -375,17 +375,23 @@ class GpuParallelSweep:
         self._check(lib().hmc_run_translate(self.h, step0, seed, kT, max_distance, n_steps, C.byref(cnt)))
         return int(cnt.accepted), int(cnt.rejected)
 
-    def tune_translate(self, step, substep, seed, kT, target, samples, steps, max_move):
-        """ParallelSweep::tune with Translate (tune_local.rs:21-89) against the
-        current state, from Counter(step, substep, seed).make_rng(): returns
-        (the tuned maximum distance, [(accepted, rejected)] per step,
-        ambiguous samples).  The state is not modified."""
-        m = C.c_double(max_move)
+    def tune(self, move, step, substep, seed, kT, target, samples, steps, size):
+        """ParallelSweep::tune (tune_local.rs:21-89) with move "translate" or
+        (convex) "rotate" against the current state, from
+        Counter(step, substep, seed).make_rng(): returns (the tuned maximum
+        size, [(accepted, rejected)] per step, ambiguous samples).  The state
+        is not modified."""
+        m = C.c_double(size)
         cnt = (Count * max(int(steps), 1))()
         amb = C.c_uint64()
-        self._check(lib().hmc_tune_translate(self.h, step, substep, seed, kT, target, int(samples), int(steps),
-                                             C.byref(m), cnt, C.byref(amb)))
+        mv = {"translate": HMC_MOVE_TRANSLATE, "rotate": HMC_MOVE_ROTATE}[move]
+        self._check(lib().hmc_tune_local(self.h, mv, step, substep, seed, kT, target, int(samples), int(steps),
+                                         C.byref(m), cnt, C.byref(amb)))
         return m.value, [(int(c.accepted), int(c.rejected)) for c in cnt[:steps]], int(amb.value)
+
+    def tune_translate(self, step, substep, seed, kT, target, samples, steps, max_move):
+        """tune("translate", ...)."""
+        return self.tune("translate", step, substep, seed, kT, target, samples, steps, max_move)
 
     def grand_canonical(self, step, substep, seed, kT, fugacity):
         """GrandCanonical(UniformIn over the box)::apply (grand_canonical.rs:145-205)

@@ -395,18 +395,19 @@ int hmc_quick_insert_apply(hmc_quick_insert* qi, hmc_ctx** pc, uint64_t step, ui
 }
 
 // ------------------------------------------------------------ tune_local --
-int hmc_tune_translate(hmc_ctx* c, uint64_t step, uint32_t substep, uint32_t seed, double kT, double target,
-                       uint64_t samples, uint32_t steps, double* maximum_distance, hmc_count* counts,
-                       uint64_t* ambiguous) {
-  if (!c || !maximum_distance) return HMC_ERR_INVALID;
+int hmc_tune_local(hmc_ctx* c, uint32_t move, uint64_t step, uint32_t substep, uint32_t seed, double kT,
+                   double target, uint64_t samples, uint32_t steps, double* maximum_size, hmc_count* counts,
+                   uint64_t* ambiguous) {
+  if (!c || !maximum_size) return HMC_ERR_INVALID;
   if (!c->have_state) return fail(c, HMC_ERR_STATE, "no state");
   const Geometry& g = c->g;
   const int D = g.dim;
-  if (!is_isotropic(g.model) && g.model != HMC_HARD_SPHERE)
-    return fail(c, HMC_ERR_UNSUPPORTED, "tuning of oriented bodies is not offloaded");
-  if (!(target > 0.0 && target < 1.0) || !(*maximum_distance > 0.0) || !std::isfinite(*maximum_distance) ||
-      !(kT > 0.0))
-    return fail(c, HMC_ERR_INVALID, "tune: target in (0, 1), maximum distance and kT > 0");
+  const bool convex = g.model == HMC_CONVEX_POLYHEDRON;
+  if (move != HMC_MOVE_TRANSLATE && move != HMC_MOVE_ROTATE) return fail(c, HMC_ERR_INVALID, "tune: unknown move");
+  if (move == HMC_MOVE_ROTATE && !convex) return fail(c, HMC_ERR_INVALID, "tune: point bodies have no orientation");
+  if (!(target > 0.0 && target < 1.0) || !(*maximum_size > 0.0) || !std::isfinite(*maximum_size) || !(kT > 0.0))
+    return fail(c, HMC_ERR_INVALID, "tune: target in (0, 1), maximum size and kT > 0");
+  double* const maximum_distance = maximum_size;
   if (g.n_bodies == 0) return fail(c, HMC_ERR_INVALID, "tune: no bodies to sample");
   if (samples > (1ull << 26)) return fail(c, HMC_ERR_INVALID, "tune: at most 2^26 samples per step");
   if (ambiguous) *ambiguous = 0;
@@ -457,7 +458,10 @@ int hmc_tune_translate(hmc_ctx* c, uint64_t step, uint32_t substep, uint32_t see
       for (uint64_t i = 0; i < M; ++i) hx[i] = r.next();
       hipError_t e = hipMemcpyAsync(dx, hx.data(), M * sizeof(uint64_t), hipMemcpyHostToDevice, s);
       if (e == hipSuccess) {
-        if (D == 2)
+        if (convex)
+          k_tune_eval_convex<<<grid_for(M, 256), 256, 0, s>>>(g, c->st, c->bodies, c->w, dx, (uint32_t)M, move,
+                                                               *maximum_distance, dn, df);
+        else if (D == 2)
           k_tune_eval<2><<<grid_for(M, kEnergyBlock), kEnergyBlock, 0, s>>>(g, c->st, c->bodies, c->w, dx,
                                                                              (uint32_t)M, *maximum_distance, kT,
                                                                              dn, df);
@@ -505,11 +509,13 @@ int hmc_tune_translate(hmc_ctx* c, uint64_t step, uint32_t substep, uint32_t see
       counts[t].rejected = rej;
     }
     // tune_by_scaling (hoomd-mc/src/lib.rs:514-524), Translate::adjust
-    // (translate.rs:87-92)
+    // (translate.rs:87-92), Rotate<Versor>::adjust (rotate/versor.rs:125-137:
+    // at most pi/2)
     if (acc + rej > 0) {
       const double ratio = (double)acc / (double)(acc + rej);
       const double scale = (ratio + 1.5) / (target + 1.5);
       *maximum_distance *= scale;
+      if (move == HMC_MOVE_ROTATE && *maximum_distance > 1.5707963267948966) *maximum_distance = 1.5707963267948966;
     }
   }
   release();

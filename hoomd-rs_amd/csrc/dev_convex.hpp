@@ -194,7 +194,8 @@ __device__ __forceinline__ bool convex_overlap(const double* vx, uint32_t nv, do
 }
 
 // VersorDisplacement::sample (hoomd-mc/src/rotate/versor.rs:39-72).
-__device__ __forceinline__ quat versor_displacement(Sfc64& rng, double sigma, const double* zx,
+template <class G>
+__device__ __forceinline__ quat versor_displacement(G& rng, double sigma, const double* zx,
                                                     const double* zf) {
   const double PI = 3.141592653589793;
   for (;;) {

@@ -643,6 +643,85 @@ __global__ void __launch_bounds__(kGuardThreads) k_guard_convex(Geometry g, Appl
   }
 }
 
+// tune_local_trial (hoomd-mc/src/tune_local.rs:21-89) for hard convex
+// polyhedra with Translate (move 0) or Rotate<Versor> (move 1): one sample
+// from every offset p of the pre-drawn stream xs[0, M), as k_tune_eval does
+// for point bodies.  The trial site is tested against the sites of every
+// other body within the circumsphere cutoff, from the cells of the current
+// state around the trial site's space; hard shapes accept without a draw.
+// next[p]: the offset after the sample (UINT32_MAX: the stream ran out);
+// flags[p]: bit 0 accepted, bit 2 a site outside the checkerboard.
+__global__ void __launch_bounds__(256) k_tune_eval_convex(Geometry g, const ApplyState* st, Bodies B, Work w,
+                                                          const uint64_t* __restrict__ xs, uint32_t M, uint32_t move,
+                                                          double size, uint32_t* __restrict__ next,
+                                                          uint8_t* __restrict__ flags) {
+  __shared__ double zx[257], zf[257];
+  __shared__ double vx[3 * kMaxVertices];
+  stage_zig(zx, zf);
+  stage_verts(g, vx);
+  __syncthreads();
+  const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= M) return;
+  StreamRng r{xs, p, M, false};
+  const uint32_t body = choose_index(r, g.n_bodies);
+  const double4 bp = B.bpos[body];
+  const double4 bq = B.bquat[body];
+  double x[3] = {bp.x, bp.y, bp.z};
+  quat q = {bq.x, {bq.y, bq.z, bq.w}};
+  if (move == 0) {
+    double v[3] = {0.0, 0.0, 0.0};
+    ball_sample<3>(r, size, zx, zf, v);
+    for (int a = 0; a < 3; ++a) x[a] += v[a];
+  } else {
+    q = qmul(versor_displacement(r, size, zx, zf), q);  // rotate/versor.rs:111-122
+  }
+  for (int a = 0; a < 3; ++a) x[a] = wrap1(x[a], g.L[a]);  // wrap(propose(..))
+  // the trial site: wrap(x + rotate(q, 0)), orientation q * identity
+  const v3 r0 = qrotate(q, v3{0.0, 0.0, 0.0});
+  const double t[3] = {wrap1(x[0] + r0.x, g.L[0]), wrap1(x[1] + r0.y, g.L[1]), wrap1(x[2] + r0.z, g.L[2])};
+  const quat qt = qmul(q, quat{1.0, {0.0, 0.0, 0.0}});
+  const v3 tv = {t[0], t[1], t[2]};
+  bool hit = false, outside = false;
+  const uint32_t s = space_of<3>(g, st->origin, t);
+  if (s == UINT32_MAX) {
+    outside = true;
+  } else {
+    int32_t m[3];
+    for (int a = 0; a < 3; ++a) m[a] = (int32_t)((s / g.stride[a]) % (uint32_t)g.shape[a]);
+    uint32_t nb[3][5];
+    int nn[3] = {1, 1, 1};
+    for (int a = 0; a < 3; ++a) {
+      const int32_t sh = g.shape[a], rr = g.reach[a];
+      int k = 0;
+      for (int off = -rr; off <= rr; ++off) {
+        const int32_t mm = ((m[a] + off) % sh + sh) % sh;
+        bool seen = false;
+        for (int j = 0; j < k; ++j) seen |= nb[a][j] == (uint32_t)mm * g.stride[a];
+        if (!seen) nb[a][k++] = (uint32_t)mm * g.stride[a];
+      }
+      nn[a] = k;
+    }
+    for (int i0 = 0; i0 < nn[0] && !hit; ++i0)
+      for (int i1 = 0; i1 < nn[1] && !hit; ++i1)
+        for (int i2 = 0; i2 < nn[2] && !hit; ++i2) {
+          const uint32_t ns = nb[0][i0] + nb[1][i1] + nb[2][i2];
+          const uint32_t q0 = w.start[ns], q1 = w.start[ns + 1];
+          for (uint32_t qq = q0; qq < q1 && !hit; ++qq) {
+            const uint32_t bj = w.tmp_body[qq];
+            if (bj == body) continue;
+            v3 rj;
+            const double d2 = convex_image(g, t, B.spos[bj], &rj);
+            if (d2 < g.cv_rcut2) {
+              const double4 qj4 = B.squat[bj];
+              hit = convex_overlap(vx, g.nverts, g.cv_rcut2, d2, tv, qt, rj, quat{qj4.x, {qj4.y, qj4.z, qj4.w}});
+            }
+          }
+        }
+  }
+  next[p] = r.over ? 0xFFFFFFFFu : r.pos;
+  flags[p] = (uint8_t)((hit ? 0u : 1u) | (outside ? 4u : 0u));
+}
+
 // Site orientations: given, or body * identity (oriented_point.rs:107-113).
 __global__ void k_init_site_quats(Geometry g, Bodies B, int have_sites) {
   const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;

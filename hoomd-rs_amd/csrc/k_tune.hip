@@ -4,20 +4,6 @@
 
 namespace hmc {
 
-// A pre-drawn stream read from a given offset.  Past its end it reports
-// `over` and returns a word with which every sampler returns at once (a
-// uniform of 1/2, a ziggurat draw in layer 1 with u = 0, an index of 0).
-struct StreamRng {
-  const uint64_t* x;
-  uint32_t pos, end;
-  bool over;
-  __device__ __forceinline__ uint64_t next() {
-    if (pos < end) return x[pos++];
-    over = true;
-    return 0x8000000000000001ull;
-  }
-  __device__ __forceinline__ uint32_t next_u32() { return (uint32_t)next(); }
-};
 
 // Energy of site t with every site of another body than `own` (UINT32_MAX:
 // none excluded), over the neighbour spaces of t's space in the cells of the

@@ -33,6 +33,7 @@ __global__ void k_guard_lj(Geometry g, ApplyState* st, Bodies B, Cells C, Work w
 template <bool MIXED>
 __global__ void k_pass_convex(Geometry g, ApplyState* st, Bodies B, Cells C, Work w, uint32_t colour, uint32_t cycle, uint32_t xcd_chunk, SatShape sat);
 __global__ void k_guard_convex(Geometry g, ApplyState* st, Bodies B, Cells C, Work w, uint32_t first_cycle);
+__global__ void k_tune_eval_convex(Geometry g, const ApplyState* st, Bodies B, Work w, const uint64_t* __restrict__ xs, uint32_t M, uint32_t move, double size, uint32_t* __restrict__ next, uint8_t* __restrict__ flags);
 __global__ void k_init_site_quats(Geometry g, Bodies B, int have_sites);
 __global__ void k_debug_convex(int kind, const double* vx, uint32_t nv, double rcut2, const double* in, double* out, uint64_t n, SatShape sat);
 __global__ void k_bsetup(const BatchRep* reps, ApplyState* sts, uint32_t R, const uint64_t* step, const uint32_t* seed, double kT, const double* max_move, int from_args, int advance);

@@ -253,6 +253,21 @@ __device__ __forceinline__ void wave_count(unsigned long long* dst, bool flag) {
   int leader = __ffsll((long long)__ballot(1)) - 1;
   if (m && (int)__lane_id() == leader) atomicAdd(dst, (unsigned long long)__popcll(m));
 }
+// A pre-drawn stream read from a given offset.  Past its end it reports
+// `over` and returns a word with which every sampler returns at once (a
+// uniform of 1/2, a ziggurat draw in layer 1 with u = 0, an index of 0).
+struct StreamRng {
+  const uint64_t* x;
+  uint32_t pos, end;
+  bool over;
+  __device__ __forceinline__ uint64_t next() {
+    if (pos < end) return x[pos++];
+    over = true;
+    return 0x8000000000000001ull;
+  }
+  __device__ __forceinline__ uint32_t next_u32() { return (uint32_t)next(); }
+};
+
 // Per-lane add of a rare count.
 __device__ __forceinline__ void lane_add(unsigned long long* dst, unsigned v) {
   if (v) atomicAdd(dst, (unsigned long long)v);

@@ -242,22 +242,24 @@ void hmc_quick_insert_init(hmc_quick_insert* qi, uint64_t target);
 int hmc_quick_insert_apply(hmc_quick_insert* qi, hmc_ctx** ctx, uint64_t step, uint32_t substep, uint32_t seed,
                            hmc_count* count);
 
-/* ParallelSweep::tune / Sweep::tune with Translate (hoomd-mc/src/tune_local.rs:21-89,
- * parallel_sweep.rs:486-506): `steps` rounds of `samples` trial moves of
- * random bodies against the unchanged microstate, from one stream
- * Counter(step, substep, seed).make_rng(); after each round
- * tune_by_scaling (lib.rs:514-524) multiplies *maximum_distance by
- * (ratio + 1.5) / (target + 1.5).  The state is not modified and no substep
- * is consumed.  counts (optional, `steps` entries) get each round's accepted
+/* ParallelSweep::tune / Sweep::tune (hoomd-mc/src/tune_local.rs:21-89,
+ * parallel_sweep.rs:486-506) with Translate (move HMC_MOVE_TRANSLATE) or,
+ * for convex polyhedra, Rotate<Versor> (HMC_MOVE_ROTATE): `steps` rounds of
+ * `samples` trial moves of random bodies against the unchanged microstate,
+ * from one stream Counter(step, substep, seed).make_rng(); after each round
+ * tune_by_scaling (lib.rs:514-524) multiplies *maximum_size by
+ * (ratio + 1.5) / (target + 1.5) (a rotation size at most pi/2,
+ * rotate/versor.rs:125-137).  The state is not modified and no substep is
+ * consumed.  counts (optional, `steps` entries) get each round's accepted
  * and rejected samples; *ambiguous (optional) the samples whose draw or
  * decision the bound on the summation order could not fix (decided by the
  * GPU's dH; expected 0).  Every sample of a round is evaluated on the device
  * at once: the round's stream is drawn on the host, one lane takes a sample
- * from every offset of it (k_tune_eval), and the samples are the chain of
- * offsets from 0.  Point bodies only. */
-int hmc_tune_translate(hmc_ctx* ctx, uint64_t step, uint32_t substep, uint32_t seed, double kT, double target,
-                       uint64_t samples, uint32_t steps, double* maximum_distance, hmc_count* counts,
-                       uint64_t* ambiguous);
+ * from every offset of it (k_tune_eval, k_tune_eval_convex), and the samples
+ * are the chain of offsets from 0. */
+int hmc_tune_local(hmc_ctx* ctx, uint32_t move, uint64_t step, uint32_t substep, uint32_t seed, double kT,
+                   double target, uint64_t samples, uint32_t steps, double* maximum_size, hmc_count* counts,
+                   uint64_t* ambiguous);
 
 /* GrandCanonical(UniformIn over the periodic box)::apply
  * (hoomd-mc/src/grand_canonical.rs:145-205): one insertion or removal trial,

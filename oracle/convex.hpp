@@ -429,6 +429,42 @@ struct ConvexSystem {
     }
   }
 
+  // tune_local_trial (hoomd-mc/src/tune_local.rs:21-89) with Translate or
+  // Rotate<Versor>, all bodies eligible, then tune_by_scaling (lib.rs:514-524)
+  // and the trial's adjust (translate.rs:87-92; rotate/versor.rs:125-137: a
+  // rotation size at most pi/2).  Hard shapes: no acceptance draw.
+  void tune(bool rotate, double target, uint64_t samples, uint32_t steps, double& size, uint64_t* counts) const {
+    Sfc64 rng = counter_rng(step, substep, seed, 0);
+    const uint64_t n = bpos.size();
+    for (uint32_t t = 0; t < steps; ++t) {
+      uint64_t acc = 0, rej = 0;
+      while (acc + rej < samples) {
+        const uint64_t bi = rand_index(rng, (uint32_t)n);
+        V3 np = bpos[bi];
+        Quat nq = bq[bi];
+        if (rotate)
+          nq = qmul(versor_displacement(rng, size), nq);
+        else
+          np = System<3>::propose_translate(rng, np, size);
+        np = boundary.wrap(np);
+        V3 sp;
+        Quat sq;
+        site_of(np, nq, sp, sq);
+        if (overlaps(sp, sq, bi))
+          ++rej;
+        else
+          ++acc;
+      }
+      counts[2 * t] = acc;
+      counts[2 * t + 1] = rej;
+      if (acc + rej > 0) {
+        const double ratio = (double)acc / (double)(acc + rej);
+        size *= (ratio + 1.5) / (target + 1.5);
+        if (rotate && size > M_PI / 2.0) size = M_PI / 2.0;
+      }
+    }
+  }
+
   void update_bodies(uint64_t& acc, uint64_t& rej) {
     for (const BodyTrial& bt : trials) {
       if (bt.status == Rejected) {

@@ -512,6 +512,11 @@ void* oracle_convex_create(const double* L, double boundary_range, double body_r
   return s;
 }
 void oracle_convex_destroy(void* h) { delete (ConvexSystem*)h; }
+// tune_local with Translate (rotate 0) or Rotate<Versor> (1); counts = 2 per step
+void oracle_convex_tune(void* h, int rotate, double target, uint64_t samples, uint32_t steps, double* size,
+                        uint64_t* counts) {
+  ((ConvexSystem*)h)->tune(rotate != 0, target, samples, steps, *size, counts);
+}
 int oracle_convex_apply(void* h, int rotate, double max_move, uint64_t* acc, uint64_t* rej) {
   return ((ConvexSystem*)h)->parallel_apply(rotate != 0, max_move, *acc, *rej) ? 0 : -1;
 }

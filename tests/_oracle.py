@@ -361,6 +361,8 @@ def _convex_lib():
         L.oracle_convex_create.restype = C.c_void_p
         L.oracle_convex_destroy.argtypes = [C.c_void_p]
         L.oracle_convex_apply.argtypes = [C.c_void_p, C.c_int, C.c_double, U64_P, U64_P]
+        L.oracle_convex_tune.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_uint64, C.c_uint32, D_P, U64_P]
+        L.oracle_convex_tune.restype = None
         L.oracle_convex_apply.restype = C.c_int
         L.oracle_convex_increment_step.argtypes = [C.c_void_p]
         L.oracle_convex_get_counter.argtypes = [C.c_void_p, U64_P, U32_P]
@@ -490,6 +492,15 @@ class ConvexOracle:
 
     def any_overlap(self):
         return bool(_convex_lib().oracle_convex_any_overlap(self.h))
+
+    def tune(self, rotate, target, samples, steps, size):
+        """ParallelSweep::tune with Translate or Rotate<Versor> (tune_local.rs:21-89):
+        returns (the tuned size, [(accepted, rejected)] per step)."""
+        m = np.array([size], np.float64)
+        cnt = np.zeros(2 * steps, np.uint64)
+        _convex_lib().oracle_convex_tune(self.h, 1 if rotate else 0, target, samples, steps, _p(m, D_P),
+                                         _p(cnt, U64_P))
+        return float(m[0]), [(int(cnt[2 * t]), int(cnt[2 * t + 1])) for t in range(steps)]
 
     def trace(self, on=True):
         _convex_lib().oracle_convex_trace_enable(self.h, 1 if on else 0)

@@ -121,3 +121,32 @@ def test_grand_canonical_with_sweeps_matches_oracle(hmc, wl, kind, fugacity, pre
     assert np.array_equal(b0.view(np.uint64), b1.view(np.uint64))
     assert np.array_equal(s0.view(np.uint64), s1.view(np.uint64))
     sim.close()
+
+
+@pytest.mark.parametrize("move", ["translate", "rotate"])
+def test_tune_convex_matches_oracle(hmc, wl, move):
+    # 216 unit cubes, simple cubic at packing fraction 0.55, orientations
+    # randomised by rotation sweeps on both sides first
+    n = 6
+    Lb = (n**3 / 0.55) ** (1.0 / 3.0)
+    a = Lb / n
+    g = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    L = np.array([Lb, Lb, Lb])
+    pos = (g + 0.5) * a - Lb / 2
+    q = np.tile([1.0, 0.0, 0.0, 0.0], (len(pos), 1))
+    ref = O.ConvexOracle(L, pos, q, O.CUBE, seed=13)
+    bp, bq, sp, sq = ref.state()
+    gpu = hmc.GpuParallelSweep(L, bp, sites=sp, model="convex", vertices=O.CUBE, orientations=bq,
+                               site_orientations=sq)
+    for _ in range(4):
+        step, sub = ref.counter()
+        gg = gpu.apply_rotate(step, sub, 13, 1.0, 0.5)
+        assert gg[:2] == ref.apply(True, 0.5)
+        ref.increment_step()
+    step, sub = ref.counter()
+    size = 0.05 if move == "translate" else 0.1
+    m_ref, c_ref = ref.tune(move == "rotate", 0.3, 2000, 6, size)
+    m_gpu, c_gpu, amb = gpu.tune(move, step, sub, 13, 1.0, 0.3, 2000, 6, size)
+    log(f"tune convex {move}: {size} -> {m_gpu:.6f}, counts {c_gpu}")
+    assert c_gpu == c_ref and m_gpu == m_ref and amb == 0
+    gpu.close()

@@ -96,3 +96,23 @@ def test_gc_ideal_gas_is_poisson():
     ns = np.array(ns, float)
     assert abs(ns.mean() - 20.0) < 1.0
     assert abs(ns.var() - 20.0) < 6.0
+
+
+def test_convex_tune_closed_forms():
+    # unit cubes 3 apart (circumsphere diameter 1.73): no move of these sizes
+    # makes an overlap, so every sample is accepted; a rotation size stops at
+    # pi/2 (rotate/versor.rs:125-137)
+    L, pos = _lattice(4, 3.0, 3)
+    q = np.tile([1.0, 0.0, 0.0, 0.0], (len(pos), 1))
+    ref = O.ConvexOracle(L, pos, q, O.CUBE, seed=3)
+    m, counts = ref.tune(True, 0.2, 200, 6, 1.0)
+    assert counts == [(200, 0)] * 6
+    want = 1.0
+    for _ in range(6):
+        want = min(want * (2.5 / 1.7), np.pi / 2.0)
+    assert m == want == np.pi / 2.0
+    m, counts = ref.tune(False, 0.2, 200, 4, 0.1)
+    want = 0.1
+    for _ in range(4):
+        want *= 2.5 / 1.7
+    assert counts == [(200, 0)] * 4 and m == want
