@@ -365,7 +365,10 @@ def other_workloads(hmc, wl, cpu=True, only=None, fp64=True):
         res["C3_cubes_262144"] = r
     if want("c5"):
         # C5: 256 replicas of hard spheres (FCC, N 2,048 .. 62,500, packing
-        # fractions 0.30 .. 0.54), seeds 42 + r, d = 0.08, one graph per step
+        # fractions 0.30 .. 0.54), seeds 42 + r, d = 0.08; mixed precision
+        # runs persistent applies (one cooperative launch per group of
+        # replicas per step), FP64 the per-colour tile passes (one graph per
+        # step)
         systems = wl.batched_replicas()
         batch = hmc.GpuBatch([(L, wl.wrap(L, p), None) for L, p in systems], precision="mixed")
         seeds = [42 + r for r in range(len(systems))]
@@ -379,8 +382,10 @@ def other_workloads(hmc, wl, cpu=True, only=None, fp64=True):
         acc = sum(a for a, _ in cnt)
         v = t / el
         nb = int(sum(len(p) for _, p in systems))
+        layout = {2: "persistent", 1: "tiles", 0: "lanes"}[batch.stats(0)["tiled"]]
         r = {"value": v, "unit": "trial moves/s", "steps": 40, "precision": "mixed", "bodies": nb,
-             "pass_layout": "tiles" if batch.stats(0)["tiled"] else "lanes", "autotuned_policy_cus": policy}
+             "pass_layout": layout, "autotuned_policy_cus": policy}
+        final_mixed = [batch.positions(i)[1] for i in range(0, len(systems), 51)]
         # whole-step roofline: the build (bodies read twice, 4 index arrays
         # written and read) and the passes (every colour pass reads every
         # site copy and cell offset once) against the step's wall time
@@ -414,7 +419,17 @@ def other_workloads(hmc, wl, cpu=True, only=None, fp64=True):
             v64, _ = timed(lambda s0, k: sum(a + b for a, b in batch.run(s0, seeds, 1.0, 0.08, k)), 2, 40)
             r["fp64_value"] = v64
             r["mixed_over_fp64"] = v / v64
+            r["identical_to_fp64"] = all(np.array_equal(batch.positions(i)[1].view(np.uint64), x.view(np.uint64))
+                                         for i, x in zip(range(0, len(systems), 51), final_mixed))
             batch.close()
+            if layout == "persistent":
+                # the layout's share: mixed precision on the per-colour tile passes
+                batch = hmc.GpuBatch([(L, wl.wrap(L, p), None) for L, p in systems], precision="mixed",
+                                     persistent=False)
+                vpc, _ = timed(lambda s0, k: sum(a + b for a, b in batch.run(s0, seeds, 1.0, 0.08, k)), 2, 40)
+                r["mixed_percolour_value"] = vpc
+                r["mixed_over_fp64_same_layout"] = vpc / v64
+                batch.close()
         if cpu:
             # every 16th replica, each for a share of the sample, one after
             # another with all cores (the replicas are independent)

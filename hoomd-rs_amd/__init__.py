@@ -556,10 +556,13 @@ class GpuBatch:
     """Independent hard-sphere/disk replicas advanced together (one set of
     launches per step, replayed from a captured hipGraph)."""
 
-    def __init__(self, systems, *, diameter=1.0, precision="fp64", device=0, lds_tiles=True):
+    def __init__(self, systems, *, diameter=1.0, precision="fp64", device=0, lds_tiles=True, persistent=True):
         """systems: list of (box_edges, body_positions, site_positions or None).
         lds_tiles=False runs every pass a lane per space from global memory
-        instead of on LDS tiles (same decisions)."""
+        instead of on LDS tiles (same decisions).  persistent=False keeps a
+        mixed-precision 3D batch on one launch per colour pass instead of
+        persistent applies (one cooperative launch per group of replicas per
+        step, not graph-captured; same decisions)."""
         self.R = len(systems)
         cfgs = (Config * self.R)()
         self.dims = []
@@ -575,7 +578,7 @@ class GpuBatch:
             c.params[0] = diameter
             c.precision = {"fp64": HMC_FP64, "mixed": HMC_MIXED}[precision]
             c.device = device
-            c.flags = 0 if lds_tiles else FLAG_GLOBAL_PASS
+            c.flags = (0 if lds_tiles else FLAG_GLOBAL_PASS) | (0 if persistent else FLAG_PER_COLOUR)
             self.dims.append((pos.shape[0], pos.shape[1]))
         h = C.c_void_p()
         rc = lib().hmc_batch_create(self.R, cfgs, C.byref(h))

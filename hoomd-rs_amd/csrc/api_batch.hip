@@ -31,6 +31,21 @@ void launch_batch_step(hmc_batch* b, int from_args, int advance, uint32_t launch
   k_scan_add<<<grid_for(b->s_total, 256), 256, 0, s>>>(b->w.start, b->w.block_sums, nullptr, b->s_total);
   k_bcycles<<<grid_for(b->R, 64), 64, 0, s>>>(b->reps, b->sts, b->R);
   k_scatter<<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->gtot, b->w);
+  if constexpr (D == 3 && PREC == HMC_MIXED) {
+    if (b->persist) {
+      // the persistent applies stage their sites from the FP64 copies: one
+      // cooperative launch runs every cycle of every replica, round by round
+      k_place<PREC, false><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->gtot, b->bodies, b->cells, b->w);
+      uint32_t nrounds = (uint32_t)b->grp_tile0.size(), nwg = b->nwg, chunk = (nwg + 7) / 8, probe = 0;
+      void* args[] = {(void*)&b->reps, (void*)&b->dptcs, (void*)&b->dptoff, (void*)&b->dptrep, (void*)&b->sts,
+                      (void*)&b->drt0, (void*)&b->drn,   (void*)&nrounds,   (void*)&nwg,       (void*)&chunk,
+                      (void*)&probe,   (void*)&b->bodies, (void*)&b->cells, (void*)&b->w};
+      const hipError_t e = hipLaunchCooperativeKernel((const void*)k_bapply_tile<3>, dim3(chunk * 8),
+                                                      dim3(b->pthreads), args, b->plds, s);
+      if (e != hipSuccess && b->launch_err == hipSuccess) b->launch_err = e;
+      return;
+    }
+  }
   k_place<PREC><<<grid_for(b->n_total, kBlock), kBlock, 0, s>>>(b->gtot, b->bodies, b->cells, b->w);
   const uint32_t blocks = grid_for(b->pc_total, kBlock);
   const uint32_t chunk = (blocks + 7) / 8;
@@ -69,8 +84,12 @@ int batch_read(hmc_batch* b, std::vector<ApplyState>& st, std::vector<unsigned l
   BHIPCHK(b, hipMemcpyAsync(acc.data(), b->w.acc_slots, b->R * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                             b->stream));
   BHIPCHK(b, hipStreamSynchronize(b->stream));
-  for (uint32_t r = 0; r < b->R; ++r)
+  for (uint32_t r = 0; r < b->R; ++r) {
+    if (st[r].error & kErrBarrier)
+      return bfail(b, HMC_ERR_DEVICE, "a neighbour wait of the persistent apply timed out (replica " +
+                                          std::to_string(r) + ")");
     if (st[r].error) return bfail(b, HMC_ERR_COVER, "a body lies outside the checkerboard (replica " + std::to_string(r) + ")");
+  }
   return HMC_OK;
 }
 
@@ -123,6 +142,167 @@ bool batch_tiles(const hmc_batch* b, int cus, BatchTiles& bt) {
     bt.tcs[r].step[0] = q;
   }
   return true;
+}
+
+// Region shapes of the persistent applies of a mixed-precision 3D batch.
+// For each workgroup size and LDS budget, every replica takes the cubic
+// shape T (as a context's regions: 2T spaces per axis and a one-space halo
+// in LDS, the persistent layout's FP32 band) with the fewest regions that
+// fits, and the replicas are packed in order into rounds of at most as many
+// regions as can be resident at once (occupancy of k_bapply_tile); the
+// choice with the fewest rounds wins.  The launch is then probed on the
+// device.  Returns false (and leaves the batch on its per-colour passes)
+// when no shape fits or the probe fails.
+bool batch_regions(hmc_batch* b, int cus) {
+  const uint32_t R = b->R;
+  constexpr int D = 3;
+  const void* fn = (const void*)k_bapply_tile<3>;
+  // the region shape T of replica r for a workgroup of `threads` and an LDS
+  // budget `cap`: the fewest regions (then the least LDS); false if none fits
+  auto shape_for = [&](uint32_t r, uint32_t threads, uint32_t cap, TileCfg& out) {
+    const Geometry& g = b->hreps[r].g;
+    bool found = false;
+    for (uint32_t T = 2; T <= 6; ++T) {
+      const uint32_t trials = T * T * T;
+      if (trials > threads) continue;
+      TileCfg tc;
+      std::memset(&tc, 0, sizeof tc);
+      bool ok = true;
+      uint32_t ncell = 1, ntiles = 1;
+      for (int a = 0; a < D; ++a) {
+        if (g.shape[a] < 6 || g.reach[a] != 1 || T > g.half_shape[a] || T > (uint32_t)(g.shape[a] - 4) / 2) ok = false;
+        tc.T[a] = T;
+        tc.nt[a] = (g.half_shape[a] + T - 1) / T;
+        tc.E[a] = 2 * T + 2;
+        ncell *= 2 * T + 2;
+        ntiles *= tc.nt[a];
+      }
+      const uint32_t EL = 2 * T + 2, rows = ncell / EL;
+      if (rows > 64u * kRowsPerLane || rows * (EL + 2) > threads * kValsPerThread) ok = false;
+      const double occ = (double)g.n_bodies / (double)g.n_spaces;
+      tc.trials = trials;
+      tc.ncell = ncell;
+      tc.rows = rows;
+      tc.maxb = (uint32_t)std::ceil(1.1 * occ * ncell) + 64;
+      tc.ntiles = ntiles;
+      tc.xcd_chunk = (ntiles + 7) / 8;
+      const size_t lr = ((16 * ((size_t)rows + 1) + 4 * (size_t)rows * (EL + 1) + 15) & ~(size_t)15) +
+                        (size_t)tc.maxb * sizeof(float4);
+      if (lr > cap || lr > kTileLds) ok = false;
+      tc.lds_bytes = (uint32_t)lr;
+      uint32_t q = threads;
+      for (int a = D - 1; a > 0; --a) {
+        const uint32_t rad = a == D - 1 ? EL + 2 : tc.E[a];
+        tc.step[a] = q % rad;
+        q /= rad;
+      }
+      tc.step[0] = q;
+      // the persistent layout's band, as for a context
+      const double sigma = g.sigma;
+      double lmax = 0.0, Rb = 0.0;
+      for (int a = 0; a < D; ++a) {
+        lmax = std::max(lmax, g.L[a]);
+        Rb = std::max(Rb, 0.5 * tc.E[a] * g.width[a] + g.width[a]);
+      }
+      const double Et = (0x1p-24 * (2.0 * Rb + 2.0 * sigma) + 0x1p-53 * (4.0 * lmax + 2.0 * sigma)) * 1.01;
+      if (!contact_band(Et, sigma, g.sigma2, g.edge2, D, &tc.band_lo, &tc.band_hi)) ok = false;
+      if (ok && (!found || tc.ntiles < out.ntiles || (tc.ntiles == out.ntiles && tc.lds_bytes < out.lds_bytes))) {
+        out = tc;
+        found = true;
+      }
+    }
+    return found;
+  };
+  size_t best_groups = 0;
+  std::vector<TileCfg> best;
+  std::vector<uint32_t> best_t0, best_nt;
+  uint32_t best_threads = 0, best_lds = 0;
+  for (uint32_t threads = 64; threads <= (uint32_t)kTileThreads; threads *= 2)
+    for (uint32_t cap = 16 * 1024; cap <= (uint32_t)kTileLds; cap += 4 * 1024) {
+      std::vector<TileCfg> tcs(R);
+      bool ok = true;
+      uint32_t lds = 0;
+      for (uint32_t r = 0; r < R && ok; ++r) {
+        ok = shape_for(r, threads, cap, tcs[r]);
+        if (ok) lds = std::max<uint32_t>(lds, tcs[r].lds_bytes);
+      }
+      if (!ok) continue;
+      int per_cu = 0;
+      if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess ||
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, lds) != hipSuccess || per_cu <= 0)
+        continue;
+      const uint64_t capr = (uint64_t)per_cu * (uint64_t)cus;
+      std::vector<uint32_t> t0, nt;
+      uint32_t first = 0, cur = 0;
+      for (uint32_t r = 0; r < R && ok; ++r) {
+        const uint32_t n = tcs[r].ntiles;
+        if ((uint64_t)n + 7 > capr) ok = false;
+        if ((uint64_t)cur + n + 7 > capr) {
+          t0.push_back(first);
+          nt.push_back(cur);
+          first += cur;
+          cur = 0;
+        }
+        cur += n;
+      }
+      if (!ok) continue;
+      t0.push_back(first);
+      nt.push_back(cur);
+      if (best.empty() || t0.size() < best_groups || (t0.size() == best_groups && threads < best_threads)) {
+        best = tcs;
+        best_groups = t0.size();
+        best_t0 = t0;
+        best_nt = nt;
+        best_threads = threads;
+        best_lds = lds;
+      }
+    }
+  if (best.empty()) return false;
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)best_lds) != hipSuccess) return false;
+  std::vector<uint32_t> toff(R + 1, 0);
+  for (uint32_t r = 0; r < R; ++r) toff[r + 1] = toff[r] + best[r].ntiles;
+  const uint32_t total = toff[R];
+  std::vector<uint32_t> trep(total);
+  for (uint32_t r = 0; r < R; ++r)
+    for (uint32_t t = toff[r]; t < toff[r + 1]; ++t) trep[t] = r;
+  const size_t fbytes = (size_t)total * kMailboxStride * sizeof(uint32_t);
+  if (dalloc(&b->dptcs, R) != hipSuccess || dalloc(&b->dptoff, R + 1) != hipSuccess ||
+      dalloc(&b->dptrep, total) != hipSuccess || hipMalloc((void**)&b->w.mbox, fbytes) != hipSuccess ||
+      hipMemset(b->w.mbox, 0, fbytes) != hipSuccess ||
+      hipMemcpy(b->dptcs, best.data(), R * sizeof(TileCfg), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->dptoff, toff.data(), (R + 1) * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->dptrep, trep.data(), total * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+    return false;
+  b->pthreads = best_threads;
+  b->plds = best_lds;
+  b->grp_tile0 = best_t0;
+  b->grp_ntiles = best_nt;
+  b->nwg = 0;
+  for (uint32_t n : best_nt) b->nwg = std::max(b->nwg, n);
+  const uint32_t nr = (uint32_t)best_t0.size();
+  if (dalloc(&b->drt0, nr) != hipSuccess || dalloc(&b->drn, nr) != hipSuccess ||
+      hipMemcpy(b->drt0, best_t0.data(), nr * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->drn, best_nt.data(), nr * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+    return false;
+  // every workgroup of the launch resident at once, checked with the kernel itself
+  {
+    uint32_t nrounds = nr, nwg = b->nwg, chunk = (nwg + 7) / 8, probe = 1;
+    void* args[] = {(void*)&b->reps, (void*)&b->dptcs, (void*)&b->dptoff, (void*)&b->dptrep, (void*)&b->sts,
+                    (void*)&b->drt0, (void*)&b->drn,   (void*)&nrounds,   (void*)&nwg,       (void*)&chunk,
+                    (void*)&probe,   (void*)&b->bodies, (void*)&b->cells, (void*)&b->w};
+    if (hipLaunchCooperativeKernel(fn, dim3(chunk * 8), dim3(best_threads), args, best_lds, b->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess)
+      return false;
+  }
+  std::vector<ApplyState> st(R);
+  if (hipMemcpy(st.data(), b->sts, R * sizeof(ApplyState), hipMemcpyDeviceToHost) != hipSuccess) return false;
+  bool resident = true;
+  for (uint32_t r = 0; r < R; ++r) resident = resident && !(st[r].error & kErrBarrier);
+  if (hipMemset(b->w.mbox, 0, fbytes) != hipSuccess ||
+      hipMemset(b->sts, 0, (R + 1) * sizeof(ApplyState)) != hipSuccess)
+    return false;
+  return resident;
 }
 
 // Makes bt the batch's tile layout (the captured step graph is dropped).
@@ -267,6 +447,14 @@ int hmc_batch_create(uint32_t n_replicas, const hmc_config* cfgs, hmc_batch** ou
       hmc_batch_destroy(b);
       return HMC_ERR_DEVICE;
     }
+    // mixed precision in 3D: persistent applies, one cooperative launch per
+    // group of replicas per step (HMC_FLAG_PER_COLOUR keeps the per-colour
+    // tile passes)
+    int cus = 0, coop = 0;
+    if (b->tiled && b->dim == 3 && b->precision == HMC_MIXED && !(cfgs[0].flags & HMC_FLAG_PER_COLOUR) &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) == hipSuccess && cus > 0 &&
+        hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, b->device) == hipSuccess && coop)
+      b->persist = batch_regions(b, cus);
   }
   *out = b;
   return HMC_OK;
@@ -342,7 +530,8 @@ int hmc_batch_run_translate(hmc_batch* b, const uint64_t* step0, const uint32_t*
   if (rc) return rc;
   const uint32_t launched = b->cycles_guess;
   for (uint32_t i = 0; i < n_steps; ++i) {
-    if (i == 0 || !use_graph) {
+    // (cooperative launches are not captured: persistent steps launch directly)
+    if (i == 0 || !use_graph || b->persist) {
       batch_step(b, i == 0 ? 1 : 0, i == 0 ? 0 : 1, launched);
       continue;
     }
@@ -360,6 +549,11 @@ int hmc_batch_run_translate(hmc_batch* b, const uint64_t* step0, const uint32_t*
     BHIPCHK(b, hipGraphLaunch(b->exec, b->stream));
   }
   BHIPCHK(b, hipGetLastError());
+  if (b->launch_err != hipSuccess) {
+    const hipError_t e = b->launch_err;
+    b->launch_err = hipSuccess;
+    return bfail(b, HMC_ERR_DEVICE, std::string("cooperative launch: ") + hipGetErrorString(e));
+  }
   rc = batch_read(b, st1, a1);
   if (rc) return rc;
   uint32_t most = 1;
@@ -525,7 +719,7 @@ int hmc_batch_get_stats(hmc_batch* b, uint32_t replica, hmc_stats* out) {
   out->band_retests = s.band_retests;
   out->guard_cycles = s.guard_cycles;
   out->tile_overflows = s.tile_overflows;
-  out->tiled = b->tiled ? 1u : 0u;
+  out->tiled = b->persist ? 2u : (b->tiled ? 1u : 0u);
   return HMC_OK;
 }
 
@@ -541,7 +735,9 @@ void hmc_batch_destroy(hmc_batch* b) {
                   (void*)b->dmove, (void*)b->bodies.bpos, (void*)b->bodies.spos, (void*)b->cells.body,
                   (void*)b->cells.pos32, (void*)b->cells.pos64, (void*)b->w.cnt, (void*)b->w.start,
                   (void*)b->w.elem_space, (void*)b->w.elem_rank, (void*)b->w.tmp_body, (void*)b->w.block_sums,
-                  (void*)b->w.block_ne, (void*)b->w.acc_slots, (void*)b->dtcs, (void*)b->dtoff, (void*)b->dtrep})
+                  (void*)b->w.block_ne, (void*)b->w.acc_slots, (void*)b->dtcs, (void*)b->dtoff, (void*)b->dtrep,
+                  (void*)b->dptcs, (void*)b->dptoff, (void*)b->dptrep, (void*)b->w.mbox, (void*)b->drt0,
+                  (void*)b->drn})
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;

@@ -81,6 +81,19 @@ struct hmc_batch {
   uint32_t* dtrep = nullptr;  // replica of each tile
   uint32_t dtrep_cap = 0;
   uint32_t ntiles = 0, tile_threads = 0, tile_lds = 0;
+  // persistent applies (mixed, 3D; k_bapply_tile): one region shape per
+  // replica, the replicas' first regions, the replica of each region, and
+  // the launch groups (whole replicas whose regions are resident together)
+  bool persist = false;
+  TileCfg* dptcs = nullptr;
+  uint32_t* dptoff = nullptr;
+  uint32_t* dptrep = nullptr;
+  uint32_t pthreads = 0, plds = 0;
+  std::vector<uint32_t> grp_tile0, grp_ntiles;  // the rounds (host copy)
+  uint32_t* drt0 = nullptr;                     // the rounds' first regions and sizes (device)
+  uint32_t* drn = nullptr;
+  uint32_t nwg = 0;                             // workgroups of the launch: the largest round
+  hipError_t launch_err = hipSuccess;
   std::string err;
 };
 

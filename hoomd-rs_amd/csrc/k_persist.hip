@@ -85,18 +85,14 @@ __device__ __noinline__ TrialOut trial_global_dev(const Geometry* gp, ApplyState
   return o;
 }
 
+// Every cycle and colour of one apply for region `tile` of the system (g, st,
+// w.start, w.mbox); the region's accepted moves are added to *acc.
 template <int D>
-__global__ void __launch_bounds__(kTileThreads, 2) k_apply_tile(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
-                                                             TileCfg tc) {
+__device__ __forceinline__ void apply_tile_run(const Geometry& g, ApplyState* st, const Bodies& B, const Cells& C,
+                                               const Work& w, const TileCfg& tc, uint32_t tile,
+                                               unsigned long long* acc) {
   constexpr int AL = D - 1;  // the row axis (stride 1)
   constexpr uint32_t NCOL = 1u << D;
-  const uint32_t b = blockIdx.x;
-  const uint32_t tile = (b & 7u) * tc.xcd_chunk + (b >> 3);  // XCD-aware region order
-  if (tile >= tc.ntiles) return;  // padding workgroups (grid rounded up to 8) take part in nothing
-  if (tc.probe) {
-    region_probe(st, w.mbox, tc.ntiles);
-    return;
-  }
   extern __shared__ __align__(16) unsigned char smem[];
   const RegionLds l = region_lds<D>(smem, tc);
   const uint32_t CW = tc.E[AL] + 1;
@@ -311,11 +307,72 @@ __global__ void __launch_bounds__(kTileThreads, 2) k_apply_tile(Geometry g, Appl
   lane_add(&st->band_retests, retests);
   if (tid == 0 && overflow) atomicAdd(&st->tile_overflows, 1ull);
   __syncthreads();
-  if (tid == 0 && blk_acc) atomicAdd(&w.acc_slots[b & (kAccSlots - 1)], (unsigned long long)blk_acc);
+  if (tid == 0 && blk_acc) atomicAdd(acc, (unsigned long long)blk_acc);
+}
+
+template <int D>
+__global__ void __launch_bounds__(kTileThreads, 2) k_apply_tile(Geometry g, ApplyState* st, Bodies B, Cells C, Work w,
+                                                             TileCfg tc) {
+  const uint32_t b = blockIdx.x;
+  const uint32_t tile = (b & 7u) * tc.xcd_chunk + (b >> 3);  // XCD-aware region order
+  if (tile >= tc.ntiles) return;  // padding workgroups (grid rounded up to 8) take part in nothing
+  if (tc.probe) {
+    region_probe(st, w.mbox, tc.ntiles);
+    return;
+  }
+  apply_tile_run<D>(g, st, B, C, w, tc, tile, &w.acc_slots[b & (kAccSlots - 1)]);
+}
+
+// The persistent applies of every replica of a batch in one cooperative
+// launch.  The batch's regions are packed, whole replicas at a time, into
+// rounds of at most one region per workgroup (round k: regions rt0[k] ..
+// rt0[k] + rn[k] - 1 of the batch; trep: a region's replica, toff: the
+// replicas' first regions).  Workgroup b runs its region of round 0 through
+// every cycle of its replica, then its region of round 1, and so on, each
+// with its replica's geometry, region shape, state, spaces and mailboxes.
+// A region waits only on regions of its own replica, which are all in the
+// same round and each run by another resident workgroup that has finished
+// the earlier rounds' regions, none of which waits on a later round: every
+// round completes.  probe: only check that every workgroup of the launch is
+// resident (counter: word 2 of the first region's mailbox; nwg workgroups).
+template <int D>
+__global__ void __launch_bounds__(kTileThreads, 2) k_bapply_tile(const BatchRep* reps, const TileCfg* tcs,
+                                                              const uint32_t* toff, const uint32_t* trep,
+                                                              ApplyState* sts, const uint32_t* rt0,
+                                                              const uint32_t* rn, uint32_t nrounds, uint32_t nwg,
+                                                              uint32_t xcd_chunk, uint32_t probe, Bodies B,
+                                                              Cells C, Work w) {
+  __shared__ Geometry sg;
+  __shared__ TileCfg stc;
+  const uint32_t b = blockIdx.x;
+  const uint32_t t = (b & 7u) * xcd_chunk + (b >> 3);
+  if (t >= nwg) return;  // uniform over the block
+  if (probe) {
+    region_probe(sts + trep[0], w.mbox, nwg);
+    return;
+  }
+  for (uint32_t k = 0; k < nrounds; ++k) {
+    if (t >= rn[k]) continue;  // uniform over the block
+    const uint32_t gt = rt0[k] + t, r = trep[gt];
+    const uint32_t* gsrc = (const uint32_t*)&reps[r].g;
+    for (uint32_t i = threadIdx.x; i < sizeof(Geometry) / 4; i += blockDim.x) ((uint32_t*)&sg)[i] = gsrc[i];
+    const uint32_t* tsrc = (const uint32_t*)&tcs[r];
+    for (uint32_t i = threadIdx.x; i < sizeof(TileCfg) / 4; i += blockDim.x) ((uint32_t*)&stc)[i] = tsrc[i];
+    __syncthreads();
+    Work wr = w;
+    wr.start = w.start + reps[r].space_off;
+    wr.mbox = w.mbox + (size_t)toff[r] * kMailboxStride;
+    wr.tracing = 0;
+    apply_tile_run<D>(sg, sts + r, B, C, wr, stc, gt - toff[r], &w.acc_slots[r]);
+    __syncthreads();  // sg and stc are rewritten for the next round
+  }
 }
 
 // explicit instantiations (launched from the host translation units)
 template __global__ void k_apply_tile<2>(Geometry g, ApplyState* st, Bodies B, Cells C, Work w, TileCfg tc);
 template __global__ void k_apply_tile<3>(Geometry g, ApplyState* st, Bodies B, Cells C, Work w, TileCfg tc);
+template __global__ void k_bapply_tile<3>(const BatchRep*, const TileCfg*, const uint32_t*, const uint32_t*,
+                                          ApplyState*, const uint32_t*, const uint32_t*, uint32_t, uint32_t, uint32_t,
+                                          uint32_t, Bodies, Cells, Work);
 
 }  // namespace hmc

@@ -23,6 +23,8 @@ __global__ void k_pass_tile(Geometry g, ApplyState* st, Bodies B, Cells C, Work 
 template <int D>
 __global__ void k_apply_tile(Geometry g, ApplyState* st, Bodies B, Cells C, Work w, TileCfg tc);
 template <int D>
+__global__ void k_bapply_tile(const BatchRep* reps, const TileCfg* tcs, const uint32_t* toff, const uint32_t* trep, ApplyState* sts, const uint32_t* rt0, const uint32_t* rn, uint32_t nrounds, uint32_t nwg, uint32_t xcd_chunk, uint32_t probe, Bodies B, Cells C, Work w);
+template <int D>
 __global__ void k_apply_lj(Geometry g, ApplyState* st, Bodies B, Cells C, Work w, TileCfg tc);
 template <int D, int PREC>
 __global__ void k_guard(Geometry g, ApplyState* st, Bodies B, Cells C, Work w, uint32_t first_cycle);

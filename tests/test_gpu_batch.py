@@ -44,7 +44,7 @@ def _oracle_run(L, pos, seed, d, steps):
     return (b, s), tuple(tot), ref.positions()
 
 
-@pytest.mark.parametrize("layout", ["tiles", "lanes"])
+@pytest.mark.parametrize("layout", ["persistent", "tiles", "lanes"])
 @pytest.mark.parametrize("use_graph", [True, False], ids=["graph", "direct"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_batch_replicas_match_their_own_oracles(hmc, wl, precision, use_graph, layout):
@@ -55,8 +55,9 @@ def test_batch_replicas_match_their_own_oracles(hmc, wl, precision, use_graph, l
     steps = 5
     refs = [_oracle_run(L, pos, seeds[r], d[r], steps) for r, (L, pos) in enumerate(systems)]
     batch = hmc.GpuBatch([(L, refs[r][0][0], refs[r][0][1]) for r, (L, _) in enumerate(systems)],
-                         precision=precision, lds_tiles=layout == "tiles")
-    assert batch.stats(0)["tiled"] == (1 if layout == "tiles" else 0)
+                         precision=precision, lds_tiles=layout != "lanes", persistent=layout == "persistent")
+    want = {"persistent": 2 if precision == "mixed" else 1, "tiles": 1, "lanes": 0}[layout]
+    assert batch.stats(0)["tiled"] == want
     counts = batch.run(0, seeds, 1.0, d, steps, use_graph=use_graph)
     for r in range(6):
         assert counts[r] == refs[r][1], (r, counts[r], refs[r][1])
@@ -85,7 +86,8 @@ def test_batch_graph_replays_equal_direct_launches(hmc, wl):
     b.close()
 
 
-def test_batch_config_c5_full(hmc, wl):
+@pytest.mark.parametrize("persistent", [True, False], ids=["persistent", "percolour"])
+def test_batch_config_c5_full(hmc, wl, persistent):
     # 256 replicas, N log-uniform over FCC counts 4 k^3 (k = 8..25), packing
     # fractions 0.30 .. 0.54, seeds 42 + r, d = 0.08: 10 steps, six replicas
     # checked against their own oracles, all replicas overlap-free
@@ -94,10 +96,11 @@ def test_batch_config_c5_full(hmc, wl):
     seeds = [42 + r for r in range(len(systems))]
     t = time.time()
     # the microstates' stored (wrapped) bodies; sites follow as wrap(body + 0)
-    batch = hmc.GpuBatch([(L, O.wrap(L, p), None) for L, p in systems], precision="mixed")
-    assert batch.stats(0)["tiled"] == 1
+    batch = hmc.GpuBatch([(L, O.wrap(L, p), None) for L, p in systems], precision="mixed", persistent=persistent)
+    assert batch.stats(0)["tiled"] == (2 if persistent else 1)
     counts = batch.run(0, seeds, 1.0, 0.08, 10)
-    log(f"C5: {len(systems)} replicas, N total {n_total}: 10 steps ({time.time() - t:.1f}s)")
+    log(f"C5 ({'persistent' if persistent else 'per-colour'}): {len(systems)} replicas, N total {n_total}: "
+        f"10 steps ({time.time() - t:.1f}s)")
     for r in (0, 51, 101, 153, 204, 255):
         L, pos = systems[r]
         _, tot, (ob, os_) = _oracle_run(L, pos, seeds[r], 0.08, 10)
@@ -115,8 +118,8 @@ def test_batch_autotune_leaves_state_and_results_unchanged(hmc, wl):
     # restores it after each; the chosen layout makes the same decisions
     systems = wl.batched_replicas(n_replicas=12, k_range=(6, 12))
     seeds = [7 + r for r in range(12)]
-    a = hmc.GpuBatch([(L, p, None) for L, p in systems], precision="mixed")
-    b = hmc.GpuBatch([(L, p, None) for L, p in systems], precision="mixed")
+    a = hmc.GpuBatch([(L, p, None) for L, p in systems], precision="mixed", persistent=False)
+    b = hmc.GpuBatch([(L, p, None) for L, p in systems], precision="mixed", persistent=False)
     ms, policy = a.autotune(0, seeds, 1.0, 0.08, n_steps=2)
     log(f"autotune: {ms:.3f} ms per step with policy {policy}")
     assert ms > 0.0 and policy in (1, 4, 16, 64, 256)
