@@ -186,7 +186,7 @@ bool batch_regions(hmc_batch* b, int cus) {
       tc.maxb = (uint32_t)std::ceil(1.1 * occ * ncell) + 64;
       tc.ntiles = ntiles;
       tc.xcd_chunk = (ntiles + 7) / 8;
-      const size_t lr = ((16 * ((size_t)rows + 1) + 4 * (size_t)rows * (EL + 1) + 15) & ~(size_t)15) +
+      const size_t lr = ((16 * ((size_t)rows + 1) + 2 * (size_t)rows * (EL + 1) + 15) & ~(size_t)15) +
                         (size_t)tc.maxb * sizeof(float4);
       if (lr > cap || lr > kTileLds) ok = false;
       tc.lds_bytes = (uint32_t)lr;

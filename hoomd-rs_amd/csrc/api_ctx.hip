@@ -316,7 +316,7 @@ bool choose_regions(const Geometry& g, int D, double occ, int cus, TileCfg* out)
         const uint32_t EL = 2 * T[D - 1] + 2, rows = ncell / EL;
         if (rows > 64u * kRowsPerLane || rows * (EL + 2) > threads * kValsPerThread) continue;
         const uint32_t maxb = (uint32_t)std::ceil(1.1 * occ * ncell) + 64;
-        const size_t lds = ((16 * ((size_t)rows + 1) + 4 * (size_t)rows * (EL + 1) + 15) & ~(size_t)15) +
+        const size_t lds = ((16 * ((size_t)rows + 1) + 2 * (size_t)rows * (EL + 1) + 15) & ~(size_t)15) +
                            (size_t)maxb * sizeof(float4);
         if (lds > kTileLds) continue;
         const uint32_t per_cu = std::min<uint32_t>((uint32_t)(cu_lds / (lds + static_lds)), cu_waves / (threads / 64));
@@ -426,7 +426,7 @@ bool choose_regions_lj(const Geometry& g, int D, double occ, int cus, TileCfg* o
         const uint32_t EL = 2 * T[D - 1] + 2, rows = ncell / EL;
         if (rows > 64u * kRowsPerLane || rows * (EL + 2) > threads * kValsPerThread) continue;
         const uint32_t maxb = (uint32_t)std::ceil(1.1 * occ * ncell) + 128;  // >= 96: the overflow path's tables
-        const size_t lds = ((16 * ((size_t)rows + 1) + 4 * (size_t)rows * (EL + 1) + 15) & ~(size_t)15) +
+        const size_t lds = ((16 * ((size_t)rows + 1) + 2 * (size_t)rows * (EL + 1) + 15) & ~(size_t)15) +
                            (size_t)maxb * sizeof(float4);
         if (lds + static_lds > cu_lds) continue;
         const uint32_t per_cu = std::min<uint32_t>((uint32_t)(cu_lds / (lds + static_lds)), cu_waves / (threads / 64));

@@ -96,7 +96,7 @@ __device__ __forceinline__ void apply_tile_run(const Geometry& g, ApplyState* st
   extern __shared__ __align__(16) unsigned char smem[];
   const RegionLds l = region_lds<D>(smem, tc);
   const uint32_t CW = tc.E[AL] + 1;
-  const uint32_t* coff = l.coff;
+  const uint16_t* coff = l.coff;
   float4* lb = l.lb;
   __shared__ double zx[257], zf[257];
   __shared__ unsigned blk_acc;

@@ -285,7 +285,7 @@ __global__ void __launch_bounds__(kLjRegionThreads, 2) k_apply_lj(Geometry g, Ap
   extern __shared__ __align__(16) unsigned char smem[];
   const RegionLds l = region_lds<D>(smem, tc);
   const uint32_t CW = tc.E[AL] + 1;
-  const uint32_t* coff = l.coff;
+  const uint16_t* coff = l.coff;
   float4* lb = l.lb;
   __shared__ double zx[257], zf[257];
   __shared__ unsigned blk_acc;

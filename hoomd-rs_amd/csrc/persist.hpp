@@ -22,8 +22,8 @@ namespace hmc {
 
 // A region's box is staged as in k_pass_tile: ri (rows + 1 uint4: LDS offset
 // of the row, first run start, first run length, second run start), coff
-// (rows x (E_last + 1) LDS offsets of the cell boundaries), then the site
-// entries.
+// (rows x (E_last + 1) LDS offsets of the cell boundaries, 16-bit: a box
+// holds at most kTileLds / 16 entries), then the site entries.
 
 // Neighbour synchronisation of the persistent apply, through mailboxes: two
 // counters per region (one per pass parity), kMailboxStride words apart.
@@ -180,7 +180,7 @@ __device__ __forceinline__ uint32_t region_neighbour(const TileCfg& tc, const Re
 // LDS of a region's box: ri, coff, then the site entries.
 struct RegionLds {
   uint4* ri;
-  uint32_t* coff;
+  uint16_t* coff;
   float4* lb;
 };
 template <int D>
@@ -188,8 +188,8 @@ __device__ __forceinline__ RegionLds region_lds(unsigned char* smem, const TileC
   const uint32_t R = tc.rows, CW = tc.E[D - 1] + 1;
   RegionLds l;
   l.ri = (uint4*)smem;
-  l.coff = (uint32_t*)(smem + 16u * (R + 1));
-  l.lb = (float4*)(smem + ((16u * (R + 1) + 4u * R * CW + 15u) & ~15u));
+  l.coff = (uint16_t*)(smem + 16u * (R + 1));
+  l.lb = (float4*)(smem + ((16u * (R + 1) + 2u * R * CW + 15u) & ~15u));
   return l;
 }
 
@@ -210,7 +210,7 @@ __device__ __forceinline__ bool region_stage(const Geometry& g, const TileCfg& t
   const uint32_t p = rg.p, m0 = rg.m0;
   const int32_t* mlo = rg.mlo;
   uint4* ri = l.ri;
-  uint32_t* coff = l.coff;
+  uint16_t* coff = l.coff;
   float4* lb = l.lb;
   const uint32_t rpl = (R + 63) / 64;
   uint32_t rq[kRowsPerLane][4];
@@ -341,7 +341,8 @@ __device__ __forceinline__ bool region_stage(const Geometry& g, const TileCfg& t
       if (key[i] != 0xffffffffu) {
         const uint32_t r = key[i] >> 8, j = key[i] & 0xffu;
         const uint4 q = ri[r];
-        coff[r * CW + (j <= p ? j : j - 1)] = j <= p ? q.x + (raw[i] - q.y) : q.x + q.z + (raw[i] - q.w);
+        coff[r * CW + (j <= p ? j : j - 1)] =
+            (uint16_t)(j <= p ? q.x + (raw[i] - q.y) : q.x + q.z + (raw[i] - q.w));
       }
     }
   }
@@ -361,7 +362,7 @@ __device__ __forceinline__ void region_reload(const Geometry& g, const TileCfg& 
   const uint32_t CW = tc.E[AL] + 1;
   const uint32_t tid = threadIdx.x, NT = blockDim.x;
   const int32_t* rl = rg.rl;
-  const uint32_t* coff = l.coff;
+  const uint16_t* coff = l.coff;
   float4* lb = l.lb;
   uint32_t hdst[kRefreshHeld];
   double4 hval[kRefreshHeld];
