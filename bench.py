@@ -382,9 +382,13 @@ def other_workloads(hmc, wl, cpu=True, only=None, fp64=True):
         acc = sum(a for a, _ in cnt)
         v = t / el
         nb = int(sum(len(p) for _, p in systems))
-        layout = {2: "persistent", 1: "tiles", 0: "lanes"}[batch.stats(0)["tiled"]]
+        st0 = batch.stats(0)
+        layout = {2: "persistent", 1: "tiles", 0: "lanes"}[st0["tiled"]]
         r = {"value": v, "unit": "trial moves/s", "steps": 40, "precision": "mixed", "bodies": nb,
              "pass_layout": layout, "autotuned_policy_cus": policy}
+        if layout == "persistent":
+            r["regions"] = {"all": st0["regions"], "threads": st0["region_threads"], "per_cu": st0["regions_per_cu"],
+                            "shapes": sorted({batch.stats(i)["tile_shape"][0] for i in range(len(systems))})}
         final_mixed = [batch.positions(i)[1] for i in range(0, len(systems), 51)]
         # whole-step roofline: the build (bodies read twice, 4 index arrays
         # written and read) and the passes (every colour pass reads every

@@ -628,8 +628,10 @@ class GpuBatch:
     def stats(self, r):
         s = Stats()
         self._check(lib().hmc_batch_get_stats(self.h, r, C.byref(s)))
-        return {k: getattr(s, k) if k.startswith("fp32") else int(getattr(s, k))
-                for k, _ in Stats._fields_ if k != "tile_shape"}
+        out = {k: getattr(s, k) if k.startswith("fp32") else int(getattr(s, k))
+               for k, _ in Stats._fields_ if k != "tile_shape"}
+        out["tile_shape"] = tuple(int(v) for v in s.tile_shape)
+        return out
 
     def sdf(self, xmax=0.02, dx=1e-4, nbins=None):
         """(hist[R, nbins], overlaps[R]): hmc_sdf of every replica."""

@@ -216,7 +216,7 @@ bool batch_regions(hmc_batch* b, int cus) {
   size_t best_groups = 0;
   std::vector<TileCfg> best;
   std::vector<uint32_t> best_t0, best_nt;
-  uint32_t best_threads = 0, best_lds = 0;
+  uint32_t best_threads = 0, best_lds = 0, best_per_cu = 0;
   for (uint32_t threads = 64; threads <= (uint32_t)kTileThreads; threads *= 2)
     for (uint32_t cap = 16 * 1024; cap <= (uint32_t)kTileLds; cap += 4 * 1024) {
       std::vector<TileCfg> tcs(R);
@@ -255,6 +255,7 @@ bool batch_regions(hmc_batch* b, int cus) {
         best_nt = nt;
         best_threads = threads;
         best_lds = lds;
+        best_per_cu = (uint32_t)per_cu;
       }
     }
   if (best.empty()) return false;
@@ -275,6 +276,9 @@ bool batch_regions(hmc_batch* b, int cus) {
     return false;
   b->pthreads = best_threads;
   b->plds = best_lds;
+  b->pper_cu = best_per_cu;
+  b->pregions = total;
+  b->hptcs = best;
   b->grp_tile0 = best_t0;
   b->grp_ntiles = best_nt;
   b->nwg = 0;
@@ -720,6 +724,12 @@ int hmc_batch_get_stats(hmc_batch* b, uint32_t replica, hmc_stats* out) {
   out->guard_cycles = s.guard_cycles;
   out->tile_overflows = s.tile_overflows;
   out->tiled = b->persist ? 2u : (b->tiled ? 1u : 0u);
+  if (b->persist) {
+    for (int a = 0; a < 3; ++a) out->tile_shape[a] = b->hptcs[replica].T[a];
+    out->regions = b->pregions;
+    out->region_threads = b->pthreads;
+    out->regions_per_cu = b->pper_cu;
+  }
   return HMC_OK;
 }
 

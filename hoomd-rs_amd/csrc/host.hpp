@@ -93,6 +93,8 @@ struct hmc_batch {
   uint32_t* drt0 = nullptr;                     // the rounds' first regions and sizes (device)
   uint32_t* drn = nullptr;
   uint32_t nwg = 0;                             // workgroups of the launch: the largest round
+  uint32_t pper_cu = 0, pregions = 0;           // resident regions per CU; all regions
+  std::vector<TileCfg> hptcs;                   // the replicas' region shapes (host copy)
   hipError_t launch_err = hipSuccess;
   std::string err;
 };

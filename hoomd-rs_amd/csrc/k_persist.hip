@@ -86,11 +86,12 @@ __device__ __noinline__ TrialOut trial_global_dev(const Geometry* gp, ApplyState
 }
 
 // Every cycle and colour of one apply for region `tile` of the system (g, st,
-// w.start, w.mbox); the region's accepted moves are added to *acc.
+// w.start, w.mbox); the region's accepted moves are added to *acc.  zx, zf:
+// the ziggurat layers (an LDS copy, or the constant tables).
 template <int D>
 __device__ __forceinline__ void apply_tile_run(const Geometry& g, ApplyState* st, const Bodies& B, const Cells& C,
                                                const Work& w, const TileCfg& tc, uint32_t tile,
-                                               unsigned long long* acc) {
+                                               unsigned long long* acc, const double* zx, const double* zf) {
   constexpr int AL = D - 1;  // the row axis (stride 1)
   constexpr uint32_t NCOL = 1u << D;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -98,7 +99,6 @@ __device__ __forceinline__ void apply_tile_run(const Geometry& g, ApplyState* st
   const uint32_t CW = tc.E[AL] + 1;
   const uint16_t* coff = l.coff;
   float4* lb = l.lb;
-  __shared__ double zx[257], zf[257];
   __shared__ unsigned blk_acc;
   const uint32_t tid = threadIdx.x;
   const uint32_t wid = tid >> 6, lane = tid & 63u;
@@ -127,7 +127,6 @@ __device__ __forceinline__ void apply_tile_run(const Geometry& g, ApplyState* st
   const uint32_t nnb = D == 3 ? 26u : 8u;
   const uint32_t nbt = wid == 0 && lane < nnb ? region_neighbour<D>(tc, rg, lane) : tile;
 
-  if (D == 3) stage_zig(zx, zf);
   if (tid == 0) blk_acc = 0;
   const bool overflow = region_stage<D>(g, tc, rg, B, C, w, l);
 
@@ -320,7 +319,9 @@ __global__ void __launch_bounds__(kTileThreads, 2) k_apply_tile(Geometry g, Appl
     region_probe(st, w.mbox, tc.ntiles);
     return;
   }
-  apply_tile_run<D>(g, st, B, C, w, tc, tile, &w.acc_slots[b & (kAccSlots - 1)]);
+  __shared__ double zx[257], zf[257];
+  if (D == 3) stage_zig(zx, zf);  // (read after region_stage's barriers)
+  apply_tile_run<D>(g, st, B, C, w, tc, tile, &w.acc_slots[b & (kAccSlots - 1)], zx, zf);
 }
 
 // The persistent applies of every replica of a batch in one cooperative
@@ -363,7 +364,9 @@ __global__ void __launch_bounds__(kTileThreads, 2) k_bapply_tile(const BatchRep*
     wr.start = w.start + reps[r].space_off;
     wr.mbox = w.mbox + (size_t)toff[r] * kMailboxStride;
     wr.tracing = 0;
-    apply_tile_run<D>(sg, sts + r, B, C, wr, stc, gt - toff[r], &w.acc_slots[r]);
+    // the ziggurat layers from the constant tables: the LDS they would take
+    // is worth more as region entries here (more regions per CU)
+    apply_tile_run<D>(sg, sts + r, B, C, wr, stc, gt - toff[r], &w.acc_slots[r], kZigX, kZigF);
     __syncthreads();  // sg and stc are rewritten for the next round
   }
 }
