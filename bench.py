@@ -388,6 +388,8 @@ def other_workloads(hmc, wl, cpu=True, only=None, fp64=True):
              "pass_layout": layout, "autotuned_policy_cus": policy}
         if layout == "persistent":
             r["regions"] = {"all": st0["regions"], "threads": st0["region_threads"], "per_cu": st0["regions_per_cu"],
+                            "rounds": st0["rounds"],
+                            "overflows": int(sum(batch.stats(i)["tile_overflows"] for i in range(len(systems)))),
                             "shapes": sorted({batch.stats(i)["tile_shape"][0] for i in range(len(systems))})}
         final_mixed = [batch.positions(i)[1] for i in range(0, len(systems), 51)]
         # whole-step roofline: the build (bodies read twice, 4 index arrays

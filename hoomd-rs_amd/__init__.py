@@ -103,7 +103,7 @@ class Stats(C.Structure):
                 ("tile_overflows", C.c_uint64), ("tiled", C.c_uint32), ("tile_shape", C.c_uint32 * 3),
                 ("ambiguous", C.c_uint64), ("fp32_error_max", C.c_double), ("fp32_error_bound", C.c_double),
                 ("regions", C.c_uint32), ("region_threads", C.c_uint32), ("regions_per_cu", C.c_uint32),
-                ("wait_timeout_at", C.c_uint32)]
+                ("wait_timeout_at", C.c_uint32), ("rounds", C.c_uint32)]
 
 
 TRACE_DTYPE = np.dtype([("substep", "<u4"), ("color", "<u4"), ("space", "<u8"),

@@ -1,6 +1,7 @@
 // The C ABI of a batch of replicas: creation, state, hipGraph-captured
 // steps, SDF and the tile autotuner.
 #include "host.hpp"
+#include <algorithm>
 
 namespace {
 
@@ -183,6 +184,9 @@ bool batch_regions(hmc_batch* b, int cus) {
       tc.trials = trials;
       tc.ncell = ncell;
       tc.rows = rows;
+      // 10% + 64 entries above the box's mean count: a lattice's planes make
+      // box counts vary by several percent (5% overflowed on C5's FCC starts);
+      // a region whose box holds more takes its trials from global memory
       tc.maxb = (uint32_t)std::ceil(1.1 * occ * ncell) + 64;
       tc.ntiles = ntiles;
       tc.xcd_chunk = (ntiles + 7) / 8;
@@ -215,7 +219,7 @@ bool batch_regions(hmc_batch* b, int cus) {
   };
   size_t best_groups = 0;
   std::vector<TileCfg> best;
-  std::vector<uint32_t> best_t0, best_nt;
+  std::vector<uint32_t> best_t0, best_nt, best_round;
   uint32_t best_threads = 0, best_lds = 0, best_per_cu = 0;
   for (uint32_t threads = 64; threads <= (uint32_t)kTileThreads; threads *= 2)
     for (uint32_t cap = 16 * 1024; cap <= (uint32_t)kTileLds; cap += 4 * 1024) {
@@ -232,27 +236,34 @@ bool batch_regions(hmc_batch* b, int cus) {
           hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, lds) != hipSuccess || per_cu <= 0)
         continue;
       const uint64_t capr = (uint64_t)per_cu * (uint64_t)cus;
-      std::vector<uint32_t> t0, nt;
-      uint32_t first = 0, cur = 0;
-      for (uint32_t r = 0; r < R && ok; ++r) {
+      // whole replicas into rounds, first fit in decreasing order of region
+      // count (7 workgroups of each round are left for the other launches)
+      std::vector<uint32_t> order(R);
+      for (uint32_t r = 0; r < R; ++r) order[r] = r;
+      std::stable_sort(order.begin(), order.end(),
+                       [&](uint32_t a, uint32_t c) { return tcs[a].ntiles > tcs[c].ntiles; });
+      std::vector<uint32_t> load, round_of(R);
+      for (uint32_t r : order) {
         const uint32_t n = tcs[r].ntiles;
-        if ((uint64_t)n + 7 > capr) ok = false;
-        if ((uint64_t)cur + n + 7 > capr) {
-          t0.push_back(first);
-          nt.push_back(cur);
-          first += cur;
-          cur = 0;
+        if ((uint64_t)n + 7 > capr) {
+          ok = false;
+          break;
         }
-        cur += n;
+        size_t k = 0;
+        while (k < load.size() && (uint64_t)load[k] + n + 7 > capr) ++k;
+        if (k == load.size()) load.push_back(0);
+        load[k] += n;
+        round_of[r] = (uint32_t)k;
       }
       if (!ok) continue;
-      t0.push_back(first);
-      nt.push_back(cur);
-      if (best.empty() || t0.size() < best_groups || (t0.size() == best_groups && threads < best_threads)) {
+      if (best.empty() || load.size() < best_groups || (load.size() == best_groups && threads < best_threads)) {
+        std::vector<uint32_t> t0(load.size(), 0);
+        for (size_t k = 1; k < load.size(); ++k) t0[k] = t0[k - 1] + load[k - 1];
         best = tcs;
-        best_groups = t0.size();
+        best_groups = load.size();
+        best_round = round_of;
         best_t0 = t0;
-        best_nt = nt;
+        best_nt = load;
         best_threads = threads;
         best_lds = lds;
         best_per_cu = (uint32_t)per_cu;
@@ -260,12 +271,18 @@ bool batch_regions(hmc_batch* b, int cus) {
     }
   if (best.empty()) return false;
   if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)best_lds) != hipSuccess) return false;
-  std::vector<uint32_t> toff(R + 1, 0);
-  for (uint32_t r = 0; r < R; ++r) toff[r + 1] = toff[r] + best[r].ntiles;
-  const uint32_t total = toff[R];
+  // regions numbered round by round: replica r owns [toff[r], toff[r] +
+  // its region count), inside its round's range [t0, t0 + nt)
+  std::vector<uint32_t> toff(R + 1, 0), fill(best_t0);
+  for (uint32_t r = 0; r < R; ++r) {
+    toff[r] = fill[best_round[r]];
+    fill[best_round[r]] += best[r].ntiles;
+  }
+  const uint32_t total = best_t0.back() + best_nt.back();
+  toff[R] = total;
   std::vector<uint32_t> trep(total);
   for (uint32_t r = 0; r < R; ++r)
-    for (uint32_t t = toff[r]; t < toff[r + 1]; ++t) trep[t] = r;
+    for (uint32_t t = toff[r]; t < toff[r] + best[r].ntiles; ++t) trep[t] = r;
   const size_t fbytes = (size_t)total * kMailboxStride * sizeof(uint32_t);
   if (dalloc(&b->dptcs, R) != hipSuccess || dalloc(&b->dptoff, R + 1) != hipSuccess ||
       dalloc(&b->dptrep, total) != hipSuccess || hipMalloc((void**)&b->w.mbox, fbytes) != hipSuccess ||
@@ -729,6 +746,7 @@ int hmc_batch_get_stats(hmc_batch* b, uint32_t replica, hmc_stats* out) {
     out->regions = b->pregions;
     out->region_threads = b->pthreads;
     out->regions_per_cu = b->pper_cu;
+    out->rounds = (uint32_t)b->grp_tile0.size();
   }
   return HMC_OK;
 }

@@ -148,6 +148,7 @@ typedef struct {
   uint32_t region_threads;  /* persistent apply: lanes per region */
   uint32_t regions_per_cu;  /* persistent apply: resident workgroups per CU (occupancy) */
   uint32_t wait_timeout_at; /* first neighbour wait that timed out: (region << 12) | (pass + 1), or 0 */
+  uint32_t rounds;          /* persistent batch apply: rounds of regions per step (0 otherwise) */
 } hmc_stats;
 
 /* Allocate device state for one system.  Replaces building a ParallelSweep and
