@@ -108,14 +108,16 @@ def run_gpu(hmc, L, body, site, precision, steps, warmup, seed, kT, d, period, p
 
 
 def measured_traffic(kernel):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain copy), or None."""
-    name = {"k_pass_tile": "pass_tile_c2_hbm_traffic.md", "k_apply_tile": "apply_tile_c2_hbm_traffic.md"}
+    """HBM bytes per launch of the dominant kernel (per step for the batch's
+    whole-step roofline) from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (calibrated against a plain copy), or None."""
+    name = {"k_pass_tile": "pass_tile_c2_hbm_traffic.md", "k_apply_tile": "apply_tile_c2_hbm_traffic.md",
+            "k_bapply_tile": "bapply_tile_c5_hbm_traffic.md"}
     f = ROOT / "profiles" / name.get(kernel.split("<")[0], "")
     if not f.is_file():
         return None
     for line in f.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(("traffic_bytes_per_launch:", "traffic_bytes_per_step:")):
             return float(line.split(":", 1)[1])
     return None
 
@@ -406,7 +408,8 @@ def other_workloads(hmc, wl, cpu=True, only=None, fp64=True):
         step_b = (2 * 32 * nb + 4 * 4 * nb + 3 * 4 * spaces) + pass_b + (36 * t + 80 * acc) / 40
         ach = step_b / (el / 40) / 1e9
         r["roofline"] = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                         "frac": ach / PEAK_HBM_GBS, "traffic": None, "kernel": "whole step (build + passes)",
+                         "frac": ach / PEAK_HBM_GBS, "traffic": measured_traffic("k_bapply_tile"),
+                         "kernel": "whole step (build + passes)",
                          "ms_per_step": el / 40 * 1e3, "algorithmic_bytes_per_step": step_b}
         # SDF of every replica (x_max 0.02, dx 1e-4): one call after the run,
         # then timed calls; beta P / rho from the single frame
