@@ -1,7 +1,10 @@
 """hmc_total_energy_noalloc (the allocation-free total energy) against
-hmc_total_energy through the C ABI: identical energy and bound, bit for bit,
-on two calls in a row, the second on a smaller system, for an isotropic model
-(finite sums) and for hard spheres (zero, then an overlap: infinite)."""
+hmc_total_energy through the C ABI, on two calls in a row, the second on a
+smaller system: for hard spheres (zero, then an overlap: infinite) identical
+bit for bit; for an isotropic model (finite sums) within the sum of the two
+returned summation-order bounds, since the order of the bodies within a cell
+comes from the binning's atomic counters and may differ between any two
+calls (the bounds agree to 1e-12 relative)."""
 import sys
 
 import numpy as np
@@ -42,10 +45,12 @@ def test_noalloc_total_energy_matches_lj(hmc, wl):
         g.run(0, 3, 1.2, 0.1, 4)
         want = g.total_energy()
         got = g.total_energy_noalloc()
-        assert _same(got, want), (got, want)
+        assert abs(got[0] - want[0]) <= got[1] + want[1], (got, want)
+        assert abs(got[1] - want[1]) <= 1e-12 * want[1]
         assert np.isfinite(got[0]) and got[0] < 0.0 and got[1] > 0.0
         # the scratch keeps the last call's partials: a second call agrees too
-        assert _same(g.total_energy_noalloc(), want)
+        again = g.total_energy_noalloc()
+        assert abs(again[0] - want[0]) <= again[1] + want[1] and abs(again[1] - want[1]) <= 1e-12 * want[1]
         g.close()
 
 
