@@ -4,6 +4,7 @@
 #include <algorithm>
 
 namespace {
+bool batch_regions(hmc_batch* b, int cus);
 
 int bfail(hmc_batch* b, int code, const std::string& msg) {
   if (b) b->err = msg;
@@ -68,6 +69,9 @@ void launch_batch_step(hmc_batch* b, int from_args, int advance, uint32_t launch
 }
 
 void batch_step(hmc_batch* b, int from_args, int advance, uint32_t launched) {
+  // regions sized for the states set since the last choice (never under
+  // graph capture: persistent steps are not captured)
+  if (b->persist && b->regions_stale) b->persist = batch_regions(b, b->pcus);
   if (b->dim == 2 && b->precision == HMC_FP64)
     launch_batch_step<2, HMC_FP64>(b, from_args, advance, launched);
   else if (b->dim == 2)
@@ -154,8 +158,66 @@ bool batch_tiles(const hmc_batch* b, int cus, BatchTiles& bt) {
 // choice with the fewest rounds wins.  The launch is then probed on the
 // device.  Returns false (and leaves the batch on its per-colour passes)
 // when no shape fits or the probe fails.
+// For each region shape T (index T = 2..6): the largest number of bodies in
+// a box of E = 2T + 2 spaces per axis over every placement of the box on the
+// periodic space grid (binning from the lower box corner); 0 where E exceeds
+// the grid.  Sliding sums along each axis in turn.
+std::array<uint32_t, 7> box_counts(const Geometry& g, const double* pos, uint32_t N) {
+  std::array<uint32_t, 7> out{};
+  const uint32_t S0 = (uint32_t)g.shape[0], S1 = (uint32_t)g.shape[1], S2 = (uint32_t)g.shape[2];
+  const size_t S = (size_t)S0 * S1 * S2;
+  if (!S || !pos) return out;
+  std::vector<uint32_t> cnt(S, 0), a(S), c(S);
+  for (uint32_t i = 0; i < N; ++i) {
+    uint32_t m[3];
+    for (int k = 0; k < 3; ++k) {
+      const double q = std::floor((pos[(size_t)i * 3 + k] + 0.5 * g.L[k]) / g.width[k]);
+      const int64_t v = q < 0.0 ? 0 : (int64_t)q;
+      m[k] = (uint32_t)std::min<int64_t>(v, (int64_t)g.shape[k] - 1);
+    }
+    cnt[((size_t)m[0] * S1 + m[1]) * S2 + m[2]] += 1;
+  }
+  for (uint32_t T = 2; T <= 6; ++T) {
+    const uint32_t E = 2 * T + 2;
+    if (E > S0 || E > S1 || E > S2) continue;
+    // axis 2, then 1, then 0: window sums of E consecutive spaces (wrapped)
+    for (size_t r = 0; r < (size_t)S0 * S1; ++r)
+      for (uint32_t z = 0; z < S2; ++z) {
+        uint32_t t = 0;
+        for (uint32_t k = 0; k < E; ++k) t += cnt[r * S2 + (z + k) % S2];
+        a[r * S2 + z] = t;
+      }
+    for (uint32_t x = 0; x < S0; ++x)
+      for (uint32_t y = 0; y < S1; ++y)
+        for (uint32_t z = 0; z < S2; ++z) {
+          uint32_t t = 0;
+          for (uint32_t k = 0; k < E; ++k) t += a[((size_t)x * S1 + (y + k) % S1) * S2 + z];
+          c[((size_t)x * S1 + y) * S2 + z] = t;
+        }
+    uint32_t best = 0;
+    for (uint32_t x = 0; x < S0; ++x)
+      for (uint32_t y = 0; y < S1; ++y)
+        for (uint32_t z = 0; z < S2; ++z) {
+          uint32_t t = 0;
+          for (uint32_t k = 0; k < E; ++k) t += c[((size_t)((x + k) % S0) * S1 + y) * S2 + z];
+          best = std::max(best, t);
+        }
+    out[T] = best;
+  }
+  return out;
+}
+
 bool batch_regions(hmc_batch* b, int cus) {
   const uint32_t R = b->R;
+  b->pcus = cus;
+  b->regions_stale = false;
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return false;
+  for (void** p : {(void**)&b->dptcs, (void**)&b->dptoff, (void**)&b->dptrep, (void**)&b->w.mbox, (void**)&b->drt0,
+                   (void**)&b->drn})
+    if (*p) {
+      (void)hipFree(*p);
+      *p = nullptr;
+    }
   constexpr int D = 3;
   const void* fn = (const void*)k_bapply_tile<3>;
   // the region shape T of replica r for a workgroup of `threads` and an LDS
@@ -184,10 +246,14 @@ bool batch_regions(hmc_batch* b, int cus) {
       tc.trials = trials;
       tc.ncell = ncell;
       tc.rows = rows;
-      // 10% + 64 entries above the box's mean count: a lattice's planes make
-      // box counts vary by several percent (5% overflowed on C5's FCC starts);
-      // a region whose box holds more takes its trials from global memory
-      tc.maxb = (uint32_t)std::ceil(1.1 * occ * ncell) + 64;
+      // the capacity of a region's box: 2% + 32 entries above the largest
+      // box count of the state last set (every placement of the box); before
+      // any state, 10% + 64 above the mean (a lattice's planes make box
+      // counts vary by several percent).  A region whose box holds more takes
+      // its trials from global memory (same results).
+      const uint32_t known = r < b->box_max.size() ? b->box_max[r][T] : 0u;
+      tc.maxb = known ? std::max((uint32_t)std::ceil(1.02 * known) + 32, (uint32_t)std::ceil(occ * ncell) + 32)
+                      : (uint32_t)std::ceil(1.1 * occ * ncell) + 64;
       tc.ntiles = ntiles;
       tc.xcd_chunk = (ntiles + 7) / 8;
       const size_t lr = ((16 * ((size_t)rows + 1) + 2 * (size_t)rows * (EL + 1) + 15) & ~(size_t)15) +
@@ -305,7 +371,13 @@ bool batch_regions(hmc_batch* b, int cus) {
       hipMemcpy(b->drt0, best_t0.data(), nr * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->drn, best_nt.data(), nr * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
     return false;
-  // every workgroup of the launch resident at once, checked with the kernel itself
+  // every workgroup of the launch resident at once, checked with the kernel
+  // itself (the replicas' states are kept across the probe)
+  std::vector<ApplyState> keep(R + 1);
+  if (hipStreamSynchronize(b->stream) != hipSuccess ||
+      hipMemcpy(keep.data(), b->sts, (R + 1) * sizeof(ApplyState), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemset(b->sts, 0, (R + 1) * sizeof(ApplyState)) != hipSuccess)
+    return false;
   {
     uint32_t nrounds = nr, nwg = b->nwg, chunk = (nwg + 7) / 8, probe = 1;
     void* args[] = {(void*)&b->reps, (void*)&b->dptcs, (void*)&b->dptoff, (void*)&b->dptrep, (void*)&b->sts,
@@ -321,7 +393,7 @@ bool batch_regions(hmc_batch* b, int cus) {
   bool resident = true;
   for (uint32_t r = 0; r < R; ++r) resident = resident && !(st[r].error & kErrBarrier);
   if (hipMemset(b->w.mbox, 0, fbytes) != hipSuccess ||
-      hipMemset(b->sts, 0, (R + 1) * sizeof(ApplyState)) != hipSuccess)
+      hipMemcpy(b->sts, keep.data(), (R + 1) * sizeof(ApplyState), hipMemcpyHostToDevice) != hipSuccess)
     return false;
   return resident;
 }
@@ -511,6 +583,11 @@ int hmc_batch_set_state(hmc_batch* b, uint32_t replica, const double* body_pos, 
     BHIPCHK(b, hipGetLastError());
   }
   BHIPCHK(b, hipStreamSynchronize(b->stream));
+  if (b->persist) {
+    if (b->box_max.size() != b->R) b->box_max.assign(b->R, std::array<uint32_t, 7>{});
+    b->box_max[replica] = box_counts(g, body_pos, N);
+    b->regions_stale = true;
+  }
   return HMC_OK;
 }
 

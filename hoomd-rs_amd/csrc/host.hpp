@@ -95,6 +95,12 @@ struct hmc_batch {
   uint32_t nwg = 0;                             // workgroups of the launch: the largest round
   uint32_t pper_cu = 0, pregions = 0;           // resident regions per CU; all regions
   std::vector<TileCfg> hptcs;                   // the replicas' region shapes (host copy)
+  // per replica and region shape T (index T): the largest body count of a
+  // box of (2T + 2)^3 spaces over the state last set, every placement of the
+  // box on the space grid (0: not known); sizes the regions' LDS capacity
+  std::vector<std::array<uint32_t, 7>> box_max;
+  bool regions_stale = false;  // a state was set: choose the regions again before the next step
+  int pcus = 0;                // compute units the regions were chosen for
   hipError_t launch_err = hipSuccess;
   std::string err;
 };
